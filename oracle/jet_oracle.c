@@ -1,0 +1,1146 @@
+/*
+ * jet_oracle.c -- CPU restatement of Jet's pressure-projection solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker): loaded by the tests, the smoke
+ * check and the CPU-baseline leg of the benchmark, never by the product.
+ *
+ * Written from the semantics of the cited reference files.  Arithmetic is
+ * done in the same operand order as Jet so that, compiled with
+ * -ffp-contract=off, results are bitwise equal to Jet's own build.
+ * Loops Jet runs with parallelFor over k are OpenMP-parallel over k here;
+ * loops Jet runs serially (dot, set, lexicographic Gauss-Seidel) are serial.
+ */
+#include "jet_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define IX(i, j, k, nx, ny) ((i) + (nx) * ((j) + (ny) * (k)))
+
+static const double kEps = DBL_EPSILON; /* include/jet/constants.h:74 */
+
+/* ======================================================= FdmBlas3 ====== */
+
+/* Array3::set(value) -- include/jet/detail/array3-inl.h:49-54 (serial). */
+void or_set(double s, double* v, size_t n) {
+    for (size_t i = 0; i < n; ++i) v[i] = s;
+}
+
+/* Array3::set(other) -- include/jet/detail/array3-inl.h:56-61 (serial). */
+void or_copy(const double* src, double* dst, size_t n) {
+    if (src != dst) memmove(dst, src, n * sizeof(double));
+}
+
+/* FdmBlas3::dot -- src/jet/fdm_linear_system3.cpp:49-65.  Serial k,j,i. */
+double or_dot(const double* a, const double* b, size_t nx, size_t ny,
+              size_t nz) {
+    double result = 0.0;
+    const size_t n = nx * ny * nz;
+    for (size_t idx = 0; idx < n; ++idx) result += a[idx] * b[idx];
+    return result;
+}
+
+/* FdmBlas3::axpy -- src/jet/fdm_linear_system3.cpp:67-77. */
+void or_axpy(double a, const double* x, const double* y, double* result,
+             size_t nx, size_t ny, size_t nz) {
+    const size_t plane = nx * ny;
+#pragma omp parallel for schedule(static)
+    for (size_t k = 0; k < nz; ++k)
+        for (size_t p = k * plane; p < (k + 1) * plane; ++p)
+            result[p] = a * x[p] + y[p];
+}
+
+/* FdmBlas3::mvm -- src/jet/fdm_linear_system3.cpp:79-96.  The -x/-y/-z
+ * couplings are the neighbour's right/up/front (upper-triangle storage). */
+void or_mvm(const or_row* m, const double* v, double* result, size_t nx,
+            size_t ny, size_t nz) {
+#pragma omp parallel for schedule(static)
+    for (size_t k = 0; k < nz; ++k)
+        for (size_t j = 0; j < ny; ++j)
+            for (size_t i = 0; i < nx; ++i) {
+                const size_t c = IX(i, j, k, nx, ny);
+                result[c] =
+                    m[c].center * v[c] +
+                    ((i > 0) ? m[c - 1].right * v[c - 1] : 0.0) +
+                    ((i + 1 < nx) ? m[c].right * v[c + 1] : 0.0) +
+                    ((j > 0) ? m[c - nx].up * v[c - nx] : 0.0) +
+                    ((j + 1 < ny) ? m[c].up * v[c + nx] : 0.0) +
+                    ((k > 0) ? m[c - nx * ny].front * v[c - nx * ny] : 0.0) +
+                    ((k + 1 < nz) ? m[c].front * v[c + nx * ny] : 0.0);
+            }
+}
+
+/* FdmBlas3::residual -- src/jet/fdm_linear_system3.cpp:98-116. */
+void or_residual(const or_row* a, const double* x, const double* b,
+                 double* result, size_t nx, size_t ny, size_t nz) {
+#pragma omp parallel for schedule(static)
+    for (size_t k = 0; k < nz; ++k)
+        for (size_t j = 0; j < ny; ++j)
+            for (size_t i = 0; i < nx; ++i) {
+                const size_t c = IX(i, j, k, nx, ny);
+                result[c] =
+                    b[c] - a[c].center * x[c] -
+                    ((i > 0) ? a[c - 1].right * x[c - 1] : 0.0) -
+                    ((i + 1 < nx) ? a[c].right * x[c + 1] : 0.0) -
+                    ((j > 0) ? a[c - nx].up * x[c - nx] : 0.0) -
+                    ((j + 1 < ny) ? a[c].up * x[c + nx] : 0.0) -
+                    ((k > 0) ? a[c - nx * ny].front * x[c - nx * ny] : 0.0) -
+                    ((k + 1 < nz) ? a[c].front * x[c + nx * ny] : 0.0);
+            }
+}
+
+/* FdmBlas3::l2Norm -- src/jet/fdm_linear_system3.cpp:118. */
+double or_l2norm(const double* v, size_t nx, size_t ny, size_t nz) {
+    return sqrt(or_dot(v, v, nx, ny, nz));
+}
+
+/* FdmBlas3::lInfNorm -- src/jet/fdm_linear_system3.cpp:120-134 (absmax,
+ * include/jet/detail/math_utils-inl.h:65-67). */
+double or_linfnorm(const double* v, size_t nx, size_t ny, size_t nz) {
+    double result = 0.0;
+    const size_t n = nx * ny * nz;
+    for (size_t idx = 0; idx < n; ++idx)
+        result = (result * result > v[idx] * v[idx]) ? result : v[idx];
+    return fabs(result);
+}
+
+/* ======================================================== CG / PCG ===== */
+
+/* pcg<BlasType, PrecondType> -- include/jet/detail/cg-inl.h:15-94.
+ * Stops when sigma = r.z <= tol^2; recomputes the true residual every 50
+ * iterations and on the iteration after sigma grew. */
+void or_pcg(const or_row* A, const double* b, size_t nx, size_t ny, size_t nz,
+            unsigned max_iter, double tol, or_precond_fn M, void* mctx,
+            double* x, double* r, double* d, double* q, double* s,
+            unsigned* last_iter, double* last_residual) {
+    const size_t n = nx * ny * nz;
+    or_set(0.0, r, n);
+    or_set(0.0, d, n);
+    or_set(0.0, q, n);
+    or_set(0.0, s, n);
+
+    or_residual(A, x, b, r, nx, ny, nz);
+    M(mctx, r, d);
+    double sigma_new = or_dot(r, d, nx, ny, nz);
+
+    unsigned iter = 0;
+    int trigger = 0;
+    while (sigma_new > tol * tol && iter < max_iter) {
+        or_mvm(A, d, q, nx, ny, nz);
+        double alpha = sigma_new / or_dot(d, q, nx, ny, nz);
+        or_axpy(alpha, d, x, x, nx, ny, nz);
+        if (trigger || (iter % 50 == 0 && iter > 0)) {
+            or_residual(A, x, b, r, nx, ny, nz);
+            trigger = 0;
+        } else {
+            or_axpy(-alpha, q, r, r, nx, ny, nz);
+        }
+        M(mctx, r, s);
+        double sigma_old = sigma_new;
+        sigma_new = or_dot(r, s, nx, ny, nz);
+        if (sigma_new > sigma_old) trigger = 1;
+        double beta = sigma_new / sigma_old;
+        or_axpy(beta, d, s, d, nx, ny, nz);
+        ++iter;
+    }
+    *last_iter = iter;
+    *last_residual = sqrt(fabs(sigma_new));
+}
+
+typedef struct {
+    size_t n;
+} null_precond_ctx;
+
+/* NullCgPreconditioner::solve -- include/jet/cg.h:20-29 (copy). */
+static void null_precond(void* ctx, const double* b, double* x) {
+    or_copy(b, x, ((null_precond_ctx*)ctx)->n);
+}
+
+/* FdmCgSolver3::solve -- src/jet/fdm_cg_solver3.cpp:20-47. */
+int or_cg_solve(const or_row* A, const double* b, double* x, size_t nx,
+                size_t ny, size_t nz, unsigned max_iter, double tol,
+                unsigned* last_iter, double* last_residual) {
+    const size_t n = nx * ny * nz;
+    double* buf = (double*)calloc(4 * (n ? n : 1), sizeof(double));
+    if (!buf) return -1;
+    or_set(0.0, x, n);
+    null_precond_ctx ctx = {n};
+    or_pcg(A, b, nx, ny, nz, max_iter, tol, null_precond, &ctx, x, buf,
+           buf + n, buf + 2 * n, buf + 3 * n, last_iter, last_residual);
+    free(buf);
+    return (*last_residual <= tol) || (*last_iter < max_iter);
+}
+
+/* ====================================================== Smoothers ====== */
+
+#define OR_OFFDIAG_SUM(A, x, c, i, j, k, nx, ny, nz)                          \
+    (((i > 0) ? A[c - 1].right * x[c - 1] : 0.0) +                          \
+     ((i + 1 < nx) ? A[c].right * x[c + 1] : 0.0) +                         \
+     ((j > 0) ? A[c - nx].up * x[c - nx] : 0.0) +                           \
+     ((j + 1 < ny) ? A[c].up * x[c + nx] : 0.0) +                           \
+     ((k > 0) ? A[c - nx * ny].front * x[c - nx * ny] : 0.0) +              \
+     ((k + 1 < nz) ? A[c].front * x[c + nx * ny] : 0.0))
+
+/* FdmGaussSeidelSolver3::relax -- src/jet/fdm_gauss_seidel_solver3.cpp:103-120.
+ * Lexicographic (k,j,i), in place, SOR factor sor.  Serial. */
+void or_gs_relax(const or_row* A, const double* b, double sor, double* x,
+                 size_t nx, size_t ny, size_t nz) {
+    for (size_t k = 0; k < nz; ++k)
+        for (size_t j = 0; j < ny; ++j)
+            for (size_t i = 0; i < nx; ++i) {
+                const size_t c = IX(i, j, k, nx, ny);
+                double r = OR_OFFDIAG_SUM(A, x, c, i, j, k, nx, ny, nz);
+                x[c] = (1.0 - sor) * x[c] + sor * (b[c] - r) / A[c].center;
+            }
+}
+
+static void rb_pass(const or_row* A, const double* b, double sor, double* x,
+                    size_t nx, size_t ny, size_t nz, size_t color) {
+#pragma omp parallel for schedule(static)
+    for (size_t k = 0; k < nz; ++k)
+        for (size_t j = 0; j < ny; ++j)
+            for (size_t i = (j + k + color) % 2; i < nx; i += 2) {
+                const size_t c = IX(i, j, k, nx, ny);
+                double r = OR_OFFDIAG_SUM(A, x, c, i, j, k, nx, ny, nz);
+                x[c] = (1.0 - sor) * x[c] + sor * (b[c] - r) / A[c].center;
+            }
+}
+
+/* FdmGaussSeidelSolver3::relaxRedBlack --
+ * src/jet/fdm_gauss_seidel_solver3.cpp:150-221.  Red = (i+j+k) even first
+ * (:163), then black = (i+j+k) odd (:196).  Each colour only reads the other
+ * colour, so the visiting order within a colour does not change results. */
+void or_gs_relax_rb(const or_row* A, const double* b, double sor, double* x,
+                    size_t nx, size_t ny, size_t nz) {
+    rb_pass(A, b, sor, x, nx, ny, nz, 0);
+    rb_pass(A, b, sor, x, nx, ny, nz, 1);
+}
+
+/* FdmJacobiSolver3::relax -- src/jet/fdm_jacobi_solver3.cpp:95-112. */
+void or_jacobi_relax(const or_row* A, const double* b, const double* x,
+                     double* xtemp, size_t nx, size_t ny, size_t nz) {
+#pragma omp parallel for schedule(static)
+    for (size_t k = 0; k < nz; ++k)
+        for (size_t j = 0; j < ny; ++j)
+            for (size_t i = 0; i < nx; ++i) {
+                const size_t c = IX(i, j, k, nx, ny);
+                double r = OR_OFFDIAG_SUM(A, x, c, i, j, k, nx, ny, nz);
+                xtemp[c] = (b[c] - r) / A[c].center;
+            }
+}
+
+/* ====================================================== Multigrid ====== */
+
+/* FdmMgUtils3::resizeArrayWithFinest / WithCoarsest --
+ * include/jet/detail/fdm_mg_linear_system3-inl.h:14-50.  Halve while all of
+ * nx, ny, nz are even, up to max_levels levels (at least one). */
+size_t or_levels_with_finest(size_t nx, size_t ny, size_t nz, size_t max_levels,
+                             size_t* lnx, size_t* lny, size_t* lnz) {
+    size_t rx = nx, ry = ny, rz = nz;
+    size_t i = 1;
+    for (; i < max_levels; ++i) {
+        if (rx % 2 == 0 && ry % 2 == 0 && rz % 2 == 0) {
+            rx >>= 1;
+            ry >>= 1;
+            rz >>= 1;
+        } else {
+            break;
+        }
+    }
+    size_t nlev = i < 1 ? 1 : i;
+    if (nlev > OR_MAX_LEVELS) nlev = OR_MAX_LEVELS;
+    for (size_t l = 0; l < nlev; ++l) {
+        lnx[nlev - 1 - l] = rx << l;
+        lny[nlev - 1 - l] = ry << l;
+        lnz[nlev - 1 - l] = rz << l;
+    }
+    return nlev;
+}
+
+/* FdmMgUtils3::restrict -- src/jet/fdm_mg_linear_system3.cpp:43-98.
+ * 1-D weights {1/8,3/8,3/8,1/8} over fine {2i-1,2i,2i+1,2i+2}, clamped to 2i
+ * at the low end and to 2i+1 at the high end.  No 4x scaling. */
+void or_mg_restrict(const double* finer, double* coarser, size_t cnx,
+                    size_t cny, size_t cnz) {
+    static const double kernel[4] = {0.125, 0.375, 0.375, 0.125};
+    const size_t fnx = 2 * cnx, fny = 2 * cny;
+#pragma omp parallel for schedule(static)
+    for (size_t k = 0; k < cnz; ++k) {
+        size_t kI[4] = {(k > 0) ? 2 * k - 1 : 2 * k, 2 * k, 2 * k + 1,
+                        (k + 1 < cnz) ? 2 * k + 2 : 2 * k + 1};
+        for (size_t j = 0; j < cny; ++j) {
+            size_t jI[4] = {(j > 0) ? 2 * j - 1 : 2 * j, 2 * j, 2 * j + 1,
+                            (j + 1 < cny) ? 2 * j + 2 : 2 * j + 1};
+            for (size_t i = 0; i < cnx; ++i) {
+                size_t iI[4] = {(i > 0) ? 2 * i - 1 : 2 * i, 2 * i, 2 * i + 1,
+                                (i + 1 < cnx) ? 2 * i + 2 : 2 * i + 1};
+                double sum = 0.0;
+                for (int z = 0; z < 4; ++z)
+                    for (int y = 0; y < 4; ++y)
+                        for (int x = 0; x < 4; ++x) {
+                            double w = kernel[x] * kernel[y] * kernel[z];
+                            sum += w * finer[IX(iI[x], jI[y], kI[z], fnx, fny)];
+                        }
+                coarser[IX(i, j, k, cnx, cny)] = sum;
+            }
+        }
+    }
+}
+
+/* FdmMgUtils3::correct -- src/jet/fdm_mg_linear_system3.cpp:100-179.
+ * Trilinear (1/4, 3/4) prolongation added to the finer grid.  Jet bounds the
+ * k upper clamp with n.y (:159); restated as written (no effect on cubes). */
+void or_mg_correct(const double* coarser, double* finer, size_t fnx,
+                   size_t fny, size_t fnz) {
+    const size_t cnx = fnx / 2, cny = fny / 2;
+#pragma omp parallel for schedule(static)
+    for (size_t k = 0; k < fnz; ++k)
+        for (size_t j = 0; j < fny; ++j)
+            for (size_t i = 0; i < fnx; ++i) {
+                size_t iI[2], jI[2], kI[2];
+                double iW[2], jW[2], kW[2];
+                const size_t ci = i / 2, cj = j / 2, ck = k / 2;
+                if (i % 2 == 0) {
+                    iI[0] = (i > 1) ? ci - 1 : ci;
+                    iI[1] = ci;
+                    iW[0] = 0.25;
+                    iW[1] = 0.75;
+                } else {
+                    iI[0] = ci;
+                    iI[1] = (i + 1 < fnx) ? ci + 1 : ci;
+                    iW[0] = 0.75;
+                    iW[1] = 0.25;
+                }
+                if (j % 2 == 0) {
+                    jI[0] = (j > 1) ? cj - 1 : cj;
+                    jI[1] = cj;
+                    jW[0] = 0.25;
+                    jW[1] = 0.75;
+                } else {
+                    jI[0] = cj;
+                    jI[1] = (j + 1 < fny) ? cj + 1 : cj;
+                    jW[0] = 0.75;
+                    jW[1] = 0.25;
+                }
+                if (k % 2 == 0) {
+                    kI[0] = (k > 1) ? ck - 1 : ck;
+                    kI[1] = ck;
+                    kW[0] = 0.25;
+                    kW[1] = 0.75;
+                } else {
+                    kI[0] = ck;
+                    kI[1] = (k + 1 < fny) ? ck + 1 : ck; /* n.y, as in Jet */
+                    kW[0] = 0.75;
+                    kW[1] = 0.25;
+                }
+                const size_t f = IX(i, j, k, fnx, fny);
+                for (int z = 0; z < 2; ++z)
+                    for (int y = 0; y < 2; ++y)
+                        for (int x = 0; x < 2; ++x) {
+                            double w = iW[x] * jW[y] * kW[z] *
+                                       coarser[IX(iI[x], jI[y], kI[z], cnx, cny)];
+                            finer[f] += w;
+                        }
+            }
+}
+
+static void mg_relax(const or_row* A, const double* b, unsigned iters,
+                     double* x, size_t nx, size_t ny, size_t nz,
+                     const or_mg_params* p) {
+    /* FdmMgSolver3 relax closures -- src/jet/fdm_mg_solver3.cpp:26-50. */
+    for (unsigned it = 0; it < iters; ++it) {
+        if (p->red_black)
+            or_gs_relax_rb(A, b, p->sor, x, nx, ny, nz);
+        else
+            or_gs_relax(A, b, p->sor, x, nx, ny, nz);
+    }
+}
+
+/* internal::mgVCycle -- include/jet/detail/mg-inl.h:16-68. */
+static double vcycle_level(const or_mg_system* s, const or_mg_params* p,
+                           size_t l, double** x, double** b, double** buf) {
+    const size_t nx = s->nx[l], ny = s->ny[l], nz = s->nz[l];
+    mg_relax(s->A[l], b[l], p->restrict_iter, x[l], nx, ny, nz, p);
+    if (l < s->nlev - 1) {
+        or_residual(s->A[l], x[l], b[l], buf[l], nx, ny, nz);
+        or_mg_restrict(buf[l], b[l + 1], s->nx[l + 1], s->ny[l + 1],
+                       s->nz[l + 1]);
+        or_set(0.0, x[l + 1], s->nx[l + 1] * s->ny[l + 1] * s->nz[l + 1]);
+        vcycle_level(s, p, l + 1, x, b, buf);
+        or_mg_correct(x[l + 1], x[l], nx, ny, nz);
+        mg_relax(s->A[l], b[l], l > 0 ? p->correct_iter : p->final_iter, x[l],
+                 nx, ny, nz, p);
+    } else {
+        mg_relax(s->A[l], b[l], p->coarsest_iter, x[l], nx, ny, nz, p);
+        or_residual(s->A[l], x[l], b[l], buf[l], nx, ny, nz);
+    }
+    or_residual(s->A[l], x[l], b[l], buf[l], nx, ny, nz);
+    return or_l2norm(buf[l], nx, ny, nz);
+}
+
+double or_mg_vcycle(const or_mg_system* sys, const or_mg_params* p,
+                    double** x, double** b, double** buffer) {
+    return vcycle_level(sys, p, 0, x, b, buffer);
+}
+
+static size_t level_n(const or_mg_system* s, size_t l) {
+    return s->nx[l] * s->ny[l] * s->nz[l];
+}
+
+static double** alloc_levels(const or_mg_system* s) {
+    double** v = (double**)calloc(s->nlev, sizeof(double*));
+    for (size_t l = 0; l < s->nlev; ++l)
+        v[l] = (double*)calloc(level_n(s, l) ? level_n(s, l) : 1,
+                               sizeof(double));
+    return v;
+}
+
+static void free_levels(const or_mg_system* s, double** v) {
+    for (size_t l = 0; l < s->nlev; ++l) free(v[l]);
+    free(v);
+}
+
+/* FdmMgSolver3::solve(FdmMgLinearSystem3*) -- src/jet/fdm_mg_solver3.cpp:69-74:
+ * buffer = copy of x, one V-cycle, converged if norm < maxTolerance. */
+int or_mg_solve(or_mg_system* sys, const or_mg_params* p, double* last_norm) {
+    double** buf = alloc_levels(sys);
+    for (size_t l = 0; l < sys->nlev; ++l)
+        or_copy(sys->x[l], buf[l], level_n(sys, l));
+    double norm = or_mg_vcycle(sys, p, sys->x, sys->b, buf);
+    free_levels(sys, buf);
+    if (last_norm) *last_norm = norm;
+    return norm < p->max_tol;
+}
+
+typedef struct {
+    or_mg_system* sys;
+    const or_mg_params* p;
+    double** mx;
+    double** mb;
+    double** mbuf;
+} mgpcg_precond_ctx;
+
+/* FdmMgpcgSolver3::Preconditioner::solve -- src/jet/fdm_mgpcg_solver3.cpp:21-36.
+ * Jet deep-copies the x hierarchy three times; the coarse copies are always
+ * overwritten before use (restriction writes b[l+1], x[l+1] is zeroed, the
+ * residual writes the buffer), so only the top-level warm start -- the output
+ * vector's current contents -- carries state.  That warm start is kept. */
+static void mgpcg_precond(void* vctx, const double* b, double* x) {
+    mgpcg_precond_ctx* c = (mgpcg_precond_ctx*)vctx;
+    or_mg_system* s = c->sys;
+    for (size_t l = 1; l < s->nlev; ++l) {
+        or_copy(s->x[l], c->mx[l], level_n(s, l));
+        or_copy(s->x[l], c->mb[l], level_n(s, l));
+        or_copy(s->x[l], c->mbuf[l], level_n(s, l));
+    }
+    or_copy(s->x[0], c->mbuf[0], level_n(s, 0));
+    or_copy(x, c->mx[0], level_n(s, 0));
+    or_copy(b, c->mb[0], level_n(s, 0));
+    or_mg_vcycle(s, c->p, c->mx, c->mb, c->mbuf);
+    or_copy(c->mx[0], x, level_n(s, 0));
+}
+
+/* FdmMgpcgSolver3::solve -- src/jet/fdm_mgpcg_solver3.cpp:54-80. */
+int or_mgpcg_solve(or_mg_system* sys, const or_mg_params* p, unsigned max_iter,
+                   unsigned* last_iter, double* last_residual) {
+    const size_t nx = sys->nx[0], ny = sys->ny[0], nz = sys->nz[0];
+    const size_t n = nx * ny * nz;
+    double* scratch = (double*)calloc(4 * (n ? n : 1), sizeof(double));
+    or_set(0.0, sys->x[0], n);
+    mgpcg_precond_ctx c = {sys, p, alloc_levels(sys), alloc_levels(sys),
+                           alloc_levels(sys)};
+    or_pcg(sys->A[0], sys->b[0], nx, ny, nz, max_iter, p->max_tol,
+           mgpcg_precond, &c, sys->x[0], scratch, scratch + n, scratch + 2 * n,
+           scratch + 3 * n, last_iter, last_residual);
+    free_levels(sys, c.mx);
+    free_levels(sys, c.mb);
+    free_levels(sys, c.mbuf);
+    free(scratch);
+    return (*last_residual <= p->max_tol) || (*last_iter < max_iter);
+}
+
+/* ============================================ Clean-mode reference solve ==
+ * Not a Jet code path: a tightly converging FP64 reference solve of the same
+ * (A, b), used to produce the reference pressure p_ref that GPU results are
+ * compared with.  It is composed only of the restated Jet operators above
+ * (relaxRedBlack, restrict, correct, residual, dot) arranged as a symmetric
+ * V-cycle (zero initial guess, red-black forward pre-smoothing, black-red
+ * backward post-smoothing, symmetric coarsest sweeps) inside plain PCG.  Jet's
+ * restriction equals 1/8 of the transpose of its prolongation, clamps
+ * included, so the preconditioner is symmetric.  Its answer is pinned by the
+ * true residual ||b - A x|| (the bitwise-pinned residual above), which the
+ * caller checks. */
+
+static void rb_sweep_ordered(const or_row* A, const double* b, double sor,
+                             double* x, size_t nx, size_t ny, size_t nz,
+                             int backward) {
+    rb_pass(A, b, sor, x, nx, ny, nz, backward ? 1 : 0);
+    rb_pass(A, b, sor, x, nx, ny, nz, backward ? 0 : 1);
+}
+
+static void clean_vcycle(const or_mg_system* s, const or_mg_params* p,
+                         size_t l, double** x, double** b, double** buf) {
+    const size_t nx = s->nx[l], ny = s->ny[l], nz = s->nz[l];
+    const size_t n = nx * ny * nz;
+    or_set(0.0, x[l], n);
+    if (l + 1 < s->nlev) {
+        for (unsigned it = 0; it < p->restrict_iter; ++it)
+            rb_sweep_ordered(s->A[l], b[l], p->sor, x[l], nx, ny, nz, 0);
+        or_residual(s->A[l], x[l], b[l], buf[l], nx, ny, nz);
+        or_mg_restrict(buf[l], b[l + 1], s->nx[l + 1], s->ny[l + 1],
+                       s->nz[l + 1]);
+        clean_vcycle(s, p, l + 1, x, b, buf);
+        or_mg_correct(x[l + 1], x[l], nx, ny, nz);
+        unsigned post = l > 0 ? p->correct_iter : p->final_iter;
+        for (unsigned it = 0; it < post; ++it)
+            rb_sweep_ordered(s->A[l], b[l], p->sor, x[l], nx, ny, nz, 1);
+    } else {
+        for (unsigned it = 0; it < p->coarsest_iter; ++it)
+            rb_sweep_ordered(s->A[l], b[l], p->sor, x[l], nx, ny, nz, 0);
+        for (unsigned it = 0; it < p->coarsest_iter; ++it)
+            rb_sweep_ordered(s->A[l], b[l], p->sor, x[l], nx, ny, nz, 1);
+    }
+}
+
+int or_mgpcg_clean_solve(or_mg_system* sys, const or_mg_params* p,
+                         unsigned max_iter, double rel_tol,
+                         unsigned* last_iter, double* true_rel_residual) {
+    const size_t nx = sys->nx[0], ny = sys->ny[0], nz = sys->nz[0];
+    const size_t n = nx * ny * nz;
+    double* r = (double*)calloc(n, sizeof(double));
+    double* d = (double*)calloc(n, sizeof(double));
+    double* q = (double*)calloc(n, sizeof(double));
+    double** mx = alloc_levels(sys);
+    double** mb = alloc_levels(sys);
+    double** mbuf = alloc_levels(sys);
+    double* x = sys->x[0];
+    or_set(0.0, x, n);
+    or_copy(sys->b[0], r, n);
+    const double bnorm = or_l2norm(sys->b[0], nx, ny, nz);
+    double rnorm = bnorm;
+    unsigned iter = 0;
+    double sigma = 0.0;
+    while (iter < max_iter && rnorm > rel_tol * bnorm && bnorm > 0.0) {
+        or_copy(r, mb[0], n);
+        clean_vcycle(sys, p, 0, mx, mb, mbuf); /* z = mx[0] */
+        double sigma_new = or_dot(r, mx[0], nx, ny, nz);
+        if (iter == 0)
+            or_copy(mx[0], d, n);
+        else
+            or_axpy(sigma_new / sigma, d, mx[0], d, nx, ny, nz);
+        sigma = sigma_new;
+        or_mvm(sys->A[0], d, q, nx, ny, nz);
+        double alpha = sigma / or_dot(d, q, nx, ny, nz);
+        or_axpy(alpha, d, x, x, nx, ny, nz);
+        or_axpy(-alpha, q, r, r, nx, ny, nz);
+        rnorm = or_l2norm(r, nx, ny, nz);
+        ++iter;
+    }
+    or_residual(sys->A[0], x, sys->b[0], r, nx, ny, nz);
+    *last_iter = iter;
+    *true_rel_residual =
+        bnorm > 0.0 ? or_l2norm(r, nx, ny, nz) / bnorm : 0.0;
+    free(r);
+    free(d);
+    free(q);
+    free_levels(sys, mx);
+    free_levels(sys, mb);
+    free_levels(sys, mbuf);
+    return *true_rel_residual <= rel_tol;
+}
+
+/* ============================================ Single-phase pressure ===== */
+
+/* GridSinglePhasePressureSolver3::buildMarkers top level --
+ * src/jet/grid_single_phase_pressure_solver3.cpp:258-269.  SDFs are sampled
+ * at cell centres (src/jet/grid3.cpp:32-38); isInsideSdf(phi) = phi < 0
+ * (include/jet/detail/level_set_utils-inl.h:23-26). */
+void or_sp_markers(const double* boundary_sdf, const double* fluid_sdf,
+                   char* markers, size_t n) {
+#pragma omp parallel for schedule(static)
+    for (size_t c = 0; c < n; ++c) {
+        if (boundary_sdf[c] < 0)
+            markers[c] = OR_BOUNDARY;
+        else if (fluid_sdf[c] < 0)
+            markers[c] = OR_FLUID;
+        else
+            markers[c] = OR_AIR;
+    }
+}
+
+/* argmax3 -- include/jet/detail/math_utils-inl.h:107-113 (ties go to the
+ * later argument). */
+static int argmax3i(int x, int y, int z) {
+    if (x > y) return (x > z) ? 0 : 2;
+    return (y > z) ? 1 : 2;
+}
+
+/* Coarse markers: 4^3 clamped neighbourhood majority vote --
+ * src/jet/grid_single_phase_pressure_solver3.cpp:271-327. */
+void or_sp_coarsen_markers(const char* finer, char* coarser, size_t cnx,
+                           size_t cny, size_t cnz) {
+    const size_t fnx = 2 * cnx, fny = 2 * cny;
+#pragma omp parallel for schedule(static)
+    for (size_t k = 0; k < cnz; ++k) {
+        size_t kI[4] = {(k > 0) ? 2 * k - 1 : 2 * k, 2 * k, 2 * k + 1,
+                        (k + 1 < cnz) ? 2 * k + 2 : 2 * k + 1};
+        for (size_t j = 0; j < cny; ++j) {
+            size_t jI[4] = {(j > 0) ? 2 * j - 1 : 2 * j, 2 * j, 2 * j + 1,
+                            (j + 1 < cny) ? 2 * j + 2 : 2 * j + 1};
+            for (size_t i = 0; i < cnx; ++i) {
+                size_t iI[4] = {(i > 0) ? 2 * i - 1 : 2 * i, 2 * i, 2 * i + 1,
+                                (i + 1 < cnx) ? 2 * i + 2 : 2 * i + 1};
+                int cnt[3] = {0, 0, 0};
+                for (int z = 0; z < 4; ++z)
+                    for (int y = 0; y < 4; ++y)
+                        for (int x = 0; x < 4; ++x) {
+                            char f = finer[IX(iI[x], jI[y], kI[z], fnx, fny)];
+                            if (f == OR_BOUNDARY)
+                                ++cnt[OR_BOUNDARY];
+                            else if (f == OR_FLUID)
+                                ++cnt[OR_FLUID];
+                            else
+                                ++cnt[OR_AIR];
+                        }
+                coarser[IX(i, j, k, cnx, cny)] =
+                    (char)argmax3i(cnt[0], cnt[1], cnt[2]);
+            }
+        }
+    }
+}
+
+/* FaceCenteredGrid3::divergenceAtCellCenter -- src/jet/face_centered_grid3.cpp:137-152.
+ * u is (nx+1)*ny*nz, v is nx*(ny+1)*nz, w is nx*ny*(nz+1), x fastest. */
+double or_divergence(const double* u, const double* v, const double* w,
+                     size_t nx, size_t ny, size_t nz, const double h[3],
+                     size_t i, size_t j, size_t k) {
+    (void)nz;
+    double leftU = u[IX(i, j, k, nx + 1, ny)];
+    double rightU = u[IX(i + 1, j, k, nx + 1, ny)];
+    double bottomV = v[IX(i, j, k, nx, ny + 1)];
+    double topV = v[IX(i, j + 1, k, nx, ny + 1)];
+    double backW = w[IX(i, j, k, nx, ny)];
+    double frontW = w[IX(i, j, k + 1, nx, ny)];
+    return (rightU - leftU) / h[0] + (topV - bottomV) / h[1] +
+           (frontW - backW) / h[2];
+}
+
+/* buildSingleSystem (flat) -- src/jet/grid_single_phase_pressure_solver3.cpp:25-79. */
+void or_sp_build_system(const char* markers, const double* u, const double* v,
+                        const double* w, size_t nx, size_t ny, size_t nz,
+                        const double h[3], or_row* A, double* b) {
+    double invH[3] = {1.0 / h[0], 1.0 / h[1], 1.0 / h[2]};
+    double invHSqr[3] = {invH[0] * invH[0], invH[1] * invH[1],
+                         invH[2] * invH[2]};
+#pragma omp parallel for schedule(static)
+    for (size_t k = 0; k < nz; ++k)
+        for (size_t j = 0; j < ny; ++j)
+            for (size_t i = 0; i < nx; ++i) {
+                const size_t c = IX(i, j, k, nx, ny);
+                or_row row = {0.0, 0.0, 0.0, 0.0};
+                b[c] = 0.0;
+                if (markers[c] == OR_FLUID) {
+                    b[c] = or_divergence(u, v, w, nx, ny, nz, h, i, j, k);
+                    if (i + 1 < nx && markers[c + 1] != OR_BOUNDARY) {
+                        row.center += invHSqr[0];
+                        if (markers[c + 1] == OR_FLUID) row.right -= invHSqr[0];
+                    }
+                    if (i > 0 && markers[c - 1] != OR_BOUNDARY)
+                        row.center += invHSqr[0];
+                    if (j + 1 < ny && markers[c + nx] != OR_BOUNDARY) {
+                        row.center += invHSqr[1];
+                        if (markers[c + nx] == OR_FLUID) row.up -= invHSqr[1];
+                    }
+                    if (j > 0 && markers[c - nx] != OR_BOUNDARY)
+                        row.center += invHSqr[1];
+                    if (k + 1 < nz && markers[c + nx * ny] != OR_BOUNDARY) {
+                        row.center += invHSqr[2];
+                        if (markers[c + nx * ny] == OR_FLUID)
+                            row.front -= invHSqr[2];
+                    }
+                    if (k > 0 && markers[c - nx * ny] != OR_BOUNDARY)
+                        row.center += invHSqr[2];
+                } else {
+                    row.center = 1.0;
+                }
+                A[c] = row;
+            }
+}
+
+/* getBarycentric -- include/jet/detail/math_utils-inl.h:147-175 with
+ * iLow = 0, iHigh = size - 1. */
+static void barycentric(double x, long iHigh, long* i, double* f) {
+    double s = floor(x);
+    *i = (long)s;
+    if (0 == iHigh) {
+        *i = 0;
+        *f = 0;
+    } else if (*i < 0) {
+        *i = 0;
+        *f = 0;
+    } else if (*i > iHigh - 1) {
+        *i = iHigh - 1;
+        *f = 1;
+    } else {
+        *f = x - s;
+    }
+}
+
+static double lerp(double a, double b, double f) {
+    return (1 - f) * a + f * b; /* include/jet/detail/math_utils-inl.h:177-180 */
+}
+
+/* LinearArraySampler3::operator() -- include/jet/detail/array_samplers3-inl.h:116-156. */
+static double linear_sample(const double* data, long sx, long sy, long sz,
+                            const double h[3], const double o[3],
+                            const double p[3]) {
+    double nxp = (p[0] - o[0]) / h[0];
+    double nyp = (p[1] - o[1]) / h[1];
+    double nzp = (p[2] - o[2]) / h[2];
+    long i, j, k;
+    double fx, fy, fz;
+    barycentric(nxp, sx - 1, &i, &fx);
+    barycentric(nyp, sy - 1, &j, &fy);
+    barycentric(nzp, sz - 1, &k, &fz);
+    long ip1 = (i + 1 < sx - 1) ? i + 1 : sx - 1;
+    long jp1 = (j + 1 < sy - 1) ? j + 1 : sy - 1;
+    long kp1 = (k + 1 < sz - 1) ? k + 1 : sz - 1;
+#define D(a, b, c) data[(a) + sx * ((b) + sy * (c))]
+    double b0 = lerp(lerp(D(i, j, k), D(ip1, j, k), fx),
+                     lerp(D(i, jp1, k), D(ip1, jp1, k), fx), fy);
+    double b1 = lerp(lerp(D(i, j, kp1), D(ip1, j, kp1), fx),
+                     lerp(D(i, jp1, kp1), D(ip1, jp1, kp1), fx), fy);
+#undef D
+    return lerp(b0, b1, fz);
+}
+
+/* Coarse-level velocity for multigrid system building:
+ * coarser.fill(finer->sampler()) -- src/jet/grid_single_phase_pressure_solver3.cpp:380-392,
+ * FaceCenteredGrid3::fill / uPosition / resetSampler --
+ * src/jet/face_centered_grid3.cpp:275-299,309-331,429-467. */
+void or_face_fill_coarse(const double* fu, const double* fv, const double* fw,
+                         size_t fnx, size_t fny, size_t fnz, const double fh[3],
+                         const double origin[3], double* cu, double* cv,
+                         double* cw, size_t cnx, size_t cny, size_t cnz,
+                         const double ch[3]) {
+    const double fou[3] = {origin[0] + 0.5 * 0.0, origin[1] + 0.5 * fh[1],
+                           origin[2] + 0.5 * fh[2]};
+    const double fov[3] = {origin[0] + 0.5 * fh[0], origin[1] + 0.5 * 0.0,
+                           origin[2] + 0.5 * fh[2]};
+    const double fow[3] = {origin[0] + 0.5 * fh[0], origin[1] + 0.5 * fh[1],
+                           origin[2] + 0.5 * 0.0};
+    const double cou[3] = {origin[0] + 0.5 * 0.0, origin[1] + 0.5 * ch[1],
+                           origin[2] + 0.5 * ch[2]};
+    const double cov[3] = {origin[0] + 0.5 * ch[0], origin[1] + 0.5 * 0.0,
+                           origin[2] + 0.5 * ch[2]};
+    const double cow[3] = {origin[0] + 0.5 * ch[0], origin[1] + 0.5 * ch[1],
+                           origin[2] + 0.5 * 0.0};
+#pragma omp parallel for schedule(static)
+    for (size_t k = 0; k < cnz; ++k)
+        for (size_t j = 0; j < cny; ++j)
+            for (size_t i = 0; i <= cnx; ++i) {
+                double p[3] = {cou[0] + ch[0] * (double)i,
+                               cou[1] + ch[1] * (double)j,
+                               cou[2] + ch[2] * (double)k};
+                cu[IX(i, j, k, cnx + 1, cny)] = linear_sample(
+                    fu, (long)fnx + 1, (long)fny, (long)fnz, fh, fou, p);
+            }
+#pragma omp parallel for schedule(static)
+    for (size_t k = 0; k < cnz; ++k)
+        for (size_t j = 0; j <= cny; ++j)
+            for (size_t i = 0; i < cnx; ++i) {
+                double p[3] = {cov[0] + ch[0] * (double)i,
+                               cov[1] + ch[1] * (double)j,
+                               cov[2] + ch[2] * (double)k};
+                cv[IX(i, j, k, cnx, cny + 1)] = linear_sample(
+                    fv, (long)fnx, (long)fny + 1, (long)fnz, fh, fov, p);
+            }
+#pragma omp parallel for schedule(static)
+    for (size_t k = 0; k <= cnz; ++k)
+        for (size_t j = 0; j < cny; ++j)
+            for (size_t i = 0; i < cnx; ++i) {
+                double p[3] = {cow[0] + ch[0] * (double)i,
+                               cow[1] + ch[1] * (double)j,
+                               cow[2] + ch[2] * (double)k};
+                cw[IX(i, j, k, cnx, cny)] = linear_sample(
+                    fw, (long)fnx, (long)fny, (long)fnz + 1, fh, fow, p);
+            }
+}
+
+/* applyPressureGradient -- src/jet/grid_single_phase_pressure_solver3.cpp:401-431.
+ * Only the +side faces of fluid cells are updated.  Output may alias input. */
+void or_sp_apply_gradient(const char* markers, const double* p,
+                          const double* u, const double* v, const double* w,
+                          double* uo, double* vo, double* wo, size_t nx,
+                          size_t ny, size_t nz, const double h[3]) {
+    double invH[3] = {1.0 / h[0], 1.0 / h[1], 1.0 / h[2]};
+#pragma omp parallel for schedule(static)
+    for (size_t k = 0; k < nz; ++k)
+        for (size_t j = 0; j < ny; ++j)
+            for (size_t i = 0; i < nx; ++i) {
+                const size_t c = IX(i, j, k, nx, ny);
+                if (markers[c] != OR_FLUID) continue;
+                if (i + 1 < nx && markers[c + 1] != OR_BOUNDARY) {
+                    size_t f = IX(i + 1, j, k, nx + 1, ny);
+                    uo[f] = u[f] + invH[0] * (p[c + 1] - p[c]);
+                }
+                if (j + 1 < ny && markers[c + nx] != OR_BOUNDARY) {
+                    size_t f = IX(i, j + 1, k, nx, ny + 1);
+                    vo[f] = v[f] + invH[1] * (p[c + nx] - p[c]);
+                }
+                if (k + 1 < nz && markers[c + nx * ny] != OR_BOUNDARY) {
+                    size_t f = IX(i, j, k + 1, nx, ny);
+                    wo[f] = w[f] + invH[2] * (p[c + nx * ny] - p[c]);
+                }
+            }
+}
+
+/* ============================================== Fractional pressure ===== */
+
+/* fractionInsideSdf -- include/jet/detail/level_set_utils-inl.h:51-62. */
+double or_fraction_inside_sdf(double phi0, double phi1) {
+    if (phi0 < 0 && phi1 < 0) return 1;
+    if (phi0 < 0 && !(phi1 < 0)) return phi0 / (phi0 - phi1);
+    if (!(phi0 < 0) && phi1 < 0) return phi1 / (phi1 - phi0);
+    return 0;
+}
+
+static void cycle4(double* a) {
+    double t = a[0];
+    a[0] = a[1];
+    a[1] = a[2];
+    a[2] = a[3];
+    a[3] = t;
+}
+
+/* fractionInside (Batty's square-corner area) --
+ * include/jet/detail/level_set_utils-inl.h:71-147. */
+double or_fraction_inside(double bl, double br, double tl, double tr) {
+    int inside = (bl < 0 ? 1 : 0) + (tl < 0 ? 1 : 0) + (br < 0 ? 1 : 0) +
+                 (tr < 0 ? 1 : 0);
+    double list[4] = {bl, br, tr, tl};
+    if (inside == 4) return 1;
+    if (inside == 3) {
+        while (list[0] < 0) cycle4(list);
+        double side0 = 1 - or_fraction_inside_sdf(list[0], list[3]);
+        double side1 = 1 - or_fraction_inside_sdf(list[0], list[1]);
+        return 1 - 0.5 * side0 * side1;
+    }
+    if (inside == 2) {
+        while (list[0] >= 0 || !(list[1] < 0 || list[2] < 0)) cycle4(list);
+        if (list[1] < 0) {
+            double side_left = or_fraction_inside_sdf(list[0], list[3]);
+            double side_right = or_fraction_inside_sdf(list[1], list[2]);
+            return 0.5 * (side_left + side_right);
+        }
+        double middle = 0.25 * (list[0] + list[1] + list[2] + list[3]);
+        if (middle < 0) {
+            double area = 0;
+            double side1 = 1 - or_fraction_inside_sdf(list[0], list[3]);
+            double side3 = 1 - or_fraction_inside_sdf(list[2], list[3]);
+            area += 0.5 * side1 * side3;
+            double side2 = 1 - or_fraction_inside_sdf(list[2], list[1]);
+            double side0 = 1 - or_fraction_inside_sdf(list[0], list[1]);
+            area += 0.5 * side0 * side2;
+            return 1 - area;
+        } else {
+            double area = 0;
+            double side0 = or_fraction_inside_sdf(list[0], list[1]);
+            double side1 = or_fraction_inside_sdf(list[0], list[3]);
+            area += 0.5 * side0 * side1;
+            double side2 = or_fraction_inside_sdf(list[2], list[1]);
+            double side3 = or_fraction_inside_sdf(list[2], list[3]);
+            area += 0.5 * side2 * side3;
+            return area;
+        }
+    }
+    if (inside == 1) {
+        while (list[0] >= 0) cycle4(list);
+        double side0 = or_fraction_inside_sdf(list[0], list[3]);
+        double side1 = or_fraction_inside_sdf(list[0], list[1]);
+        return 0.5 * side0 * side1;
+    }
+    return 0;
+}
+
+static float face_weight(double p0, double p1, double p2, double p3) {
+    double frac = or_fraction_inside(p0, p1, p2, p3);
+    double weight = 1.0 - frac;
+    if (weight < 0.0) weight = 0.0;
+    if (weight > 1.0) weight = 1.0;
+    if (weight < 0.01 && weight > 0.0) weight = 0.01; /* kMinWeight */
+    return (float)weight;
+}
+
+/* buildWeights top level -- src/jet/grid_fractional_single_phase_pressure_solver3.cpp:553-617.
+ * The four corner samples of every face are grid vertices; vertex_sdf holds
+ * the boundary SDF at vertex (i,j,k) = origin + h*(i,j,k), an
+ * (nx+1)*(ny+1)*(nz+1) array. */
+void or_fr_face_weights(const double* vs, size_t nx, size_t ny, size_t nz,
+                        float* uw, float* vw, float* ww) {
+    const size_t vx = nx + 1, vy = ny + 1;
+#define V(a, b, c) vs[(a) + vx * ((b) + vy * (c))]
+#pragma omp parallel for schedule(static)
+    for (size_t k = 0; k < nz; ++k)
+        for (size_t j = 0; j < ny; ++j)
+            for (size_t i = 0; i <= nx; ++i)
+                uw[IX(i, j, k, nx + 1, ny)] =
+                    face_weight(V(i, j, k), V(i, j + 1, k), V(i, j, k + 1),
+                                V(i, j + 1, k + 1));
+#pragma omp parallel for schedule(static)
+    for (size_t k = 0; k < nz; ++k)
+        for (size_t j = 0; j <= ny; ++j)
+            for (size_t i = 0; i < nx; ++i)
+                vw[IX(i, j, k, nx, ny + 1)] =
+                    face_weight(V(i, j, k), V(i, j, k + 1), V(i + 1, j, k),
+                                V(i + 1, j, k + 1));
+#pragma omp parallel for schedule(static)
+    for (size_t k = 0; k <= nz; ++k)
+        for (size_t j = 0; j < ny; ++j)
+            for (size_t i = 0; i < nx; ++i)
+                ww[IX(i, j, k, nx, ny)] =
+                    face_weight(V(i, j, k), V(i, j + 1, k), V(i + 1, j, k),
+                                V(i + 1, j + 1, k));
+#undef V
+}
+
+/* Float restriction of fluid SDF / face weights -- anonymous restrict() in
+ * src/jet/grid_fractional_single_phase_pressure_solver3.cpp:29-115.  A
+ * direction whose fine size is not twice the coarse size (the staggered
+ * direction of a face array) uses injection {0,1,0}. */
+void or_fr_restrict_float(const float* finer, size_t fnx, size_t fny,
+                          size_t fnz, float* coarser, size_t cnx, size_t cny,
+                          size_t cnz) {
+    static const float centered[4] = {0.125f, 0.375f, 0.375f, 0.125f};
+    static const float staggered[4] = {0.f, 1.f, 0.f, 0.f};
+    int ks[3] = {fnx != 2 * cnx ? 3 : 4, fny != 2 * cny ? 3 : 4,
+                 fnz != 2 * cnz ? 3 : 4};
+    const float* kern[3] = {ks[0] == 3 ? staggered : centered,
+                            ks[1] == 3 ? staggered : centered,
+                            ks[2] == 3 ? staggered : centered};
+    size_t cn[3] = {cnx, cny, cnz};
+#pragma omp parallel for schedule(static)
+    for (size_t k = 0; k < cnz; ++k) {
+        size_t idx[3][4];
+        size_t c3[3];
+        c3[2] = k;
+        for (size_t j = 0; j < cny; ++j) {
+            c3[1] = j;
+            for (size_t i = 0; i < cnx; ++i) {
+                c3[0] = i;
+                for (int d = 0; d < 3; ++d) {
+                    size_t c = c3[d];
+                    if (ks[d] == 3) {
+                        idx[d][0] = (c > 0) ? 2 * c - 1 : 2 * c;
+                        idx[d][1] = 2 * c;
+                        idx[d][2] = (c + 1 < cn[d]) ? 2 * c + 1 : 2 * c;
+                    } else {
+                        idx[d][0] = (c > 0) ? 2 * c - 1 : 2 * c;
+                        idx[d][1] = 2 * c;
+                        idx[d][2] = 2 * c + 1;
+                        idx[d][3] = (c + 1 < cn[d]) ? 2 * c + 2 : 2 * c + 1;
+                    }
+                }
+                float sum = 0.0f;
+                for (int z = 0; z < ks[2]; ++z)
+                    for (int y = 0; y < ks[1]; ++y)
+                        for (int x = 0; x < ks[0]; ++x) {
+                            float w = kern[0][x] * kern[1][y] * kern[2][z];
+                            sum += w * finer[IX(idx[0][x], idx[1][y], idx[2][z],
+                                                fnx, fny)];
+                        }
+                coarser[IX(i, j, k, cnx, cny)] = sum;
+            }
+        }
+    }
+}
+
+/* Fractional buildSingleSystem (flat) --
+ * src/jet/grid_fractional_single_phase_pressure_solver3.cpp:117-267.
+ * bu/bv/bw: boundary-velocity x/y/z components at u/v/w face centres (NULL
+ * means a zero boundary velocity). */
+void or_fr_build_system(const float* fsdf, const float* uw, const float* vw,
+                        const float* ww, const double* u, const double* v,
+                        const double* w, const double* bu, const double* bv,
+                        const double* bw, size_t nx, size_t ny, size_t nz,
+                        const double h[3], or_row* A, double* b) {
+    double invH[3] = {1.0 / h[0], 1.0 / h[1], 1.0 / h[2]};
+    double invHSqr[3] = {invH[0] * invH[0], invH[1] * invH[1],
+                         invH[2] * invH[2]};
+    const size_t ux = nx + 1, vy = ny + 1;
+#pragma omp parallel for schedule(static)
+    for (size_t k = 0; k < nz; ++k)
+        for (size_t j = 0; j < ny; ++j)
+            for (size_t i = 0; i < nx; ++i) {
+                const size_t c = IX(i, j, k, nx, ny);
+                or_row row = {0.0, 0.0, 0.0, 0.0};
+                double bc = 0.0;
+                double centerPhi = fsdf[c];
+                if (centerPhi < 0) {
+                    double term;
+                    const size_t uR = IX(i + 1, j, k, ux, ny);
+                    const size_t uL = IX(i, j, k, ux, ny);
+                    const size_t vU = IX(i, j + 1, k, nx, vy);
+                    const size_t vD = IX(i, j, k, nx, vy);
+                    const size_t wF = IX(i, j, k + 1, nx, ny);
+                    const size_t wB = IX(i, j, k, nx, ny);
+                    if (i + 1 < nx) {
+                        term = uw[uR] * invHSqr[0];
+                        double rightPhi = fsdf[c + 1];
+                        if (rightPhi < 0) {
+                            row.center += term;
+                            row.right -= term;
+                        } else {
+                            double theta =
+                                or_fraction_inside_sdf(centerPhi, rightPhi);
+                            theta = (theta < 0.01) ? 0.01 : theta; /* std::max */
+                            row.center += term / theta;
+                        }
+                        bc += uw[uR] * u[uR] * invH[0];
+                    } else {
+                        bc += u[uR] * invH[0];
+                    }
+                    if (i > 0) {
+                        term = uw[uL] * invHSqr[0];
+                        double leftPhi = fsdf[c - 1];
+                        if (leftPhi < 0) {
+                            row.center += term;
+                        } else {
+                            double theta =
+                                or_fraction_inside_sdf(centerPhi, leftPhi);
+                            theta = (theta < 0.01) ? 0.01 : theta; /* std::max */
+                            row.center += term / theta;
+                        }
+                        bc -= uw[uL] * u[uL] * invH[0];
+                    } else {
+                        bc -= u[uL] * invH[0];
+                    }
+                    if (j + 1 < ny) {
+                        term = vw[vU] * invHSqr[1];
+                        double upPhi = fsdf[c + nx];
+                        if (upPhi < 0) {
+                            row.center += term;
+                            row.up -= term;
+                        } else {
+                            double theta =
+                                or_fraction_inside_sdf(centerPhi, upPhi);
+                            theta = (theta < 0.01) ? 0.01 : theta; /* std::max */
+                            row.center += term / theta;
+                        }
+                        bc += vw[vU] * v[vU] * invH[1];
+                    } else {
+                        bc += v[vU] * invH[1];
+                    }
+                    if (j > 0) {
+                        term = vw[vD] * invHSqr[1];
+                        double downPhi = fsdf[c - nx];
+                        if (downPhi < 0) {
+                            row.center += term;
+                        } else {
+                            double theta =
+                                or_fraction_inside_sdf(centerPhi, downPhi);
+                            theta = (theta < 0.01) ? 0.01 : theta; /* std::max */
+                            row.center += term / theta;
+                        }
+                        bc -= vw[vD] * v[vD] * invH[1];
+                    } else {
+                        bc -= v[vD] * invH[1];
+                    }
+                    if (k + 1 < nz) {
+                        term = ww[wF] * invHSqr[2];
+                        double frontPhi = fsdf[c + nx * ny];
+                        if (frontPhi < 0) {
+                            row.center += term;
+                            row.front -= term;
+                        } else {
+                            double theta =
+                                or_fraction_inside_sdf(centerPhi, frontPhi);
+                            theta = (theta < 0.01) ? 0.01 : theta; /* std::max */
+                            row.center += term / theta;
+                        }
+                        bc += ww[wF] * w[wF] * invH[2];
+                    } else {
+                        bc += w[wF] * invH[2];
+                    }
+                    if (k > 0) {
+                        term = ww[wB] * invHSqr[2];
+                        double backPhi = fsdf[c - nx * ny];
+                        if (backPhi < 0) {
+                            row.center += term;
+                        } else {
+                            double theta =
+                                or_fraction_inside_sdf(centerPhi, backPhi);
+                            theta = (theta < 0.01) ? 0.01 : theta; /* std::max */
+                            row.center += term / theta;
+                        }
+                        bc -= ww[wB] * w[wB] * invH[2];
+                    } else {
+                        bc -= w[wB] * invH[2];
+                    }
+                    double buR = bu ? bu[uR] : 0.0, buL = bu ? bu[uL] : 0.0;
+                    double bvU = bv ? bv[vU] : 0.0, bvD = bv ? bv[vD] : 0.0;
+                    double bwF = bw ? bw[wF] : 0.0, bwB = bw ? bw[wB] : 0.0;
+                    double contrib = (1.0 - uw[uR]) * buR * invH[0] -
+                                     (1.0 - uw[uL]) * buL * invH[0] +
+                                     (1.0 - vw[vU]) * bvU * invH[1] -
+                                     (1.0 - vw[vD]) * bvD * invH[1] +
+                                     (1.0 - ww[wF]) * bwF * invH[2] -
+                                     (1.0 - ww[wB]) * bwB * invH[2];
+                    bc += contrib;
+                    if (row.center < kEps) {
+                        row.center = 1.0;
+                        bc = 0.0;
+                    }
+                } else {
+                    row.center = 1.0;
+                }
+                A[c] = row;
+                b[c] = bc;
+            }
+}
+
+/* Fractional applyPressureGradient --
+ * src/jet/grid_fractional_single_phase_pressure_solver3.cpp:715-765.
+ * A face is updated when its weight is positive and either side is fluid;
+ * the gradient is divided by the ghost-fluid fraction theta (>= 0.01). */
+void or_fr_apply_gradient(const float* fsdf, const float* uw, const float* vw,
+                          const float* ww, const double* p, const double* u,
+                          const double* v, const double* w, double* uo,
+                          double* vo, double* wo, size_t nx, size_t ny,
+                          size_t nz, const double h[3]) {
+    double invH[3] = {1.0 / h[0], 1.0 / h[1], 1.0 / h[2]};
+#pragma omp parallel for schedule(static)
+    for (size_t k = 0; k < nz; ++k)
+        for (size_t j = 0; j < ny; ++j)
+            for (size_t i = 0; i < nx; ++i) {
+                const size_t c = IX(i, j, k, nx, ny);
+                double centerPhi = fsdf[c];
+                if (i + 1 < nx) {
+                    size_t f = IX(i + 1, j, k, nx + 1, ny);
+                    if (uw[f] > 0.0 && (centerPhi < 0 || fsdf[c + 1] < 0)) {
+                        double theta = or_fraction_inside_sdf(centerPhi, fsdf[c + 1]);
+                        theta = (theta < 0.01) ? 0.01 : theta; /* std::max */
+                        uo[f] = u[f] + invH[0] / theta * (p[c + 1] - p[c]);
+                    }
+                }
+                if (j + 1 < ny) {
+                    size_t f = IX(i, j + 1, k, nx, ny + 1);
+                    if (vw[f] > 0.0 && (centerPhi < 0 || fsdf[c + nx] < 0)) {
+                        double theta = or_fraction_inside_sdf(centerPhi, fsdf[c + nx]);
+                        theta = (theta < 0.01) ? 0.01 : theta; /* std::max */
+                        vo[f] = v[f] + invH[1] / theta * (p[c + nx] - p[c]);
+                    }
+                }
+                if (k + 1 < nz) {
+                    size_t f = IX(i, j, k + 1, nx, ny);
+                    if (ww[f] > 0.0 && (centerPhi < 0 || fsdf[c + nx * ny] < 0)) {
+                        double theta =
+                            or_fraction_inside_sdf(centerPhi, fsdf[c + nx * ny]);
+                        theta = (theta < 0.01) ? 0.01 : theta; /* std::max */
+                        wo[f] = w[f] + invH[2] / theta * (p[c + nx * ny] - p[c]);
+                    }
+                }
+            }
+}

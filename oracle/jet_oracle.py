@@ -1,0 +1,462 @@
+"""ctypes front end of the CPU oracle (test infrastructure only).
+
+Arrays follow Jet's Array3 layout (x fastest): a cell vector is a C-ordered
+numpy array of shape (nz, ny, nx); a matrix is (nz, ny, nx, 4) holding
+FdmMatrixRow3 {center, right, up, front}; face arrays are u:(nz, ny, nx+1),
+v:(nz, ny+1, nx), w:(nz+1, ny, nx).
+
+``Oracle`` wraps ``liboracle.so`` (the C restatement).  ``Ref`` wraps
+``_ref/libjetref.so`` (Jet's own linear solvers, compiled from the reference
+sources) when it has been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libjetref.so")
+
+FLUID, AIR, BOUNDARY = 0, 1, 2
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_cp = C.c_char_p
+_sz = C.c_size_t
+
+
+def _d(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def _f(a):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_fp)
+
+
+def _c(a):
+    assert a.dtype == np.int8 and a.flags.c_contiguous
+    return C.cast(a.ctypes.data, _cp)
+
+
+def _shape3(v):
+    nz, ny, nx = v.shape[:3]
+    return nx, ny, nz
+
+
+class MgParams(C.Structure):
+    """or_mg_params; defaults are MgParameters' (include/jet/mg.h:58-86) with
+    FdmMgSolver3's sorFactor 1.5 (include/jet/fdm_mg_solver3.h:24-30)."""
+
+    _fields_ = [
+        ("max_levels", C.c_size_t),
+        ("restrict_iter", C.c_uint),
+        ("correct_iter", C.c_uint),
+        ("coarsest_iter", C.c_uint),
+        ("final_iter", C.c_uint),
+        ("max_tol", C.c_double),
+        ("sor", C.c_double),
+        ("red_black", C.c_int),
+    ]
+
+    @classmethod
+    def make(cls, max_levels=1, restrict_iter=5, correct_iter=5,
+             coarsest_iter=20, final_iter=20, max_tol=1e-9, sor=1.5,
+             red_black=False):
+        return cls(max_levels, restrict_iter, correct_iter, coarsest_iter,
+                   final_iter, max_tol, sor, int(bool(red_black)))
+
+
+MAX_LEVELS = 16
+
+
+class MgSystemC(C.Structure):
+    _fields_ = [
+        ("nlev", C.c_size_t),
+        ("nx", C.c_size_t * MAX_LEVELS),
+        ("ny", C.c_size_t * MAX_LEVELS),
+        ("nz", C.c_size_t * MAX_LEVELS),
+        ("A", _dp * MAX_LEVELS),
+        ("x", _dp * MAX_LEVELS),
+        ("b", _dp * MAX_LEVELS),
+    ]
+
+
+@dataclass
+class MgSystem:
+    """FdmMgLinearSystem3: per-level A, x, b (level 0 finest)."""
+
+    A: list = field(default_factory=list)
+    x: list = field(default_factory=list)
+    b: list = field(default_factory=list)
+
+    def to_c(self):
+        s = MgSystemC()
+        s.nlev = len(self.A)
+        for l, (A, x, b) in enumerate(zip(self.A, self.x, self.b)):
+            nx, ny, nz = _shape3(x)
+            s.nx[l], s.ny[l], s.nz[l] = nx, ny, nz
+            s.A[l], s.x[l], s.b[l] = _d(A), _d(x), _d(b)
+        return s
+
+
+def _setup(lib):
+    lib.or_dot.restype = C.c_double
+    lib.or_l2norm.restype = C.c_double
+    lib.or_linfnorm.restype = C.c_double
+    lib.or_fraction_inside.restype = C.c_double
+    lib.or_fraction_inside.argtypes = [C.c_double] * 4
+    lib.or_fraction_inside_sdf.restype = C.c_double
+    lib.or_fraction_inside_sdf.argtypes = [C.c_double] * 2
+    lib.or_mg_solve.argtypes = [C.POINTER(MgSystemC), C.POINTER(MgParams), _dp]
+    lib.or_mgpcg_solve.argtypes = [C.POINTER(MgSystemC), C.POINTER(MgParams),
+                                   C.c_uint, C.POINTER(C.c_uint), _dp]
+    lib.or_mgpcg_clean_solve.argtypes = [C.POINTER(MgSystemC), C.POINTER(MgParams), C.c_uint,
+                                         C.c_double, C.POINTER(C.c_uint), _dp]
+    lib.or_levels_with_finest.restype = C.c_size_t
+    lib.or_divergence.restype = C.c_double
+    return lib
+
+
+class Oracle:
+    """The C restatement (liboracle.so)."""
+
+    def __init__(self, path=LIB_PATH):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: run `make -C oracle`")
+        self.lib = _setup(C.CDLL(path))
+
+    # ----------------------------------------------------------- FdmBlas3
+    def mvm(self, A, v):
+        out = np.empty_like(v)
+        self.lib.or_mvm(_d(A), _d(v), _d(out), *map(_sz, _shape3(v)))
+        return out
+
+    def residual(self, A, x, b):
+        out = np.empty_like(x)
+        self.lib.or_residual(_d(A), _d(x), _d(b), _d(out), *map(_sz, _shape3(x)))
+        return out
+
+    def dot(self, a, b):
+        return self.lib.or_dot(_d(a), _d(b), *map(_sz, _shape3(a)))
+
+    def l2norm(self, a):
+        return self.lib.or_l2norm(_d(a), *map(_sz, _shape3(a)))
+
+    def linfnorm(self, a):
+        return self.lib.or_linfnorm(_d(a), *map(_sz, _shape3(a)))
+
+    def axpy(self, a, x, y):
+        out = np.empty_like(x)
+        self.lib.or_axpy(C.c_double(a), _d(x), _d(y), _d(out), *map(_sz, _shape3(x)))
+        return out
+
+    # ---------------------------------------------------------------- CG
+    def cg_solve(self, A, b, max_iter, tol):
+        x = np.zeros_like(b)
+        it, res = C.c_uint(), C.c_double()
+        ok = self.lib.or_cg_solve(_d(A), _d(b), _d(x), *map(_sz, _shape3(b)),
+                                  C.c_uint(max_iter), C.c_double(tol),
+                                  C.byref(it), C.byref(res))
+        return x, it.value, res.value, bool(ok)
+
+    # ---------------------------------------------------------- smoothers
+    def gs_relax(self, A, b, sor, x, red_black=False):
+        x = x.copy()
+        fn = self.lib.or_gs_relax_rb if red_black else self.lib.or_gs_relax
+        fn(_d(A), _d(b), C.c_double(sor), _d(x), *map(_sz, _shape3(x)))
+        return x
+
+    def jacobi_relax(self, A, b, x):
+        out = np.empty_like(x)
+        self.lib.or_jacobi_relax(_d(A), _d(b), _d(x), _d(out), *map(_sz, _shape3(x)))
+        return out
+
+    # --------------------------------------------------------- multigrid
+    def levels_with_finest(self, shape_xyz, max_levels):
+        lx, ly, lz = (C.c_size_t * MAX_LEVELS)(), (C.c_size_t * MAX_LEVELS)(), (C.c_size_t * MAX_LEVELS)()
+        n = self.lib.or_levels_with_finest(*map(_sz, shape_xyz), _sz(max_levels), lx, ly, lz)
+        return [(lx[l], ly[l], lz[l]) for l in range(n)]
+
+    def restrict(self, fine):
+        nz, ny, nx = fine.shape
+        out = np.empty((nz // 2, ny // 2, nx // 2))
+        self.lib.or_mg_restrict(_d(fine), _d(out), _sz(nx // 2), _sz(ny // 2), _sz(nz // 2))
+        return out
+
+    def correct(self, coarse, fine):
+        fine = fine.copy()
+        self.lib.or_mg_correct(_d(coarse), _d(fine), *map(_sz, _shape3(fine)))
+        return fine
+
+    def mg_solve(self, sys: MgSystem, params: MgParams):
+        """FdmMgSolver3::solve -- updates sys.x in place; returns (ok, norm)."""
+        s = sys.to_c()
+        norm = C.c_double()
+        ok = self.lib.or_mg_solve(C.byref(s), C.byref(params), C.byref(norm))
+        return bool(ok), norm.value
+
+    def mgpcg_solve(self, sys: MgSystem, params: MgParams, max_iter):
+        """FdmMgpcgSolver3::solve -- sys.x[0] is the solution."""
+        s = sys.to_c()
+        it, res = C.c_uint(), C.c_double()
+        ok = self.lib.or_mgpcg_solve(C.byref(s), C.byref(params), C.c_uint(max_iter),
+                                     C.byref(it), C.byref(res))
+        return bool(ok), it.value, res.value
+
+    def mgpcg_clean_solve(self, sys: MgSystem, params: MgParams, max_iter, rel_tol):
+        """Clean-mode FP64 reference solve; returns (ok, iters, true_rel_residual)."""
+        s = sys.to_c()
+        it, res = C.c_uint(), C.c_double()
+        ok = self.lib.or_mgpcg_clean_solve(C.byref(s), C.byref(params), C.c_uint(max_iter),
+                                           C.c_double(rel_tol), C.byref(it), C.byref(res))
+        return bool(ok), it.value, res.value
+
+    # ------------------------------------------------ single-phase build
+    def sp_markers(self, bsdf, fsdf):
+        m = np.empty(bsdf.shape, np.int8)
+        self.lib.or_sp_markers(_d(bsdf), _d(fsdf), _c(m), _sz(m.size))
+        return m
+
+    def sp_coarsen_markers(self, fine):
+        nz, ny, nx = fine.shape
+        out = np.empty((nz // 2, ny // 2, nx // 2), np.int8)
+        self.lib.or_sp_coarsen_markers(_c(fine), _c(out), _sz(nx // 2), _sz(ny // 2), _sz(nz // 2))
+        return out
+
+    def sp_build_system(self, markers, u, v, w, h):
+        nx, ny, nz = _shape3(markers)
+        A = np.empty((nz, ny, nx, 4))
+        b = np.empty((nz, ny, nx))
+        hh = (C.c_double * 3)(*h)
+        self.lib.or_sp_build_system(_c(markers), _d(u), _d(v), _d(w), _sz(nx), _sz(ny), _sz(nz),
+                                    hh, _d(A), _d(b))
+        return A, b
+
+    def face_fill_coarse(self, u, v, w, h, origin):
+        nz, ny, nx = w.shape[0] - 1, w.shape[1], w.shape[2]
+        cx, cy, cz = nx // 2, ny // 2, nz // 2
+        ch = tuple(2.0 * x for x in h)
+        cu = np.empty((cz, cy, cx + 1))
+        cv = np.empty((cz, cy + 1, cx))
+        cw = np.empty((cz + 1, cy, cx))
+        self.lib.or_face_fill_coarse(_d(u), _d(v), _d(w), _sz(nx), _sz(ny), _sz(nz),
+                                     (C.c_double * 3)(*h), (C.c_double * 3)(*origin),
+                                     _d(cu), _d(cv), _d(cw), _sz(cx), _sz(cy), _sz(cz),
+                                     (C.c_double * 3)(*ch))
+        return cu, cv, cw, ch
+
+    def sp_apply_gradient(self, markers, p, u, v, w, h):
+        uo, vo, wo = u.copy(), v.copy(), w.copy()
+        nx, ny, nz = _shape3(markers)
+        self.lib.or_sp_apply_gradient(_c(markers), _d(p), _d(u), _d(v), _d(w), _d(uo), _d(vo),
+                                      _d(wo), _sz(nx), _sz(ny), _sz(nz), (C.c_double * 3)(*h))
+        return uo, vo, wo
+
+    def sp_build_mg(self, u, v, w, h, origin, bsdf, fsdf, max_levels):
+        """GridSinglePhasePressureSolver3::buildMarkers + buildSystem
+        (src/jet/grid_single_phase_pressure_solver3.cpp:248-399): returns
+        (markers per level, MgSystem)."""
+        nz, ny, nx = bsdf.shape
+        sizes = self.levels_with_finest((nx, ny, nz), max_levels)
+        markers = [self.sp_markers(bsdf, fsdf)]
+        for _ in sizes[1:]:
+            markers.append(self.sp_coarsen_markers(markers[-1]))
+        sys = MgSystem()
+        cu, cv, cw, ch = u, v, w, tuple(h)
+        for l, m in enumerate(markers):
+            if l > 0:
+                cu, cv, cw, ch = self.face_fill_coarse(cu, cv, cw, ch, origin)
+            A, b = self.sp_build_system(m, cu, cv, cw, ch)
+            sys.A.append(A)
+            sys.b.append(b)
+            sys.x.append(np.zeros_like(b))
+        return markers, sys
+
+    # -------------------------------------------------- fractional build
+    def fraction_inside_sdf(self, a, b):
+        return self.lib.or_fraction_inside_sdf(a, b)
+
+    def fraction_inside(self, bl, br, tl, tr):
+        return self.lib.or_fraction_inside(bl, br, tl, tr)
+
+    def fr_face_weights(self, vertex_sdf):
+        nz, ny, nx = (s - 1 for s in vertex_sdf.shape)
+        uw = np.empty((nz, ny, nx + 1), np.float32)
+        vw = np.empty((nz, ny + 1, nx), np.float32)
+        ww = np.empty((nz + 1, ny, nx), np.float32)
+        self.lib.or_fr_face_weights(_d(vertex_sdf), _sz(nx), _sz(ny), _sz(nz), _f(uw), _f(vw), _f(ww))
+        return uw, vw, ww
+
+    def fr_restrict_float(self, fine, coarse_shape):
+        out = np.empty(coarse_shape, np.float32)
+        fz, fy, fx = fine.shape
+        cz, cy, cx = coarse_shape
+        self.lib.or_fr_restrict_float(_f(fine), _sz(fx), _sz(fy), _sz(fz), _f(out), _sz(cx), _sz(cy), _sz(cz))
+        return out
+
+    def fr_build_system(self, fsdf, uw, vw, ww, u, v, w, h, bu=None, bv=None, bw=None):
+        nx, ny, nz = _shape3(fsdf)
+        A = np.empty((nz, ny, nx, 4))
+        b = np.empty((nz, ny, nx))
+        nul = C.cast(None, _dp)
+        self.lib.or_fr_build_system(_f(fsdf), _f(uw), _f(vw), _f(ww), _d(u), _d(v), _d(w),
+                                    _d(bu) if bu is not None else nul,
+                                    _d(bv) if bv is not None else nul,
+                                    _d(bw) if bw is not None else nul,
+                                    _sz(nx), _sz(ny), _sz(nz), (C.c_double * 3)(*h), _d(A), _d(b))
+        return A, b
+
+    def fr_apply_gradient(self, fsdf, uw, vw, ww, p, u, v, w, h):
+        uo, vo, wo = u.copy(), v.copy(), w.copy()
+        nx, ny, nz = _shape3(fsdf)
+        self.lib.or_fr_apply_gradient(_f(fsdf), _f(uw), _f(vw), _f(ww), _d(p), _d(u), _d(v), _d(w),
+                                      _d(uo), _d(vo), _d(wo), _sz(nx), _sz(ny), _sz(nz),
+                                      (C.c_double * 3)(*h))
+        return uo, vo, wo
+
+    def fr_build_mg(self, u, v, w, h, origin, vertex_sdf, fluid_sdf_cc, max_levels,
+                    bu=None, bv=None, bw=None):
+        """GridFractionalSinglePhasePressureSolver3::buildWeights + buildSystem
+        (src/jet/grid_fractional_single_phase_pressure_solver3.cpp:520-713).
+        fluid_sdf_cc is cast to float as Jet stores it (:548-551)."""
+        nz, ny, nx = fluid_sdf_cc.shape
+        sizes = self.levels_with_finest((nx, ny, nz), max_levels)
+        fs = [np.ascontiguousarray(fluid_sdf_cc, dtype=np.float32)]
+        uw, vw, ww = self.fr_face_weights(vertex_sdf)
+        UW, VW, WW = [uw], [vw], [ww]
+        for (cx, cy, cz) in sizes[1:]:
+            fs.append(self.fr_restrict_float(fs[-1], (cz, cy, cx)))
+            UW.append(self.fr_restrict_float(UW[-1], (cz, cy, cx + 1)))
+            VW.append(self.fr_restrict_float(VW[-1], (cz, cy + 1, cx)))
+            WW.append(self.fr_restrict_float(WW[-1], (cz + 1, cy, cx)))
+        sys = MgSystem()
+        cu, cv, cw, ch = u, v, w, tuple(h)
+        for l in range(len(sizes)):
+            if l > 0:
+                # Coarse right-hand sides are overwritten by the V-cycle's
+                # restriction before use, so a moving boundary is only
+                # applied on the finest level here.
+                cu, cv, cw, ch = self.face_fill_coarse(cu, cv, cw, ch, origin)
+            A, b = self.fr_build_system(fs[l], UW[l], VW[l], WW[l], cu, cv, cw, ch,
+                                        bu if l == 0 else None, bv if l == 0 else None,
+                                        bw if l == 0 else None)
+            sys.A.append(A)
+            sys.b.append(b)
+            sys.x.append(np.zeros_like(b))
+        return (fs, UW, VW, WW), sys
+
+
+class Ref:
+    """Jet's own compiled linear solvers (oracle/_ref/libjetref.so)."""
+
+    def __init__(self, path=REF_PATH):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        lib = C.CDLL(path)
+        lib.ref_dot.restype = C.c_double
+        lib.ref_linfnorm.restype = C.c_double
+        lib.ref_fraction_inside.restype = C.c_double
+        lib.ref_fraction_inside.argtypes = [C.c_double] * 4
+        lib.ref_fraction_inside_sdf.restype = C.c_double
+        lib.ref_fraction_inside_sdf.argtypes = [C.c_double] * 2
+        lib.ref_levels_with_finest.restype = C.c_size_t
+        lib.ref_quiet()
+        self.lib = lib
+
+    def mvm(self, A, v):
+        out = np.empty_like(v)
+        self.lib.ref_mvm(_d(A), _d(v), _d(out), *map(_sz, _shape3(v)))
+        return out
+
+    def residual(self, A, x, b):
+        out = np.empty_like(x)
+        self.lib.ref_residual(_d(A), _d(x), _d(b), _d(out), *map(_sz, _shape3(x)))
+        return out
+
+    def dot(self, a, b):
+        return self.lib.ref_dot(_d(a), _d(b), *map(_sz, _shape3(a)))
+
+    def linfnorm(self, a):
+        return self.lib.ref_linfnorm(_d(a), *map(_sz, _shape3(a)))
+
+    def axpy(self, a, x, y):
+        out = np.empty_like(x)
+        self.lib.ref_axpy(C.c_double(a), _d(x), _d(y), _d(out), *map(_sz, _shape3(x)))
+        return out
+
+    def cg_solve(self, A, b, max_iter, tol):
+        x = np.zeros_like(b)
+        it, res = C.c_uint(), C.c_double()
+        ok = self.lib.ref_cg_solve(_d(A), _d(b), _d(x), *map(_sz, _shape3(b)), C.c_uint(max_iter),
+                                   C.c_double(tol), C.byref(it), C.byref(res))
+        return x, it.value, res.value, bool(ok)
+
+    def gs_relax(self, A, b, sor, x, red_black=False):
+        x = x.copy()
+        self.lib.ref_gs_relax(_d(A), _d(b), C.c_double(sor), _d(x), *map(_sz, _shape3(x)),
+                              C.c_int(int(red_black)))
+        return x
+
+    def jacobi_relax(self, A, b, x):
+        out = np.empty_like(x)
+        self.lib.ref_jacobi_relax(_d(A), _d(b), _d(x), _d(out), *map(_sz, _shape3(x)))
+        return out
+
+    def restrict(self, fine):
+        nz, ny, nx = fine.shape
+        out = np.empty((nz // 2, ny // 2, nx // 2))
+        self.lib.ref_mg_restrict(_d(fine), _d(out), _sz(nx // 2), _sz(ny // 2), _sz(nz // 2))
+        return out
+
+    def correct(self, coarse, fine):
+        fine = fine.copy()
+        self.lib.ref_mg_correct(_d(coarse), _d(fine), *map(_sz, _shape3(fine)))
+        return fine
+
+    def levels_with_finest(self, shape_xyz, max_levels):
+        lx, ly, lz = (C.c_size_t * MAX_LEVELS)(), (C.c_size_t * MAX_LEVELS)(), (C.c_size_t * MAX_LEVELS)()
+        n = self.lib.ref_levels_with_finest(*map(_sz, shape_xyz), _sz(max_levels), lx, ly, lz)
+        return [(lx[l], ly[l], lz[l]) for l in range(n)]
+
+    def _mg_args(self, sys: MgSystem):
+        n = len(sys.A)
+        SZ = C.c_size_t * n
+        nx = SZ(*[_shape3(x)[0] for x in sys.x])
+        ny = SZ(*[_shape3(x)[1] for x in sys.x])
+        nz = SZ(*[_shape3(x)[2] for x in sys.x])
+        A = (_dp * n)(*[_d(a) for a in sys.A])
+        X = (_dp * n)(*[_d(a) for a in sys.x])
+        B = (_dp * n)(*[_d(a) for a in sys.b])
+        return _sz(n), nx, ny, nz, A, X, B
+
+    def mg_solve(self, sys: MgSystem, p: MgParams):
+        ok = self.lib.ref_mg_solve(*self._mg_args(sys), _sz(p.max_levels), C.c_uint(p.restrict_iter),
+                                   C.c_uint(p.correct_iter), C.c_uint(p.coarsest_iter),
+                                   C.c_uint(p.final_iter), C.c_double(p.max_tol),
+                                   C.c_double(p.sor), C.c_int(p.red_black))
+        return bool(ok)
+
+    def mgpcg_solve(self, sys: MgSystem, p: MgParams, max_iter):
+        it, res = C.c_uint(), C.c_double()
+        ok = self.lib.ref_mgpcg_solve(*self._mg_args(sys), C.c_uint(max_iter), _sz(p.max_levels),
+                                      C.c_uint(p.restrict_iter), C.c_uint(p.correct_iter),
+                                      C.c_uint(p.coarsest_iter), C.c_uint(p.final_iter),
+                                      C.c_double(p.max_tol), C.c_double(p.sor),
+                                      C.c_int(p.red_black), C.byref(it), C.byref(res))
+        return bool(ok), it.value, res.value
+
+    def fraction_inside_sdf(self, a, b):
+        return self.lib.ref_fraction_inside_sdf(a, b)
+
+    def fraction_inside(self, bl, br, tl, tr):
+        return self.lib.ref_fraction_inside(bl, br, tl, tr)
+
+
+def have_ref():
+    return os.path.exists(REF_PATH)

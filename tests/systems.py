@@ -1,0 +1,86 @@
+"""Small linear systems used by the tests (numpy, Jet layout).
+
+Matrices are (nz, ny, nx, 4) arrays of FdmMatrixRow3 {center, right, up,
+front}; vectors are (nz, ny, nx).
+"""
+import numpy as np
+
+
+def poisson_helper(nx, ny, nz, scale=1.0, b_scale=1.0):
+    """Jet's FdmLinearSystemSolverTestHelper3::buildTestLinearSystem
+    (src/tests/unit_tests/fdm_linear_system_solver_test_helper3.h:13-49),
+    optionally scaled per level as in fdm_mgpcg_solver3_tests.cpp:20-55
+    (A by invdx^2, b by invdx).  All-Neumann; b = +1 at j=0, -1 at j=ny-1,
+    so the solution is x = c - j."""
+    A = np.zeros((nz, ny, nx, 4))
+    b = np.zeros((nz, ny, nx))
+    i = np.arange(nx)[None, None, :]
+    j = np.arange(ny)[None, :, None]
+    k = np.arange(nz)[:, None, None]
+    c = np.zeros((nz, ny, nx))
+    for t in ((i > 0), (i < nx - 1), (j > 0), (j < ny - 1), (k > 0), (k < nz - 1)):
+        c += t.astype(float)
+    A[..., 0] = c * scale
+    A[..., 1] = -(i < nx - 1).astype(float) * scale * np.ones((nz, ny, 1))
+    A[..., 2] = -(j < ny - 1).astype(float) * scale * np.ones((nz, 1, nx))
+    A[..., 3] = -(k < nz - 1).astype(float) * scale * np.ones((1, ny, nx))
+    b += (j == 0).astype(float) * b_scale
+    b -= (j == ny - 1).astype(float) * b_scale
+    return np.ascontiguousarray(A), np.ascontiguousarray(b)
+
+
+def random_stencil(nx, ny, nz, rng, spd=True):
+    """Random 7-point symmetric stencil.  With spd=True the diagonal strictly
+    dominates (center = sum|offdiag| + U(0.1, 1)), so A is SPD."""
+    A = np.zeros((nz, ny, nx, 4))
+    A[..., 1] = -rng.uniform(0.0, 1.0, (nz, ny, nx))
+    A[..., 2] = -rng.uniform(0.0, 1.0, (nz, ny, nx))
+    A[..., 3] = -rng.uniform(0.0, 1.0, (nz, ny, nx))
+    A[:, :, nx - 1, 1] = 0.0
+    A[:, ny - 1, :, 2] = 0.0
+    A[nz - 1, :, :, 3] = 0.0
+    if spd:
+        s = -(A[..., 1] + A[..., 2] + A[..., 3])
+        s[:, :, 1:] += -A[:, :, :-1, 1]
+        s[:, 1:, :] += -A[:, :-1, :, 2]
+        s[1:, :, :] += -A[:-1, :, :, 3]
+        A[..., 0] = s + rng.uniform(0.1, 1.0, (nz, ny, nx))
+    else:
+        A[..., 0] = rng.uniform(0.0, 1.0, (nz, ny, nx))
+    return np.ascontiguousarray(A)
+
+
+def dense(A):
+    """Dense n x n matrix of a stencil (brute force for tiny grids)."""
+    nz, ny, nx = A.shape[:3]
+    n = nx * ny * nz
+    M = np.zeros((n, n))
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                c = i + nx * (j + ny * k)
+                M[c, c] = A[k, j, i, 0]
+                if i + 1 < nx:
+                    M[c, c + 1] = M[c + 1, c] = A[k, j, i, 1]
+                if j + 1 < ny:
+                    M[c, c + nx] = M[c + nx, c] = A[k, j, i, 2]
+                if k + 1 < nz:
+                    M[c, c + nx * ny] = M[c + nx * ny, c] = A[k, j, i, 3]
+    return M
+
+
+def mg_poisson_hierarchy(coarsest, levels):
+    """FdmMgpcgSolver3.Solve / FdmMgSolver3.Solve test system
+    (src/tests/unit_tests/fdm_mgpcg_solver3_tests.cpp:13-60): resizeWithCoarsest,
+    A scaled by invdx^2 and b by invdx with invdx = 0.5^l."""
+    from oracle.jet_oracle import MgSystem
+
+    sys = MgSystem()
+    for l in range(levels):
+        n = coarsest << (levels - 1 - l)
+        invdx = 0.5 ** l
+        A, b = poisson_helper(n, n, n, scale=invdx * invdx, b_scale=invdx)
+        sys.A.append(A)
+        sys.b.append(b)
+        sys.x.append(np.zeros_like(b))
+    return sys
