@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""Headline benchmark: 256^3 MGPCG pressure solve, mixed FP32/FP64 vs FP64.
+
+Workload (the metric's configuration): a 256^3 single-phase pressure system
+(GridSinglePhasePressureSolver3 layout) with 8 seeded solid spheres of radius
+U(8,32) cells and ~60% fluid below a free surface, velocities U(-1,1) with the
+blocked boundary condition; 6 multigrid levels (256 -> 8), 2 pre / 2 post
+red-black sweeps, 20 coarsest sweeps.  Inputs are generated and the system
+is assembled on the GPU, so everything is resident in HBM before timing.
+
+A step is one full solve from x = 0 to ||b - A x|| <= 1e-8 ||b||:
+  - mixed: FP64 residual/refinement, FP32 stencil + V-cycle, FP64 dots;
+  - fp64 : plain FP64 MGPCG.
+Timed with a device synchronisation on both sides of K solves.  The CPU leg
+(rank 0, N=1) solves the same system with the oracle's FP64 reference solve
+on the host cores, which also gives p_ref for the error report.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "fluid-engine-dev_amd")
+sys.path.insert(0, PKG)
+sys.path.insert(0, ROOT)
+
+METRIC = ("256³ MGPCG solve ms + HBM GB/s: mixed FP32/FP64 vs FP64, rel err ≤1e-6")
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_rbgs_fp32_256.json")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--levels", type=int, default=6)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--rel-tol", type=float, default=1e-8)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / oracle leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(multi-GPU sharding is a design note, not a measured path)"}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import jetgpu as jg
+
+    n, L = args.n, args.levels
+    solver = jg.Solver(0)
+    solver.set_levels([(n >> l,) * 3 for l in range(L)])
+    f64 = torch.float64
+    u = torch.empty((n, n, n + 1), dtype=f64, device="cuda")
+    v = torch.empty((n, n + 1, n), dtype=f64, device="cuda")
+    w = torch.empty((n + 1, n, n), dtype=f64, device="cuda")
+    bs = torch.empty((n, n, n), dtype=f64, device="cuda")
+    fs = torch.empty((n, n, n), dtype=f64, device="cuda")
+    torch.cuda.synchronize()
+    desc = jg.scene_desc(jg.JG_SCENE_SPHERE_POOL, n, seed=args.seed)
+    solver.generate_scene(desc, u, v, w, bs, fs)
+    solver.sp_build(u, v, w, bs, fs, 1.0)
+    fluid_fraction = float(((bs >= 0) & (fs < 0)).double().mean().item())
+    del u, v, w, bs, fs
+    torch.cuda.empty_cache()
+    log(f"bench: system built n={n} levels={L} fluid={fluid_fraction:.3f}")
+
+    params = jg.mg_params(max_levels=L, restrict_iter=2, correct_iter=2, coarsest_iter=20,
+                          final_iter=2)
+    opts = {
+        "fp64": jg.gpu_options(precision=jg.JG_FP64, stop_rule=jg.JG_STOP_RELATIVE,
+                               rel_tol=args.rel_tol),
+        "mixed": jg.gpu_options(precision=jg.JG_MIXED, stop_rule=jg.JG_STOP_RELATIVE,
+                                rel_tol=args.rel_tol),
+    }
+    for _ in range(args.warmup):
+        for name in ("fp64", "mixed"):
+            solver.mgpcg(500, params, opts[name])
+    log("bench: warmup done")
+
+    res = {}
+    for name in ("fp64", "mixed"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        infos = [solver.mgpcg(500, params, opts[name]) for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        x = solver.download_x()
+        res[name] = {"ms": wall_ms, "event_ms": float(np.mean([i.solve_ms for i in infos])),
+                     "iterations": infos[-1].iterations, "refinements": infos[-1].refinements,
+                     "rel_residual": infos[-1].rel_residual, "x": x}
+        log(f"bench: {name} {wall_ms:.3f} ms/solve, {infos[-1].iterations} it, "
+            f"rel {infos[-1].rel_residual:.2e}")
+
+    # Dominant kernel of the mixed solve: one colour of a red-black sweep on
+    # the finest level (7 of them per PCG iteration).  HIP events on the
+    # solver's stream around 50 back-to-back launches.
+    kern = {}
+    for kname, kid in (("rbgs", jg.JG_KERNEL_RBGS), ("dir_spmv", jg.JG_KERNEL_DIR_SPMV),
+                       ("residual", jg.JG_KERNEL_RESIDUAL), ("update_xr", jg.JG_KERNEL_UPDATE_XR),
+                       ("copy", jg.JG_KERNEL_COPY)):
+        ms, nbytes = solver.time_kernel(kid, jg.JG_MIXED, 50)
+        kern[kname] = {"avg_ms": ms, "bytes": nbytes, "gbs": nbytes / ms / 1e6}
+    dom = kern["rbgs"]
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    n_cells = n ** 3
+    out = {
+        "metric": METRIC,
+        "value": round(res["mixed"]["ms"], 4),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(res["mixed"]["ms"], 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "sphere_pool_256 (BASELINE config 3): 256^3, 8 spheres r~U(8,32), "
+                               "~60% fluid, 6 levels, V(2,2), 20 coarsest sweeps, rel tol 1e-8",
+                   "n": n, "levels": L, "fluid_fraction": round(fluid_fraction, 4)},
+        "fp64_ms": round(res["fp64"]["ms"], 4),
+        "speedup_mixed_vs_fp64": round(res["fp64"]["ms"] / res["mixed"]["ms"], 3),
+        "iterations": {k: res[k]["iterations"] for k in res},
+        "refinements_mixed": res["mixed"]["refinements"],
+        "rel_residual": {k: res[k]["rel_residual"] for k in res},
+        "cells_per_s": {k: n_cells / (res[k]["ms"] * 1e-3) for k in res},
+        "kernels_mixed": {k: {"avg_us": round(vv["avg_ms"] * 1e3, 2), "alg_gbs": round(vv["gbs"], 1)}
+                          for k, vv in kern.items()},
+        "roofline": {"bound": "hbm", "kernel": "k_rbgs<float> (one colour, finest level)",
+                     "achieved": round(dom["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic},
+    }
+
+    if not args.no_cpu:
+        out.update(cpu_leg(jg, n, L, args, res))
+    print(json.dumps(out))
+    solver.close()
+
+
+def cpu_leg(jg, n, L, args, res):
+    """Oracle FP64 solve of the same system on the host cores: the CPU
+    baseline (timed to the same 1e-8 relative residual) and p_ref (a second
+    solve to 1e-12) for the error of both GPU solutions."""
+    import numpy as np
+
+    from oracle.jet_oracle import MgParams, Oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    oracle = Oracle()
+    scene = jg.generate_scene_host(jg.scene_desc(jg.JG_SCENE_SPHERE_POOL, n, seed=args.seed))
+    markers, sys_ = oracle.sp_build_mg(scene["u"], scene["v"], scene["w"], (1.0,) * 3, (0.0,) * 3,
+                                       scene["bsdf"], scene["fsdf"], L)
+    del scene
+    p = MgParams.make(max_levels=L, restrict_iter=2, correct_iter=2, coarsest_iter=20, final_iter=2,
+                      sor=1.0)
+    t0 = time.perf_counter()
+    ok, it_cpu, rel_cpu = oracle.mgpcg_clean_solve(sys_, p, 200, args.rel_tol)
+    cpu_ms = (time.perf_counter() - t0) * 1e3
+    log(f"bench: cpu oracle solve {cpu_ms:.0f} ms, {it_cpu} it, rel {rel_cpu:.1e}")
+    ok_ref, it_ref, rel_ref = oracle.mgpcg_clean_solve(sys_, p, 200, 1e-12)
+    p_ref = sys_.x[0]
+    nref = np.linalg.norm(p_ref)
+    err = {k: float(np.linalg.norm(res[k]["x"] - p_ref) / nref) for k in res}
+    log(f"bench: error vs oracle {err} (p_ref rel residual {rel_ref:.1e})")
+    return {
+        "err_vs_oracle": err,
+        "oracle_p_ref_rel_residual": rel_ref,
+        "cpu_baseline": {"value": round(cpu_ms, 1), "unit": "ms", "cores": threads, "kind": "port",
+                         "sample": f"full {n}^3 solve to rel residual {args.rel_tol:g} by the oracle's "
+                                   f"FP64 MGPCG (restated FdmBlas3 ops, red-black V(2,2)), "
+                                   f"{it_cpu} iterations, OpenMP over k"},
+    }
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,135 @@
+// jg_device.h -- device-side building blocks for the gfx950 pressure solver.
+//
+// Data layout in HBM (same as Jet's Array3 / FdmMatrix3, so an upload is a
+// plain copy): a matrix level is an array of rows {center, right, up, front}
+// with x fastest, index = i + nx*(j + ny*k).  FP64 rows are 32 B, the FP32
+// shadow rows of the mixed-precision preconditioner are 16 B; one row is one
+// (or two) 16-byte vector loads per lane.  The stencil is symmetric and stored
+// as upper couplings only: the -x/-y/-z couplings of a cell are the right/up/
+// front of the neighbour on that side (src/jet/fdm_linear_system3.cpp:79-96).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace jg {
+
+template <typename T>
+struct alignas(4 * sizeof(T)) Row {
+    T c, r, u, f;  // center, right (+x), up (+y), front (+z)
+};
+
+using idx_t = long;
+
+struct Dims {
+    int nx, ny, nz;
+    __host__ __device__ idx_t n() const { return (idx_t)nx * ny * nz; }
+};
+
+// Device-resident scalars of one solve.  Only the finalize kernels write
+// them; every other kernel reads them, so an iteration needs no host round
+// trip and can be captured in a graph.
+struct Ctl {
+    double sigma;      // r.z of the current iterate
+    double sigma_old;
+    double dq;         // d.q
+    double alpha;
+    double beta;
+    double rr;         // r.r of the recursively updated residual
+    double bb;         // b.b of the current right-hand side
+    double tol2;       // Jet's stop rule: sigma <= tol2
+    double rel2;       // relative rule: rr <= rel2 * bb (0 disables)
+    double scale;      // mixed precision: scale of the FP32 right-hand side
+    unsigned iter;
+    unsigned max_iter;
+    int trigger;       // Jet: recompute the true residual next iteration
+    int refresh;       // this iteration recomputes the true residual
+    int done;          // stop flag: set once converged or out of iterations
+    int stop_rule;     // 0: Jet (sigma), 1: relative residual, 2: both
+    int nonfinite;     // a reduction produced NaN/Inf
+    int pad;
+};
+
+constexpr int kWave = 64;
+constexpr int NT = 256;  // threads per block of the streaming kernels
+constexpr int kMaxPartials = 16384;  // per reduction slot
+
+// ---------------------------------------------------------------- reduce
+
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_down(v, o, kWave);
+    if ((lane & (kWave - 1)) == 0) sh[lane / kWave] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (lane == 0) {
+#pragma unroll
+        for (int w = 0; w < NT / kWave; ++w) s += sh[w];
+    }
+    __syncthreads();
+    return s;  // valid in thread 0 only
+}
+
+// Writes this block's partial sum of v (one slot per block, deterministic
+// order) for a later finalize kernel.
+template <int NT>
+__device__ __forceinline__ void block_partial(double v, double* partials) {
+    __shared__ double sh[NT / kWave];
+    double s = block_sum<NT>(v, sh);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        const idx_t b = blockIdx.x + (idx_t)gridDim.x * (blockIdx.y + (idx_t)gridDim.y * blockIdx.z);
+        partials[b] = s;
+    }
+}
+
+template <int NT>
+__device__ __forceinline__ void block_partial2(double v0, double v1, double* p0, double* p1) {
+    __shared__ double sh[2][NT / kWave];
+    double s0 = block_sum<NT>(v0, sh[0]);
+    double s1 = block_sum<NT>(v1, sh[1]);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        const idx_t b = blockIdx.x + (idx_t)gridDim.x * (blockIdx.y + (idx_t)gridDim.y * blockIdx.z);
+        p0[b] = s0;
+        p1[b] = s1;
+    }
+}
+
+// Sum of n partials by one block of NT threads, fixed order (deterministic).
+template <int NT>
+__device__ __forceinline__ double reduce_partials(const double* p, int n, double* sh) {
+    const int t = threadIdx.x;
+    double s = 0.0;
+    for (int i = t; i < n; i += NT) s += p[i];
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) s += __shfl_down(s, o, kWave);
+    if ((t & (kWave - 1)) == 0) sh[t / kWave] = s;
+    __syncthreads();
+    double r = 0.0;
+#pragma unroll
+    for (int w = 0; w < NT / kWave; ++w) r += sh[w];
+    __syncthreads();
+    return r;  // valid in every thread
+}
+
+// --------------------------------------------------------------- stencil
+
+// Off-diagonal sum  sum_nbr a(c,nbr) * x(nbr)  of a 7-point row at (i,j,k).
+template <typename TA, typename TX, typename TC>
+__device__ __forceinline__ TC offdiag(const Row<TA>* __restrict__ A, const TX* __restrict__ x,
+                                      const Row<TA>& row, idx_t c, int i, int j, int k,
+                                      const Dims& d) {
+#pragma clang fp contract(off)
+    // Jet's relax operand order (src/jet/fdm_gauss_seidel_solver3.cpp:110-116)
+    const idx_t sx = 1, sy = d.nx, sz = (idx_t)d.nx * d.ny;
+    TC s = (i > 0) ? (TC)A[c - sx].r * (TC)x[c - sx] : TC(0);
+    s += (i + 1 < d.nx) ? (TC)row.r * (TC)x[c + sx] : TC(0);
+    s += (j > 0) ? (TC)A[c - sy].u * (TC)x[c - sy] : TC(0);
+    s += (j + 1 < d.ny) ? (TC)row.u * (TC)x[c + sy] : TC(0);
+    s += (k > 0) ? (TC)A[c - sz].f * (TC)x[c - sz] : TC(0);
+    s += (k + 1 < d.nz) ? (TC)row.f * (TC)x[c + sz] : TC(0);
+    return s;
+}
+
+}  // namespace jg

@@ -1,0 +1,174 @@
+// jg_internal.h -- solver state shared by the translation units of
+// libjetgpu (not part of the public ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/jetgpu.h"
+#include "jg_device.h"
+
+#define JG_HIP(call)                                                             \
+    do {                                                                         \
+        hipError_t e_ = (call);                                                  \
+        if (e_ != hipSuccess) {                                                  \
+            s->err = std::string(#call) + ": " + hipGetErrorString(e_);          \
+            return e_ == hipErrorOutOfMemory ? JG_ERR_OUT_OF_MEMORY : JG_ERR_HIP; \
+        }                                                                        \
+    } while (0)
+
+#define JG_TRY(expr)            \
+    do {                        \
+        int st_ = (expr);       \
+        if (st_ != JG_OK) return st_; \
+    } while (0)
+
+namespace jg {
+
+struct Level {
+    Dims d{0, 0, 0};
+    Row<double>* A64 = nullptr;
+    Row<float>* A32 = nullptr;
+    // Preconditioner work vectors (levels > 0 use x/b; every level uses t).
+    double *x64 = nullptr, *b64 = nullptr, *t64 = nullptr;
+    float *x32 = nullptr, *b32 = nullptr, *t32 = nullptr;
+    int8_t* markers = nullptr;  // pressure-system markers of this level
+};
+
+template <typename T>
+struct Prec;
+template <>
+struct Prec<double> {
+    static Row<double>* A(Level& l) { return l.A64; }
+    static double* x(Level& l) { return l.x64; }
+    static double* b(Level& l) { return l.b64; }
+    static double* t(Level& l) { return l.t64; }
+};
+template <>
+struct Prec<float> {
+    static Row<float>* A(Level& l) { return l.A32; }
+    static float* x(Level& l) { return l.x32; }
+    static float* b(Level& l) { return l.b32; }
+    static float* t(Level& l) { return l.t32; }
+};
+
+struct Geo {
+    dim3 grid, block;
+    int zc;
+};
+
+inline int pow2ceil(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+// 2.5-D launch geometry: bx*by = 256 columns per block, zc planes per block.
+inline Geo geo_stencil(const Dims& d) {
+    Geo g;
+    const int bx = std::min(64, pow2ceil(std::max(1, d.nx)));
+    const int by = NT / bx;
+    const int gx = (d.nx + bx - 1) / bx;
+    const int gy = (d.ny + by - 1) / by;
+    const long cols = (long)gx * gy;
+    // Aim for >= 2048 blocks; at most 8 planes per block; bounded partials.
+    long zc = std::max<long>(1, std::min<long>(8, cols * d.nz / 2048));
+    while ((long)gx * gy * ((d.nz + zc - 1) / zc) > kMaxPartials) ++zc;
+    g.zc = (int)zc;
+    g.block = dim3(bx, by, 1);
+    g.grid = dim3(gx, gy, (d.nz + g.zc - 1) / g.zc);
+    return g;
+}
+
+// One thread per cell of a plane, one block row of planes per z.
+inline Geo geo_planes(const Dims& d) {
+    Geo g;
+    const int bx = std::min(64, pow2ceil(std::max(1, d.nx)));
+    const int by = NT / bx;
+    g.zc = 1;
+    g.block = dim3(bx, by, 1);
+    g.grid = dim3((d.nx + bx - 1) / bx, (d.ny + by - 1) / by, d.nz);
+    return g;
+}
+
+inline int nblocks_1d(idx_t n) {
+    idx_t b = (n + NT - 1) / NT;
+    return (int)std::max<idx_t>(1, std::min<idx_t>(b, 4096));
+}
+
+inline int nblocks(const Geo& g) { return (int)(g.grid.x * g.grid.y * g.grid.z); }
+
+}  // namespace jg
+
+struct jg_solver {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    std::vector<jg::Level> lv;
+    // Finest-level Krylov vectors.
+    double *b = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *dA = nullptr, *dB = nullptr,
+           *q = nullptr;
+    float *rhs32 = nullptr, *r32 = nullptr, *z32 = nullptr, *d32A = nullptr, *d32B = nullptr,
+          *q32 = nullptr, *e32 = nullptr;
+    // Reduction slots and device scalars.
+    double* part = nullptr;  // kSlots * kMaxPartials
+    double* scal = nullptr;  // small result area
+    jg::Ctl* ctl = nullptr;      // [0]: FP64 / outer, [1]: FP32 inner
+    jg::Ctl* ctl_host = nullptr; // pinned mirror
+    bool f32_valid = false;
+    std::vector<void*> allocs;
+
+    static constexpr int kSlots = 4;
+    double* slot(int i) { return part + (size_t)i * jg::kMaxPartials; }
+
+    ~jg_solver() {
+        release();
+        if (ctl_host) (void)hipHostFree(ctl_host);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    void release() {
+        for (void* p : allocs) (void)hipFree(p);
+        allocs.clear();
+        lv.clear();
+        b = x = r = z = dA = dB = q = nullptr;
+        rhs32 = r32 = z32 = d32A = d32B = q32 = e32 = nullptr;
+        part = scal = nullptr;
+        ctl = nullptr;
+        f32_valid = false;
+    }
+};
+
+namespace jg {
+
+template <typename T>
+inline int dalloc(jg_solver* s, T** p, size_t count) {
+    void* q = nullptr;
+    JG_HIP(hipMalloc(&q, std::max<size_t>(1, count) * sizeof(T)));
+    s->allocs.push_back(q);
+    *p = (T*)q;
+    return JG_OK;
+}
+
+inline int check_launch(jg_solver* s, const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        s->err = std::string(what) + ": " + hipGetErrorString(e);
+        return JG_ERR_HIP;
+    }
+    return JG_OK;
+}
+
+inline int sync(jg_solver* s) {
+    JG_HIP(hipStreamSynchronize(s->stream));
+    return JG_OK;
+}
+
+}  // namespace jg

@@ -1,0 +1,484 @@
+// jg_kernels.h -- HIP kernels of the gfx950 pressure solver.
+//
+// Geometry: a 256-thread block covers a bx*by tile of (i, j) columns and
+// marches over a chunk of zc planes, keeping the z-neighbours of the current
+// plane in registers.  x/y neighbours come from the block's own loads through
+// the vector L1 / L2.  Reductions write one FP64 partial per block (fixed
+// order) that a one-block finalize kernel sums, so every dot product is
+// deterministic and no cross-XCD fence is needed inside a kernel.
+#pragma once
+
+#include "jg_device.h"
+
+namespace jg {
+
+// ----------------------------------------------------------- SpMV (+ dot)
+
+// Jet's mvm operand order (src/jet/fdm_linear_system3.cpp:87-95): centre,
+// -x, +x, -y, +y, -z, +z, each product rounded before the add (the caller
+// disables contraction), so FP64 results are bitwise Jet's.  fm/xm and xp
+// are the z neighbours kept in registers (zero outside the domain).
+template <typename TA, typename TX>
+__device__ __forceinline__ TX stencil_sum(const Row<TA>* __restrict__ A, const TX* __restrict__ x,
+                                          const Row<TA>& row, idx_t c, int i, int j, const Dims& d,
+                                          TX xc, TA fm, TX xm, TX xp) {
+#pragma clang fp contract(off)
+    TX s = (TX)row.c * xc;
+    s += (i > 0) ? (TX)A[c - 1].r * x[c - 1] : TX(0);
+    s += (i + 1 < d.nx) ? (TX)row.r * x[c + 1] : TX(0);
+    s += (j > 0) ? (TX)A[c - d.nx].u * x[c - d.nx] : TX(0);
+    s += (j + 1 < d.ny) ? (TX)row.u * x[c + d.nx] : TX(0);
+    s += (TX)fm * xm;
+    s += (TX)row.f * xp;
+    return s;
+}
+
+
+// y = A x  (FdmBlas3::mvm, src/jet/fdm_linear_system3.cpp:79-96); optional
+// partial sum of x.y per block (the d.q of CG).
+template <typename TA, typename TX, bool DOT>
+__global__ __launch_bounds__(NT) void k_spmv(const Row<TA>* __restrict__ A,
+                                             const TX* __restrict__ x, TX* __restrict__ y,
+                                             Dims d, int zc, double* __restrict__ partial,
+                                             const Ctl* __restrict__ ctl) {
+    if (ctl && ctl->done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k0 = blockIdx.z * zc;
+    const int k1 = min(d.nz, k0 + zc);
+    double acc = 0.0;
+    if (i < d.nx && j < d.ny) {
+        const idx_t sz = (idx_t)d.nx * d.ny;
+        idx_t c = i + (idx_t)d.nx * (j + (idx_t)d.ny * k0);
+        TX xm = k0 > 0 ? x[c - sz] : TX(0);
+        TA fm = k0 > 0 ? A[c - sz].f : TA(0);
+        TX xc = x[c];
+        for (int k = k0; k < k1; ++k, c += sz) {
+            const Row<TA> row = A[c];
+            const TX xp = (k + 1 < d.nz) ? x[c + sz] : TX(0);
+            const TX s = stencil_sum<TA, TX>(A, x, row, c, i, j, d, xc, fm, xm, xp);
+            y[c] = s;
+            if (DOT) acc += (double)xc * (double)s;
+            xm = xc;
+            xc = xp;
+            fm = row.f;
+        }
+    }
+    if (DOT) block_partial<NT>(acc, partial);
+}
+
+// Fused CG direction update and SpMV:  d = z + beta*d_old (Jet's
+// axpy(beta, d, s, d)), q = A d, partial d.q.  d_old and d_new are separate
+// buffers because neighbours read d_old while d_new is written.
+template <typename TA, typename TX>
+__global__ __launch_bounds__(NT) void k_dir_spmv(const Row<TA>* __restrict__ A,
+                                                 const TX* __restrict__ z,
+                                                 const TX* __restrict__ dold,
+                                                 TX* __restrict__ dnew, TX* __restrict__ q,
+                                                 Dims d, int zc, double* __restrict__ partial,
+                                                 const Ctl* __restrict__ ctl) {
+#pragma clang fp contract(off)
+    if (ctl->done) return;
+    const double beta = ctl->beta;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k0 = blockIdx.z * zc;
+    const int k1 = min(d.nz, k0 + zc);
+    double acc = 0.0;
+    auto D = [&](idx_t n) -> TX { return (TX)((double)z[n] + beta * (double)dold[n]); };
+    if (i < d.nx && j < d.ny) {
+        const idx_t sz = (idx_t)d.nx * d.ny;
+        idx_t c = i + (idx_t)d.nx * (j + (idx_t)d.ny * k0);
+        TX xm = k0 > 0 ? D(c - sz) : TX(0);
+        TA fm = k0 > 0 ? A[c - sz].f : TA(0);
+        TX xc = D(c);
+        for (int k = k0; k < k1; ++k, c += sz) {
+            const Row<TA> row = A[c];
+            const TX xp = (k + 1 < d.nz) ? D(c + sz) : TX(0);
+            // Jet's mvm operand order (src/jet/fdm_linear_system3.cpp:87-95)
+            TX s = (TX)row.c * xc;
+            s += (i > 0) ? (TX)A[c - 1].r * D(c - 1) : TX(0);
+            s += (i + 1 < d.nx) ? (TX)row.r * D(c + 1) : TX(0);
+            s += (j > 0) ? (TX)A[c - d.nx].u * D(c - d.nx) : TX(0);
+            s += (j + 1 < d.ny) ? (TX)row.u * D(c + d.nx) : TX(0);
+            s += (TX)fm * xm;
+            s += (TX)row.f * xp;
+            q[c] = s;
+            dnew[c] = xc;
+            acc += (double)xc * (double)s;
+            xm = xc;
+            xc = xp;
+            fm = row.f;
+        }
+    }
+    block_partial<NT>(acc, partial);
+}
+
+// r = b - A x (FdmBlas3::residual, src/jet/fdm_linear_system3.cpp:98-116);
+// optional partial r.r.  When REFRESH, runs only on iterations whose Ctl
+// asks for Jet's true-residual recompute (include/jet/detail/cg-inl.h:58-63).
+template <typename TA, typename TX, typename TR, bool DOT, bool REFRESH>
+__global__ __launch_bounds__(NT) void k_residual(const Row<TA>* __restrict__ A,
+                                                 const TX* __restrict__ x,
+                                                 const TX* __restrict__ b, TR* __restrict__ r,
+                                                 Dims d, int zc, double* __restrict__ partial,
+                                                 const Ctl* __restrict__ ctl) {
+#pragma clang fp contract(off)
+    if (REFRESH && (ctl->done || !ctl->refresh)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k0 = blockIdx.z * zc;
+    const int k1 = min(d.nz, k0 + zc);
+    double acc = 0.0;
+    if (i < d.nx && j < d.ny) {
+        const idx_t sz = (idx_t)d.nx * d.ny;
+        idx_t c = i + (idx_t)d.nx * (j + (idx_t)d.ny * k0);
+        TX xm = k0 > 0 ? x[c - sz] : TX(0);
+        TA fm = k0 > 0 ? A[c - sz].f : TA(0);
+        TX xc = x[c];
+        for (int k = k0; k < k1; ++k, c += sz) {
+            const Row<TA> row = A[c];
+            const TX xp = (k + 1 < d.nz) ? x[c + sz] : TX(0);
+            // Jet's residual operand order (src/jet/fdm_linear_system3.cpp:107-115)
+            TX rv = b[c] - (TX)row.c * xc;
+            rv -= (i > 0) ? (TX)A[c - 1].r * x[c - 1] : TX(0);
+            rv -= (i + 1 < d.nx) ? (TX)row.r * x[c + 1] : TX(0);
+            rv -= (j > 0) ? (TX)A[c - d.nx].u * x[c - d.nx] : TX(0);
+            rv -= (j + 1 < d.ny) ? (TX)row.u * x[c + d.nx] : TX(0);
+            rv -= (TX)fm * xm;
+            rv -= (TX)row.f * xp;
+            r[c] = (TR)rv;
+            if (DOT) acc += (double)rv * (double)rv;
+            xm = xc;
+            xc = xp;
+            fm = row.f;
+        }
+    }
+    if (DOT) block_partial<NT>(acc, partial);
+}
+
+// ---------------------------------------------------- elementwise updates
+
+// CG step: x += alpha d; r -= alpha q (skipped on true-residual refresh
+// iterations, where k_residual<..., REFRESH> recomputes r from the new x).
+template <typename TX>
+__global__ __launch_bounds__(NT) void k_update_xr(TX* __restrict__ x, TX* __restrict__ r,
+                                                  const TX* __restrict__ dd,
+                                                  const TX* __restrict__ q, idx_t n,
+                                                  const Ctl* __restrict__ ctl) {
+#pragma clang fp contract(off)
+    if (ctl->done) return;
+    const double alpha = ctl->alpha;
+    const bool upd_r = !ctl->refresh;
+    for (idx_t c = blockIdx.x * (idx_t)NT + threadIdx.x; c < n; c += (idx_t)gridDim.x * NT) {
+        x[c] = (TX)((double)x[c] + alpha * (double)dd[c]);
+        if (upd_r) r[c] = (TX)((double)r[c] - alpha * (double)q[c]);
+    }
+}
+
+template <typename TA, typename TB>
+__global__ __launch_bounds__(NT) void k_dot(const TA* __restrict__ a, const TB* __restrict__ b,
+                                            idx_t n, double* __restrict__ partial,
+                                            const Ctl* __restrict__ ctl) {
+    if (ctl && ctl->done) return;
+    double acc = 0.0;
+    for (idx_t c = blockIdx.x * (idx_t)NT + threadIdx.x; c < n; c += (idx_t)gridDim.x * NT)
+        acc += (double)a[c] * (double)b[c];
+    block_partial<NT>(acc, partial);
+}
+
+template <typename TX>
+__global__ __launch_bounds__(NT) void k_axpy(double a, const TX* __restrict__ x,
+                                             const TX* __restrict__ y, TX* __restrict__ out,
+                                             idx_t n) {
+#pragma clang fp contract(off)
+    for (idx_t c = blockIdx.x * (idx_t)NT + threadIdx.x; c < n; c += (idx_t)gridDim.x * NT)
+        out[c] = a * x[c] + y[c];
+}
+
+template <typename TX>
+__global__ __launch_bounds__(NT) void k_fill(TX* __restrict__ x, TX v, idx_t n) {
+    for (idx_t c = blockIdx.x * (idx_t)NT + threadIdx.x; c < n; c += (idx_t)gridDim.x * NT)
+        x[c] = v;
+}
+
+template <typename TS, typename TD>
+__global__ __launch_bounds__(NT) void k_convert(const TS* __restrict__ s, TD* __restrict__ dst,
+                                                idx_t n) {
+    for (idx_t c = blockIdx.x * (idx_t)NT + threadIdx.x; c < n; c += (idx_t)gridDim.x * NT)
+        dst[c] = (TD)s[c];
+}
+
+// FP32 shadow of an FP64 matrix (mixed-precision preconditioner).
+__global__ __launch_bounds__(NT) void k_rows_to_f32(const Row<double>* __restrict__ s,
+                                                    Row<float>* __restrict__ dst, idx_t n) {
+    for (idx_t c = blockIdx.x * (idx_t)NT + threadIdx.x; c < n; c += (idx_t)gridDim.x * NT) {
+        const Row<double> r = s[c];
+        Row<float> o;
+        o.c = (float)r.c;
+        o.r = (float)r.r;
+        o.u = (float)r.u;
+        o.f = (float)r.f;
+        dst[c] = o;
+    }
+}
+
+// Mixed precision: x64 += e32 (iterative-refinement correction).
+__global__ __launch_bounds__(NT) void k_add_correction(double* __restrict__ x,
+                                                       const float* __restrict__ e, idx_t n) {
+    for (idx_t c = blockIdx.x * (idx_t)NT + threadIdx.x; c < n; c += (idx_t)gridDim.x * NT)
+        x[c] += (double)e[c];
+}
+
+// ------------------------------------------------------------- smoothers
+
+// One colour of a red-black SOR sweep (FdmGaussSeidelSolver3::relaxRedBlack,
+// src/jet/fdm_gauss_seidel_solver3.cpp:150-221): red = (i+j+k) even.
+// ZERO_INIT: x is treated as all zeros before this red pass (first
+// pre-smoothing sweep from a zero initial guess): red cells get omega*b/c,
+// black cells 0 -- no read of x.
+template <typename TA, typename TX, bool ZERO_INIT>
+__global__ __launch_bounds__(NT) void k_rbgs(const Row<TA>* __restrict__ A,
+                                             const TX* __restrict__ b, TX* __restrict__ x,
+                                             Dims d, int zc, int color, TX omega,
+                                             const Ctl* __restrict__ ctl) {
+#pragma clang fp contract(off)
+    if (ctl && ctl->done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= d.nx || j >= d.ny) return;
+    const int k0 = blockIdx.z * zc;
+    const int k1 = min(d.nz, k0 + zc);
+    idx_t c = i + (idx_t)d.nx * (j + (idx_t)d.ny * k0);
+    const idx_t sz = (idx_t)d.nx * d.ny;
+    for (int k = k0; k < k1; ++k, c += sz) {
+        const bool mine = ((i + j + k) & 1) == color;
+        if (ZERO_INIT) {
+            x[c] = mine ? omega * b[c] / (TX)A[c].c : TX(0);
+        } else if (mine) {
+            const Row<TA> row = A[c];
+            const TX s = offdiag<TA, TX, TX>(A, x, row, c, i, j, k, d);
+            x[c] = (TX(1) - omega) * x[c] + omega * (b[c] - s) / (TX)row.c;
+        }
+    }
+}
+
+// Weighted Jacobi sweep: xout = x + omega (b - A x) / c  (omega = 1 is
+// FdmJacobiSolver3::relax, src/jet/fdm_jacobi_solver3.cpp:95-112).
+template <typename TA, typename TX>
+__global__ __launch_bounds__(NT) void k_jacobi(const Row<TA>* __restrict__ A,
+                                               const TX* __restrict__ b,
+                                               const TX* __restrict__ x, TX* __restrict__ xo,
+                                               Dims d, int zc, TX omega,
+                                               const Ctl* __restrict__ ctl) {
+#pragma clang fp contract(off)
+    if (ctl && ctl->done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= d.nx || j >= d.ny) return;
+    const int k0 = blockIdx.z * zc;
+    const int k1 = min(d.nz, k0 + zc);
+    idx_t c = i + (idx_t)d.nx * (j + (idx_t)d.ny * k0);
+    const idx_t sz = (idx_t)d.nx * d.ny;
+    for (int k = k0; k < k1; ++k, c += sz) {
+        const Row<TA> row = A[c];
+        const TX s = offdiag<TA, TX, TX>(A, x, row, c, i, j, k, d);
+        if (omega == TX(1))
+            xo[c] = (b[c] - s) / (TX)row.c;
+        else
+            xo[c] = x[c] + omega * ((b[c] - s) / (TX)row.c - x[c]);
+    }
+}
+
+// ------------------------------------------------------------- transfers
+
+// FdmMgUtils3::restrict (src/jet/fdm_mg_linear_system3.cpp:43-98): coarse
+// (i,j,k) = sum w_x w_y w_z fine(idx), 1-D weights {1/8,3/8,3/8,1/8} over
+// {2i-1, 2i, 2i+1, 2i+2} clamped at both ends.  One thread per coarse cell.
+template <typename TX>
+__global__ __launch_bounds__(NT) void k_restrict(const TX* __restrict__ fine,
+                                                 TX* __restrict__ coarse, Dims cd,
+                                                 const Ctl* __restrict__ ctl) {
+#pragma clang fp contract(off)
+    if (ctl && ctl->done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= cd.nx || j >= cd.ny) return;
+    const int fnx = 2 * cd.nx, fny = 2 * cd.ny;
+    const TX kern[4] = {TX(0.125), TX(0.375), TX(0.375), TX(0.125)};
+    const int ii[4] = {i > 0 ? 2 * i - 1 : 2 * i, 2 * i, 2 * i + 1, i + 1 < cd.nx ? 2 * i + 2 : 2 * i + 1};
+    const int jj[4] = {j > 0 ? 2 * j - 1 : 2 * j, 2 * j, 2 * j + 1, j + 1 < cd.ny ? 2 * j + 2 : 2 * j + 1};
+    const int kk[4] = {k > 0 ? 2 * k - 1 : 2 * k, 2 * k, 2 * k + 1, k + 1 < cd.nz ? 2 * k + 2 : 2 * k + 1};
+    // Jet's accumulation order (z, y, x; weight product first).
+    TX sum = 0;
+#pragma unroll
+    for (int z = 0; z < 4; ++z)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            const TX* row = fine + (idx_t)fnx * (jj[y] + (idx_t)fny * kk[z]);
+#pragma unroll
+            for (int x = 0; x < 4; ++x) {
+                const TX w = kern[x] * kern[y] * kern[z];
+                sum += w * row[ii[x]];
+            }
+        }
+    coarse[i + (idx_t)cd.nx * (j + (idx_t)cd.ny * k)] = sum;
+}
+
+// FdmMgUtils3::correct (src/jet/fdm_mg_linear_system3.cpp:100-179): fine +=
+// trilinear (1/4, 3/4) prolongation of coarse, clamped.  Jet clamps the k
+// upper index against n.y (:159); this kernel clamps against min(n.y, n.z),
+// which equals Jet wherever Jet stays in bounds (n.y <= n.z) and stays in
+// bounds where Jet would read past the coarse array (n.y > n.z).
+template <typename TX>
+__global__ __launch_bounds__(NT) void k_correct(const TX* __restrict__ coarse,
+                                                TX* __restrict__ fine, Dims fd,
+                                                const Ctl* __restrict__ ctl) {
+#pragma clang fp contract(off)
+    if (ctl && ctl->done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= fd.nx || j >= fd.ny) return;
+    const int cnx = fd.nx / 2, cny = fd.ny / 2;
+    const int kclamp = min(fd.ny, fd.nz);
+    const int ci = i / 2, cj = j / 2, ck = k / 2;
+    int i0, i1, j0, j1, k0, k1;
+    TX wi0, wi1, wj0, wj1, wk0, wk1;
+    if ((i & 1) == 0) { i0 = i > 1 ? ci - 1 : ci; i1 = ci; wi0 = TX(0.25); wi1 = TX(0.75); }
+    else { i0 = ci; i1 = i + 1 < fd.nx ? ci + 1 : ci; wi0 = TX(0.75); wi1 = TX(0.25); }
+    if ((j & 1) == 0) { j0 = j > 1 ? cj - 1 : cj; j1 = cj; wj0 = TX(0.25); wj1 = TX(0.75); }
+    else { j0 = cj; j1 = j + 1 < fd.ny ? cj + 1 : cj; wj0 = TX(0.75); wj1 = TX(0.25); }
+    if ((k & 1) == 0) { k0 = k > 1 ? ck - 1 : ck; k1 = ck; wk0 = TX(0.25); wk1 = TX(0.75); }
+    else { k0 = ck; k1 = k + 1 < kclamp ? ck + 1 : ck; wk0 = TX(0.75); wk1 = TX(0.25); }
+    auto C = [&](int a, int b_, int c_) { return coarse[a + (idx_t)cnx * (b_ + (idx_t)cny * c_)]; };
+    const int iI[2] = {i0, i1}, jI[2] = {j0, j1}, kI[2] = {k0, k1};
+    const TX iW[2] = {wi0, wi1}, jW[2] = {wj0, wj1}, kW[2] = {wk0, wk1};
+    // Jet's accumulation order: eight separate += of (wi*wj*wk*coarse).
+    const idx_t f = i + (idx_t)fd.nx * (j + (idx_t)fd.ny * k);
+    TX v = fine[f];
+#pragma unroll
+    for (int z = 0; z < 2; ++z)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int x = 0; x < 2; ++x) v += iW[x] * jW[y] * kW[z] * C(iI[x], jI[y], kI[z]);
+    fine[f] = v;
+}
+
+// Coarsest level: symmetric red-black sweeps by one block, whole level
+// staged in LDS (A rows, b, x), x starts at zero.  fwd forward (red, black)
+// sweeps, then bwd backward (black, red) sweeps.
+template <typename TA, typename TX, int MAXN>
+__global__ __launch_bounds__(1024) void k_coarsest_lds(const Row<TA>* __restrict__ A,
+                                                       const TX* __restrict__ b,
+                                                       TX* __restrict__ x, Dims d, int fwd,
+                                                       int bwd, TX omega,
+                                                       const Ctl* __restrict__ ctl) {
+#pragma clang fp contract(off)
+    if (ctl && ctl->done) return;
+    __shared__ Row<TA> sA[MAXN];
+    __shared__ TX sb[MAXN];
+    __shared__ TX sx[MAXN];
+    const int n = d.nx * d.ny * d.nz;
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+        sA[c] = A[c];
+        sb[c] = b[c];
+        sx[c] = 0;
+    }
+    __syncthreads();
+    const int total = 2 * (fwd + bwd);
+    for (int h = 0; h < total; ++h) {
+        const int sweep = h / 2;
+        const int color = sweep < fwd ? (h & 1) : 1 - (h & 1);
+        for (int c = threadIdx.x; c < n; c += blockDim.x) {
+            const int i = c % d.nx;
+            const int j = (c / d.nx) % d.ny;
+            const int k = c / (d.nx * d.ny);
+            if (((i + j + k) & 1) != color) continue;
+            const Row<TA> row = sA[c];
+            const TX s = offdiag<TA, TX, TX>(sA, sx, row, c, i, j, k, d);
+            sx[c] = (TX(1) - omega) * sx[c] + omega * (sb[c] - s) / (TX)row.c;
+        }
+        __syncthreads();
+    }
+    for (int c = threadIdx.x; c < n; c += blockDim.x) x[c] = sx[c];
+}
+
+// ------------------------------------------------------------- finalize
+
+// Initial PCG scalars: sigma = r.z, rr = r.r, bb = b.b (for the relative
+// rule), beta = 0 so the first fused direction update gives d = z.
+__global__ __launch_bounds__(NT) void k_fin_init(Ctl* ctl, const double* p_rz, const double* p_rr,
+                                                 int n) {
+    __shared__ double sh[NT / kWave];
+    const double rz = reduce_partials<NT>(p_rz, n, sh);
+    const double rr = reduce_partials<NT>(p_rr, n, sh);
+    if (threadIdx.x == 0) {
+        ctl->sigma = rz;
+        ctl->sigma_old = rz;
+        ctl->rr = rr;
+        ctl->bb = rr;
+        ctl->beta = 0.0;
+        ctl->alpha = 0.0;
+        ctl->iter = 0;
+        ctl->trigger = 0;
+        ctl->refresh = 0;
+        ctl->nonfinite = !isfinite(rz) || !isfinite(rr);
+        const bool go = (ctl->stop_rule == 1) ? (rr > ctl->rel2 * ctl->bb) : (rz > ctl->tol2);
+        ctl->done = !(go && ctl->iter < ctl->max_iter) || ctl->nonfinite;
+    }
+}
+
+// alpha = sigma / d.q ; Jet's refresh rule for this iteration.
+__global__ __launch_bounds__(NT) void k_fin_alpha(Ctl* ctl, const double* p_dq, int n) {
+    if (ctl->done) return;
+    __shared__ double sh[NT / kWave];
+    const double dq = reduce_partials<NT>(p_dq, n, sh);
+    if (threadIdx.x == 0) {
+        ctl->dq = dq;
+        ctl->alpha = ctl->sigma / dq;
+        ctl->refresh = ctl->trigger || (ctl->iter % 50 == 0 && ctl->iter > 0);
+        ctl->trigger = 0;
+        if (!isfinite(ctl->alpha)) {
+            ctl->nonfinite = 1;
+            ctl->done = 1;
+        }
+    }
+}
+
+// End of a PCG iteration (include/jet/detail/cg-inl.h:65-90): sigma_new =
+// r.z, trigger on growth, beta, iteration count and the stop test.
+__global__ __launch_bounds__(NT) void k_fin_sigma(Ctl* ctl, const double* p_rz, const double* p_rr,
+                                                  int n) {
+    if (ctl->done) return;
+    __shared__ double sh[NT / kWave];
+    const double rz = reduce_partials<NT>(p_rz, n, sh);
+    const double rr = reduce_partials<NT>(p_rr, n, sh);
+    if (threadIdx.x == 0) {
+        const double sold = ctl->sigma;
+        ctl->sigma_old = sold;
+        ctl->sigma = rz;
+        ctl->rr = rr;
+        if (rz > sold) ctl->trigger = 1;
+        ctl->beta = rz / sold;
+        ctl->iter += 1;
+        bool go;
+        if (ctl->stop_rule == 1)
+            go = rr > ctl->rel2 * ctl->bb;
+        else
+            go = rz > ctl->tol2;
+        if (!isfinite(rz) || !isfinite(rr)) ctl->nonfinite = 1;
+        ctl->done = !(go && ctl->iter < ctl->max_iter) || ctl->nonfinite;
+    }
+}
+
+// Generic: result = sum of partials (used by the BLAS entry points).
+__global__ __launch_bounds__(NT) void k_fin_sum(const double* p, int n, double* out) {
+    __shared__ double sh[NT / kWave];
+    const double s = reduce_partials<NT>(p, n, sh);
+    if (threadIdx.x == 0) *out = s;
+}
+
+}  // namespace jg

@@ -1,0 +1,228 @@
+// jg_pressure.hip -- C ABI of the pressure-system assembly, the gradient
+// projection and the synthetic-scene generators.
+#include <thread>
+
+#include "jg_internal.h"
+#include "jg_pressure.h"
+
+using namespace jg;
+
+namespace {
+
+// Scene parameters from the description (stream 4 of the RNG).  The plane
+// of JG_SCENE_SPHERE_POOL is chosen afterwards from row counts.
+int fill_scene(const jg_scene_desc* d, synth::Scene* s) {
+    if (!d || d->n < 2 || d->n > 4096) return JG_ERR_INVALID_ARG;
+    std::memset(s, 0, sizeof(*s));
+    s->kind = d->kind;
+    s->n = (int)d->n;
+    s->h = d->h > 0 ? d->h : 1.0;
+    s->seed = d->seed;
+    s->plane_y = 2.0 * synth::kBig;  // all fluid
+    const double n = (double)d->n;
+    const double L = n * s->h;
+    auto U = [&](uint64_t idx, double a, double b) { return synth::uniform(d->seed, 4, idx, a, b); };
+    switch (d->kind) {
+        case JG_SCENE_CLOSED_SPHERE:
+            s->nspheres = 1;
+            s->sc[0][0] = s->sc[0][1] = s->sc[0][2] = 0.5 * L;
+            s->sr[0] = 0.25 * L;
+            break;
+        case JG_SCENE_DAM_BREAK:
+            s->plane_y = 0.4 * L;
+            s->dam_x = 0.5 * L;
+            s->gravity_dt = -9.8 * (1.0 / 60.0);
+            break;
+        case JG_SCENE_SPHERE_POOL:
+            s->nspheres = 8;
+            for (int q = 0; q < 8; ++q) {
+                const double r = U(4 * q, 8.0, 32.0) * n / 256.0;
+                s->sr[q] = r * s->h;
+                for (int a = 0; a < 3; ++a) s->sc[q][a] = U(4 * q + 1 + a, r + 1.0, n - 1.0 - r) * s->h;
+            }
+            break;
+        case JG_SCENE_BATCH_MEMBER: {
+            s->nspheres = 1 + (int)(4.0 * U(100, 0.0, 1.0));
+            if (s->nspheres > 4) s->nspheres = 4;
+            for (int q = 0; q < s->nspheres; ++q) {
+                const double r = U(4 * q, n / 16.0, n / 6.0);
+                s->sr[q] = r * s->h;
+                for (int a = 0; a < 3; ++a) s->sc[q][a] = U(4 * q + 1 + a, r + 1.0, n - 1.0 - r) * s->h;
+            }
+            s->plane_y = U(101, 0.3, 0.8) * L;
+            break;
+        }
+        case JG_SCENE_TORUS_POOL:
+            s->torus = 1;
+            s->tc[0] = s->tc[1] = s->tc[2] = 0.5 * L;
+            s->tR = 150.0 / 512.0 * L;
+            s->tr = 50.0 / 512.0 * L;
+            s->plane_y = 0.35 * L;
+            break;
+        default:
+            return JG_ERR_INVALID_ARG;
+    }
+    return JG_OK;
+}
+
+// Fluid-below-plane height giving the requested fluid fraction: the lowest
+// whole row j* with sum_{j<j*} open(j) >= fraction * n^3.
+void choose_plane(synth::Scene* s, const std::vector<unsigned>& open, double fraction) {
+    const double goal = fraction * (double)s->n * s->n * s->n;
+    double acc = 0.0;
+    int j = 0;
+    while (j < s->n && acc < goal) acc += open[j++];
+    s->plane_y = s->h * (double)j;
+}
+
+template <typename F>
+void parallel_planes(int nz, F f) {
+    const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&, t]() {
+            for (int k = t; k < nz; k += nt) f(k);
+        });
+    for (auto& x : th) x.join();
+}
+
+dim3 plane_grid(int nx, int ny, int nz, dim3* block) {
+    const int bx = std::min(64, pow2ceil(std::max(1, nx)));
+    *block = dim3(bx, 256 / bx, 1);
+    return dim3((nx + bx - 1) / bx, (ny + block->y - 1) / block->y, nz);
+}
+
+}  // namespace
+
+extern "C" {
+
+int jg_scene_generate_host(const jg_scene_desc* d, double* u, double* v, double* w,
+                           double* bsdf_cc, double* fsdf_cc, double* bsdf_vertex) {
+    synth::Scene s;
+    int st = fill_scene(d, &s);
+    if (st != JG_OK) return st;
+    const int n = s.n;
+    if (d->kind == JG_SCENE_SPHERE_POOL) {
+        std::vector<unsigned> open(n, 0);
+        std::vector<std::vector<unsigned>> part(n, std::vector<unsigned>(n, 0));
+        parallel_planes(n, [&](int k) {
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) part[k][j] += synth::solid_cell(s, i, j, k) ? 0u : 1u;
+        });
+        for (int k = 0; k < n; ++k)
+            for (int j = 0; j < n; ++j) open[j] += part[k][j];
+        choose_plane(&s, open, d->fluid_fraction > 0 ? d->fluid_fraction : 0.6);
+    }
+    const long n1 = n + 1;
+    parallel_planes(n + 1, [&](int k) {
+        for (int j = 0; j <= n; ++j)
+            for (int i = 0; i <= n; ++i) {
+                if (u && k < n && j < n) u[i + n1 * (j + (long)n * k)] = synth::face_velocity(s, 0, i, j, k);
+                if (v && k < n && i < n) v[i + (long)n * (j + n1 * k)] = synth::face_velocity(s, 1, i, j, k);
+                if (w && j < n && i < n) w[i + (long)n * (j + (long)n * k)] = synth::face_velocity(s, 2, i, j, k);
+                if (k < n && j < n && i < n) {
+                    const long c = i + (long)n * (j + (long)n * k);
+                    const double x = synth::cc(s, i), y = synth::cc(s, j), z = synth::cc(s, k);
+                    if (bsdf_cc) bsdf_cc[c] = synth::boundary_sdf(s, x, y, z);
+                    if (fsdf_cc) fsdf_cc[c] = synth::fluid_sdf(s, x, y, z);
+                }
+                if (bsdf_vertex)
+                    bsdf_vertex[i + n1 * (j + n1 * k)] =
+                        synth::boundary_sdf(s, s.h * (double)i, s.h * (double)j, s.h * (double)k);
+            }
+    });
+    return JG_OK;
+}
+
+int jg_scene_generate_device(jg_solver* s, const jg_scene_desc* d, double* u, double* v,
+                             double* w, double* bsdf_cc, double* fsdf_cc, double* bsdf_vertex) {
+    if (!s) return JG_ERR_INVALID_ARG;
+    synth::Scene sc;
+    int st = fill_scene(d, &sc);
+    if (st != JG_OK) return st;
+    JG_HIP(hipSetDevice(s->device));
+    const int n = sc.n;
+    dim3 blk;
+    if (d->kind == JG_SCENE_SPHERE_POOL) {
+        unsigned* rows = nullptr;
+        JG_HIP(hipMalloc(&rows, n * sizeof(unsigned)));
+        std::vector<unsigned> open(n, 0);
+        hipError_t e = hipMemsetAsync(rows, 0, n * sizeof(unsigned), s->stream);
+        if (e == hipSuccess) {
+            dim3 g = plane_grid(n, n, n, &blk);
+            k_synth_row_open<<<g, blk, 0, s->stream>>>(sc, rows);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(open.data(), rows, n * sizeof(unsigned), hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        (void)hipFree(rows);
+        if (e != hipSuccess) {
+            s->err = std::string("scene row count: ") + hipGetErrorString(e);
+            return JG_ERR_HIP;
+        }
+        choose_plane(&sc, open, d->fluid_fraction > 0 ? d->fluid_fraction : 0.6);
+    }
+    if (u) k_synth_faces<<<plane_grid(n + 1, n, n, &blk), blk, 0, s->stream>>>(sc, 0, u);
+    if (v) k_synth_faces<<<plane_grid(n, n + 1, n, &blk), blk, 0, s->stream>>>(sc, 1, v);
+    if (w) k_synth_faces<<<plane_grid(n, n, n + 1, &blk), blk, 0, s->stream>>>(sc, 2, w);
+    if (bsdf_cc && fsdf_cc)
+        k_synth_cells<<<plane_grid(n, n, n, &blk), blk, 0, s->stream>>>(sc, bsdf_cc, fsdf_cc);
+    else if (bsdf_cc || fsdf_cc) {
+        s->err = "bsdf_cc and fsdf_cc are generated together";
+        return JG_ERR_INVALID_ARG;
+    }
+    if (bsdf_vertex)
+        k_synth_vertices<<<plane_grid(n + 1, n + 1, n + 1, &blk), blk, 0, s->stream>>>(sc, bsdf_vertex);
+    JG_TRY(check_launch(s, "scene"));
+    return sync(s);
+}
+
+int jg_sp_build(jg_solver* s, const double* u, const double* v, const double* w,
+                const double* boundary_sdf, const double* fluid_sdf, double hx, double hy,
+                double hz) {
+    if (!s || s->lv.empty() || !u || !v || !w || !boundary_sdf || !fluid_sdf || !(hx > 0) ||
+        !(hy > 0) || !(hz > 0))
+        return JG_ERR_INVALID_ARG;
+    JG_HIP(hipSetDevice(s->device));
+    const Dims& d0 = s->lv[0].d;
+    k_sp_markers<<<nblocks_1d(d0.n()), 256, 0, s->stream>>>(boundary_sdf, fluid_sdf,
+                                                            s->lv[0].markers, d0.n());
+    for (size_t l = 1; l < s->lv.size(); ++l) {
+        Geo g = geo_planes(s->lv[l].d);
+        k_sp_coarsen<<<g.grid, g.block, 0, s->stream>>>(s->lv[l - 1].markers, s->lv[l].markers,
+                                                        s->lv[l].d);
+    }
+    H3 h{hx, hy, hz};
+    for (size_t l = 0; l < s->lv.size(); ++l) {
+        Geo g = geo_planes(s->lv[l].d);
+        k_sp_build<<<g.grid, g.block, 0, s->stream>>>(s->lv[l].markers, l == 0 ? u : nullptr, v, w,
+                                                      s->lv[l].d, h, s->lv[l].A64,
+                                                      l == 0 ? s->b : nullptr);
+        h.x *= 2.0;
+        h.y *= 2.0;
+        h.z *= 2.0;
+    }
+    s->f32_valid = false;
+    JG_TRY(check_launch(s, "sp_build"));
+    return sync(s);
+}
+
+int jg_sp_apply_gradient(jg_solver* s, const double* u, const double* v, const double* w,
+                         double* uo, double* vo, double* wo, double hx, double hy, double hz) {
+    if (!s || s->lv.empty() || !u || !v || !w || !uo || !vo || !wo) return JG_ERR_INVALID_ARG;
+    JG_HIP(hipSetDevice(s->device));
+    const Dims& d0 = s->lv[0].d;
+    Geo g = geo_planes(d0);
+    k_sp_gradient<<<g.grid, g.block, 0, s->stream>>>(s->lv[0].markers, s->x, u, v, w, uo, vo, wo,
+                                                     d0, H3{hx, hy, hz});
+    JG_TRY(check_launch(s, "sp_gradient"));
+    return sync(s);
+}
+
+const signed char* jg_solver_markers(jg_solver* s, size_t level) {
+    if (!s || level >= s->lv.size()) return nullptr;
+    return (const signed char*)s->lv[level].markers;
+}
+
+}  // extern "C"

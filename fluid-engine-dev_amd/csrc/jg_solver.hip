@@ -1,0 +1,693 @@
+// jg_solver.hip -- solver objects, CG / MGPCG drivers and the C ABI.
+//
+// Host code only launches kernels; every scalar of the Krylov loop (alpha,
+// beta, sigma, the stop test) lives in device memory (jg::Ctl) and is
+// computed by one-block finalize kernels, so iterations run back to back on
+// one stream and the host only polls the stop flag.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "jg_internal.h"
+#include "jg_kernels.h"
+
+using namespace jg;
+
+namespace {
+
+// ------------------------------------------------------- level kernels
+
+template <typename TA, typename TX>
+void launch_rbgs(jg_solver* s, const Row<TA>* A, const TX* b, TX* x, const Dims& d, int color,
+                 TX omega, bool zero_init, const Ctl* ctl) {
+    Geo g = geo_stencil(d);
+    if (zero_init)
+        k_rbgs<TA, TX, true><<<g.grid, g.block, 0, s->stream>>>(A, b, x, d, g.zc, color, omega, ctl);
+    else
+        k_rbgs<TA, TX, false><<<g.grid, g.block, 0, s->stream>>>(A, b, x, d, g.zc, color, omega, ctl);
+}
+
+template <typename T>
+struct MgCfg {
+    unsigned pre, post0, post, coarsest;
+    T omega;
+};
+
+// z = M r: one symmetric V-cycle with zero initial guess.  Level 0 reads
+// its right-hand side from r0 and writes its result to z0.
+template <typename T>
+int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl) {
+    const size_t L = s->lv.size();
+    auto B = [&](size_t l) -> const T* { return l == 0 ? r0 : Prec<T>::b(s->lv[l]); };
+    auto X = [&](size_t l) -> T* { return l == 0 ? z0 : Prec<T>::x(s->lv[l]); };
+    for (size_t l = 0; l < L; ++l) {
+        Level& lv = s->lv[l];
+        const Row<T>* A = Prec<T>::A(lv);
+        const Dims& d = lv.d;
+        if (l + 1 == L) {
+            const int n = (int)d.n();
+            const int fwd = std::max(1u, (c.coarsest + 1) / 2);
+            if (n <= 1024) {
+                k_coarsest_lds<T, T, 1024><<<1, 1024, 0, s->stream>>>(A, B(l), X(l), d, fwd, fwd,
+                                                                      c.omega, ctl);
+            } else {
+                launch_rbgs<T, T>(s, A, B(l), X(l), d, 0, c.omega, true, ctl);
+                launch_rbgs<T, T>(s, A, B(l), X(l), d, 1, c.omega, false, ctl);
+                for (int it = 1; it < fwd; ++it) {
+                    launch_rbgs<T, T>(s, A, B(l), X(l), d, 0, c.omega, false, ctl);
+                    launch_rbgs<T, T>(s, A, B(l), X(l), d, 1, c.omega, false, ctl);
+                }
+                for (int it = 0; it < fwd; ++it) {
+                    launch_rbgs<T, T>(s, A, B(l), X(l), d, 1, c.omega, false, ctl);
+                    launch_rbgs<T, T>(s, A, B(l), X(l), d, 0, c.omega, false, ctl);
+                }
+            }
+            JG_TRY(check_launch(s, "coarsest"));
+            break;
+        }
+        // pre-smoothing: forward red-black sweeps from x = 0
+        if (c.pre == 0) {
+            k_fill<T><<<nblocks_1d(d.n()), NT, 0, s->stream>>>(X(l), T(0), d.n());
+        } else {
+            launch_rbgs<T, T>(s, A, B(l), X(l), d, 0, c.omega, true, ctl);
+            launch_rbgs<T, T>(s, A, B(l), X(l), d, 1, c.omega, false, ctl);
+            for (unsigned it = 1; it < c.pre; ++it) {
+                launch_rbgs<T, T>(s, A, B(l), X(l), d, 0, c.omega, false, ctl);
+                launch_rbgs<T, T>(s, A, B(l), X(l), d, 1, c.omega, false, ctl);
+            }
+        }
+        // residual -> restrict to the next level's right-hand side
+        Geo g = geo_stencil(d);
+        k_residual<T, T, T, false, false><<<g.grid, g.block, 0, s->stream>>>(
+            A, X(l), B(l), Prec<T>::t(lv), d, g.zc, nullptr, ctl);
+        const Dims& cd = s->lv[l + 1].d;
+        Geo gc = geo_planes(cd);
+        k_restrict<T><<<gc.grid, gc.block, 0, s->stream>>>(Prec<T>::t(lv), Prec<T>::b(s->lv[l + 1]),
+                                                           cd, ctl);
+        JG_TRY(check_launch(s, "vcycle down"));
+    }
+    for (size_t l = L - 1; l-- > 0;) {
+        Level& lv = s->lv[l];
+        const Row<T>* A = Prec<T>::A(lv);
+        const Dims& d = lv.d;
+        Geo gf = geo_planes(d);
+        k_correct<T><<<gf.grid, gf.block, 0, s->stream>>>(X(l + 1), X(l), d, ctl);
+        const unsigned post = l == 0 ? c.post0 : c.post;
+        for (unsigned it = 0; it < post; ++it) {
+            launch_rbgs<T, T>(s, A, B(l), X(l), d, 1, c.omega, false, ctl);
+            launch_rbgs<T, T>(s, A, B(l), X(l), d, 0, c.omega, false, ctl);
+        }
+        JG_TRY(check_launch(s, "vcycle up"));
+    }
+    return JG_OK;
+}
+
+// --------------------------------------------------------------- PCG
+
+// Jet's pcg control flow (include/jet/detail/cg-inl.h:15-94) on the device.
+// Vectors of type T; the matrix of level 0 is A; M is either the identity
+// (z aliases r) or the V-cycle.  x must be zero on entry (Jet zeroes it).
+template <typename T>
+int pcg_device(jg_solver* s, const Row<T>* A, const T* b, T* x, T* r, T* z, T* dA, T* dB, T* q,
+               const Dims& d, Ctl* ctl, const MgCfg<T>* mg, unsigned check_every) {
+    const idx_t n = d.n();
+    const int nb1 = nblocks_1d(n);
+    Geo g = geo_stencil(d);
+    const int nbs = nblocks(g);
+    double* p_rz = s->slot(0);
+    double* p_rr = s->slot(1);
+    double* p_dq = s->slot(2);
+    // r = b - A x  (x = 0, so r = b; keep the general form)
+    k_residual<T, T, T, false, false><<<g.grid, g.block, 0, s->stream>>>(A, x, b, r, d, g.zc,
+                                                                        nullptr, nullptr);
+    k_fill<T><<<nb1, NT, 0, s->stream>>>(dA, T(0), n);
+    k_fill<T><<<nb1, NT, 0, s->stream>>>(dB, T(0), n);
+    if (mg) JG_TRY(vcycle<T>(s, r, z, *mg, nullptr));
+    k_dot<T, T><<<nb1, NT, 0, s->stream>>>(r, z, n, p_rz, nullptr);
+    k_dot<T, T><<<nb1, NT, 0, s->stream>>>(r, r, n, p_rr, nullptr);
+    k_fin_init<<<1, NT, 0, s->stream>>>(ctl, p_rz, p_rr, nb1);
+    JG_TRY(check_launch(s, "pcg init"));
+    JG_HIP(hipMemcpyAsync(s->ctl_host, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
+    JG_TRY(sync(s));
+    if (s->ctl_host->done) return JG_OK;
+    T* dold = dA;
+    T* dnew = dB;
+    unsigned launched = 0;
+    const unsigned max_iter = s->ctl_host->max_iter;
+    while (true) {
+        k_dir_spmv<T, T><<<g.grid, g.block, 0, s->stream>>>(A, z, dold, dnew, q, d, g.zc, p_dq, ctl);
+        k_fin_alpha<<<1, NT, 0, s->stream>>>(ctl, p_dq, nbs);
+        k_update_xr<T><<<nb1, NT, 0, s->stream>>>(x, r, dnew, q, n, ctl);
+        k_residual<T, T, T, false, true><<<g.grid, g.block, 0, s->stream>>>(A, x, b, r, d, g.zc,
+                                                                           nullptr, ctl);
+        if (mg) JG_TRY(vcycle<T>(s, r, z, *mg, ctl));
+        k_dot<T, T><<<nb1, NT, 0, s->stream>>>(r, z, n, p_rz, ctl);
+        if (mg) k_dot<T, T><<<nb1, NT, 0, s->stream>>>(r, r, n, p_rr, ctl);
+        k_fin_sigma<<<1, NT, 0, s->stream>>>(ctl, p_rz, mg ? p_rr : p_rz, nb1);
+        JG_TRY(check_launch(s, "pcg iteration"));
+        std::swap(dold, dnew);
+        ++launched;
+        if (launched % check_every == 0 || launched >= max_iter) {
+            JG_HIP(hipMemcpyAsync(s->ctl_host, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
+            JG_TRY(sync(s));
+            if (s->ctl_host->done) break;
+            if (launched >= max_iter + 1) break;  // safety: the device stops at max_iter
+        }
+    }
+    return JG_OK;
+}
+
+void init_ctl(Ctl* h, unsigned max_iter, double tol, int stop_rule, double rel_tol) {
+    std::memset(h, 0, sizeof(Ctl));
+    h->max_iter = max_iter;
+    h->tol2 = tol * tol;
+    h->stop_rule = stop_rule;
+    h->rel2 = rel_tol * rel_tol;
+}
+
+int upload_ctl(jg_solver* s, Ctl* dev, const Ctl& h) {
+    JG_HIP(hipMemcpyAsync(dev, &h, sizeof(Ctl), hipMemcpyHostToDevice, s->stream));
+    return JG_OK;
+}
+
+int ensure_f32(jg_solver* s) {
+    if (s->f32_valid) return JG_OK;
+    for (auto& l : s->lv)
+        k_rows_to_f32<<<nblocks_1d(l.d.n()), NT, 0, s->stream>>>(l.A64, l.A32, l.d.n());
+    JG_TRY(check_launch(s, "rows_to_f32"));
+    s->f32_valid = true;
+    return JG_OK;
+}
+
+// FP64 ||b - A x|| / ||b|| of the finest level (reporting).
+int true_rel_residual(jg_solver* s, double* out) {
+    Level& l0 = s->lv[0];
+    Geo g = geo_stencil(l0.d);
+    const int nb1 = nblocks_1d(l0.d.n());
+    k_residual<double, double, double, true, false><<<g.grid, g.block, 0, s->stream>>>(
+        l0.A64, s->x, s->b, s->r, l0.d, g.zc, s->slot(0), nullptr);
+    k_fin_sum<<<1, NT, 0, s->stream>>>(s->slot(0), nblocks(g), s->scal + 0);
+    k_dot<double, double><<<nb1, NT, 0, s->stream>>>(s->b, s->b, l0.d.n(), s->slot(1), nullptr);
+    k_fin_sum<<<1, NT, 0, s->stream>>>(s->slot(1), nb1, s->scal + 1);
+    JG_TRY(check_launch(s, "true residual"));
+    double h[2];
+    JG_HIP(hipMemcpyAsync(h, s->scal, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+    JG_TRY(sync(s));
+    *out = h[1] > 0 ? std::sqrt(h[0] / h[1]) : std::sqrt(h[0]);
+    return JG_OK;
+}
+
+bool valid_dims(size_t nx, size_t ny, size_t nz) {
+    const size_t lim = (size_t)1 << 30;
+    return nx > 0 && ny > 0 && nz > 0 && nx < 65536 && ny < 65536 && nz < 65536 &&
+           nx * ny * nz < lim * 2;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+
+extern "C" {
+
+void jg_mg_params_default(jg_mg_params* p) {
+    // MgParameters defaults (include/jet/mg.h:58-86) and FdmMgSolver3's
+    // sorFactor / ordering defaults (include/jet/fdm_mg_solver3.h:24-30).
+    p->max_levels = 1;
+    p->restrict_iter = 5;
+    p->correct_iter = 5;
+    p->coarsest_iter = 20;
+    p->final_iter = 20;
+    p->max_tol = 1e-9;
+    p->sor = 1.5;
+    p->red_black = 0;
+}
+
+void jg_gpu_options_default(jg_gpu_options* o) {
+    std::memset(o, 0, sizeof(*o));
+    o->precision = JG_FP64;
+    o->stop_rule = JG_STOP_JET;
+    o->rel_tol = 1e-8;
+    o->smoother = JG_SMOOTH_RBGS;
+    o->omega = 0.0;
+    o->max_refine = 8;
+    o->inner_rel_tol = 1e-4;
+    o->check_every = 1;
+}
+
+int jg_solver_create(int device, jg_solver** out) {
+    if (!out) return JG_ERR_INVALID_ARG;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= device || device < 0)
+        return JG_ERR_NO_DEVICE;
+    jg_solver* s = new jg_solver();
+    s->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&s->stream) != hipSuccess ||
+        hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess ||
+        hipHostMalloc((void**)&s->ctl_host, 2 * sizeof(Ctl), hipHostMallocDefault) != hipSuccess) {
+        delete s;
+        return JG_ERR_HIP;
+    }
+    *out = s;
+    return JG_OK;
+}
+
+void jg_solver_destroy(jg_solver* s) {
+    if (!s) return;
+    (void)hipSetDevice(s->device);
+    (void)hipStreamSynchronize(s->stream);
+    delete s;
+}
+
+const char* jg_last_error(const jg_solver* s) { return s ? s->err.c_str() : "null solver"; }
+
+void* jg_solver_stream(jg_solver* s) { return s ? (void*)s->stream : nullptr; }
+
+int jg_solver_set_levels(jg_solver* s, size_t nlev, const size_t* nx, const size_t* ny,
+                         const size_t* nz) {
+    if (!s || nlev == 0 || nlev > 16 || !nx || !ny || !nz) return JG_ERR_INVALID_ARG;
+    for (size_t l = 0; l < nlev; ++l) {
+        if (!valid_dims(nx[l], ny[l], nz[l])) {
+            s->err = "invalid level size";
+            return JG_ERR_INVALID_ARG;
+        }
+        if (l > 0 && (nx[l - 1] != 2 * nx[l] || ny[l - 1] != 2 * ny[l] || nz[l - 1] != 2 * nz[l])) {
+            s->err = "level sizes must halve exactly";
+            return JG_ERR_INVALID_ARG;
+        }
+    }
+    bool same = s->lv.size() == nlev;
+    for (size_t l = 0; same && l < nlev; ++l)
+        same = s->lv[l].d.nx == (int)nx[l] && s->lv[l].d.ny == (int)ny[l] && s->lv[l].d.nz == (int)nz[l];
+    if (same) return JG_OK;
+    JG_HIP(hipSetDevice(s->device));
+    JG_HIP(hipStreamSynchronize(s->stream));
+    s->release();
+    s->lv.resize(nlev);
+    for (size_t l = 0; l < nlev; ++l) {
+        Level& lv = s->lv[l];
+        lv.d = Dims{(int)nx[l], (int)ny[l], (int)nz[l]};
+        const size_t n = lv.d.n();
+        JG_TRY(dalloc(s, &lv.A64, n));
+        JG_TRY(dalloc(s, &lv.A32, n));
+        JG_TRY(dalloc(s, &lv.t64, n));
+        JG_TRY(dalloc(s, &lv.t32, n));
+        JG_TRY(dalloc(s, &lv.markers, n));
+        if (l > 0) {
+            JG_TRY(dalloc(s, &lv.x64, n));
+            JG_TRY(dalloc(s, &lv.b64, n));
+            JG_TRY(dalloc(s, &lv.x32, n));
+            JG_TRY(dalloc(s, &lv.b32, n));
+        }
+    }
+    const size_t n0 = s->lv[0].d.n();
+    for (double** p : {&s->b, &s->x, &s->r, &s->z, &s->dA, &s->dB, &s->q}) JG_TRY(dalloc(s, p, n0));
+    for (float** p : {&s->rhs32, &s->r32, &s->z32, &s->d32A, &s->d32B, &s->q32, &s->e32})
+        JG_TRY(dalloc(s, p, n0));
+    JG_TRY(dalloc(s, &s->part, (size_t)jg_solver::kSlots * kMaxPartials));
+    JG_TRY(dalloc(s, &s->scal, 16));
+    JG_TRY(dalloc(s, &s->ctl, 2));
+    return JG_OK;
+}
+
+double* jg_solver_matrix(jg_solver* s, size_t level) {
+    if (!s || level >= s->lv.size()) return nullptr;
+    return (double*)s->lv[level].A64;
+}
+
+double* jg_solver_rhs(jg_solver* s) { return s ? s->b : nullptr; }
+
+double* jg_solver_solution(jg_solver* s) { return s ? s->x : nullptr; }
+
+int jg_solver_upload(jg_solver* s, size_t level, const double* rows, const double* b) {
+    if (!s || level >= s->lv.size() || !rows) return JG_ERR_INVALID_ARG;
+    JG_HIP(hipSetDevice(s->device));
+    const size_t n = s->lv[level].d.n();
+    JG_HIP(hipMemcpyAsync(s->lv[level].A64, rows, n * sizeof(Row<double>), hipMemcpyHostToDevice,
+                          s->stream));
+    if (b && level == 0)
+        JG_HIP(hipMemcpyAsync(s->b, b, n * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    s->f32_valid = false;
+    return sync(s);
+}
+
+int jg_solver_download_x(jg_solver* s, double* x) {
+    if (!s || s->lv.empty() || !x) return JG_ERR_INVALID_ARG;
+    JG_HIP(hipSetDevice(s->device));
+    JG_HIP(hipMemcpyAsync(x, s->x, s->lv[0].d.n() * sizeof(double), hipMemcpyDeviceToHost,
+                          s->stream));
+    return sync(s);
+}
+
+int jg_solver_matrices_changed(jg_solver* s) {
+    if (!s) return JG_ERR_INVALID_ARG;
+    s->f32_valid = false;
+    return JG_OK;
+}
+
+int jg_solver_cg(jg_solver* s, unsigned max_iter, double tol, jg_solve_info* info) {
+    if (!s || s->lv.empty()) return JG_ERR_INVALID_ARG;
+    JG_HIP(hipSetDevice(s->device));
+    Level& l0 = s->lv[0];
+    Ctl h;
+    init_ctl(&h, max_iter, tol, 0, 0.0);
+    JG_TRY(upload_ctl(s, s->ctl, h));
+    k_fill<double><<<nblocks_1d(l0.d.n()), NT, 0, s->stream>>>(s->x, 0.0, l0.d.n());
+    JG_HIP(hipEventRecord(s->ev0, s->stream));
+    JG_TRY(pcg_device<double>(s, l0.A64, s->b, s->x, s->r, s->r, s->dA, s->dB, s->q, l0.d, s->ctl,
+                              nullptr, 1));
+    JG_HIP(hipEventRecord(s->ev1, s->stream));
+    JG_HIP(hipMemcpyAsync(s->ctl_host, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
+    JG_TRY(sync(s));
+    const Ctl& c = *s->ctl_host;
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        float ms = 0.f;
+        JG_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        info->solve_ms = ms;
+        info->iterations = c.iter;
+        info->last_residual = std::sqrt(std::fabs(c.sigma));
+        // FdmCgSolver3::solve's return (src/jet/fdm_cg_solver3.cpp:45-46)
+        info->converged = (info->last_residual <= tol) || (c.iter < max_iter);
+        JG_TRY(true_rel_residual(s, &info->rel_residual));
+    }
+    if (c.nonfinite) {
+        s->err = "non-finite value in CG reduction";
+        return JG_ERR_NONFINITE;
+    }
+    return JG_OK;
+}
+
+int jg_solver_mgpcg(jg_solver* s, unsigned max_iter, const jg_mg_params* p,
+                    const jg_gpu_options* o, jg_solve_info* info) {
+    if (!s || s->lv.empty() || !p) return JG_ERR_INVALID_ARG;
+    JG_HIP(hipSetDevice(s->device));
+    jg_gpu_options opt;
+    if (o)
+        opt = *o;
+    else
+        jg_gpu_options_default(&opt);
+    if (opt.check_every == 0) opt.check_every = 1;
+    Level& l0 = s->lv[0];
+    const idx_t n = l0.d.n();
+    // Every level of the caller's hierarchy is used, as Jet's mgVCycle recurses
+    // over A.levels.size() (include/jet/detail/mg-inl.h:26); max_levels only
+    // tells callers how deep to build it.
+    const double omega = opt.omega > 0 ? opt.omega : 1.0;
+    int status = JG_OK;
+    unsigned iters = 0, refinements = 0;
+    double last_res = 0.0;
+    Ctl* ctl = s->ctl;
+    k_fill<double><<<nblocks_1d(n), NT, 0, s->stream>>>(s->x, 0.0, n);
+    if (opt.precision == JG_MIXED) JG_TRY(ensure_f32(s));
+    JG_HIP(hipEventRecord(s->ev0, s->stream));
+    if (opt.precision != JG_MIXED) {
+        MgCfg<double> mg{p->restrict_iter, p->final_iter, p->correct_iter, p->coarsest_iter, omega};
+        Ctl h;
+        init_ctl(&h, max_iter, p->max_tol, opt.stop_rule == JG_STOP_RELATIVE ? 1 : 0, opt.rel_tol);
+        JG_TRY(upload_ctl(s, ctl, h));
+        JG_TRY(pcg_device<double>(s, l0.A64, s->b, s->x, s->r, s->z, s->dA, s->dB, s->q, l0.d, ctl,
+                                  &mg, opt.check_every));
+        JG_HIP(hipMemcpyAsync(s->ctl_host, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
+        JG_TRY(sync(s));
+        iters = s->ctl_host->iter;
+        last_res = std::sqrt(std::fabs(s->ctl_host->sigma));
+        if (s->ctl_host->nonfinite) status = JG_ERR_NONFINITE;
+    } else {
+        // Iterative refinement: FP64 residual and correction, FP32 MGPCG
+        // (FP32 stencil, smoother and V-cycle; FP64 dot accumulation).
+        MgCfg<float> mg{p->restrict_iter, p->final_iter, p->correct_iter, p->coarsest_iter,
+                        (float)omega};
+        Geo g = geo_stencil(l0.d);
+        const int nb1 = nblocks_1d(n);
+        k_dot<double, double><<<nb1, NT, 0, s->stream>>>(s->b, s->b, n, s->slot(3), nullptr);
+        k_fin_sum<<<1, NT, 0, s->stream>>>(s->slot(3), nb1, s->scal + 2);
+        double bb = 0.0;
+        JG_HIP(hipMemcpyAsync(&bb, s->scal + 2, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        JG_TRY(sync(s));
+        const double bnorm = std::sqrt(bb);
+        const double goal = opt.stop_rule == JG_STOP_RELATIVE ? opt.rel_tol * bnorm : p->max_tol;
+        Ctl* ictl = s->ctl + 1;
+        for (unsigned outer = 0;; ++outer) {
+            k_residual<double, double, float, true, false><<<g.grid, g.block, 0, s->stream>>>(
+                l0.A64, s->x, s->b, s->rhs32, l0.d, g.zc, s->slot(3), nullptr);
+            k_fin_sum<<<1, NT, 0, s->stream>>>(s->slot(3), nblocks(g), s->scal + 3);
+            double rr = 0.0;
+            JG_HIP(hipMemcpyAsync(&rr, s->scal + 3, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+            JG_TRY(sync(s));
+            last_res = std::sqrt(rr);
+            if (!std::isfinite(rr)) {
+                status = JG_ERR_NONFINITE;
+                break;
+            }
+            if (last_res <= goal || outer >= opt.max_refine || iters >= max_iter) break;
+            Ctl h;
+            init_ctl(&h, max_iter - iters, 0.0, 1, opt.inner_rel_tol);
+            JG_TRY(upload_ctl(s, ictl, h));
+            k_fill<float><<<nb1, NT, 0, s->stream>>>(s->e32, 0.f, n);
+            JG_TRY(pcg_device<float>(s, l0.A32, s->rhs32, s->e32, s->r32, s->z32, s->d32A, s->d32B,
+                                     s->q32, l0.d, ictl, &mg, opt.check_every));
+            iters += s->ctl_host->iter;
+            k_add_correction<<<nb1, NT, 0, s->stream>>>(s->x, s->e32, n);
+            JG_TRY(check_launch(s, "refinement"));
+            ++refinements;
+        }
+    }
+    JG_HIP(hipEventRecord(s->ev1, s->stream));
+    JG_TRY(sync(s));
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        float ms = 0.f;
+        JG_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        info->solve_ms = ms;
+        info->iterations = iters;
+        info->refinements = refinements;
+        info->last_residual = last_res;
+        info->converged = (last_res <= p->max_tol) || (iters < max_iter);
+        JG_TRY(true_rel_residual(s, &info->rel_residual));
+        if (opt.stop_rule == JG_STOP_RELATIVE) info->converged = info->rel_residual <= opt.rel_tol;
+    }
+    if (status == JG_ERR_NONFINITE) s->err = "non-finite value in MGPCG reduction";
+    return status;
+}
+
+int jg_cg_solve_host(jg_solver* s, size_t nx, size_t ny, size_t nz, const double* rows,
+                     const double* b, double* x, unsigned max_iter, double tol,
+                     jg_solve_info* info) {
+    if (!s || !rows || !b || !x) return JG_ERR_INVALID_ARG;
+    size_t sx[1] = {nx}, sy[1] = {ny}, sz[1] = {nz};
+    JG_TRY(jg_solver_set_levels(s, 1, sx, sy, sz));
+    JG_TRY(jg_solver_upload(s, 0, rows, b));
+    JG_TRY(jg_solver_cg(s, max_iter, tol, info));
+    return jg_solver_download_x(s, x);
+}
+
+int jg_mgpcg_solve_host(jg_solver* s, size_t nlev, const size_t* nx, const size_t* ny,
+                        const size_t* nz, const double* const* rows, const double* b, double* x,
+                        unsigned max_iter, const jg_mg_params* p, const jg_gpu_options* o,
+                        jg_solve_info* info) {
+    if (!s || !rows || !b || !x || !p) return JG_ERR_INVALID_ARG;
+    JG_TRY(jg_solver_set_levels(s, nlev, nx, ny, nz));
+    for (size_t l = 0; l < nlev; ++l) JG_TRY(jg_solver_upload(s, l, rows[l], l == 0 ? b : nullptr));
+    JG_TRY(jg_solver_mgpcg(s, max_iter, p, o, info));
+    return jg_solver_download_x(s, x);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------ kernel timing
+
+namespace {
+
+template <typename T>
+struct Vecs {
+    T *x, *b, *r, *z, *dA, *dB, *q;
+};
+
+Vecs<double> vecs64(jg_solver* s) { return {s->x, s->b, s->r, s->z, s->dA, s->dB, s->q}; }
+Vecs<float> vecs32(jg_solver* s) { return {s->e32, s->rhs32, s->r32, s->z32, s->d32A, s->d32B, s->q32}; }
+
+template <typename T>
+int time_kernel_t(jg_solver* s, const Vecs<T>& v, int kernel, unsigned reps, double* avg_ms,
+                  double* bytes) {
+    Level& l0 = s->lv[0];
+    const Dims d = l0.d;
+    const idx_t n = d.n();
+    const int nb1 = nblocks_1d(n);
+    Geo g = geo_stencil(d);
+    const Row<T>* A = Prec<T>::A(l0);
+    Ctl h;
+    init_ctl(&h, 1u << 30, 0.0, 0, 0.0);
+    h.alpha = 1e-3;
+    h.beta = 0.5;
+    h.sigma = 1.0;
+    JG_TRY(upload_ctl(s, s->ctl, h));
+    for (T* p : {v.x, v.b, v.r, v.z, v.dA, v.dB, v.q})
+        k_fill<T><<<nb1, NT, 0, s->stream>>>(p, T(1), n);
+    const double sa = sizeof(Row<T>), sv = sizeof(T);
+    auto launch = [&]() {
+        switch (kernel) {
+            case JG_KERNEL_RBGS:
+                k_rbgs<T, T, false><<<g.grid, g.block, 0, s->stream>>>(A, v.b, v.x, d, g.zc, 1, T(1),
+                                                                       s->ctl);
+                break;
+            case JG_KERNEL_DIR_SPMV:
+                k_dir_spmv<T, T><<<g.grid, g.block, 0, s->stream>>>(A, v.z, v.dA, v.dB, v.q, d, g.zc,
+                                                                    s->slot(2), s->ctl);
+                break;
+            case JG_KERNEL_RESIDUAL:
+                k_residual<T, T, T, false, false><<<g.grid, g.block, 0, s->stream>>>(
+                    A, v.x, v.b, v.r, d, g.zc, nullptr, nullptr);
+                break;
+            case JG_KERNEL_UPDATE_XR:
+                k_update_xr<T><<<nb1, NT, 0, s->stream>>>(v.x, v.r, v.dA, v.q, n, s->ctl);
+                break;
+            case JG_KERNEL_DOT:
+                k_dot<T, T><<<nb1, NT, 0, s->stream>>>(v.r, v.z, n, s->slot(0), s->ctl);
+                break;
+            default:
+                k_convert<T, T><<<nb1, NT, 0, s->stream>>>(v.r, v.q, n);
+                break;
+        }
+    };
+    // Compulsory bytes per launch under Jet's 4-coefficient data model.
+    double per_cell;
+    switch (kernel) {
+        case JG_KERNEL_RBGS: per_cell = sa + 2 * sv; break;      // A (own + neighbour couplings), x, b/2, x/2 written
+        case JG_KERNEL_DIR_SPMV: per_cell = sa + 4 * sv; break;  // A, z, d_old -> d_new, q
+        case JG_KERNEL_RESIDUAL: per_cell = sa + 3 * sv; break;  // A, x, b -> r
+        case JG_KERNEL_UPDATE_XR: per_cell = 6 * sv; break;      // x, d, r, q -> x, r
+        case JG_KERNEL_DOT: per_cell = 2 * sv; break;
+        default: per_cell = 2 * sv; break;
+    }
+    launch();  // warm
+    JG_TRY(check_launch(s, "time_kernel"));
+    JG_HIP(hipEventRecord(s->ev0, s->stream));
+    for (unsigned i = 0; i < reps; ++i) launch();
+    JG_HIP(hipEventRecord(s->ev1, s->stream));
+    JG_TRY(sync(s));
+    float ms = 0.f;
+    JG_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    *avg_ms = (double)ms / reps;
+    *bytes = per_cell * (double)n;
+    return JG_OK;
+}
+
+}  // namespace
+
+extern "C" int jg_solver_time_kernel(jg_solver* s, int kernel, int precision, unsigned reps,
+                                     double* avg_ms, double* bytes) {
+    if (!s || s->lv.empty() || !avg_ms || !bytes || reps == 0 || kernel < 0 || kernel > 5)
+        return JG_ERR_INVALID_ARG;
+    JG_HIP(hipSetDevice(s->device));
+    if (precision == JG_MIXED) {
+        JG_TRY(ensure_f32(s));
+        return time_kernel_t<float>(s, vecs32(s), kernel, reps, avg_ms, bytes);
+    }
+    return time_kernel_t<double>(s, vecs64(s), kernel, reps, avg_ms, bytes);
+}
+
+extern "C" {
+
+// --------------------------------------------------------- BLAS entries
+
+int jg_blas_mvm(jg_solver* s, size_t nx, size_t ny, size_t nz, const double* rows,
+                const double* x, double* y) {
+    if (!s || !valid_dims(nx, ny, nz) || !rows || !x || !y) return JG_ERR_INVALID_ARG;
+    JG_HIP(hipSetDevice(s->device));
+    Dims d{(int)nx, (int)ny, (int)nz};
+    Geo g = geo_stencil(d);
+    k_spmv<double, double, false><<<g.grid, g.block, 0, s->stream>>>(
+        (const Row<double>*)rows, x, y, d, g.zc, nullptr, nullptr);
+    JG_TRY(check_launch(s, "mvm"));
+    return sync(s);
+}
+
+int jg_blas_residual(jg_solver* s, size_t nx, size_t ny, size_t nz, const double* rows,
+                     const double* x, const double* b, double* r) {
+    if (!s || !valid_dims(nx, ny, nz) || !rows || !x || !b || !r) return JG_ERR_INVALID_ARG;
+    JG_HIP(hipSetDevice(s->device));
+    Dims d{(int)nx, (int)ny, (int)nz};
+    Geo g = geo_stencil(d);
+    k_residual<double, double, double, false, false><<<g.grid, g.block, 0, s->stream>>>(
+        (const Row<double>*)rows, x, b, r, d, g.zc, nullptr, nullptr);
+    JG_TRY(check_launch(s, "residual"));
+    return sync(s);
+}
+
+int jg_blas_dot(jg_solver* s, size_t n, const double* a, const double* b, double* result) {
+    if (!s || !a || !b || !result) return JG_ERR_INVALID_ARG;
+    JG_HIP(hipSetDevice(s->device));
+    if (!s->part) {
+        JG_TRY(dalloc(s, &s->part, (size_t)jg_solver::kSlots * kMaxPartials));
+        JG_TRY(dalloc(s, &s->scal, 16));
+    }
+    const int nb = nblocks_1d((idx_t)n);
+    k_dot<double, double><<<nb, NT, 0, s->stream>>>(a, b, (idx_t)n, s->slot(0), nullptr);
+    k_fin_sum<<<1, NT, 0, s->stream>>>(s->slot(0), nb, s->scal + 4);
+    JG_TRY(check_launch(s, "dot"));
+    JG_HIP(hipMemcpyAsync(result, s->scal + 4, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    return sync(s);
+}
+
+int jg_blas_axpy(jg_solver* s, size_t n, double a, const double* x, const double* y, double* out) {
+    if (!s || !x || !y || !out) return JG_ERR_INVALID_ARG;
+    JG_HIP(hipSetDevice(s->device));
+    k_axpy<double><<<nblocks_1d((idx_t)n), NT, 0, s->stream>>>(a, x, y, out, (idx_t)n);
+    JG_TRY(check_launch(s, "axpy"));
+    return sync(s);
+}
+
+int jg_relax_red_black(jg_solver* s, size_t nx, size_t ny, size_t nz, const double* rows,
+                       const double* b, double sor, double* x) {
+    if (!s || !valid_dims(nx, ny, nz) || !rows || !b || !x) return JG_ERR_INVALID_ARG;
+    JG_HIP(hipSetDevice(s->device));
+    Dims d{(int)nx, (int)ny, (int)nz};
+    launch_rbgs<double, double>(s, (const Row<double>*)rows, b, x, d, 0, sor, false, nullptr);
+    launch_rbgs<double, double>(s, (const Row<double>*)rows, b, x, d, 1, sor, false, nullptr);
+    JG_TRY(check_launch(s, "relax_red_black"));
+    return sync(s);
+}
+
+int jg_relax_jacobi(jg_solver* s, size_t nx, size_t ny, size_t nz, const double* rows,
+                    const double* b, const double* x, double* xtemp) {
+    if (!s || !valid_dims(nx, ny, nz) || !rows || !b || !x || !xtemp || x == xtemp)
+        return JG_ERR_INVALID_ARG;
+    JG_HIP(hipSetDevice(s->device));
+    Dims d{(int)nx, (int)ny, (int)nz};
+    Geo g = geo_stencil(d);
+    k_jacobi<double, double><<<g.grid, g.block, 0, s->stream>>>((const Row<double>*)rows, b, x,
+                                                               xtemp, d, g.zc, 1.0, nullptr);
+    JG_TRY(check_launch(s, "relax_jacobi"));
+    return sync(s);
+}
+
+int jg_mg_restrict(jg_solver* s, size_t cnx, size_t cny, size_t cnz, const double* fine,
+                   double* coarse) {
+    if (!s || !valid_dims(cnx, cny, cnz) || !fine || !coarse) return JG_ERR_INVALID_ARG;
+    JG_HIP(hipSetDevice(s->device));
+    Dims cd{(int)cnx, (int)cny, (int)cnz};
+    Geo g = geo_planes(cd);
+    k_restrict<double><<<g.grid, g.block, 0, s->stream>>>(fine, coarse, cd, nullptr);
+    JG_TRY(check_launch(s, "restrict"));
+    return sync(s);
+}
+
+int jg_mg_correct(jg_solver* s, size_t fnx, size_t fny, size_t fnz, const double* coarse,
+                  double* fine) {
+    if (!s || !valid_dims(fnx, fny, fnz) || !coarse || !fine || (fnx | fny | fnz) & 1)
+        return JG_ERR_INVALID_ARG;
+    JG_HIP(hipSetDevice(s->device));
+    Dims fd{(int)fnx, (int)fny, (int)fnz};
+    Geo g = geo_planes(fd);
+    k_correct<double><<<g.grid, g.block, 0, s->stream>>>(coarse, fine, fd, nullptr);
+    JG_TRY(check_launch(s, "correct"));
+    return sync(s);
+}
+
+}  // extern "C"

@@ -1,0 +1,294 @@
+"""ctypes binding of libjetgpu (include/jetgpu.h) for tests and benchmarks.
+
+This module is plumbing: it declares the C ABI and offers small helpers that
+pass torch device tensors (or numpy host arrays) to it.  All computation is
+in lib/libjetgpu.so; there is no fallback -- importing this module raises if
+the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libjetgpu.so")
+
+JG_OK = 0
+ERRORS = {1: "invalid argument", 2: "out of memory", 3: "HIP error", 4: "non-finite value",
+          5: "no device", 6: "unsupported"}
+JG_FP64, JG_MIXED = 0, 1
+JG_STOP_JET, JG_STOP_RELATIVE = 0, 1
+JG_SMOOTH_RBGS, JG_SMOOTH_JACOBI = 0, 1
+(JG_KERNEL_RBGS, JG_KERNEL_DIR_SPMV, JG_KERNEL_RESIDUAL, JG_KERNEL_UPDATE_XR, JG_KERNEL_DOT,
+ JG_KERNEL_COPY) = range(6)
+(JG_SCENE_CLOSED_SPHERE, JG_SCENE_DAM_BREAK, JG_SCENE_SPHERE_POOL, JG_SCENE_BATCH_MEMBER,
+ JG_SCENE_TORUS_POOL) = 1, 2, 3, 4, 5
+
+
+class MgParams(C.Structure):
+    _fields_ = [("max_levels", C.c_size_t), ("restrict_iter", C.c_uint),
+                ("correct_iter", C.c_uint), ("coarsest_iter", C.c_uint),
+                ("final_iter", C.c_uint), ("max_tol", C.c_double), ("sor", C.c_double),
+                ("red_black", C.c_int)]
+
+
+class GpuOptions(C.Structure):
+    _fields_ = [("precision", C.c_int), ("stop_rule", C.c_int), ("rel_tol", C.c_double),
+                ("smoother", C.c_int), ("omega", C.c_double), ("max_refine", C.c_uint),
+                ("inner_rel_tol", C.c_double), ("check_every", C.c_uint)]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("iterations", C.c_uint), ("refinements", C.c_uint),
+                ("last_residual", C.c_double), ("rel_residual", C.c_double),
+                ("converged", C.c_int), ("solve_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("kind", C.c_int), ("n", C.c_size_t), ("h", C.c_double),
+                ("seed", C.c_ulonglong), ("fluid_fraction", C.c_double)]
+
+
+_vp = C.c_void_p
+_sz = C.c_size_t
+_d = C.c_double
+_u = C.c_uint
+
+# name -> (restype, argtypes): every symbol declared in include/jetgpu.h.
+SIGNATURES = {
+    "jg_mg_params_default": (None, [C.POINTER(MgParams)]),
+    "jg_gpu_options_default": (None, [C.POINTER(GpuOptions)]),
+    "jg_solver_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "jg_solver_destroy": (None, [_vp]),
+    "jg_last_error": (C.c_char_p, [_vp]),
+    "jg_solver_stream": (_vp, [_vp]),
+    "jg_solver_set_levels": (C.c_int, [_vp, _sz, C.POINTER(_sz), C.POINTER(_sz), C.POINTER(_sz)]),
+    "jg_solver_matrix": (_vp, [_vp, _sz]),
+    "jg_solver_rhs": (_vp, [_vp]),
+    "jg_solver_solution": (_vp, [_vp]),
+    "jg_solver_upload": (C.c_int, [_vp, _sz, _vp, _vp]),
+    "jg_solver_download_x": (C.c_int, [_vp, _vp]),
+    "jg_solver_matrices_changed": (C.c_int, [_vp]),
+    "jg_solver_cg": (C.c_int, [_vp, _u, _d, C.POINTER(SolveInfo)]),
+    "jg_solver_mgpcg": (C.c_int, [_vp, _u, C.POINTER(MgParams), C.POINTER(GpuOptions),
+                                  C.POINTER(SolveInfo)]),
+    "jg_cg_solve_host": (C.c_int, [_vp, _sz, _sz, _sz, _vp, _vp, _vp, _u, _d, C.POINTER(SolveInfo)]),
+    "jg_mgpcg_solve_host": (C.c_int, [_vp, _sz, C.POINTER(_sz), C.POINTER(_sz), C.POINTER(_sz),
+                                      C.POINTER(_vp), _vp, _vp, _u, C.POINTER(MgParams),
+                                      C.POINTER(GpuOptions), C.POINTER(SolveInfo)]),
+    "jg_blas_mvm": (C.c_int, [_vp, _sz, _sz, _sz, _vp, _vp, _vp]),
+    "jg_blas_residual": (C.c_int, [_vp, _sz, _sz, _sz, _vp, _vp, _vp, _vp]),
+    "jg_blas_dot": (C.c_int, [_vp, _sz, _vp, _vp, C.POINTER(_d)]),
+    "jg_blas_axpy": (C.c_int, [_vp, _sz, _d, _vp, _vp, _vp]),
+    "jg_relax_red_black": (C.c_int, [_vp, _sz, _sz, _sz, _vp, _vp, _d, _vp]),
+    "jg_relax_jacobi": (C.c_int, [_vp, _sz, _sz, _sz, _vp, _vp, _vp, _vp]),
+    "jg_mg_restrict": (C.c_int, [_vp, _sz, _sz, _sz, _vp, _vp]),
+    "jg_mg_correct": (C.c_int, [_vp, _sz, _sz, _sz, _vp, _vp]),
+    "jg_solver_time_kernel": (C.c_int, [_vp, C.c_int, C.c_int, _u, C.POINTER(_d), C.POINTER(_d)]),
+    "jg_sp_build": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d]),
+    "jg_sp_apply_gradient": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d]),
+    "jg_solver_markers": (_vp, [_vp, _sz]),
+    "jg_scene_generate_host": (C.c_int, [C.POINTER(SceneDesc), _vp, _vp, _vp, _vp, _vp, _vp]),
+    "jg_scene_generate_device": (C.c_int, [_vp, C.POINTER(SceneDesc), _vp, _vp, _vp, _vp, _vp, _vp]),
+}
+
+
+def load(path: str = LIB_PATH):
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built: run __graft_entry__.build() or `make -C fluid-engine-dev_amd`")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        _LIB = load()
+    return _LIB
+
+
+def mg_params(**kw):
+    p = MgParams()
+    lib().jg_mg_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def gpu_options(**kw):
+    o = GpuOptions()
+    lib().jg_gpu_options_default(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def scene_desc(kind, n, seed=0, h=1.0, fluid_fraction=0.6):
+    return SceneDesc(kind, n, h, seed, fluid_fraction)
+
+
+def generate_scene_host(desc: SceneDesc, vertex=False):
+    """Host generator -> dict of numpy arrays (Jet layout)."""
+    n = desc.n
+    out = {
+        "u": np.empty((n, n, n + 1)), "v": np.empty((n, n + 1, n)), "w": np.empty((n + 1, n, n)),
+        "bsdf": np.empty((n, n, n)), "fsdf": np.empty((n, n, n)),
+    }
+    vs = np.empty((n + 1, n + 1, n + 1)) if vertex else None
+    ptr = lambda a: a.ctypes.data_as(_vp) if a is not None else None
+    st = lib().jg_scene_generate_host(C.byref(desc), ptr(out["u"]), ptr(out["v"]), ptr(out["w"]),
+                                      ptr(out["bsdf"]), ptr(out["fsdf"]), ptr(vs))
+    if st != JG_OK:
+        raise RuntimeError(f"jg_scene_generate_host: {ERRORS.get(st, st)}")
+    if vertex:
+        out["vertex"] = vs
+    return out
+
+
+class JgError(RuntimeError):
+    pass
+
+
+class Solver:
+    """Owns one jg_solver handle (one HIP stream on one device)."""
+
+    def __init__(self, device=0):
+        h = _vp()
+        st = lib().jg_solver_create(device, C.byref(h))
+        if st != JG_OK:
+            raise JgError(f"jg_solver_create: {ERRORS.get(st, st)}")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().jg_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, st, what):
+        if st != JG_OK:
+            msg = lib().jg_last_error(self.h)
+            raise JgError(f"{what}: {ERRORS.get(st, st)}: {msg.decode() if msg else ''}")
+
+    # -------------------------------------------------- resident system
+    def set_levels(self, sizes):
+        n = len(sizes)
+        SZ = _sz * n
+        self.check(lib().jg_solver_set_levels(self.h, n, SZ(*[s[0] for s in sizes]),
+                                              SZ(*[s[1] for s in sizes]), SZ(*[s[2] for s in sizes])),
+                   "set_levels")
+        self.sizes = list(sizes)
+
+    def matrix_ptr(self, level):
+        return lib().jg_solver_matrix(self.h, level)
+
+    def rhs_ptr(self):
+        return lib().jg_solver_rhs(self.h)
+
+    def solution_ptr(self):
+        return lib().jg_solver_solution(self.h)
+
+    def upload(self, level, A, b=None):
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        bp = np.ascontiguousarray(b, dtype=np.float64).ctypes.data_as(_vp) if b is not None else None
+        self.check(lib().jg_solver_upload(self.h, level, A.ctypes.data_as(_vp), bp), "upload")
+
+    def download_x(self):
+        nx, ny, nz = self.sizes[0]
+        x = np.empty((nz, ny, nx))
+        self.check(lib().jg_solver_download_x(self.h, x.ctypes.data_as(_vp)), "download_x")
+        return x
+
+    def cg(self, max_iter, tol):
+        info = SolveInfo()
+        self.check(lib().jg_solver_cg(self.h, max_iter, tol, C.byref(info)), "cg")
+        return info
+
+    def mgpcg(self, max_iter, params: MgParams, options: GpuOptions | None = None):
+        info = SolveInfo()
+        self.check(lib().jg_solver_mgpcg(self.h, max_iter, C.byref(params),
+                                         C.byref(options) if options is not None else None,
+                                         C.byref(info)), "mgpcg")
+        return info
+
+    def time_kernel(self, kernel, precision, reps=20):
+        """(average launch ms, algorithmic bytes per launch) of a level-0 kernel."""
+        ms, nbytes = _d(), _d()
+        self.check(lib().jg_solver_time_kernel(self.h, kernel, precision, reps, C.byref(ms),
+                                               C.byref(nbytes)), "time_kernel")
+        return ms.value, nbytes.value
+
+    # ------------------------------------------------ pressure assembly
+    def sp_build(self, u, v, w, bsdf, fsdf, h):
+        self.check(lib().jg_sp_build(self.h, _ptr(u), _ptr(v), _ptr(w), _ptr(bsdf), _ptr(fsdf),
+                                     h, h, h), "sp_build")
+
+    def sp_apply_gradient(self, u, v, w, uo, vo, wo, h):
+        self.check(lib().jg_sp_apply_gradient(self.h, _ptr(u), _ptr(v), _ptr(w), _ptr(uo),
+                                              _ptr(vo), _ptr(wo), h, h, h), "sp_apply_gradient")
+
+    def generate_scene(self, desc, u=None, v=None, w=None, bsdf=None, fsdf=None, vertex=None):
+        self.check(lib().jg_scene_generate_device(self.h, C.byref(desc), _ptr(u), _ptr(v), _ptr(w),
+                                                  _ptr(bsdf), _ptr(fsdf), _ptr(vertex)), "scene")
+
+    # ------------------------------------------------- device BLAS entries
+    def mvm(self, shape, A, x, y):
+        nx, ny, nz = shape
+        self.check(lib().jg_blas_mvm(self.h, nx, ny, nz, _ptr(A), _ptr(x), _ptr(y)), "mvm")
+
+    def residual(self, shape, A, x, b, r):
+        nx, ny, nz = shape
+        self.check(lib().jg_blas_residual(self.h, nx, ny, nz, _ptr(A), _ptr(x), _ptr(b), _ptr(r)),
+                   "residual")
+
+    def dot(self, n, a, b):
+        out = _d()
+        self.check(lib().jg_blas_dot(self.h, n, _ptr(a), _ptr(b), C.byref(out)), "dot")
+        return out.value
+
+    def axpy(self, n, a, x, y, out):
+        self.check(lib().jg_blas_axpy(self.h, n, a, _ptr(x), _ptr(y), _ptr(out)), "axpy")
+
+    def relax_red_black(self, shape, A, b, sor, x):
+        nx, ny, nz = shape
+        self.check(lib().jg_relax_red_black(self.h, nx, ny, nz, _ptr(A), _ptr(b), sor, _ptr(x)),
+                   "relax_red_black")
+
+    def relax_jacobi(self, shape, A, b, x, xt):
+        nx, ny, nz = shape
+        self.check(lib().jg_relax_jacobi(self.h, nx, ny, nz, _ptr(A), _ptr(b), _ptr(x), _ptr(xt)),
+                   "relax_jacobi")
+
+    def restrict(self, cshape, fine, coarse):
+        nx, ny, nz = cshape
+        self.check(lib().jg_mg_restrict(self.h, nx, ny, nz, _ptr(fine), _ptr(coarse)), "restrict")
+
+    def correct(self, fshape, coarse, fine):
+        nx, ny, nz = fshape
+        self.check(lib().jg_mg_correct(self.h, nx, ny, nz, _ptr(coarse), _ptr(fine)), "correct")
+
+
+def _ptr(t):
+    """Device pointer of a torch tensor, an int address, or None."""
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return t
+    return t.data_ptr()

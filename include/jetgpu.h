@@ -1,0 +1,262 @@
+/*
+ * jetgpu.h -- C ABI of the MI355X (gfx950) pressure-projection solver.
+ *
+ * This is the boundary a host build of Jet (doyubkim/fluid-engine-dev) binds
+ * to replace its CPU linear solvers for grid pressure projection.  Plain C
+ * types only: sizes, host pointers, device pointers, opaque handles.  Every
+ * function returns a status code (JG_OK == 0); jg_last_error() describes the
+ * last failure of a solver.
+ *
+ * Data model (identical to Jet's, so no conversion is needed at the
+ * boundary):
+ *   - cell arrays are x-fastest, index = i + nx*(j + ny*k)
+ *     (Jet Array3, include/jet/detail/array3-inl.h:142-151);
+ *   - a matrix is an array of FdmMatrixRow3 {center, right, up, front}
+ *     doubles, 32 bytes per cell (include/jet/fdm_linear_system3.h:18-30):
+ *     the 7-point symmetric stencil stored as its upper couplings;
+ *   - a multigrid system is a list of levels, level 0 finest, each level
+ *     exactly half the previous one (FdmMgLinearSystem3,
+ *     include/jet/fdm_mg_linear_system3.h:23-55).
+ *
+ * Threading: one jg_solver is used by one host thread at a time (like Jet's
+ * solver objects, which keep scratch vectors as members).
+ */
+#ifndef JETGPU_H_
+#define JETGPU_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status */
+enum {
+    JG_OK = 0,
+    JG_ERR_INVALID_ARG = 1,  /* Jet: JET_THROW_INVALID_ARG_IF (size mismatch) */
+    JG_ERR_OUT_OF_MEMORY = 2,
+    JG_ERR_HIP = 3,          /* HIP runtime error (message in jg_last_error) */
+    JG_ERR_NONFINITE = 4,    /* NaN/Inf met in a reduction */
+    JG_ERR_NO_DEVICE = 5,
+    JG_ERR_UNSUPPORTED = 6   /* e.g. Jet's no-op entry points */
+};
+
+enum { JG_FP64 = 0, JG_MIXED = 1 };
+
+/* Stop rules.  JG_STOP_JET is Jet's pcg rule: stop when r.z <= tol^2
+ * (include/jet/detail/cg-inl.h:48).  JG_STOP_RELATIVE stops when
+ * ||b - A x||_2 <= rel_tol * ||b||_2 (FP64 residual). */
+enum { JG_STOP_JET = 0, JG_STOP_RELATIVE = 1 };
+
+enum { JG_SMOOTH_RBGS = 0, JG_SMOOTH_JACOBI = 1 };
+
+typedef struct jg_solver jg_solver;
+
+/* MgParameters + FdmMgSolver3 knobs (include/jet/mg.h:58-86,
+ * include/jet/fdm_mg_solver3.h:24-39).  Defaults: see jg_mg_params_default. */
+typedef struct {
+    size_t max_levels;
+    unsigned restrict_iter;  /* pre-smoothing sweeps per level           */
+    unsigned correct_iter;   /* post-smoothing sweeps on levels > 0      */
+    unsigned coarsest_iter;  /* sweeps on the coarsest level             */
+    unsigned final_iter;     /* post-smoothing sweeps on level 0         */
+    double max_tol;          /* MGPCG tolerance (Jet: sqrt(r.z) <= max_tol) */
+    double sor;              /* Jet's sorFactor                          */
+    int red_black;           /* Jet's useRedBlackOrdering                */
+} jg_mg_params;
+
+/* GPU-only options (no Jet counterpart).  Zero-initialised = defaults. */
+typedef struct {
+    int precision;           /* JG_FP64 (default) or JG_MIXED            */
+    int stop_rule;           /* JG_STOP_JET (default) or JG_STOP_RELATIVE */
+    double rel_tol;          /* for JG_STOP_RELATIVE (default 1e-8)      */
+    int smoother;            /* JG_SMOOTH_RBGS (default) or JG_SMOOTH_JACOBI */
+    double omega;            /* smoother relaxation; 0 -> 1.0 (RBGS) / 6/7 (Jacobi) */
+    unsigned max_refine;     /* mixed: outer FP64 refinement steps (default 8) */
+    double inner_rel_tol;    /* mixed: FP32 inner reduction per step (default 1e-4) */
+    unsigned check_every;    /* host convergence poll interval in iterations (default 1) */
+} jg_gpu_options;
+
+typedef struct {
+    unsigned iterations;     /* PCG iterations (all inner iterations in mixed mode) */
+    unsigned refinements;    /* mixed: outer refinement steps taken */
+    double last_residual;    /* Jet's lastResidual(): sqrt(|r.z|) at exit */
+    double rel_residual;     /* ||b - A x||_2 / ||b||_2 in FP64 at exit */
+    int converged;           /* Jet's bool return value */
+    double solve_ms;         /* device time of the solve (HIP events) */
+} jg_solve_info;
+
+/* ----------------------------------------------------------- lifetime */
+void jg_mg_params_default(jg_mg_params* p);
+void jg_gpu_options_default(jg_gpu_options* o);
+
+/* Creates a solver bound to HIP device `device` with its own stream. */
+int jg_solver_create(int device, jg_solver** out);
+void jg_solver_destroy(jg_solver* s);
+const char* jg_last_error(const jg_solver* s);
+/* Device stream used by the solver (a hipStream_t). */
+void* jg_solver_stream(jg_solver* s);
+
+/* --------------------------------------------- device-resident system */
+/* Declares the level sizes and allocates device storage (cached: a call
+ * with the same sizes allocates nothing).  Level l+1 must be exactly half of
+ * level l in every direction. */
+int jg_solver_set_levels(jg_solver* s, size_t nlev, const size_t* nx,
+                         const size_t* ny, const size_t* nz);
+/* Device pointers of the resident system: matrix rows of level l (32-byte
+ * FdmMatrixRow3), the finest right-hand side b and solution x. */
+double* jg_solver_matrix(jg_solver* s, size_t level);
+double* jg_solver_rhs(jg_solver* s);
+double* jg_solver_solution(jg_solver* s);
+/* Host <-> device copies of the resident system (b may be NULL). */
+int jg_solver_upload(jg_solver* s, size_t level, const double* rows,
+                     const double* b);
+int jg_solver_download_x(jg_solver* s, double* x);
+/* Marks the matrices as changed after they were written in place on the
+ * device (refreshes the FP32 shadow copies used by JG_MIXED). */
+int jg_solver_matrices_changed(jg_solver* s);
+
+/* FdmCgSolver3::solve(FdmLinearSystem3*) on the resident level 0
+ * (src/jet/fdm_cg_solver3.cpp:20-47): x <- 0, Jet's cg<FdmBlas3> control
+ * flow, Jet's bool in info->converged. */
+int jg_solver_cg(jg_solver* s, unsigned max_iter, double tol,
+                 jg_solve_info* info);
+
+/* FdmMgpcgSolver3::solve(FdmMgLinearSystem3*) on the resident hierarchy
+ * (src/jet/fdm_mgpcg_solver3.cpp:54-80): x <- 0, PCG on level 0
+ * preconditioned by one V-cycle over all levels.  The GPU V-cycle is
+ * symmetric (zero initial guess, red-black pre-sweeps, black-red
+ * post-sweeps), unlike Jet's warm-started lexicographic SOR. */
+int jg_solver_mgpcg(jg_solver* s, unsigned max_iter, const jg_mg_params* p,
+                    const jg_gpu_options* o, jg_solve_info* info);
+
+/* ------------------------------------------------ host-memory drop-ins */
+/* One-call forms of the two solves on host arrays: upload, solve, download
+ * x.  rows[l] / b are host arrays in Jet's layout. */
+int jg_cg_solve_host(jg_solver* s, size_t nx, size_t ny, size_t nz,
+                     const double* rows, const double* b, double* x,
+                     unsigned max_iter, double tol, jg_solve_info* info);
+int jg_mgpcg_solve_host(jg_solver* s, size_t nlev, const size_t* nx,
+                        const size_t* ny, const size_t* nz,
+                        const double* const* rows, const double* b, double* x,
+                        unsigned max_iter, const jg_mg_params* p,
+                        const jg_gpu_options* o, jg_solve_info* info);
+
+/* ------------------------------------ FdmBlas3 / FdmMgUtils3 on device */
+/* All pointers are device pointers; these run on the solver's stream and
+ * return after the work is complete. */
+/* FdmBlas3::mvm -- src/jet/fdm_linear_system3.cpp:79-96 */
+int jg_blas_mvm(jg_solver* s, size_t nx, size_t ny, size_t nz,
+                const double* rows, const double* x, double* y);
+/* FdmBlas3::residual -- src/jet/fdm_linear_system3.cpp:98-116 */
+int jg_blas_residual(jg_solver* s, size_t nx, size_t ny, size_t nz,
+                     const double* rows, const double* x, const double* b,
+                     double* r);
+/* FdmBlas3::dot -- src/jet/fdm_linear_system3.cpp:49-65 (FP64) */
+int jg_blas_dot(jg_solver* s, size_t n, const double* a, const double* b,
+                double* result);
+/* FdmBlas3::axpy -- src/jet/fdm_linear_system3.cpp:67-77: out = a*x + y */
+int jg_blas_axpy(jg_solver* s, size_t n, double a, const double* x,
+                 const double* y, double* out);
+/* FdmGaussSeidelSolver3::relaxRedBlack -- src/jet/fdm_gauss_seidel_solver3.cpp:150-221 */
+int jg_relax_red_black(jg_solver* s, size_t nx, size_t ny, size_t nz,
+                       const double* rows, const double* b, double sor,
+                       double* x);
+/* FdmJacobiSolver3::relax -- src/jet/fdm_jacobi_solver3.cpp:95-112 */
+int jg_relax_jacobi(jg_solver* s, size_t nx, size_t ny, size_t nz,
+                    const double* rows, const double* b, const double* x,
+                    double* xtemp);
+/* FdmMgUtils3::restrict -- src/jet/fdm_mg_linear_system3.cpp:43-98
+ * (coarse size cnx*cny*cnz, fine size twice that) */
+int jg_mg_restrict(jg_solver* s, size_t cnx, size_t cny, size_t cnz,
+                   const double* fine, double* coarse);
+/* FdmMgUtils3::correct -- src/jet/fdm_mg_linear_system3.cpp:100-179
+ * (fine += prolongation of coarse; fine size fnx*fny*fnz) */
+int jg_mg_correct(jg_solver* s, size_t fnx, size_t fny, size_t fnz,
+                  const double* coarse, double* fine);
+
+/* ------------------------------------------------- kernel measurement */
+/* Times `reps` back-to-back launches of one level-0 kernel of the solve on
+ * the resident system with HIP events on the solver's stream (the solution
+ * and work vectors are overwritten).  Returns the average launch duration
+ * and the kernel's algorithmic HBM bytes per launch (compulsory reads and
+ * writes of the cells it touches, 4 coefficients per matrix row). */
+enum {
+    JG_KERNEL_RBGS = 0,       /* one colour of a red-black sweep              */
+    JG_KERNEL_DIR_SPMV = 1,   /* d = z + beta d, q = A d, partial d.q          */
+    JG_KERNEL_RESIDUAL = 2,   /* r = b - A x                                   */
+    JG_KERNEL_UPDATE_XR = 3,  /* x += alpha d, r -= alpha q                    */
+    JG_KERNEL_DOT = 4,        /* partial r.z                                   */
+    JG_KERNEL_COPY = 5        /* plain copy of one vector (bandwidth reference) */
+};
+int jg_solver_time_kernel(jg_solver* s, int kernel, int precision,
+                          unsigned reps, double* avg_ms, double* bytes);
+
+/* ------------------------------------- pressure-system assembly on device */
+/* GridSinglePhasePressureSolver3::buildMarkers + buildSystem
+ * (src/jet/grid_single_phase_pressure_solver3.cpp:248-399) on the solver's
+ * resident hierarchy (jg_solver_set_levels first).  Device inputs: face
+ * velocities u ((nx+1)*ny*nz), v (nx*(ny+1)*nz), w (nx*ny*(nz+1)) and the
+ * boundary / fluid SDFs sampled at the cell centres of the finest level.
+ * Markers on every level (coarse: 4^3 majority vote), A on every level
+ * (re-discretised at spacing 2^l h), b = div u on level 0.  The coarse
+ * right-hand sides Jet also builds are not built: its V-cycle overwrites them
+ * by restriction before any use (include/jet/detail/mg-inl.h:29-31). */
+int jg_sp_build(jg_solver* s, const double* u, const double* v, const double* w,
+                const double* boundary_sdf, const double* fluid_sdf, double hx,
+                double hy, double hz);
+/* applyPressureGradient (src/jet/grid_single_phase_pressure_solver3.cpp:401-431)
+ * with the resident solution x and level-0 markers; device arrays; the
+ * output may alias the input. */
+int jg_sp_apply_gradient(jg_solver* s, const double* u, const double* v,
+                         const double* w, double* uo, double* vo, double* wo,
+                         double hx, double hy, double hz);
+/* Device pointer of the markers of a level (int8: 0 fluid, 1 air, 2 boundary). */
+const signed char* jg_solver_markers(jg_solver* s, size_t level);
+
+/* --------------------------------------------------- synthetic scenes */
+/* Deterministic synthetic inputs (counter-based SplitMix64, jg_synth.h):
+ *   JG_SCENE_CLOSED_SPHERE : solid sphere r = n/4 at the centre, all fluid,
+ *                            solid walls (all-Neumann, singular);
+ *   JG_SCENE_DAM_BREAK     : fluid in x < n/2, y < 0.4 n, air elsewhere,
+ *                            v += -9.8/60;
+ *   JG_SCENE_SPHERE_POOL   : 8 solid spheres, radii U(8,32)*n/256, fluid
+ *                            below a plane placed so the fluid fraction is
+ *                            `fluid_fraction`;
+ *   JG_SCENE_BATCH_MEMBER  : 1-4 spheres, radii U(n/16, n/6), fluid below
+ *                            y = U(0.3, 0.8) n;
+ *   JG_SCENE_TORUS_POOL    : torus (R = 150/512 n, r = 50/512 n) around the
+ *                            y axis through the centre, fluid below 0.35 n.
+ * Velocities are U(-1,1) with domain-wall faces and faces touching a solid
+ * cell set to zero (the blocked boundary condition). */
+enum {
+    JG_SCENE_CLOSED_SPHERE = 1,
+    JG_SCENE_DAM_BREAK = 2,
+    JG_SCENE_SPHERE_POOL = 3,
+    JG_SCENE_BATCH_MEMBER = 4,
+    JG_SCENE_TORUS_POOL = 5
+};
+
+typedef struct {
+    int kind;
+    size_t n;                 /* cubic grid edge (cells)           */
+    double h;                 /* grid spacing (0 -> 1)             */
+    unsigned long long seed;
+    double fluid_fraction;    /* JG_SCENE_SPHERE_POOL (0 -> 0.6)   */
+} jg_scene_desc;
+
+/* Host generator (no GPU needed).  Any output pointer may be NULL.
+ * bsdf_vertex is the boundary SDF at the (n+1)^3 grid vertices. */
+int jg_scene_generate_host(const jg_scene_desc* d, double* u, double* v,
+                           double* w, double* bsdf_cc, double* fsdf_cc,
+                           double* bsdf_vertex);
+/* Same inputs generated on the device (device pointers; any may be NULL). */
+int jg_scene_generate_device(jg_solver* s, const jg_scene_desc* d, double* u,
+                             double* v, double* w, double* bsdf_cc,
+                             double* fsdf_cc, double* bsdf_vertex);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JETGPU_H_ */

@@ -291,7 +291,7 @@ void or_mg_restrict(const double* finer, double* coarser, size_t cnx,
 
 /* FdmMgUtils3::correct -- src/jet/fdm_mg_linear_system3.cpp:100-179.
  * Trilinear (1/4, 3/4) prolongation added to the finer grid.  Jet bounds the
- * k upper clamp with n.y (:159); restated as written (no effect on cubes). */
+ * k upper clamp with n.y (:159); restated (no effect on cubes). */
 void or_mg_correct(const double* coarser, double* finer, size_t fnx,
                    size_t fny, size_t fnz) {
     const size_t cnx = fnx / 2, cny = fny / 2;
@@ -331,7 +331,11 @@ void or_mg_correct(const double* coarser, double* finer, size_t fnx,
                     kW[1] = 0.75;
                 } else {
                     kI[0] = ck;
-                    kI[1] = (k + 1 < fny) ? ck + 1 : ck; /* n.y, as in Jet */
+                    /* Jet tests k + 1 < n.y.  Where n.y > n.z that reads past
+                     * the coarse array (undefined behaviour, nothing to
+                     * restate), so the bound is min(n.y, n.z): identical to
+                     * Jet wherever Jet stays in bounds. */
+                    kI[1] = (k + 1 < (fny < fnz ? fny : fnz)) ? ck + 1 : ck;
                     kW[0] = 0.75;
                     kW[1] = 0.25;
                 }
