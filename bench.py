@@ -30,7 +30,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = ("256³ MGPCG solve ms + HBM GB/s: mixed FP32/FP64 vs FP64, rel err ≤1e-6")
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_rbgs_fp32_256.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_sweep_fp32_256.json")
 
 
 def log(msg):
@@ -103,16 +103,17 @@ def main():
         log(f"bench: {name} {wall_ms:.3f} ms/solve, {infos[-1].iterations} it, "
             f"rel {infos[-1].rel_residual:.2e}")
 
-    # Dominant kernel of the mixed solve: one colour of a red-black sweep on
-    # the finest level (7 of them per PCG iteration).  HIP events on the
-    # solver's stream around 50 back-to-back launches.
+    # Dominant kernel of the mixed solve: the fused red-black sweep on the
+    # finest level (4 per PCG iteration).  HIP events on the solver's stream
+    # around 50 back-to-back launches.
     kern = {}
-    for kname, kid in (("rbgs", jg.JG_KERNEL_RBGS), ("dir_spmv", jg.JG_KERNEL_DIR_SPMV),
+    for kname, kid in (("sweep", jg.JG_KERNEL_SWEEP), ("rbgs", jg.JG_KERNEL_RBGS),
+                       ("dir_spmv", jg.JG_KERNEL_DIR_SPMV),
                        ("residual", jg.JG_KERNEL_RESIDUAL), ("update_xr", jg.JG_KERNEL_UPDATE_XR),
                        ("copy", jg.JG_KERNEL_COPY)):
         ms, nbytes = solver.time_kernel(kid, jg.JG_MIXED, 50)
         kern[kname] = {"avg_ms": ms, "bytes": nbytes, "gbs": nbytes / ms / 1e6}
-    dom = kern["rbgs"]
+    dom = kern["sweep"]
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
@@ -143,7 +144,7 @@ def main():
         "cells_per_s": {k: n_cells / (res[k]["ms"] * 1e-3) for k in res},
         "kernels_mixed": {k: {"avg_us": round(vv["avg_ms"] * 1e3, 2), "alg_gbs": round(vv["gbs"], 1)}
                           for k, vv in kern.items()},
-        "roofline": {"bound": "hbm", "kernel": "k_rbgs<float> (one colour, finest level)",
+        "roofline": {"bound": "hbm", "kernel": "k_rb_sweep<float> (both colours, finest level)",
                      "achieved": round(dom["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic},
     }

@@ -113,6 +113,23 @@ __device__ __forceinline__ double reduce_partials(const double* p, int n, double
     return r;  // valid in every thread
 }
 
+// XCD-aware tile order.  Workgroups are handed to the 8 XCDs round-robin
+// (workgroup id mod 8), and each XCD has its own L2.  Tiles overlap by
+// their halos, so spatial neighbours should share an L2: workgroup id w is
+// mapped to logical tile (w mod 8) * (W / 8) + w / 8, which gives every XCD a
+// contiguous run of tiles (x fastest, then y, then z).  The last W mod 8
+// workgroups keep the identity mapping.
+__device__ __forceinline__ void xcd_tile(int& bx, int& by, int& bz) {
+    const int nbx = gridDim.x, nby = gridDim.y;
+    const int total = nbx * nby * (int)gridDim.z;
+    int id = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
+    const int per = total / 8;
+    if (id < per * 8) id = (id % 8) * per + id / 8;
+    bx = id % nbx;
+    by = (id / nbx) % nby;
+    bz = id / (nbx * nby);
+}
+
 // --------------------------------------------------------------- stencil
 
 // Off-diagonal sum  sum_nbr a(c,nbr) * x(nbr)  of a 7-point row at (i,j,k).

@@ -37,7 +37,10 @@ struct Level {
     // Preconditioner work vectors (levels > 0 use x/b; every level uses t).
     double *x64 = nullptr, *b64 = nullptr, *t64 = nullptr;
     float *x32 = nullptr, *b32 = nullptr, *t32 = nullptr;
+    double* y64 = nullptr;  // ping-pong partner of x for the fused sweeps
+    float* y32 = nullptr;
     int8_t* markers = nullptr;  // pressure-system markers of this level
+    int zc64 = 0, zc32 = 0;     // tuned z-chunk of the fused sweep (0: untuned)
 };
 
 template <typename T>
@@ -48,6 +51,8 @@ struct Prec<double> {
     static double* x(Level& l) { return l.x64; }
     static double* b(Level& l) { return l.b64; }
     static double* t(Level& l) { return l.t64; }
+    static double* y(Level& l) { return l.y64; }
+    static int& zc(Level& l) { return l.zc64; }
 };
 template <>
 struct Prec<float> {
@@ -55,6 +60,8 @@ struct Prec<float> {
     static float* x(Level& l) { return l.x32; }
     static float* b(Level& l) { return l.b32; }
     static float* t(Level& l) { return l.t32; }
+    static float* y(Level& l) { return l.y32; }
+    static int& zc(Level& l) { return l.zc32; }
 };
 
 struct Geo {

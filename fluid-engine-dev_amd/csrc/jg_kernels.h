@@ -42,9 +42,11 @@ __global__ __launch_bounds__(NT) void k_spmv(const Row<TA>* __restrict__ A,
                                              Dims d, int zc, double* __restrict__ partial,
                                              const Ctl* __restrict__ ctl) {
     if (ctl && ctl->done) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k0 = blockIdx.z * zc;
+    int bx, by, bz;
+    xcd_tile(bx, by, bz);
+    const int i = bx * blockDim.x + threadIdx.x;
+    const int j = by * blockDim.y + threadIdx.y;
+    const int k0 = bz * zc;
     const int k1 = min(d.nz, k0 + zc);
     double acc = 0.0;
     if (i < d.nx && j < d.ny) {
@@ -80,9 +82,11 @@ __global__ __launch_bounds__(NT) void k_dir_spmv(const Row<TA>* __restrict__ A,
 #pragma clang fp contract(off)
     if (ctl->done) return;
     const double beta = ctl->beta;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k0 = blockIdx.z * zc;
+    int bx, by, bz;
+    xcd_tile(bx, by, bz);
+    const int i = bx * blockDim.x + threadIdx.x;
+    const int j = by * blockDim.y + threadIdx.y;
+    const int k0 = bz * zc;
     const int k1 = min(d.nz, k0 + zc);
     double acc = 0.0;
     auto D = [&](idx_t n) -> TX { return (TX)((double)z[n] + beta * (double)dold[n]); };
@@ -125,9 +129,11 @@ __global__ __launch_bounds__(NT) void k_residual(const Row<TA>* __restrict__ A,
                                                  const Ctl* __restrict__ ctl) {
 #pragma clang fp contract(off)
     if (REFRESH && (ctl->done || !ctl->refresh)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k0 = blockIdx.z * zc;
+    int bx, by, bz;
+    xcd_tile(bx, by, bz);
+    const int i = bx * blockDim.x + threadIdx.x;
+    const int j = by * blockDim.y + threadIdx.y;
+    const int k0 = bz * zc;
     const int k1 = min(d.nz, k0 + zc);
     double acc = 0.0;
     if (i < d.nx && j < d.ny) {

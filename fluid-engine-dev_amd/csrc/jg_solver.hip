@@ -15,6 +15,9 @@
 
 #include "jg_internal.h"
 #include "jg_kernels.h"
+#include "jg_sweep.h"
+
+#include <cstdlib>
 
 using namespace jg;
 
@@ -22,14 +25,64 @@ namespace {
 
 // ------------------------------------------------------- level kernels
 
-template <typename TA, typename TX>
-void launch_rbgs(jg_solver* s, const Row<TA>* A, const TX* b, TX* x, const Dims& d, int color,
-                 TX omega, bool zero_init, const Ctl* ctl) {
-    Geo g = geo_stencil(d);
-    if (zero_init)
-        k_rbgs<TA, TX, true><<<g.grid, g.block, 0, s->stream>>>(A, b, x, d, g.zc, color, omega, ctl);
-    else
-        k_rbgs<TA, TX, false><<<g.grid, g.block, 0, s->stream>>>(A, b, x, d, g.zc, color, omega, ctl);
+// z-chunk of the fused sweep when no tuned value exists: enough blocks to
+// fill the chip, chunks long enough that the 4-plane pipeline fill stays a
+// small overhead.  JG_SWEEP_ZC overrides everything (tuning experiments).
+int sweep_zc_env() {
+    static const int env = [] {
+        const char* e = std::getenv("JG_SWEEP_ZC");
+        return e ? std::atoi(e) : 0;
+    }();
+    return env;
+}
+
+int sweep_zc(const Dims& d, int tuned) {
+    if (sweep_zc_env() > 0) return std::min(sweep_zc_env(), d.nz);
+    if (tuned > 0) return std::min(tuned, d.nz);
+    const long tiles = (long)((d.nx + kSwIX - 1) / kSwIX) * ((d.ny + kSwIY - 1) / kSwIY);
+    const long gz = std::max<long>(1, (1024 + tiles - 1) / tiles);
+    long zc = (d.nz + gz - 1) / gz;
+    zc = std::max<long>(8, std::min<long>(64, zc));
+    return (int)std::min<long>(zc, d.nz);
+}
+
+// One fused red-black sweep (see jg_sweep.h).  first: colour updated first
+// (0 = red, forward; 1 = black, backward).  xin == nullptr: x starts at zero.
+// coarse != nullptr: add the prolongation of coarse to x on load.
+// p_bx/p_bb != nullptr: emit b.x and b.b partials; returns the block count.
+template <typename T>
+int launch_sweep(jg_solver* s, const Row<T>* A, const T* b, const T* xin, T* xout, const Dims& d,
+                 int first, const T* coarse, double* p_bx, double* p_bb, T omega, const Ctl* ctl,
+                 int tuned_zc = 0) {
+    const int zc = sweep_zc(d, tuned_zc);
+    dim3 block(kSwLX, kSwTY, 1);
+    dim3 grid((d.nx + kSwIX - 1) / kSwIX, (d.ny + kSwIY - 1) / kSwIY, (d.nz + zc - 1) / zc);
+    const int code = (first ? 8 : 0) | (xin ? 0 : 4) | (coarse ? 2 : 0) | (p_bx ? 1 : 0);
+#define JG_SWEEP_CASE(C, F, Z, K, D)                                                          \
+    case C:                                                                                   \
+        k_rb_sweep<T, F, Z, K, D><<<grid, block, 0, s->stream>>>(A, b, xin, xout, d, zc, omega, \
+                                                                 coarse, p_bx, p_bb, ctl);    \
+        break;
+    switch (code) {
+        JG_SWEEP_CASE(0, 0, false, false, false)
+        JG_SWEEP_CASE(1, 0, false, false, true)
+        JG_SWEEP_CASE(2, 0, false, true, false)
+        JG_SWEEP_CASE(3, 0, false, true, true)
+        JG_SWEEP_CASE(4, 0, true, false, false)
+        JG_SWEEP_CASE(5, 0, true, false, true)
+        JG_SWEEP_CASE(6, 0, true, true, false)
+        JG_SWEEP_CASE(7, 0, true, true, true)
+        JG_SWEEP_CASE(8, 1, false, false, false)
+        JG_SWEEP_CASE(9, 1, false, false, true)
+        JG_SWEEP_CASE(10, 1, false, true, false)
+        JG_SWEEP_CASE(11, 1, false, true, true)
+        JG_SWEEP_CASE(12, 1, true, false, false)
+        JG_SWEEP_CASE(13, 1, true, false, true)
+        JG_SWEEP_CASE(14, 1, true, true, false)
+        JG_SWEEP_CASE(15, 1, true, true, true)
+    }
+#undef JG_SWEEP_CASE
+    return (int)(grid.x * grid.y * grid.z);
 }
 
 template <typename T>
@@ -39,16 +92,25 @@ struct MgCfg {
 };
 
 // z = M r: one symmetric V-cycle with zero initial guess.  Level 0 reads
-// its right-hand side from r0 and writes its result to z0.
+// its right-hand side from r0 and writes its result to z0.  Each level runs
+// `pre` forward sweeps (red then black) from x = 0, restricts the residual,
+// recurses, then `post` backward sweeps (black then red), the first of which
+// adds the prolonged coarse correction as it loads x.  Sweeps ping-pong
+// between the level's x and y vectors so that the last one lands in x.
+// Returns the number of b.x / b.b partials the last level-0 sweep wrote to
+// slots 0 and 1 (0 if it wrote none).
 template <typename T>
-int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl) {
+int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, int* dot_blocks) {
     const size_t L = s->lv.size();
+    *dot_blocks = 0;
     auto B = [&](size_t l) -> const T* { return l == 0 ? r0 : Prec<T>::b(s->lv[l]); };
     auto X = [&](size_t l) -> T* { return l == 0 ? z0 : Prec<T>::x(s->lv[l]); };
+    std::vector<const T*> pre_out(L, nullptr);
     for (size_t l = 0; l < L; ++l) {
         Level& lv = s->lv[l];
         const Row<T>* A = Prec<T>::A(lv);
         const Dims& d = lv.d;
+        T* Y = Prec<T>::y(lv);
         if (l + 1 == L) {
             const int n = (int)d.n();
             const int fwd = std::max(1u, (c.coarsest + 1) / 2);
@@ -56,55 +118,113 @@ int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl) 
                 k_coarsest_lds<T, T, 1024><<<1, 1024, 0, s->stream>>>(A, B(l), X(l), d, fwd, fwd,
                                                                       c.omega, ctl);
             } else {
-                launch_rbgs<T, T>(s, A, B(l), X(l), d, 0, c.omega, true, ctl);
-                launch_rbgs<T, T>(s, A, B(l), X(l), d, 1, c.omega, false, ctl);
-                for (int it = 1; it < fwd; ++it) {
-                    launch_rbgs<T, T>(s, A, B(l), X(l), d, 0, c.omega, false, ctl);
-                    launch_rbgs<T, T>(s, A, B(l), X(l), d, 1, c.omega, false, ctl);
-                }
-                for (int it = 0; it < fwd; ++it) {
-                    launch_rbgs<T, T>(s, A, B(l), X(l), d, 1, c.omega, false, ctl);
-                    launch_rbgs<T, T>(s, A, B(l), X(l), d, 0, c.omega, false, ctl);
+                const int S = 2 * fwd;
+                const T* in = nullptr;
+                for (int k = 0; k < S; ++k) {
+                    T* out = ((S - 1 - k) % 2 == 0) ? X(l) : Y;
+                    launch_sweep<T>(s, A, B(l), in, out, d, k < fwd ? 0 : 1, nullptr, nullptr, nullptr,
+                                    c.omega, ctl, Prec<T>::zc(lv));
+                    in = out;
                 }
             }
             JG_TRY(check_launch(s, "coarsest"));
             break;
         }
-        // pre-smoothing: forward red-black sweeps from x = 0
-        if (c.pre == 0) {
-            k_fill<T><<<nblocks_1d(d.n()), NT, 0, s->stream>>>(X(l), T(0), d.n());
-        } else {
-            launch_rbgs<T, T>(s, A, B(l), X(l), d, 0, c.omega, true, ctl);
-            launch_rbgs<T, T>(s, A, B(l), X(l), d, 1, c.omega, false, ctl);
-            for (unsigned it = 1; it < c.pre; ++it) {
-                launch_rbgs<T, T>(s, A, B(l), X(l), d, 0, c.omega, false, ctl);
-                launch_rbgs<T, T>(s, A, B(l), X(l), d, 1, c.omega, false, ctl);
-            }
+        const unsigned post = l == 0 ? c.post0 : c.post;
+        const unsigned S = c.pre + post;
+        auto buf = [&](unsigned k) -> T* { return ((S - 1 - k) % 2 == 0) ? X(l) : Y; };
+        const T* in = nullptr;
+        for (unsigned k = 0; k < c.pre; ++k) {
+            launch_sweep<T>(s, A, B(l), in, buf(k), d, 0, nullptr, nullptr, nullptr, c.omega, ctl,
+                            Prec<T>::zc(lv));
+            in = buf(k);
         }
-        // residual -> restrict to the next level's right-hand side
-        Geo g = geo_stencil(d);
-        k_residual<T, T, T, false, false><<<g.grid, g.block, 0, s->stream>>>(
-            A, X(l), B(l), Prec<T>::t(lv), d, g.zc, nullptr, ctl);
+        pre_out[l] = in;  // nullptr: x is still zero
         const Dims& cd = s->lv[l + 1].d;
         Geo gc = geo_planes(cd);
-        k_restrict<T><<<gc.grid, gc.block, 0, s->stream>>>(Prec<T>::t(lv), Prec<T>::b(s->lv[l + 1]),
-                                                           cd, ctl);
+        if (in) {
+            Geo g = geo_stencil(d);
+            k_residual<T, T, T, false, false><<<g.grid, g.block, 0, s->stream>>>(
+                A, in, B(l), Prec<T>::t(lv), d, g.zc, nullptr, ctl);
+            k_restrict<T><<<gc.grid, gc.block, 0, s->stream>>>(Prec<T>::t(lv),
+                                                               Prec<T>::b(s->lv[l + 1]), cd, ctl);
+        } else {
+            k_restrict<T><<<gc.grid, gc.block, 0, s->stream>>>(B(l), Prec<T>::b(s->lv[l + 1]), cd,
+                                                               ctl);
+        }
         JG_TRY(check_launch(s, "vcycle down"));
     }
     for (size_t l = L - 1; l-- > 0;) {
         Level& lv = s->lv[l];
         const Row<T>* A = Prec<T>::A(lv);
         const Dims& d = lv.d;
-        Geo gf = geo_planes(d);
-        k_correct<T><<<gf.grid, gf.block, 0, s->stream>>>(X(l + 1), X(l), d, ctl);
+        T* Y = Prec<T>::y(lv);
         const unsigned post = l == 0 ? c.post0 : c.post;
-        for (unsigned it = 0; it < post; ++it) {
-            launch_rbgs<T, T>(s, A, B(l), X(l), d, 1, c.omega, false, ctl);
-            launch_rbgs<T, T>(s, A, B(l), X(l), d, 0, c.omega, false, ctl);
+        const unsigned S = c.pre + post;
+        auto buf = [&](unsigned k) -> T* { return ((S - 1 - k) % 2 == 0) ? X(l) : Y; };
+        const T* coarse = X(l + 1);
+        if (post == 0) {
+            // No post-smoothing: x = pre-smoothed x (already in X) + P e.
+            if (!pre_out[l]) k_fill<T><<<nblocks_1d(d.n()), NT, 0, s->stream>>>(X(l), T(0), d.n());
+            Geo gf = geo_planes(d);
+            k_correct<T><<<gf.grid, gf.block, 0, s->stream>>>(coarse, X(l), d, ctl);
+        } else {
+            const T* in = pre_out[l];
+            for (unsigned k = c.pre; k < S; ++k) {
+                const bool last0 = (l == 0 && k + 1 == S);
+                const int nb = launch_sweep<T>(s, A, B(l), in, buf(k), d, 1, k == c.pre ? coarse : nullptr,
+                                               last0 ? s->slot(0) : nullptr,
+                                               last0 ? s->slot(1) : nullptr, c.omega, ctl,
+                                               Prec<T>::zc(lv));
+                if (last0) *dot_blocks = nb;
+                in = buf(k);
+            }
         }
         JG_TRY(check_launch(s, "vcycle up"));
     }
     return JG_OK;
+}
+
+// z-chunk autotuner of the fused sweep: times each candidate on the level
+// with the level's own matrix and scratch vectors (run once per hierarchy
+// and precision, before a solve, so nothing live is overwritten).
+template <typename T>
+int autotune_sweeps(jg_solver* s) {
+    if (sweep_zc_env() > 0) return JG_OK;
+    for (auto& lv : s->lv) {
+        if (Prec<T>::zc(lv) > 0) continue;
+        const Dims& d = lv.d;
+        if (d.n() < 32 * 32 * 32) {
+            Prec<T>::zc(lv) = sweep_zc(d, 0);
+            continue;
+        }
+        const idx_t n = d.n();
+        k_fill<T><<<nblocks_1d(n), NT, 0, s->stream>>>(Prec<T>::t(lv), T(1), n);
+        k_fill<T><<<nblocks_1d(n), NT, 0, s->stream>>>(Prec<T>::y(lv), T(0.5), n);
+        const int cands[] = {8, 12, 16, 24, 32, 48, 64};
+        float best = 1e30f;
+        int best_zc = sweep_zc(d, 0);
+        for (int zc : cands) {
+            if (zc > d.nz) break;
+            auto run = [&]() {
+                launch_sweep<T>(s, Prec<T>::A(lv), Prec<T>::t(lv), Prec<T>::y(lv), Prec<T>::x(lv), d, 1,
+                                nullptr, nullptr, nullptr, T(1), nullptr, zc);
+            };
+            run();
+            JG_HIP(hipEventRecord(s->ev0, s->stream));
+            for (int r = 0; r < 3; ++r) run();
+            JG_HIP(hipEventRecord(s->ev1, s->stream));
+            JG_TRY(sync(s));
+            float ms = 0.f;
+            JG_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+            if (ms < best) {
+                best = ms;
+                best_zc = zc;
+            }
+        }
+        Prec<T>::zc(lv) = best_zc;
+    }
+    return check_launch(s, "autotune");
 }
 
 // --------------------------------------------------------------- PCG
@@ -122,15 +242,29 @@ int pcg_device(jg_solver* s, const Row<T>* A, const T* b, T* x, T* r, T* z, T* d
     double* p_rz = s->slot(0);
     double* p_rr = s->slot(1);
     double* p_dq = s->slot(2);
+    // z = M r and the r.z / r.r partials (from the V-cycle's last sweep when
+    // it provides them, else from dot kernels); returns the partial count.
+    auto precondition = [&](const Ctl* c) -> int {
+        int nb = 0;
+        if (mg) {
+            int st = vcycle<T>(s, r, z, *mg, c, &nb);
+            if (st != JG_OK) return -st;
+        }
+        if (nb == 0) {
+            k_dot<T, T><<<nb1, NT, 0, s->stream>>>(r, z, n, p_rz, c);
+            if (mg) k_dot<T, T><<<nb1, NT, 0, s->stream>>>(r, r, n, p_rr, c);
+            nb = nb1;
+        }
+        return nb;
+    };
     // r = b - A x  (x = 0, so r = b; keep the general form)
     k_residual<T, T, T, false, false><<<g.grid, g.block, 0, s->stream>>>(A, x, b, r, d, g.zc,
                                                                         nullptr, nullptr);
     k_fill<T><<<nb1, NT, 0, s->stream>>>(dA, T(0), n);
     k_fill<T><<<nb1, NT, 0, s->stream>>>(dB, T(0), n);
-    if (mg) JG_TRY(vcycle<T>(s, r, z, *mg, nullptr));
-    k_dot<T, T><<<nb1, NT, 0, s->stream>>>(r, z, n, p_rz, nullptr);
-    k_dot<T, T><<<nb1, NT, 0, s->stream>>>(r, r, n, p_rr, nullptr);
-    k_fin_init<<<1, NT, 0, s->stream>>>(ctl, p_rz, p_rr, nb1);
+    int nb = precondition(nullptr);
+    if (nb < 0) return -nb;
+    k_fin_init<<<1, NT, 0, s->stream>>>(ctl, p_rz, mg ? p_rr : p_rz, nb);
     JG_TRY(check_launch(s, "pcg init"));
     JG_HIP(hipMemcpyAsync(s->ctl_host, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
     JG_TRY(sync(s));
@@ -145,10 +279,9 @@ int pcg_device(jg_solver* s, const Row<T>* A, const T* b, T* x, T* r, T* z, T* d
         k_update_xr<T><<<nb1, NT, 0, s->stream>>>(x, r, dnew, q, n, ctl);
         k_residual<T, T, T, false, true><<<g.grid, g.block, 0, s->stream>>>(A, x, b, r, d, g.zc,
                                                                            nullptr, ctl);
-        if (mg) JG_TRY(vcycle<T>(s, r, z, *mg, ctl));
-        k_dot<T, T><<<nb1, NT, 0, s->stream>>>(r, z, n, p_rz, ctl);
-        if (mg) k_dot<T, T><<<nb1, NT, 0, s->stream>>>(r, r, n, p_rr, ctl);
-        k_fin_sigma<<<1, NT, 0, s->stream>>>(ctl, p_rz, mg ? p_rr : p_rz, nb1);
+        nb = precondition(ctl);
+        if (nb < 0) return -nb;
+        k_fin_sigma<<<1, NT, 0, s->stream>>>(ctl, p_rz, mg ? p_rr : p_rz, nb);
         JG_TRY(check_launch(s, "pcg iteration"));
         std::swap(dold, dnew);
         ++launched;
@@ -298,10 +431,12 @@ int jg_solver_set_levels(jg_solver* s, size_t nlev, const size_t* nx, const size
         JG_TRY(dalloc(s, &lv.t64, n));
         JG_TRY(dalloc(s, &lv.t32, n));
         JG_TRY(dalloc(s, &lv.markers, n));
+        JG_TRY(dalloc(s, &lv.y64, n));
+        JG_TRY(dalloc(s, &lv.y32, n));
+        JG_TRY(dalloc(s, &lv.x64, n));
+        JG_TRY(dalloc(s, &lv.x32, n));
         if (l > 0) {
-            JG_TRY(dalloc(s, &lv.x64, n));
             JG_TRY(dalloc(s, &lv.b64, n));
-            JG_TRY(dalloc(s, &lv.x32, n));
             JG_TRY(dalloc(s, &lv.b32, n));
         }
     }
@@ -403,8 +538,13 @@ int jg_solver_mgpcg(jg_solver* s, unsigned max_iter, const jg_mg_params* p,
     unsigned iters = 0, refinements = 0;
     double last_res = 0.0;
     Ctl* ctl = s->ctl;
+    if (opt.precision == JG_MIXED) {
+        JG_TRY(ensure_f32(s));
+        JG_TRY(autotune_sweeps<float>(s));
+    } else {
+        JG_TRY(autotune_sweeps<double>(s));
+    }
     k_fill<double><<<nblocks_1d(n), NT, 0, s->stream>>>(s->x, 0.0, n);
-    if (opt.precision == JG_MIXED) JG_TRY(ensure_f32(s));
     JG_HIP(hipEventRecord(s->ev0, s->stream));
     if (opt.precision != JG_MIXED) {
         MgCfg<double> mg{p->restrict_iter, p->final_iter, p->correct_iter, p->coarsest_iter, omega};
@@ -550,6 +690,10 @@ int time_kernel_t(jg_solver* s, const Vecs<T>& v, int kernel, unsigned reps, dou
             case JG_KERNEL_DOT:
                 k_dot<T, T><<<nb1, NT, 0, s->stream>>>(v.r, v.z, n, s->slot(0), s->ctl);
                 break;
+            case JG_KERNEL_SWEEP:
+                launch_sweep<T>(s, A, v.b, v.r, v.x, d, 1, nullptr, nullptr, nullptr, T(1), s->ctl,
+                                Prec<T>::zc(l0));
+                break;
             default:
                 k_convert<T, T><<<nb1, NT, 0, s->stream>>>(v.r, v.q, n);
                 break;
@@ -563,6 +707,7 @@ int time_kernel_t(jg_solver* s, const Vecs<T>& v, int kernel, unsigned reps, dou
         case JG_KERNEL_RESIDUAL: per_cell = sa + 3 * sv; break;  // A, x, b -> r
         case JG_KERNEL_UPDATE_XR: per_cell = 6 * sv; break;      // x, d, r, q -> x, r
         case JG_KERNEL_DOT: per_cell = 2 * sv; break;
+        case JG_KERNEL_SWEEP: per_cell = sa + 3 * sv; break;     // A, b, x -> x (both colours)
         default: per_cell = 2 * sv; break;
     }
     launch();  // warm
@@ -582,13 +727,15 @@ int time_kernel_t(jg_solver* s, const Vecs<T>& v, int kernel, unsigned reps, dou
 
 extern "C" int jg_solver_time_kernel(jg_solver* s, int kernel, int precision, unsigned reps,
                                      double* avg_ms, double* bytes) {
-    if (!s || s->lv.empty() || !avg_ms || !bytes || reps == 0 || kernel < 0 || kernel > 5)
+    if (!s || s->lv.empty() || !avg_ms || !bytes || reps == 0 || kernel < 0 || kernel > 6)
         return JG_ERR_INVALID_ARG;
     JG_HIP(hipSetDevice(s->device));
     if (precision == JG_MIXED) {
         JG_TRY(ensure_f32(s));
+        JG_TRY(autotune_sweeps<float>(s));
         return time_kernel_t<float>(s, vecs32(s), kernel, reps, avg_ms, bytes);
     }
+    JG_TRY(autotune_sweeps<double>(s));
     return time_kernel_t<double>(s, vecs64(s), kernel, reps, avg_ms, bytes);
 }
 
@@ -648,10 +795,23 @@ int jg_relax_red_black(jg_solver* s, size_t nx, size_t ny, size_t nz, const doub
     if (!s || !valid_dims(nx, ny, nz) || !rows || !b || !x) return JG_ERR_INVALID_ARG;
     JG_HIP(hipSetDevice(s->device));
     Dims d{(int)nx, (int)ny, (int)nz};
-    launch_rbgs<double, double>(s, (const Row<double>*)rows, b, x, d, 0, sor, false, nullptr);
-    launch_rbgs<double, double>(s, (const Row<double>*)rows, b, x, d, 1, sor, false, nullptr);
-    JG_TRY(check_launch(s, "relax_red_black"));
-    return sync(s);
+    // One fused forward sweep (red then black) reading a copy of x: the
+    // sweep is out of place because its tiles overlap.
+    double* tmp = nullptr;
+    JG_HIP(hipMalloc(&tmp, d.n() * sizeof(double)));
+    hipError_t e = hipMemcpyAsync(tmp, x, d.n() * sizeof(double), hipMemcpyDeviceToDevice, s->stream);
+    if (e == hipSuccess) {
+        launch_sweep<double>(s, (const Row<double>*)rows, b, tmp, x, d, 0, nullptr, nullptr, nullptr,
+                             sor, nullptr);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) {
+        s->err = std::string("relax_red_black: ") + hipGetErrorString(e);
+        return JG_ERR_HIP;
+    }
+    return JG_OK;
 }
 
 int jg_relax_jacobi(jg_solver* s, size_t nx, size_t ny, size_t nz, const double* rows,
