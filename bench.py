@@ -46,6 +46,8 @@ def main():
     ap.add_argument("--levels", type=int, default=6)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--rel-tol", type=float, default=1e-8)
+    ap.add_argument("--omega", type=float, default=0.0,
+                    help="red-black smoother relaxation (0: the MG parameters' sorFactor)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / oracle leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -80,9 +82,9 @@ def main():
                           final_iter=2)
     opts = {
         "fp64": jg.gpu_options(precision=jg.JG_FP64, stop_rule=jg.JG_STOP_RELATIVE,
-                               rel_tol=args.rel_tol),
+                               rel_tol=args.rel_tol, omega=args.omega),
         "mixed": jg.gpu_options(precision=jg.JG_MIXED, stop_rule=jg.JG_STOP_RELATIVE,
-                                rel_tol=args.rel_tol),
+                                rel_tol=args.rel_tol, omega=args.omega),
     }
     for _ in range(args.warmup):
         for name in ("fp64", "mixed"):

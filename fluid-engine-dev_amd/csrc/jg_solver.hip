@@ -533,7 +533,18 @@ int jg_solver_mgpcg(jg_solver* s, unsigned max_iter, const jg_mg_params* p,
     // Every level of the caller's hierarchy is used, as Jet's mgVCycle recurses
     // over A.levels.size() (include/jet/detail/mg-inl.h:26); max_levels only
     // tells callers how deep to build it.
-    const double omega = opt.omega > 0 ? opt.omega : 1.0;
+    // Relaxation factor of the red-black smoother: the GPU option if given,
+    // else the caller's Jet sorFactor (FdmMgSolver3's, default 1.5).  Any
+    // 0 < omega < 2 keeps the symmetric V-cycle positive definite.
+    if (opt.smoother != JG_SMOOTH_RBGS) {
+        s->err = "unsupported smoother";
+        return JG_ERR_UNSUPPORTED;
+    }
+    double omega = opt.omega > 0 ? opt.omega : p->sor;
+    if (!(omega > 0.0 && omega < 2.0)) {
+        s->err = "smoother relaxation factor must lie in (0, 2)";
+        return JG_ERR_INVALID_ARG;
+    }
     int status = JG_OK;
     unsigned iters = 0, refinements = 0;
     double last_res = 0.0;

@@ -20,7 +20,7 @@ ERRORS = {1: "invalid argument", 2: "out of memory", 3: "HIP error", 4: "non-fin
           5: "no device", 6: "unsupported"}
 JG_FP64, JG_MIXED = 0, 1
 JG_STOP_JET, JG_STOP_RELATIVE = 0, 1
-JG_SMOOTH_RBGS, JG_SMOOTH_JACOBI = 0, 1
+JG_SMOOTH_RBGS = 0
 (JG_KERNEL_RBGS, JG_KERNEL_DIR_SPMV, JG_KERNEL_RESIDUAL, JG_KERNEL_UPDATE_XR, JG_KERNEL_DOT,
  JG_KERNEL_COPY, JG_KERNEL_SWEEP) = range(7)
 (JG_SCENE_CLOSED_SPHERE, JG_SCENE_DAM_BREAK, JG_SCENE_SPHERE_POOL, JG_SCENE_BATCH_MEMBER,

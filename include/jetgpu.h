@@ -48,7 +48,7 @@ enum { JG_FP64 = 0, JG_MIXED = 1 };
  * ||b - A x||_2 <= rel_tol * ||b||_2 (FP64 residual). */
 enum { JG_STOP_JET = 0, JG_STOP_RELATIVE = 1 };
 
-enum { JG_SMOOTH_RBGS = 0, JG_SMOOTH_JACOBI = 1 };
+enum { JG_SMOOTH_RBGS = 0 };
 
 typedef struct jg_solver jg_solver;
 
@@ -70,8 +70,8 @@ typedef struct {
     int precision;           /* JG_FP64 (default) or JG_MIXED            */
     int stop_rule;           /* JG_STOP_JET (default) or JG_STOP_RELATIVE */
     double rel_tol;          /* for JG_STOP_RELATIVE (default 1e-8)      */
-    int smoother;            /* JG_SMOOTH_RBGS (default) or JG_SMOOTH_JACOBI */
-    double omega;            /* smoother relaxation; 0 -> 1.0 (RBGS) / 6/7 (Jacobi) */
+    int smoother;            /* JG_SMOOTH_RBGS (the only smoother so far)  */
+    double omega;            /* red-black smoother relaxation in (0,2); 0 -> Jet's sor */
     unsigned max_refine;     /* mixed: outer FP64 refinement steps (default 8) */
     double inner_rel_tol;    /* mixed: FP32 inner reduction per step (default 1e-4) */
     unsigned check_every;    /* host convergence poll interval in iterations (default 1) */
