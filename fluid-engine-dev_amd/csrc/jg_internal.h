@@ -41,6 +41,7 @@ struct Level {
     float* y32 = nullptr;
     int8_t* markers = nullptr;  // pressure-system markers of this level
     int zc64 = 0, zc32 = 0;     // tuned z-chunk of the fused sweep (0: untuned)
+    int sh64 = 0, sh32 = 0;     // tuned launch shape of the fused sweep
 };
 
 template <typename T>
@@ -53,6 +54,7 @@ struct Prec<double> {
     static double* t(Level& l) { return l.t64; }
     static double* y(Level& l) { return l.y64; }
     static int& zc(Level& l) { return l.zc64; }
+    static int& sh(Level& l) { return l.sh64; }
 };
 template <>
 struct Prec<float> {
@@ -62,6 +64,7 @@ struct Prec<float> {
     static float* t(Level& l) { return l.t32; }
     static float* y(Level& l) { return l.y32; }
     static int& zc(Level& l) { return l.zc32; }
+    static int& sh(Level& l) { return l.sh32; }
 };
 
 struct Geo {

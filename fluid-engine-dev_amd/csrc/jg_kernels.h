@@ -14,25 +14,108 @@ namespace jg {
 
 // ----------------------------------------------------------- SpMV (+ dot)
 
-// Jet's mvm operand order (src/jet/fdm_linear_system3.cpp:87-95): centre,
-// -x, +x, -y, +y, -z, +z, each product rounded before the add (the caller
-// disables contraction), so FP64 results are bitwise Jet's.  fm/xm and xp
-// are the z neighbours kept in registers (zero outside the domain).
-template <typename TA, typename TX>
-__device__ __forceinline__ TX stencil_sum(const Row<TA>* __restrict__ A, const TX* __restrict__ x,
-                                          const Row<TA>& row, idx_t c, int i, int j, const Dims& d,
-                                          TX xc, TA fm, TX xm, TX xp) {
+// Everything one cell of a z-marching stencil kernel reads in its plane:
+// the row, the -x / -y couplings (neighbours' r and u) and, for each of NV
+// source vectors, the centre value and the four lateral neighbour values
+// (slots 0 centre, 1 -x, 2 +x, 3 -y, 4 +y).  Loads at the domain edge use the
+// cell's own address (always valid) and are masked at use.  Loads are stored
+// raw and combined where used, so the next plane's loads can all be issued
+// before the current plane is computed (software pipelining) without the
+// compiler waiting on them early.
+template <typename TA, typename TX, int NV>
+struct StPlane {
+    Row<TA> row;
+    TA rl, ud;
+    TX v[NV][5];
+};
+
+// Column flags of one thread: which lateral neighbours exist.
+struct StCol {
+    bool l, r, d, u;
+};
+
+// crow: cell whose row and lateral values are loaded; cx: cell whose centre
+// values are loaded (they differ on the last plane of a chunk, where only the
+// +z value of the next plane is needed and the rest re-reads cached data).
+template <typename TA, typename TX, int NV>
+__device__ __forceinline__ void st_load(StPlane<TA, TX, NV>& p, const Row<TA>* __restrict__ A,
+                                        const TX* const (&src)[NV], idx_t crow, idx_t cx,
+                                        const StCol& h, int nx) {
+    p.row = A[crow];
+    p.rl = A[h.l ? crow - 1 : crow].r;
+    p.ud = A[h.d ? crow - nx : crow].u;
+#pragma unroll
+    for (int s = 0; s < NV; ++s) {
+        p.v[s][0] = src[s][cx];
+        p.v[s][1] = src[s][h.l ? crow - 1 : crow];
+        p.v[s][2] = src[s][h.r ? crow + 1 : crow];
+        p.v[s][3] = src[s][h.d ? crow - nx : crow];
+        p.v[s][4] = src[s][h.u ? crow + nx : crow];
+    }
+}
+
+// Value of slot q: the source itself (NV = 1) or the CG direction
+// z + beta d_old (NV = 2; Jet's axpy(beta, d, s, d) with s = z).
+template <typename TX, int NV>
+__device__ __forceinline__ TX st_val(const TX (&v)[NV][5], int q, double beta) {
 #pragma clang fp contract(off)
-    TX s = (TX)row.c * xc;
-    s += (i > 0) ? (TX)A[c - 1].r * x[c - 1] : TX(0);
-    s += (i + 1 < d.nx) ? (TX)row.r * x[c + 1] : TX(0);
-    s += (j > 0) ? (TX)A[c - d.nx].u * x[c - d.nx] : TX(0);
-    s += (j + 1 < d.ny) ? (TX)row.u * x[c + d.nx] : TX(0);
+    if constexpr (NV == 1) {
+        return v[0][q];
+    } else {
+        return (TX)((double)v[0][q] + beta * (double)v[1][q]);
+    }
+}
+
+// Jet's mvm operand order (src/jet/fdm_linear_system3.cpp:87-95): centre,
+// -x, +x, -y, +y, -z, +z, each product rounded before the add (contraction
+// off), so FP64 results are bitwise Jet's.  fm/xm and xp are the z
+// neighbours (zero outside the domain).
+template <typename TA, typename TX, int NV>
+__device__ __forceinline__ TX st_sum(const StPlane<TA, TX, NV>& p, const StCol& h, double beta,
+                                     TA fm, TX xm, TX xp) {
+#pragma clang fp contract(off)
+    TX s = (TX)p.row.c * st_val<TX, NV>(p.v, 0, beta);
+    s += h.l ? (TX)p.rl * st_val<TX, NV>(p.v, 1, beta) : TX(0);
+    s += h.r ? (TX)p.row.r * st_val<TX, NV>(p.v, 2, beta) : TX(0);
+    s += h.d ? (TX)p.ud * st_val<TX, NV>(p.v, 3, beta) : TX(0);
+    s += h.u ? (TX)p.row.u * st_val<TX, NV>(p.v, 4, beta) : TX(0);
     s += (TX)fm * xm;
-    s += (TX)row.f * xp;
+    s += (TX)p.row.f * xp;
     return s;
 }
 
+// The z-march shared by the stencil kernels: for each plane k of the chunk
+// calls op(c, plane, h, fm, xm, xp) with the next plane's loads already
+// issued.  xm / xp are the -z / +z neighbours' values (zero outside).
+template <typename TA, typename TX, int NV, typename Op>
+__device__ __forceinline__ void st_march(const Row<TA>* __restrict__ A, const TX* const (&src)[NV],
+                                         double beta, int i, int j, int k0, int k1, const Dims& d,
+                                         Op&& op) {
+    const idx_t sz = (idx_t)d.nx * d.ny;
+    const StCol h{i > 0, i + 1 < d.nx, j > 0, j + 1 < d.ny};
+    idx_t c = i + (idx_t)d.nx * (j + (idx_t)d.ny * k0);
+    TX xm = TX(0);
+    TA fm = TA(0);
+    if (k0 > 0) {
+        TX vm[NV][5];
+#pragma unroll
+        for (int s = 0; s < NV; ++s) vm[s][0] = src[s][c - sz];
+        xm = st_val<TX, NV>(vm, 0, beta);
+        fm = A[c - sz].f;
+    }
+    StPlane<TA, TX, NV> cur;
+    st_load(cur, A, src, c, c, h, d.nx);
+    for (int k = k0; k < k1; ++k, c += sz) {
+        StPlane<TA, TX, NV> nxt;
+        const bool zp = k + 1 < d.nz;
+        st_load(nxt, A, src, k + 1 < k1 ? c + sz : c, zp ? c + sz : c, h, d.nx);
+        const TX xp = zp ? st_val<TX, NV>(nxt.v, 0, beta) : TX(0);
+        op(c, cur, h, fm, xm, xp);
+        xm = st_val<TX, NV>(cur.v, 0, beta);
+        fm = cur.row.f;
+        cur = nxt;
+    }
+}
 
 // y = A x  (FdmBlas3::mvm, src/jet/fdm_linear_system3.cpp:79-96); optional
 // partial sum of x.y per block (the d.q of CG).
@@ -50,21 +133,14 @@ __global__ __launch_bounds__(NT) void k_spmv(const Row<TA>* __restrict__ A,
     const int k1 = min(d.nz, k0 + zc);
     double acc = 0.0;
     if (i < d.nx && j < d.ny) {
-        const idx_t sz = (idx_t)d.nx * d.ny;
-        idx_t c = i + (idx_t)d.nx * (j + (idx_t)d.ny * k0);
-        TX xm = k0 > 0 ? x[c - sz] : TX(0);
-        TA fm = k0 > 0 ? A[c - sz].f : TA(0);
-        TX xc = x[c];
-        for (int k = k0; k < k1; ++k, c += sz) {
-            const Row<TA> row = A[c];
-            const TX xp = (k + 1 < d.nz) ? x[c + sz] : TX(0);
-            const TX s = stencil_sum<TA, TX>(A, x, row, c, i, j, d, xc, fm, xm, xp);
-            y[c] = s;
-            if (DOT) acc += (double)xc * (double)s;
-            xm = xc;
-            xc = xp;
-            fm = row.f;
-        }
+        const TX* const src[1] = {x};
+        st_march<TA, TX, 1>(A, src, 0.0, i, j, k0, k1, d,
+                            [&](idx_t c, const StPlane<TA, TX, 1>& p, const StCol& h, TA fm, TX xm,
+                                TX xp) {
+                                const TX s = st_sum<TA, TX, 1>(p, h, 0.0, fm, xm, xp);
+                                y[c] = s;
+                                if (DOT) acc += (double)p.v[0][0] * (double)s;
+                            });
     }
     if (DOT) block_partial<NT>(acc, partial);
 }
@@ -79,7 +155,6 @@ __global__ __launch_bounds__(NT) void k_dir_spmv(const Row<TA>* __restrict__ A,
                                                  TX* __restrict__ dnew, TX* __restrict__ q,
                                                  Dims d, int zc, double* __restrict__ partial,
                                                  const Ctl* __restrict__ ctl) {
-#pragma clang fp contract(off)
     if (ctl->done) return;
     const double beta = ctl->beta;
     int bx, by, bz;
@@ -89,38 +164,25 @@ __global__ __launch_bounds__(NT) void k_dir_spmv(const Row<TA>* __restrict__ A,
     const int k0 = bz * zc;
     const int k1 = min(d.nz, k0 + zc);
     double acc = 0.0;
-    auto D = [&](idx_t n) -> TX { return (TX)((double)z[n] + beta * (double)dold[n]); };
     if (i < d.nx && j < d.ny) {
-        const idx_t sz = (idx_t)d.nx * d.ny;
-        idx_t c = i + (idx_t)d.nx * (j + (idx_t)d.ny * k0);
-        TX xm = k0 > 0 ? D(c - sz) : TX(0);
-        TA fm = k0 > 0 ? A[c - sz].f : TA(0);
-        TX xc = D(c);
-        for (int k = k0; k < k1; ++k, c += sz) {
-            const Row<TA> row = A[c];
-            const TX xp = (k + 1 < d.nz) ? D(c + sz) : TX(0);
-            // Jet's mvm operand order (src/jet/fdm_linear_system3.cpp:87-95)
-            TX s = (TX)row.c * xc;
-            s += (i > 0) ? (TX)A[c - 1].r * D(c - 1) : TX(0);
-            s += (i + 1 < d.nx) ? (TX)row.r * D(c + 1) : TX(0);
-            s += (j > 0) ? (TX)A[c - d.nx].u * D(c - d.nx) : TX(0);
-            s += (j + 1 < d.ny) ? (TX)row.u * D(c + d.nx) : TX(0);
-            s += (TX)fm * xm;
-            s += (TX)row.f * xp;
-            q[c] = s;
-            dnew[c] = xc;
-            acc += (double)xc * (double)s;
-            xm = xc;
-            xc = xp;
-            fm = row.f;
-        }
+        const TX* const src[2] = {z, dold};
+        st_march<TA, TX, 2>(A, src, beta, i, j, k0, k1, d,
+                            [&](idx_t c, const StPlane<TA, TX, 2>& p, const StCol& h, TA fm, TX xm,
+                                TX xp) {
+                                const TX s = st_sum<TA, TX, 2>(p, h, beta, fm, xm, xp);
+                                const TX dc = st_val<TX, 2>(p.v, 0, beta);
+                                q[c] = s;
+                                dnew[c] = dc;
+                                acc += (double)dc * (double)s;
+                            });
     }
     block_partial<NT>(acc, partial);
 }
 
 // r = b - A x (FdmBlas3::residual, src/jet/fdm_linear_system3.cpp:98-116);
-// optional partial r.r.  When REFRESH, runs only on iterations whose Ctl
-// asks for Jet's true-residual recompute (include/jet/detail/cg-inl.h:58-63).
+// TR may be narrower than TX (FP32 right-hand side of the mixed-precision
+// inner solve).  DOT: partial r.r.  REFRESH: only on iterations where the
+// control block asks for a true residual (Jet's every-50 / sigma rule).
 template <typename TA, typename TX, typename TR, bool DOT, bool REFRESH>
 __global__ __launch_bounds__(NT) void k_residual(const Row<TA>* __restrict__ A,
                                                  const TX* __restrict__ x,
@@ -138,27 +200,24 @@ __global__ __launch_bounds__(NT) void k_residual(const Row<TA>* __restrict__ A,
     double acc = 0.0;
     if (i < d.nx && j < d.ny) {
         const idx_t sz = (idx_t)d.nx * d.ny;
-        idx_t c = i + (idx_t)d.nx * (j + (idx_t)d.ny * k0);
-        TX xm = k0 > 0 ? x[c - sz] : TX(0);
-        TA fm = k0 > 0 ? A[c - sz].f : TA(0);
-        TX xc = x[c];
-        for (int k = k0; k < k1; ++k, c += sz) {
-            const Row<TA> row = A[c];
-            const TX xp = (k + 1 < d.nz) ? x[c + sz] : TX(0);
-            // Jet's residual operand order (src/jet/fdm_linear_system3.cpp:107-115)
-            TX rv = b[c] - (TX)row.c * xc;
-            rv -= (i > 0) ? (TX)A[c - 1].r * x[c - 1] : TX(0);
-            rv -= (i + 1 < d.nx) ? (TX)row.r * x[c + 1] : TX(0);
-            rv -= (j > 0) ? (TX)A[c - d.nx].u * x[c - d.nx] : TX(0);
-            rv -= (j + 1 < d.ny) ? (TX)row.u * x[c + d.nx] : TX(0);
-            rv -= (TX)fm * xm;
-            rv -= (TX)row.f * xp;
-            r[c] = (TR)rv;
-            if (DOT) acc += (double)rv * (double)rv;
-            xm = xc;
-            xc = xp;
-            fm = row.f;
-        }
+        const TX* const src[1] = {x};
+        TX bn = b[i + (idx_t)d.nx * (j + (idx_t)d.ny * k0)];
+        st_march<TA, TX, 1>(A, src, 0.0, i, j, k0, k1, d,
+                            [&](idx_t c, const StPlane<TA, TX, 1>& p, const StCol& h, TA fm, TX xm,
+                                TX xp) {
+                                const TX bc = bn;
+                                bn = b[c + sz < (idx_t)d.nz * sz ? c + sz : c];  // next plane's b
+                                // Jet's residual operand order (src/jet/fdm_linear_system3.cpp:107-115)
+                                TX rv = bc - (TX)p.row.c * p.v[0][0];
+                                rv -= h.l ? (TX)p.rl * p.v[0][1] : TX(0);
+                                rv -= h.r ? (TX)p.row.r * p.v[0][2] : TX(0);
+                                rv -= h.d ? (TX)p.ud * p.v[0][3] : TX(0);
+                                rv -= h.u ? (TX)p.row.u * p.v[0][4] : TX(0);
+                                rv -= (TX)fm * xm;
+                                rv -= (TX)p.row.f * xp;
+                                r[c] = (TR)rv;
+                                if (DOT) acc += (double)rv * (double)rv;
+                            });
     }
     if (DOT) block_partial<NT>(acc, partial);
 }
@@ -371,45 +430,6 @@ __global__ __launch_bounds__(NT) void k_correct(const TX* __restrict__ coarse,
 #pragma unroll
             for (int x = 0; x < 2; ++x) v += iW[x] * jW[y] * kW[z] * C(iI[x], jI[y], kI[z]);
     fine[f] = v;
-}
-
-// Coarsest level: symmetric red-black sweeps by one block, whole level
-// staged in LDS (A rows, b, x), x starts at zero.  fwd forward (red, black)
-// sweeps, then bwd backward (black, red) sweeps.
-template <typename TA, typename TX, int MAXN>
-__global__ __launch_bounds__(1024) void k_coarsest_lds(const Row<TA>* __restrict__ A,
-                                                       const TX* __restrict__ b,
-                                                       TX* __restrict__ x, Dims d, int fwd,
-                                                       int bwd, TX omega,
-                                                       const Ctl* __restrict__ ctl) {
-#pragma clang fp contract(off)
-    if (ctl && ctl->done) return;
-    __shared__ Row<TA> sA[MAXN];
-    __shared__ TX sb[MAXN];
-    __shared__ TX sx[MAXN];
-    const int n = d.nx * d.ny * d.nz;
-    for (int c = threadIdx.x; c < n; c += blockDim.x) {
-        sA[c] = A[c];
-        sb[c] = b[c];
-        sx[c] = 0;
-    }
-    __syncthreads();
-    const int total = 2 * (fwd + bwd);
-    for (int h = 0; h < total; ++h) {
-        const int sweep = h / 2;
-        const int color = sweep < fwd ? (h & 1) : 1 - (h & 1);
-        for (int c = threadIdx.x; c < n; c += blockDim.x) {
-            const int i = c % d.nx;
-            const int j = (c / d.nx) % d.ny;
-            const int k = c / (d.nx * d.ny);
-            if (((i + j + k) & 1) != color) continue;
-            const Row<TA> row = sA[c];
-            const TX s = offdiag<TA, TX, TX>(sA, sx, row, c, i, j, k, d);
-            sx[c] = (TX(1) - omega) * sx[c] + omega * (sb[c] - s) / (TX)row.c;
-        }
-        __syncthreads();
-    }
-    for (int c = threadIdx.x; c < n; c += blockDim.x) x[c] = sx[c];
 }
 
 // ------------------------------------------------------------- finalize

@@ -16,6 +16,8 @@
 #include "jg_internal.h"
 #include "jg_kernels.h"
 #include "jg_sweep.h"
+#include "jg_bottom.h"
+#include "jg_transfer.h"
 
 #include <cstdlib>
 
@@ -36,6 +38,15 @@ int sweep_zc_env() {
     return env;
 }
 
+// JG_TUNE_LOG=1 prints every autotuner timing to stderr.
+bool tune_log() {
+    static const bool on = [] {
+        const char* e = std::getenv("JG_TUNE_LOG");
+        return e && *e && *e != '0';
+    }();
+    return on;
+}
+
 int sweep_zc(const Dims& d, int tuned) {
     if (sweep_zc_env() > 0) return std::min(sweep_zc_env(), d.nz);
     if (tuned > 0) return std::min(tuned, d.nz);
@@ -49,19 +60,18 @@ int sweep_zc(const Dims& d, int tuned) {
 // One fused red-black sweep (see jg_sweep.h).  first: colour updated first
 // (0 = red, forward; 1 = black, backward).  xin == nullptr: x starts at zero.
 // coarse != nullptr: add the prolongation of coarse to x on load.
-// p_bx/p_bb != nullptr: emit b.x and b.b partials; returns the block count.
-template <typename T>
-int launch_sweep(jg_solver* s, const Row<T>* A, const T* b, const T* xin, T* xout, const Dims& d,
-                 int first, const T* coarse, double* p_bx, double* p_bb, T omega, const Ctl* ctl,
-                 int tuned_zc = 0) {
-    const int zc = sweep_zc(d, tuned_zc);
-    dim3 block(kSwLX, kSwTY, 1);
-    dim3 grid((d.nx + kSwIX - 1) / kSwIX, (d.ny + kSwIY - 1) / kSwIY, (d.nz + zc - 1) / zc);
-    const int code = (first ? 8 : 0) | (xin ? 0 : 4) | (coarse ? 2 : 0) | (p_bx ? 1 : 0);
-#define JG_SWEEP_CASE(C, F, Z, K, D)                                                          \
-    case C:                                                                                   \
-        k_rb_sweep<T, F, Z, K, D><<<grid, block, 0, s->stream>>>(A, b, xin, xout, d, zc, omega, \
-                                                                 coarse, p_bx, p_bb, ctl);    \
+// p_bx/p_bb != nullptr: emit b.x and b.b partials.  shape: index into
+// kSweepShapes.  Returns the block count.
+template <typename T, int RP, int PD>
+void launch_sweep_shape(jg_solver* s, int code, dim3 grid, const Row<T>* A, const T* b,
+                        const T* xin, T* xout, const Dims& d, int zc, T omega, const T* coarse,
+                        double* p_bx, double* p_bb, const Ctl* ctl) {
+    const dim3 block(kSwLX, kSwLY / RP, 1);
+#define JG_SWEEP_CASE(C, F, Z, K, D)                                                            \
+    case C:                                                                                     \
+        k_rb_sweep<T, F, Z, K, D, RP, PD><<<grid, block, 0, s->stream>>>(A, b, xin, xout, d, zc, \
+                                                                       omega, coarse, p_bx,    \
+                                                                       p_bb, ctl);             \
         break;
     switch (code) {
         JG_SWEEP_CASE(0, 0, false, false, false)
@@ -82,7 +92,42 @@ int launch_sweep(jg_solver* s, const Row<T>* A, const T* b, const T* xin, T* xou
         JG_SWEEP_CASE(15, 1, true, true, true)
     }
 #undef JG_SWEEP_CASE
+}
+
+template <typename T>
+int launch_sweep(jg_solver* s, const Row<T>* A, const T* b, const T* xin, T* xout, const Dims& d,
+                 int first, const T* coarse, double* p_bx, double* p_bb, T omega, const Ctl* ctl,
+                 int tuned_zc = 0, int shape = 0) {
+    const int zc = sweep_zc(d, tuned_zc);
+    dim3 grid((d.nx + kSwIX - 1) / kSwIX, (d.ny + kSwIY - 1) / kSwIY, (d.nz + zc - 1) / zc);
+    const int code = (first ? 8 : 0) | (xin ? 0 : 4) | (coarse ? 2 : 0) | (p_bx ? 1 : 0);
+    switch (shape) {
+        case 1:
+            launch_sweep_shape<T, 2, 2>(s, code, grid, A, b, xin, xout, d, zc, omega, coarse, p_bx,
+                                        p_bb, ctl);
+            break;
+        case 2:
+            launch_sweep_shape<T, 1, 1>(s, code, grid, A, b, xin, xout, d, zc, omega, coarse, p_bx,
+                                        p_bb, ctl);
+            break;
+        case 3:
+            launch_sweep_shape<T, 1, 2>(s, code, grid, A, b, xin, xout, d, zc, omega, coarse, p_bx,
+                                        p_bb, ctl);
+            break;
+        default:
+            launch_sweep_shape<T, 2, 1>(s, code, grid, A, b, xin, xout, d, zc, omega, coarse, p_bx,
+                                        p_bb, ctl);
+    }
     return (int)(grid.x * grid.y * grid.z);
+}
+
+// Coarse planes per block of the fused residual-restriction: enough blocks
+// to fill the chip; each block recomputes one fine plane per chunk.
+int rr_zc(const Dims& cd) {
+    const long tiles = (long)((cd.nx + kRrCX - 1) / kRrCX) * ((cd.ny + kRrCY - 1) / kRrCY);
+    long zc = (long)cd.nz * tiles / 1024;
+    zc = std::max<long>(2, std::min<long>(32, zc));
+    return (int)std::min<long>(zc, cd.nz);
 }
 
 template <typename T>
@@ -106,28 +151,51 @@ int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, 
     auto B = [&](size_t l) -> const T* { return l == 0 ? r0 : Prec<T>::b(s->lv[l]); };
     auto X = [&](size_t l) -> T* { return l == 0 ? z0 : Prec<T>::x(s->lv[l]); };
     std::vector<const T*> pre_out(L, nullptr);
+    // Levels from `bot` down run in one workgroup (k_bottom).
+    size_t bot = L;
+    for (size_t l = 0; l < L; ++l)
+        if (s->lv[l].d.n() <= BotCfg<T>::MaxCells) {
+            bot = l;
+            break;
+        }
+    const int fwd = (int)std::max(1u, (c.coarsest + 1) / 2);
+    size_t top = L;  // first level of the upward pass
     for (size_t l = 0; l < L; ++l) {
         Level& lv = s->lv[l];
         const Row<T>* A = Prec<T>::A(lv);
         const Dims& d = lv.d;
         T* Y = Prec<T>::y(lv);
+        if (l == bot) {
+            if (L - l > (size_t)kBotMaxLevels) return JG_ERR_UNSUPPORTED;
+            BotLevels<T> bl;
+            bl.L = (int)(L - l);
+            bl.d[0] = d;
+            for (size_t m = l; m < L; ++m) bl.A[m - l] = Prec<T>::A(s->lv[m]);
+            const int post_top = (int)(l == 0 ? c.post0 : c.post);
+            if (d.nx % 2 == 0)
+                k_bottom<T, true><<<1, kBotThreads, 0, s->stream>>>(bl, B(l), X(l), (int)c.pre,
+                                                                    post_top, (int)c.post, fwd,
+                                                                    c.omega, ctl);
+            else
+                k_bottom<T, false><<<1, kBotThreads, 0, s->stream>>>(bl, B(l), X(l), (int)c.pre,
+                                                                     post_top, (int)c.post, fwd,
+                                                                     c.omega, ctl);
+            JG_TRY(check_launch(s, "bottom"));
+            top = l;
+            break;
+        }
         if (l + 1 == L) {
-            const int n = (int)d.n();
-            const int fwd = std::max(1u, (c.coarsest + 1) / 2);
-            if (n <= 1024) {
-                k_coarsest_lds<T, T, 1024><<<1, 1024, 0, s->stream>>>(A, B(l), X(l), d, fwd, fwd,
-                                                                      c.omega, ctl);
-            } else {
-                const int S = 2 * fwd;
-                const T* in = nullptr;
-                for (int k = 0; k < S; ++k) {
-                    T* out = ((S - 1 - k) % 2 == 0) ? X(l) : Y;
-                    launch_sweep<T>(s, A, B(l), in, out, d, k < fwd ? 0 : 1, nullptr, nullptr, nullptr,
-                                    c.omega, ctl, Prec<T>::zc(lv));
-                    in = out;
-                }
+            // coarsest level too large for one workgroup: symmetric fused sweeps
+            const int S = 2 * fwd;
+            const T* in = nullptr;
+            for (int k = 0; k < S; ++k) {
+                T* out = ((S - 1 - k) % 2 == 0) ? X(l) : Y;
+                launch_sweep<T>(s, A, B(l), in, out, d, k < fwd ? 0 : 1, nullptr, nullptr, nullptr,
+                                c.omega, ctl, Prec<T>::zc(lv), Prec<T>::sh(lv));
+                in = out;
             }
             JG_TRY(check_launch(s, "coarsest"));
+            top = l;
             break;
         }
         const unsigned post = l == 0 ? c.post0 : c.post;
@@ -136,25 +204,24 @@ int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, 
         const T* in = nullptr;
         for (unsigned k = 0; k < c.pre; ++k) {
             launch_sweep<T>(s, A, B(l), in, buf(k), d, 0, nullptr, nullptr, nullptr, c.omega, ctl,
-                            Prec<T>::zc(lv));
+                            Prec<T>::zc(lv), Prec<T>::sh(lv));
             in = buf(k);
         }
         pre_out[l] = in;  // nullptr: x is still zero
         const Dims& cd = s->lv[l + 1].d;
         Geo gc = geo_planes(cd);
         if (in) {
-            Geo g = geo_stencil(d);
-            k_residual<T, T, T, false, false><<<g.grid, g.block, 0, s->stream>>>(
-                A, in, B(l), Prec<T>::t(lv), d, g.zc, nullptr, ctl);
-            k_restrict<T><<<gc.grid, gc.block, 0, s->stream>>>(Prec<T>::t(lv),
-                                                               Prec<T>::b(s->lv[l + 1]), cd, ctl);
+            const int zc = rr_zc(cd);
+            dim3 grid((cd.nx + kRrCX - 1) / kRrCX, (cd.ny + kRrCY - 1) / kRrCY, (cd.nz + zc - 1) / zc);
+            k_residual_restrict<T><<<grid, dim3(kRrLX, kRrTY, 1), 0, s->stream>>>(
+                A, in, B(l), Prec<T>::b(s->lv[l + 1]), d, cd, zc, ctl);
         } else {
             k_restrict<T><<<gc.grid, gc.block, 0, s->stream>>>(B(l), Prec<T>::b(s->lv[l + 1]), cd,
                                                                ctl);
         }
         JG_TRY(check_launch(s, "vcycle down"));
     }
-    for (size_t l = L - 1; l-- > 0;) {
+    for (size_t l = top; l-- > 0;) {
         Level& lv = s->lv[l];
         const Row<T>* A = Prec<T>::A(lv);
         const Dims& d = lv.d;
@@ -175,7 +242,7 @@ int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, 
                 const int nb = launch_sweep<T>(s, A, B(l), in, buf(k), d, 1, k == c.pre ? coarse : nullptr,
                                                last0 ? s->slot(0) : nullptr,
                                                last0 ? s->slot(1) : nullptr, c.omega, ctl,
-                                               Prec<T>::zc(lv));
+                                               Prec<T>::zc(lv), Prec<T>::sh(lv));
                 if (last0) *dot_blocks = nb;
                 in = buf(k);
             }
@@ -194,34 +261,42 @@ int autotune_sweeps(jg_solver* s) {
     for (auto& lv : s->lv) {
         if (Prec<T>::zc(lv) > 0) continue;
         const Dims& d = lv.d;
-        if (d.n() < 32 * 32 * 32) {
+        if (d.n() <= BotCfg<T>::MaxCells) {  // handled by k_bottom
             Prec<T>::zc(lv) = sweep_zc(d, 0);
             continue;
         }
         const idx_t n = d.n();
         k_fill<T><<<nblocks_1d(n), NT, 0, s->stream>>>(Prec<T>::t(lv), T(1), n);
         k_fill<T><<<nblocks_1d(n), NT, 0, s->stream>>>(Prec<T>::y(lv), T(0.5), n);
-        const int cands[] = {8, 12, 16, 24, 32, 48, 64};
+        const int cands[] = {2, 4, 8, 12, 16, 24, 32, 48, 64};
         float best = 1e30f;
-        int best_zc = sweep_zc(d, 0);
-        for (int zc : cands) {
-            if (zc > d.nz) break;
-            auto run = [&]() {
-                launch_sweep<T>(s, Prec<T>::A(lv), Prec<T>::t(lv), Prec<T>::y(lv), Prec<T>::x(lv), d, 1,
-                                nullptr, nullptr, nullptr, T(1), nullptr, zc);
-            };
-            run();
-            JG_HIP(hipEventRecord(s->ev0, s->stream));
-            for (int r = 0; r < 3; ++r) run();
-            JG_HIP(hipEventRecord(s->ev1, s->stream));
-            JG_TRY(sync(s));
-            float ms = 0.f;
-            JG_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
-            if (ms < best) {
-                best = ms;
-                best_zc = zc;
+        int best_zc = sweep_zc(d, 0), best_sh = 0;
+        for (int sh = 0; sh < kNumSweepShapes; ++sh) {
+            for (int zc : cands) {
+                if (zc > d.nz) break;
+                auto run = [&]() {
+                    launch_sweep<T>(s, Prec<T>::A(lv), Prec<T>::t(lv), Prec<T>::y(lv), Prec<T>::x(lv),
+                                    d, 1, nullptr, nullptr, nullptr, T(1), nullptr, zc, sh);
+                };
+                run();
+                JG_HIP(hipEventRecord(s->ev0, s->stream));
+                for (int r = 0; r < 3; ++r) run();
+                JG_HIP(hipEventRecord(s->ev1, s->stream));
+                JG_TRY(sync(s));
+                float ms = 0.f;
+                JG_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+                if (tune_log())
+                    std::fprintf(stderr, "jetgpu tune: %dx%dx%d %s shape %d (rp %d pd %d) zc %d: %.1f us\n",
+                                 d.nx, d.ny, d.nz, sizeof(T) == 4 ? "f32" : "f64", sh,
+                                 kSweepShapes[sh].rp, kSweepShapes[sh].pd, zc, ms * 1e3f / 3);
+                if (ms < best) {
+                    best = ms;
+                    best_zc = zc;
+                    best_sh = sh;
+                }
             }
         }
+        Prec<T>::sh(lv) = best_sh;
         Prec<T>::zc(lv) = best_zc;
     }
     return check_launch(s, "autotune");
@@ -339,6 +414,35 @@ bool valid_dims(size_t nx, size_t ny, size_t nz) {
     const size_t lim = (size_t)1 << 30;
     return nx > 0 && ny > 0 && nz > 0 && nx < 65536 && ny < 65536 && nz < 65536 &&
            nx * ny * nz < lim * 2;
+}
+
+}  // namespace
+
+namespace {
+
+// Options of an MGPCG solve or V-cycle.  Every level of the caller's
+// hierarchy is used, as Jet's mgVCycle recurses over A.levels.size()
+// (include/jet/detail/mg-inl.h:26); max_levels only tells callers how deep to
+// build it.  Relaxation factor of the red-black smoother: the GPU option if
+// given, else the caller's Jet sorFactor (FdmMgSolver3's, default 1.5).  Any
+// 0 < omega < 2 keeps the symmetric V-cycle positive definite.
+int mg_options(jg_solver* s, const jg_mg_params* p, const jg_gpu_options* o, jg_gpu_options* opt,
+               double* omega) {
+    if (o)
+        *opt = *o;
+    else
+        jg_gpu_options_default(opt);
+    if (opt->check_every == 0) opt->check_every = 1;
+    if (opt->smoother != JG_SMOOTH_RBGS) {
+        s->err = "unsupported smoother";
+        return JG_ERR_UNSUPPORTED;
+    }
+    *omega = opt->omega > 0 ? opt->omega : p->sor;
+    if (!(*omega > 0.0 && *omega < 2.0)) {
+        s->err = "smoother relaxation factor must lie in (0, 2)";
+        return JG_ERR_INVALID_ARG;
+    }
+    return JG_OK;
 }
 
 }  // namespace
@@ -518,33 +622,42 @@ int jg_solver_cg(jg_solver* s, unsigned max_iter, double tol, jg_solve_info* inf
     return JG_OK;
 }
 
+int jg_mg_vcycle(jg_solver* s, const jg_mg_params* p, const jg_gpu_options* o, const double* r,
+                 double* z) {
+    if (!s || s->lv.empty() || !p || !r || !z || r == z) return JG_ERR_INVALID_ARG;
+    JG_HIP(hipSetDevice(s->device));
+    jg_gpu_options opt;
+    double omega = 0.0;
+    JG_TRY(mg_options(s, p, o, &opt, &omega));
+    const idx_t n = s->lv[0].d.n();
+    const int nb1 = nblocks_1d(n);
+    int nb = 0;
+    if (opt.precision == JG_MIXED) {
+        JG_TRY(ensure_f32(s));
+        JG_TRY(autotune_sweeps<float>(s));
+        MgCfg<float> mg{p->restrict_iter, p->final_iter, p->correct_iter, p->coarsest_iter,
+                        (float)omega};
+        k_convert<double, float><<<nb1, NT, 0, s->stream>>>(r, s->r32, n);
+        JG_TRY(vcycle<float>(s, s->r32, s->z32, mg, nullptr, &nb));
+        k_convert<float, double><<<nb1, NT, 0, s->stream>>>(s->z32, z, n);
+    } else {
+        JG_TRY(autotune_sweeps<double>(s));
+        MgCfg<double> mg{p->restrict_iter, p->final_iter, p->correct_iter, p->coarsest_iter, omega};
+        JG_TRY(vcycle<double>(s, r, z, mg, nullptr, &nb));
+    }
+    JG_TRY(check_launch(s, "vcycle"));
+    return sync(s);
+}
+
 int jg_solver_mgpcg(jg_solver* s, unsigned max_iter, const jg_mg_params* p,
                     const jg_gpu_options* o, jg_solve_info* info) {
     if (!s || s->lv.empty() || !p) return JG_ERR_INVALID_ARG;
     JG_HIP(hipSetDevice(s->device));
     jg_gpu_options opt;
-    if (o)
-        opt = *o;
-    else
-        jg_gpu_options_default(&opt);
-    if (opt.check_every == 0) opt.check_every = 1;
+    double omega = 0.0;
+    JG_TRY(mg_options(s, p, o, &opt, &omega));
     Level& l0 = s->lv[0];
     const idx_t n = l0.d.n();
-    // Every level of the caller's hierarchy is used, as Jet's mgVCycle recurses
-    // over A.levels.size() (include/jet/detail/mg-inl.h:26); max_levels only
-    // tells callers how deep to build it.
-    // Relaxation factor of the red-black smoother: the GPU option if given,
-    // else the caller's Jet sorFactor (FdmMgSolver3's, default 1.5).  Any
-    // 0 < omega < 2 keeps the symmetric V-cycle positive definite.
-    if (opt.smoother != JG_SMOOTH_RBGS) {
-        s->err = "unsupported smoother";
-        return JG_ERR_UNSUPPORTED;
-    }
-    double omega = opt.omega > 0 ? opt.omega : p->sor;
-    if (!(omega > 0.0 && omega < 2.0)) {
-        s->err = "smoother relaxation factor must lie in (0, 2)";
-        return JG_ERR_INVALID_ARG;
-    }
     int status = JG_OK;
     unsigned iters = 0, refinements = 0;
     double last_res = 0.0;
@@ -703,7 +816,7 @@ int time_kernel_t(jg_solver* s, const Vecs<T>& v, int kernel, unsigned reps, dou
                 break;
             case JG_KERNEL_SWEEP:
                 launch_sweep<T>(s, A, v.b, v.r, v.x, d, 1, nullptr, nullptr, nullptr, T(1), s->ctl,
-                                Prec<T>::zc(l0));
+                                Prec<T>::zc(l0), Prec<T>::sh(l0));
                 break;
             default:
                 k_convert<T, T><<<nb1, NT, 0, s->stream>>>(v.r, v.q, n);

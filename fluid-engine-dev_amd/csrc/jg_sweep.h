@@ -1,18 +1,19 @@
 // jg_sweep.h -- fused red-black sweep: both colours of one Gauss-Seidel/SOR
 // sweep in a single pass over HBM.
 //
-// A 256-thread block owns a 64 x 16 tile of (i, j) positions (60 x 12
+// A block owns a 64 x 16 tile of (i, j) positions (60 x 12
 // interior plus a 2-cell halo that is recomputed, not exchanged) and marches
 // through a chunk of z planes.  x lives in a 4-plane LDS ring; the +x / +y
-// couplings (A.r, A.u) of the planes in flight live in LDS rings as well
-// (they are the -x / -y couplings of the neighbours); A.c, A.f and b stay in
-// registers.  At step t plane t is loaded, the first colour is updated on
-// plane t-1 (interior + 1-cell ring, which needs only original values of the
-// other colour) and the second colour on plane t-2 (interior), whose values
-// are then final and written back.  Each colour update is Jet's
-// relaxRedBlack expression (src/jet/fdm_gauss_seidel_solver3.cpp:163-190) in
-// Jet's operand order with contraction off, so one forward sweep is
-// bit-identical to Jet's relaxRedBlack.
+// couplings (A.r, A.u) of the two planes being relaxed live in 2-slot LDS
+// rings (they are the -x / -y couplings of the neighbours); A.c, A.f and b
+// stay in registers.  At step t plane t is committed from the registers it
+// was prefetched into (and plane t+1 is prefetched), the first colour is
+// updated on plane t-1 (interior + 1-cell ring, which needs only original
+// values of the other colour) and the second colour on plane t-2 (interior),
+// whose values are then final and written back.  Each colour update is
+// Jet's relaxRedBlack expression (src/jet/fdm_gauss_seidel_solver3.cpp:
+// 163-190) in Jet's operand order with contraction off, so one forward sweep
+// is bit-identical to Jet's relaxRedBlack.
 //
 // The sweep reads xin and writes xout (xin != xout): tiles overlap, so a
 // block must never overwrite values a neighbouring block may still load as
@@ -20,22 +21,26 @@
 //
 // Options: ZERO treats the incoming x as zero (no read of xin); CORRECT adds
 // the trilinear prolongation of a coarse correction to x as it is loaded
-// (FdmMgUtils3::correct, src/jet/fdm_mg_linear_system3.cpp:100-179); DOT
-// accumulates b.x and b.b of the written x (r.z and r.r of PCG when b = r,
-// x = z) into two partial slots.
+// (the operator of FdmMgUtils3::correct, src/jet/fdm_mg_linear_system3.cpp:
+// 100-179, evaluated separably: the block interpolates each coarse plane pair
+// in z into an LDS tile, then every position interpolates that tile in x and
+// y); DOT accumulates b.x and b.b of the written x (r.z and r.r of PCG when
+// b = r, x = z) into two partial slots.
 #pragma once
 
 #include "jg_device.h"
 
 namespace jg {
 
-constexpr int kSwLX = 64, kSwTY = 8, kSwRows = 2, kSwLY = kSwTY * kSwRows, kSwH = 2;
+constexpr int kSwLX = 64, kSwLY = 16, kSwH = 2;  // tile of positions, halo
 constexpr int kSwIX = kSwLX - 2 * kSwH;  // 60 interior columns
 constexpr int kSwIY = kSwLY - 2 * kSwH;  // 12 interior rows
-constexpr int kSwThreads = kSwLX * kSwTY;
+// Coarse tile behind one block's positions: columns gx0/2 - 1 .. gx0/2 + 32.
+constexpr int kCzX = kSwLX / 2 + 2, kCzY = kSwLY / 2 + 2;
+static_assert(kCzX * kCzY <= kSwLX * kSwLY / 2, "one coarse entry per thread");
 
 // Prolongation value at fine (i, j, k) (clamped trilinear 1/4-3/4, with the
-// k clamp bounded by min(ny, nz) like k_correct).
+// k clamp bounded by min(ny, nz) like k_correct), in Jet's accumulation order.
 template <typename T>
 __device__ __forceinline__ T prolong_at(const T* __restrict__ coarse, int i, int j, int k,
                                         const Dims& fd) {
@@ -62,171 +67,283 @@ __device__ __forceinline__ T prolong_at(const T* __restrict__ coarse, int i, int
     return v;
 }
 
-template <typename T, int FIRST, bool ZERO, bool CORRECT, bool DOT>
-__global__ __launch_bounds__(kSwThreads) void k_rb_sweep(const Row<T>* __restrict__ A,
-                                                         const T* __restrict__ b,
-                                                         const T* __restrict__ xin,
-                                                         T* __restrict__ xout, Dims d, int zc,
-                                                         T omega, const T* __restrict__ coarse,
-                                                         double* __restrict__ p_bx,
-                                                         double* __restrict__ p_bb,
-                                                         const Ctl* __restrict__ ctl) {
+// z taps of the prolongation for fine plane t (clamped into the domain).
+template <typename T>
+__device__ __forceinline__ void prolong_z(int t, const Dims& fd, int& k0, int& k1, T& w0, T& w1) {
+    const int tc = min(max(t, 0), fd.nz - 1);
+    const int ck = tc / 2;
+    if ((tc & 1) == 0) {
+        k0 = tc > 1 ? ck - 1 : ck;
+        k1 = ck;
+        w0 = T(0.25);
+        w1 = T(0.75);
+    } else {
+        k0 = ck;
+        k1 = tc + 1 < min(fd.ny, fd.nz) ? ck + 1 : ck;
+        w0 = T(0.75);
+        w1 = T(0.25);
+    }
+}
+
+// Launch shapes of the fused sweep: RP tile rows per thread (block of
+// 64 x 16/RP threads) and PD planes of loads in flight ahead of the plane
+// being committed.  The autotuner picks one per level and precision.
+struct SweepShape {
+    int rp, pd;
+};
+constexpr SweepShape kSweepShapes[] = {{2, 1}, {2, 2}, {1, 1}, {1, 2}};
+constexpr int kNumSweepShapes = 4;
+
+// Occupancy each shape is compiled for (waves per SIMD): as many blocks per
+// CU as the VGPR budget allows without spilling the common variants.
+constexpr int sweep_waves_per_eu(int tsize, int rp, int pd) {
+    return rp == 2 ? (pd == 1 ? (tsize == 4 ? 6 : 4) : (tsize == 4 ? 5 : 2)) : (tsize == 4 ? 8 : 4);
+}
+
+template <typename T, int FIRST, bool ZERO, bool CORRECT, bool DOT, int RP, int PD>
+__global__ __launch_bounds__(kSwLX * kSwLY / RP, sweep_waves_per_eu(sizeof(T), RP, PD)) void k_rb_sweep(
+    const Row<T>* __restrict__ A, const T* __restrict__ b, const T* __restrict__ xin,
+    T* __restrict__ xout, Dims d, int zc, T omega, const T* __restrict__ coarse,
+    double* __restrict__ p_bx, double* __restrict__ p_bb, const Ctl* __restrict__ ctl) {
 #pragma clang fp contract(off)
+    static_assert(RP == 1 || RP == 2, "rows per thread");
+    static_assert(PD == 1 || PD == 2, "prefetch depth");
+    constexpr int TY = kSwLY / RP;
+    constexpr int NTH = kSwLX * TY;
     if (ctl && ctl->done) return;
-    __shared__ T sx[4][kSwLY][kSwLX];  // x, planes t .. t-3 (slot = plane & 3)
-    __shared__ T sr[3][kSwLY][kSwLX];  // A.r, planes t .. t-2 (rotating slots)
-    __shared__ T su[3][kSwLY][kSwLX];  // A.u
+    __shared__ __align__(16) T sx[4][kSwLY][kSwLX];  // x, planes t .. t-3 (slot = plane & 3)
+    __shared__ T sr[2][kSwLY][kSwLX];                // A.r, planes t-1, t-2 (slot = plane & 1)
+    __shared__ T su[2][kSwLY][kSwLX];                // A.u
+    __shared__ T sz[CORRECT ? kCzY : 1][CORRECT ? kCzX : 1];  // coarse plane pair, z-interpolated
 
     const int tx = threadIdx.x, ty = threadIdx.y;
     int bx, by, bz;
     xcd_tile(bx, by, bz);
-    const int gx = bx * kSwIX - kSwH + tx;
-    const int gy0 = by * kSwIY - kSwH + ty;
+    const int gx0 = bx * kSwIX - kSwH, gy0 = by * kSwIY - kSwH;  // both even
+    const int gx = gx0 + tx;
     const int k0 = bz * zc;
     const int k1 = min(d.nz, k0 + zc);
     const idx_t sxy = (idx_t)d.nx * d.ny;
     const bool xok = gx >= 0 && gx < d.nx;
 
-    // Each thread owns two positions, rows ly = ty and ty + kSwTY.  Per
-    // position: c and b of planes t, t-1, t-2, f of planes t .. t-3
-    // (committed) and the prefetched plane t+1.  Plain structs (no arrays)
-    // keep everything in registers.
+    // CORRECT: thread e < kCzX * kCzY stages coarse column (gx0/2 - 1 + e % kCzX,
+    // gy0/2 - 1 + e / kCzX), clamped to the coarse grid (the clamp is exactly
+    // the prolongation's edge rule).  Each position then reads the 2 x 2 entries
+    // around it: x taps (tx >> 1) + {0, 1} for odd tx, (tx >> 1) + {-1, 0} + 1 for even.
+    const int e = tx + kSwLX * ty;
+    const bool zact = CORRECT && e < kCzX * kCzY;
+    idx_t zoff = 0;
+    const idx_t csz = (idx_t)(d.nx / 2) * (d.ny / 2);
+    if (CORRECT) {
+        const int ci = min(max(gx0 / 2 - 1 + e % kCzX, 0), d.nx / 2 - 1);
+        const int cj = min(max(gy0 / 2 - 1 + e / kCzX, 0), d.ny / 2 - 1);
+        zoff = ci + (idx_t)(d.nx / 2) * cj;
+    }
+    const int ax0 = (tx >> 1) + (tx & 1);  // left x tap in the coarse tile
+    const T wx0 = (tx & 1) ? T(0.75) : T(0.25), wx1 = (tx & 1) ? T(0.25) : T(0.75);
+    T zc0 = 0, zc1 = 0;  // prefetched coarse pair of the next plane
+    auto zfetch = [&](int t) __attribute__((always_inline)) {
+        if (!zact) return;
+        int c0, c1;
+        T w0, w1;
+        prolong_z<T>(t, d, c0, c1, w0, w1);
+        zc0 = coarse[zoff + csz * c0];
+        zc1 = coarse[zoff + csz * c1];
+    };
+    auto zstore = [&](int t) __attribute__((always_inline)) {
+        if (!zact) return;
+        int c0, c1;
+        T w0, w1;
+        prolong_z<T>(t, d, c0, c1, w0, w1);
+        sz[e / kCzX][e % kCzX] = w0 * zc0 + w1 * zc1;
+    };
+
+    // Each thread owns RP positions, rows ly = ty + TY * r.  Per position: c
+    // and b of planes t, t-1, t-2, f of planes t .. t-3, the held r/u of
+    // plane t (written to LDS one step later) and PD prefetched planes.
+    struct Pre {
+        T c, f, r, u, b, x;
+        bool ok;
+    };
     struct Pos {
-        T c0, c1, c2, f0, f1, f2, f3, b0, b1, b2;
-        T pc, pf, pr, pu, pb, px;
+        T c0, c1, c2, f0, f1, f2, f3, b0, b1, b2, hr, hu;
+        Pre p[PD];
         int ly, gy;
     };
-    Pos P0, P1;
-    P0.ly = ty;
-    P1.ly = ty + kSwTY;
-    P0.gy = gy0;
-    P1.gy = gy0 + kSwTY;
-    auto init = [&](Pos& P) __attribute__((always_inline)) {
-        P.c0 = P.c1 = P.c2 = T(1);
-        P.f0 = P.f1 = P.f2 = P.f3 = T(0);
-        P.b0 = P.b1 = P.b2 = T(0);
+    Pos P[RP];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) sx[r][P.ly][tx] = T(0);
+    for (int r = 0; r < RP; ++r) {
+        P[r].ly = ty + TY * r;
+        P[r].gy = gy0 + ty + TY * r;
+        P[r].c0 = P[r].c1 = P[r].c2 = T(1);
+        P[r].f0 = P[r].f1 = P[r].f2 = P[r].f3 = T(0);
+        P[r].b0 = P[r].b1 = P[r].b2 = T(0);
+        P[r].hr = P[r].hu = T(0);
 #pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            sr[r][P.ly][tx] = T(0);
-            su[r][P.ly][tx] = T(0);
+        for (int s = 0; s < 4; ++s) sx[s][P[r].ly][tx] = T(0);
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            sr[s][P[r].ly][tx] = T(0);
+            su[s][P[r].ly][tx] = T(0);
         }
-    };
+    }
     // Branch-free: lanes outside the domain load cell 0 (always valid) and
     // select the neutral values (c = 1, everything else 0).
-    auto fetch = [&](Pos& P, int t) __attribute__((always_inline)) {
-        const bool ok = xok && t >= 0 && t < d.nz && P.gy >= 0 && P.gy < d.ny;
-        const idx_t c = ok ? gx + (idx_t)d.nx * P.gy + sxy * t : 0;
+    auto fetch = [&](Pre& q, int gy, int t) __attribute__((always_inline)) {
+        const bool ok = xok && t >= 0 && t < d.nz && gy >= 0 && gy < d.ny;
+        const idx_t c = ok ? gx + (idx_t)d.nx * gy + sxy * t : 0;
         const Row<T> row = A[c];
         const T bv = b[c];
-        T xv = ZERO ? T(0) : xin[c];
-        if (CORRECT) xv += prolong_at<T>(coarse, ok ? gx : 0, ok ? P.gy : 0, ok ? t : 0, d);
-        P.pc = ok ? row.c : T(1);
-        P.pf = ok ? row.f : T(0);
-        P.pr = ok ? row.r : T(0);
-        P.pu = ok ? row.u : T(0);
-        P.pb = ok ? bv : T(0);
-        P.px = ok ? xv : T(0);
+        const T xv = ZERO ? T(0) : xin[c];
+        q.ok = ok;
+        q.c = ok ? row.c : T(1);
+        q.f = ok ? row.f : T(0);
+        q.r = ok ? row.r : T(0);
+        q.u = ok ? row.u : T(0);
+        q.b = ok ? bv : T(0);
+        q.x = ok ? xv : T(0);
     };
-    auto commit = [&](Pos& P, int st, int s0) __attribute__((always_inline)) {
-        P.c2 = P.c1;
-        P.c1 = P.c0;
-        P.c0 = P.pc;
-        P.f3 = P.f2;
-        P.f2 = P.f1;
-        P.f1 = P.f0;
-        P.f0 = P.pf;
-        P.b2 = P.b1;
-        P.b1 = P.b0;
-        P.b0 = P.pb;
-        sx[st][P.ly][tx] = P.px;
-        sr[s0][P.ly][tx] = P.pr;
-        su[s0][P.ly][tx] = P.pu;
+    auto commit = [&](Pos& Q, const Pre& q, int t) __attribute__((always_inline)) {
+        Q.c2 = Q.c1;
+        Q.c1 = Q.c0;
+        Q.c0 = q.c;
+        Q.f3 = Q.f2;
+        Q.f2 = Q.f1;
+        Q.f1 = Q.f0;
+        Q.f0 = q.f;
+        Q.b2 = Q.b1;
+        Q.b1 = Q.b0;
+        Q.b0 = q.b;
+        T xv = q.x;
+        if (CORRECT) {
+            const int ay0 = (Q.ly >> 1) + (Q.ly & 1);
+            const T wy0 = (Q.ly & 1) ? T(0.75) : T(0.25), wy1 = (Q.ly & 1) ? T(0.25) : T(0.75);
+            const T lo = wx0 * sz[ay0][ax0] + wx1 * sz[ay0][ax0 + 1];
+            const T hi = wx0 * sz[ay0 + 1][ax0] + wx1 * sz[ay0 + 1][ax0 + 1];
+            const T corr = wy0 * lo + wy1 * hi;
+            xv += q.ok ? corr : T(0);
+        }
+        sx[t & 3][Q.ly][tx] = xv;
+        sr[(t - 1) & 1][Q.ly][tx] = Q.hr;
+        su[(t - 1) & 1][Q.ly][tx] = Q.hu;
+        Q.hr = q.r;
+        Q.hu = q.u;
     };
-    // First colour on plane p (interior + 1-cell ring); r/u slot s1.
-    auto first_colour = [&](Pos& P, int p, int s1) __attribute__((always_inline)) {
-        const int ly = P.ly;
+    // First colour on plane p (interior + 1-cell ring).
+    auto first_colour = [&](Pos& Q, int p) __attribute__((always_inline)) {
+        const int ly = Q.ly;
         const bool act = tx >= 1 && tx < kSwLX - 1 && ly >= 1 && ly < kSwLY - 1 && xok &&
-                         P.gy >= 0 && P.gy < d.ny && ((gx + P.gy + p) & 1) == FIRST;
+                         Q.gy >= 0 && Q.gy < d.ny && ((gx + Q.gy + p) & 1) == FIRST;
         if (!act) return;
-        const int sp = p & 3, sm = (p - 1) & 3, sn = (p + 1) & 3;
+        const int sp = p & 3, sm = (p - 1) & 3, sn = (p + 1) & 3, s1 = p & 1;
         if (ZERO && !CORRECT) {
             // x is zero everywhere before this colour: (1-w)0 + w(b - 0)/c
-            sx[sp][ly][tx] = omega * (P.b1 - T(0)) / P.c1;
+            sx[sp][ly][tx] = omega * (Q.b1 - T(0)) / Q.c1;
         } else {
             const T xo = sx[sp][ly][tx];
             T s = sr[s1][ly][tx - 1] * sx[sp][ly][tx - 1];
             s += sr[s1][ly][tx] * sx[sp][ly][tx + 1];
             s += su[s1][ly - 1][tx] * sx[sp][ly - 1][tx];
             s += su[s1][ly][tx] * sx[sp][ly + 1][tx];
-            s += P.f2 * sx[sm][ly][tx];
-            s += P.f1 * sx[sn][ly][tx];
-            sx[sp][ly][tx] = (T(1) - omega) * xo + omega * (P.b1 - s) / P.c1;
+            s += Q.f2 * sx[sm][ly][tx];
+            s += Q.f1 * sx[sn][ly][tx];
+            sx[sp][ly][tx] = (T(1) - omega) * xo + omega * (Q.b1 - s) / Q.c1;
         }
     };
     double acc = 0.0, acc2 = 0.0;
-    // Second colour on plane q (interior), write back; r/u slot s2.
-    auto second_colour = [&](Pos& P, int q, int s2) __attribute__((always_inline)) {
-        const int ly = P.ly;
+    // Second colour on plane q (interior), write back.
+    auto second_colour = [&](Pos& Q, int q) __attribute__((always_inline)) {
+        const int ly = Q.ly;
         const bool act = tx >= kSwH && tx < kSwLX - kSwH && ly >= kSwH && ly < kSwLY - kSwH &&
-                         xok && P.gy >= 0 && P.gy < d.ny;
+                         xok && Q.gy >= 0 && Q.gy < d.ny;
         if (!act) return;
-        const int sq = q & 3, sm = (q - 1) & 3, sn = (q + 1) & 3;
+        const int sq = q & 3, sm = (q - 1) & 3, sn = (q + 1) & 3, s2 = q & 1;
         T xv = sx[sq][ly][tx];
-        if (((gx + P.gy + q) & 1) != FIRST) {
+        if (((gx + Q.gy + q) & 1) != FIRST) {
             T s = sr[s2][ly][tx - 1] * sx[sq][ly][tx - 1];
             s += sr[s2][ly][tx] * sx[sq][ly][tx + 1];
             s += su[s2][ly - 1][tx] * sx[sq][ly - 1][tx];
             s += su[s2][ly][tx] * sx[sq][ly + 1][tx];
-            s += P.f3 * sx[sm][ly][tx];
-            s += P.f2 * sx[sn][ly][tx];
-            xv = (T(1) - omega) * xv + omega * (P.b2 - s) / P.c2;
+            s += Q.f3 * sx[sm][ly][tx];
+            s += Q.f2 * sx[sn][ly][tx];
+            xv = (T(1) - omega) * xv + omega * (Q.b2 - s) / Q.c2;
             sx[sq][ly][tx] = xv;
         }
-        xout[gx + (idx_t)d.nx * P.gy + sxy * q] = xv;
+        xout[gx + (idx_t)d.nx * Q.gy + sxy * q] = xv;
         if (DOT) {
-            acc += (double)P.b2 * (double)xv;
-            acc2 += (double)P.b2 * (double)P.b2;
+            acc += (double)Q.b2 * (double)xv;
+            acc2 += (double)Q.b2 * (double)Q.b2;
         }
     };
 
-    init(P0);
-    init(P1);
-    int s0 = 0, s1 = 2, s2 = 1;  // r/u slots of planes t, t-1, t-2
-    fetch(P0, k0 - 2);
-    fetch(P1, k0 - 2);
+    // Prologue: planes k0-2 .. k0-3+PD in flight, the coarse tile of k0-2.
+    if (CORRECT) zfetch(k0 - 2);
+#pragma unroll
+    for (int s = 0; s < PD; ++s)
+#pragma unroll
+        for (int r = 0; r < RP; ++r) fetch(P[r].p[s], P[r].gy, k0 - 2 + s);
+    if (CORRECT) zstore(k0 - 2);
     __syncthreads();
 
-    for (int t = k0 - 2; t <= k1 + 1; ++t) {
-        // ---- commit plane t (prefetched), then prefetch plane t+1
-        {
-            const int tmp = s2;
-            s2 = s1;
-            s1 = s0;
-            s0 = tmp;
+    // Step t: commit plane t from prefetch slot (t - k0 + 2) % PD and refill
+    // that slot with plane t + PD.  The loop is unrolled PD times so every
+    // slot index is static: no register moves wait on loads in flight.
+    for (int t0 = k0 - 2; t0 <= k1 + 1; t0 += PD) {
+#pragma unroll
+        for (int u = 0; u < PD; ++u) {
+            const int t = t0 + u;
+            if (t > k1 + 1) break;
+            // ---- commit plane t (prefetched), then prefetch plane t+PD
+#pragma unroll
+            for (int r = 0; r < RP; ++r) commit(P[r], P[r].p[u], t);
+            if (CORRECT && t + 1 <= k1 + 1) zfetch(t + 1);
+            if (t + PD <= k1 + 1) {
+#pragma unroll
+                for (int r = 0; r < RP; ++r) fetch(P[r].p[u], P[r].gy, t + PD);
+            }
+            __syncthreads();
+            // ---- first colour on plane t - 1
+            if (t - 1 >= 0 && t - 1 < d.nz) {
+#pragma unroll
+                for (int r = 0; r < RP; ++r) first_colour(P[r], t - 1);
+            }
+            __syncthreads();
+            // ---- second colour on plane t - 2, final values written back
+            if (t - 2 >= k0 && t - 2 < k1) {
+#pragma unroll
+                for (int r = 0; r < RP; ++r) second_colour(P[r], t - 2);
+            }
+            // the coarse tile of plane t+1 (read by the next commit)
+            if (CORRECT && t + 1 <= k1 + 1) zstore(t + 1);
+            __syncthreads();
         }
-        commit(P0, t & 3, s0);
-        commit(P1, t & 3, s0);
-        if (t + 1 <= k1 + 1) {
-            fetch(P0, t + 1);
-            fetch(P1, t + 1);
-        }
-        __syncthreads();
-        // ---- first colour on plane t - 1
-        if (t - 1 >= 0 && t - 1 < d.nz) {
-            first_colour(P0, t - 1, s1);
-            first_colour(P1, t - 1, s1);
-        }
-        __syncthreads();
-        // ---- second colour on plane t - 2, final values written back
-        if (t - 2 >= k0 && t - 2 < k1) {
-            second_colour(P0, t - 2, s2);
-            second_colour(P1, t - 2, s2);
-        }
-        __syncthreads();
     }
-    if (DOT) block_partial2<kSwThreads>(acc, acc2, p_bx, p_bb);
+    if (DOT) {
+        // Block partials of b.x and b.b; the x ring is free after the last barrier.
+        double* sh = reinterpret_cast<double*>(&sx[0][0][0]);
+        constexpr int W = NTH / kWave;
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            acc += __shfl_down(acc, o, kWave);
+            acc2 += __shfl_down(acc2, o, kWave);
+        }
+        if ((e & (kWave - 1)) == 0) {
+            sh[e / kWave] = acc;
+            sh[W + e / kWave] = acc2;
+        }
+        __syncthreads();
+        if (e == 0) {
+            double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                s0 += sh[w];
+                s1 += sh[W + w];
+            }
+            const idx_t blk = blockIdx.x + (idx_t)gridDim.x * (blockIdx.y + (idx_t)gridDim.y * blockIdx.z);
+            p_bx[blk] = s0;
+            p_bb[blk] = s1;
+        }
+    }
 }
 
 }  // namespace jg

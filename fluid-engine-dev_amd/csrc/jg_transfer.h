@@ -1,0 +1,144 @@
+// jg_transfer.h -- fused residual + restriction of the V-cycle's downward
+// pass: coarse b = restrict(b - A x) in one pass over the fine level, without
+// writing or re-reading the fine residual.
+//
+// A 512-thread block owns a 31 x 7 tile of coarse (I, J) columns and marches
+// through a chunk of coarse planes.  Restriction tap t of coarse index I is
+// fine index 2I - 1 + t (clamped at the grid edges), so the tile's taps cover
+// fine columns 2*I0 - 1 .. 2*I0 + 62 and rows 2*J0 - 1 .. 2*J0 + 14: a 64 x 16
+// fine tile, two rows per thread.  For every fine plane the block computes
+// the residual (FdmBlas3::residual's expression, src/jet/fdm_linear_system3.cpp:
+// 98-116, in Jet's operand order) into LDS; threads e < 217 then restrict that
+// plane in x and y (16 taps) for their coarse column and keep the result in a
+// 4-plane register window, from which every second plane the z taps give the
+// coarse value.  This is FdmMgUtils3::restrict's 4 x 4 x 4 kernel
+// (src/jet/fdm_mg_linear_system3.cpp:43-98) evaluated separably.  Tiles
+// overlap by one fine column/row on each side; each block recomputes one fine
+// plane at each end of its chunk.
+#pragma once
+
+#include "jg_device.h"
+#include "jg_kernels.h"
+
+namespace jg {
+
+constexpr int kRrCX = 31, kRrCY = 7;                     // coarse tile
+constexpr int kRrLX = 64, kRrTY = 8, kRrLY = 2 * kRrTY;  // fine tile, thread rows
+constexpr int kRrThreads = kRrLX * kRrTY;
+static_assert(2 * kRrCX + 2 == kRrLX && 2 * kRrCY + 2 == kRrLY, "tile taps");
+
+template <typename T>
+__global__ __launch_bounds__(kRrThreads, sizeof(T) == 4 ? 4 : 2) void k_residual_restrict(
+    const Row<T>* __restrict__ A, const T* __restrict__ x, const T* __restrict__ b,
+    T* __restrict__ cb, Dims d, Dims cd, int zc, const Ctl* __restrict__ ctl) {
+#pragma clang fp contract(off)
+    if (ctl && ctl->done) return;
+    __shared__ T sres[2][kRrLY][kRrLX];  // fine residual of plane p in buffer p & 1
+
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    int bx, by, bz;
+    xcd_tile(bx, by, bz);
+    const int I0 = bx * kRrCX, J0 = by * kRrCY;
+    const int fx0 = 2 * I0 - 1, fy0 = 2 * J0 - 1;  // fine origin of the tile
+    const int K0 = bz * zc, K1 = min(cd.nz, K0 + zc);
+    const idx_t sz = (idx_t)d.nx * d.ny;
+    const int gx = fx0 + tx;
+    const bool xok = gx >= 0 && gx < d.nx;
+    const T* const src[1] = {x};
+
+    // Two fine positions per thread (rows ty and ty + kRrTY of the tile).
+    struct Cell {
+        int ly;
+        bool ok;  // inside the fine grid
+        StCol h;
+        idx_t c0;  // index in plane 0
+        StPlane<T, T, 1> cur, nxt;
+        T bc, bn, xm, fm;
+    };
+    Cell C[2];
+    // Fine planes of this chunk: the taps of coarse planes K0 .. K1-1.
+    const int p0 = max(2 * K0 - 1, 0), p1 = min(2 * K1, d.nz - 1);
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        Cell& c = C[r];
+        c.ly = ty + r * kRrTY;
+        const int gy = fy0 + c.ly;
+        c.ok = xok && gy >= 0 && gy < d.ny;
+        c.h = c.ok ? StCol{gx > 0, gx + 1 < d.nx, gy > 0, gy + 1 < d.ny}
+                   : StCol{false, false, false, false};
+        c.c0 = c.ok ? gx + (idx_t)d.nx * gy : 0;  // outside: cell 0, result unused
+        const idx_t cc = c.c0 + (c.ok ? sz * p0 : 0);
+        st_load(c.cur, A, src, cc, cc, c.h, d.nx);
+        c.bc = b[cc];
+        c.xm = (c.ok && p0 > 0) ? x[cc - sz] : T(0);
+        c.fm = (c.ok && p0 > 0) ? A[cc - sz].f : T(0);
+    }
+
+    // Coarse column of threads e < kRrCX * kRrCY and its x / y taps in the
+    // tile (local index 2*ci + t is fine index 2I - 1 + t; clamped like Jet's).
+    const int e = tx + kRrLX * ty;
+    const int ci = e % kRrCX, cj = min(e / kRrCX, kRrCY - 1);
+    const int I = I0 + ci, J = J0 + cj;
+    const bool cact = e < kRrCX * kRrCY && I < cd.nx && J < cd.ny;
+    const int li = 2 * ci, lj = 2 * cj;
+    const int i0 = I > 0 ? li : li + 1, i3 = I + 1 < cd.nx ? li + 3 : li + 2;
+    const int j0 = J > 0 ? lj : lj + 1, j3 = J + 1 < cd.ny ? lj + 3 : lj + 2;
+    const T wa = T(0.125), wb = T(0.375);
+    T w0 = 0, w1 = 0, w2 = 0, w3 = 0;  // x-y restricted planes p-3 .. p
+
+    int K = K0;
+    for (int p = p0; p <= p1; ++p) {
+        const bool zp = p + 1 < d.nz;
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            Cell& c = C[r];
+            // plane p + 1: the full plane if it is ours, else only its centre
+            const idx_t cc = c.c0 + (c.ok ? sz * p : 0);
+            const idx_t cn = c.ok && zp ? cc + sz : cc;
+            st_load(c.nxt, A, src, p + 1 <= p1 ? cn : cc, cn, c.h, d.nx);
+            c.bn = b[p + 1 <= p1 ? cn : cc];
+            const T xp = (c.ok && zp) ? c.nxt.v[0][0] : T(0);
+            // Jet's residual operand order (src/jet/fdm_linear_system3.cpp:107-115)
+            const StPlane<T, T, 1>& q = c.cur;
+            T rv = c.bc - q.row.c * q.v[0][0];
+            rv -= c.h.l ? q.rl * q.v[0][1] : T(0);
+            rv -= c.h.r ? q.row.r * q.v[0][2] : T(0);
+            rv -= c.h.d ? q.ud * q.v[0][3] : T(0);
+            rv -= c.h.u ? q.row.u * q.v[0][4] : T(0);
+            rv -= c.fm * c.xm;
+            rv -= q.row.f * xp;
+            sres[p & 1][c.ly][tx] = c.ok ? rv : T(0);
+            c.xm = q.v[0][0];
+            c.fm = q.row.f;
+            c.cur = c.nxt;
+            c.bc = c.bn;
+        }
+        __syncthreads();
+        // x-y restriction of plane p for this thread's coarse column (the
+        // buffer is rewritten two planes later, after the next barrier).
+        if (e < kRrCX * kRrCY) {
+            const T(*R)[kRrLX] = sres[p & 1];
+            auto rowsum = [&](int y) {
+                return (wa * R[y][i0] + wb * R[y][li + 1]) + (wb * R[y][li + 2] + wa * R[y][i3]);
+            };
+            const T P = (wa * rowsum(j0) + wb * rowsum(lj + 1)) + (wb * rowsum(lj + 2) + wa * rowsum(j3));
+            w0 = w1;
+            w1 = w2;
+            w2 = w3;
+            w3 = P;
+        }
+        // coarse plane K once its last tap plane is in the window
+        if (K < K1 && p == min(2 * K + 2, d.nz - 1)) {
+            // z taps 2K-1 .. 2K+2 (clamped) as window slots q - (p - 3)
+            const int k0 = (K > 0 ? 2 * K - 1 : 2 * K) - (p - 3);
+            const int k1 = 2 * K - (p - 3), k2 = 2 * K + 1 - (p - 3);
+            const int k3 = (K + 1 < cd.nz ? 2 * K + 2 : 2 * K + 1) - (p - 3);
+            auto win = [&](int q) { return q == 0 ? w0 : (q == 1 ? w1 : (q == 2 ? w2 : w3)); };
+            const T v = (wa * win(k0) + wb * win(k1)) + (wb * win(k2) + wa * win(k3));
+            if (cact) cb[I + (idx_t)cd.nx * (J + (idx_t)cd.ny * K)] = v;
+            ++K;
+        }
+    }
+}
+
+}  // namespace jg

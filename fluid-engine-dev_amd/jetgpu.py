@@ -77,6 +77,7 @@ SIGNATURES = {
     "jg_solver_cg": (C.c_int, [_vp, _u, _d, C.POINTER(SolveInfo)]),
     "jg_solver_mgpcg": (C.c_int, [_vp, _u, C.POINTER(MgParams), C.POINTER(GpuOptions),
                                   C.POINTER(SolveInfo)]),
+    "jg_mg_vcycle": (C.c_int, [_vp, C.POINTER(MgParams), C.POINTER(GpuOptions), _vp, _vp]),
     "jg_cg_solve_host": (C.c_int, [_vp, _sz, _sz, _sz, _vp, _vp, _vp, _u, _d, C.POINTER(SolveInfo)]),
     "jg_mgpcg_solve_host": (C.c_int, [_vp, _sz, C.POINTER(_sz), C.POINTER(_sz), C.POINTER(_sz),
                                       C.POINTER(_vp), _vp, _vp, _u, C.POINTER(MgParams),
@@ -227,6 +228,12 @@ class Solver:
                                          C.byref(options) if options is not None else None,
                                          C.byref(info)), "mgpcg")
         return info
+
+    def mg_vcycle(self, params: MgParams, options: GpuOptions | None, r, z):
+        """z = M r (device vectors of the finest level's size)."""
+        self.check(lib().jg_mg_vcycle(self.h, C.byref(params),
+                                      C.byref(options) if options is not None else None,
+                                      _ptr(r), _ptr(z)), "mg_vcycle")
 
     def time_kernel(self, kernel, precision, reps=20):
         """(average launch ms, algorithmic bytes per launch) of a level-0 kernel."""

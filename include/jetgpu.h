@@ -130,6 +130,15 @@ int jg_solver_cg(jg_solver* s, unsigned max_iter, double tol,
 int jg_solver_mgpcg(jg_solver* s, unsigned max_iter, const jg_mg_params* p,
                     const jg_gpu_options* o, jg_solve_info* info);
 
+/* z = M r: one application of jg_solver_mgpcg's preconditioner -- the
+ * symmetric V-cycle over the solver's hierarchy -- to device vectors r and z
+ * (finest-level size, r != z), in opt->precision (JG_MIXED: FP32 V-cycle,
+ * r and z converted).  Replaces the mgVCycle call FdmMgpcgSolver3's
+ * preconditioner makes (include/jet/mg.h:101, include/jet/detail/mg-inl.h:
+ * 17-113, called at src/jet/fdm_mgpcg_solver3.cpp:24-29). */
+int jg_mg_vcycle(jg_solver* s, const jg_mg_params* p, const jg_gpu_options* o,
+                 const double* r, double* z);
+
 /* ------------------------------------------------ host-memory drop-ins */
 /* One-call forms of the two solves on host arrays: upload, solve, download
  * x.  rows[l] / b are host arrays in Jet's layout. */

@@ -501,11 +501,28 @@ static void clean_vcycle(const or_mg_system* s, const or_mg_params* p,
         for (unsigned it = 0; it < post; ++it)
             rb_sweep_ordered(s->A[l], b[l], p->sor, x[l], nx, ny, nz, 1);
     } else {
-        for (unsigned it = 0; it < p->coarsest_iter; ++it)
+        /* coarsest_iter sweeps in total (Jet's count), split into forward
+         * then backward halves to keep the cycle symmetric */
+        const unsigned half = p->coarsest_iter > 1 ? (p->coarsest_iter + 1) / 2 : 1;
+        for (unsigned it = 0; it < half; ++it)
             rb_sweep_ordered(s->A[l], b[l], p->sor, x[l], nx, ny, nz, 0);
-        for (unsigned it = 0; it < p->coarsest_iter; ++it)
+        for (unsigned it = 0; it < half; ++it)
             rb_sweep_ordered(s->A[l], b[l], p->sor, x[l], nx, ny, nz, 1);
     }
+}
+
+void or_mg_clean_vcycle(const or_mg_system* sys, const or_mg_params* p,
+                        const double* r, double* z) {
+    double** mx = alloc_levels(sys);
+    double** mb = alloc_levels(sys);
+    double** mbuf = alloc_levels(sys);
+    const size_t n = sys->nx[0] * sys->ny[0] * sys->nz[0];
+    or_copy(r, mb[0], n);
+    clean_vcycle(sys, p, 0, mx, mb, mbuf);
+    or_copy(mx[0], z, n);
+    free_levels(sys, mx);
+    free_levels(sys, mb);
+    free_levels(sys, mbuf);
 }
 
 int or_mgpcg_clean_solve(or_mg_system* sys, const or_mg_params* p,

@@ -112,6 +112,12 @@ int or_mgpcg_solve(or_mg_system* sys, const or_mg_params* p, unsigned max_iter,
 int or_mgpcg_clean_solve(or_mg_system* sys, const or_mg_params* p,
                          unsigned max_iter, double rel_tol,
                          unsigned* last_iter, double* true_rel_residual);
+/* One application z = M r of that symmetric V-cycle (the preconditioner
+ * alone): restrict_iter forward sweeps, correct_iter / final_iter backward
+ * sweeps, coarsest_iter sweeps on the coarsest level split into forward and
+ * backward halves. */
+void or_mg_clean_vcycle(const or_mg_system* sys, const or_mg_params* p,
+                        const double* r, double* z);
 
 /* ------------------------------------------------ Single-phase pressure */
 enum { OR_FLUID = 0, OR_AIR = 1, OR_BOUNDARY = 2 };

@@ -118,6 +118,7 @@ def _setup(lib):
                                    C.c_uint, C.POINTER(C.c_uint), _dp]
     lib.or_mgpcg_clean_solve.argtypes = [C.POINTER(MgSystemC), C.POINTER(MgParams), C.c_uint,
                                          C.c_double, C.POINTER(C.c_uint), _dp]
+    lib.or_mg_clean_vcycle.argtypes = [C.POINTER(MgSystemC), C.POINTER(MgParams), _dp, _dp]
     lib.or_levels_with_finest.restype = C.c_size_t
     lib.or_divergence.restype = C.c_double
     return lib
@@ -216,6 +217,14 @@ class Oracle:
         ok = self.lib.or_mgpcg_clean_solve(C.byref(s), C.byref(params), C.c_uint(max_iter),
                                            C.c_double(rel_tol), C.byref(it), C.byref(res))
         return bool(ok), it.value, res.value
+
+    def mg_clean_vcycle(self, sys: MgSystem, params: MgParams, r):
+        """z = M r: one application of the clean-mode symmetric V-cycle."""
+        s = sys.to_c()
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.empty_like(r)
+        self.lib.or_mg_clean_vcycle(C.byref(s), C.byref(params), _d(r), _d(z))
+        return z
 
     # ------------------------------------------------ single-phase build
     def sp_markers(self, bsdf, fsdf):

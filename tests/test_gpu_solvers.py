@@ -122,7 +122,11 @@ def test_mgpcg_scene_matches_oracle(solver, oracle, kind, n, levels, precision):
     x = solver.download_x()
     assert info.rel_residual <= 1e-8
     assert rel_err(x, p_ref) <= 1e-6
-    assert np.all(x[markers[0] != 0] == 0.0)
+    # Non-fluid cells are identity rows with b = 0: p = 0 there up to the
+    # solve tolerance (exactly 0 only when the smoother's SOR factor is 1,
+    # since the V-cycle's correction reaches them and a sweep scales them by
+    # 1 - omega).
+    assert np.max(np.abs(x[markers[0] != 0])) <= 1e-6 * np.max(np.abs(p_ref))
     assert info.iterations <= 60
 
 
