@@ -40,6 +40,9 @@ struct Level {
     double* y64 = nullptr;  // ping-pong partner of x for the fused sweeps
     float* y32 = nullptr;
     int8_t* markers = nullptr;  // pressure-system markers of this level
+    // Variational pressure data (allocated by the first jg_fr_build): fluid
+    // SDF at cell centres and face weights, float as Jet stores them.
+    float *fs = nullptr, *uw = nullptr, *vw = nullptr, *ww = nullptr;
     int zc64 = 0, zc32 = 0;     // tuned z-chunk of the fused sweep (0: untuned)
     int sh64 = 0, sh32 = 0;     // tuned launch shape of the fused sweep
 };

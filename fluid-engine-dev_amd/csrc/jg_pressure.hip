@@ -4,6 +4,7 @@
 
 #include "jg_internal.h"
 #include "jg_pressure.h"
+#include "jg_fractional.h"
 
 using namespace jg;
 
@@ -223,6 +224,85 @@ int jg_sp_apply_gradient(jg_solver* s, const double* u, const double* v, const d
 const signed char* jg_solver_markers(jg_solver* s, size_t level) {
     if (!s || level >= s->lv.size()) return nullptr;
     return (const signed char*)s->lv[level].markers;
+}
+
+int jg_fr_build(jg_solver* s, const double* u, const double* v, const double* w,
+                const double* boundary_sdf_vertex, const double* fluid_sdf, const double* bu,
+                const double* bv, const double* bw, double hx, double hy, double hz) {
+    if (!s || s->lv.empty() || !u || !v || !w || !boundary_sdf_vertex || !fluid_sdf ||
+        !(hx > 0) || !(hy > 0) || !(hz > 0))
+        return JG_ERR_INVALID_ARG;
+    JG_HIP(hipSetDevice(s->device));
+    for (auto& lv : s->lv) {
+        if (lv.fs) continue;
+        const Dims& d = lv.d;
+        JG_TRY(dalloc(s, &lv.fs, d.n()));
+        JG_TRY(dalloc(s, &lv.uw, (size_t)(d.nx + 1) * d.ny * d.nz));
+        JG_TRY(dalloc(s, &lv.vw, (size_t)d.nx * (d.ny + 1) * d.nz));
+        JG_TRY(dalloc(s, &lv.ww, (size_t)d.nx * d.ny * (d.nz + 1)));
+    }
+    const Dims& d0 = s->lv[0].d;
+    auto faces = [](const Dims& d, int a) {
+        return Dims{d.nx + (a == 0), d.ny + (a == 1), d.nz + (a == 2)};
+    };
+    k_fr_weights<0><<<nblocks_1d(faces(d0, 0).n()), 256, 0, s->stream>>>(boundary_sdf_vertex, d0,
+                                                                         s->lv[0].uw);
+    k_fr_weights<1><<<nblocks_1d(faces(d0, 1).n()), 256, 0, s->stream>>>(boundary_sdf_vertex, d0,
+                                                                         s->lv[0].vw);
+    k_fr_weights<2><<<nblocks_1d(faces(d0, 2).n()), 256, 0, s->stream>>>(boundary_sdf_vertex, d0,
+                                                                         s->lv[0].ww);
+    k_fr_sdf_to_float<<<nblocks_1d(d0.n()), 256, 0, s->stream>>>(fluid_sdf, s->lv[0].fs, d0.n());
+    for (size_t l = 1; l < s->lv.size(); ++l) {
+        const Dims& fd = s->lv[l - 1].d;
+        const Dims& cd = s->lv[l].d;
+        k_fr_restrict<<<nblocks_1d(cd.n()), 256, 0, s->stream>>>(s->lv[l - 1].fs, fd, s->lv[l].fs, cd);
+        float* fw[3] = {s->lv[l - 1].uw, s->lv[l - 1].vw, s->lv[l - 1].ww};
+        float* cw[3] = {s->lv[l].uw, s->lv[l].vw, s->lv[l].ww};
+        for (int a = 0; a < 3; ++a)
+            k_fr_restrict<<<nblocks_1d(faces(cd, a).n()), 256, 0, s->stream>>>(fw[a], faces(fd, a), cw[a],
+                                                                             faces(cd, a));
+    }
+    H3 h{hx, hy, hz};
+    for (size_t l = 0; l < s->lv.size(); ++l) {
+        Level& lv = s->lv[l];
+        const bool top = l == 0;
+        k_fr_build<<<nblocks_1d(lv.d.n()), 256, 0, s->stream>>>(
+            lv.fs, lv.uw, lv.vw, lv.ww, top ? u : nullptr, v, w, top ? bu : nullptr,
+            top ? bv : nullptr, top ? bw : nullptr, lv.d, h, lv.A64, top ? s->b : nullptr);
+        h.x *= 2.0;
+        h.y *= 2.0;
+        h.z *= 2.0;
+    }
+    s->f32_valid = false;
+    JG_TRY(check_launch(s, "fr_build"));
+    return sync(s);
+}
+
+int jg_fr_apply_gradient(jg_solver* s, const double* u, const double* v, const double* w,
+                         double* uo, double* vo, double* wo, double hx, double hy, double hz) {
+    if (!s || s->lv.empty() || !u || !v || !w || !uo || !vo || !wo) return JG_ERR_INVALID_ARG;
+    if (!s->lv[0].fs) {
+        s->err = "jg_fr_apply_gradient before jg_fr_build";
+        return JG_ERR_INVALID_ARG;
+    }
+    JG_HIP(hipSetDevice(s->device));
+    const Level& l0 = s->lv[0];
+    k_fr_gradient<<<nblocks_1d(l0.d.n()), 256, 0, s->stream>>>(l0.fs, l0.uw, l0.vw, l0.ww, s->x, u, v,
+                                                                w, uo, vo, wo, l0.d, H3{hx, hy, hz});
+    JG_TRY(check_launch(s, "fr_gradient"));
+    return sync(s);
+}
+
+const float* jg_fr_array(jg_solver* s, size_t level, int which) {
+    if (!s || level >= s->lv.size()) return nullptr;
+    const Level& lv = s->lv[level];
+    switch (which) {
+        case 0: return lv.uw;
+        case 1: return lv.vw;
+        case 2: return lv.ww;
+        case 3: return lv.fs;
+        default: return nullptr;
+    }
 }
 
 }  // extern "C"

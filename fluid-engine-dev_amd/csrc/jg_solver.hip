@@ -711,7 +711,10 @@ int jg_solver_mgpcg(jg_solver* s, unsigned max_iter, const jg_mg_params* p,
             }
             if (last_res <= goal || outer >= opt.max_refine || iters >= max_iter) break;
             Ctl h;
-            init_ctl(&h, max_iter - iters, 0.0, 1, opt.inner_rel_tol);
+            // The last refinement only needs to close the remaining gap to
+            // the goal (with a factor 2 margin for the FP32 inner residual).
+            const double inner = std::max(opt.inner_rel_tol, std::min(0.5, 0.5 * goal / last_res));
+            init_ctl(&h, max_iter - iters, 0.0, 1, inner);
             JG_TRY(upload_ctl(s, ictl, h));
             k_fill<float><<<nb1, NT, 0, s->stream>>>(s->e32, 0.f, n);
             JG_TRY(pcg_device<float>(s, l0.A32, s->rhs32, s->e32, s->r32, s->z32, s->d32A, s->d32B,

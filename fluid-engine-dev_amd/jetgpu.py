@@ -94,6 +94,9 @@ SIGNATURES = {
     "jg_sp_build": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d]),
     "jg_sp_apply_gradient": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d]),
     "jg_solver_markers": (_vp, [_vp, _sz]),
+    "jg_fr_build": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d]),
+    "jg_fr_apply_gradient": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d]),
+    "jg_fr_array": (_vp, [_vp, _sz, C.c_int]),
     "jg_scene_generate_host": (C.c_int, [C.POINTER(SceneDesc), _vp, _vp, _vp, _vp, _vp, _vp]),
     "jg_scene_generate_device": (C.c_int, [_vp, C.POINTER(SceneDesc), _vp, _vp, _vp, _vp, _vp, _vp]),
 }
@@ -250,6 +253,18 @@ class Solver:
     def sp_apply_gradient(self, u, v, w, uo, vo, wo, h):
         self.check(lib().jg_sp_apply_gradient(self.h, _ptr(u), _ptr(v), _ptr(w), _ptr(uo),
                                               _ptr(vo), _ptr(wo), h, h, h), "sp_apply_gradient")
+
+    def fr_build(self, u, v, w, vertex_sdf, fsdf, h, bu=None, bv=None, bw=None):
+        self.check(lib().jg_fr_build(self.h, _ptr(u), _ptr(v), _ptr(w), _ptr(vertex_sdf), _ptr(fsdf),
+                                     _ptr(bu), _ptr(bv), _ptr(bw), h, h, h), "fr_build")
+
+    def fr_apply_gradient(self, u, v, w, uo, vo, wo, h):
+        self.check(lib().jg_fr_apply_gradient(self.h, _ptr(u), _ptr(v), _ptr(w), _ptr(uo),
+                                              _ptr(vo), _ptr(wo), h, h, h), "fr_apply_gradient")
+
+    def fr_array(self, level, which):
+        """Device pointer of a level's face weights (which 0-2) or fluid SDF (3)."""
+        return lib().jg_fr_array(self.h, level, which)
 
     def generate_scene(self, desc, u=None, v=None, w=None, bsdf=None, fsdf=None, vertex=None):
         self.check(lib().jg_scene_generate_device(self.h, C.byref(desc), _ptr(u), _ptr(v), _ptr(w),

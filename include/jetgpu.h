@@ -224,6 +224,31 @@ int jg_sp_apply_gradient(jg_solver* s, const double* u, const double* v,
 /* Device pointer of the markers of a level (int8: 0 fluid, 1 air, 2 boundary). */
 const signed char* jg_solver_markers(jg_solver* s, size_t level);
 
+/* GridFractionalSinglePhasePressureSolver3::buildWeights + buildSystem
+ * (src/jet/grid_fractional_single_phase_pressure_solver3.cpp:520-713) on the
+ * solver's resident hierarchy.  Device inputs: face velocities u, v, w; the
+ * collider SDF at the (nx+1)*(ny+1)*(nz+1) grid vertices of the finest level
+ * (the corners buildWeights samples, :553-617); the fluid SDF at the finest
+ * cell centres (stored as float, as Jet does, :548-551); collider velocity
+ * components at the u / v / w face centres (bu, bv, bw; any may be NULL for
+ * a resting collider).  Face weights and fluid SDF on every level (coarse:
+ * float restriction, :29-115, :619-635), A on every level (spacing 2^l h),
+ * b on level 0; coarse right-hand sides are not built (as jg_sp_build). */
+int jg_fr_build(jg_solver* s, const double* u, const double* v, const double* w,
+                const double* boundary_sdf_vertex, const double* fluid_sdf,
+                const double* bu, const double* bv, const double* bw, double hx,
+                double hy, double hz);
+/* applyPressureGradient, variational form
+ * (src/jet/grid_fractional_single_phase_pressure_solver3.cpp:715-765) with the
+ * resident solution x and the level-0 weights / fluid SDF of jg_fr_build;
+ * device arrays; the output may alias the input. */
+int jg_fr_apply_gradient(jg_solver* s, const double* u, const double* v,
+                         const double* w, double* uo, double* vo, double* wo,
+                         double hx, double hy, double hz);
+/* Device pointers of a level's float arrays built by jg_fr_build: axis 0/1/2
+ * = u/v/w face weights, 3 = fluid SDF (NULL before the first build). */
+const float* jg_fr_array(jg_solver* s, size_t level, int which);
+
 /* --------------------------------------------------- synthetic scenes */
 /* Deterministic synthetic inputs (counter-based SplitMix64, jg_synth.h):
  *   JG_SCENE_CLOSED_SPHERE : solid sphere r = n/4 at the centre, all fluid,
