@@ -8,6 +8,7 @@
 // top level (BotCfg<T>::MC cells per thread) and of the coarsest level (one
 // cell per thread) in registers, the rows of the levels between staged in
 // LDS with precomputed cell coordinates, __syncthreads between phases.
+// Zero-centre rows are set to zero by the smoother, as in k_rb_sweep.
 // The sequence is exactly the multi-kernel V-cycle's: `pre` forward (red,
 // black) sweeps from zero, residual, Jet's restriction, recursion, Jet's
 // prolongation-correction, `post` backward (black, red) sweeps; on the
@@ -85,7 +86,7 @@ struct BotRegs {
             fl[m] = (hxm ? 1 : 0) | (hxp ? 2 : 0) | (hym ? 4 : 0) | (hyp ? 8 : 0) | (hzm ? 16 : 0) |
                     (hzp ? 32 : 0) | (((i + j + k) & 1) << 6) | (ok ? 128 : 0) | (swap << 8);
             const Row<T> row = A[cc];
-            wc[m] = ok ? omega / row.c : T(0);
+            wc[m] = (ok && row.c != T(0)) ? omega / row.c : T(0);  // 0: zero-centre row
             xp[m] = (ok && hxp) ? row.r : T(0);
             yp[m] = (ok && hyp) ? row.u : T(0);
             zp[m] = (ok && hzp) ? row.f : T(0);
@@ -120,7 +121,7 @@ struct BotRegs {
                 const int m = 2 * h + colour;
                 const int c = (fl[m] & 128) ? cell(m) : 0;  // 0: no cell (flags of cell 0)
                 cs[h] = c;
-                nv[h] = (T(1) - omega) * X[c] + wc[m] * (B[c] - offdiag(m, X, c));
+                nv[h] = wc[m] == T(0) ? T(0) : (T(1) - omega) * X[c] + wc[m] * (B[c] - offdiag(m, X, c));
             }
 #pragma unroll
             for (int h = 0; h < H; ++h)
@@ -131,7 +132,7 @@ struct BotRegs {
             for (int m = 0; m < MC; ++m) {
                 if ((fl[m] & 192) != want) continue;
                 const int c = cell(m);
-                X[c] = (T(1) - omega) * X[c] + wc[m] * (B[c] - offdiag(m, X, c));
+                X[c] = wc[m] == T(0) ? T(0) : (T(1) - omega) * X[c] + wc[m] * (B[c] - offdiag(m, X, c));
             }
         }
     }
@@ -168,7 +169,7 @@ __device__ __forceinline__ void bot_relax_lds(const Row<T>* R, const int* Q, T* 
         if (((i + j + k) & 1) != colour) continue;
         const Row<T> row = R[c];
         const T s = offdiag<T, T, T>(R, X, row, c, i, j, k, d);
-        X[c] = (T(1) - omega) * X[c] + omega * (B[c] - s) / row.c;
+        X[c] = row.c == T(0) ? T(0) : (T(1) - omega) * X[c] + omega * (B[c] - s) / row.c;
     }
 }
 

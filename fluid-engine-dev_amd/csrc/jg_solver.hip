@@ -304,64 +304,95 @@ int autotune_sweeps(jg_solver* s) {
 
 // --------------------------------------------------------------- PCG
 
+// Vectors and control block of one PCG solve.
+template <typename T>
+struct Pcg {
+    const Row<T>* A;
+    const T* b;
+    T *x, *r, *z, *dA, *dB, *q;
+    Dims d;
+    Ctl* ctl;
+    const MgCfg<T>* mg;  // nullptr: no preconditioner (z aliases r)
+};
+
+// z = M r and the r.z / r.r partials in slots 0 / 1 (from the V-cycle's last
+// sweep when it provides them, else from dot kernels); returns the partial
+// count, or minus a status.
+template <typename T>
+int pcg_precondition(jg_solver* s, const Pcg<T>& v, const Ctl* c) {
+    const idx_t n = v.d.n();
+    const int nb1 = nblocks_1d(n);
+    int nb = 0;
+    if (v.mg) {
+        const int st = vcycle<T>(s, v.r, v.z, *v.mg, c, &nb);
+        if (st != JG_OK) return -st;
+    }
+    if (nb == 0) {
+        k_dot<T, T><<<nb1, NT, 0, s->stream>>>(v.r, v.z, n, s->slot(0), c);
+        if (v.mg) k_dot<T, T><<<nb1, NT, 0, s->stream>>>(v.r, v.r, n, s->slot(1), c);
+        nb = nb1;
+    }
+    return nb;
+}
+
+// Start of Jet's pcg (include/jet/detail/cg-inl.h:15-50): r = b - A x, d = 0,
+// z = M r, sigma = r.z.  Launches only; the control block says whether to go on.
+template <typename T>
+int pcg_init(jg_solver* s, const Pcg<T>& v) {
+    const idx_t n = v.d.n();
+    const int nb1 = nblocks_1d(n);
+    Geo g = geo_stencil(v.d);
+    // r = b - A x  (x = 0, so r = b; keep the general form)
+    k_residual<T, T, T, false, false><<<g.grid, g.block, 0, s->stream>>>(v.A, v.x, v.b, v.r, v.d, g.zc,
+                                                                        nullptr, nullptr);
+    k_fill<T><<<nb1, NT, 0, s->stream>>>(v.dA, T(0), n);
+    k_fill<T><<<nb1, NT, 0, s->stream>>>(v.dB, T(0), n);
+    const int nb = pcg_precondition<T>(s, v, nullptr);
+    if (nb < 0) return -nb;
+    k_fin_init<<<1, NT, 0, s->stream>>>(v.ctl, s->slot(0), v.mg ? s->slot(1) : s->slot(0), nb);
+    return check_launch(s, "pcg init");
+}
+
+// One iteration of Jet's pcg loop (include/jet/detail/cg-inl.h:52-91) on the
+// device; every kernel is gated by the control block, so iterations can be
+// launched (or replayed from a graph) past convergence.  odd: the two
+// direction buffers swap roles every iteration.
+template <typename T>
+int pcg_iteration(jg_solver* s, const Pcg<T>& v, bool odd) {
+    const idx_t n = v.d.n();
+    const int nb1 = nblocks_1d(n);
+    Geo g = geo_stencil(v.d);
+    T* dold = odd ? v.dB : v.dA;
+    T* dnew = odd ? v.dA : v.dB;
+    k_dir_spmv<T, T><<<g.grid, g.block, 0, s->stream>>>(v.A, v.z, dold, dnew, v.q, v.d, g.zc,
+                                                        s->slot(2), v.ctl);
+    k_fin_alpha<<<1, NT, 0, s->stream>>>(v.ctl, s->slot(2), nblocks(g));
+    k_update_xr<T><<<nb1, NT, 0, s->stream>>>(v.x, v.r, dnew, v.q, n, v.ctl);
+    k_residual<T, T, T, false, true><<<g.grid, g.block, 0, s->stream>>>(v.A, v.x, v.b, v.r, v.d, g.zc,
+                                                                       nullptr, v.ctl);
+    const int nb = pcg_precondition<T>(s, v, v.ctl);
+    if (nb < 0) return -nb;
+    k_fin_sigma<<<1, NT, 0, s->stream>>>(v.ctl, s->slot(0), v.mg ? s->slot(1) : s->slot(0), nb);
+    return check_launch(s, "pcg iteration");
+}
+
 // Jet's pcg control flow (include/jet/detail/cg-inl.h:15-94) on the device.
 // Vectors of type T; the matrix of level 0 is A; M is either the identity
 // (z aliases r) or the V-cycle.  x must be zero on entry (Jet zeroes it).
+// The host only polls the stop flag, every check_every iterations.
 template <typename T>
-int pcg_device(jg_solver* s, const Row<T>* A, const T* b, T* x, T* r, T* z, T* dA, T* dB, T* q,
-               const Dims& d, Ctl* ctl, const MgCfg<T>* mg, unsigned check_every) {
-    const idx_t n = d.n();
-    const int nb1 = nblocks_1d(n);
-    Geo g = geo_stencil(d);
-    const int nbs = nblocks(g);
-    double* p_rz = s->slot(0);
-    double* p_rr = s->slot(1);
-    double* p_dq = s->slot(2);
-    // z = M r and the r.z / r.r partials (from the V-cycle's last sweep when
-    // it provides them, else from dot kernels); returns the partial count.
-    auto precondition = [&](const Ctl* c) -> int {
-        int nb = 0;
-        if (mg) {
-            int st = vcycle<T>(s, r, z, *mg, c, &nb);
-            if (st != JG_OK) return -st;
-        }
-        if (nb == 0) {
-            k_dot<T, T><<<nb1, NT, 0, s->stream>>>(r, z, n, p_rz, c);
-            if (mg) k_dot<T, T><<<nb1, NT, 0, s->stream>>>(r, r, n, p_rr, c);
-            nb = nb1;
-        }
-        return nb;
-    };
-    // r = b - A x  (x = 0, so r = b; keep the general form)
-    k_residual<T, T, T, false, false><<<g.grid, g.block, 0, s->stream>>>(A, x, b, r, d, g.zc,
-                                                                        nullptr, nullptr);
-    k_fill<T><<<nb1, NT, 0, s->stream>>>(dA, T(0), n);
-    k_fill<T><<<nb1, NT, 0, s->stream>>>(dB, T(0), n);
-    int nb = precondition(nullptr);
-    if (nb < 0) return -nb;
-    k_fin_init<<<1, NT, 0, s->stream>>>(ctl, p_rz, mg ? p_rr : p_rz, nb);
-    JG_TRY(check_launch(s, "pcg init"));
-    JG_HIP(hipMemcpyAsync(s->ctl_host, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
+int pcg_device(jg_solver* s, const Pcg<T>& v, unsigned check_every) {
+    JG_TRY(pcg_init<T>(s, v));
+    JG_HIP(hipMemcpyAsync(s->ctl_host, v.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
     JG_TRY(sync(s));
     if (s->ctl_host->done) return JG_OK;
-    T* dold = dA;
-    T* dnew = dB;
     unsigned launched = 0;
     const unsigned max_iter = s->ctl_host->max_iter;
     while (true) {
-        k_dir_spmv<T, T><<<g.grid, g.block, 0, s->stream>>>(A, z, dold, dnew, q, d, g.zc, p_dq, ctl);
-        k_fin_alpha<<<1, NT, 0, s->stream>>>(ctl, p_dq, nbs);
-        k_update_xr<T><<<nb1, NT, 0, s->stream>>>(x, r, dnew, q, n, ctl);
-        k_residual<T, T, T, false, true><<<g.grid, g.block, 0, s->stream>>>(A, x, b, r, d, g.zc,
-                                                                           nullptr, ctl);
-        nb = precondition(ctl);
-        if (nb < 0) return -nb;
-        k_fin_sigma<<<1, NT, 0, s->stream>>>(ctl, p_rz, mg ? p_rr : p_rz, nb);
-        JG_TRY(check_launch(s, "pcg iteration"));
-        std::swap(dold, dnew);
+        JG_TRY(pcg_iteration<T>(s, v, (launched & 1) != 0));
         ++launched;
         if (launched % check_every == 0 || launched >= max_iter) {
-            JG_HIP(hipMemcpyAsync(s->ctl_host, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
+            JG_HIP(hipMemcpyAsync(s->ctl_host, v.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
             JG_TRY(sync(s));
             if (s->ctl_host->done) break;
             if (launched >= max_iter + 1) break;  // safety: the device stops at max_iter
@@ -598,8 +629,9 @@ int jg_solver_cg(jg_solver* s, unsigned max_iter, double tol, jg_solve_info* inf
     JG_TRY(upload_ctl(s, s->ctl, h));
     k_fill<double><<<nblocks_1d(l0.d.n()), NT, 0, s->stream>>>(s->x, 0.0, l0.d.n());
     JG_HIP(hipEventRecord(s->ev0, s->stream));
-    JG_TRY(pcg_device<double>(s, l0.A64, s->b, s->x, s->r, s->r, s->dA, s->dB, s->q, l0.d, s->ctl,
-                              nullptr, 1));
+    JG_TRY(pcg_device<double>(s, Pcg<double>{l0.A64, s->b, s->x, s->r, s->r, s->dA, s->dB, s->q, l0.d,
+                                              s->ctl, nullptr},
+                              1));
     JG_HIP(hipEventRecord(s->ev1, s->stream));
     JG_HIP(hipMemcpyAsync(s->ctl_host, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
     JG_TRY(sync(s));
@@ -675,8 +707,9 @@ int jg_solver_mgpcg(jg_solver* s, unsigned max_iter, const jg_mg_params* p,
         Ctl h;
         init_ctl(&h, max_iter, p->max_tol, opt.stop_rule == JG_STOP_RELATIVE ? 1 : 0, opt.rel_tol);
         JG_TRY(upload_ctl(s, ctl, h));
-        JG_TRY(pcg_device<double>(s, l0.A64, s->b, s->x, s->r, s->z, s->dA, s->dB, s->q, l0.d, ctl,
-                                  &mg, opt.check_every));
+        JG_TRY(pcg_device<double>(s, Pcg<double>{l0.A64, s->b, s->x, s->r, s->z, s->dA, s->dB, s->q,
+                                                  l0.d, ctl, &mg},
+                                  opt.check_every));
         JG_HIP(hipMemcpyAsync(s->ctl_host, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
         JG_TRY(sync(s));
         iters = s->ctl_host->iter;
@@ -717,8 +750,9 @@ int jg_solver_mgpcg(jg_solver* s, unsigned max_iter, const jg_mg_params* p,
             init_ctl(&h, max_iter - iters, 0.0, 1, inner);
             JG_TRY(upload_ctl(s, ictl, h));
             k_fill<float><<<nb1, NT, 0, s->stream>>>(s->e32, 0.f, n);
-            JG_TRY(pcg_device<float>(s, l0.A32, s->rhs32, s->e32, s->r32, s->z32, s->d32A, s->d32B,
-                                     s->q32, l0.d, ictl, &mg, opt.check_every));
+            JG_TRY(pcg_device<float>(s, Pcg<float>{l0.A32, s->rhs32, s->e32, s->r32, s->z32, s->d32A,
+                                                   s->d32B, s->q32, l0.d, ictl, &mg},
+                                     opt.check_every));
             iters += s->ctl_host->iter;
             k_add_correction<<<nb1, NT, 0, s->stream>>>(s->x, s->e32, n);
             JG_TRY(check_launch(s, "refinement"));
@@ -975,6 +1009,438 @@ int jg_mg_correct(jg_solver* s, size_t fnx, size_t fny, size_t fnz, const double
     k_correct<double><<<g.grid, g.block, 0, s->stream>>>(coarse, fine, fd, nullptr);
     JG_TRY(check_launch(s, "correct"));
     return sync(s);
+}
+
+}  // extern "C"
+
+// ============================================================ batched solves
+//
+// Many independent systems of uneven sizes, each held by its own jg_solver
+// (its own hierarchy, buffers, control blocks and stream).  The PCG
+// iterations of all systems are captured once into one hipGraph: a fork
+// from the batch stream, one branch per system (K iterations on that
+// system's stream), a join.  Every kernel of an iteration is gated by its
+// system's control block, which is the per-system convergence mask: a
+// converged system's branch replays as no-ops.  The host replays the graph
+// and gathers all stop flags with one kernel and one copy per replay.
+
+namespace {
+
+constexpr int kBatchIters = 4;  // PCG iterations per graph replay (even: direction buffers realign)
+
+// done flag and iteration count of every system's control block
+__global__ void k_gather_ctl(Ctl* const* ctls, int n, int* out) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        out[2 * i] = ctls[i]->done | ctls[i]->nonfinite << 1;
+        out[2 * i + 1] = (int)ctls[i]->iter;
+    }
+}
+
+__global__ void k_gather_scal(double* const* scal, int idx, int n, double* out) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) out[i] = scal[i][idx];
+}
+
+}  // namespace
+
+struct jg_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr, ev0 = nullptr, ev1 = nullptr;
+    std::vector<jg_solver*> sys;
+    std::vector<hipEvent_t> join;
+    Ctl** dctl = nullptr;       // per system: ctl (FP64 solve), then ctl + 1 (mixed inner solve)
+    double** dscal = nullptr;   // per system: scal
+    int* dflags = nullptr;      // gathered (flags, iteration) pairs
+    double* dvals = nullptr;    // gathered scalars
+    int* hflags = nullptr;      // pinned
+    double* hvals = nullptr;    // pinned
+    struct Graph {
+        int precision;
+        unsigned pre, post0, post, coarsest;
+        double omega;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<Graph> graphs;
+    std::string err;
+
+    ~jg_batch() {
+        for (auto& g : graphs) {
+            (void)hipGraphExecDestroy(g.exec);
+            (void)hipGraphDestroy(g.graph);
+        }
+        for (auto* s : sys) jg_solver_destroy(s);
+        for (auto e : join) (void)hipEventDestroy(e);
+        for (void* p : {(void*)dctl, (void*)dscal, (void*)dflags, (void*)dvals}) (void)hipFree(p);
+        if (hflags) (void)hipHostFree(hflags);
+        if (hvals) (void)hipHostFree(hvals);
+        for (auto e : {fork, ev0, ev1})
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+#define JG_HIPB(call)                                                    \
+    do {                                                                 \
+        hipError_t e_ = (call);                                          \
+        if (e_ != hipSuccess) {                                          \
+            B->err = std::string(#call) + ": " + hipGetErrorString(e_);  \
+            return JG_ERR_HIP;                                           \
+        }                                                                \
+    } while (0)
+
+namespace {
+
+// The batch stream waits for the work queued so far on every system stream.
+int batch_join(jg_batch* B) {
+    for (size_t i = 0; i < B->sys.size(); ++i) {
+        JG_HIPB(hipEventRecord(B->join[i], B->sys[i]->stream));
+        JG_HIPB(hipStreamWaitEvent(B->stream, B->join[i], 0));
+    }
+    return JG_OK;
+}
+
+// Gathers (flags, iteration) of control block `which` (0: FP64, 1: inner) of
+// every system into hflags; synchronises.
+int batch_poll(jg_batch* B, int which) {
+    const int S = (int)B->sys.size();
+    k_gather_ctl<<<1, 256, 0, B->stream>>>(B->dctl + which * S, S, B->dflags);
+    JG_HIPB(hipGetLastError());
+    JG_HIPB(hipMemcpyAsync(B->hflags, B->dflags, 2 * S * sizeof(int), hipMemcpyDeviceToHost,
+                           B->stream));
+    JG_HIPB(hipStreamSynchronize(B->stream));
+    return JG_OK;
+}
+
+int batch_scal(jg_batch* B, int idx) {
+    const int S = (int)B->sys.size();
+    k_gather_scal<<<1, 256, 0, B->stream>>>(B->dscal, idx, S, B->dvals);
+    JG_HIPB(hipGetLastError());
+    JG_HIPB(hipMemcpyAsync(B->hvals, B->dvals, S * sizeof(double), hipMemcpyDeviceToHost, B->stream));
+    JG_HIPB(hipStreamSynchronize(B->stream));
+    return JG_OK;
+}
+
+// Precision-specific views of a system's level-0 PCG buffers.
+Pcg<double> pcg64(jg_solver* s, const MgCfg<double>* mg) {
+    Level& l0 = s->lv[0];
+    return Pcg<double>{l0.A64, s->b, s->x, s->r, s->z, s->dA, s->dB, s->q, l0.d, s->ctl, mg};
+}
+Pcg<float> pcg32(jg_solver* s, const MgCfg<float>* mg) {
+    Level& l0 = s->lv[0];
+    return Pcg<float>{l0.A32, s->rhs32, s->e32, s->r32, s->z32, s->d32A, s->d32B, s->q32, l0.d,
+                      s->ctl + 1, mg};
+}
+
+// Captures kBatchIters PCG iterations of every system into one graph (or
+// returns the cached one for the same precision and V-cycle parameters).
+template <typename T>
+int batch_graph(jg_batch* B, int precision, const MgCfg<T>& mg, hipGraphExec_t* out) {
+    for (auto& g : B->graphs)
+        if (g.precision == precision && g.pre == mg.pre && g.post0 == mg.post0 && g.post == mg.post &&
+            g.coarsest == mg.coarsest && g.omega == (double)mg.omega) {
+            *out = g.exec;
+            return JG_OK;
+        }
+    JG_HIPB(hipStreamBeginCapture(B->stream, hipStreamCaptureModeRelaxed));
+    int st = JG_OK;
+    if (hipEventRecord(B->fork, B->stream) != hipSuccess) st = JG_ERR_HIP;
+    for (size_t i = 0; i < B->sys.size() && st == JG_OK; ++i) {
+        jg_solver* s = B->sys[i];
+        if (hipStreamWaitEvent(s->stream, B->fork, 0) != hipSuccess) {
+            st = JG_ERR_HIP;
+            break;
+        }
+        for (int k = 0; k < kBatchIters && st == JG_OK; ++k) {
+            if constexpr (sizeof(T) == 8)
+                st = pcg_iteration<double>(s, pcg64(s, &mg), (k & 1) != 0);
+            else
+                st = pcg_iteration<float>(s, pcg32(s, &mg), (k & 1) != 0);
+        }
+        if (st == JG_OK && (hipEventRecord(B->join[i], s->stream) != hipSuccess ||
+                            hipStreamWaitEvent(B->stream, B->join[i], 0) != hipSuccess))
+            st = JG_ERR_HIP;
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(B->stream, &graph);
+    if (st != JG_OK || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        B->err = "graph capture failed";
+        return st != JG_OK ? st : JG_ERR_HIP;
+    }
+    hipGraphExec_t exec = nullptr;
+    JG_HIPB(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    B->graphs.push_back({precision, mg.pre, mg.post0, mg.post, mg.coarsest, (double)mg.omega, graph, exec});
+    *out = exec;
+    return JG_OK;
+}
+
+// Replays the iteration graph until every system's control block `which`
+// is done (or the iteration budget is spent).
+int batch_replay(jg_batch* B, hipGraphExec_t exec, int which, unsigned max_iter) {
+    const size_t S = B->sys.size();
+    for (unsigned launched = 0;; launched += kBatchIters) {
+        JG_TRY(batch_poll(B, which));
+        bool all = true;
+        for (size_t i = 0; i < S; ++i) all = all && (B->hflags[2 * i] & 1);
+        if (all || launched > max_iter + kBatchIters) break;
+        JG_HIPB(hipGraphLaunch(exec, B->stream));
+    }
+    return JG_OK;
+}
+
+// Sweep tuning once per distinct hierarchy; identical hierarchies copy it.
+template <typename T>
+int batch_tune(jg_batch* B) {
+    for (size_t i = 0; i < B->sys.size(); ++i) {
+        jg_solver* s = B->sys[i];
+        if (Prec<T>::zc(s->lv[0]) > 0) continue;
+        bool copied = false;
+        for (size_t j = 0; j < i && !copied; ++j) {
+            jg_solver* t = B->sys[j];
+            if (t->lv.size() != s->lv.size()) continue;
+            bool same = true;
+            for (size_t l = 0; same && l < s->lv.size(); ++l)
+                same = t->lv[l].d.nx == s->lv[l].d.nx && t->lv[l].d.ny == s->lv[l].d.ny &&
+                       t->lv[l].d.nz == s->lv[l].d.nz;
+            if (!same) continue;
+            for (size_t l = 0; l < s->lv.size(); ++l) {
+                Prec<T>::zc(s->lv[l]) = Prec<T>::zc(t->lv[l]);
+                Prec<T>::sh(s->lv[l]) = Prec<T>::sh(t->lv[l]);
+            }
+            copied = true;
+        }
+        if (!copied) {
+            const int st = autotune_sweeps<T>(s);
+            if (st != JG_OK) {
+                B->err = s->err;
+                return st;
+            }
+        }
+    }
+    return JG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int jg_batch_create(int device, size_t nsys, const size_t* nx, const size_t* ny, const size_t* nz,
+                    const size_t* nlev, jg_batch** out) {
+    if (!out || nsys == 0 || nsys > 4096 || !nx || !ny || !nz || !nlev) return JG_ERR_INVALID_ARG;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= device || device < 0)
+        return JG_ERR_NO_DEVICE;
+    jg_batch* B = new jg_batch();
+    B->device = device;
+    auto fail = [&](int st) {
+        delete B;
+        return st;
+    };
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&B->stream) != hipSuccess ||
+        hipEventCreateWithFlags(&B->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreate(&B->ev0) != hipSuccess || hipEventCreate(&B->ev1) != hipSuccess)
+        return fail(JG_ERR_HIP);
+    for (size_t i = 0; i < nsys; ++i) {
+        if (nlev[i] == 0 || nlev[i] > 16) return fail(JG_ERR_INVALID_ARG);
+        std::vector<size_t> lx(nlev[i]), ly(nlev[i]), lz(nlev[i]);
+        for (size_t l = 0; l < nlev[i]; ++l) {
+            lx[l] = nx[i] >> l;
+            ly[l] = ny[i] >> l;
+            lz[l] = nz[i] >> l;
+        }
+        jg_solver* s = nullptr;
+        int st = jg_solver_create(device, &s);
+        if (st != JG_OK) return fail(st);
+        B->sys.push_back(s);
+        st = jg_solver_set_levels(s, nlev[i], lx.data(), ly.data(), lz.data());
+        if (st != JG_OK) return fail(st);
+        hipEvent_t e = nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return fail(JG_ERR_HIP);
+        B->join.push_back(e);
+    }
+    std::vector<Ctl*> ctls(2 * nsys);
+    std::vector<double*> scals(nsys);
+    for (size_t i = 0; i < nsys; ++i) {
+        ctls[i] = B->sys[i]->ctl;
+        ctls[nsys + i] = B->sys[i]->ctl + 1;
+        scals[i] = B->sys[i]->scal;
+    }
+    if (hipMalloc((void**)&B->dctl, 2 * nsys * sizeof(Ctl*)) != hipSuccess ||
+        hipMalloc((void**)&B->dscal, nsys * sizeof(double*)) != hipSuccess ||
+        hipMalloc((void**)&B->dflags, 2 * nsys * sizeof(int)) != hipSuccess ||
+        hipMalloc((void**)&B->dvals, nsys * sizeof(double)) != hipSuccess ||
+        hipHostMalloc((void**)&B->hflags, 2 * nsys * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&B->hvals, nsys * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+        hipMemcpy(B->dctl, ctls.data(), 2 * nsys * sizeof(Ctl*), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(B->dscal, scals.data(), nsys * sizeof(double*), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(JG_ERR_OUT_OF_MEMORY);
+    *out = B;
+    return JG_OK;
+}
+
+void jg_batch_destroy(jg_batch* B) {
+    if (!B) return;
+    (void)hipSetDevice(B->device);
+    (void)hipStreamSynchronize(B->stream);
+    delete B;
+}
+
+const char* jg_batch_last_error(const jg_batch* B) { return B ? B->err.c_str() : "null batch"; }
+
+size_t jg_batch_size(const jg_batch* B) { return B ? B->sys.size() : 0; }
+
+jg_solver* jg_batch_system(jg_batch* B, size_t i) {
+    return (B && i < B->sys.size()) ? B->sys[i] : nullptr;
+}
+
+int jg_batch_mgpcg(jg_batch* B, unsigned max_iter, const jg_mg_params* p, const jg_gpu_options* o,
+                   jg_solve_info* infos, double* total_ms) {
+    if (!B || !p) return JG_ERR_INVALID_ARG;
+    JG_HIPB(hipSetDevice(B->device));
+    const size_t S = B->sys.size();
+    jg_gpu_options opt;
+    double omega = 0.0;
+    {
+        const int st = mg_options(B->sys[0], p, o, &opt, &omega);
+        if (st != JG_OK) {
+            B->err = B->sys[0]->err;
+            return st;
+        }
+    }
+    const bool mixed = opt.precision == JG_MIXED;
+    for (auto* s : B->sys)
+        if (mixed) JG_TRY(ensure_f32(s));
+    JG_TRY(mixed ? batch_tune<float>(B) : batch_tune<double>(B));
+    for (auto* s : B->sys) JG_TRY(sync(s));
+    std::vector<unsigned> iters(S, 0), refinements(S, 0);
+    std::vector<double> last(S, 0.0);
+    std::vector<int> status(S, JG_OK);
+    JG_TRY(batch_join(B));
+    JG_HIPB(hipEventRecord(B->ev0, B->stream));
+    if (!mixed) {
+        MgCfg<double> mg{p->restrict_iter, p->final_iter, p->correct_iter, p->coarsest_iter, omega};
+        hipGraphExec_t exec = nullptr;
+        JG_TRY(batch_graph<double>(B, JG_FP64, mg, &exec));
+        for (auto* s : B->sys) {
+            JG_HIPB(hipStreamWaitEvent(s->stream, B->ev0, 0));
+            const Dims& d0 = s->lv[0].d;
+            k_fill<double><<<nblocks_1d(d0.n()), NT, 0, s->stream>>>(s->x, 0.0, d0.n());
+            Ctl h;
+            init_ctl(&h, max_iter, p->max_tol, opt.stop_rule == JG_STOP_RELATIVE ? 1 : 0, opt.rel_tol);
+            JG_TRY(upload_ctl(s, s->ctl, h));
+            JG_TRY(pcg_init<double>(s, pcg64(s, &mg)));
+        }
+        JG_TRY(batch_join(B));
+        JG_TRY(batch_replay(B, exec, 0, max_iter));
+        JG_TRY(batch_poll(B, 0));
+        for (size_t i = 0; i < S; ++i) {
+            iters[i] = (unsigned)B->hflags[2 * i + 1];
+            if (B->hflags[2 * i] & 2) status[i] = JG_ERR_NONFINITE;
+        }
+    } else {
+        MgCfg<float> mg{p->restrict_iter, p->final_iter, p->correct_iter, p->coarsest_iter, (float)omega};
+        hipGraphExec_t exec = nullptr;
+        JG_TRY(batch_graph<float>(B, JG_MIXED, mg, &exec));
+        // ||b|| of every system
+        for (auto* s : B->sys) {
+            JG_HIPB(hipStreamWaitEvent(s->stream, B->ev0, 0));
+            const Dims& d0 = s->lv[0].d;
+            const int nb1 = nblocks_1d(d0.n());
+            k_fill<double><<<nb1, NT, 0, s->stream>>>(s->x, 0.0, d0.n());
+            k_dot<double, double><<<nb1, NT, 0, s->stream>>>(s->b, s->b, d0.n(), s->slot(3), nullptr);
+            k_fin_sum<<<1, NT, 0, s->stream>>>(s->slot(3), nb1, s->scal + 2);
+        }
+        JG_TRY(batch_join(B));
+        JG_TRY(batch_scal(B, 2));
+        std::vector<double> goal(S);
+        for (size_t i = 0; i < S; ++i)
+            goal[i] = opt.stop_rule == JG_STOP_RELATIVE ? opt.rel_tol * std::sqrt(B->hvals[i]) : p->max_tol;
+        std::vector<char> done(S, 0);
+        for (unsigned outer = 0;; ++outer) {
+            // FP64 residual r = b - A x of every unfinished system, into the FP32 right-hand side
+            for (size_t i = 0; i < S; ++i) {
+                if (done[i]) continue;
+                jg_solver* s = B->sys[i];
+                Level& l0 = s->lv[0];
+                Geo g = geo_stencil(l0.d);
+                k_residual<double, double, float, true, false><<<g.grid, g.block, 0, s->stream>>>(
+                    l0.A64, s->x, s->b, s->rhs32, l0.d, g.zc, s->slot(3), nullptr);
+                k_fin_sum<<<1, NT, 0, s->stream>>>(s->slot(3), nblocks(g), s->scal + 3);
+            }
+            JG_TRY(batch_join(B));
+            JG_TRY(batch_scal(B, 3));
+            bool any = false;
+            for (size_t i = 0; i < S; ++i) {
+                jg_solver* s = B->sys[i];
+                Ctl h;
+                if (!done[i]) {
+                    const double rr = B->hvals[i];
+                    last[i] = std::sqrt(rr);
+                    if (!std::isfinite(rr)) {
+                        status[i] = JG_ERR_NONFINITE;
+                        done[i] = 1;
+                    } else if (last[i] <= goal[i] || outer >= opt.max_refine || iters[i] >= max_iter) {
+                        done[i] = 1;
+                    }
+                }
+                if (done[i]) {
+                    init_ctl(&h, 0, 0.0, 1, 0.0);
+                    h.done = 1;  // this system's graph branch replays as no-ops
+                    JG_TRY(upload_ctl(s, s->ctl + 1, h));
+                    continue;
+                }
+                any = true;
+                const double inner =
+                    std::max(opt.inner_rel_tol, std::min(0.5, 0.5 * goal[i] / last[i]));
+                init_ctl(&h, max_iter - iters[i], 0.0, 1, inner);
+                JG_TRY(upload_ctl(s, s->ctl + 1, h));
+                const Dims& d0 = s->lv[0].d;
+                k_fill<float><<<nblocks_1d(d0.n()), NT, 0, s->stream>>>(s->e32, 0.f, d0.n());
+                JG_TRY(pcg_init<float>(s, pcg32(s, &mg)));
+            }
+            if (!any) break;
+            JG_TRY(batch_join(B));
+            JG_TRY(batch_replay(B, exec, 1, max_iter));
+            JG_TRY(batch_poll(B, 1));
+            for (size_t i = 0; i < S; ++i) {
+                if (done[i]) continue;
+                jg_solver* s = B->sys[i];
+                iters[i] += (unsigned)B->hflags[2 * i + 1];
+                const Dims& d0 = s->lv[0].d;
+                k_add_correction<<<nblocks_1d(d0.n()), NT, 0, s->stream>>>(s->x, s->e32, d0.n());
+                ++refinements[i];
+            }
+        }
+    }
+    JG_TRY(batch_join(B));
+    JG_HIPB(hipEventRecord(B->ev1, B->stream));
+    JG_HIPB(hipStreamSynchronize(B->stream));
+    float ms = 0.f;
+    JG_HIPB(hipEventElapsedTime(&ms, B->ev0, B->ev1));
+    if (total_ms) *total_ms = ms;
+    int worst = JG_OK;
+    for (size_t i = 0; i < S; ++i) {
+        jg_solver* s = B->sys[i];
+        if (status[i] != JG_OK) worst = status[i];
+        if (!infos) continue;
+        jg_solve_info& info = infos[i];
+        std::memset(&info, 0, sizeof(info));
+        info.iterations = iters[i];
+        info.refinements = refinements[i];
+        info.solve_ms = ms;
+        JG_TRY(true_rel_residual(s, &info.rel_residual));
+        if (!mixed) {
+            JG_HIPB(hipMemcpy(s->ctl_host, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+            last[i] = std::sqrt(std::fabs(s->ctl_host->sigma));
+        }
+        info.last_residual = last[i];
+        info.converged = opt.stop_rule == JG_STOP_RELATIVE ? info.rel_residual <= opt.rel_tol
+                                                           : (last[i] <= p->max_tol || iters[i] < max_iter);
+    }
+    if (worst == JG_ERR_NONFINITE) B->err = "non-finite value in a batched solve";
+    return worst;
 }
 
 }  // extern "C"

@@ -19,6 +19,11 @@
 // block must never overwrite values a neighbouring block may still load as
 // its halo.
 //
+// A row with a zero centre (a fluid cell whose neighbours are all solid,
+// which Jet's single-phase build leaves as an all-zero row with b = 0, and
+// which Jet's smoothers divide by) is set to zero instead, so the smoother
+// stays a symmetric, finite operator.
+//
 // Options: ZERO treats the incoming x as zero (no read of xin); CORRECT adds
 // the trilinear prolongation of a coarse correction to x as it is loaded
 // (the operator of FdmMgUtils3::correct, src/jet/fdm_mg_linear_system3.cpp:
@@ -236,7 +241,9 @@ __global__ __launch_bounds__(kSwLX * kSwLY / RP, sweep_waves_per_eu(sizeof(T), R
                          Q.gy >= 0 && Q.gy < d.ny && ((gx + Q.gy + p) & 1) == FIRST;
         if (!act) return;
         const int sp = p & 3, sm = (p - 1) & 3, sn = (p + 1) & 3, s1 = p & 1;
-        if (ZERO && !CORRECT) {
+        if (Q.c1 == T(0)) {
+            sx[sp][ly][tx] = T(0);  // zero-centre row (see the file comment)
+        } else if (ZERO && !CORRECT) {
             // x is zero everywhere before this colour: (1-w)0 + w(b - 0)/c
             sx[sp][ly][tx] = omega * (Q.b1 - T(0)) / Q.c1;
         } else {
@@ -266,7 +273,7 @@ __global__ __launch_bounds__(kSwLX * kSwLY / RP, sweep_waves_per_eu(sizeof(T), R
             s += su[s2][ly][tx] * sx[sq][ly + 1][tx];
             s += Q.f3 * sx[sm][ly][tx];
             s += Q.f2 * sx[sn][ly][tx];
-            xv = (T(1) - omega) * xv + omega * (Q.b2 - s) / Q.c2;
+            xv = Q.c2 == T(0) ? T(0) : (T(1) - omega) * xv + omega * (Q.b2 - s) / Q.c2;
             sx[sq][ly][tx] = xv;
         }
         xout[gx + (idx_t)d.nx * Q.gy + sxy * q] = xv;

@@ -97,6 +97,14 @@ SIGNATURES = {
     "jg_fr_build": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d]),
     "jg_fr_apply_gradient": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d]),
     "jg_fr_array": (_vp, [_vp, _sz, C.c_int]),
+    "jg_batch_create": (C.c_int, [C.c_int, _sz, C.POINTER(_sz), C.POINTER(_sz), C.POINTER(_sz),
+                                  C.POINTER(_sz), C.POINTER(_vp)]),
+    "jg_batch_destroy": (None, [_vp]),
+    "jg_batch_last_error": (C.c_char_p, [_vp]),
+    "jg_batch_size": (_sz, [_vp]),
+    "jg_batch_system": (_vp, [_vp, _sz]),
+    "jg_batch_mgpcg": (C.c_int, [_vp, _u, C.POINTER(MgParams), C.POINTER(GpuOptions),
+                                 C.POINTER(SolveInfo), C.POINTER(_d)]),
     "jg_scene_generate_host": (C.c_int, [C.POINTER(SceneDesc), _vp, _vp, _vp, _vp, _vp, _vp]),
     "jg_scene_generate_device": (C.c_int, [_vp, C.POINTER(SceneDesc), _vp, _vp, _vp, _vp, _vp, _vp]),
 }
@@ -165,8 +173,57 @@ class JgError(RuntimeError):
     pass
 
 
+class Batch:
+    """Owns one jg_batch: independent systems solved by one replayed graph."""
+
+    def __init__(self, sizes, levels, device=0):
+        n = len(sizes)
+        arr = lambda v: (_sz * n)(*v)
+        h = _vp()
+        st = lib().jg_batch_create(device, n, arr([s[0] for s in sizes]), arr([s[1] for s in sizes]),
+                                   arr([s[2] for s in sizes]), arr(levels), C.byref(h))
+        if st != JG_OK:
+            raise JgError(f"jg_batch_create: {ERRORS.get(st, st)}")
+        self.h = h
+        self.systems = [Solver.wrap(lib().jg_batch_system(h, i)) for i in range(n)]
+        for sys_, (nx, ny, nz), lv in zip(self.systems, sizes, levels):
+            sys_.sizes = [(nx >> l, ny >> l, nz >> l) for l in range(lv)]
+
+    def close(self):
+        if self.h:
+            lib().jg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def mgpcg(self, max_iter, params: MgParams, options: GpuOptions | None = None):
+        """-> (list of SolveInfo, total device ms)"""
+        n = len(self.systems)
+        infos = (SolveInfo * n)()
+        ms = _d()
+        st = lib().jg_batch_mgpcg(self.h, max_iter, C.byref(params),
+                                  C.byref(options) if options is not None else None, infos,
+                                  C.byref(ms))
+        if st != JG_OK:
+            msg = lib().jg_batch_last_error(self.h)
+            raise JgError(f"batch mgpcg: {ERRORS.get(st, st)}: {msg.decode() if msg else ''}")
+        return list(infos), ms.value
+
+
 class Solver:
     """Owns one jg_solver handle (one HIP stream on one device)."""
+
+    @classmethod
+    def wrap(cls, handle):
+        """A view of a solver owned elsewhere (a batch member): never destroyed here."""
+        obj = cls.__new__(cls)
+        obj.h = _vp(handle)
+        obj._owned = False
+        return obj
 
     def __init__(self, device=0):
         h = _vp()
@@ -174,11 +231,12 @@ class Solver:
         if st != JG_OK:
             raise JgError(f"jg_solver_create: {ERRORS.get(st, st)}")
         self.h = h
+        self._owned = True
 
     def close(self):
-        if self.h:
+        if self.h and getattr(self, "_owned", True):
             lib().jg_solver_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

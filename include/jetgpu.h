@@ -139,6 +139,27 @@ int jg_solver_mgpcg(jg_solver* s, unsigned max_iter, const jg_mg_params* p,
 int jg_mg_vcycle(jg_solver* s, const jg_mg_params* p, const jg_gpu_options* o,
                  const double* r, double* z);
 
+/* ---------------------------------------------------- batched solves */
+/* Many independent systems of uneven sizes solved together: each system is
+ * a jg_solver of the batch (fill it with jg_solver_upload / jg_sp_build /
+ * jg_fr_build through jg_batch_system), and jg_batch_mgpcg runs the MGPCG of
+ * all of them as one captured hipGraph (one branch per system, each kernel
+ * gated by that system's own convergence flag) replayed until every system
+ * has stopped.  No counterpart in Jet, which solves one system per
+ * FdmLinearSystemSolver3::solve call (include/jet/fdm_linear_system_solver3.h:
+ * 17-31); per system the semantics are jg_solver_mgpcg's. */
+typedef struct jg_batch jg_batch;
+/* nsys systems; system i has nlev[i] levels halving from nx[i] x ny[i] x nz[i]. */
+int jg_batch_create(int device, size_t nsys, const size_t* nx, const size_t* ny,
+                    const size_t* nz, const size_t* nlev, jg_batch** out);
+void jg_batch_destroy(jg_batch* b);
+const char* jg_batch_last_error(const jg_batch* b);
+size_t jg_batch_size(const jg_batch* b);
+jg_solver* jg_batch_system(jg_batch* b, size_t i);
+/* infos: one per system (may be NULL); total_ms: device time of the batch. */
+int jg_batch_mgpcg(jg_batch* b, unsigned max_iter, const jg_mg_params* p,
+                   const jg_gpu_options* o, jg_solve_info* infos, double* total_ms);
+
 /* ------------------------------------------------ host-memory drop-ins */
 /* One-call forms of the two solves on host arrays: upload, solve, download
  * x.  rows[l] / b are host arrays in Jet's layout. */

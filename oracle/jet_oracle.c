@@ -196,16 +196,26 @@ void or_gs_relax(const or_row* A, const double* b, double sor, double* x,
             }
 }
 
-static void rb_pass(const or_row* A, const double* b, double sor, double* x,
-                    size_t nx, size_t ny, size_t nz, size_t color) {
+/* zero_rows: a row with a zero centre is set to 0 instead of divided by
+ * (the clean-mode smoother below; Jet's relaxRedBlack divides). */
+static void rb_pass_rows(const or_row* A, const double* b, double sor, double* x,
+                         size_t nx, size_t ny, size_t nz, size_t color, int zero_rows) {
 #pragma omp parallel for schedule(static)
     for (size_t k = 0; k < nz; ++k)
         for (size_t j = 0; j < ny; ++j)
             for (size_t i = (j + k + color) % 2; i < nx; i += 2) {
                 const size_t c = IX(i, j, k, nx, ny);
                 double r = OR_OFFDIAG_SUM(A, x, c, i, j, k, nx, ny, nz);
-                x[c] = (1.0 - sor) * x[c] + sor * (b[c] - r) / A[c].center;
+                if (zero_rows && A[c].center == 0.0)
+                    x[c] = 0.0;
+                else
+                    x[c] = (1.0 - sor) * x[c] + sor * (b[c] - r) / A[c].center;
             }
+}
+
+static void rb_pass(const or_row* A, const double* b, double sor, double* x,
+                    size_t nx, size_t ny, size_t nz, size_t color) {
+    rb_pass_rows(A, b, sor, x, nx, ny, nz, color, 0);
 }
 
 /* FdmGaussSeidelSolver3::relaxRedBlack --
@@ -473,15 +483,18 @@ int or_mgpcg_solve(or_mg_system* sys, const or_mg_params* p, unsigned max_iter,
  * V-cycle (zero initial guess, red-black forward pre-smoothing, black-red
  * backward post-smoothing, symmetric coarsest sweeps) inside plain PCG.  Jet's
  * restriction equals 1/8 of the transpose of its prolongation, clamps
- * included, so the preconditioner is symmetric.  Its answer is pinned by the
+ * included, so the preconditioner is symmetric.  A zero-centre row (a fluid
+ * cell with only solid neighbours: an all-zero row with b = 0 in Jet's
+ * single-phase build) is set to 0 by the smoother instead of divided by, so
+ * the preconditioner stays finite and symmetric.  Its answer is pinned by the
  * true residual ||b - A x|| (the bitwise-pinned residual above), which the
  * caller checks. */
 
 static void rb_sweep_ordered(const or_row* A, const double* b, double sor,
                              double* x, size_t nx, size_t ny, size_t nz,
                              int backward) {
-    rb_pass(A, b, sor, x, nx, ny, nz, backward ? 1 : 0);
-    rb_pass(A, b, sor, x, nx, ny, nz, backward ? 0 : 1);
+    rb_pass_rows(A, b, sor, x, nx, ny, nz, backward ? 1 : 0, 1);
+    rb_pass_rows(A, b, sor, x, nx, ny, nz, backward ? 0 : 1, 1);
 }
 
 static void clean_vcycle(const or_mg_system* s, const or_mg_params* p,

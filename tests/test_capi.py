@@ -42,6 +42,27 @@ def test_solver_create_without_gpu_fails_cleanly():
         jetgpu.Solver(0)
 
 
+def test_batch_create_without_gpu_fails_cleanly():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(jetgpu.JgError):
+        jetgpu.Batch([(32, 32, 32), (48, 48, 48)], [3, 4])
+
+
+def test_batch_rejects_bad_arguments():
+    import ctypes as C
+
+    lib = jetgpu.lib()
+    h = C.c_void_p()
+    one = (C.c_size_t * 1)(8)
+    assert lib.jg_batch_create(0, 0, one, one, one, one, C.byref(h)) == 1  # no systems
+    assert lib.jg_batch_create(0, 1, one, one, one, one, None) == 1        # no output
+    assert lib.jg_batch_mgpcg(None, 10, None, None, None, None) == 1
+    assert lib.jg_batch_system(None, 0) is None and lib.jg_batch_size(None) == 0
+
+
 @pytest.mark.parametrize("kind,n", [(1, 16), (2, 16), (3, 32), (4, 24), (5, 32)])
 def test_scene_generator_deterministic_and_bc(kind, n):
     d = jetgpu.scene_desc(kind, n, seed=7)

@@ -115,6 +115,38 @@ def test_vcycle_symmetric_positive_linear(solver, shape, levels, precision, tol)
     assert np.array_equal(apply(solver, gp, precision, r1), z1)
 
 
+@pytest.mark.parametrize("shape,levels", [((32, 32, 32), 4), ((16, 16, 16), 3)])
+def test_vcycle_zero_centre_rows(solver, oracle, shape, levels):
+    # Fluid cells whose neighbours are all solid have all-zero rows in Jet's
+    # single-phase build; the smoothers set them to zero instead of dividing
+    # by the zero centre, on every level.
+    print(f"case vcycle zero rows shape={shape}", flush=True)
+    s = random_hierarchy(shape, levels, seed=36)
+    rng = np.random.default_rng(37)
+    for A in s.A:
+        nz, ny, nx = A.shape[:3]
+        mask = rng.random((nz, ny, nx)) < 0.02
+        A[mask] = 0.0
+        # the neighbours' couplings toward a zero row vanish too (symmetry)
+        A[:, :, 1:, 1][mask[:, :, 1:]] = 0.0
+        A[:, :, :-1, 1][mask[:, :, 1:]] = 0.0
+        A[:, 1:, :, 2][mask[:, 1:, :]] = 0.0
+        A[:, :-1, :, 2][mask[:, 1:, :]] = 0.0
+        A[1:, :, :, 3][mask[1:, :, :]] = 0.0
+        A[:-1, :, :, 3][mask[1:, :, :]] = 0.0
+    upload(solver, s)
+    gp, op = params((2, 2, 20, 2), 1.3)
+    r = rng.uniform(-1, 1, s.x[0].shape)
+    zero = s.A[0][..., 0] == 0.0
+    r[zero] = 0.0
+    want = oracle.mg_clean_vcycle(s, op, r)
+    assert np.isfinite(want).all() and np.all(want[zero] == 0.0)
+    for prec, tol in ((jetgpu.JG_FP64, 1e-12), (jetgpu.JG_MIXED, 2e-5)):
+        got = apply(solver, gp, prec, r)
+        assert np.isfinite(got).all() and np.all(got[zero] == 0.0)
+        assert rel_err(got, want) <= tol
+
+
 def test_vcycle_invalid_arguments(solver):
     s = random_hierarchy((8, 8, 8), 2, seed=35)
     upload(solver, s)
