@@ -43,9 +43,14 @@ struct Level {
     // Variational pressure data (allocated by the first jg_fr_build): fluid
     // SDF at cell centres and face weights, float as Jet stores them.
     float *fs = nullptr, *uw = nullptr, *vw = nullptr, *ww = nullptr;
-    int zc64 = 0, zc32 = 0;     // tuned z-chunk of the fused sweep (0: untuned)
-    int sh64 = 0, sh32 = 0;     // tuned launch shape of the fused sweep
+    // Tuned z-chunk (0: untuned) and launch shape of the fused sweep, per
+    // sweep kind (kSweepPlain, kSweepZero, kSweepCorrect).
+    int zc64[3] = {0, 0, 0}, zc32[3] = {0, 0, 0};
+    int sh64[3] = {0, 0, 0}, sh32[3] = {0, 0, 0};
 };
+
+// Sweep kinds tuned separately: they differ in registers and loads.
+enum { kSweepPlain = 0, kSweepZero = 1, kSweepCorrect = 2, kNumSweepKinds = 3 };
 
 template <typename T>
 struct Prec;
@@ -56,8 +61,8 @@ struct Prec<double> {
     static double* b(Level& l) { return l.b64; }
     static double* t(Level& l) { return l.t64; }
     static double* y(Level& l) { return l.y64; }
-    static int& zc(Level& l) { return l.zc64; }
-    static int& sh(Level& l) { return l.sh64; }
+    static int& zc(Level& l, int kind = kSweepPlain) { return l.zc64[kind]; }
+    static int& sh(Level& l, int kind = kSweepPlain) { return l.sh64[kind]; }
 };
 template <>
 struct Prec<float> {
@@ -66,8 +71,8 @@ struct Prec<float> {
     static float* b(Level& l) { return l.b32; }
     static float* t(Level& l) { return l.t32; }
     static float* y(Level& l) { return l.y32; }
-    static int& zc(Level& l) { return l.zc32; }
-    static int& sh(Level& l) { return l.sh32; }
+    static int& zc(Level& l, int kind = kSweepPlain) { return l.zc32[kind]; }
+    static int& sh(Level& l, int kind = kSweepPlain) { return l.sh32[kind]; }
 };
 
 struct Geo {

@@ -190,8 +190,9 @@ int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, 
             const T* in = nullptr;
             for (int k = 0; k < S; ++k) {
                 T* out = ((S - 1 - k) % 2 == 0) ? X(l) : Y;
+                const int kind = in ? kSweepPlain : kSweepZero;
                 launch_sweep<T>(s, A, B(l), in, out, d, k < fwd ? 0 : 1, nullptr, nullptr, nullptr,
-                                c.omega, ctl, Prec<T>::zc(lv), Prec<T>::sh(lv));
+                                c.omega, ctl, Prec<T>::zc(lv, kind), Prec<T>::sh(lv, kind));
                 in = out;
             }
             JG_TRY(check_launch(s, "coarsest"));
@@ -203,8 +204,9 @@ int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, 
         auto buf = [&](unsigned k) -> T* { return ((S - 1 - k) % 2 == 0) ? X(l) : Y; };
         const T* in = nullptr;
         for (unsigned k = 0; k < c.pre; ++k) {
+            const int kind = in ? kSweepPlain : kSweepZero;
             launch_sweep<T>(s, A, B(l), in, buf(k), d, 0, nullptr, nullptr, nullptr, c.omega, ctl,
-                            Prec<T>::zc(lv), Prec<T>::sh(lv));
+                            Prec<T>::zc(lv, kind), Prec<T>::sh(lv, kind));
             in = buf(k);
         }
         pre_out[l] = in;  // nullptr: x is still zero
@@ -239,10 +241,11 @@ int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, 
             const T* in = pre_out[l];
             for (unsigned k = c.pre; k < S; ++k) {
                 const bool last0 = (l == 0 && k + 1 == S);
+                const int kind = k == c.pre ? kSweepCorrect : kSweepPlain;
                 const int nb = launch_sweep<T>(s, A, B(l), in, buf(k), d, 1, k == c.pre ? coarse : nullptr,
                                                last0 ? s->slot(0) : nullptr,
                                                last0 ? s->slot(1) : nullptr, c.omega, ctl,
-                                               Prec<T>::zc(lv), Prec<T>::sh(lv));
+                                               Prec<T>::zc(lv, kind), Prec<T>::sh(lv, kind));
                 if (last0) *dot_blocks = nb;
                 in = buf(k);
             }
@@ -252,52 +255,73 @@ int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, 
     return JG_OK;
 }
 
-// z-chunk autotuner of the fused sweep: times each candidate on the level
-// with the level's own matrix and scratch vectors (run once per hierarchy
-// and precision, before a solve, so nothing live is overwritten).
+// Launch-shape and z-chunk autotuner of the fused sweep: times each
+// candidate on the level with the level's own matrix and scratch vectors,
+// separately for the three sweep kinds (plain, zero initial x, fused
+// correction), which differ in loads and registers.  Runs once per hierarchy
+// and precision, before a solve, so nothing live is overwritten.
 template <typename T>
 int autotune_sweeps(jg_solver* s) {
     if (sweep_zc_env() > 0) return JG_OK;
-    for (auto& lv : s->lv) {
+    const size_t L = s->lv.size();
+    for (size_t l = 0; l < L; ++l) {
+        Level& lv = s->lv[l];
         if (Prec<T>::zc(lv) > 0) continue;
         const Dims& d = lv.d;
         if (d.n() <= BotCfg<T>::MaxCells) {  // handled by k_bottom
-            Prec<T>::zc(lv) = sweep_zc(d, 0);
+            for (int kind = 0; kind < kNumSweepKinds; ++kind) Prec<T>::zc(lv, kind) = sweep_zc(d, 0);
             continue;
         }
         const idx_t n = d.n();
         k_fill<T><<<nblocks_1d(n), NT, 0, s->stream>>>(Prec<T>::t(lv), T(1), n);
         k_fill<T><<<nblocks_1d(n), NT, 0, s->stream>>>(Prec<T>::y(lv), T(0.5), n);
+        // the correction kind reads the next level's x as its coarse vector
+        const T* coarse = nullptr;
+        if (l + 1 < L) {
+            const idx_t cn = s->lv[l + 1].d.n();
+            coarse = Prec<T>::x(s->lv[l + 1]);
+            k_fill<T><<<nblocks_1d(cn), NT, 0, s->stream>>>(Prec<T>::x(s->lv[l + 1]), T(0.25), cn);
+        }
         const int cands[] = {2, 4, 8, 12, 16, 24, 32, 48, 64};
-        float best = 1e30f;
-        int best_zc = sweep_zc(d, 0), best_sh = 0;
-        for (int sh = 0; sh < kNumSweepShapes; ++sh) {
-            for (int zc : cands) {
-                if (zc > d.nz) break;
-                auto run = [&]() {
-                    launch_sweep<T>(s, Prec<T>::A(lv), Prec<T>::t(lv), Prec<T>::y(lv), Prec<T>::x(lv),
-                                    d, 1, nullptr, nullptr, nullptr, T(1), nullptr, zc, sh);
-                };
-                run();
-                JG_HIP(hipEventRecord(s->ev0, s->stream));
-                for (int r = 0; r < 3; ++r) run();
-                JG_HIP(hipEventRecord(s->ev1, s->stream));
-                JG_TRY(sync(s));
-                float ms = 0.f;
-                JG_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
-                if (tune_log())
-                    std::fprintf(stderr, "jetgpu tune: %dx%dx%d %s shape %d (rp %d pd %d) zc %d: %.1f us\n",
-                                 d.nx, d.ny, d.nz, sizeof(T) == 4 ? "f32" : "f64", sh,
-                                 kSweepShapes[sh].rp, kSweepShapes[sh].pd, zc, ms * 1e3f / 3);
-                if (ms < best) {
-                    best = ms;
-                    best_zc = zc;
-                    best_sh = sh;
+        for (int kind = 0; kind < kNumSweepKinds; ++kind) {
+            if (kind == kSweepCorrect && !coarse) {
+                Prec<T>::zc(lv, kind) = Prec<T>::zc(lv, kSweepPlain);
+                Prec<T>::sh(lv, kind) = Prec<T>::sh(lv, kSweepPlain);
+                continue;
+            }
+            float best = 1e30f;
+            int best_zc = sweep_zc(d, 0), best_sh = 0;
+            for (int sh = 0; sh < kNumSweepShapes; ++sh) {
+                for (int zc : cands) {
+                    if (zc > d.nz) break;
+                    auto run = [&]() {
+                        launch_sweep<T>(s, Prec<T>::A(lv), Prec<T>::t(lv),
+                                        kind == kSweepZero ? nullptr : Prec<T>::y(lv), Prec<T>::x(lv),
+                                        d, 1, kind == kSweepCorrect ? coarse : nullptr, nullptr,
+                                        nullptr, T(1), nullptr, zc, sh);
+                    };
+                    run();
+                    JG_HIP(hipEventRecord(s->ev0, s->stream));
+                    for (int r = 0; r < 3; ++r) run();
+                    JG_HIP(hipEventRecord(s->ev1, s->stream));
+                    JG_TRY(sync(s));
+                    float ms = 0.f;
+                    JG_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+                    if (tune_log())
+                        std::fprintf(stderr,
+                                     "jetgpu tune: %dx%dx%d %s kind %d shape %d (rp %d pd %d) zc %d: %.1f us\n",
+                                     d.nx, d.ny, d.nz, sizeof(T) == 4 ? "f32" : "f64", kind, sh,
+                                     kSweepShapes[sh].rp, kSweepShapes[sh].pd, zc, ms * 1e3f / 3);
+                    if (ms < best) {
+                        best = ms;
+                        best_zc = zc;
+                        best_sh = sh;
+                    }
                 }
             }
+            Prec<T>::sh(lv, kind) = best_sh;
+            Prec<T>::zc(lv, kind) = best_zc;
         }
-        Prec<T>::sh(lv) = best_sh;
-        Prec<T>::zc(lv) = best_zc;
     }
     return check_launch(s, "autotune");
 }
@@ -1204,10 +1228,11 @@ int batch_tune(jg_batch* B) {
                 same = t->lv[l].d.nx == s->lv[l].d.nx && t->lv[l].d.ny == s->lv[l].d.ny &&
                        t->lv[l].d.nz == s->lv[l].d.nz;
             if (!same) continue;
-            for (size_t l = 0; l < s->lv.size(); ++l) {
-                Prec<T>::zc(s->lv[l]) = Prec<T>::zc(t->lv[l]);
-                Prec<T>::sh(s->lv[l]) = Prec<T>::sh(t->lv[l]);
-            }
+            for (size_t l = 0; l < s->lv.size(); ++l)
+                for (int kind = 0; kind < kNumSweepKinds; ++kind) {
+                    Prec<T>::zc(s->lv[l], kind) = Prec<T>::zc(t->lv[l], kind);
+                    Prec<T>::sh(s->lv[l], kind) = Prec<T>::sh(t->lv[l], kind);
+                }
             copied = true;
         }
         if (!copied) {

@@ -100,13 +100,17 @@ constexpr SweepShape kSweepShapes[] = {{2, 1}, {2, 2}, {1, 1}, {1, 2}};
 constexpr int kNumSweepShapes = 4;
 
 // Occupancy each shape is compiled for (waves per SIMD): as many blocks per
-// CU as the VGPR budget allows without spilling the common variants.
-constexpr int sweep_waves_per_eu(int tsize, int rp, int pd) {
-    return rp == 2 ? (pd == 1 ? (tsize == 4 ? 6 : 4) : (tsize == 4 ? 5 : 2)) : (tsize == 4 ? 8 : 4);
+// CU as the VGPR budget allows without spilling.  The correction variant
+// holds the coarse taps too; in FP32 with two rows per thread it gets a
+// larger budget (one block per CU fewer) instead of spilling.
+constexpr int sweep_waves_per_eu(int tsize, int rp, int pd, bool correct) {
+    return rp == 2 ? (pd == 1 ? (tsize == 4 ? (correct ? 5 : 6) : 4)
+                              : (tsize == 4 ? (correct ? 4 : 5) : 2))
+                   : (tsize == 4 ? 8 : 4);
 }
 
 template <typename T, int FIRST, bool ZERO, bool CORRECT, bool DOT, int RP, int PD>
-__global__ __launch_bounds__(kSwLX * kSwLY / RP, sweep_waves_per_eu(sizeof(T), RP, PD)) void k_rb_sweep(
+__global__ __launch_bounds__(kSwLX * kSwLY / RP, sweep_waves_per_eu(sizeof(T), RP, PD, CORRECT)) void k_rb_sweep(
     const Row<T>* __restrict__ A, const T* __restrict__ b, const T* __restrict__ xin,
     T* __restrict__ xout, Dims d, int zc, T omega, const T* __restrict__ coarse,
     double* __restrict__ p_bx, double* __restrict__ p_bb, const Ctl* __restrict__ ctl) {
@@ -137,15 +141,13 @@ __global__ __launch_bounds__(kSwLX * kSwLY / RP, sweep_waves_per_eu(sizeof(T), R
     // around it: x taps (tx >> 1) + {0, 1} for odd tx, (tx >> 1) + {-1, 0} + 1 for even.
     const int e = tx + kSwLX * ty;
     const bool zact = CORRECT && e < kCzX * kCzY;
-    idx_t zoff = 0;
+    unsigned zoff = 0;  // column offset within a coarse plane (< 2^32 cells)
     const idx_t csz = (idx_t)(d.nx / 2) * (d.ny / 2);
     if (CORRECT) {
         const int ci = min(max(gx0 / 2 - 1 + e % kCzX, 0), d.nx / 2 - 1);
         const int cj = min(max(gy0 / 2 - 1 + e / kCzX, 0), d.ny / 2 - 1);
-        zoff = ci + (idx_t)(d.nx / 2) * cj;
+        zoff = (unsigned)ci + (unsigned)(d.nx / 2) * (unsigned)cj;
     }
-    const int ax0 = (tx >> 1) + (tx & 1);  // left x tap in the coarse tile
-    const T wx0 = (tx & 1) ? T(0.75) : T(0.25), wx1 = (tx & 1) ? T(0.25) : T(0.75);
     T zc0 = 0, zc1 = 0;  // prefetched coarse pair of the next plane
     auto zfetch = [&](int t) __attribute__((always_inline)) {
         if (!zact) return;
@@ -166,12 +168,17 @@ __global__ __launch_bounds__(kSwLX * kSwLY / RP, sweep_waves_per_eu(sizeof(T), R
     // Each thread owns RP positions, rows ly = ty + TY * r.  Per position: c
     // and b of planes t, t-1, t-2, f of planes t .. t-3, the held r/u of
     // plane t (written to LDS one step later) and PD prefetched planes.
+    // Prefetched plane: raw loads (lanes outside the domain load cell 0),
+    // masked only when committed, so nothing waits on them before then.
     struct Pre {
-        T c, f, r, u, b, x;
+        Row<T> row;
+        T b, x;
         bool ok;
     };
     struct Pos {
         T c0, c1, c2, f0, f1, f2, f3, b0, b1, b2, hr, hu;
+        T px;      // final value of the last second-colour plane, stored one step later
+        bool pst;  // px is pending
         Pre p[PD];
         int ly, gy;
     };
@@ -184,6 +191,8 @@ __global__ __launch_bounds__(kSwLX * kSwLY / RP, sweep_waves_per_eu(sizeof(T), R
         P[r].f0 = P[r].f1 = P[r].f2 = P[r].f3 = T(0);
         P[r].b0 = P[r].b1 = P[r].b2 = T(0);
         P[r].hr = P[r].hu = T(0);
+        P[r].px = T(0);
+        P[r].pst = false;
 #pragma unroll
         for (int s = 0; s < 4; ++s) sx[s][P[r].ly][tx] = T(0);
 #pragma unroll
@@ -192,35 +201,31 @@ __global__ __launch_bounds__(kSwLX * kSwLY / RP, sweep_waves_per_eu(sizeof(T), R
             su[s][P[r].ly][tx] = T(0);
         }
     }
-    // Branch-free: lanes outside the domain load cell 0 (always valid) and
-    // select the neutral values (c = 1, everything else 0).
+    // Branch-free: lanes outside the domain load cell 0 (always valid); the
+    // commit selects the neutral values for them (c = 1, everything else 0).
     auto fetch = [&](Pre& q, int gy, int t) __attribute__((always_inline)) {
         const bool ok = xok && t >= 0 && t < d.nz && gy >= 0 && gy < d.ny;
         const idx_t c = ok ? gx + (idx_t)d.nx * gy + sxy * t : 0;
-        const Row<T> row = A[c];
-        const T bv = b[c];
-        const T xv = ZERO ? T(0) : xin[c];
+        q.row = A[c];
+        q.b = b[c];
+        q.x = ZERO ? T(0) : xin[c];
         q.ok = ok;
-        q.c = ok ? row.c : T(1);
-        q.f = ok ? row.f : T(0);
-        q.r = ok ? row.r : T(0);
-        q.u = ok ? row.u : T(0);
-        q.b = ok ? bv : T(0);
-        q.x = ok ? xv : T(0);
     };
     auto commit = [&](Pos& Q, const Pre& q, int t) __attribute__((always_inline)) {
         Q.c2 = Q.c1;
         Q.c1 = Q.c0;
-        Q.c0 = q.c;
+        Q.c0 = q.ok ? q.row.c : T(1);
         Q.f3 = Q.f2;
         Q.f2 = Q.f1;
         Q.f1 = Q.f0;
-        Q.f0 = q.f;
+        Q.f0 = q.ok ? q.row.f : T(0);
         Q.b2 = Q.b1;
         Q.b1 = Q.b0;
-        Q.b0 = q.b;
-        T xv = q.x;
+        Q.b0 = q.ok ? q.b : T(0);
+        T xv = q.ok ? q.x : T(0);
         if (CORRECT) {
+            const int ax0 = (tx >> 1) + (tx & 1);  // left x tap in the coarse tile
+            const T wx0 = (tx & 1) ? T(0.75) : T(0.25), wx1 = (tx & 1) ? T(0.25) : T(0.75);
             const int ay0 = (Q.ly >> 1) + (Q.ly & 1);
             const T wy0 = (Q.ly & 1) ? T(0.75) : T(0.25), wy1 = (Q.ly & 1) ? T(0.25) : T(0.75);
             const T lo = wx0 * sz[ay0][ax0] + wx1 * sz[ay0][ax0 + 1];
@@ -231,8 +236,8 @@ __global__ __launch_bounds__(kSwLX * kSwLY / RP, sweep_waves_per_eu(sizeof(T), R
         sx[t & 3][Q.ly][tx] = xv;
         sr[(t - 1) & 1][Q.ly][tx] = Q.hr;
         su[(t - 1) & 1][Q.ly][tx] = Q.hu;
-        Q.hr = q.r;
-        Q.hu = q.u;
+        Q.hr = q.ok ? q.row.r : T(0);
+        Q.hu = q.ok ? q.row.u : T(0);
     };
     // First colour on plane p (interior + 1-cell ring).
     auto first_colour = [&](Pos& Q, int p) __attribute__((always_inline)) {
@@ -258,11 +263,14 @@ __global__ __launch_bounds__(kSwLX * kSwLY / RP, sweep_waves_per_eu(sizeof(T), R
         }
     };
     double acc = 0.0, acc2 = 0.0;
-    // Second colour on plane q (interior), write back.
+    // Second colour on plane q (interior); the final value is stored by
+    // flush() at the next step, after that step's loads are issued, so the
+    // wait for those loads never includes the store.
     auto second_colour = [&](Pos& Q, int q) __attribute__((always_inline)) {
         const int ly = Q.ly;
         const bool act = tx >= kSwH && tx < kSwLX - kSwH && ly >= kSwH && ly < kSwLY - kSwH &&
                          xok && Q.gy >= 0 && Q.gy < d.ny;
+        Q.pst = act;
         if (!act) return;
         const int sq = q & 3, sm = (q - 1) & 3, sn = (q + 1) & 3, s2 = q & 1;
         T xv = sx[sq][ly][tx];
@@ -276,11 +284,16 @@ __global__ __launch_bounds__(kSwLX * kSwLY / RP, sweep_waves_per_eu(sizeof(T), R
             xv = Q.c2 == T(0) ? T(0) : (T(1) - omega) * xv + omega * (Q.b2 - s) / Q.c2;
             sx[sq][ly][tx] = xv;
         }
-        xout[gx + (idx_t)d.nx * Q.gy + sxy * q] = xv;
+        Q.px = xv;
         if (DOT) {
             acc += (double)Q.b2 * (double)xv;
             acc2 += (double)Q.b2 * (double)Q.b2;
         }
+    };
+
+    auto flush = [&](Pos& Q, int q) __attribute__((always_inline)) {
+        if (Q.pst) xout[gx + (idx_t)d.nx * Q.gy + sxy * q] = Q.px;
+        Q.pst = false;
     };
 
     // Prologue: planes k0-2 .. k0-3+PD in flight, the coarse tile of k0-2.
@@ -308,6 +321,9 @@ __global__ __launch_bounds__(kSwLX * kSwLY / RP, sweep_waves_per_eu(sizeof(T), R
 #pragma unroll
                 for (int r = 0; r < RP; ++r) fetch(P[r].p[u], P[r].gy, t + PD);
             }
+            // the final values of plane t - 3 (second colour of the last step)
+#pragma unroll
+            for (int r = 0; r < RP; ++r) flush(P[r], t - 3);
             __syncthreads();
             // ---- first colour on plane t - 1
             if (t - 1 >= 0 && t - 1 < d.nz) {
@@ -325,6 +341,9 @@ __global__ __launch_bounds__(kSwLX * kSwLY / RP, sweep_waves_per_eu(sizeof(T), R
             __syncthreads();
         }
     }
+    // the last second-colour plane, k1 - 1
+#pragma unroll
+    for (int r = 0; r < RP; ++r) flush(P[r], k1 - 1);
     if (DOT) {
         // Block partials of b.x and b.b; the x ring is free after the last barrier.
         double* sh = reinterpret_cast<double*>(&sx[0][0][0]);
