@@ -183,7 +183,9 @@ __global__ __launch_bounds__(NT) void k_dir_spmv(const Row<TA>* __restrict__ A,
 // TR may be narrower than TX (FP32 right-hand side of the mixed-precision
 // inner solve).  DOT: partial r.r.  REFRESH: only on iterations where the
 // control block asks for a true residual (Jet's every-50 / sigma rule).
-template <typename TA, typename TX, typename TR, bool DOT, bool REFRESH>
+// GATED: skipped once the control block says the solve has stopped (a
+// V-cycle residual inside a replayed batch graph).
+template <typename TA, typename TX, typename TR, bool DOT, bool REFRESH, bool GATED = false>
 __global__ __launch_bounds__(NT) void k_residual(const Row<TA>* __restrict__ A,
                                                  const TX* __restrict__ x,
                                                  const TX* __restrict__ b, TR* __restrict__ r,
@@ -191,6 +193,7 @@ __global__ __launch_bounds__(NT) void k_residual(const Row<TA>* __restrict__ A,
                                                  const Ctl* __restrict__ ctl) {
 #pragma clang fp contract(off)
     if (REFRESH && (ctl->done || !ctl->refresh)) return;
+    if (GATED && ctl && ctl->done) return;
     int bx, by, bz;
     xcd_tile(bx, by, bz);
     const int i = bx * blockDim.x + threadIdx.x;

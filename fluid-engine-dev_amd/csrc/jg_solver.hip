@@ -212,7 +212,16 @@ int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, 
         pre_out[l] = in;  // nullptr: x is still zero
         const Dims& cd = s->lv[l + 1].d;
         Geo gc = geo_planes(cd);
-        if (in) {
+        if (in && sizeof(T) == 8) {
+            // FP64: residual into t, then restriction (the fused kernel's
+            // register footprint halves its occupancy in FP64, and the pair
+            // measures faster)
+            Geo gf = geo_stencil(d);
+            k_residual<T, T, T, false, false, true><<<gf.grid, gf.block, 0, s->stream>>>(
+                A, in, B(l), Prec<T>::t(lv), d, gf.zc, nullptr, ctl);
+            k_restrict<T><<<gc.grid, gc.block, 0, s->stream>>>(Prec<T>::t(lv), Prec<T>::b(s->lv[l + 1]),
+                                                               cd, ctl);
+        } else if (in) {
             const int zc = rr_zc(cd);
             dim3 grid((cd.nx + kRrCX - 1) / kRrCX, (cd.ny + kRrCY - 1) / kRrCY, (cd.nz + zc - 1) / zc);
             k_residual_restrict<T><<<grid, dim3(kRrLX, kRrTY, 1), 0, s->stream>>>(

@@ -47,17 +47,36 @@ __global__ __launch_bounds__(kRrThreads, sizeof(T) == 4 ? 4 : 2) void k_residual
     const T* const src[1] = {x};
 
     // Two fine positions per thread (rows ty and ty + kRrTY of the tile).
+    // Planes are double-buffered in registers (buf[p & 1] for plane p, with
+    // the plane counted from p0): plane p + 1 is loaded while plane p is
+    // reduced, and the +z value x(p + 1) that plane p's residual needs was
+    // loaded one plane earlier, so no load is waited for in the plane it is
+    // issued.
     struct Cell {
         int ly;
         bool ok;  // inside the fine grid
         StCol h;
         idx_t c0;  // index in plane 0
-        StPlane<T, T, 1> cur, nxt;
-        T bc, bn, xm, fm;
+        StPlane<T, T, 1> buf[2];
+        T bb[2], xz[2];
+        T xm, fm;
     };
     Cell C[2];
     // Fine planes of this chunk: the taps of coarse planes K0 .. K1-1.
     const int p0 = max(2 * K0 - 1, 0), p1 = min(2 * K1, d.nz - 1);
+    // Plane q of this chunk (clamped to p1: the load past the chunk end reads
+    // cached data and is never used); the +z centre value of plane q.
+    auto load_plane = [&](Cell& c, int slot, int q) __attribute__((always_inline)) {
+        const int qc = min(q, p1);
+        const idx_t cc = c.c0 + (c.ok ? sz * qc : 0);
+        st_load(c.buf[slot], A, src, cc, cc, c.h, d.nx);
+        c.bb[slot] = b[cc];
+    };
+    auto load_xz = [&](Cell& c, int slot, int q) __attribute__((always_inline)) {
+        // x at plane q + 1 (0 past the grid), for plane q's residual
+        const bool zp = c.ok && q + 1 < d.nz;
+        c.xz[slot] = x[zp ? c.c0 + sz * (q + 1) : 0];
+    };
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
         Cell& c = C[r];
@@ -67,9 +86,9 @@ __global__ __launch_bounds__(kRrThreads, sizeof(T) == 4 ? 4 : 2) void k_residual
         c.h = c.ok ? StCol{gx > 0, gx + 1 < d.nx, gy > 0, gy + 1 < d.ny}
                    : StCol{false, false, false, false};
         c.c0 = c.ok ? gx + (idx_t)d.nx * gy : 0;  // outside: cell 0, result unused
+        load_plane(c, 0, p0);
+        load_xz(c, 0, p0);
         const idx_t cc = c.c0 + (c.ok ? sz * p0 : 0);
-        st_load(c.cur, A, src, cc, cc, c.h, d.nx);
-        c.bc = b[cc];
         c.xm = (c.ok && p0 > 0) ? x[cc - sz] : T(0);
         c.fm = (c.ok && p0 > 0) ? A[cc - sz].f : T(0);
     }
@@ -87,56 +106,56 @@ __global__ __launch_bounds__(kRrThreads, sizeof(T) == 4 ? 4 : 2) void k_residual
     T w0 = 0, w1 = 0, w2 = 0, w3 = 0;  // x-y restricted planes p-3 .. p
 
     int K = K0;
-    for (int p = p0; p <= p1; ++p) {
-        const bool zp = p + 1 < d.nz;
+    for (int p2 = p0; p2 <= p1; p2 += 2) {
 #pragma unroll
-        for (int r = 0; r < 2; ++r) {
-            Cell& c = C[r];
-            // plane p + 1: the full plane if it is ours, else only its centre
-            const idx_t cc = c.c0 + (c.ok ? sz * p : 0);
-            const idx_t cn = c.ok && zp ? cc + sz : cc;
-            st_load(c.nxt, A, src, p + 1 <= p1 ? cn : cc, cn, c.h, d.nx);
-            c.bn = b[p + 1 <= p1 ? cn : cc];
-            const T xp = (c.ok && zp) ? c.nxt.v[0][0] : T(0);
-            // Jet's residual operand order (src/jet/fdm_linear_system3.cpp:107-115)
-            const StPlane<T, T, 1>& q = c.cur;
-            T rv = c.bc - q.row.c * q.v[0][0];
-            rv -= c.h.l ? q.rl * q.v[0][1] : T(0);
-            rv -= c.h.r ? q.row.r * q.v[0][2] : T(0);
-            rv -= c.h.d ? q.ud * q.v[0][3] : T(0);
-            rv -= c.h.u ? q.row.u * q.v[0][4] : T(0);
-            rv -= c.fm * c.xm;
-            rv -= q.row.f * xp;
-            sres[p & 1][c.ly][tx] = c.ok ? rv : T(0);
-            c.xm = q.v[0][0];
-            c.fm = q.row.f;
-            c.cur = c.nxt;
-            c.bc = c.bn;
-        }
-        __syncthreads();
-        // x-y restriction of plane p for this thread's coarse column (the
-        // buffer is rewritten two planes later, after the next barrier).
-        if (e < kRrCX * kRrCY) {
-            const T(*R)[kRrLX] = sres[p & 1];
-            auto rowsum = [&](int y) {
-                return (wa * R[y][i0] + wb * R[y][li + 1]) + (wb * R[y][li + 2] + wa * R[y][i3]);
-            };
-            const T P = (wa * rowsum(j0) + wb * rowsum(lj + 1)) + (wb * rowsum(lj + 2) + wa * rowsum(j3));
-            w0 = w1;
-            w1 = w2;
-            w2 = w3;
-            w3 = P;
-        }
-        // coarse plane K once its last tap plane is in the window
-        if (K < K1 && p == min(2 * K + 2, d.nz - 1)) {
-            // z taps 2K-1 .. 2K+2 (clamped) as window slots q - (p - 3)
-            const int k0 = (K > 0 ? 2 * K - 1 : 2 * K) - (p - 3);
-            const int k1 = 2 * K - (p - 3), k2 = 2 * K + 1 - (p - 3);
-            const int k3 = (K + 1 < cd.nz ? 2 * K + 2 : 2 * K + 1) - (p - 3);
-            auto win = [&](int q) { return q == 0 ? w0 : (q == 1 ? w1 : (q == 2 ? w2 : w3)); };
-            const T v = (wa * win(k0) + wb * win(k1)) + (wb * win(k2) + wa * win(k3));
-            if (cact) cb[I + (idx_t)cd.nx * (J + (idx_t)cd.ny * K)] = v;
-            ++K;
+        for (int u = 0; u < 2; ++u) {
+            const int p = p2 + u;
+            if (p > p1) break;
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                Cell& c = C[r];
+                // loads of plane p + 1 into the other slot
+                load_plane(c, u ^ 1, p + 1);
+                load_xz(c, u ^ 1, p + 1);
+                const T xp = c.ok && p + 1 < d.nz ? c.xz[u] : T(0);
+                // Jet's residual operand order (src/jet/fdm_linear_system3.cpp:107-115)
+                const StPlane<T, T, 1>& q = c.buf[u];
+                T rv = c.bb[u] - q.row.c * q.v[0][0];
+                rv -= c.h.l ? q.rl * q.v[0][1] : T(0);
+                rv -= c.h.r ? q.row.r * q.v[0][2] : T(0);
+                rv -= c.h.d ? q.ud * q.v[0][3] : T(0);
+                rv -= c.h.u ? q.row.u * q.v[0][4] : T(0);
+                rv -= c.fm * c.xm;
+                rv -= q.row.f * xp;
+                sres[p & 1][c.ly][tx] = c.ok ? rv : T(0);
+                c.xm = q.v[0][0];
+                c.fm = q.row.f;
+            }
+            __syncthreads();
+            // x-y restriction of plane p for this thread's coarse column (the
+            // buffer is rewritten two planes later, after the next barrier).
+            if (e < kRrCX * kRrCY) {
+                const T(*R)[kRrLX] = sres[p & 1];
+                auto rowsum = [&](int y) {
+                    return (wa * R[y][i0] + wb * R[y][li + 1]) + (wb * R[y][li + 2] + wa * R[y][i3]);
+                };
+                const T P = (wa * rowsum(j0) + wb * rowsum(lj + 1)) + (wb * rowsum(lj + 2) + wa * rowsum(j3));
+                w0 = w1;
+                w1 = w2;
+                w2 = w3;
+                w3 = P;
+            }
+            // coarse plane K once its last tap plane is in the window
+            if (K < K1 && p == min(2 * K + 2, d.nz - 1)) {
+                // z taps 2K-1 .. 2K+2 (clamped) as window slots q - (p - 3)
+                const int k0 = (K > 0 ? 2 * K - 1 : 2 * K) - (p - 3);
+                const int k1 = 2 * K - (p - 3), k2 = 2 * K + 1 - (p - 3);
+                const int k3 = (K + 1 < cd.nz ? 2 * K + 2 : 2 * K + 1) - (p - 3);
+                auto win = [&](int q) { return q == 0 ? w0 : (q == 1 ? w1 : (q == 2 ? w2 : w3)); };
+                const T v = (wa * win(k0) + wb * win(k1)) + (wb * win(k2) + wa * win(k3));
+                if (cact) cb[I + (idx_t)cd.nx * (J + (idx_t)cd.ny * K)] = v;
+                ++K;
+            }
         }
     }
 }
