@@ -1,0 +1,173 @@
+// fdm_gpu_solvers3.cpp -- Jet solver classes over the C ABI of jetgpu.h.
+//
+// Compiled as part of a Jet build (include/jet on the include path, linked
+// against libjetgpu.so).  Jet's FdmMatrixRow3 {center, right, up, front} and
+// x-fastest Array3 are the library's layout, so systems are handed over as
+// plain pointers without conversion.
+#include "fdm_gpu_solvers3.h"
+
+#include <jet/fdm_linear_system3.h>
+#include <jet/fdm_mg_linear_system3.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace jet {
+
+namespace {
+
+static_assert(sizeof(FdmMatrixRow3) == 4 * sizeof(double),
+              "FdmMatrixRow3 must be four packed doubles");
+
+// The library's status -> Jet's error convention.
+void check(int status, const jg_solver* s, const char* what) {
+    if (status == JG_OK) return;
+    std::string msg = std::string(what) + ": " + (s ? jg_last_error(s) : "");
+    if (status == JG_ERR_INVALID_ARG) throw std::invalid_argument(msg);
+    throw std::runtime_error(msg);
+}
+
+jg_solver* lazySolver(jg_solver** slot, int device) {
+    if (!*slot) {
+        const int st = jg_solver_create(device, slot);
+        if (st != JG_OK) {
+            *slot = nullptr;
+            throw std::runtime_error("jetgpu: no usable GPU device " +
+                                     std::to_string(device) + " (status " +
+                                     std::to_string(st) + ")");
+        }
+    }
+    return *slot;
+}
+
+bool sameSize(const Size3& a, const Size3& b) {
+    return a.x == b.x && a.y == b.y && a.z == b.z;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ CG
+
+FdmGpuCgSolver3::FdmGpuCgSolver3(unsigned int maxNumberOfIterations,
+                                 double tolerance, int device)
+    : _maxNumberOfIterations(maxNumberOfIterations),
+      _tolerance(tolerance),
+      _device(device) {
+    _info.last_residual = kMaxD;
+}
+
+FdmGpuCgSolver3::~FdmGpuCgSolver3() { jg_solver_destroy(_solver); }
+
+bool FdmGpuCgSolver3::solve(FdmLinearSystem3* system) {
+    JET_THROW_INVALID_ARG_IF(system == nullptr);
+    const Size3 n = system->A.size();
+    JET_THROW_INVALID_ARG_IF(!sameSize(n, system->b.size()) ||
+                             !sameSize(n, system->x.size()));
+    if (n.x * n.y * n.z == 0) {
+        _info = jg_solve_info{};
+        return true;
+    }
+    jg_solver* s = lazySolver(&_solver, _device);
+    check(jg_cg_solve_host(s, n.x, n.y, n.z,
+                           reinterpret_cast<const double*>(system->A.data()),
+                           system->b.data(), system->x.data(),
+                           _maxNumberOfIterations, _tolerance, &_info),
+          s, "FdmGpuCgSolver3::solve");
+    return _info.converged != 0;
+}
+
+unsigned int FdmGpuCgSolver3::maxNumberOfIterations() const {
+    return _maxNumberOfIterations;
+}
+unsigned int FdmGpuCgSolver3::lastNumberOfIterations() const {
+    return _info.iterations;
+}
+double FdmGpuCgSolver3::tolerance() const { return _tolerance; }
+double FdmGpuCgSolver3::lastResidual() const { return _info.last_residual; }
+const jg_solve_info& FdmGpuCgSolver3::lastSolveInfo() const { return _info; }
+
+// --------------------------------------------------------- MGPCG
+
+FdmGpuMgpcgSolver3::FdmGpuMgpcgSolver3(
+    unsigned int numberOfCgIter, size_t maxNumberOfLevels,
+    unsigned int numberOfRestrictionIter, unsigned int numberOfCorrectionIter,
+    unsigned int numberOfCoarsestIter, unsigned int numberOfFinalIter,
+    double maxTolerance, double sorFactor, bool useRedBlackOrdering, int device)
+    : FdmMgSolver3(maxNumberOfLevels, numberOfRestrictionIter,
+                   numberOfCorrectionIter, numberOfCoarsestIter,
+                   numberOfFinalIter, maxTolerance, sorFactor,
+                   useRedBlackOrdering),
+      _maxNumberOfIterations(numberOfCgIter),
+      _tolerance(maxTolerance),
+      _device(device) {
+    jg_gpu_options_default(&_options);
+    _info.last_residual = kMaxD;
+}
+
+FdmGpuMgpcgSolver3::~FdmGpuMgpcgSolver3() { jg_solver_destroy(_solver); }
+
+bool FdmGpuMgpcgSolver3::solve(FdmMgLinearSystem3* system) {
+    JET_THROW_INVALID_ARG_IF(system == nullptr);
+    const size_t L = system->A.levels.size();
+    JET_THROW_INVALID_ARG_IF(L == 0 || system->x.levels.empty() ||
+                             system->b.levels.empty());
+    const Size3 n0 = system->A.levels.front().size();
+    JET_THROW_INVALID_ARG_IF(
+        !sameSize(n0, system->x.levels.front().size()) ||
+        !sameSize(n0, system->b.levels.front().size()));
+    if (n0.x * n0.y * n0.z == 0) {
+        _info = jg_solve_info{};
+        return true;
+    }
+    std::vector<size_t> nx(L), ny(L), nz(L);
+    std::vector<const double*> rows(L);
+    for (size_t l = 0; l < L; ++l) {
+        const Size3 n = system->A.levels[l].size();
+        nx[l] = n.x;
+        ny[l] = n.y;
+        nz[l] = n.z;
+        rows[l] = reinterpret_cast<const double*>(system->A.levels[l].data());
+    }
+
+    // Jet's MgParameters and smoother factor, as FdmMgpcgSolver3 uses them
+    jg_mg_params p;
+    jg_mg_params_default(&p);
+    const MgParameters<FdmBlas3>& mg = params();
+    p.max_levels = mg.maxNumberOfLevels;
+    p.restrict_iter = mg.numberOfRestrictionIter;
+    p.correct_iter = mg.numberOfCorrectionIter;
+    p.coarsest_iter = mg.numberOfCoarsestIter;
+    p.final_iter = mg.numberOfFinalIter;
+    p.max_tol = _tolerance;
+    p.sor = sorFactor();
+    p.red_black = useRedBlackOrdering() ? 1 : 0;
+
+    jg_solver* s = lazySolver(&_solver, _device);
+    check(jg_mgpcg_solve_host(s, L, nx.data(), ny.data(), nz.data(),
+                              rows.data(), system->b.levels.front().data(),
+                              system->x.levels.front().data(),
+                              _maxNumberOfIterations, &p, &_options, &_info),
+          s, "FdmGpuMgpcgSolver3::solve");
+    return _info.converged != 0;
+}
+
+unsigned int FdmGpuMgpcgSolver3::maxNumberOfIterations() const {
+    return _maxNumberOfIterations;
+}
+unsigned int FdmGpuMgpcgSolver3::lastNumberOfIterations() const {
+    return _info.iterations;
+}
+double FdmGpuMgpcgSolver3::tolerance() const { return _tolerance; }
+double FdmGpuMgpcgSolver3::lastResidual() const { return _info.last_residual; }
+void FdmGpuMgpcgSolver3::setGpuOptions(const jg_gpu_options& options) {
+    _options = options;
+}
+const jg_gpu_options& FdmGpuMgpcgSolver3::gpuOptions() const {
+    return _options;
+}
+const jg_solve_info& FdmGpuMgpcgSolver3::lastSolveInfo() const {
+    return _info;
+}
+
+}  // namespace jet
