@@ -47,6 +47,10 @@ struct Level {
     // sweep kind (kSweepPlain, kSweepZero, kSweepCorrect).
     int zc64[3] = {0, 0, 0}, zc32[3] = {0, 0, 0};
     int sh64[3] = {0, 0, 0}, sh32[3] = {0, 0, 0};
+    // Two-sweep kernel (FP32): tuned z-chunk per kind, and whether one launch
+    // of it beats two single sweeps for that kind.
+    int zc2_32[3] = {0, 0, 0};
+    bool pair32[3] = {false, false, false};
 };
 
 // Sweep kinds tuned separately: they differ in registers and loads.
@@ -63,6 +67,8 @@ struct Prec<double> {
     static double* y(Level& l) { return l.y64; }
     static int& zc(Level& l, int kind = kSweepPlain) { return l.zc64[kind]; }
     static int& sh(Level& l, int kind = kSweepPlain) { return l.sh64[kind]; }
+    static bool pair(const Level&, int) { return false; }  // no FP64 two-sweep kernel
+    static int zc2(const Level&, int) { return 0; }
 };
 template <>
 struct Prec<float> {
@@ -73,6 +79,8 @@ struct Prec<float> {
     static float* y(Level& l) { return l.y32; }
     static int& zc(Level& l, int kind = kSweepPlain) { return l.zc32[kind]; }
     static int& sh(Level& l, int kind = kSweepPlain) { return l.sh32[kind]; }
+    static bool pair(const Level& l, int kind) { return l.pair32[kind]; }
+    static int zc2(const Level& l, int kind) { return l.zc2_32[kind]; }
 };
 
 struct Geo {

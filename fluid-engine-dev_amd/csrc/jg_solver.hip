@@ -16,6 +16,7 @@
 #include "jg_internal.h"
 #include "jg_kernels.h"
 #include "jg_sweep.h"
+#include "jg_sweep2.h"
 #include "jg_bottom.h"
 #include "jg_transfer.h"
 
@@ -61,7 +62,9 @@ int sweep_zc(const Dims& d, int tuned) {
 // (0 = red, forward; 1 = black, backward).  xin == nullptr: x starts at zero.
 // coarse != nullptr: add the prolongation of coarse to x on load.
 // p_bx/p_bb != nullptr: emit b.x and b.b partials.  shape: index into
-// kSweepShapes.  Returns the block count.
+// kSweepShapes.  Returns the number of b.x / b.b partials written (the
+// block count; 0 without partials, or when the grid has more blocks than a
+// slot holds -- the caller then reduces with k_dot).
 template <typename T, int RP, int PD>
 void launch_sweep_shape(jg_solver* s, int code, dim3 grid, const Row<T>* A, const T* b,
                         const T* xin, T* xout, const Dims& d, int zc, T omega, const T* coarse,
@@ -98,8 +101,12 @@ template <typename T>
 int launch_sweep(jg_solver* s, const Row<T>* A, const T* b, const T* xin, T* xout, const Dims& d,
                  int first, const T* coarse, double* p_bx, double* p_bb, T omega, const Ctl* ctl,
                  int tuned_zc = 0, int shape = 0) {
-    const int zc = sweep_zc(d, tuned_zc);
+    int zc = sweep_zc(d, tuned_zc);
+    const long cols = (long)((d.nx + kSwIX - 1) / kSwIX) * ((d.ny + kSwIY - 1) / kSwIY);
+    // one b.x / b.b partial per block: keep the block count within a slot
+    while (p_bx && zc < d.nz && cols * ((d.nz + zc - 1) / zc) > kMaxPartials) ++zc;
     dim3 grid((d.nx + kSwIX - 1) / kSwIX, (d.ny + kSwIY - 1) / kSwIY, (d.nz + zc - 1) / zc);
+    if ((long)grid.x * grid.y * grid.z > kMaxPartials) p_bx = p_bb = nullptr;  // caller uses k_dot
     const int code = (first ? 8 : 0) | (xin ? 0 : 4) | (coarse ? 2 : 0) | (p_bx ? 1 : 0);
     switch (shape) {
         case 1:
@@ -118,7 +125,70 @@ int launch_sweep(jg_solver* s, const Row<T>* A, const T* b, const T* xin, T* xou
             launch_sweep_shape<T, 2, 1>(s, code, grid, A, b, xin, xout, d, zc, omega, coarse, p_bx,
                                         p_bb, ctl);
     }
-    return (int)(grid.x * grid.y * grid.z);
+    return p_bx ? (int)(grid.x * grid.y * grid.z) : 0;
+}
+
+// Two sweeps in one launch (jg_sweep2.h, FP32 only); same arguments and
+// return value as launch_sweep.
+template <typename T>
+int launch_sweep2(jg_solver* s, const Row<T>* A, const T* b, const T* xin, T* xout, const Dims& d,
+                  int first, const T* coarse, double* p_bx, double* p_bb, T omega, const Ctl* ctl,
+                  int tuned_zc) {
+    if constexpr (sizeof(T) != 4) {
+        return 0;
+    } else {
+        int zc = std::min(tuned_zc > 0 ? tuned_zc : 32, d.nz);
+        const long cols = (long)((d.nx + kS2IX - 1) / kS2IX) * ((d.ny + kS2IY - 1) / kS2IY);
+        while (p_bx && zc < d.nz && cols * ((d.nz + zc - 1) / zc) > kMaxPartials) ++zc;
+        dim3 grid((d.nx + kS2IX - 1) / kS2IX, (d.ny + kS2IY - 1) / kS2IY, (d.nz + zc - 1) / zc);
+        if ((long)grid.x * grid.y * grid.z > kMaxPartials) p_bx = p_bb = nullptr;
+        const dim3 block(kS2LX, kS2TY, 1);
+        const int code = (first ? 8 : 0) | (xin ? 0 : 4) | (coarse ? 2 : 0) | (p_bx ? 1 : 0);
+#define JG_SWEEP2_CASE(C, F, Z, K, D)                                                     \
+    case C:                                                                               \
+        k_rb_sweep2<T, F, Z, K, D><<<grid, block, 0, s->stream>>>(A, b, xin, xout, d, zc, \
+                                                                  omega, coarse, p_bx,    \
+                                                                  p_bb, ctl);             \
+        break;
+        switch (code) {
+            JG_SWEEP2_CASE(0, 0, false, false, false)
+            JG_SWEEP2_CASE(1, 0, false, false, true)
+            JG_SWEEP2_CASE(2, 0, false, true, false)
+            JG_SWEEP2_CASE(3, 0, false, true, true)
+            JG_SWEEP2_CASE(4, 0, true, false, false)
+            JG_SWEEP2_CASE(5, 0, true, false, true)
+            JG_SWEEP2_CASE(6, 0, true, true, false)
+            JG_SWEEP2_CASE(7, 0, true, true, true)
+            JG_SWEEP2_CASE(8, 1, false, false, false)
+            JG_SWEEP2_CASE(9, 1, false, false, true)
+            JG_SWEEP2_CASE(10, 1, false, true, false)
+            JG_SWEEP2_CASE(11, 1, false, true, true)
+            JG_SWEEP2_CASE(12, 1, true, false, false)
+            JG_SWEEP2_CASE(13, 1, true, false, true)
+            JG_SWEEP2_CASE(14, 1, true, true, false)
+            JG_SWEEP2_CASE(15, 1, true, true, true)
+        }
+#undef JG_SWEEP2_CASE
+        return p_bx ? (int)(grid.x * grid.y * grid.z) : 0;
+    }
+}
+
+// Sweeps of one smoothing step, grouped into launches: a launch runs two
+// sweeps (k_rb_sweep2, FP32) when the tuner found that faster for the kind of
+// its first sweep (kind_first for the first launch, plain afterwards);
+// pairs > 0 forces two-sweep launches wherever two sweeps follow, pairs < 0
+// forbids them.  Returns the sweep count of each launch.
+template <typename T>
+std::vector<int> sweep_plan(const Level& lv, unsigned n, int kind_first, int pairs) {
+    std::vector<int> v;
+    for (unsigned k = 0; k < n;) {
+        const bool tuned = Prec<T>::pair(lv, k == 0 ? kind_first : kSweepPlain);
+        const bool two = sizeof(T) == 4 && n - k >= 2 && lv.d.n() > BotCfg<T>::MaxCells &&
+                         (pairs > 0 || (pairs == 0 && tuned));
+        v.push_back(two ? 2 : 1);
+        k += two ? 2 : 1;
+    }
+    return v;
 }
 
 // Coarse planes per block of the fused residual-restriction: enough blocks
@@ -134,6 +204,7 @@ template <typename T>
 struct MgCfg {
     unsigned pre, post0, post, coarsest;
     T omega;
+    int pairs = 0;  // jg_gpu_options.sweep_pairs
 };
 
 // z = M r: one symmetric V-cycle with zero initial guess.  Level 0 reads
@@ -200,14 +271,20 @@ int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, 
             break;
         }
         const unsigned post = l == 0 ? c.post0 : c.post;
-        const unsigned S = c.pre + post;
-        auto buf = [&](unsigned k) -> T* { return ((S - 1 - k) % 2 == 0) ? X(l) : Y; };
+        // launches of the level ping-pong between x and y so the last lands in x
+        const std::vector<int> pre_plan = sweep_plan<T>(lv, c.pre, kSweepZero, c.pairs);
+        const size_t M = pre_plan.size() + sweep_plan<T>(lv, post, kSweepCorrect, c.pairs).size();
+        auto buf = [&](size_t m) -> T* { return ((M - 1 - m) % 2 == 0) ? X(l) : Y; };
         const T* in = nullptr;
-        for (unsigned k = 0; k < c.pre; ++k) {
+        for (size_t m = 0; m < pre_plan.size(); ++m) {
             const int kind = in ? kSweepPlain : kSweepZero;
-            launch_sweep<T>(s, A, B(l), in, buf(k), d, 0, nullptr, nullptr, nullptr, c.omega, ctl,
-                            Prec<T>::zc(lv, kind), Prec<T>::sh(lv, kind));
-            in = buf(k);
+            if (pre_plan[m] == 2)
+                launch_sweep2<T>(s, A, B(l), in, buf(m), d, 0, nullptr, nullptr, nullptr, c.omega, ctl,
+                                 Prec<T>::zc2(lv, kind));
+            else
+                launch_sweep<T>(s, A, B(l), in, buf(m), d, 0, nullptr, nullptr, nullptr, c.omega, ctl,
+                                Prec<T>::zc(lv, kind), Prec<T>::sh(lv, kind));
+            in = buf(m);
         }
         pre_out[l] = in;  // nullptr: x is still zero
         const Dims& cd = s->lv[l + 1].d;
@@ -238,8 +315,10 @@ int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, 
         const Dims& d = lv.d;
         T* Y = Prec<T>::y(lv);
         const unsigned post = l == 0 ? c.post0 : c.post;
-        const unsigned S = c.pre + post;
-        auto buf = [&](unsigned k) -> T* { return ((S - 1 - k) % 2 == 0) ? X(l) : Y; };
+        const size_t P0 = sweep_plan<T>(lv, c.pre, kSweepZero, c.pairs).size();
+        const std::vector<int> post_plan = sweep_plan<T>(lv, post, kSweepCorrect, c.pairs);
+        const size_t M = P0 + post_plan.size();
+        auto buf = [&](size_t m) -> T* { return ((M - 1 - m) % 2 == 0) ? X(l) : Y; };
         const T* coarse = X(l + 1);
         if (post == 0) {
             // No post-smoothing: x = pre-smoothed x (already in X) + P e.
@@ -248,15 +327,21 @@ int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, 
             k_correct<T><<<gf.grid, gf.block, 0, s->stream>>>(coarse, X(l), d, ctl);
         } else {
             const T* in = pre_out[l];
-            for (unsigned k = c.pre; k < S; ++k) {
-                const bool last0 = (l == 0 && k + 1 == S);
-                const int kind = k == c.pre ? kSweepCorrect : kSweepPlain;
-                const int nb = launch_sweep<T>(s, A, B(l), in, buf(k), d, 1, k == c.pre ? coarse : nullptr,
-                                               last0 ? s->slot(0) : nullptr,
-                                               last0 ? s->slot(1) : nullptr, c.omega, ctl,
-                                               Prec<T>::zc(lv, kind), Prec<T>::sh(lv, kind));
+            for (size_t q = 0; q < post_plan.size(); ++q) {
+                const size_t m = P0 + q;
+                const bool last0 = (l == 0 && m + 1 == M);
+                const int kind = q == 0 ? kSweepCorrect : kSweepPlain;
+                const T* cor = q == 0 ? coarse : nullptr;
+                double* pbx = last0 ? s->slot(0) : nullptr;
+                double* pbb = last0 ? s->slot(1) : nullptr;
+                const int nb =
+                    post_plan[q] == 2
+                        ? launch_sweep2<T>(s, A, B(l), in, buf(m), d, 1, cor, pbx, pbb, c.omega, ctl,
+                                           Prec<T>::zc2(lv, kind))
+                        : launch_sweep<T>(s, A, B(l), in, buf(m), d, 1, cor, pbx, pbb, c.omega, ctl,
+                                          Prec<T>::zc(lv, kind), Prec<T>::sh(lv, kind));
                 if (last0) *dot_blocks = nb;
-                in = buf(k);
+                in = buf(m);
             }
         }
         JG_TRY(check_launch(s, "vcycle up"));
@@ -292,6 +377,7 @@ int autotune_sweeps(jg_solver* s) {
             k_fill<T><<<nblocks_1d(cn), NT, 0, s->stream>>>(Prec<T>::x(s->lv[l + 1]), T(0.25), cn);
         }
         const int cands[] = {2, 4, 8, 12, 16, 24, 32, 48, 64};
+        float tbest[kNumSweepKinds] = {1e30f, 1e30f, 1e30f};
         for (int kind = 0; kind < kNumSweepKinds; ++kind) {
             if (kind == kSweepCorrect && !coarse) {
                 Prec<T>::zc(lv, kind) = Prec<T>::zc(lv, kSweepPlain);
@@ -330,6 +416,43 @@ int autotune_sweeps(jg_solver* s) {
             }
             Prec<T>::sh(lv, kind) = best_sh;
             Prec<T>::zc(lv, kind) = best_zc;
+            tbest[kind] = best;
+        }
+        if constexpr (sizeof(T) == 4) {
+            // two-sweep kernel: a launch of it replaces a sweep of this kind
+            // followed by a plain sweep
+            const int cands2[] = {8, 12, 16, 24, 32, 48, 64};
+            for (int kind = 0; kind < kNumSweepKinds; ++kind) {
+                lv.pair32[kind] = false;
+                if (kind == kSweepCorrect && !coarse) continue;
+                float best = 1e30f;
+                int best_zc = 32;
+                for (int zc : cands2) {
+                    if (zc > d.nz && zc != cands2[0]) break;
+                    auto run = [&]() {
+                        launch_sweep2<T>(s, Prec<T>::A(lv), Prec<T>::t(lv),
+                                         kind == kSweepZero ? nullptr : Prec<T>::y(lv), Prec<T>::x(lv),
+                                         d, 1, kind == kSweepCorrect ? coarse : nullptr, nullptr,
+                                         nullptr, T(1), nullptr, zc);
+                    };
+                    run();
+                    JG_HIP(hipEventRecord(s->ev0, s->stream));
+                    for (int r = 0; r < 3; ++r) run();
+                    JG_HIP(hipEventRecord(s->ev1, s->stream));
+                    JG_TRY(sync(s));
+                    float ms = 0.f;
+                    JG_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+                    if (tune_log())
+                        std::fprintf(stderr, "jetgpu tune: %dx%dx%d f32 kind %d two-sweep zc %d: %.1f us\n",
+                                     d.nx, d.ny, d.nz, kind, zc, ms * 1e3f / 3);
+                    if (ms < best) {
+                        best = ms;
+                        best_zc = zc;
+                    }
+                }
+                lv.zc2_32[kind] = best_zc;
+                lv.pair32[kind] = best < tbest[kind] + tbest[kSweepPlain];
+            }
         }
     }
     return check_launch(s, "autotune");
@@ -702,12 +825,14 @@ int jg_mg_vcycle(jg_solver* s, const jg_mg_params* p, const jg_gpu_options* o, c
         JG_TRY(autotune_sweeps<float>(s));
         MgCfg<float> mg{p->restrict_iter, p->final_iter, p->correct_iter, p->coarsest_iter,
                         (float)omega};
+        mg.pairs = opt.sweep_pairs;
         k_convert<double, float><<<nb1, NT, 0, s->stream>>>(r, s->r32, n);
         JG_TRY(vcycle<float>(s, s->r32, s->z32, mg, nullptr, &nb));
         k_convert<float, double><<<nb1, NT, 0, s->stream>>>(s->z32, z, n);
     } else {
         JG_TRY(autotune_sweeps<double>(s));
         MgCfg<double> mg{p->restrict_iter, p->final_iter, p->correct_iter, p->coarsest_iter, omega};
+        mg.pairs = opt.sweep_pairs;
         JG_TRY(vcycle<double>(s, r, z, mg, nullptr, &nb));
     }
     JG_TRY(check_launch(s, "vcycle"));
@@ -737,6 +862,7 @@ int jg_solver_mgpcg(jg_solver* s, unsigned max_iter, const jg_mg_params* p,
     JG_HIP(hipEventRecord(s->ev0, s->stream));
     if (opt.precision != JG_MIXED) {
         MgCfg<double> mg{p->restrict_iter, p->final_iter, p->correct_iter, p->coarsest_iter, omega};
+        mg.pairs = opt.sweep_pairs;
         Ctl h;
         init_ctl(&h, max_iter, p->max_tol, opt.stop_rule == JG_STOP_RELATIVE ? 1 : 0, opt.rel_tol);
         JG_TRY(upload_ctl(s, ctl, h));
@@ -753,6 +879,7 @@ int jg_solver_mgpcg(jg_solver* s, unsigned max_iter, const jg_mg_params* p,
         // (FP32 stencil, smoother and V-cycle; FP64 dot accumulation).
         MgCfg<float> mg{p->restrict_iter, p->final_iter, p->correct_iter, p->coarsest_iter,
                         (float)omega};
+        mg.pairs = opt.sweep_pairs;
         Geo g = geo_stencil(l0.d);
         const int nb1 = nblocks_1d(n);
         k_dot<double, double><<<nb1, NT, 0, s->stream>>>(s->b, s->b, n, s->slot(3), nullptr);
@@ -1091,6 +1218,7 @@ struct jg_batch {
         int precision;
         unsigned pre, post0, post, coarsest;
         double omega;
+        int pairs;
         hipGraph_t graph;
         hipGraphExec_t exec;
     };
@@ -1171,7 +1299,7 @@ template <typename T>
 int batch_graph(jg_batch* B, int precision, const MgCfg<T>& mg, hipGraphExec_t* out) {
     for (auto& g : B->graphs)
         if (g.precision == precision && g.pre == mg.pre && g.post0 == mg.post0 && g.post == mg.post &&
-            g.coarsest == mg.coarsest && g.omega == (double)mg.omega) {
+            g.coarsest == mg.coarsest && g.omega == (double)mg.omega && g.pairs == mg.pairs) {
             *out = g.exec;
             return JG_OK;
         }
@@ -1203,7 +1331,8 @@ int batch_graph(jg_batch* B, int precision, const MgCfg<T>& mg, hipGraphExec_t* 
     }
     hipGraphExec_t exec = nullptr;
     JG_HIPB(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    B->graphs.push_back({precision, mg.pre, mg.post0, mg.post, mg.coarsest, (double)mg.omega, graph, exec});
+    B->graphs.push_back(
+        {precision, mg.pre, mg.post0, mg.post, mg.coarsest, (double)mg.omega, mg.pairs, graph, exec});
     *out = exec;
     return JG_OK;
 }
@@ -1355,6 +1484,7 @@ int jg_batch_mgpcg(jg_batch* B, unsigned max_iter, const jg_mg_params* p, const 
     JG_HIPB(hipEventRecord(B->ev0, B->stream));
     if (!mixed) {
         MgCfg<double> mg{p->restrict_iter, p->final_iter, p->correct_iter, p->coarsest_iter, omega};
+        mg.pairs = opt.sweep_pairs;
         hipGraphExec_t exec = nullptr;
         JG_TRY(batch_graph<double>(B, JG_FP64, mg, &exec));
         for (auto* s : B->sys) {
@@ -1375,6 +1505,7 @@ int jg_batch_mgpcg(jg_batch* B, unsigned max_iter, const jg_mg_params* p, const 
         }
     } else {
         MgCfg<float> mg{p->restrict_iter, p->final_iter, p->correct_iter, p->coarsest_iter, (float)omega};
+        mg.pairs = opt.sweep_pairs;
         hipGraphExec_t exec = nullptr;
         JG_TRY(batch_graph<float>(B, JG_MIXED, mg, &exec));
         // ||b|| of every system

@@ -24,6 +24,10 @@
 // which Jet's smoothers divide by) is set to zero instead, so the smoother
 // stays a symmetric, finite operator.
 //
+// FP64 colour updates are Jet's expression; FP32 ones (the mixed-precision
+// inner solve) precompute w = omega / c once per cell and compute
+// (1 - omega) x + w (b - s): the same operator, rounded differently.
+//
 // Options: ZERO treats the incoming x as zero (no read of xin); CORRECT adds
 // the trilinear prolongation of a coarse correction to x as it is loaded
 // (the operator of FdmMgUtils3::correct, src/jet/fdm_mg_linear_system3.cpp:
@@ -119,6 +123,10 @@ __global__ __launch_bounds__(kSwLX * kSwLY / RP, sweep_waves_per_eu(sizeof(T), R
     static_assert(PD == 1 || PD == 2, "prefetch depth");
     constexpr int TY = kSwLY / RP;
     constexpr int NTH = kSwLX * TY;
+    // FP32: the c ring holds w = omega / c (0 for a zero-centre row) and the
+    // update is (1 - omega) x + w (b - s), one division per cell and sweep;
+    // FP64 keeps Jet's omega (b - s) / c bit for bit.
+    constexpr bool kWeighted = sizeof(T) == 4;
     if (ctl && ctl->done) return;
     __shared__ __align__(16) T sx[4][kSwLY][kSwLX];  // x, planes t .. t-3 (slot = plane & 3)
     __shared__ T sr[2][kSwLY][kSwLX];                // A.r, planes t-1, t-2 (slot = plane & 1)
@@ -215,6 +223,7 @@ __global__ __launch_bounds__(kSwLX * kSwLY / RP, sweep_waves_per_eu(sizeof(T), R
         Q.c2 = Q.c1;
         Q.c1 = Q.c0;
         Q.c0 = q.ok ? q.row.c : T(1);
+        if (kWeighted) Q.c0 = Q.c0 == T(0) ? T(0) : omega / Q.c0;  // w = omega / c
         Q.f3 = Q.f2;
         Q.f2 = Q.f1;
         Q.f1 = Q.f0;
@@ -250,7 +259,7 @@ __global__ __launch_bounds__(kSwLX * kSwLY / RP, sweep_waves_per_eu(sizeof(T), R
             sx[sp][ly][tx] = T(0);  // zero-centre row (see the file comment)
         } else if (ZERO && !CORRECT) {
             // x is zero everywhere before this colour: (1-w)0 + w(b - 0)/c
-            sx[sp][ly][tx] = omega * (Q.b1 - T(0)) / Q.c1;
+            sx[sp][ly][tx] = kWeighted ? Q.c1 * (Q.b1 - T(0)) : omega * (Q.b1 - T(0)) / Q.c1;
         } else {
             const T xo = sx[sp][ly][tx];
             T s = sr[s1][ly][tx - 1] * sx[sp][ly][tx - 1];
@@ -259,7 +268,8 @@ __global__ __launch_bounds__(kSwLX * kSwLY / RP, sweep_waves_per_eu(sizeof(T), R
             s += su[s1][ly][tx] * sx[sp][ly + 1][tx];
             s += Q.f2 * sx[sm][ly][tx];
             s += Q.f1 * sx[sn][ly][tx];
-            sx[sp][ly][tx] = (T(1) - omega) * xo + omega * (Q.b1 - s) / Q.c1;
+            sx[sp][ly][tx] = kWeighted ? (T(1) - omega) * xo + Q.c1 * (Q.b1 - s)
+                                       : (T(1) - omega) * xo + omega * (Q.b1 - s) / Q.c1;
         }
     };
     double acc = 0.0, acc2 = 0.0;
@@ -281,7 +291,9 @@ __global__ __launch_bounds__(kSwLX * kSwLY / RP, sweep_waves_per_eu(sizeof(T), R
             s += su[s2][ly][tx] * sx[sq][ly + 1][tx];
             s += Q.f3 * sx[sm][ly][tx];
             s += Q.f2 * sx[sn][ly][tx];
-            xv = Q.c2 == T(0) ? T(0) : (T(1) - omega) * xv + omega * (Q.b2 - s) / Q.c2;
+            xv = Q.c2 == T(0)  ? T(0)
+                 : kWeighted ? (T(1) - omega) * xv + Q.c2 * (Q.b2 - s)
+                             : (T(1) - omega) * xv + omega * (Q.b2 - s) / Q.c2;
             sx[sq][ly][tx] = xv;
         }
         Q.px = xv;
@@ -308,19 +320,23 @@ __global__ __launch_bounds__(kSwLX * kSwLY / RP, sweep_waves_per_eu(sizeof(T), R
     // Step t: commit plane t from prefetch slot (t - k0 + 2) % PD and refill
     // that slot with plane t + PD.  The loop is unrolled PD times so every
     // slot index is static: no register moves wait on loads in flight.
-    for (int t0 = k0 - 2; t0 <= k1 + 1; t0 += PD) {
+    // The step count is rounded up to a multiple of PD so the unrolled body
+    // has no exit (which would keep the compiler from counting the loads in
+    // flight); a step past k1 + 1 commits a re-read plane and its colour
+    // stages touch only planes no later stage reads.
+    const int tend = k0 - 2 + (k1 + 4 - k0 + PD - 1) / PD * PD;
+    for (int t0 = k0 - 2; t0 < tend; t0 += PD) {
 #pragma unroll
         for (int u = 0; u < PD; ++u) {
             const int t = t0 + u;
-            if (t > k1 + 1) break;
             // ---- commit plane t (prefetched), then prefetch plane t+PD
 #pragma unroll
             for (int r = 0; r < RP; ++r) commit(P[r], P[r].p[u], t);
             if (CORRECT && t + 1 <= k1 + 1) zfetch(t + 1);
-            if (t + PD <= k1 + 1) {
+            // unconditional (past the end it re-reads plane k1+1, unused) so
+            // the compiler can count the loads in flight
 #pragma unroll
-                for (int r = 0; r < RP; ++r) fetch(P[r].p[u], P[r].gy, t + PD);
-            }
+            for (int r = 0; r < RP; ++r) fetch(P[r].p[u], P[r].gy, min(t + PD, k1 + 1));
             // the final values of plane t - 3 (second colour of the last step)
 #pragma unroll
             for (int r = 0; r < RP; ++r) flush(P[r], t - 3);

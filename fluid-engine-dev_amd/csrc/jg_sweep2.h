@@ -1,0 +1,321 @@
+// jg_sweep2.h -- two red-black sweeps in one pass over HBM (temporal
+// blocking of the fused sweep in jg_sweep.h).
+//
+// Two consecutive sweeps with the same colour order are four colour
+// half-sweeps h = 1..4 (colours c1, c2, c1, c2).  A block owns a 64 x 32 tile
+// of (i, j) positions -- a 56 x 24 interior plus a 4-cell halo that is
+// recomputed, not exchanged -- and marches through a chunk of z planes.  At
+// step t it commits plane t from the registers it was prefetched into
+// (prefetching plane t + 2), then runs h1 on plane t-1, h2 on t-2, h3 on t-3
+// and h4 on t-4, each after a barrier; plane t-4 is then final and written
+// back.  Half-sweep h is valid on positions at least h cells inside the tile
+// (it reads neighbours updated by h-1, valid one cell further out), so the
+// interior (ring >= 4) of plane t-4 holds exactly the values two full
+// sweeps give.  Each block also recomputes the 4 planes below and above its
+// chunk.
+//
+// x lives in a 6-plane LDS ring (planes t-5 .. t), the +x / +y couplings in
+// 4-plane rings (planes t-1 .. t-4; they are the -x / -y couplings of the
+// neighbours); w = omega / c, A.f and b stay in registers.  A thread owns two
+// vertically adjacent positions, which have opposite colours, so every
+// half-sweep updates one position per thread.  Each update is the FP32 form
+// of the single sweep, (1 - omega) x + w (b - s) with Jet's relaxRedBlack
+// operand order for s (src/jet/fdm_gauss_seidel_solver3.cpp:163-190) and
+// contraction off, so the result is bit-identical to two single FP32 fused
+// sweeps.  Zero-centre rows are set to zero, as in the single sweep.
+//
+// Options as for the single sweep: ZERO (x starts at zero, xin not read),
+// CORRECT (the prolongation of a coarse correction is added to x as it is
+// loaded), DOT (b.x and b.b partials of the written x).
+#pragma once
+
+#include <type_traits>
+
+#include "jg_device.h"
+#include "jg_sweep.h"
+
+namespace jg {
+
+constexpr int kS2LX = 64, kS2LY = 32, kS2H = 4;
+constexpr int kS2IX = kS2LX - 2 * kS2H;  // 56 interior columns
+constexpr int kS2IY = kS2LY - 2 * kS2H;  // 24 interior rows
+constexpr int kS2TY = 16;                // thread rows (two tile rows each)
+constexpr int kS2Threads = kS2LX * kS2TY;
+constexpr int kS2XR = 6, kS2CR = 4;      // x ring and coupling ring depths
+constexpr int kS2CzX = kS2LX / 2 + 2, kS2CzY = kS2LY / 2 + 2;
+static_assert(kS2CzX * kS2CzY <= kS2Threads, "one coarse entry per thread");
+static_assert(kS2IX % 2 == 0 && kS2IY % 2 == 0, "tile origins stay even");
+
+// FP32 only: the FP64 rings would need 234 KB of LDS.
+template <typename T, int FIRST, bool ZERO, bool CORRECT, bool DOT>
+__global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
+    const Row<T>* __restrict__ A, const T* __restrict__ b, const T* __restrict__ xin,
+    T* __restrict__ xout, Dims d, int zc, T omega, const T* __restrict__ coarse,
+    double* __restrict__ p_bx, double* __restrict__ p_bb, const Ctl* __restrict__ ctl) {
+#pragma clang fp contract(off)
+    static_assert(sizeof(T) == 4, "two-sweep kernel is FP32 only");
+    if (ctl && ctl->done) return;
+    __shared__ __align__(16) T sx[kS2XR][kS2LY][kS2LX];  // x, slot = plane mod 6
+    __shared__ T sr[kS2CR][kS2LY][kS2LX];                // A.r, slot = plane & 3
+    __shared__ T su[kS2CR][kS2LY][kS2LX];                // A.u
+    __shared__ T sz[CORRECT ? kS2CzY : 1][CORRECT ? kS2CzX : 1];
+
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    int bx, by, bz;
+    xcd_tile(bx, by, bz);
+    const int gx0 = bx * kS2IX - kS2H, gy0 = by * kS2IY - kS2H;  // both even
+    const int gx = gx0 + tx;
+    const int k0 = bz * zc;
+    const int k1 = min(d.nz, k0 + zc);
+    const idx_t sxy = (idx_t)d.nx * d.ny;
+    const bool xok = gx >= 0 && gx < d.nx;
+    // planes are >= k0 - 5 >= -5: slot arithmetic on t + 12 stays non-negative
+    auto xs = [](int p) { return (p + 12) % kS2XR; };
+    auto cs = [](int p) { return (p + 12) & (kS2CR - 1); };
+
+    // CORRECT: as in k_rb_sweep, thread e < kS2CzX * kS2CzY stages one coarse
+    // column, z-interpolated for the plane about to be committed.
+    const int e = tx + kS2LX * ty;
+    const bool zact = CORRECT && e < kS2CzX * kS2CzY;
+    unsigned zoff = 0;
+    const idx_t csz = (idx_t)(d.nx / 2) * (d.ny / 2);
+    if (CORRECT) {
+        const int ci = min(max(gx0 / 2 - 1 + e % kS2CzX, 0), d.nx / 2 - 1);
+        const int cj = min(max(gy0 / 2 - 1 + e / kS2CzX, 0), d.ny / 2 - 1);
+        zoff = (unsigned)ci + (unsigned)(d.nx / 2) * (unsigned)cj;
+    }
+    T zc0 = 0, zc1 = 0;
+    auto zfetch = [&](int t) __attribute__((always_inline)) {
+        if (!zact) return;
+        int c0, c1;
+        T w0, w1;
+        prolong_z<T>(t, d, c0, c1, w0, w1);
+        zc0 = coarse[zoff + csz * c0];
+        zc1 = coarse[zoff + csz * c1];
+    };
+    auto zstore = [&](int t) __attribute__((always_inline)) {
+        if (!zact) return;
+        int c0, c1;
+        T w0, w1;
+        prolong_z<T>(t, d, c0, c1, w0, w1);
+        sz[e / kS2CzX][e % kS2CzX] = w0 * zc0 + w1 * zc1;
+    };
+
+    struct Pre {
+        Row<T> row;
+        T b, x;
+        bool ok;
+    };
+    // A thread owns the two vertically adjacent positions (tx, 2 ty) and
+    // (tx, 2 ty + 1), which always have opposite colours: every half-sweep
+    // updates exactly one of them per thread (no idle lanes).  Per position:
+    // w = omega / c (0 for a zero-centre row) and b of planes t .. t-4, f of
+    // planes t .. t-5, the held r / u of plane t (written to LDS at the next
+    // commit), the pending final value, and the prefetched planes.
+    struct Pos {
+        T w[5], bb[5], f[6];
+        T hr, hu, px;
+        Pre p[2];  // planes in flight: slot (t - t0) & 1 holds plane t
+        int ly, gy;
+        bool in4;  // in the written interior
+    };
+    Pos P[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        Pos& Q = P[r];
+        Q.ly = 2 * ty + r;
+        Q.gy = gy0 + Q.ly;
+        Q.in4 = tx >= kS2H && tx < kS2LX - kS2H && Q.ly >= kS2H && Q.ly < kS2LY - kS2H && xok &&
+                Q.gy >= 0 && Q.gy < d.ny;
+#pragma unroll
+        for (int s = 0; s < 5; ++s) {
+            Q.w[s] = omega;
+            Q.bb[s] = T(0);
+        }
+#pragma unroll
+        for (int s = 0; s < 6; ++s) Q.f[s] = T(0);
+        Q.hr = Q.hu = Q.px = T(0);
+#pragma unroll
+        for (int s = 0; s < kS2XR; ++s) sx[s][Q.ly][tx] = T(0);
+#pragma unroll
+        for (int s = 0; s < kS2CR; ++s) {
+            sr[s][Q.ly][tx] = T(0);
+            su[s][Q.ly][tx] = T(0);
+        }
+    }
+    auto fetch = [&](Pre& q, int gy, int t) __attribute__((always_inline)) {
+        const bool ok = xok && t >= 0 && t < d.nz && gy >= 0 && gy < d.ny;
+        const idx_t c = ok ? gx + (idx_t)d.nx * gy + sxy * t : 0;
+        q.row = A[c];
+        q.b = b[c];
+        q.x = ZERO ? T(0) : xin[c];
+        q.ok = ok;
+    };
+    auto commit = [&](Pos& Q, const Pre& q, int t) __attribute__((always_inline)) {
+#pragma unroll
+        for (int s = 4; s > 0; --s) {
+            Q.w[s] = Q.w[s - 1];
+            Q.bb[s] = Q.bb[s - 1];
+        }
+#pragma unroll
+        for (int s = 5; s > 0; --s) Q.f[s] = Q.f[s - 1];
+        const T c = q.ok ? q.row.c : T(1);
+        Q.w[0] = c == T(0) ? T(0) : omega / c;
+        Q.f[0] = q.ok ? q.row.f : T(0);
+        Q.bb[0] = q.ok ? q.b : T(0);
+        T xv = q.ok ? q.x : T(0);
+        if (CORRECT) {
+            const int ax0 = (tx >> 1) + (tx & 1);
+            const T wx0 = (tx & 1) ? T(0.75) : T(0.25), wx1 = (tx & 1) ? T(0.25) : T(0.75);
+            const int ay0 = (Q.ly >> 1) + (Q.ly & 1);
+            const T wy0 = (Q.ly & 1) ? T(0.75) : T(0.25), wy1 = (Q.ly & 1) ? T(0.25) : T(0.75);
+            const T lo = wx0 * sz[ay0][ax0] + wx1 * sz[ay0][ax0 + 1];
+            const T hi = wx0 * sz[ay0 + 1][ax0] + wx1 * sz[ay0 + 1][ax0 + 1];
+            const T corr = wy0 * lo + wy1 * hi;
+            xv += q.ok ? corr : T(0);
+        }
+        sx[xs(t)][Q.ly][tx] = xv;
+        sr[cs(t - 1)][Q.ly][tx] = Q.hr;
+        su[cs(t - 1)][Q.ly][tx] = Q.hu;
+        Q.hr = q.ok ? q.row.r : T(0);
+        Q.hu = q.ok ? q.row.u : T(0);
+    };
+    // Half-sweep H (1..4) on plane p = t - H: colour FIRST for odd H, the
+    // other for even H, on positions at least H cells inside the tile.  The
+    // row of this thread's position of that colour is 2 ty + sel.  FP32
+    // update x = (1 - w) x + w (b - s) with w = omega / c precomputed (the
+    // operators of Jet's relaxRedBlack, rounded differently from its
+    // omega (b - s) / c; FP64 sweeps keep Jet's expression).
+    double acc = 0.0, acc2 = 0.0;
+    auto half = [&](auto hc, int p) __attribute__((always_inline)) {
+        constexpr int H = decltype(hc)::value;
+        const int colour = (H & 1) ? FIRST : 1 - FIRST;
+        const int sel = ((gx + gy0 + p) & 1) != colour;  // 0: row 2ty, 1: row 2ty + 1
+        const int ly = 2 * ty + sel;
+        const int gy = gy0 + ly;
+        const bool in = tx >= H && tx < kS2LX - H && ly >= H && ly < kS2LY - H && xok && gy >= 0 &&
+                        gy < d.ny;
+        if (in) {
+            const T w = sel ? P[1].w[H] : P[0].w[H];
+            const T bv = sel ? P[1].bb[H] : P[0].bb[H];
+            const T fm = sel ? P[1].f[H + 1] : P[0].f[H + 1];
+            const T fp = sel ? P[1].f[H] : P[0].f[H];
+            const int sp = xs(p), sm = xs(p - 1), sn = xs(p + 1), s1 = cs(p);
+            const T xo = sx[sp][ly][tx];
+            T xv;
+            if (w == T(0)) {
+                xv = T(0);  // zero-centre row
+            } else if (ZERO && !CORRECT && H == 1) {
+                // x is zero everywhere before the first half-sweep
+                xv = w * (bv - T(0));
+            } else {
+                T s = sr[s1][ly][tx - 1] * sx[sp][ly][tx - 1];
+                s += sr[s1][ly][tx] * sx[sp][ly][tx + 1];
+                s += su[s1][ly - 1][tx] * sx[sp][ly - 1][tx];
+                s += su[s1][ly][tx] * sx[sp][ly + 1][tx];
+                s += fm * sx[sm][ly][tx];
+                s += fp * sx[sn][ly][tx];
+                xv = (T(1) - omega) * xo + w * (bv - s);
+            }
+            sx[sp][ly][tx] = xv;
+        }
+    };
+    // After h4 on plane q: the final values of both positions (the one h4
+    // updated and the one h3 left) become the pending stores.
+    auto finish = [&](int q) __attribute__((always_inline)) {
+        const int sq = xs(q);
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            Pos& Q = P[r];
+            Q.px = sx[sq][Q.ly][tx];
+            if (DOT && Q.in4) {
+                acc += (double)Q.bb[4] * (double)Q.px;
+                acc2 += (double)Q.bb[4] * (double)Q.bb[4];
+            }
+        }
+    };
+    auto flush = [&](int q) __attribute__((always_inline)) {
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+            if (P[r].in4) xout[gx + (idx_t)d.nx * P[r].gy + sxy * q] = P[r].px;
+    };
+    using H1 = std::integral_constant<int, 1>;
+    using H2 = std::integral_constant<int, 2>;
+    using H3 = std::integral_constant<int, 3>;
+    using H4 = std::integral_constant<int, 4>;
+
+    // Planes k0-4 .. k1+3 are committed; half-sweep H covers planes
+    // k0-4+H .. k1+4-H (those within the grid): exactly the planes the
+    // next half-sweep reads for the chunk's planes k0 .. k1-1.
+    // Two planes of loads are in flight ahead of the committed one; the loop
+    // is unrolled twice so the prefetch slot of every step is static.
+    const int t0 = k0 - 4, t1 = k1 + 3;  // t1 - t0 >= 8
+    if (CORRECT) zfetch(t0);
+    // plane by plane, in the order the loop issues them
+#pragma unroll
+    for (int r = 0; r < 2; ++r) fetch(P[r].p[0], P[r].gy, t0);
+#pragma unroll
+    for (int r = 0; r < 2; ++r) fetch(P[r].p[1], P[r].gy, t0 + 1);
+    if (CORRECT) zstore(t0);
+    __syncthreads();
+
+    // An even number of steps, so the unrolled body has no exit: a last
+    // step past t1 commits a re-read plane and runs no half-sweep.
+    const int tend = t0 + ((t1 - t0 + 2) & ~1);
+    bool pending = false;  // uniform: the pending stores hold plane t - 5
+    for (int t2 = t0; t2 < tend; t2 += 2) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int t = t2 + u;
+#pragma unroll
+            for (int r = 0; r < 2; ++r) commit(P[r], P[r].p[u], t);
+            if (CORRECT && t + 1 <= t1) zfetch(t + 1);
+            // unconditional (past the end it re-reads plane t1, unused) so the
+            // compiler can count the loads in flight and wait only for plane t
+#pragma unroll
+            for (int r = 0; r < 2; ++r) fetch(P[r].p[u], P[r].gy, min(t + 2, t1));
+            if (pending) flush(t - 5);
+            __syncthreads();
+            if (t - 1 >= max(k0 - 3, 0) && t - 1 < min(k1 + 3, d.nz)) half(H1{}, t - 1);
+            __syncthreads();
+            if (t - 2 >= max(k0 - 2, 0) && t - 2 < min(k1 + 2, d.nz)) half(H2{}, t - 2);
+            __syncthreads();
+            if (t - 3 >= max(k0 - 1, 0) && t - 3 < min(k1 + 1, d.nz)) half(H3{}, t - 3);
+            __syncthreads();
+            pending = t - 4 >= k0 && t - 4 < k1;
+            if (pending) half(H4{}, t - 4);
+            if (CORRECT && t + 1 <= t1) zstore(t + 1);
+            __syncthreads();
+            if (pending) finish(t - 4);
+        }
+    }
+    if (pending) flush(tend - 5);
+    if (DOT) {
+        __syncthreads();
+        double* sh = reinterpret_cast<double*>(&sx[0][0][0]);
+        constexpr int W = kS2Threads / kWave;
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            acc += __shfl_down(acc, o, kWave);
+            acc2 += __shfl_down(acc2, o, kWave);
+        }
+        if ((e & (kWave - 1)) == 0) {
+            sh[e / kWave] = acc;
+            sh[W + e / kWave] = acc2;
+        }
+        __syncthreads();
+        if (e == 0) {
+            double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                s0 += sh[w];
+                s1 += sh[W + w];
+            }
+            const idx_t blk = blockIdx.x + (idx_t)gridDim.x * (blockIdx.y + (idx_t)gridDim.y * blockIdx.z);
+            p_bx[blk] = s0;
+            p_bb[blk] = s1;
+        }
+    }
+}
+
+}  // namespace jg

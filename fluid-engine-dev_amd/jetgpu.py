@@ -37,7 +37,8 @@ class MgParams(C.Structure):
 class GpuOptions(C.Structure):
     _fields_ = [("precision", C.c_int), ("stop_rule", C.c_int), ("rel_tol", C.c_double),
                 ("smoother", C.c_int), ("omega", C.c_double), ("max_refine", C.c_uint),
-                ("inner_rel_tol", C.c_double), ("check_every", C.c_uint)]
+                ("inner_rel_tol", C.c_double), ("check_every", C.c_uint),
+                ("sweep_pairs", C.c_int)]
 
 
 class SolveInfo(C.Structure):

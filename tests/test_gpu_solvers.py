@@ -130,6 +130,24 @@ def test_mgpcg_scene_matches_oracle(solver, oracle, kind, n, levels, precision):
     assert info.iterations <= 60
 
 
+def test_mgpcg_mixed_two_sweep_kernel_bitwise(solver, oracle):
+    # The mixed solve with FP32 sweeps paired into two-sweep launches gives
+    # exactly the solve with single-sweep launches (same iterations, same x).
+    markers, s = scene_system(oracle, 3, 128, 5, seed=9)
+    upload_mg(solver, s)
+    p = jetgpu.mg_params(max_levels=5, restrict_iter=2, correct_iter=2, coarsest_iter=20,
+                         final_iter=2)
+    out = {}
+    for pairs in (-1, 1):
+        o = jetgpu.gpu_options(precision=jetgpu.JG_MIXED, stop_rule=jetgpu.JG_STOP_RELATIVE,
+                               rel_tol=1e-9, sweep_pairs=pairs)
+        info = solver.mgpcg(200, p, o)
+        assert info.rel_residual <= 1e-8
+        out[pairs] = (info.iterations, solver.download_x())
+    assert out[-1][0] == out[1][0]
+    assert np.array_equal(out[-1][1], out[1][1])
+
+
 def test_mgpcg_closed_sphere_singular(solver, oracle):
     # All-Neumann (config-1 geometry): compare modulo the mean.
     markers, s = scene_system(oracle, 1, 32, 3, seed=2)
