@@ -135,6 +135,7 @@ struct jg_solver {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t evp[2] = {nullptr, nullptr};  // stop-flag polls (two in flight)
     std::string err;
     std::vector<jg::Level> lv;
     // Finest-level Krylov vectors.
@@ -146,7 +147,7 @@ struct jg_solver {
     double* part = nullptr;  // kSlots * kMaxPartials
     double* scal = nullptr;  // small result area
     jg::Ctl* ctl = nullptr;      // [0]: FP64 / outer, [1]: FP32 inner
-    jg::Ctl* ctl_host = nullptr; // pinned mirror
+    jg::Ctl* ctl_host = nullptr; // pinned mirror, two slots
     bool f32_valid = false;
     std::vector<void*> allocs;
 
@@ -158,6 +159,8 @@ struct jg_solver {
         if (ctl_host) (void)hipHostFree(ctl_host);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        for (hipEvent_t e : evp)
+            if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
     void release() {

@@ -255,6 +255,22 @@ __global__ __launch_bounds__(NT) void k_dot(const TA* __restrict__ a, const TB* 
     block_partial<NT>(acc, partial);
 }
 
+// r.z and r.r partials (slots p_rz, p_rr) in one pass over r and z.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_dot_rz_rr(const T* __restrict__ r, const T* __restrict__ z,
+                                                  idx_t n, double* __restrict__ p_rz,
+                                                  double* __restrict__ p_rr,
+                                                  const Ctl* __restrict__ ctl) {
+    if (ctl && ctl->done) return;
+    double a0 = 0.0, a1 = 0.0;
+    for (idx_t c = blockIdx.x * (idx_t)NT + threadIdx.x; c < n; c += (idx_t)gridDim.x * NT) {
+        const double rv = (double)r[c];
+        a0 += rv * (double)z[c];
+        a1 += rv * rv;
+    }
+    block_partial2<NT>(a0, a1, p_rz, p_rr);
+}
+
 template <typename TX>
 __global__ __launch_bounds__(NT) void k_axpy(double a, const TX* __restrict__ x,
                                              const TX* __restrict__ y, TX* __restrict__ out,

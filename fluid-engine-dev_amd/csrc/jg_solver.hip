@@ -332,15 +332,18 @@ int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, 
                 const bool last0 = (l == 0 && m + 1 == M);
                 const int kind = q == 0 ? kSweepCorrect : kSweepPlain;
                 const T* cor = q == 0 ? coarse : nullptr;
-                double* pbx = last0 ? s->slot(0) : nullptr;
-                double* pbb = last0 ? s->slot(1) : nullptr;
+                // the two-sweep kernel leaves r.z / r.r to one fused dot kernel
+                // (accumulating them in it costs more than that pass)
+                const bool dot = last0 && post_plan[q] == 1;
+                double* pbx = dot ? s->slot(0) : nullptr;
+                double* pbb = dot ? s->slot(1) : nullptr;
                 const int nb =
                     post_plan[q] == 2
                         ? launch_sweep2<T>(s, A, B(l), in, buf(m), d, 1, cor, pbx, pbb, c.omega, ctl,
                                            Prec<T>::zc2(lv, kind))
                         : launch_sweep<T>(s, A, B(l), in, buf(m), d, 1, cor, pbx, pbb, c.omega, ctl,
                                           Prec<T>::zc(lv, kind), Prec<T>::sh(lv, kind));
-                if (last0) *dot_blocks = nb;
+                if (dot) *dot_blocks = nb;
                 in = buf(m);
             }
         }
@@ -484,8 +487,10 @@ int pcg_precondition(jg_solver* s, const Pcg<T>& v, const Ctl* c) {
         if (st != JG_OK) return -st;
     }
     if (nb == 0) {
-        k_dot<T, T><<<nb1, NT, 0, s->stream>>>(v.r, v.z, n, s->slot(0), c);
-        if (v.mg) k_dot<T, T><<<nb1, NT, 0, s->stream>>>(v.r, v.r, n, s->slot(1), c);
+        if (v.mg)
+            k_dot_rz_rr<T><<<nb1, NT, 0, s->stream>>>(v.r, v.z, n, s->slot(0), s->slot(1), c);
+        else
+            k_dot<T, T><<<nb1, NT, 0, s->stream>>>(v.r, v.z, n, s->slot(0), c);
         nb = nb1;
     }
     return nb;
@@ -542,19 +547,38 @@ int pcg_device(jg_solver* s, const Pcg<T>& v, unsigned check_every) {
     JG_HIP(hipMemcpyAsync(s->ctl_host, v.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
     JG_TRY(sync(s));
     if (s->ctl_host->done) return JG_OK;
-    unsigned launched = 0;
     const unsigned max_iter = s->ctl_host->max_iter;
+    // One iteration of lookahead: iteration k+1 is enqueued before the host
+    // waits for the stop flag iteration k left (every kernel of an iteration
+    // returns at once once the control block says the solve has stopped), so
+    // the device does not idle while the host polls.
+    unsigned launched = 0;
+    int slot = 0;
+    JG_TRY(pcg_iteration<T>(s, v, false));
+    ++launched;
     while (true) {
-        JG_TRY(pcg_iteration<T>(s, v, (launched & 1) != 0));
-        ++launched;
-        if (launched % check_every == 0 || launched >= max_iter) {
-            JG_HIP(hipMemcpyAsync(s->ctl_host, v.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
-            JG_TRY(sync(s));
-            if (s->ctl_host->done) break;
-            if (launched >= max_iter + 1) break;  // safety: the device stops at max_iter
+        const bool poll = launched % check_every == 0 || launched >= max_iter;
+        if (poll) {
+            JG_HIP(hipMemcpyAsync(s->ctl_host + slot, v.ctl, sizeof(Ctl), hipMemcpyDeviceToHost,
+                                  s->stream));
+            JG_HIP(hipEventRecord(s->evp[slot], s->stream));
         }
+        const bool more = launched <= max_iter;  // the device stops by itself at max_iter
+        if (more) {
+            JG_TRY(pcg_iteration<T>(s, v, (launched & 1) != 0));
+            ++launched;
+        }
+        if (poll) {
+            JG_HIP(hipEventSynchronize(s->evp[slot]));
+            if (s->ctl_host[slot].done) break;
+            slot ^= 1;
+        }
+        if (!more) break;
     }
-    return JG_OK;
+    // final state in slot 0 for the callers (the look-ahead iteration, a
+    // no-op, has drained by then)
+    JG_HIP(hipMemcpyAsync(s->ctl_host, v.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
+    return sync(s);
 }
 
 void init_ctl(Ctl* h, unsigned max_iter, double tol, int stop_rule, double rel_tol) {
@@ -673,6 +697,8 @@ int jg_solver_create(int device, jg_solver** out) {
     s->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&s->stream) != hipSuccess ||
         hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&s->evp[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&s->evp[1], hipEventDisableTiming) != hipSuccess ||
         hipHostMalloc((void**)&s->ctl_host, 2 * sizeof(Ctl), hipHostMallocDefault) != hipSuccess) {
         delete s;
         return JG_ERR_HIP;
@@ -1015,6 +1041,10 @@ int time_kernel_t(jg_solver* s, const Vecs<T>& v, int kernel, unsigned reps, dou
                 launch_sweep<T>(s, A, v.b, v.r, v.x, d, 1, nullptr, nullptr, nullptr, T(1), s->ctl,
                                 Prec<T>::zc(l0), Prec<T>::sh(l0));
                 break;
+            case JG_KERNEL_SWEEP2:
+                launch_sweep2<T>(s, A, v.b, v.r, v.x, d, 1, nullptr, nullptr, nullptr, T(1), s->ctl,
+                                 Prec<T>::zc2(l0, kSweepPlain));
+                break;
             default:
                 k_convert<T, T><<<nb1, NT, 0, s->stream>>>(v.r, v.q, n);
                 break;
@@ -1029,6 +1059,7 @@ int time_kernel_t(jg_solver* s, const Vecs<T>& v, int kernel, unsigned reps, dou
         case JG_KERNEL_UPDATE_XR: per_cell = 6 * sv; break;      // x, d, r, q -> x, r
         case JG_KERNEL_DOT: per_cell = 2 * sv; break;
         case JG_KERNEL_SWEEP: per_cell = sa + 3 * sv; break;     // A, b, x -> x (both colours)
+        case JG_KERNEL_SWEEP2: per_cell = 2 * (sa + 3 * sv); break;  // two sweeps, each as above
         default: per_cell = 2 * sv; break;
     }
     launch();  // warm
@@ -1048,8 +1079,12 @@ int time_kernel_t(jg_solver* s, const Vecs<T>& v, int kernel, unsigned reps, dou
 
 extern "C" int jg_solver_time_kernel(jg_solver* s, int kernel, int precision, unsigned reps,
                                      double* avg_ms, double* bytes) {
-    if (!s || s->lv.empty() || !avg_ms || !bytes || reps == 0 || kernel < 0 || kernel > 6)
+    if (!s || s->lv.empty() || !avg_ms || !bytes || reps == 0 || kernel < 0 || kernel > 7)
         return JG_ERR_INVALID_ARG;
+    if (kernel == JG_KERNEL_SWEEP2 && precision != JG_MIXED) {
+        s->err = "the two-sweep kernel is FP32 only";
+        return JG_ERR_UNSUPPORTED;
+    }
     JG_HIP(hipSetDevice(s->device));
     if (precision == JG_MIXED) {
         JG_TRY(ensure_f32(s));

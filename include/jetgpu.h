@@ -222,7 +222,9 @@ enum {
     JG_KERNEL_UPDATE_XR = 3,  /* x += alpha d, r -= alpha q                    */
     JG_KERNEL_DOT = 4,        /* partial r.z                                   */
     JG_KERNEL_COPY = 5,       /* plain copy of one vector (bandwidth reference) */
-    JG_KERNEL_SWEEP = 6       /* fused red-black sweep, both colours           */
+    JG_KERNEL_SWEEP = 6,      /* fused red-black sweep, both colours           */
+    JG_KERNEL_SWEEP2 = 7      /* two fused sweeps in one pass (JG_MIXED only);
+                                 bytes = two sweeps' algorithmic bytes          */
 };
 int jg_solver_time_kernel(jg_solver* s, int kernel, int precision,
                           unsigned reps, double* avg_ms, double* bytes);
