@@ -96,21 +96,56 @@ __device__ __forceinline__ void block_partial2(double v0, double v1, double* p0,
     }
 }
 
-// Sum of n partials by one block of NT threads, fixed order (deterministic).
+// Block size of the one-block finalize kernels.
+constexpr int kFinThreads = 1024;
+
+// Sums of n partials of one or two arrays by one block of NT threads, fixed
+// order (deterministic): thread t adds p[t], p[t + NT], ... in that order
+// (four loads in flight before the adds), then the waves and the block.
+template <int NT, int K>
+__device__ __forceinline__ void reduce_partials_k(const double* const (&p)[K], int n, double* sh,
+                                                  double (&r)[K]) {
+    const int t = threadIdx.x;
+    double s[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) s[k] = 0.0;
+    int i = t;
+    for (; i + 3 * NT < n; i += 4 * NT) {
+        double v[K][4];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[k][u] = p[k][i + u * NT];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) s[k] += v[k][u];
+    }
+    for (; i < n; i += NT)
+#pragma unroll
+        for (int k = 0; k < K; ++k) s[k] += p[k][i];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) s[k] += __shfl_down(s[k], o, kWave);
+        if ((t & (kWave - 1)) == 0) sh[k * (NT / kWave) + t / kWave] = s[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        r[k] = 0.0;
+#pragma unroll
+        for (int w = 0; w < NT / kWave; ++w) r[k] += sh[k * (NT / kWave) + w];
+    }
+    __syncthreads();  // results valid in every thread
+}
+
 template <int NT>
 __device__ __forceinline__ double reduce_partials(const double* p, int n, double* sh) {
-    const int t = threadIdx.x;
-    double s = 0.0;
-    for (int i = t; i < n; i += NT) s += p[i];
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) s += __shfl_down(s, o, kWave);
-    if ((t & (kWave - 1)) == 0) sh[t / kWave] = s;
-    __syncthreads();
-    double r = 0.0;
-#pragma unroll
-    for (int w = 0; w < NT / kWave; ++w) r += sh[w];
-    __syncthreads();
-    return r;  // valid in every thread
+    const double* const a[1] = {p};
+    double r[1];
+    reduce_partials_k<NT, 1>(a, n, sh, r);
+    return r[0];
 }
 
 // XCD-aware tile order.  Workgroups are handed to the 8 XCDs round-robin

@@ -455,11 +455,13 @@ __global__ __launch_bounds__(NT) void k_correct(const TX* __restrict__ coarse,
 
 // Initial PCG scalars: sigma = r.z, rr = r.r, bb = b.b (for the relative
 // rule), beta = 0 so the first fused direction update gives d = z.
-__global__ __launch_bounds__(NT) void k_fin_init(Ctl* ctl, const double* p_rz, const double* p_rr,
-                                                 int n) {
-    __shared__ double sh[NT / kWave];
-    const double rz = reduce_partials<NT>(p_rz, n, sh);
-    const double rr = reduce_partials<NT>(p_rr, n, sh);
+__global__ __launch_bounds__(kFinThreads) void k_fin_init(Ctl* ctl, const double* p_rz,
+                                                          const double* p_rr, int n) {
+    __shared__ double sh[2 * kFinThreads / kWave];
+    const double* const pp[2] = {p_rz, p_rr};
+    double rs[2];
+    reduce_partials_k<kFinThreads, 2>(pp, n, sh, rs);
+    const double rz = rs[0], rr = rs[1];
     if (threadIdx.x == 0) {
         ctl->sigma = rz;
         ctl->sigma_old = rz;
@@ -477,10 +479,10 @@ __global__ __launch_bounds__(NT) void k_fin_init(Ctl* ctl, const double* p_rz, c
 }
 
 // alpha = sigma / d.q ; Jet's refresh rule for this iteration.
-__global__ __launch_bounds__(NT) void k_fin_alpha(Ctl* ctl, const double* p_dq, int n) {
+__global__ __launch_bounds__(kFinThreads) void k_fin_alpha(Ctl* ctl, const double* p_dq, int n) {
     if (ctl->done) return;
-    __shared__ double sh[NT / kWave];
-    const double dq = reduce_partials<NT>(p_dq, n, sh);
+    __shared__ double sh[kFinThreads / kWave];
+    const double dq = reduce_partials<kFinThreads>(p_dq, n, sh);
     if (threadIdx.x == 0) {
         ctl->dq = dq;
         ctl->alpha = ctl->sigma / dq;
@@ -495,12 +497,14 @@ __global__ __launch_bounds__(NT) void k_fin_alpha(Ctl* ctl, const double* p_dq, 
 
 // End of a PCG iteration (include/jet/detail/cg-inl.h:65-90): sigma_new =
 // r.z, trigger on growth, beta, iteration count and the stop test.
-__global__ __launch_bounds__(NT) void k_fin_sigma(Ctl* ctl, const double* p_rz, const double* p_rr,
-                                                  int n) {
+__global__ __launch_bounds__(kFinThreads) void k_fin_sigma(Ctl* ctl, const double* p_rz,
+                                                           const double* p_rr, int n) {
     if (ctl->done) return;
-    __shared__ double sh[NT / kWave];
-    const double rz = reduce_partials<NT>(p_rz, n, sh);
-    const double rr = reduce_partials<NT>(p_rr, n, sh);
+    __shared__ double sh[2 * kFinThreads / kWave];
+    const double* const pp[2] = {p_rz, p_rr};
+    double rs[2];
+    reduce_partials_k<kFinThreads, 2>(pp, n, sh, rs);
+    const double rz = rs[0], rr = rs[1];
     if (threadIdx.x == 0) {
         const double sold = ctl->sigma;
         ctl->sigma_old = sold;
@@ -520,9 +524,9 @@ __global__ __launch_bounds__(NT) void k_fin_sigma(Ctl* ctl, const double* p_rz, 
 }
 
 // Generic: result = sum of partials (used by the BLAS entry points).
-__global__ __launch_bounds__(NT) void k_fin_sum(const double* p, int n, double* out) {
-    __shared__ double sh[NT / kWave];
-    const double s = reduce_partials<NT>(p, n, sh);
+__global__ __launch_bounds__(kFinThreads) void k_fin_sum(const double* p, int n, double* out) {
+    __shared__ double sh[kFinThreads / kWave];
+    const double s = reduce_partials<kFinThreads>(p, n, sh);
     if (threadIdx.x == 0) *out = s;
 }
 

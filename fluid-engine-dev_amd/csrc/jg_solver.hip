@@ -510,7 +510,7 @@ int pcg_init(jg_solver* s, const Pcg<T>& v) {
     k_fill<T><<<nb1, NT, 0, s->stream>>>(v.dB, T(0), n);
     const int nb = pcg_precondition<T>(s, v, nullptr);
     if (nb < 0) return -nb;
-    k_fin_init<<<1, NT, 0, s->stream>>>(v.ctl, s->slot(0), v.mg ? s->slot(1) : s->slot(0), nb);
+    k_fin_init<<<1, kFinThreads, 0, s->stream>>>(v.ctl, s->slot(0), v.mg ? s->slot(1) : s->slot(0), nb);
     return check_launch(s, "pcg init");
 }
 
@@ -527,13 +527,13 @@ int pcg_iteration(jg_solver* s, const Pcg<T>& v, bool odd) {
     T* dnew = odd ? v.dA : v.dB;
     k_dir_spmv<T, T><<<g.grid, g.block, 0, s->stream>>>(v.A, v.z, dold, dnew, v.q, v.d, g.zc,
                                                         s->slot(2), v.ctl);
-    k_fin_alpha<<<1, NT, 0, s->stream>>>(v.ctl, s->slot(2), nblocks(g));
+    k_fin_alpha<<<1, kFinThreads, 0, s->stream>>>(v.ctl, s->slot(2), nblocks(g));
     k_update_xr<T><<<nb1, NT, 0, s->stream>>>(v.x, v.r, dnew, v.q, n, v.ctl);
     k_residual<T, T, T, false, true><<<g.grid, g.block, 0, s->stream>>>(v.A, v.x, v.b, v.r, v.d, g.zc,
                                                                        nullptr, v.ctl);
     const int nb = pcg_precondition<T>(s, v, v.ctl);
     if (nb < 0) return -nb;
-    k_fin_sigma<<<1, NT, 0, s->stream>>>(v.ctl, s->slot(0), v.mg ? s->slot(1) : s->slot(0), nb);
+    k_fin_sigma<<<1, kFinThreads, 0, s->stream>>>(v.ctl, s->slot(0), v.mg ? s->slot(1) : s->slot(0), nb);
     return check_launch(s, "pcg iteration");
 }
 
@@ -610,9 +610,9 @@ int true_rel_residual(jg_solver* s, double* out) {
     const int nb1 = nblocks_1d(l0.d.n());
     k_residual<double, double, double, true, false><<<g.grid, g.block, 0, s->stream>>>(
         l0.A64, s->x, s->b, s->r, l0.d, g.zc, s->slot(0), nullptr);
-    k_fin_sum<<<1, NT, 0, s->stream>>>(s->slot(0), nblocks(g), s->scal + 0);
+    k_fin_sum<<<1, kFinThreads, 0, s->stream>>>(s->slot(0), nblocks(g), s->scal + 0);
     k_dot<double, double><<<nb1, NT, 0, s->stream>>>(s->b, s->b, l0.d.n(), s->slot(1), nullptr);
-    k_fin_sum<<<1, NT, 0, s->stream>>>(s->slot(1), nb1, s->scal + 1);
+    k_fin_sum<<<1, kFinThreads, 0, s->stream>>>(s->slot(1), nb1, s->scal + 1);
     JG_TRY(check_launch(s, "true residual"));
     double h[2];
     JG_HIP(hipMemcpyAsync(h, s->scal, sizeof(h), hipMemcpyDeviceToHost, s->stream));
@@ -909,7 +909,7 @@ int jg_solver_mgpcg(jg_solver* s, unsigned max_iter, const jg_mg_params* p,
         Geo g = geo_stencil(l0.d);
         const int nb1 = nblocks_1d(n);
         k_dot<double, double><<<nb1, NT, 0, s->stream>>>(s->b, s->b, n, s->slot(3), nullptr);
-        k_fin_sum<<<1, NT, 0, s->stream>>>(s->slot(3), nb1, s->scal + 2);
+        k_fin_sum<<<1, kFinThreads, 0, s->stream>>>(s->slot(3), nb1, s->scal + 2);
         double bb = 0.0;
         JG_HIP(hipMemcpyAsync(&bb, s->scal + 2, sizeof(double), hipMemcpyDeviceToHost, s->stream));
         JG_TRY(sync(s));
@@ -919,7 +919,7 @@ int jg_solver_mgpcg(jg_solver* s, unsigned max_iter, const jg_mg_params* p,
         for (unsigned outer = 0;; ++outer) {
             k_residual<double, double, float, true, false><<<g.grid, g.block, 0, s->stream>>>(
                 l0.A64, s->x, s->b, s->rhs32, l0.d, g.zc, s->slot(3), nullptr);
-            k_fin_sum<<<1, NT, 0, s->stream>>>(s->slot(3), nblocks(g), s->scal + 3);
+            k_fin_sum<<<1, kFinThreads, 0, s->stream>>>(s->slot(3), nblocks(g), s->scal + 3);
             double rr = 0.0;
             JG_HIP(hipMemcpyAsync(&rr, s->scal + 3, sizeof(double), hipMemcpyDeviceToHost, s->stream));
             JG_TRY(sync(s));
@@ -1132,7 +1132,7 @@ int jg_blas_dot(jg_solver* s, size_t n, const double* a, const double* b, double
     }
     const int nb = nblocks_1d((idx_t)n);
     k_dot<double, double><<<nb, NT, 0, s->stream>>>(a, b, (idx_t)n, s->slot(0), nullptr);
-    k_fin_sum<<<1, NT, 0, s->stream>>>(s->slot(0), nb, s->scal + 4);
+    k_fin_sum<<<1, kFinThreads, 0, s->stream>>>(s->slot(0), nb, s->scal + 4);
     JG_TRY(check_launch(s, "dot"));
     JG_HIP(hipMemcpyAsync(result, s->scal + 4, sizeof(double), hipMemcpyDeviceToHost, s->stream));
     return sync(s);
@@ -1550,7 +1550,7 @@ int jg_batch_mgpcg(jg_batch* B, unsigned max_iter, const jg_mg_params* p, const 
             const int nb1 = nblocks_1d(d0.n());
             k_fill<double><<<nb1, NT, 0, s->stream>>>(s->x, 0.0, d0.n());
             k_dot<double, double><<<nb1, NT, 0, s->stream>>>(s->b, s->b, d0.n(), s->slot(3), nullptr);
-            k_fin_sum<<<1, NT, 0, s->stream>>>(s->slot(3), nb1, s->scal + 2);
+            k_fin_sum<<<1, kFinThreads, 0, s->stream>>>(s->slot(3), nb1, s->scal + 2);
         }
         JG_TRY(batch_join(B));
         JG_TRY(batch_scal(B, 2));
@@ -1567,7 +1567,7 @@ int jg_batch_mgpcg(jg_batch* B, unsigned max_iter, const jg_mg_params* p, const 
                 Geo g = geo_stencil(l0.d);
                 k_residual<double, double, float, true, false><<<g.grid, g.block, 0, s->stream>>>(
                     l0.A64, s->x, s->b, s->rhs32, l0.d, g.zc, s->slot(3), nullptr);
-                k_fin_sum<<<1, NT, 0, s->stream>>>(s->slot(3), nblocks(g), s->scal + 3);
+                k_fin_sum<<<1, kFinThreads, 0, s->stream>>>(s->slot(3), nblocks(g), s->scal + 3);
             }
             JG_TRY(batch_join(B));
             JG_TRY(batch_scal(B, 3));
