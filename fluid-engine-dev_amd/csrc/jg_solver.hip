@@ -877,6 +877,7 @@ int jg_solver_mgpcg(jg_solver* s, unsigned max_iter, const jg_mg_params* p,
     int status = JG_OK;
     unsigned iters = 0, refinements = 0;
     double last_res = 0.0;
+    double bnorm = 0.0;  // mixed: ||b||, for the relative residual of the final x
     Ctl* ctl = s->ctl;
     if (opt.precision == JG_MIXED) {
         JG_TRY(ensure_f32(s));
@@ -913,16 +914,23 @@ int jg_solver_mgpcg(jg_solver* s, unsigned max_iter, const jg_mg_params* p,
         double bb = 0.0;
         JG_HIP(hipMemcpyAsync(&bb, s->scal + 2, sizeof(double), hipMemcpyDeviceToHost, s->stream));
         JG_TRY(sync(s));
-        const double bnorm = std::sqrt(bb);
+        bnorm = std::sqrt(bb);
         const double goal = opt.stop_rule == JG_STOP_RELATIVE ? opt.rel_tol * bnorm : p->max_tol;
         Ctl* ictl = s->ctl + 1;
         for (unsigned outer = 0;; ++outer) {
-            k_residual<double, double, float, true, false><<<g.grid, g.block, 0, s->stream>>>(
-                l0.A64, s->x, s->b, s->rhs32, l0.d, g.zc, s->slot(3), nullptr);
-            k_fin_sum<<<1, kFinThreads, 0, s->stream>>>(s->slot(3), nblocks(g), s->scal + 3);
             double rr = 0.0;
-            JG_HIP(hipMemcpyAsync(&rr, s->scal + 3, sizeof(double), hipMemcpyDeviceToHost, s->stream));
-            JG_TRY(sync(s));
+            if (outer == 0) {
+                // x = 0: b - A x is b itself (bit for bit), and its norm is known
+                k_convert<double, float><<<nb1, NT, 0, s->stream>>>(s->b, s->rhs32, n);
+                rr = bb;
+            } else {
+                k_residual<double, double, float, true, false><<<g.grid, g.block, 0, s->stream>>>(
+                    l0.A64, s->x, s->b, s->rhs32, l0.d, g.zc, s->slot(3), nullptr);
+                k_fin_sum<<<1, kFinThreads, 0, s->stream>>>(s->slot(3), nblocks(g), s->scal + 3);
+                JG_HIP(hipMemcpyAsync(&rr, s->scal + 3, sizeof(double), hipMemcpyDeviceToHost,
+                                      s->stream));
+                JG_TRY(sync(s));
+            }
             last_res = std::sqrt(rr);
             if (!std::isfinite(rr)) {
                 status = JG_ERR_NONFINITE;
@@ -956,7 +964,10 @@ int jg_solver_mgpcg(jg_solver* s, unsigned max_iter, const jg_mg_params* p,
         info->refinements = refinements;
         info->last_residual = last_res;
         info->converged = (last_res <= p->max_tol) || (iters < max_iter);
-        JG_TRY(true_rel_residual(s, &info->rel_residual));
+        if (opt.precision == JG_MIXED && bnorm > 0.0)
+            info->rel_residual = last_res / bnorm;  // the FP64 b - A x of the final x
+        else
+            JG_TRY(true_rel_residual(s, &info->rel_residual));
         if (opt.stop_rule == JG_STOP_RELATIVE) info->converged = info->rel_residual <= opt.rel_tol;
     }
     if (status == JG_ERR_NONFINITE) s->err = "non-finite value in MGPCG reduction";
