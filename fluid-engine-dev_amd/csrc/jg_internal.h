@@ -47,11 +47,14 @@ struct Level {
     // sweep kind (kSweepPlain, kSweepZero, kSweepCorrect).
     int zc64[3] = {0, 0, 0}, zc32[3] = {0, 0, 0};
     int sh64[3] = {0, 0, 0}, sh32[3] = {0, 0, 0};
-    // Two-sweep kernel (FP32): tuned z-chunk per kind, and whether one launch
-    // of it beats two single sweeps for that kind.
+    // Two sweeps per launch, per kind: kPairNone (two single sweeps),
+    // kPairMarch (k_rb_sweep2, FP32) or kPairBrick (k_brick_sweep2), as the
+    // tuner measured fastest; and the z-chunk of k_rb_sweep2.
+    int pair64[3] = {0, 0, 0}, pair32[3] = {0, 0, 0};
     int zc2_32[3] = {0, 0, 0};
-    bool pair32[3] = {false, false, false};
 };
+
+enum { kPairNone = 0, kPairMarch = 1, kPairBrick = 2 };
 
 // Sweep kinds tuned separately: they differ in registers and loads.
 enum { kSweepPlain = 0, kSweepZero = 1, kSweepCorrect = 2, kNumSweepKinds = 3 };
@@ -67,8 +70,9 @@ struct Prec<double> {
     static double* y(Level& l) { return l.y64; }
     static int& zc(Level& l, int kind = kSweepPlain) { return l.zc64[kind]; }
     static int& sh(Level& l, int kind = kSweepPlain) { return l.sh64[kind]; }
-    static bool pair(const Level&, int) { return false; }  // no FP64 two-sweep kernel
-    static int zc2(const Level&, int) { return 0; }
+    static int& pair(Level& l, int kind) { return l.pair64[kind]; }
+    static int pair(const Level& l, int kind) { return l.pair64[kind]; }
+    static int zc2(const Level&, int) { return 0; }  // no FP64 z-marching two-sweep kernel
 };
 template <>
 struct Prec<float> {
@@ -79,7 +83,8 @@ struct Prec<float> {
     static float* y(Level& l) { return l.y32; }
     static int& zc(Level& l, int kind = kSweepPlain) { return l.zc32[kind]; }
     static int& sh(Level& l, int kind = kSweepPlain) { return l.sh32[kind]; }
-    static bool pair(const Level& l, int kind) { return l.pair32[kind]; }
+    static int& pair(Level& l, int kind) { return l.pair32[kind]; }
+    static int pair(const Level& l, int kind) { return l.pair32[kind]; }
     static int zc2(const Level& l, int kind) { return l.zc2_32[kind]; }
 };
 

@@ -17,6 +17,7 @@
 #include "jg_kernels.h"
 #include "jg_sweep.h"
 #include "jg_sweep2.h"
+#include "jg_brick.h"
 #include "jg_bottom.h"
 #include "jg_transfer.h"
 
@@ -173,20 +174,54 @@ int launch_sweep2(jg_solver* s, const Row<T>* A, const T* b, const T* xin, T* xo
     }
 }
 
-// Sweeps of one smoothing step, grouped into launches: a launch runs two
-// sweeps (k_rb_sweep2, FP32) when the tuner found that faster for the kind of
-// its first sweep (kind_first for the first launch, plain afterwards);
-// pairs > 0 forces two-sweep launches wherever two sweeps follow, pairs < 0
-// forbids them.  Returns the sweep count of each launch.
+// Two sweeps of a small level in one launch by 8^3 bricks (jg_brick.h).
+template <typename T>
+void launch_brick2(jg_solver* s, const Row<T>* A, const T* b, const T* xin, T* xout, const Dims& d,
+                   int first, const T* coarse, T omega, const Ctl* ctl) {
+    const dim3 grid((d.nx + kBkB - 1) / kBkB, (d.ny + kBkB - 1) / kBkB, (d.nz + kBkB - 1) / kBkB);
+    const int code = (first ? 4 : 0) | (xin ? 0 : 2) | (coarse ? 1 : 0);
+#define JG_BRICK_CASE(C, F, Z, K)                                                               \
+    case C:                                                                                     \
+        k_brick_sweep2<T, F, Z, K><<<grid, kBkThreads, 0, s->stream>>>(A, b, xin, xout, d, omega, \
+                                                                       coarse, ctl);            \
+        break;
+    switch (code) {
+        JG_BRICK_CASE(0, 0, false, false)
+        JG_BRICK_CASE(1, 0, false, true)
+        JG_BRICK_CASE(2, 0, true, false)
+        JG_BRICK_CASE(3, 0, true, true)
+        JG_BRICK_CASE(4, 1, false, false)
+        JG_BRICK_CASE(5, 1, false, true)
+        JG_BRICK_CASE(6, 1, true, false)
+        JG_BRICK_CASE(7, 1, true, true)
+    }
+#undef JG_BRICK_CASE
+}
+
+// Levels up to this size may use the brick kernel (it reloads a 16^3 region
+// per 8^3 brick, which only pays where launches are latency-bound).
+constexpr idx_t kBrickMaxCells = idx_t(1) << 21;
+
+// Sweeps of one smoothing step, grouped into launches: 1 = one sweep,
+// kLaunchMarch / kLaunchBrick = two sweeps in one launch of k_rb_sweep2 /
+// k_brick_sweep2, where the tuner found that faster for the kind of the
+// launch's first sweep (kind_first for the first launch, plain afterwards).
+// pairs > 0 forces two-sweep launches wherever two sweeps follow (FP32:
+// bricks up to 2^18 cells, else the z-marching kernel; FP64: bricks up to
+// kBrickMaxCells cells), pairs < 0 forbids them.
+constexpr int kLaunchSingle = 1, kLaunchMarch = 2, kLaunchBrick = 3;
 template <typename T>
 std::vector<int> sweep_plan(const Level& lv, unsigned n, int kind_first, int pairs) {
     std::vector<int> v;
+    const bool big = lv.d.n() > BotCfg<T>::MaxCells;
     for (unsigned k = 0; k < n;) {
-        const bool tuned = Prec<T>::pair(lv, k == 0 ? kind_first : kSweepPlain);
-        const bool two = sizeof(T) == 4 && n - k >= 2 && lv.d.n() > BotCfg<T>::MaxCells &&
-                         (pairs > 0 || (pairs == 0 && tuned));
-        v.push_back(two ? 2 : 1);
-        k += two ? 2 : 1;
+        int mode = Prec<T>::pair(lv, k == 0 ? kind_first : kSweepPlain);
+        if (pairs > 0)  // FP32: bricks up to 64^3 cells, the z-marching kernel above
+            mode = sizeof(T) == 4 ? (lv.d.n() <= (idx_t(1) << 18) ? kPairBrick : kPairMarch)
+                                  : (lv.d.n() <= kBrickMaxCells ? kPairBrick : kPairNone);
+        if (pairs < 0 || !big || n - k < 2) mode = kPairNone;
+        v.push_back(mode == kPairBrick ? kLaunchBrick : mode == kPairMarch ? kLaunchMarch : kLaunchSingle);
+        k += mode == kPairNone ? 1 : 2;
     }
     return v;
 }
@@ -278,7 +313,9 @@ int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, 
         const T* in = nullptr;
         for (size_t m = 0; m < pre_plan.size(); ++m) {
             const int kind = in ? kSweepPlain : kSweepZero;
-            if (pre_plan[m] == 2)
+            if (pre_plan[m] == kLaunchBrick)
+                launch_brick2<T>(s, A, B(l), in, buf(m), d, 0, nullptr, c.omega, ctl);
+            else if (pre_plan[m] == kLaunchMarch)
                 launch_sweep2<T>(s, A, B(l), in, buf(m), d, 0, nullptr, nullptr, nullptr, c.omega, ctl,
                                  Prec<T>::zc2(lv, kind));
             else
@@ -334,15 +371,18 @@ int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, 
                 const T* cor = q == 0 ? coarse : nullptr;
                 // the two-sweep kernel leaves r.z / r.r to one fused dot kernel
                 // (accumulating them in it costs more than that pass)
-                const bool dot = last0 && post_plan[q] == 1;
+                const bool dot = last0 && post_plan[q] == kLaunchSingle;
                 double* pbx = dot ? s->slot(0) : nullptr;
                 double* pbb = dot ? s->slot(1) : nullptr;
-                const int nb =
-                    post_plan[q] == 2
-                        ? launch_sweep2<T>(s, A, B(l), in, buf(m), d, 1, cor, pbx, pbb, c.omega, ctl,
-                                           Prec<T>::zc2(lv, kind))
-                        : launch_sweep<T>(s, A, B(l), in, buf(m), d, 1, cor, pbx, pbb, c.omega, ctl,
-                                          Prec<T>::zc(lv, kind), Prec<T>::sh(lv, kind));
+                int nb = 0;
+                if (post_plan[q] == kLaunchBrick)
+                    launch_brick2<T>(s, A, B(l), in, buf(m), d, 1, cor, c.omega, ctl);
+                else if (post_plan[q] == kLaunchMarch)
+                    nb = launch_sweep2<T>(s, A, B(l), in, buf(m), d, 1, cor, pbx, pbb, c.omega, ctl,
+                                          Prec<T>::zc2(lv, kind));
+                else
+                    nb = launch_sweep<T>(s, A, B(l), in, buf(m), d, 1, cor, pbx, pbb, c.omega, ctl,
+                                         Prec<T>::zc(lv, kind), Prec<T>::sh(lv, kind));
                 if (dot) *dot_blocks = nb;
                 in = buf(m);
             }
@@ -421,40 +461,62 @@ int autotune_sweeps(jg_solver* s) {
             Prec<T>::zc(lv, kind) = best_zc;
             tbest[kind] = best;
         }
-        if constexpr (sizeof(T) == 4) {
-            // two-sweep kernel: a launch of it replaces a sweep of this kind
-            // followed by a plain sweep
-            const int cands2[] = {8, 12, 16, 24, 32, 48, 64};
-            for (int kind = 0; kind < kNumSweepKinds; ++kind) {
-                lv.pair32[kind] = false;
-                if (kind == kSweepCorrect && !coarse) continue;
-                float best = 1e30f;
+        // Two sweeps per launch: one launch replaces a sweep of this kind
+        // followed by a plain sweep, when it is faster than that pair.
+        auto time3 = [&](auto&& run) -> float {
+            run();
+            float ms = 0.f;
+            if (hipEventRecord(s->ev0, s->stream) != hipSuccess) return -1.f;
+            for (int r = 0; r < 3; ++r) run();
+            if (hipEventRecord(s->ev1, s->stream) != hipSuccess || sync(s) != JG_OK ||
+                hipEventElapsedTime(&ms, s->ev0, s->ev1) != hipSuccess)
+                return -1.f;
+            return ms;
+        };
+        for (int kind = 0; kind < kNumSweepKinds; ++kind) {
+            Prec<T>::pair(lv, kind) = kPairNone;
+            if (kind == kSweepCorrect && !coarse) continue;
+            const T* xin = kind == kSweepZero ? nullptr : Prec<T>::y(lv);
+            const T* cor = kind == kSweepCorrect ? coarse : nullptr;
+            float best = tbest[kind] + tbest[kSweepPlain];
+            if constexpr (sizeof(T) == 4) {
+                const int cands2[] = {8, 12, 16, 24, 32, 48, 64};
+                float best2 = 1e30f;
                 int best_zc = 32;
                 for (int zc : cands2) {
                     if (zc > d.nz && zc != cands2[0]) break;
-                    auto run = [&]() {
-                        launch_sweep2<T>(s, Prec<T>::A(lv), Prec<T>::t(lv),
-                                         kind == kSweepZero ? nullptr : Prec<T>::y(lv), Prec<T>::x(lv),
-                                         d, 1, kind == kSweepCorrect ? coarse : nullptr, nullptr,
-                                         nullptr, T(1), nullptr, zc);
-                    };
-                    run();
-                    JG_HIP(hipEventRecord(s->ev0, s->stream));
-                    for (int r = 0; r < 3; ++r) run();
-                    JG_HIP(hipEventRecord(s->ev1, s->stream));
-                    JG_TRY(sync(s));
-                    float ms = 0.f;
-                    JG_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+                    const float ms = time3([&]() {
+                        launch_sweep2<T>(s, Prec<T>::A(lv), Prec<T>::t(lv), xin, Prec<T>::x(lv), d, 1,
+                                         cor, nullptr, nullptr, T(1), nullptr, zc);
+                    });
+                    if (ms < 0.f) return JG_ERR_HIP;
                     if (tune_log())
                         std::fprintf(stderr, "jetgpu tune: %dx%dx%d f32 kind %d two-sweep zc %d: %.1f us\n",
                                      d.nx, d.ny, d.nz, kind, zc, ms * 1e3f / 3);
-                    if (ms < best) {
-                        best = ms;
+                    if (ms < best2) {
+                        best2 = ms;
                         best_zc = zc;
                     }
                 }
                 lv.zc2_32[kind] = best_zc;
-                lv.pair32[kind] = best < tbest[kind] + tbest[kSweepPlain];
+                if (best2 < best) {
+                    best = best2;
+                    Prec<T>::pair(lv, kind) = kPairMarch;
+                }
+            }
+            if (d.n() <= kBrickMaxCells) {
+                const float ms = time3([&]() {
+                    launch_brick2<T>(s, Prec<T>::A(lv), Prec<T>::t(lv), xin, Prec<T>::x(lv), d, 1, cor,
+                                     T(1), nullptr);
+                });
+                if (ms < 0.f) return JG_ERR_HIP;
+                if (tune_log())
+                    std::fprintf(stderr, "jetgpu tune: %dx%dx%d %s kind %d brick: %.1f us\n", d.nx, d.ny,
+                                 d.nz, sizeof(T) == 4 ? "f32" : "f64", kind, ms * 1e3f / 3);
+                if (ms < best) {
+                    best = ms;
+                    Prec<T>::pair(lv, kind) = kPairBrick;
+                }
             }
         }
     }
@@ -1416,6 +1478,8 @@ int batch_tune(jg_batch* B) {
                 for (int kind = 0; kind < kNumSweepKinds; ++kind) {
                     Prec<T>::zc(s->lv[l], kind) = Prec<T>::zc(t->lv[l], kind);
                     Prec<T>::sh(s->lv[l], kind) = Prec<T>::sh(t->lv[l], kind);
+                    Prec<T>::pair(s->lv[l], kind) = Prec<T>::pair(t->lv[l], kind);
+                    s->lv[l].zc2_32[kind] = t->lv[l].zc2_32[kind];
                 }
             copied = true;
         }

@@ -75,7 +75,7 @@ typedef struct {
     unsigned max_refine;     /* mixed: outer FP64 refinement steps (default 8) */
     double inner_rel_tol;    /* mixed: FP32 inner reduction per step (default 1e-4) */
     unsigned check_every;    /* host convergence poll interval in iterations (default 1) */
-    int sweep_pairs;         /* FP32 smoothing: two sweeps per launch where the tuner
+    int sweep_pairs;         /* smoothing: two sweeps per launch where the tuner
                                 measured it faster (0, default), wherever two sweeps
                                 follow each other (1), never (-1); the result is
                                 bit-identical either way */

@@ -12,9 +12,10 @@ path and by the one-workgroup bottom solve (FP32 bottoms start at 4096 cells,
 FP64 at 2048), a V-cycle that is entirely one k_bottom launch, a coarsest
 level too large for it, non-cubic grids with n.y > n.z (the prolongation's
 k clamp), no pre-smoothing (restriction of b, zero-initial fused correction)
-and no post-smoothing (separate correction kernel).  In FP32 two
-consecutive sweeps may run as one launch (k_rb_sweep2); that path is checked
-to be bit-identical to the single sweeps.
+and no post-smoothing (separate correction kernel).  Two consecutive sweeps
+may run as one launch (k_rb_sweep2 in FP32, the brick kernel on small levels
+in both precisions); those paths are checked to be bit-identical to the
+single sweeps.
 """
 import numpy as np
 import pytest
@@ -118,23 +119,28 @@ def test_vcycle_symmetric_positive_linear(solver, shape, levels, precision, tol)
 
 
 @pytest.mark.parametrize("shape,levels", [((64, 64, 64), 5), ((48, 32, 16), 3), ((32, 64, 16), 3),
-                                          ((40, 24, 20), 2), ((120, 56, 36), 2)])
+                                          ((40, 24, 20), 2), ((120, 56, 36), 2), ((96, 64, 48), 3),
+                                          ((34, 18, 10), 2)])
 @pytest.mark.parametrize("sched", [(2, 2, 20, 2), (3, 3, 6, 3), (2, 4, 6, 4), (1, 2, 4, 0)])
 def test_vcycle_two_sweep_kernel_bitwise(solver, oracle, shape, levels, sched):
-    # FP32 smoothing with two sweeps per launch (k_rb_sweep2) wherever two
-    # sweeps follow each other must give exactly the single-sweep result;
-    # shapes cover partial tiles in x (56-wide interiors) and y (24 rows),
-    # z extents that are not multiples of the chunk, and pairs that start
-    # with a zero-start, a correction or a plain sweep.
+    # Two sweeps per launch wherever two sweeps follow each other must give
+    # exactly the single-sweep result: in FP32 the z-marching k_rb_sweep2
+    # (levels above 64^3 cells) and the 8^3-brick kernel (below), in FP64 the
+    # brick kernel.  Shapes cover partial tiles in x (56-wide interiors), y
+    # (24 rows) and bricks (sizes not multiples of 8), z extents that are not
+    # multiples of the chunk, and pairs that start with a zero-start, a
+    # correction or a plain sweep.
     print(f"case two-sweep shape={shape} levels={levels} sched={sched}", flush=True)
     s = random_hierarchy(shape, levels, seed=41)
     upload(solver, s)
     gp, op = params(sched, 1.3)
     r = np.random.default_rng(42).uniform(-1, 1, s.x[0].shape)
-    single = apply(solver, gp, jetgpu.JG_MIXED, r, sweep_pairs=-1)
-    paired = apply(solver, gp, jetgpu.JG_MIXED, r, sweep_pairs=1)
-    assert np.array_equal(single, paired)
-    assert rel_err(paired, oracle.mg_clean_vcycle(s, op, r)) <= 2e-5
+    want = oracle.mg_clean_vcycle(s, op, r)
+    for prec, tol in ((jetgpu.JG_MIXED, 2e-5), (jetgpu.JG_FP64, 1e-12)):
+        single = apply(solver, gp, prec, r, sweep_pairs=-1)
+        paired = apply(solver, gp, prec, r, sweep_pairs=1)
+        assert np.array_equal(single, paired), prec
+        assert rel_err(paired, want) <= tol
 
 
 @pytest.mark.parametrize("shape,levels", [((32, 32, 32), 4), ((16, 16, 16), 3)])
