@@ -564,10 +564,9 @@ template <typename T>
 int pcg_init(jg_solver* s, const Pcg<T>& v) {
     const idx_t n = v.d.n();
     const int nb1 = nblocks_1d(n);
-    Geo g = geo_stencil(v.d);
-    // r = b - A x  (x = 0, so r = b; keep the general form)
-    k_residual<T, T, T, false, false><<<g.grid, g.block, 0, s->stream>>>(v.A, v.x, v.b, v.r, v.d, g.zc,
-                                                                        nullptr, nullptr);
+    // r = b - A x with x = 0 (every caller zeroes x first, as Jet's solvers
+    // do): b - A 0 is b bit for bit, so a copy
+    k_convert<T, T><<<nb1, NT, 0, s->stream>>>(v.b, v.r, n);
     k_fill<T><<<nb1, NT, 0, s->stream>>>(v.dA, T(0), n);
     k_fill<T><<<nb1, NT, 0, s->stream>>>(v.dB, T(0), n);
     const int nb = pcg_precondition<T>(s, v, nullptr);
