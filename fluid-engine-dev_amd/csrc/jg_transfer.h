@@ -106,11 +106,14 @@ __global__ __launch_bounds__(kRrThreads, sizeof(T) == 4 ? 4 : 2) void k_residual
     T w0 = 0, w1 = 0, w2 = 0, w3 = 0;  // x-y restricted planes p-3 .. p
 
     int K = K0;
-    for (int p2 = p0; p2 <= p1; p2 += 2) {
+    // An even number of planes, so the unrolled body has no exit (which would
+    // keep the compiler from counting the loads in flight): a plane past p1
+    // re-reads plane p1 and produces no coarse value.
+    const int pend = p0 + ((p1 - p0 + 2) & ~1);
+    for (int p2 = p0; p2 < pend; p2 += 2) {
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
             const int p = p2 + u;
-            if (p > p1) break;
 #pragma unroll
             for (int r = 0; r < 2; ++r) {
                 Cell& c = C[r];
