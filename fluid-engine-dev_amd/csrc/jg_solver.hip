@@ -612,7 +612,14 @@ int pcg_device(jg_solver* s, const Pcg<T>& v, unsigned check_every) {
     // One iteration of lookahead: iteration k+1 is enqueued before the host
     // waits for the stop flag iteration k left (every kernel of an iteration
     // returns at once once the control block says the solve has stopped), so
-    // the device does not idle while the host polls.
+    // the device does not idle while the host polls.  When the last two
+    // polled iterations predict (by their ratio) that the iteration in flight
+    // meets the stop test, the host waits first instead: a speculative
+    // iteration after the last one still costs its launches.
+    auto metric = [](const Ctl& c) { return c.stop_rule == 1 ? c.rr : c.sigma; };
+    const Ctl& c0 = *s->ctl_host;  // stable until the first poll below
+    const double lim = c0.stop_rule == 1 ? c0.rel2 * c0.bb : c0.tol2;
+    double m_prev = -1.0, m_last = metric(c0);
     unsigned launched = 0;
     int slot = 0;
     JG_TRY(pcg_iteration<T>(s, v, false));
@@ -624,17 +631,28 @@ int pcg_device(jg_solver* s, const Pcg<T>& v, unsigned check_every) {
                                   s->stream));
             JG_HIP(hipEventRecord(s->evp[slot], s->stream));
         }
-        const bool more = launched <= max_iter;  // the device stops by itself at max_iter
-        if (more) {
+        bool more = launched <= max_iter;  // the device stops by itself at max_iter
+        const bool likely_last = poll && m_prev > 0.0 && m_last > 0.0 && m_last < m_prev &&
+                                 m_last * (m_last / m_prev) <= lim;
+        if (more && !likely_last) {
             JG_TRY(pcg_iteration<T>(s, v, (launched & 1) != 0));
             ++launched;
+            more = false;  // already launched this pass
         }
         if (poll) {
             JG_HIP(hipEventSynchronize(s->evp[slot]));
-            if (s->ctl_host[slot].done) break;
+            const Ctl& c = s->ctl_host[slot];
+            if (c.done) break;
+            m_prev = m_last;
+            m_last = metric(c);
             slot ^= 1;
         }
-        if (!more) break;
+        if (more) {  // the prediction was wrong: go on
+            JG_TRY(pcg_iteration<T>(s, v, (launched & 1) != 0));
+            ++launched;
+        } else if (launched > max_iter) {
+            break;
+        }
     }
     // final state in slot 0 for the callers (the look-ahead iteration, a
     // no-op, has drained by then)
