@@ -14,7 +14,7 @@
 // sweeps give.  Each block also recomputes the 4 planes below and above its
 // chunk.
 //
-// x lives in a 6-plane LDS ring (planes t-5 .. t), the +x / +y couplings in
+// x lives in an LDS ring (planes t-5 .. t in 8 slots), the +x / +y couplings in
 // 4-plane rings (planes t-1 .. t-4; they are the -x / -y couplings of the
 // neighbours); w = omega / c, A.f and b stay in registers.  A thread owns two
 // vertically adjacent positions, which have opposite colours, so every
@@ -41,7 +41,8 @@ constexpr int kS2IX = kS2LX - 2 * kS2H;  // 56 interior columns
 constexpr int kS2IY = kS2LY - 2 * kS2H;  // 24 interior rows
 constexpr int kS2TY = 16;                // thread rows (two tile rows each)
 constexpr int kS2Threads = kS2LX * kS2TY;
-constexpr int kS2XR = 6, kS2CR = 4;      // x ring and coupling ring depths
+// x ring (6 planes are live; 8 slots make the slot a mask) and coupling ring
+constexpr int kS2XR = 8, kS2CR = 4;
 constexpr int kS2CzX = kS2LX / 2 + 2, kS2CzY = kS2LY / 2 + 2;
 static_assert(kS2CzX * kS2CzY <= kS2Threads, "one coarse entry per thread");
 static_assert(kS2IX % 2 == 0 && kS2IY % 2 == 0, "tile origins stay even");
@@ -55,7 +56,7 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
 #pragma clang fp contract(off)
     static_assert(sizeof(T) == 4, "two-sweep kernel is FP32 only");
     if (ctl && ctl->done) return;
-    __shared__ __align__(16) T sx[kS2XR][kS2LY][kS2LX];  // x, slot = plane mod 6
+    __shared__ __align__(16) T sx[kS2XR][kS2LY][kS2LX];  // x, slot = plane mod 8
     __shared__ T sr[kS2CR][kS2LY][kS2LX];                // A.r, slot = plane & 3
     __shared__ T su[kS2CR][kS2LY][kS2LX];                // A.u
     __shared__ T sz[CORRECT ? kS2CzY : 1][CORRECT ? kS2CzX : 1];
@@ -69,9 +70,9 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
     const int k1 = min(d.nz, k0 + zc);
     const idx_t sxy = (idx_t)d.nx * d.ny;
     const bool xok = gx >= 0 && gx < d.nx;
-    // planes are >= k0 - 5 >= -5: slot arithmetic on t + 12 stays non-negative
-    auto xs = [](int p) { return (p + 12) % kS2XR; };
-    auto cs = [](int p) { return (p + 12) & (kS2CR - 1); };
+    // planes are >= k0 - 5 >= -5: slot arithmetic on t + 16 stays non-negative
+    auto xs = [](int p) { return (p + 16) & (kS2XR - 1); };
+    auto cs = [](int p) { return (p + 16) & (kS2CR - 1); };
 
     // CORRECT: as in k_rb_sweep, thread e < kS2CzX * kS2CzY stages one coarse
     // column, z-interpolated for the plane about to be committed.
