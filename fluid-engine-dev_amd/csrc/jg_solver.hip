@@ -129,48 +129,45 @@ int launch_sweep(jg_solver* s, const Row<T>* A, const T* b, const T* xin, T* xou
     return p_bx ? (int)(grid.x * grid.y * grid.z) : 0;
 }
 
-// Two sweeps in one launch (jg_sweep2.h, FP32 only); same arguments and
-// return value as launch_sweep.
+// Two sweeps in one launch (jg_sweep2.h, FP32 only).  res != nullptr: also
+// write the residual b - A x of the result there (not with a coarse
+// correction).  Returns JG_OK, or JG_ERR_UNSUPPORTED for FP64 or for res with
+// a coarse correction.
 template <typename T>
 int launch_sweep2(jg_solver* s, const Row<T>* A, const T* b, const T* xin, T* xout, const Dims& d,
-                  int first, const T* coarse, double* p_bx, double* p_bb, T omega, const Ctl* ctl,
-                  int tuned_zc) {
+                  int first, const T* coarse, T* res, T omega, const Ctl* ctl, int tuned_zc) {
     if constexpr (sizeof(T) != 4) {
-        return 0;
+        return JG_ERR_UNSUPPORTED;
     } else {
-        int zc = std::min(tuned_zc > 0 ? tuned_zc : 32, d.nz);
-        const long cols = (long)((d.nx + kS2IX - 1) / kS2IX) * ((d.ny + kS2IY - 1) / kS2IY);
-        while (p_bx && zc < d.nz && cols * ((d.nz + zc - 1) / zc) > kMaxPartials) ++zc;
-        dim3 grid((d.nx + kS2IX - 1) / kS2IX, (d.ny + kS2IY - 1) / kS2IY, (d.nz + zc - 1) / zc);
-        if ((long)grid.x * grid.y * grid.z > kMaxPartials) p_bx = p_bb = nullptr;
+        if (res && coarse) return JG_ERR_UNSUPPORTED;
+        const int zc = std::min(tuned_zc > 0 ? tuned_zc : 32, d.nz);
+        const int ix = res ? S2Tile<true>::IX : S2Tile<false>::IX;
+        const int iy = res ? S2Tile<true>::IY : S2Tile<false>::IY;
+        dim3 grid((d.nx + ix - 1) / ix, (d.ny + iy - 1) / iy, (d.nz + zc - 1) / zc);
         const dim3 block(kS2LX, kS2TY, 1);
-        const int code = (first ? 8 : 0) | (xin ? 0 : 4) | (coarse ? 2 : 0) | (p_bx ? 1 : 0);
-#define JG_SWEEP2_CASE(C, F, Z, K, D)                                                     \
+        const int code = (first ? 8 : 0) | (xin ? 0 : 4) | (coarse ? 2 : 0) | (res ? 1 : 0);
+#define JG_SWEEP2_CASE(C, F, Z, K, R)                                                     \
     case C:                                                                               \
-        k_rb_sweep2<T, F, Z, K, D><<<grid, block, 0, s->stream>>>(A, b, xin, xout, d, zc, \
-                                                                  omega, coarse, p_bx,    \
-                                                                  p_bb, ctl);             \
+        k_rb_sweep2<T, F, Z, K, R><<<grid, block, 0, s->stream>>>(A, b, xin, xout, d, zc, \
+                                                                  omega, coarse, res,     \
+                                                                  ctl);                   \
         break;
         switch (code) {
             JG_SWEEP2_CASE(0, 0, false, false, false)
             JG_SWEEP2_CASE(1, 0, false, false, true)
             JG_SWEEP2_CASE(2, 0, false, true, false)
-            JG_SWEEP2_CASE(3, 0, false, true, true)
             JG_SWEEP2_CASE(4, 0, true, false, false)
             JG_SWEEP2_CASE(5, 0, true, false, true)
             JG_SWEEP2_CASE(6, 0, true, true, false)
-            JG_SWEEP2_CASE(7, 0, true, true, true)
             JG_SWEEP2_CASE(8, 1, false, false, false)
             JG_SWEEP2_CASE(9, 1, false, false, true)
             JG_SWEEP2_CASE(10, 1, false, true, false)
-            JG_SWEEP2_CASE(11, 1, false, true, true)
             JG_SWEEP2_CASE(12, 1, true, false, false)
             JG_SWEEP2_CASE(13, 1, true, false, true)
             JG_SWEEP2_CASE(14, 1, true, true, false)
-            JG_SWEEP2_CASE(15, 1, true, true, true)
         }
 #undef JG_SWEEP2_CASE
-        return p_bx ? (int)(grid.x * grid.y * grid.z) : 0;
+        return JG_OK;
     }
 }
 
@@ -316,8 +313,9 @@ int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, 
             if (pre_plan[m] == kLaunchBrick)
                 launch_brick2<T>(s, A, B(l), in, buf(m), d, 0, nullptr, c.omega, ctl);
             else if (pre_plan[m] == kLaunchMarch)
-                launch_sweep2<T>(s, A, B(l), in, buf(m), d, 0, nullptr, nullptr, nullptr, c.omega, ctl,
-                                 Prec<T>::zc2(lv, kind));
+                JG_TRY(launch_sweep2<T>(s, A, B(l), in, buf(m), d, 0, nullptr,
+                                        m + 1 == pre_plan.size() ? Prec<T>::t(lv) : nullptr,
+                                        c.omega, ctl, Prec<T>::zc2(lv, kind)));
             else
                 launch_sweep<T>(s, A, B(l), in, buf(m), d, 0, nullptr, nullptr, nullptr, c.omega, ctl,
                                 Prec<T>::zc(lv, kind), Prec<T>::sh(lv, kind));
@@ -338,8 +336,12 @@ int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, 
         } else if (in) {
             const int zc = rr_zc(cd);
             dim3 grid((cd.nx + kRrCX - 1) / kRrCX, (cd.ny + kRrCY - 1) / kRrCY, (cd.nz + zc - 1) / zc);
-            k_residual_restrict<T><<<grid, dim3(kRrLX, kRrTY, 1), 0, s->stream>>>(
-                A, in, B(l), Prec<T>::b(s->lv[l + 1]), d, cd, zc, ctl);
+            if (pre_plan.back() == kLaunchMarch)  // that launch wrote the residual into t
+                k_residual_restrict<T, true><<<grid, dim3(kRrLX, kRrTY, 1), 0, s->stream>>>(
+                    nullptr, nullptr, Prec<T>::t(lv), Prec<T>::b(s->lv[l + 1]), d, cd, zc, ctl);
+            else
+                k_residual_restrict<T><<<grid, dim3(kRrLX, kRrTY, 1), 0, s->stream>>>(
+                    A, in, B(l), Prec<T>::b(s->lv[l + 1]), d, cd, zc, ctl);
         } else {
             k_restrict<T><<<gc.grid, gc.block, 0, s->stream>>>(B(l), Prec<T>::b(s->lv[l + 1]), cd,
                                                                ctl);
@@ -378,8 +380,8 @@ int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, 
                 if (post_plan[q] == kLaunchBrick)
                     launch_brick2<T>(s, A, B(l), in, buf(m), d, 1, cor, c.omega, ctl);
                 else if (post_plan[q] == kLaunchMarch)
-                    nb = launch_sweep2<T>(s, A, B(l), in, buf(m), d, 1, cor, pbx, pbb, c.omega, ctl,
-                                          Prec<T>::zc2(lv, kind));
+                    JG_TRY(launch_sweep2<T>(s, A, B(l), in, buf(m), d, 1, cor, nullptr, c.omega,
+                                            ctl, Prec<T>::zc2(lv, kind)));
                 else
                     nb = launch_sweep<T>(s, A, B(l), in, buf(m), d, 1, cor, pbx, pbb, c.omega, ctl,
                                          Prec<T>::zc(lv, kind), Prec<T>::sh(lv, kind));
@@ -487,7 +489,7 @@ int autotune_sweeps(jg_solver* s) {
                     if (zc > d.nz && zc != cands2[0]) break;
                     const float ms = time3([&]() {
                         launch_sweep2<T>(s, Prec<T>::A(lv), Prec<T>::t(lv), xin, Prec<T>::x(lv), d, 1,
-                                         cor, nullptr, nullptr, T(1), nullptr, zc);
+                                         cor, nullptr, T(1), nullptr, zc);
                     });
                     if (ms < 0.f) return JG_ERR_HIP;
                     if (tune_log())
@@ -1132,7 +1134,7 @@ int time_kernel_t(jg_solver* s, const Vecs<T>& v, int kernel, unsigned reps, dou
                                 Prec<T>::zc(l0), Prec<T>::sh(l0));
                 break;
             case JG_KERNEL_SWEEP2:
-                launch_sweep2<T>(s, A, v.b, v.r, v.x, d, 1, nullptr, nullptr, nullptr, T(1), s->ctl,
+                launch_sweep2<T>(s, A, v.b, v.r, v.x, d, 1, nullptr, nullptr, T(1), s->ctl,
                                  Prec<T>::zc2(l0, kSweepPlain));
                 break;
             default:

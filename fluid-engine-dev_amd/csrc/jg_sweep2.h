@@ -24,9 +24,15 @@
 // contraction off, so the result is bit-identical to two single FP32 fused
 // sweeps.  Zero-centre rows are set to zero, as in the single sweep.
 //
-// Options as for the single sweep: ZERO (x starts at zero, xin not read),
-// CORRECT (the prolongation of a coarse correction is added to x as it is
-// loaded), DOT (b.x and b.b partials of the written x).
+// Options: ZERO (x starts at zero, xin not read), CORRECT (the prolongation
+// of a coarse correction is added to x as it is loaded), RES (the residual
+// b - A x of the two-sweep result is written too, for the restriction that
+// follows pre-smoothing).  RES needs final x one cell around every written
+// residual, so its tile has a 5-cell halo (a 54 x 22 interior, 5 extra planes
+// below and above the chunk).  The residual of plane q is evaluated in the
+// operand order of k_residual_restrict (Jet's FdmBlas3::residual order), so it
+// is bit-identical to that kernel's: all terms but the last once plane q is
+// final (after h4 on q), the +z term one step later (after h4 on q + 1).
 #pragma once
 
 #include <type_traits>
@@ -36,43 +42,57 @@
 
 namespace jg {
 
-constexpr int kS2LX = 64, kS2LY = 32, kS2H = 4;
-constexpr int kS2IX = kS2LX - 2 * kS2H;  // 56 interior columns
-constexpr int kS2IY = kS2LY - 2 * kS2H;  // 24 interior rows
-constexpr int kS2TY = 16;                // thread rows (two tile rows each)
+constexpr int kS2LX = 64, kS2LY = 32;
+constexpr int kS2TY = 16;  // thread rows (two tile rows each)
 constexpr int kS2Threads = kS2LX * kS2TY;
-// x ring (6 planes are live; 8 slots make the slot a mask) and coupling ring
-constexpr int kS2XR = 8, kS2CR = 4;
+// Halo (in x, y and z) and interior of the tile: 4 cells for the two sweeps,
+// 5 when the residual is written too.
+template <bool RES>
+struct S2Tile {
+    static constexpr int H = RES ? 5 : 4;
+    static constexpr int IX = kS2LX - 2 * H, IY = kS2LY - 2 * H;
+    // r / u coupling ring: planes t-1 .. t-4 for the half-sweeps; RES reads
+    // plane t-4 after the last barrier of step t, before the next commit
+    // (which writes plane t) has passed a barrier, so it needs a fifth slot
+    static constexpr int CR = RES ? 5 : 4;
+    static_assert(IX % 2 == 0 && IY % 2 == 0, "tile interiors stay even");
+};
+// x ring: planes t-7 .. t are live (8 slots make the slot a mask)
+constexpr int kS2XR = 8;
 constexpr int kS2CzX = kS2LX / 2 + 2, kS2CzY = kS2LY / 2 + 2;
 static_assert(kS2CzX * kS2CzY <= kS2Threads, "one coarse entry per thread");
-static_assert(kS2IX % 2 == 0 && kS2IY % 2 == 0, "tile origins stay even");
 
 // FP32 only: the FP64 rings would need 234 KB of LDS.
-template <typename T, int FIRST, bool ZERO, bool CORRECT, bool DOT>
+template <typename T, int FIRST, bool ZERO, bool CORRECT, bool RES>
 __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
     const Row<T>* __restrict__ A, const T* __restrict__ b, const T* __restrict__ xin,
     T* __restrict__ xout, Dims d, int zc, T omega, const T* __restrict__ coarse,
-    double* __restrict__ p_bx, double* __restrict__ p_bb, const Ctl* __restrict__ ctl) {
+    T* __restrict__ res, const Ctl* __restrict__ ctl) {
 #pragma clang fp contract(off)
     static_assert(sizeof(T) == 4, "two-sweep kernel is FP32 only");
+    // CORRECT's prolongation weights take the parity of x and y from the
+    // tile-local index, which needs the even origins of the 4-cell halo
+    static_assert(!(RES && CORRECT), "the residual variant has no correction");
+    using G = S2Tile<RES>;
+    constexpr int kH = G::H, kCR = G::CR;
     if (ctl && ctl->done) return;
     __shared__ __align__(16) T sx[kS2XR][kS2LY][kS2LX];  // x, slot = plane mod 8
-    __shared__ T sr[kS2CR][kS2LY][kS2LX];                // A.r, slot = plane & 3
-    __shared__ T su[kS2CR][kS2LY][kS2LX];                // A.u
+    __shared__ T sr[kCR][kS2LY][kS2LX];                  // A.r, slot = plane mod kCR
+    __shared__ T su[kCR][kS2LY][kS2LX];                  // A.u
     __shared__ T sz[CORRECT ? kS2CzY : 1][CORRECT ? kS2CzX : 1];
 
     const int tx = threadIdx.x, ty = threadIdx.y;
     int bx, by, bz;
     xcd_tile(bx, by, bz);
-    const int gx0 = bx * kS2IX - kS2H, gy0 = by * kS2IY - kS2H;  // both even
+    const int gx0 = bx * G::IX - kH, gy0 = by * G::IY - kH;  // even unless RES
     const int gx = gx0 + tx;
     const int k0 = bz * zc;
     const int k1 = min(d.nz, k0 + zc);
     const idx_t sxy = (idx_t)d.nx * d.ny;
     const bool xok = gx >= 0 && gx < d.nx;
-    // planes are >= k0 - 5 >= -5: slot arithmetic on t + 16 stays non-negative
-    auto xs = [](int p) { return (p + 16) & (kS2XR - 1); };
-    auto cs = [](int p) { return (p + 16) & (kS2CR - 1); };
+    // planes are >= k0 - 6 >= -6: slot arithmetic on p + 24 stays non-negative
+    auto xs = [](int p) { return (unsigned)(p + 24) & (kS2XR - 1); };
+    auto cs = [](int p) { return (unsigned)(p + 24) % (unsigned)kCR; };
 
     // CORRECT: as in k_rb_sweep, thread e < kS2CzX * kS2CzY stages one coarse
     // column, z-interpolated for the plane about to be committed.
@@ -113,10 +133,16 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
     // w = omega / c (0 for a zero-centre row) and b of planes t .. t-4, f of
     // planes t .. t-5, the held r / u of plane t (written to LDS at the next
     // commit), the pending final value, and the prefetched planes.
+    // RES also keeps A.c of planes t .. t-4 and the partial residual of
+    // plane t-4 (rv);
+    // to stay within the registers of 1024 threads it prefetches one plane
+    // ahead instead of two.
+    constexpr int kC = RES ? 5 : 1;
+    constexpr int kPD = RES ? 1 : 2;
     struct Pos {
-        T w[5], bb[5], f[6];
-        T hr, hu, px;
-        Pre p[2];  // planes in flight: slot (t - t0) & 1 holds plane t
+        T w[5], bb[5], f[6], cc[kC];
+        T hr, hu, px, rv;
+        Pre p[kPD];  // planes in flight: slot (t - t0) % kPD holds plane t
         int ly, gy;
         bool in4;  // in the written interior
     };
@@ -126,7 +152,7 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
         Pos& Q = P[r];
         Q.ly = 2 * ty + r;
         Q.gy = gy0 + Q.ly;
-        Q.in4 = tx >= kS2H && tx < kS2LX - kS2H && Q.ly >= kS2H && Q.ly < kS2LY - kS2H && xok &&
+        Q.in4 = tx >= kH && tx < kS2LX - kH && Q.ly >= kH && Q.ly < kS2LY - kH && xok &&
                 Q.gy >= 0 && Q.gy < d.ny;
 #pragma unroll
         for (int s = 0; s < 5; ++s) {
@@ -135,11 +161,13 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
         }
 #pragma unroll
         for (int s = 0; s < 6; ++s) Q.f[s] = T(0);
-        Q.hr = Q.hu = Q.px = T(0);
+#pragma unroll
+        for (int s = 0; s < kC; ++s) Q.cc[s] = T(0);
+        Q.hr = Q.hu = Q.px = Q.rv = T(0);
 #pragma unroll
         for (int s = 0; s < kS2XR; ++s) sx[s][Q.ly][tx] = T(0);
 #pragma unroll
-        for (int s = 0; s < kS2CR; ++s) {
+        for (int s = 0; s < kCR; ++s) {
             sr[s][Q.ly][tx] = T(0);
             su[s][Q.ly][tx] = T(0);
         }
@@ -162,6 +190,11 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
         for (int s = 5; s > 0; --s) Q.f[s] = Q.f[s - 1];
         const T c = q.ok ? q.row.c : T(1);
         Q.w[0] = c == T(0) ? T(0) : omega / c;
+        if (RES) {
+#pragma unroll
+            for (int s = kC - 1; s > 0; --s) Q.cc[s] = Q.cc[s - 1];
+            Q.cc[0] = q.row.c;
+        }
         Q.f[0] = q.ok ? q.row.f : T(0);
         Q.bb[0] = q.ok ? q.b : T(0);
         T xv = q.ok ? q.x : T(0);
@@ -187,7 +220,6 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
     // update x = (1 - w) x + w (b - s) with w = omega / c precomputed (the
     // operators of Jet's relaxRedBlack, rounded differently from its
     // omega (b - s) / c; FP64 sweeps keep Jet's expression).
-    double acc = 0.0, acc2 = 0.0;
     auto half = [&](auto hc, int p) __attribute__((always_inline)) {
         constexpr int H = decltype(hc)::value;
         const int colour = (H & 1) ? FIRST : 1 - FIRST;
@@ -224,99 +256,108 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
     // After h4 on plane q: the final values of both positions (the one h4
     // updated and the one h3 left) become the pending stores.
     auto finish = [&](int q) __attribute__((always_inline)) {
-        const int sq = xs(q);
+        const unsigned sq = xs(q);
 #pragma unroll
-        for (int r = 0; r < 2; ++r) {
-            Pos& Q = P[r];
-            Q.px = sx[sq][Q.ly][tx];
-            if (DOT && Q.in4) {
-                acc += (double)Q.bb[4] * (double)Q.px;
-                acc2 += (double)Q.bb[4] * (double)Q.bb[4];
-            }
-        }
+        for (int r = 0; r < 2; ++r) P[r].px = sx[sq][P[r].ly][tx];
     };
     auto flush = [&](int q) __attribute__((always_inline)) {
 #pragma unroll
         for (int r = 0; r < 2; ++r)
             if (P[r].in4) xout[gx + (idx_t)d.nx * P[r].gy + sxy * q] = P[r].px;
     };
+    // RES at step t, after h4 on plane t - 4: the +z term of plane t - 5's
+    // residual, which is then stored, and plane t - 4's without that term, at
+    // both positions.  k_residual_restrict's operand order and masks (Jet's
+    // FdmBlas3::residual, src/jet/fdm_linear_system3.cpp:107-115); outside
+    // the grid x, A.f and the couplings in LDS are zero.
+    auto resid_finish = [&](int t) __attribute__((always_inline)) {
+        const unsigned sn = xs(t - 4);
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+            if (P[r].in4)
+                res[gx + (idx_t)d.nx * P[r].gy + sxy * (t - 5)] =
+                    P[r].rv - P[r].f[5] * sx[sn][P[r].ly][tx];
+    };
+    // The -x / -y terms need no mask: past the grid edge the LDS coupling and
+    // x are both +0, so the product is the +0 the masked form subtracts.
+    auto resid_start = [&](int t) __attribute__((always_inline)) {
+        const unsigned sq = xs(t - 4), sm = xs(t - 5), s1 = cs(t - 4);
+        const bool hr = gx + 1 < d.nx;
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            Pos& Q = P[r];
+            if (!Q.in4) continue;  // the written interior: ring >= 5, inside the grid
+            const int ly = Q.ly;
+            const bool hu = Q.gy + 1 < d.ny;
+            T rv = Q.bb[4] - Q.cc[kC - 1] * sx[sq][ly][tx];
+            rv -= sr[s1][ly][tx - 1] * sx[sq][ly][tx - 1];
+            rv -= hr ? sr[s1][ly][tx] * sx[sq][ly][tx + 1] : T(0);
+            rv -= su[s1][ly - 1][tx] * sx[sq][ly - 1][tx];
+            rv -= hu ? su[s1][ly][tx] * sx[sq][ly + 1][tx] : T(0);
+            rv -= Q.f[5] * sx[sm][ly][tx];
+            Q.rv = rv;
+        }
+    };
+
     using H1 = std::integral_constant<int, 1>;
     using H2 = std::integral_constant<int, 2>;
     using H3 = std::integral_constant<int, 3>;
     using H4 = std::integral_constant<int, 4>;
 
-    // Planes k0-4 .. k1+3 are committed; half-sweep H covers planes
-    // k0-4+H .. k1+4-H (those within the grid): exactly the planes the
-    // next half-sweep reads for the chunk's planes k0 .. k1-1.
+    // Planes k0-kH .. k1+kH-1 are committed; half-sweep H covers planes
+    // k0-kH+H .. k1+kH-1-H (those within the grid): exactly the planes the
+    // next half-sweep reads for the chunk's planes k0 .. k1-1 (and, for RES,
+    // h4 covers k0-1 .. k1, the planes the residual reads).
     // Two planes of loads are in flight ahead of the committed one; the loop
     // is unrolled twice so the prefetch slot of every step is static.
-    const int t0 = k0 - 4, t1 = k1 + 3;  // t1 - t0 >= 8
+    const int t0 = k0 - kH, t1 = k1 + kH - 1;  // t1 - t0 >= 8
+    auto live = [&](int p, int H) { return p >= max(k0 - kH + H, 0) && p < min(k1 + kH - H, d.nz); };
     if (CORRECT) zfetch(t0);
     // plane by plane, in the order the loop issues them
 #pragma unroll
     for (int r = 0; r < 2; ++r) fetch(P[r].p[0], P[r].gy, t0);
+    if (kPD == 2) {
 #pragma unroll
-    for (int r = 0; r < 2; ++r) fetch(P[r].p[1], P[r].gy, t0 + 1);
+        for (int r = 0; r < 2; ++r) fetch(P[r].p[kPD - 1], P[r].gy, t0 + 1);
+    }
     if (CORRECT) zstore(t0);
     __syncthreads();
 
     // An even number of steps, so the unrolled body has no exit: a last
     // step past t1 commits a re-read plane and runs no half-sweep.
     const int tend = t0 + ((t1 - t0 + 2) & ~1);
-    bool pending = false;  // uniform: the pending stores hold plane t - 5
+    bool pending = false;  // uniform: the pending x stores hold plane t - 5
     for (int t2 = t0; t2 < tend; t2 += 2) {
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
             const int t = t2 + u;
 #pragma unroll
-            for (int r = 0; r < 2; ++r) commit(P[r], P[r].p[u], t);
+            for (int r = 0; r < 2; ++r) commit(P[r], P[r].p[u % kPD], t);
             if (CORRECT && t + 1 <= t1) zfetch(t + 1);
             // unconditional (past the end it re-reads plane t1, unused) so the
             // compiler can count the loads in flight and wait only for plane t
 #pragma unroll
-            for (int r = 0; r < 2; ++r) fetch(P[r].p[u], P[r].gy, min(t + 2, t1));
+            for (int r = 0; r < 2; ++r) fetch(P[r].p[u % kPD], P[r].gy, min(t + kPD, t1));
             if (pending) flush(t - 5);
             __syncthreads();
-            if (t - 1 >= max(k0 - 3, 0) && t - 1 < min(k1 + 3, d.nz)) half(H1{}, t - 1);
+            if (live(t - 1, 1)) half(H1{}, t - 1);
             __syncthreads();
-            if (t - 2 >= max(k0 - 2, 0) && t - 2 < min(k1 + 2, d.nz)) half(H2{}, t - 2);
+            if (live(t - 2, 2)) half(H2{}, t - 2);
             __syncthreads();
-            if (t - 3 >= max(k0 - 1, 0) && t - 3 < min(k1 + 1, d.nz)) half(H3{}, t - 3);
+            if (live(t - 3, 3)) half(H3{}, t - 3);
             __syncthreads();
             pending = t - 4 >= k0 && t - 4 < k1;
-            if (pending) half(H4{}, t - 4);
+            if (live(t - 4, 4)) half(H4{}, t - 4);
             if (CORRECT && t + 1 <= t1) zstore(t + 1);
             __syncthreads();
             if (pending) finish(t - 4);
+            if (RES) {
+                if (t - 5 >= k0 && t - 5 < k1) resid_finish(t);
+                if (pending) resid_start(t);
+            }
         }
     }
     if (pending) flush(tend - 5);
-    if (DOT) {
-        __syncthreads();
-        double* sh = reinterpret_cast<double*>(&sx[0][0][0]);
-        constexpr int W = kS2Threads / kWave;
-#pragma unroll
-        for (int o = kWave / 2; o > 0; o >>= 1) {
-            acc += __shfl_down(acc, o, kWave);
-            acc2 += __shfl_down(acc2, o, kWave);
-        }
-        if ((e & (kWave - 1)) == 0) {
-            sh[e / kWave] = acc;
-            sh[W + e / kWave] = acc2;
-        }
-        __syncthreads();
-        if (e == 0) {
-            double s0 = 0.0, s1 = 0.0;
-#pragma unroll
-            for (int w = 0; w < W; ++w) {
-                s0 += sh[w];
-                s1 += sh[W + w];
-            }
-            const idx_t blk = blockIdx.x + (idx_t)gridDim.x * (blockIdx.y + (idx_t)gridDim.y * blockIdx.z);
-            p_bx[blk] = s0;
-            p_bb[blk] = s1;
-        }
-    }
 }
 
 }  // namespace jg

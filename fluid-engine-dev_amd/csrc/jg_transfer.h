@@ -27,7 +27,17 @@ constexpr int kRrLX = 64, kRrTY = 8, kRrLY = 2 * kRrTY;  // fine tile, thread ro
 constexpr int kRrThreads = kRrLX * kRrTY;
 static_assert(2 * kRrCX + 2 == kRrLX && 2 * kRrCY + 2 == kRrLY, "tile taps");
 
-template <typename T>
+// Element o of a plane that starts at a block-uniform base: the byte offset
+// is formed in 32 bits, so the load takes the base in scalar registers and
+// only a 32-bit offset per lane (a plane holds far fewer than 2^32 bytes).
+template <typename U>
+__device__ __forceinline__ U plane_at(const U* base, unsigned o) {
+    return *reinterpret_cast<const U*>(reinterpret_cast<const char*>(base) + o * (unsigned)sizeof(U));
+}
+
+// FROMT: b already holds the fine residual (written by the pre-smoothing
+// launch); A and x are not read, and the restriction is the same.
+template <typename T, bool FROMT = false>
 __global__ __launch_bounds__(kRrThreads, sizeof(T) == 4 ? 4 : 2) void k_residual_restrict(
     const Row<T>* __restrict__ A, const T* __restrict__ x, const T* __restrict__ b,
     T* __restrict__ cb, Dims d, Dims cd, int zc, const Ctl* __restrict__ ctl) {
@@ -44,7 +54,6 @@ __global__ __launch_bounds__(kRrThreads, sizeof(T) == 4 ? 4 : 2) void k_residual
     const idx_t sz = (idx_t)d.nx * d.ny;
     const int gx = fx0 + tx;
     const bool xok = gx >= 0 && gx < d.nx;
-    const T* const src[1] = {x};
 
     // Two fine positions per thread (rows ty and ty + kRrTY of the tile).
     // Planes are double-buffered in registers (buf[p & 1] for plane p, with
@@ -56,7 +65,7 @@ __global__ __launch_bounds__(kRrThreads, sizeof(T) == 4 ? 4 : 2) void k_residual
         int ly;
         bool ok;  // inside the fine grid
         StCol h;
-        idx_t c0;  // index in plane 0
+        unsigned col;  // index within a plane (0 outside the grid)
         StPlane<T, T, 1> buf[2];
         T bb[2], xz[2];
         T xm, fm;
@@ -65,17 +74,30 @@ __global__ __launch_bounds__(kRrThreads, sizeof(T) == 4 ? 4 : 2) void k_residual
     // Fine planes of this chunk: the taps of coarse planes K0 .. K1-1.
     const int p0 = max(2 * K0 - 1, 0), p1 = min(2 * K1, d.nz - 1);
     // Plane q of this chunk (clamped to p1: the load past the chunk end reads
-    // cached data and is never used); the +z centre value of plane q.
+    // cached data and is never used).  Addresses are a plane base common to
+    // the block plus a 32-bit offset within the plane.
     auto load_plane = [&](Cell& c, int slot, int q) __attribute__((always_inline)) {
-        const int qc = min(q, p1);
-        const idx_t cc = c.c0 + (c.ok ? sz * qc : 0);
-        st_load(c.buf[slot], A, src, cc, cc, c.h, d.nx);
-        c.bb[slot] = b[cc];
+        const idx_t base = sz * min(q, p1);
+        c.bb[slot] = plane_at(b + base, c.col);
+        if (FROMT) return;
+        const Row<T>* Ap = A + base;
+        const T* xp = x + base;
+        const unsigned o = c.col;
+        const unsigned ol = c.h.l ? o - 1 : o, orr = c.h.r ? o + 1 : o;
+        const unsigned od = c.h.d ? o - d.nx : o, ou = c.h.u ? o + d.nx : o;
+        StPlane<T, T, 1>& P = c.buf[slot];
+        P.row = plane_at(Ap, o);
+        P.rl = plane_at(Ap, ol).r;
+        P.ud = plane_at(Ap, od).u;
+        P.v[0][0] = plane_at(xp, o);
+        P.v[0][1] = plane_at(xp, ol);
+        P.v[0][2] = plane_at(xp, orr);
+        P.v[0][3] = plane_at(xp, od);
+        P.v[0][4] = plane_at(xp, ou);
     };
     auto load_xz = [&](Cell& c, int slot, int q) __attribute__((always_inline)) {
-        // x at plane q + 1 (0 past the grid), for plane q's residual
-        const bool zp = c.ok && q + 1 < d.nz;
-        c.xz[slot] = x[zp ? c.c0 + sz * (q + 1) : 0];
+        // x at plane q + 1 for plane q's residual (masked past the grid at use)
+        if (!FROMT) c.xz[slot] = plane_at(x + sz * min(q + 1, d.nz - 1), c.col);
     };
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
@@ -85,12 +107,15 @@ __global__ __launch_bounds__(kRrThreads, sizeof(T) == 4 ? 4 : 2) void k_residual
         c.ok = xok && gy >= 0 && gy < d.ny;
         c.h = c.ok ? StCol{gx > 0, gx + 1 < d.nx, gy > 0, gy + 1 < d.ny}
                    : StCol{false, false, false, false};
-        c.c0 = c.ok ? gx + (idx_t)d.nx * gy : 0;  // outside: cell 0, result unused
+        c.col = c.ok ? (unsigned)(gx + d.nx * gy) : 0u;  // outside: column 0, result unused
         load_plane(c, 0, p0);
         load_xz(c, 0, p0);
-        const idx_t cc = c.c0 + (c.ok ? sz * p0 : 0);
-        c.xm = (c.ok && p0 > 0) ? x[cc - sz] : T(0);
-        c.fm = (c.ok && p0 > 0) ? A[cc - sz].f : T(0);
+        c.xm = c.fm = T(0);
+        if (!FROMT && c.ok && p0 > 0) {
+            const idx_t below = sz * (p0 - 1);
+            c.xm = plane_at(x + below, c.col);
+            c.fm = plane_at(A + below, c.col).f;
+        }
     }
 
     // Coarse column of threads e < kRrCX * kRrCY and its x / y taps in the
@@ -120,19 +145,22 @@ __global__ __launch_bounds__(kRrThreads, sizeof(T) == 4 ? 4 : 2) void k_residual
                 // loads of plane p + 1 into the other slot
                 load_plane(c, u ^ 1, p + 1);
                 load_xz(c, u ^ 1, p + 1);
-                const T xp = c.ok && p + 1 < d.nz ? c.xz[u] : T(0);
-                // Jet's residual operand order (src/jet/fdm_linear_system3.cpp:107-115)
-                const StPlane<T, T, 1>& q = c.buf[u];
-                T rv = c.bb[u] - q.row.c * q.v[0][0];
-                rv -= c.h.l ? q.rl * q.v[0][1] : T(0);
-                rv -= c.h.r ? q.row.r * q.v[0][2] : T(0);
-                rv -= c.h.d ? q.ud * q.v[0][3] : T(0);
-                rv -= c.h.u ? q.row.u * q.v[0][4] : T(0);
-                rv -= c.fm * c.xm;
-                rv -= q.row.f * xp;
+                T rv = c.bb[u];
+                if (!FROMT) {
+                    const T xp = c.ok && p + 1 < d.nz ? c.xz[u] : T(0);
+                    // Jet's residual operand order (src/jet/fdm_linear_system3.cpp:107-115)
+                    const StPlane<T, T, 1>& q = c.buf[u];
+                    rv = rv - q.row.c * q.v[0][0];
+                    rv -= c.h.l ? q.rl * q.v[0][1] : T(0);
+                    rv -= c.h.r ? q.row.r * q.v[0][2] : T(0);
+                    rv -= c.h.d ? q.ud * q.v[0][3] : T(0);
+                    rv -= c.h.u ? q.row.u * q.v[0][4] : T(0);
+                    rv -= c.fm * c.xm;
+                    rv -= q.row.f * xp;
+                    c.xm = q.v[0][0];
+                    c.fm = q.row.f;
+                }
                 sres[p & 1][c.ly][tx] = c.ok ? rv : T(0);
-                c.xm = q.v[0][0];
-                c.fm = q.row.f;
             }
             __syncthreads();
             // x-y restriction of plane p for this thread's coarse column (the

@@ -120,14 +120,17 @@ def test_vcycle_symmetric_positive_linear(solver, shape, levels, precision, tol)
 
 @pytest.mark.parametrize("shape,levels", [((64, 64, 64), 5), ((48, 32, 16), 3), ((32, 64, 16), 3),
                                           ((40, 24, 20), 2), ((120, 56, 36), 2), ((96, 64, 48), 3),
-                                          ((34, 18, 10), 2)])
+                                          ((34, 18, 10), 2), ((130, 70, 44), 2), ((108, 44, 60), 3)])
 @pytest.mark.parametrize("sched", [(2, 2, 20, 2), (3, 3, 6, 3), (2, 4, 6, 4), (1, 2, 4, 0)])
 def test_vcycle_two_sweep_kernel_bitwise(solver, oracle, shape, levels, sched):
     # Two sweeps per launch wherever two sweeps follow each other must give
     # exactly the single-sweep result: in FP32 the z-marching k_rb_sweep2
     # (levels above 64^3 cells) and the 8^3-brick kernel (below), in FP64 the
-    # brick kernel.  Shapes cover partial tiles in x (56-wide interiors), y
-    # (24 rows) and bricks (sizes not multiples of 8), z extents that are not
+    # brick kernel.  Where the pre-smoothing ends with a z-marching pair, that
+    # launch also writes the residual for the restriction (54 x 22 interiors),
+    # which must equal the separate residual-restriction's bit for bit.
+    # Shapes cover partial tiles in x (56- and 54-wide interiors), y (24 and
+    # 22 rows) and bricks (sizes not multiples of 8), z extents that are not
     # multiples of the chunk, and pairs that start with a zero-start, a
     # correction or a plain sweep.
     print(f"case two-sweep shape={shape} levels={levels} sched={sched}", flush=True)
