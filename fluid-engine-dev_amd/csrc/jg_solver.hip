@@ -223,11 +223,11 @@ std::vector<int> sweep_plan(const Level& lv, unsigned n, int kind_first, int pai
     return v;
 }
 
-// Coarse planes per block of the fused residual-restriction: enough blocks
-// to fill the chip; each block recomputes one fine plane per chunk.
-int rr_zc(const Dims& cd) {
+// Coarse planes per block of the fused residual-restriction: about `blocks`
+// blocks, to fill the chip; each block recomputes one fine plane per chunk.
+int rr_zc(const Dims& cd, long blocks = 1024) {
     const long tiles = (long)((cd.nx + kRrCX - 1) / kRrCX) * ((cd.ny + kRrCY - 1) / kRrCY);
-    long zc = (long)cd.nz * tiles / 1024;
+    long zc = (long)cd.nz * tiles / blocks;
     zc = std::max<long>(2, std::min<long>(32, zc));
     return (int)std::min<long>(zc, cd.nz);
 }
@@ -334,9 +334,12 @@ int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, 
             k_restrict<T><<<gc.grid, gc.block, 0, s->stream>>>(Prec<T>::t(lv), Prec<T>::b(s->lv[l + 1]),
                                                                cd, ctl);
         } else if (in) {
-            const int zc = rr_zc(cd);
+            const bool from_t = pre_plan.back() == kLaunchMarch;  // that launch wrote the residual into t
+            // restricting a stored residual is one load per cell: twice the
+            // blocks (shorter chunks) keep more loads in flight
+            const int zc = from_t ? rr_zc(cd, 2048) : rr_zc(cd);
             dim3 grid((cd.nx + kRrCX - 1) / kRrCX, (cd.ny + kRrCY - 1) / kRrCY, (cd.nz + zc - 1) / zc);
-            if (pre_plan.back() == kLaunchMarch)  // that launch wrote the residual into t
+            if (from_t)
                 k_residual_restrict<T, true><<<grid, dim3(kRrLX, kRrTY, 1), 0, s->stream>>>(
                     nullptr, nullptr, Prec<T>::t(lv), Prec<T>::b(s->lv[l + 1]), d, cd, zc, ctl);
             else
