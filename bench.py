@@ -178,10 +178,70 @@ def main():
                            * n_cells / (res["mixed"]["ms"] * 1e-3) / 1e9, 1)},
     }
 
+    bc = bc_leg(jg, solver, n, args)
+    out["constrain_velocity"] = bc["gpu"]
     if not args.no_cpu:
         out.update(cpu_leg(jg, n, L, args, res))
+        out["constrain_velocity"].update(bc_cpu_leg(bc, n))
     print(json.dumps(out))
     solver.close()
+
+
+BC_DEPTH = 5  # ceil(max CFL) with GridFluidSolver3's default max CFL of 5
+
+
+def bc_leg(jg, solver, n, args):
+    """The step after the projection: GridBlockedBoundaryConditionSolver3::
+    constrainVelocity on the same scene's faces, its spheres as a resting
+    collider (cell-centre SDF), extrapolation depth 5, all walls closed."""
+    import torch
+
+    f64 = torch.float64
+    shapes = [(n, n, n + 1), (n, n + 1, n), (n + 1, n, n)]
+    u, v, w = (torch.empty(sh, dtype=f64, device="cuda") for sh in shapes)
+    bs = torch.empty((n, n, n), dtype=f64, device="cuda")
+    fs = torch.empty((n, n, n), dtype=f64, device="cuda")
+    desc = jg.scene_desc(jg.JG_SCENE_SPHERE_POOL, n, seed=args.seed)
+    solver.generate_scene(desc, u, v, w, bs, fs)
+    torch.cuda.synchronize()
+    vel0 = [a.cpu().numpy() for a in (u, v, w)] if not args.no_cpu else None
+    h, o = (1.0, 1.0, 1.0), (0.0, 0.0, 0.0)
+    call = lambda: solver.constrain_velocity(u, v, w, bs, h, o, None, BC_DEPTH, jg.JG_DIR_ALL, True)
+    call()  # first call: scratch allocation, and the result checked against the oracle
+    first = [a.cpu().numpy() for a in (u, v, w)] if vel0 is not None else None
+    reps = 10
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    faces = sum(a.numel() for a in (u, v, w))
+    log(f"bench: constrain_velocity {ms:.3f} ms ({faces} faces)")
+    bsh = bs.cpu().numpy() if vel0 is not None else None
+    del u, v, w, bs, fs
+    torch.cuda.empty_cache()
+    return {"gpu": {"gpu_ms": round(ms, 4), "faces": faces, "depth": BC_DEPTH, "blocked": True,
+                    "collider": "the scene's spheres, resting (cell-centre SDF)"},
+            "vel0": vel0, "first": first, "sdf": bsh}
+
+
+def bc_cpu_leg(bc, n):
+    """The oracle's constrainVelocity on the host (one core) on the same
+    input: the CPU time beside the GPU's, and bitwise equality of the first
+    GPU call's result."""
+    import numpy as np
+
+    from oracle.jet_oracle import DIR_ALL, Oracle
+
+    t0 = time.perf_counter()
+    got = Oracle().constrain_velocity(*bc["vel0"], bc["sdf"], (1.0,) * 3, (0.0,) * 3, None,
+                                      BC_DEPTH, DIR_ALL, True)
+    cpu_ms = (time.perf_counter() - t0) * 1e3
+    same = all(np.array_equal(a, b) for a, b in zip(got, bc["first"]))
+    log(f"bench: constrain_velocity oracle {cpu_ms:.0f} ms, GPU result bitwise equal: {same}")
+    return {"cpu_ms": round(cpu_ms, 1), "cpu_cores": 1, "cpu_kind": "port",
+            "gpu_bitwise_equal_to_oracle": bool(same)}
 
 
 def cpu_leg(jg, n, L, args, res):

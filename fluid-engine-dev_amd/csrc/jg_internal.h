@@ -148,6 +148,9 @@ struct jg_solver {
            *q = nullptr;
     float *rhs32 = nullptr, *r32 = nullptr, *z32 = nullptr, *d32A = nullptr, *d32B = nullptr,
           *q32 = nullptr, *e32 = nullptr;
+    // Velocity boundary conditions: projected faces, two validity masks.
+    double* bc_tmp = nullptr;
+    int8_t* bc_valid = nullptr;
     // Reduction slots and device scalars.
     double* part = nullptr;  // kSlots * kMaxPartials
     double* scal = nullptr;  // small result area
@@ -174,6 +177,8 @@ struct jg_solver {
         lv.clear();
         b = x = r = z = dA = dB = q = nullptr;
         rhs32 = r32 = z32 = d32A = d32B = q32 = e32 = nullptr;
+        bc_tmp = nullptr;
+        bc_valid = nullptr;
         part = scal = nullptr;
         ctl = nullptr;
         f32_valid = false;

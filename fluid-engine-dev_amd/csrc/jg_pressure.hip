@@ -5,6 +5,7 @@
 #include "jg_internal.h"
 #include "jg_pressure.h"
 #include "jg_fractional.h"
+#include "jg_boundary.h"
 
 using namespace jg;
 
@@ -290,6 +291,71 @@ int jg_fr_apply_gradient(jg_solver* s, const double* u, const double* v, const d
     k_fr_gradient<<<nblocks_1d(l0.d.n()), 256, 0, s->stream>>>(l0.fs, l0.uw, l0.vw, l0.ww, s->x, u, v,
                                                                 w, uo, vo, wo, l0.d, H3{hx, hy, hz});
     JG_TRY(check_launch(s, "fr_gradient"));
+    return sync(s);
+}
+
+int jg_constrain_velocity(jg_solver* s, int blocked, const double h[3], const double o[3],
+                          double* u, double* v, double* w, const double* collider_sdf,
+                          const jg_rigid_collider* collider, unsigned extrapolation_depth,
+                          int closed_walls) {
+    if (!s || s->lv.empty() || !h || !o || !u || !v || !w || !collider_sdf ||
+        !(h[0] > 0) || !(h[1] > 0) || !(h[2] > 0))
+        return JG_ERR_INVALID_ARG;
+    JG_HIP(hipSetDevice(s->device));
+    BcGrid g;
+    g.d = s->lv[0].d;
+    for (int a = 0; a < 3; ++a) {
+        g.h[a] = h[a];
+        g.o[a] = o[a];
+    }
+    BcCollider c{};
+    if (collider) {
+        for (int a = 0; a < 3; ++a) {
+            c.lin[a] = collider->linear_velocity[a];
+            c.ang[a] = collider->angular_velocity[a];
+            c.tr[a] = collider->translation[a];
+        }
+        c.friction = collider->friction;
+    }
+    const Dims& d = g.d;
+    const idx_t nf[3] = {(idx_t)(d.nx + 1) * d.ny * d.nz, (idx_t)d.nx * (d.ny + 1) * d.nz,
+                         (idx_t)d.nx * d.ny * (d.nz + 1)};
+    const idx_t total = nf[0] + nf[1] + nf[2];
+    if (!s->bc_tmp) {
+        JG_TRY(dalloc(s, &s->bc_tmp, total));
+        JG_TRY(dalloc(s, &s->bc_valid, 2 * total));
+    }
+    double* comp[3] = {u, v, w};
+    double* tmp[3] = {s->bc_tmp, s->bc_tmp + nf[0], s->bc_tmp + nf[0] + nf[1]};
+    int8_t* va[3] = {s->bc_valid, s->bc_valid + nf[0], s->bc_valid + nf[0] + nf[1]};
+    int8_t* vb[3] = {va[0] + total, va[1] + total, va[2] + total};
+    hipStream_t st = s->stream;
+    auto nb = [&](int a) { return nblocks_1d(nf[a]); };
+    // markers and collider velocity on covered faces
+    k_bc_init<0><<<nb(0), 256, 0, st>>>(g, collider_sdf, c, u, va[0]);
+    k_bc_init<1><<<nb(1), 256, 0, st>>>(g, collider_sdf, c, v, va[1]);
+    k_bc_init<2><<<nb(2), 256, 0, st>>>(g, collider_sdf, c, w, va[2]);
+    // free slip: extrapolation into the collider, validity ping-pong
+    for (unsigned it = 0; it < extrapolation_depth; ++it) {
+        int8_t* const* in = (it & 1) ? vb : va;
+        int8_t* const* out = (it & 1) ? va : vb;
+        k_bc_extrapolate<0><<<nb(0), 256, 0, st>>>(d, u, in[0], out[0]);
+        k_bc_extrapolate<1><<<nb(1), 256, 0, st>>>(d, v, in[1], out[1]);
+        k_bc_extrapolate<2><<<nb(2), 256, 0, st>>>(d, w, in[2], out[2]);
+    }
+    // no flux inside the collider (reads all three components), then back
+    k_bc_project<0><<<nb(0), 256, 0, st>>>(g, collider_sdf, c, u, v, w, tmp[0]);
+    k_bc_project<1><<<nb(1), 256, 0, st>>>(g, collider_sdf, c, u, v, w, tmp[1]);
+    k_bc_project<2><<<nb(2), 256, 0, st>>>(g, collider_sdf, c, u, v, w, tmp[2]);
+    k_bc_finish<0><<<nb(0), 256, 0, st>>>(d, tmp[0], comp[0], closed_walls);
+    k_bc_finish<1><<<nb(1), 256, 0, st>>>(d, tmp[1], comp[1], closed_walls);
+    k_bc_finish<2><<<nb(2), 256, 0, st>>>(d, tmp[2], comp[2], closed_walls);
+    if (blocked) {
+        k_bc_blocked<0><<<nb(0), 256, 0, st>>>(g, collider_sdf, c, u);
+        k_bc_blocked<1><<<nb(1), 256, 0, st>>>(g, collider_sdf, c, v);
+        k_bc_blocked<2><<<nb(2), 256, 0, st>>>(g, collider_sdf, c, w);
+    }
+    JG_TRY(check_launch(s, "constrain_velocity"));
     return sync(s);
 }
 

@@ -55,6 +55,21 @@ class SceneDesc(C.Structure):
                 ("seed", C.c_ulonglong), ("fluid_fraction", C.c_double)]
 
 
+class RigidCollider(C.Structure):
+    """jg_rigid_collider: RigidBodyCollider3 velocities, surface translation, friction."""
+    _fields_ = [("linear_velocity", C.c_double * 3), ("angular_velocity", C.c_double * 3),
+                ("translation", C.c_double * 3), ("friction", C.c_double)]
+
+    @classmethod
+    def make(cls, linear=(0, 0, 0), angular=(0, 0, 0), translation=(0, 0, 0), friction=0.0):
+        return cls((C.c_double * 3)(*linear), (C.c_double * 3)(*angular),
+                   (C.c_double * 3)(*translation), friction)
+
+
+# closed domain walls (Jet's kDirection* bits)
+JG_DIR_LEFT, JG_DIR_RIGHT, JG_DIR_DOWN, JG_DIR_UP, JG_DIR_BACK, JG_DIR_FRONT = 1, 2, 4, 8, 16, 32
+JG_DIR_ALL = 63
+
 _vp = C.c_void_p
 _sz = C.c_size_t
 _d = C.c_double
@@ -98,6 +113,8 @@ SIGNATURES = {
     "jg_fr_build": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d]),
     "jg_fr_apply_gradient": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d]),
     "jg_fr_array": (_vp, [_vp, _sz, C.c_int]),
+    "jg_constrain_velocity": (C.c_int, [_vp, C.c_int, C.POINTER(_d), C.POINTER(_d), _vp, _vp, _vp,
+                                        _vp, C.POINTER(RigidCollider), _u, C.c_int]),
     "jg_batch_create": (C.c_int, [C.c_int, _sz, C.POINTER(_sz), C.POINTER(_sz), C.POINTER(_sz),
                                   C.POINTER(_sz), C.POINTER(_vp)]),
     "jg_batch_destroy": (None, [_vp]),
@@ -320,6 +337,17 @@ class Solver:
     def fr_apply_gradient(self, u, v, w, uo, vo, wo, h):
         self.check(lib().jg_fr_apply_gradient(self.h, _ptr(u), _ptr(v), _ptr(w), _ptr(uo),
                                               _ptr(vo), _ptr(wo), h, h, h), "fr_apply_gradient")
+
+    def constrain_velocity(self, u, v, w, sdf, h, origin, collider=None, depth=5,
+                           closed=JG_DIR_ALL, blocked=False):
+        """Velocity boundary conditions in place on device face arrays u, v, w
+        (the finest level's size), collider SDF at the cell centres."""
+        hh = (_d * 3)(*h)
+        oo = (_d * 3)(*origin)
+        self.check(lib().jg_constrain_velocity(self.h, 1 if blocked else 0, hh, oo, _ptr(u), _ptr(v),
+                                               _ptr(w), _ptr(sdf),
+                                               C.byref(collider) if collider is not None else None,
+                                               depth, closed), "constrain_velocity")
 
     def fr_array(self, level, which):
         """Device pointer of a level's face weights (which 0-2) or fluid SDF (3)."""

@@ -276,6 +276,45 @@ int jg_fr_apply_gradient(jg_solver* s, const double* u, const double* v,
  * = u/v/w face weights, 3 = fluid SDF (NULL before the first build). */
 const float* jg_fr_array(jg_solver* s, size_t level, int which);
 
+/* ------------------------------- velocity boundary conditions on device */
+/* A rigid-body collider: RigidBodyCollider3's linear and angular velocity
+ * (include/jet/rigid_body_collider3.h), its surface's transform translation
+ * (the centre of the rotation, src/jet/rigid_body_collider3.cpp:25-28) and
+ * Collider3's friction coefficient. */
+typedef struct jg_rigid_collider {
+    double linear_velocity[3];
+    double angular_velocity[3];
+    double translation[3];
+    double friction;
+} jg_rigid_collider;
+
+/* Closed domain walls: Jet's kDirection* bits (include/jet/constants.h:312-335). */
+enum {
+    JG_DIR_LEFT = 1, JG_DIR_RIGHT = 2, JG_DIR_DOWN = 4, JG_DIR_UP = 8,
+    JG_DIR_BACK = 16, JG_DIR_FRONT = 32, JG_DIR_ALL = 63
+};
+
+/* GridFractionalBoundaryConditionSolver3::constrainVelocity
+ * (src/jet/grid_fractional_boundary_condition_solver3.cpp:26-232): faces fully
+ * inside the collider take its velocity, fluid velocity is extrapolated
+ * extrapolation_depth cells into the collider, velocities inside it lose
+ * their normal component relative to the collider (friction scales the
+ * tangential one), and closed domain walls get zero normal velocity.
+ * blocked != 0: GridBlockedBoundaryConditionSolver3::constrainVelocity
+ * (src/jet/grid_blocked_boundary_condition_solver3.cpp:23-69), which then also
+ * sets every face between a collider cell and a fluid cell to the collider
+ * velocity.  The grid is the solver's finest level (nx*ny*nz cells) with
+ * spacing h and origin o; u, v, w are device face arrays updated in place;
+ * collider_sdf is the collider's signed distance at the cell centres (Jet's
+ * colliderSdf grid; DBL_MAX everywhere when there is no collider); collider
+ * NULL means a resting collider (zero velocity and friction).  Jet's caller
+ * passes extrapolation_depth = ceil(max CFL) (src/jet/grid_fluid_solver3.cpp:
+ * 346-353).  Scratch: 8 B + 2 B per face, allocated on first use. */
+int jg_constrain_velocity(jg_solver* s, int blocked, const double h[3], const double o[3],
+                          double* u, double* v, double* w, const double* collider_sdf,
+                          const jg_rigid_collider* collider, unsigned extrapolation_depth,
+                          int closed_walls);
+
 /* --------------------------------------------------- synthetic scenes */
 /* Deterministic synthetic inputs (counter-based SplitMix64, jg_synth.h):
  *   JG_SCENE_CLOSED_SPHERE : solid sphere r = n/4 at the centre, all fluid,

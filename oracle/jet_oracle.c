@@ -1178,3 +1178,312 @@ void or_fr_apply_gradient(const float* fsdf, const float* uw, const float* vw,
                 }
             }
 }
+
+/* ============================================ Boundary conditions ======= */
+
+/* RigidBodyCollider3::velocityAt -- src/jet/rigid_body_collider3.cpp:25-28:
+ * linearVelocity + angularVelocity x (p - translation), with Vector3::cross
+ * (include/jet/detail/vector3-inl.h:125-127).  NULL: zero velocity. */
+void or_collider_velocity(const or_rigid_collider* c, const double p[3], double out[3]) {
+    if (!c) {
+        out[0] = out[1] = out[2] = 0.0;
+        return;
+    }
+    const double* a = c->angular_velocity;
+    double r[3] = {p[0] - c->translation[0], p[1] - c->translation[1], p[2] - c->translation[2]};
+    out[0] = c->linear_velocity[0] + (a[1] * r[2] - r[1] * a[2]);
+    out[1] = c->linear_velocity[1] + (a[2] * r[0] - r[2] * a[0]);
+    out[2] = c->linear_velocity[2] + (a[0] * r[1] - r[0] * a[1]);
+}
+
+/* projectAndApplyFriction -- src/jet/physics_helpers.h:34-45, with
+ * Vector3::projected / dot / length (include/jet/detail/vector3-inl.h:
+ * 120-122, 276-282, 302-305). */
+void or_project_and_apply_friction(const double vel[3], const double n[3], double friction,
+                                   double out[3]) {
+    double d = vel[0] * n[0] + vel[1] * n[1] + vel[2] * n[2];
+    out[0] = vel[0] - n[0] * d;
+    out[1] = vel[1] - n[1] * d;
+    out[2] = vel[2] - n[2] * d;
+    double l2 = out[0] * out[0] + out[1] * out[1] + out[2] * out[2];
+    if (l2 > 0) {
+        double nd = -(vel[0] * n[0] + vel[1] * n[1] + vel[2] * n[2]);
+        double veln = nd < 0.0 ? 0.0 : nd; /* std::max(nd, 0.0) */
+        double sc = 1.0 - friction * veln / sqrt(out[0] * out[0] + out[1] * out[1] + out[2] * out[2]);
+        if (sc < 0.0) sc = 0.0; /* std::max(sc, 0.0) */
+        out[0] *= sc;
+        out[1] *= sc;
+        out[2] *= sc;
+    }
+}
+
+/* CellCenteredScalarGrid3 sample: LinearArraySampler3::operator() over the
+ * data at origin o + h/2 (src/jet/cell_centered_scalar_grid3.cpp:60-62,
+ * src/jet/scalar_grid3.cpp:83,214-218). */
+static void cell_origin(const double h[3], const double o[3], double d[3]) {
+    d[0] = o[0] + 0.5 * h[0];
+    d[1] = o[1] + 0.5 * h[1];
+    d[2] = o[2] + 0.5 * h[2];
+}
+
+double or_sample_cell(const double* data, size_t nx, size_t ny, size_t nz, const double h[3],
+                      const double o[3], const double p[3]) {
+    double d[3];
+    cell_origin(h, o, d);
+    return linear_sample(data, (long)nx, (long)ny, (long)nz, h, d, p);
+}
+
+/* ScalarGrid3::gradient -- src/jet/scalar_grid3.cpp:89-102: the weights of
+ * LinearArraySampler3::getCoordinatesAndWeights (include/jet/detail/
+ * array_samplers3-inl.h:152-194, which scales by 1/h instead of dividing)
+ * applied to gradient3 at the 8 data points (src/jet/fdm_utils.cpp:49-67). */
+void or_gradient_cell(const double* data, size_t nx, size_t ny, size_t nz, const double h[3],
+                      const double o[3], const double p[3], double g[3]) {
+    double d[3], inv[3] = {1.0 / h[0], 1.0 / h[1], 1.0 / h[2]};
+    cell_origin(h, o, d);
+    long sz3[3] = {(long)nx, (long)ny, (long)nz};
+    long lo[3], hi[3];
+    double f[3];
+    for (int a = 0; a < 3; ++a) {
+        barycentric((p[a] - d[a]) * inv[a], sz3[a] - 1, &lo[a], &f[a]);
+        hi[a] = lo[a] + 1 < sz3[a] - 1 ? lo[a] + 1 : sz3[a] - 1;
+    }
+    const double fx = f[0], fy = f[1], fz = f[2];
+    const double wt[8] = {(1 - fx) * (1 - fy) * (1 - fz), fx * (1 - fy) * (1 - fz),
+                          (1 - fx) * fy * (1 - fz),       fx * fy * (1 - fz),
+                          (1 - fx) * (1 - fy) * fz,       fx * (1 - fy) * fz,
+                          (1 - fx) * fy * fz,             fx * fy * fz};
+    g[0] = g[1] = g[2] = 0.0;
+    for (int q = 0; q < 8; ++q) {
+        const long i = (q & 1) ? hi[0] : lo[0];
+        const long j = (q & 2) ? hi[1] : lo[1];
+        const long k = (q & 4) ? hi[2] : lo[2];
+#define D(a, b, c) data[(a) + (long)nx * ((b) + (long)ny * (c))]
+        const double left = D(i > 0 ? i - 1 : i, j, k);
+        const double right = D(i + 1 < (long)nx ? i + 1 : i, j, k);
+        const double down = D(i, j > 0 ? j - 1 : j, k);
+        const double up = D(i, j + 1 < (long)ny ? j + 1 : j, k);
+        const double back = D(i, j, k > 0 ? k - 1 : k);
+        const double front = D(i, j, k + 1 < (long)nz ? k + 1 : k);
+#undef D
+        g[0] += wt[q] * (0.5 * (right - left) / h[0]);
+        g[1] += wt[q] * (0.5 * (up - down) / h[1]);
+        g[2] += wt[q] * (0.5 * (front - back) / h[2]);
+    }
+}
+
+/* extrapolateToRegion, 3-D -- include/jet/detail/array_utils-inl.h:172-241.
+ * Each pass writes only cells that were invalid and reads only cells that
+ * were valid at its start, so the passes are order-independent. */
+void or_extrapolate_to_region(const double* input, const char* valid, size_t sx, size_t sy,
+                              size_t sz, unsigned iterations, double* output) {
+    const size_t n = sx * sy * sz;
+    char* v0 = (char*)malloc(n);
+    char* v1 = (char*)calloc(n, 1);
+    for (size_t c = 0; c < n; ++c) {
+        v0[c] = valid[c];
+        output[c] = input[c];
+    }
+    for (unsigned it = 0; it < iterations; ++it) {
+        for (size_t k = 0; k < sz; ++k)
+            for (size_t j = 0; j < sy; ++j)
+                for (size_t i = 0; i < sx; ++i) {
+                    const size_t c = IX(i, j, k, sx, sy);
+                    if (v0[c]) {
+                        v1[c] = 1;
+                        continue;
+                    }
+                    double sum = 0.0;
+                    unsigned count = 0;
+                    if (i + 1 < sx && v0[c + 1]) { sum += output[c + 1]; ++count; }
+                    if (i > 0 && v0[c - 1]) { sum += output[c - 1]; ++count; }
+                    if (j + 1 < sy && v0[c + sx]) { sum += output[c + sx]; ++count; }
+                    if (j > 0 && v0[c - sx]) { sum += output[c - sx]; ++count; }
+                    if (k + 1 < sz && v0[c + sx * sy]) { sum += output[c + sx * sy]; ++count; }
+                    if (k > 0 && v0[c - sx * sy]) { sum += output[c - sx * sy]; ++count; }
+                    if (count > 0) {
+                        output[c] = sum / (double)count;
+                        v1[c] = 1;
+                    }
+                }
+        char* t = v0;
+        v0 = v1;
+        v1 = t;
+    }
+    free(v0);
+    free(v1);
+}
+
+/* Face grid a = 0 / 1 / 2 (u / v / w): sizes and data origin
+ * o + h/2 in the two other axes (src/jet/face_centered_grid3.cpp:216-238,
+ * 437-447). */
+static void face_dims(int a, size_t nx, size_t ny, size_t nz, size_t s[3]) {
+    s[0] = nx + (a == 0);
+    s[1] = ny + (a == 1);
+    s[2] = nz + (a == 2);
+}
+static void face_origin(int a, const double h[3], const double o[3], double d[3]) {
+    for (int b = 0; b < 3; ++b) d[b] = o[b] + 0.5 * (b == a ? 0.0 : h[b]);
+}
+static void face_pos(const double d[3], const double h[3], size_t i, size_t j, size_t k,
+                     double p[3]) {
+    p[0] = d[0] + h[0] * (double)i;
+    p[1] = d[1] + h[1] * (double)j;
+    p[2] = d[2] + h[2] * (double)k;
+}
+
+/* clamp -- include/jet/detail/math_utils-inl.h */
+static double clamp01(double x) {
+    if (x < 0.0) return 0.0;
+    if (x > 1.0) return 1.0;
+    return x;
+}
+
+/* GridFractionalBoundaryConditionSolver3::constrainVelocity --
+ * src/jet/grid_fractional_boundary_condition_solver3.cpp:26-232, then for
+ * blocked GridBlockedBoundaryConditionSolver3::constrainVelocity's marker
+ * pass -- src/jet/grid_blocked_boundary_condition_solver3.cpp:23-69 with the
+ * marker of :86-94 (collider where the cell-centre SDF is negative). */
+void or_constrain_velocity(int blocked, size_t nx, size_t ny, size_t nz, const double h[3],
+                           const double o[3], double* u, double* v, double* w,
+                           const double* sdf, const or_rigid_collider* collider,
+                           unsigned depth, int closed) {
+    double* comp[3] = {u, v, w};
+    size_t dims[3][3];
+    double org[3][3];
+    for (int a = 0; a < 3; ++a) {
+        face_dims(a, nx, ny, nz, dims[a]);
+        face_origin(a, h, o, org[a]);
+    }
+    /* collider velocity at the face; phi at the two cell centres it joins */
+    for (int a = 0; a < 3; ++a) {
+        const size_t* s = dims[a];
+        const size_t n = s[0] * s[1] * s[2];
+        char* marker = (char*)malloc(n);
+        for (size_t k = 0; k < s[2]; ++k)
+            for (size_t j = 0; j < s[1]; ++j)
+                for (size_t i = 0; i < s[0]; ++i) {
+                    double p[3], q0[3], q1[3], cv[3];
+                    face_pos(org[a], h, i, j, k, p);
+                    for (int b = 0; b < 3; ++b) {
+                        const double off = b == a ? 0.5 * h[a] : 0.0;
+                        q0[b] = p[b] - off;
+                        q1[b] = p[b] + off;
+                    }
+                    const double phi0 = or_sample_cell(sdf, nx, ny, nz, h, o, q0);
+                    const double phi1 = or_sample_cell(sdf, nx, ny, nz, h, o, q1);
+                    double frac = or_fraction_inside_sdf(phi0, phi1);
+                    frac = 1.0 - clamp01(frac);
+                    const size_t c = IX(i, j, k, s[0], s[1]);
+                    if (frac > 0.0) {
+                        marker[c] = 1;
+                    } else {
+                        or_collider_velocity(collider, p, cv);
+                        comp[a][c] = cv[a];
+                        marker[c] = 0;
+                    }
+                }
+        /* free slip: extrapolate fluid velocity into the collider */
+        double* tmp = (double*)malloc(n * sizeof(double));
+        or_extrapolate_to_region(comp[a], marker, s[0], s[1], s[2], depth, tmp);
+        memcpy(comp[a], tmp, n * sizeof(double));
+        free(tmp);
+        free(marker);
+    }
+    /* no flux: project onto the collider surface (reads all three
+     * extrapolated components; results land after all three) */
+    double* out[3];
+    for (int a = 0; a < 3; ++a) {
+        const size_t* s = dims[a];
+        out[a] = (double*)malloc(s[0] * s[1] * s[2] * sizeof(double));
+        for (size_t k = 0; k < s[2]; ++k)
+            for (size_t j = 0; j < s[1]; ++j)
+                for (size_t i = 0; i < s[0]; ++i) {
+                    const size_t c = IX(i, j, k, s[0], s[1]);
+                    double p[3];
+                    face_pos(org[a], h, i, j, k, p);
+                    if (!(or_sample_cell(sdf, nx, ny, nz, h, o, p) < 0)) {
+                        out[a][c] = comp[a][c];
+                        continue;
+                    }
+                    double cv[3], vel[3], g[3];
+                    or_collider_velocity(collider, p, cv);
+                    for (int b = 0; b < 3; ++b)
+                        vel[b] = linear_sample(comp[b], (long)dims[b][0], (long)dims[b][1],
+                                               (long)dims[b][2], h, org[b], p);
+                    or_gradient_cell(sdf, nx, ny, nz, h, o, p, g);
+                    if (g[0] * g[0] + g[1] * g[1] + g[2] * g[2] > 0.0) {
+                        const double l = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+                        const double nrm[3] = {g[0] / l, g[1] / l, g[2] / l};
+                        const double velr[3] = {vel[0] - cv[0], vel[1] - cv[1], vel[2] - cv[2]};
+                        double velt[3];
+                        or_project_and_apply_friction(velr, nrm, collider ? collider->friction : 0.0,
+                                                      velt);
+                        out[a][c] = velt[a] + cv[a];
+                    } else {
+                        out[a][c] = cv[a];
+                    }
+                }
+    }
+    for (int a = 0; a < 3; ++a) {
+        memcpy(comp[a], out[a], dims[a][0] * dims[a][1] * dims[a][2] * sizeof(double));
+        free(out[a]);
+    }
+    /* no flux through closed domain walls */
+    for (size_t k = 0; k < nz; ++k)
+        for (size_t j = 0; j < ny; ++j) {
+            if (closed & OR_DIR_LEFT) u[IX(0, j, k, nx + 1, ny)] = 0;
+            if (closed & OR_DIR_RIGHT) u[IX(nx, j, k, nx + 1, ny)] = 0;
+        }
+    for (size_t k = 0; k < nz; ++k)
+        for (size_t i = 0; i < nx; ++i) {
+            if (closed & OR_DIR_DOWN) v[IX(i, 0, k, nx, ny + 1)] = 0;
+            if (closed & OR_DIR_UP) v[IX(i, ny, k, nx, ny + 1)] = 0;
+        }
+    for (size_t j = 0; j < ny; ++j)
+        for (size_t i = 0; i < nx; ++i) {
+            if (closed & OR_DIR_BACK) w[IX(i, j, 0, nx, ny)] = 0;
+            if (closed & OR_DIR_FRONT) w[IX(i, j, nz, nx, ny)] = 0;
+        }
+    if (!blocked) return;
+    /* blocked: faces between a collider cell and a fluid cell take the
+     * collider velocity */
+    for (size_t k = 0; k < nz; ++k)
+        for (size_t j = 0; j < ny; ++j)
+            for (size_t i = 0; i < nx; ++i) {
+#define COLL(a, b, c) (sdf[IX(a, b, c, nx, ny)] < 0)
+                if (!COLL(i, j, k)) continue;
+                double p[3], cv[3];
+                if (i > 0 && !COLL(i - 1, j, k)) {
+                    face_pos(org[0], h, i, j, k, p);
+                    or_collider_velocity(collider, p, cv);
+                    u[IX(i, j, k, nx + 1, ny)] = cv[0];
+                }
+                if (i + 1 < nx && !COLL(i + 1, j, k)) {
+                    face_pos(org[0], h, i + 1, j, k, p);
+                    or_collider_velocity(collider, p, cv);
+                    u[IX(i + 1, j, k, nx + 1, ny)] = cv[0];
+                }
+                if (j > 0 && !COLL(i, j - 1, k)) {
+                    face_pos(org[1], h, i, j, k, p);
+                    or_collider_velocity(collider, p, cv);
+                    v[IX(i, j, k, nx, ny + 1)] = cv[1];
+                }
+                if (j + 1 < ny && !COLL(i, j + 1, k)) {
+                    face_pos(org[1], h, i, j + 1, k, p);
+                    or_collider_velocity(collider, p, cv);
+                    v[IX(i, j + 1, k, nx, ny + 1)] = cv[1];
+                }
+                if (k > 0 && !COLL(i, j, k - 1)) {
+                    face_pos(org[2], h, i, j, k, p);
+                    or_collider_velocity(collider, p, cv);
+                    w[IX(i, j, k, nx, ny)] = cv[2];
+                }
+                if (k + 1 < nz && !COLL(i, j, k + 1)) {
+                    face_pos(org[2], h, i, j, k + 1, p);
+                    or_collider_velocity(collider, p, cv);
+                    w[IX(i, j, k + 1, nx, ny)] = cv[2];
+                }
+#undef COLL
+            }
+}

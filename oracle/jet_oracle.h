@@ -162,6 +162,43 @@ void or_fr_apply_gradient(const float* fluid_sdf, const float* uw,
                           double* uo, double* vo, double* wo, size_t nx,
                           size_t ny, size_t nz, const double h[3]);
 
+/* ---------------------------------------- boundary conditions (velocity) */
+/* Rigid-body collider: RigidBodyCollider3 (include/jet/rigid_body_collider3.h)
+ * with its surface's transform translation and Collider3's friction. */
+typedef struct {
+    double linear_velocity[3];
+    double angular_velocity[3];
+    double translation[3];
+    double friction;
+} or_rigid_collider;
+
+/* Jet's kDirection* bits (include/jet/constants.h:312-335). */
+enum { OR_DIR_LEFT = 1, OR_DIR_RIGHT = 2, OR_DIR_DOWN = 4, OR_DIR_UP = 8, OR_DIR_BACK = 16,
+       OR_DIR_FRONT = 32 };
+
+/* GridFractionalBoundaryConditionSolver3::constrainVelocity and, with
+ * blocked != 0, GridBlockedBoundaryConditionSolver3::constrainVelocity, on a
+ * face-centred grid of nx*ny*nz cells (u: (nx+1)*ny*nz, v: nx*(ny+1)*nz,
+ * w: nx*ny*(nz+1), updated in place) with spacing h and origin o.  sdf: the
+ * collider SDF at the cell centres (DBL_MAX everywhere without a collider);
+ * collider NULL: zero collider velocity. */
+void or_constrain_velocity(int blocked, size_t nx, size_t ny, size_t nz, const double h[3],
+                           const double o[3], double* u, double* v, double* w,
+                           const double* sdf, const or_rigid_collider* collider,
+                           unsigned depth, int closed);
+/* Building blocks, exposed for their own checks: LinearArraySampler3 of a
+ * cell-centred grid, ScalarGrid3::gradient, extrapolateToRegion (3-D),
+ * projectAndApplyFriction, RigidBodyCollider3::velocityAt. */
+double or_sample_cell(const double* data, size_t nx, size_t ny, size_t nz, const double h[3],
+                      const double o[3], const double p[3]);
+void or_gradient_cell(const double* data, size_t nx, size_t ny, size_t nz, const double h[3],
+                      const double o[3], const double p[3], double g[3]);
+void or_extrapolate_to_region(const double* input, const char* valid, size_t sx, size_t sy,
+                              size_t sz, unsigned iterations, double* output);
+void or_project_and_apply_friction(const double vel[3], const double normal[3], double friction,
+                                   double out[3]);
+void or_collider_velocity(const or_rigid_collider* c, const double p[3], double out[3]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -124,7 +124,79 @@ def _setup(lib):
     return lib
 
 
-class Oracle:
+class RigidCollider(C.Structure):
+    """RigidBodyCollider3 state: velocities, surface translation, friction."""
+    _fields_ = [("linear_velocity", C.c_double * 3), ("angular_velocity", C.c_double * 3),
+                ("translation", C.c_double * 3), ("friction", C.c_double)]
+
+    @classmethod
+    def make(cls, linear=(0, 0, 0), angular=(0, 0, 0), translation=(0, 0, 0), friction=0.0):
+        return cls((C.c_double * 3)(*linear), (C.c_double * 3)(*angular),
+                   (C.c_double * 3)(*translation), friction)
+
+
+# Jet's kDirection* bits (include/jet/constants.h:312-335)
+DIR_LEFT, DIR_RIGHT, DIR_DOWN, DIR_UP, DIR_BACK, DIR_FRONT = 1, 2, 4, 8, 16, 32
+DIR_ALL = 63
+
+
+class _BoundaryCalls:
+    """Velocity boundary conditions; the same calls on the restatement (or_*)
+    and on Jet's compiled parts (ref_*)."""
+
+    _p = "or_"
+
+    def _fn(self, name):
+        return getattr(self.lib, self._p + name)
+
+    def sample_cell(self, data, h, origin, p):
+        nx, ny, nz = _shape3(data)
+        f = self._fn("sample_cell")
+        f.restype = C.c_double
+        return f(_d(data), _sz(nx), _sz(ny), _sz(nz), _d(np.asarray(h, np.float64)),
+                 _d(np.asarray(origin, np.float64)), _d(np.asarray(p, np.float64)))
+
+    def gradient_cell(self, data, h, origin, p):
+        nx, ny, nz = _shape3(data)
+        out = np.empty(3)
+        self._fn("gradient_cell")(_d(data), _sz(nx), _sz(ny), _sz(nz),
+                                  _d(np.asarray(h, np.float64)), _d(np.asarray(origin, np.float64)),
+                                  _d(np.asarray(p, np.float64)), _d(out))
+        return out
+
+    def extrapolate_to_region(self, data, valid, iterations):
+        nx, ny, nz = _shape3(data)
+        out = np.empty_like(data)
+        self._fn("extrapolate_to_region")(_d(data), _c(valid), _sz(nx), _sz(ny), _sz(nz),
+                                          C.c_uint(iterations), _d(out))
+        return out
+
+    def project_and_apply_friction(self, vel, normal, friction):
+        out = np.empty(3)
+        self._fn("project_and_apply_friction")(_d(np.asarray(vel, np.float64)),
+                                               _d(np.asarray(normal, np.float64)),
+                                               C.c_double(friction), _d(out))
+        return out
+
+    def collider_velocity(self, collider, p):
+        out = np.empty(3)
+        self._fn("collider_velocity")(C.byref(collider), _d(np.asarray(p, np.float64)), _d(out))
+        return out
+
+    def constrain_velocity(self, u, v, w, sdf, h, origin, collider=None, depth=5,
+                           closed=DIR_ALL, blocked=False):
+        """Returns the constrained (u, v, w); the inputs are not modified."""
+        nx, ny, nz = _shape3(sdf)
+        uo, vo, wo = (np.ascontiguousarray(a, np.float64).copy() for a in (u, v, w))
+        self._fn("constrain_velocity")(C.c_int(1 if blocked else 0), _sz(nx), _sz(ny), _sz(nz),
+                                       _d(np.asarray(h, np.float64)),
+                                       _d(np.asarray(origin, np.float64)), _d(uo), _d(vo), _d(wo),
+                                       _d(sdf), C.byref(collider) if collider is not None else None,
+                                       C.c_uint(depth), C.c_int(closed))
+        return uo, vo, wo
+
+
+class Oracle(_BoundaryCalls):
     """The C restatement (liboracle.so)."""
 
     def __init__(self, path=LIB_PATH):
@@ -361,8 +433,12 @@ class Oracle:
         return (fs, UW, VW, WW), sys
 
 
-class Ref:
-    """Jet's own compiled linear solvers (oracle/_ref/libjetref.so)."""
+class Ref(_BoundaryCalls):
+    """Jet's own compiled linear solvers (oracle/_ref/libjetref.so), and the
+    compiled grids, samplers, extrapolation, friction helper and rigid
+    collider that Jet's velocity boundary conditions are made of."""
+
+    _p = "ref_"
 
     def __init__(self, path=REF_PATH):
         if not os.path.exists(path):

@@ -7,7 +7,13 @@
 // benchmark host, as the "reference" CPU baseline.  Nothing here is product
 // code and nothing is copied from the reference: this file only calls Jet's
 // public API (include/jet/*.h).
+#include <jet/array_utils.h>
+#include <jet/cell_centered_scalar_grid3.h>
+#include <jet/face_centered_grid3.h>
 #include <jet/fdm_cg_solver3.h>
+#include <jet/rigid_body_collider3.h>
+#include <jet/sphere3.h>
+#include <physics_helpers.h>
 #include <jet/fdm_gauss_seidel_solver3.h>
 #include <jet/fdm_jacobi_solver3.h>
 #include <jet/fdm_linear_system3.h>
@@ -238,6 +244,206 @@ double ref_fraction_inside_sdf(double a, double b) {
 
 double ref_fraction_inside(double bl, double br, double tl, double tr) {
     return fractionInside(bl, br, tl, tr);
+}
+
+// ------------------------------------------------ boundary conditions
+// Jet's grids, sampler, gradient, extrapolation, friction helper and rigid
+// collider.  GridFractionalBoundaryConditionSolver3 itself cannot be linked
+// (its onColliderUpdated constructs SurfaceToImplicit3, which needs
+// TriangleMesh3 and the absent tinyobj sources), so ref_constrain_velocity
+// drives these compiled parts in the order its constrainVelocity
+// (src/jet/grid_fractional_boundary_condition_solver3.cpp:26-232) and the
+// blocked variant (src/jet/grid_blocked_boundary_condition_solver3.cpp:23-69)
+// call them, with the collider SDF given at the cell centres.
+
+struct RefCollider {
+    double lin[3], ang[3], trans[3], friction;
+};
+
+static std::shared_ptr<RigidBodyCollider3> makeCollider(const RefCollider* c) {
+    auto surf = std::make_shared<Sphere3>(Vector3D(), 1.0);
+    surf->transform = Transform3(Vector3D(c->trans[0], c->trans[1], c->trans[2]), QuaternionD());
+    auto col = std::make_shared<RigidBodyCollider3>(surf, Vector3D(c->lin[0], c->lin[1], c->lin[2]),
+                                                    Vector3D(c->ang[0], c->ang[1], c->ang[2]));
+    col->setFrictionCoefficient(c->friction);
+    return col;
+}
+
+static void fillCells(CellCenteredScalarGrid3* g, const double* d) {
+    const Size3 s = g->dataSize();
+    g->forEachDataPointIndex([&](size_t i, size_t j, size_t k) {
+        (*g)(i, j, k) = d[i + s.x * (j + s.y * k)];
+    });
+}
+
+static void fillFaces(FaceCenteredGrid3* f, const double* u, const double* v, const double* w) {
+    const Size3 su = f->uSize(), sv = f->vSize(), sw = f->wSize();
+    f->forEachUIndex([&](size_t i, size_t j, size_t k) { f->u(i, j, k) = u[i + su.x * (j + su.y * k)]; });
+    f->forEachVIndex([&](size_t i, size_t j, size_t k) { f->v(i, j, k) = v[i + sv.x * (j + sv.y * k)]; });
+    f->forEachWIndex([&](size_t i, size_t j, size_t k) { f->w(i, j, k) = w[i + sw.x * (j + sw.y * k)]; });
+}
+
+static void readFaces(const FaceCenteredGrid3& f, double* u, double* v, double* w) {
+    const Size3 su = f.uSize(), sv = f.vSize(), sw = f.wSize();
+    f.forEachUIndex([&](size_t i, size_t j, size_t k) { u[i + su.x * (j + su.y * k)] = f.u(i, j, k); });
+    f.forEachVIndex([&](size_t i, size_t j, size_t k) { v[i + sv.x * (j + sv.y * k)] = f.v(i, j, k); });
+    f.forEachWIndex([&](size_t i, size_t j, size_t k) { w[i + sw.x * (j + sw.y * k)] = f.w(i, j, k); });
+}
+
+double ref_sample_cell(const double* data, size_t nx, size_t ny, size_t nz, const double* h,
+                       const double* o, const double* p) {
+    CellCenteredScalarGrid3 g(Size3(nx, ny, nz), Vector3D(h[0], h[1], h[2]), Vector3D(o[0], o[1], o[2]));
+    fillCells(&g, data);
+    return g.sample(Vector3D(p[0], p[1], p[2]));
+}
+
+void ref_gradient_cell(const double* data, size_t nx, size_t ny, size_t nz, const double* h,
+                       const double* o, const double* p, double* out) {
+    CellCenteredScalarGrid3 g(Size3(nx, ny, nz), Vector3D(h[0], h[1], h[2]), Vector3D(o[0], o[1], o[2]));
+    fillCells(&g, data);
+    const Vector3D r = g.gradient(Vector3D(p[0], p[1], p[2]));
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+}
+
+void ref_extrapolate_to_region(const double* input, const char* valid, size_t sx, size_t sy,
+                               size_t sz, unsigned iterations, double* output) {
+    Array3<double> in(sx, sy, sz), out(sx, sy, sz);
+    Array3<char> va(sx, sy, sz);
+    std::memcpy(in.data(), input, sizeof(double) * sx * sy * sz);
+    std::memcpy(va.data(), valid, sx * sy * sz);
+    extrapolateToRegion(in.constAccessor(), va.constAccessor(), iterations, out.accessor());
+    std::memcpy(output, out.data(), sizeof(double) * sx * sy * sz);
+}
+
+void ref_project_and_apply_friction(const double* vel, const double* n, double friction,
+                                    double* out) {
+    const Vector3D r = projectAndApplyFriction(Vector3D(vel[0], vel[1], vel[2]),
+                                               Vector3D(n[0], n[1], n[2]), friction);
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+}
+
+void ref_collider_velocity(const RefCollider* c, const double* p, double* out) {
+    const Vector3D r = makeCollider(c)->velocityAt(Vector3D(p[0], p[1], p[2]));
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+}
+
+void ref_constrain_velocity(int blocked, size_t nx, size_t ny, size_t nz, const double* hh,
+                            const double* oo, double* uo, double* vo, double* wo,
+                            const double* sdfd, const RefCollider* c, unsigned depth,
+                            int closed) {
+    const Size3 size(nx, ny, nz);
+    const Vector3D h(hh[0], hh[1], hh[2]), origin(oo[0], oo[1], oo[2]);
+    FaceCenteredGrid3 velocity(size, h, origin);
+    fillFaces(&velocity, uo, vo, wo);
+    CellCenteredScalarGrid3 sdf(size, h, origin);
+    fillCells(&sdf, sdfd);
+    RefCollider zero = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, 0};
+    auto collider = makeCollider(c ? c : &zero);
+
+    auto u = velocity.uAccessor();
+    auto v = velocity.vAccessor();
+    auto w = velocity.wAccessor();
+    auto uPos = velocity.uPosition();
+    auto vPos = velocity.vPosition();
+    auto wPos = velocity.wPosition();
+    Array3<double> uTemp(u.size()), vTemp(v.size()), wTemp(w.size());
+    Array3<char> uMarker(u.size(), 1), vMarker(v.size(), 1), wMarker(w.size(), 1);
+
+    auto init = [&](ArrayAccessor3<double> a, Array3<char>& m, const VectorGrid3::DataPositionFunc& pos,
+                    int axis) {
+        Vector3D off;
+        off[axis] = 0.5 * h[axis];
+        a.forEachIndex([&](size_t i, size_t j, size_t k) {
+            const Vector3D pt = pos(i, j, k);
+            const double phi0 = sdf.sample(pt - off);
+            const double phi1 = sdf.sample(pt + off);
+            double frac = fractionInsideSdf(phi0, phi1);
+            frac = 1.0 - clamp(frac, 0.0, 1.0);
+            if (frac > 0.0) {
+                m(i, j, k) = 1;
+            } else {
+                a(i, j, k) = collider->velocityAt(pt)[axis];
+                m(i, j, k) = 0;
+            }
+        });
+    };
+    init(u, uMarker, uPos, 0);
+    init(v, vMarker, vPos, 1);
+    init(w, wMarker, wPos, 2);
+    extrapolateToRegion(velocity.uConstAccessor(), uMarker.constAccessor(), depth, u);
+    extrapolateToRegion(velocity.vConstAccessor(), vMarker.constAccessor(), depth, v);
+    extrapolateToRegion(velocity.wConstAccessor(), wMarker.constAccessor(), depth, w);
+
+    auto project = [&](ConstArrayAccessor3<double> a, Array3<double>& t,
+                       const VectorGrid3::DataPositionFunc& pos, int axis) {
+        a.forEachIndex([&](size_t i, size_t j, size_t k) {
+            const Vector3D pt = pos(i, j, k);
+            if (isInsideSdf(sdf.sample(pt))) {
+                const Vector3D colliderVel = collider->velocityAt(pt);
+                const Vector3D vel = velocity.sample(pt);
+                const Vector3D g = sdf.gradient(pt);
+                if (g.lengthSquared() > 0.0) {
+                    const Vector3D n = g.normalized();
+                    const Vector3D velr = vel - colliderVel;
+                    const Vector3D velt =
+                        projectAndApplyFriction(velr, n, collider->frictionCoefficient());
+                    t(i, j, k) = (velt + colliderVel)[axis];
+                } else {
+                    t(i, j, k) = colliderVel[axis];
+                }
+            } else {
+                t(i, j, k) = a(i, j, k);
+            }
+        });
+    };
+    project(velocity.uConstAccessor(), uTemp, uPos, 0);
+    project(velocity.vConstAccessor(), vTemp, vPos, 1);
+    project(velocity.wConstAccessor(), wTemp, wPos, 2);
+    u.forEachIndex([&](size_t i, size_t j, size_t k) { u(i, j, k) = uTemp(i, j, k); });
+    v.forEachIndex([&](size_t i, size_t j, size_t k) { v(i, j, k) = vTemp(i, j, k); });
+    w.forEachIndex([&](size_t i, size_t j, size_t k) { w(i, j, k) = wTemp(i, j, k); });
+
+    for (size_t k = 0; k < nz; ++k)
+        for (size_t j = 0; j < ny; ++j) {
+            if (closed & kDirectionLeft) u(0, j, k) = 0;
+            if (closed & kDirectionRight) u(nx, j, k) = 0;
+        }
+    for (size_t k = 0; k < nz; ++k)
+        for (size_t i = 0; i < nx; ++i) {
+            if (closed & kDirectionDown) v(i, 0, k) = 0;
+            if (closed & kDirectionUp) v(i, ny, k) = 0;
+        }
+    for (size_t j = 0; j < ny; ++j)
+        for (size_t i = 0; i < nx; ++i) {
+            if (closed & kDirectionBack) w(i, j, 0) = 0;
+            if (closed & kDirectionFront) w(i, j, nz) = 0;
+        }
+
+    if (blocked) {
+        Array3<char> marker(size);
+        marker.forEachIndex([&](size_t i, size_t j, size_t k) {
+            marker(i, j, k) = isInsideSdf(sdf(i, j, k)) ? 0 : 1;
+        });
+        marker.forEachIndex([&](size_t i, size_t j, size_t k) {
+            if (marker(i, j, k) != 0) return;
+            if (i > 0 && marker(i - 1, j, k) == 1) u(i, j, k) = collider->velocityAt(uPos(i, j, k)).x;
+            if (i < nx - 1 && marker(i + 1, j, k) == 1)
+                u(i + 1, j, k) = collider->velocityAt(uPos(i + 1, j, k)).x;
+            if (j > 0 && marker(i, j - 1, k) == 1) v(i, j, k) = collider->velocityAt(vPos(i, j, k)).y;
+            if (j < ny - 1 && marker(i, j + 1, k) == 1)
+                v(i, j + 1, k) = collider->velocityAt(vPos(i, j + 1, k)).y;
+            if (k > 0 && marker(i, j, k - 1) == 1) w(i, j, k) = collider->velocityAt(wPos(i, j, k)).z;
+            if (k < nz - 1 && marker(i, j, k + 1) == 1)
+                w(i, j, k + 1) = collider->velocityAt(wPos(i, j, k + 1)).z;
+        });
+    }
+    readFaces(velocity, uo, vo, wo);
 }
 
 }  // extern "C"

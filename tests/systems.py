@@ -84,3 +84,23 @@ def mg_poisson_hierarchy(coarsest, levels):
         sys.b.append(b)
         sys.x.append(np.zeros_like(b))
     return sys
+
+
+def collider_scene(n, seed, spacing=None, origin=(0.1, -0.2, 0.05), radius=0.22,
+                   centre=(0.55, 0.25, 0.45), moving=True, friction=0.3):
+    """A grid of n = (nx, ny, nz) cells with a sphere collider: cell-centre
+    SDF, random face velocities, and the collider's rigid motion as
+    (linear, angular, translation, friction)."""
+    nx, ny, nz = n
+    rng = np.random.default_rng(seed)
+    h = np.array(spacing if spacing is not None else (1.0 / nx,) * 3, np.float64)
+    o = np.array(origin, np.float64)
+    z, y, x = np.meshgrid(o[2] + (np.arange(nz) + 0.5) * h[2], o[1] + (np.arange(ny) + 0.5) * h[1],
+                          o[0] + (np.arange(nx) + 0.5) * h[0], indexing="ij")
+    c = np.array(centre, np.float64)
+    sdf = np.ascontiguousarray(np.sqrt((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2) - radius)
+    u = rng.uniform(-1, 1, (nz, ny, nx + 1))
+    v = rng.uniform(-1, 1, (nz, ny + 1, nx))
+    w = rng.uniform(-1, 1, (nz + 1, ny, nx))
+    motion = (((0.3, -0.1, 0.2), (0.5, 1.0, -0.7)) if moving else ((0, 0, 0), (0, 0, 0)))
+    return h, o, sdf, (u, v, w), (motion[0], motion[1], tuple(c), friction)
