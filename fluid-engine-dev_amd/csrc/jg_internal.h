@@ -151,6 +151,13 @@ struct jg_solver {
     // Velocity boundary conditions: projected faces, two validity masks.
     double* bc_tmp = nullptr;
     int8_t* bc_valid = nullptr;
+    // Compressed (CSR) system of jg_csr_cg_solve_host and its CG vectors.
+    struct Csr {
+        size_t rows = 0, nnz = 0, cap_rows = 0, cap_nnz = 0;
+        double* val = nullptr;
+        int *rp = nullptr, *ci = nullptr;
+        double *b = nullptr, *x = nullptr, *r = nullptr, *dA = nullptr, *dB = nullptr, *q = nullptr;
+    } csr;
     // Reduction slots and device scalars.
     double* part = nullptr;  // kSlots * kMaxPartials
     double* scal = nullptr;  // small result area
@@ -179,6 +186,7 @@ struct jg_solver {
         rhs32 = r32 = z32 = d32A = d32B = q32 = e32 = nullptr;
         bc_tmp = nullptr;
         bc_valid = nullptr;
+        csr = Csr{};
         part = scal = nullptr;
         ctl = nullptr;
         f32_valid = false;

@@ -20,6 +20,7 @@
 #include "jg_brick.h"
 #include "jg_bottom.h"
 #include "jg_transfer.h"
+#include "jg_csr.h"
 
 #include <cstdlib>
 
@@ -539,6 +540,7 @@ struct Pcg {
     Dims d;
     Ctl* ctl;
     const MgCfg<T>* mg;  // nullptr: no preconditioner (z aliases r)
+    const CsrDev* csr = nullptr;  // compressed system (FP64, no preconditioner): A unused, d = {rows, 1, 1}
 };
 
 // z = M r and the r.z / r.r partials in slots 0 / 1 (from the V-cycle's last
@@ -591,6 +593,20 @@ int pcg_iteration(jg_solver* s, const Pcg<T>& v, bool odd) {
     Geo g = geo_stencil(v.d);
     T* dold = odd ? v.dB : v.dA;
     T* dnew = odd ? v.dA : v.dB;
+    if constexpr (sizeof(T) == 8) {
+        if (v.csr) {
+            k_csr_dir_spmv<<<nb1, NT, 0, s->stream>>>(*v.csr, v.z, dold, dnew, v.q, n, s->slot(2),
+                                                      v.ctl);
+            k_fin_alpha<<<1, kFinThreads, 0, s->stream>>>(v.ctl, s->slot(2), nb1);
+            k_update_xr<T><<<nb1, NT, 0, s->stream>>>(v.x, v.r, dnew, v.q, n, v.ctl);
+            k_csr_residual<false, true><<<nb1, NT, 0, s->stream>>>(*v.csr, v.x, v.b, v.r, n,
+                                                                   nullptr, v.ctl);
+            const int nb = pcg_precondition<T>(s, v, v.ctl);
+            if (nb < 0) return -nb;
+            k_fin_sigma<<<1, kFinThreads, 0, s->stream>>>(v.ctl, s->slot(0), s->slot(0), nb);
+            return check_launch(s, "csr pcg iteration");
+        }
+    }
     k_dir_spmv<T, T><<<g.grid, g.block, 0, s->stream>>>(v.A, v.z, dold, dnew, v.q, v.d, g.zc,
                                                         s->slot(2), v.ctl);
     k_fin_alpha<<<1, kFinThreads, 0, s->stream>>>(v.ctl, s->slot(2), nblocks(g));
@@ -743,6 +759,60 @@ int mg_options(jg_solver* s, const jg_mg_params* p, const jg_gpu_options* o, jg_
 }  // namespace
 
 // ====================================================================== C ABI
+
+// Validates a host MatrixCsr (32-bit sizes, row pointers from 0 to nnz and
+// non-decreasing, columns inside the square matrix) and uploads it to the
+// solver's compressed-system arrays, growing them when needed; the CG
+// vectors and reduction slots come with it.  Synchronous on return.
+static int csr_upload(jg_solver* s, size_t rows, size_t nnz, const double* values, const size_t* row_ptr,
+               const size_t* col_idx) {
+    if (rows == 0 || !row_ptr || (nnz > 0 && (!values || !col_idx))) return JG_ERR_INVALID_ARG;
+    if (rows >= (size_t(1) << 31) || nnz >= (size_t(1) << 31) || row_ptr[0] != 0 ||
+        row_ptr[rows] != nnz) {
+        s->err = "compressed matrix: row pointers must run from 0 to nnz (< 2^31)";
+        return JG_ERR_INVALID_ARG;
+    }
+    std::vector<int> rp(rows + 1), ci(nnz);
+    for (size_t i = 0; i <= rows; ++i) {
+        if (i > 0 && row_ptr[i] < row_ptr[i - 1]) {
+            s->err = "compressed matrix: row pointers decrease";
+            return JG_ERR_INVALID_ARG;
+        }
+        rp[i] = (int)row_ptr[i];
+    }
+    for (size_t k = 0; k < nnz; ++k) {
+        if (col_idx[k] >= rows) {
+            s->err = "compressed matrix: column index outside the square matrix";
+            return JG_ERR_INVALID_ARG;
+        }
+        ci[k] = (int)col_idx[k];
+    }
+    JG_HIP(hipSetDevice(s->device));
+    jg_solver::Csr& c = s->csr;
+    if (rows > c.cap_rows || nnz > c.cap_nnz || !s->part) {
+        // grow (the old arrays stay owned by the solver until it is released)
+        c = jg_solver::Csr{};
+        JG_TRY(dalloc(s, &c.val, nnz));
+        JG_TRY(dalloc(s, &c.rp, rows + 1));
+        JG_TRY(dalloc(s, &c.ci, nnz));
+        for (double** p : {&c.b, &c.x, &c.r, &c.dA, &c.dB, &c.q}) JG_TRY(dalloc(s, p, rows));
+        c.cap_rows = rows;
+        c.cap_nnz = nnz;
+        if (!s->part) {
+            JG_TRY(dalloc(s, &s->part, (size_t)jg_solver::kSlots * kMaxPartials));
+            JG_TRY(dalloc(s, &s->scal, 16));
+            JG_TRY(dalloc(s, &s->ctl, 2));
+        }
+    }
+    c.rows = rows;
+    c.nnz = nnz;
+    JG_HIP(hipMemcpyAsync(c.rp, rp.data(), (rows + 1) * sizeof(int), hipMemcpyHostToDevice, s->stream));
+    if (nnz > 0) {
+        JG_HIP(hipMemcpyAsync(c.ci, ci.data(), nnz * sizeof(int), hipMemcpyHostToDevice, s->stream));
+        JG_HIP(hipMemcpyAsync(c.val, values, nnz * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    }
+    return sync(s);  // the staging vectors go out of scope
+}
 
 extern "C" {
 
@@ -1067,6 +1137,85 @@ int jg_cg_solve_host(jg_solver* s, size_t nx, size_t ny, size_t nz, const double
     JG_TRY(jg_solver_upload(s, 0, rows, b));
     JG_TRY(jg_solver_cg(s, max_iter, tol, info));
     return jg_solver_download_x(s, x);
+}
+
+int jg_csr_cg_solve_host(jg_solver* s, size_t rows, size_t nnz, const double* values,
+                         const size_t* row_ptr, const size_t* col_idx, const double* b, double* x,
+                         unsigned max_iter, double tol, jg_solve_info* info) {
+    if (!s || !b || !x) return JG_ERR_INVALID_ARG;
+    JG_TRY(csr_upload(s, rows, nnz, values, row_ptr, col_idx));
+    jg_solver::Csr& c = s->csr;
+    JG_HIP(hipMemcpyAsync(c.b, b, rows * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    const CsrDev m{c.rp, c.ci, c.val};
+    const idx_t n = (idx_t)rows;
+    const int nb1 = nblocks_1d(n);
+    Ctl h;
+    init_ctl(&h, max_iter, tol, 0, 0.0);
+    JG_TRY(upload_ctl(s, s->ctl, h));
+    k_fill<double><<<nb1, NT, 0, s->stream>>>(c.x, 0.0, n);  // Jet zeroes x (fdm_cg_solver3.cpp:62)
+    JG_HIP(hipEventRecord(s->ev0, s->stream));
+    Pcg<double> v{nullptr, c.b, c.x, c.r, c.r, c.dA, c.dB, c.q, Dims{(int)rows, 1, 1}, s->ctl, nullptr};
+    v.csr = &m;
+    JG_TRY(pcg_device<double>(s, v, 1));
+    JG_HIP(hipEventRecord(s->ev1, s->stream));
+    JG_HIP(hipMemcpyAsync(s->ctl_host, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
+    JG_TRY(sync(s));
+    const Ctl ch = *s->ctl_host;
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        float ms = 0.f;
+        JG_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        info->solve_ms = ms;
+        info->iterations = ch.iter;
+        info->last_residual = std::sqrt(std::fabs(ch.sigma));
+        // FdmCgSolver3::solveCompressed's return (src/jet/fdm_cg_solver3.cpp:72-73)
+        info->converged = (info->last_residual <= tol) || (ch.iter < max_iter);
+        // ||b - A x|| / ||b|| in FP64 (reporting)
+        k_csr_residual<true, false><<<nb1, NT, 0, s->stream>>>(m, c.x, c.b, c.r, n, s->slot(0), nullptr);
+        k_fin_sum<<<1, kFinThreads, 0, s->stream>>>(s->slot(0), nb1, s->scal + 0);
+        k_dot<double, double><<<nb1, NT, 0, s->stream>>>(c.b, c.b, n, s->slot(1), nullptr);
+        k_fin_sum<<<1, kFinThreads, 0, s->stream>>>(s->slot(1), nb1, s->scal + 1);
+        JG_TRY(check_launch(s, "csr true residual"));
+        double sc[2];
+        JG_HIP(hipMemcpyAsync(sc, s->scal, sizeof(sc), hipMemcpyDeviceToHost, s->stream));
+        JG_TRY(sync(s));
+        info->rel_residual = sc[1] > 0 ? std::sqrt(sc[0] / sc[1]) : std::sqrt(sc[0]);
+    }
+    JG_HIP(hipMemcpyAsync(x, c.x, rows * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    JG_TRY(sync(s));
+    if (ch.nonfinite) {
+        s->err = "non-finite value in CG reduction";
+        return JG_ERR_NONFINITE;
+    }
+    return JG_OK;
+}
+
+int jg_csr_mvm_host(jg_solver* s, size_t rows, size_t nnz, const double* values,
+                    const size_t* row_ptr, const size_t* col_idx, const double* v, double* result) {
+    if (!s || !v || !result) return JG_ERR_INVALID_ARG;
+    JG_TRY(csr_upload(s, rows, nnz, values, row_ptr, col_idx));
+    jg_solver::Csr& c = s->csr;
+    JG_HIP(hipMemcpyAsync(c.x, v, rows * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    k_csr_mvm<<<nblocks_1d((idx_t)rows), NT, 0, s->stream>>>(CsrDev{c.rp, c.ci, c.val}, c.x, c.q,
+                                                             (idx_t)rows);
+    JG_TRY(check_launch(s, "csr mvm"));
+    JG_HIP(hipMemcpyAsync(result, c.q, rows * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    return sync(s);
+}
+
+int jg_csr_residual_host(jg_solver* s, size_t rows, size_t nnz, const double* values,
+                         const size_t* row_ptr, const size_t* col_idx, const double* x,
+                         const double* b, double* result) {
+    if (!s || !x || !b || !result) return JG_ERR_INVALID_ARG;
+    JG_TRY(csr_upload(s, rows, nnz, values, row_ptr, col_idx));
+    jg_solver::Csr& c = s->csr;
+    JG_HIP(hipMemcpyAsync(c.x, x, rows * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    JG_HIP(hipMemcpyAsync(c.b, b, rows * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    k_csr_residual<false, false><<<nblocks_1d((idx_t)rows), NT, 0, s->stream>>>(
+        CsrDev{c.rp, c.ci, c.val}, c.x, c.b, c.r, (idx_t)rows, nullptr, nullptr);
+    JG_TRY(check_launch(s, "csr residual"));
+    JG_HIP(hipMemcpyAsync(result, c.r, rows * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    return sync(s);
 }
 
 int jg_mgpcg_solve_host(jg_solver* s, size_t nlev, const size_t* nx, const size_t* ny,

@@ -77,6 +77,26 @@ bool FdmGpuCgSolver3::solve(FdmLinearSystem3* system) {
     return _info.converged != 0;
 }
 
+bool FdmGpuCgSolver3::solveCompressed(FdmCompressedLinearSystem3* system) {
+    JET_THROW_INVALID_ARG_IF(system == nullptr);
+    const MatrixCsrD& A = system->A;
+    const size_t rows = A.rows();
+    JET_THROW_INVALID_ARG_IF(A.cols() > rows || system->b.size() != rows);
+    system->x.resize(rows, 0.0);
+    if (rows == 0) {
+        _info = jg_solve_info{};
+        return true;
+    }
+    jg_solver* s = lazySolver(&_solver, _device);
+    const size_t nnz = A.numberOfNonZeros();
+    check(jg_csr_cg_solve_host(s, rows, nnz, nnz ? &*A.nonZeroBegin() : nullptr,
+                               &*A.rowPointersBegin(), nnz ? &*A.columnIndicesBegin() : nullptr,
+                               system->b.data(), system->x.data(), _maxNumberOfIterations,
+                               _tolerance, &_info),
+          s, "FdmGpuCgSolver3::solveCompressed");
+    return _info.converged != 0;
+}
+
 unsigned int FdmGpuCgSolver3::maxNumberOfIterations() const {
     return _maxNumberOfIterations;
 }

@@ -22,9 +22,11 @@
 // Semantics kept from Jet: solve() zeroes and overwrites x, returns
 // lastResidual <= tolerance || iterations < max; a size mismatch between A,
 // x and b throws std::invalid_argument (Jet's JET_THROW_INVALID_ARG_IF,
-// include/jet/macros.h:37-39); solveCompressed() returns false (the base
-// class no-op, include/jet/fdm_linear_system_solver3.h:28) and the flat
-// solve of the multigrid solver is FdmMgSolver3's final no-op.  A failure of
+// include/jet/macros.h:37-39); FdmGpuCgSolver3::solveCompressed() runs the
+// same CG on the compressed CSR system (as FdmCgSolver3's does), while the
+// multigrid solver's solveCompressed() returns false (the base class no-op,
+// include/jet/fdm_linear_system_solver3.h:28, as for Jet's MGPCG) and its
+// flat solve is FdmMgSolver3's final no-op.  A failure of
 // the device itself (no GPU, out of device memory) throws std::runtime_error
 // with the library's message; Jet's CPU solvers have no such failure.
 //
@@ -59,6 +61,9 @@ class FdmGpuCgSolver3 final : public FdmLinearSystemSolver3 {
     FdmGpuCgSolver3& operator=(const FdmGpuCgSolver3&) = delete;
 
     bool solve(FdmLinearSystem3* system) override;
+    //! FdmCgSolver3::solveCompressed (src/jet/fdm_cg_solver3.cpp:49-74): the
+    //! same CG on the compressed (fluid-rows-only) CSR system.
+    bool solveCompressed(FdmCompressedLinearSystem3* system) override;
 
     unsigned int maxNumberOfIterations() const;
     unsigned int lastNumberOfIterations() const;

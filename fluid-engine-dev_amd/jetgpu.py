@@ -95,6 +95,10 @@ SIGNATURES = {
                                   C.POINTER(SolveInfo)]),
     "jg_mg_vcycle": (C.c_int, [_vp, C.POINTER(MgParams), C.POINTER(GpuOptions), _vp, _vp]),
     "jg_cg_solve_host": (C.c_int, [_vp, _sz, _sz, _sz, _vp, _vp, _vp, _u, _d, C.POINTER(SolveInfo)]),
+    "jg_csr_cg_solve_host": (C.c_int, [_vp, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _u, _d,
+                                       C.POINTER(SolveInfo)]),
+    "jg_csr_mvm_host": (C.c_int, [_vp, _sz, _sz, _vp, _vp, _vp, _vp, _vp]),
+    "jg_csr_residual_host": (C.c_int, [_vp, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _vp]),
     "jg_mgpcg_solve_host": (C.c_int, [_vp, _sz, C.POINTER(_sz), C.POINTER(_sz), C.POINTER(_sz),
                                       C.POINTER(_vp), _vp, _vp, _u, C.POINTER(MgParams),
                                       C.POINTER(GpuOptions), C.POINTER(SolveInfo)]),
@@ -301,6 +305,58 @@ class Solver:
         self.check(lib().jg_solver_cg(self.h, max_iter, tol, C.byref(info)), "cg")
         return info
 
+    def cg_host(self, A, b, max_iter, tol):
+        """jg_cg_solve_host on host arrays: A (nz, ny, nx, 4) Jet rows, b (nz, ny, nx)."""
+        import numpy as np
+
+        A = np.ascontiguousarray(A, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        nz, ny, nx = b.shape
+        x = np.empty_like(b)
+        info = SolveInfo()
+        self.check(lib().jg_cg_solve_host(self.h, nx, ny, nz, A.ctypes.data, b.ctypes.data,
+                                          x.ctypes.data, max_iter, tol, C.byref(info)), "cg_host")
+        return x, info
+
+    def csr_cg(self, values, row_ptr, col_idx, b, max_iter, tol):
+        """FdmCgSolver3::solveCompressed on host CSR arrays (numpy: float64
+        values, uint64 row pointers / column indices); returns (x, info)."""
+        import numpy as np
+
+        values = np.ascontiguousarray(values, np.float64)
+        row_ptr = np.ascontiguousarray(row_ptr, np.uint64)
+        col_idx = np.ascontiguousarray(col_idx, np.uint64)
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.empty_like(b)
+        info = SolveInfo()
+        self.check(lib().jg_csr_cg_solve_host(self.h, b.size, values.size, values.ctypes.data,
+                                              row_ptr.ctypes.data, col_idx.ctypes.data,
+                                              b.ctypes.data, x.ctypes.data, max_iter, tol,
+                                              C.byref(info)), "csr_cg")
+        return x, info
+
+    def csr_mvm(self, values, row_ptr, col_idx, v):
+        import numpy as np
+
+        values, row_ptr, col_idx, v = _csr_host(values, row_ptr, col_idx, v)
+        out = np.empty_like(v)
+        self.check(lib().jg_csr_mvm_host(self.h, v.size, values.size, values.ctypes.data,
+                                         row_ptr.ctypes.data, col_idx.ctypes.data, v.ctypes.data,
+                                         out.ctypes.data), "csr_mvm")
+        return out
+
+    def csr_residual(self, values, row_ptr, col_idx, x, b):
+        import numpy as np
+
+        values, row_ptr, col_idx, x = _csr_host(values, row_ptr, col_idx, x)
+        b = np.ascontiguousarray(b, np.float64)
+        out = np.empty_like(x)
+        self.check(lib().jg_csr_residual_host(self.h, x.size, values.size, values.ctypes.data,
+                                              row_ptr.ctypes.data, col_idx.ctypes.data,
+                                              x.ctypes.data, b.ctypes.data, out.ctypes.data),
+                   "csr_residual")
+        return out
+
     def mgpcg(self, max_iter, params: MgParams, options: GpuOptions | None = None):
         info = SolveInfo()
         self.check(lib().jg_solver_mgpcg(self.h, max_iter, C.byref(params),
@@ -392,6 +448,13 @@ class Solver:
     def correct(self, fshape, coarse, fine):
         nx, ny, nz = fshape
         self.check(lib().jg_mg_correct(self.h, nx, ny, nz, _ptr(coarse), _ptr(fine)), "correct")
+
+
+def _csr_host(values, row_ptr, col_idx, v):
+    import numpy as np
+
+    return (np.ascontiguousarray(values, np.float64), np.ascontiguousarray(row_ptr, np.uint64),
+            np.ascontiguousarray(col_idx, np.uint64), np.ascontiguousarray(v, np.float64))
 
 
 def _ptr(t):

@@ -165,6 +165,29 @@ int jg_batch_mgpcg(jg_batch* b, unsigned max_iter, const jg_mg_params* p,
                    const jg_gpu_options* o, jg_solve_info* infos, double* total_ms);
 
 /* ------------------------------------------------ host-memory drop-ins */
+/* FdmCgSolver3::solveCompressed (src/jet/fdm_cg_solver3.cpp:49-74): Jet's cg
+ * (include/jet/detail/cg-inl.h:15-94) on a compressed system
+ * (FdmCompressedLinearSystem3, include/jet/fdm_linear_system3.h: rows for the
+ * fluid cells only), with FdmCompressedBlas3's mvm and residual
+ * (src/jet/fdm_linear_system3.cpp:160-199) on the device.  Host arrays
+ * exactly as MatrixCsrD stores them: `values` and `col_idx` (nnz entries,
+ * each row's in ascending column order), `row_ptr` (rows + 1, from 0 to
+ * nnz); b and x have `rows` entries.  x is overwritten (Jet zeroes it).
+ * Rejects (JG_ERR_INVALID_ARG, before any device work) row pointers that do
+ * not run from 0 to nnz or decrease, columns >= rows, and sizes >= 2^31. */
+int jg_csr_cg_solve_host(jg_solver* s, size_t rows, size_t nnz, const double* values,
+                         const size_t* row_ptr, const size_t* col_idx, const double* b,
+                         double* x, unsigned max_iter, double tol, jg_solve_info* info);
+/* FdmCompressedBlas3::mvm and ::residual (src/jet/fdm_linear_system3.cpp:
+ * 160-199) on the device for a host MatrixCsrD as above: result = A v,
+ * result = b - A x (rows entries each); bitwise Jet's. */
+int jg_csr_mvm_host(jg_solver* s, size_t rows, size_t nnz, const double* values,
+                    const size_t* row_ptr, const size_t* col_idx, const double* v,
+                    double* result);
+int jg_csr_residual_host(jg_solver* s, size_t rows, size_t nnz, const double* values,
+                         const size_t* row_ptr, const size_t* col_idx, const double* x,
+                         const double* b, double* result);
+
 /* One-call forms of the two solves on host arrays: upload, solve, download
  * x.  rows[l] / b are host arrays in Jet's layout. */
 int jg_cg_solve_host(jg_solver* s, size_t nx, size_t ny, size_t nz,

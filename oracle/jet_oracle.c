@@ -1487,3 +1487,145 @@ void or_constrain_velocity(int blocked, size_t nx, size_t ny, size_t nz, const d
 #undef COLL
             }
 }
+
+/* ============================================= Compressed systems ====== */
+
+/* FdmCompressedBlas3::mvm -- src/jet/fdm_linear_system3.cpp:160-179. */
+void or_csr_mvm(const double* val, const size_t* rp, const size_t* ci, size_t rows,
+                const double* v, double* result) {
+    for (size_t i = 0; i < rows; ++i) {
+        double sum = 0.0;
+        for (size_t jj = rp[i]; jj < rp[i + 1]; ++jj) sum += val[jj] * v[ci[jj]];
+        result[i] = sum;
+    }
+}
+
+/* FdmCompressedBlas3::residual -- src/jet/fdm_linear_system3.cpp:181-199. */
+void or_csr_residual(const double* val, const size_t* rp, const size_t* ci, size_t rows,
+                     const double* x, const double* b, double* result) {
+    for (size_t i = 0; i < rows; ++i) {
+        double sum = 0.0;
+        for (size_t jj = rp[i]; jj < rp[i + 1]; ++jj) sum += val[jj] * x[ci[jj]];
+        result[i] = b[i] - sum;
+    }
+}
+
+static double vdot(const double* a, const double* b, size_t n) {
+    double s = 0.0; /* VectorN::dot (include/jet/detail/vector_n-inl.h:361-372), one chunk */
+    for (size_t i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+}
+
+static void vaxpy(double a, const double* x, const double* y, double* r, size_t n) {
+    for (size_t i = 0; i < n; ++i) r[i] = a * x[i] + y[i]; /* fdm_linear_system3.cpp:155-158 */
+}
+
+/* FdmCgSolver3::solveCompressed -- src/jet/fdm_cg_solver3.cpp:49-74 -- with
+ * cg / pcg of include/jet/detail/cg-inl.h:15-94 (NullCgPreconditioner: s = r). */
+int or_csr_cg_solve(const double* val, const size_t* rp, const size_t* ci, size_t rows,
+                    const double* b, double* x, unsigned max_iter, double tol,
+                    unsigned* last_iter, double* last_residual) {
+    const size_t n = rows;
+    double* buf = (double*)calloc(4 * (n ? n : 1), sizeof(double));
+    if (!buf) return -1;
+    double *r = buf, *d = buf + n, *q = buf + 2 * n, *s = buf + 3 * n;
+    for (size_t i = 0; i < n; ++i) x[i] = 0.0;
+    or_csr_residual(val, rp, ci, n, x, b, r);
+    memcpy(d, r, n * sizeof(double));
+    double sigma_new = vdot(r, d, n);
+    unsigned iter = 0;
+    int trigger = 0;
+    while (sigma_new > tol * tol && iter < max_iter) {
+        or_csr_mvm(val, rp, ci, n, d, q);
+        const double alpha = sigma_new / vdot(d, q, n);
+        vaxpy(alpha, d, x, x, n);
+        if (trigger || (iter % 50 == 0 && iter > 0)) {
+            or_csr_residual(val, rp, ci, n, x, b, r);
+            trigger = 0;
+        } else {
+            vaxpy(-alpha, q, r, r, n);
+        }
+        memcpy(s, r, n * sizeof(double));
+        const double sigma_old = sigma_new;
+        sigma_new = vdot(r, s, n);
+        if (sigma_new > sigma_old) trigger = 1;
+        const double beta = sigma_new / sigma_old;
+        vaxpy(beta, d, s, d, n);
+        ++iter;
+    }
+    *last_iter = iter;
+    *last_residual = sqrt(fabs(sigma_new));
+    free(buf);
+    return (*last_residual <= tol) || (*last_iter < max_iter);
+}
+
+/* buildSingleSystem, compressed -- src/jet/grid_single_phase_pressure_solver3.cpp:81-171:
+ * fluid cells numbered in x-fastest order; a row holds the centre (the sum of
+ * 1/h^2 over non-boundary neighbours, in the order +x, -x, +y, -y, +z, -z)
+ * and -1/h^2 for each fluid neighbour, sorted by column as MatrixCsr::addRow
+ * does (include/jet/detail/matrix_csr-inl.h:253-275) -- for x-fastest
+ * numbering that is -z, -y, -x, centre, +x, +y, +z. */
+size_t or_sp_build_compressed(const char* m, const double* u, const double* v, const double* w,
+                              size_t nx, size_t ny, size_t nz, const double h[3], double* val,
+                              size_t* rp, size_t* ci, double* b, size_t* nnz_out) {
+    const double ih[3] = {1.0 / h[0], 1.0 / h[1], 1.0 / h[2]};
+    const double ih2[3] = {ih[0] * ih[0], ih[1] * ih[1], ih[2] * ih[2]};
+    const size_t n = nx * ny * nz;
+    size_t* idx = (size_t*)malloc((n ? n : 1) * sizeof(size_t));
+    size_t rows = 0, nnz = 0;
+    for (size_t c = 0; c < n; ++c)
+        if (m[c] == 0 /* fluid */) idx[c] = rows++;
+    if (rp) rp[0] = 0;
+    size_t row = 0;
+    for (size_t k = 0; k < nz; ++k)
+        for (size_t j = 0; j < ny; ++j)
+            for (size_t i = 0; i < nx; ++i) {
+                const size_t c = IX(i, j, k, nx, ny);
+                if (m[c] != 0) continue;
+                double centre = 0.0;
+                /* neighbour (cell, axis, exists), in Jet's order */
+                const long nb[6] = {i + 1 < nx ? (long)(c + 1) : -1, i > 0 ? (long)(c - 1) : -1,
+                                    j + 1 < ny ? (long)(c + nx) : -1, j > 0 ? (long)(c - nx) : -1,
+                                    k + 1 < nz ? (long)(c + nx * ny) : -1,
+                                    k > 0 ? (long)(c - nx * ny) : -1};
+                int fluid[6] = {0, 0, 0, 0, 0, 0};
+                for (int q = 0; q < 6; ++q) {
+                    if (nb[q] < 0 || m[nb[q]] == 2 /* boundary */) continue;
+                    centre += ih2[q / 2];
+                    fluid[q] = m[nb[q]] == 0;
+                }
+                /* ascending column order: -z, -y, -x, centre, +x, +y, +z */
+                const int order[6] = {5, 3, 1, 0, 2, 4};
+                size_t e = nnz;
+                for (int o = 0; o < 3; ++o) {
+                    const int q = order[o];
+                    if (!fluid[q]) continue;
+                    if (val) {
+                        val[e] = -ih2[q / 2];
+                        ci[e] = idx[nb[q]];
+                    }
+                    ++e;
+                }
+                if (val) {
+                    val[e] = centre;
+                    ci[e] = idx[c];
+                }
+                ++e;
+                for (int o = 3; o < 6; ++o) {
+                    const int q = order[o];
+                    if (!fluid[q]) continue;
+                    if (val) {
+                        val[e] = -ih2[q / 2];
+                        ci[e] = idx[nb[q]];
+                    }
+                    ++e;
+                }
+                nnz = e;
+                if (b) b[row] = or_divergence(u, v, w, nx, ny, nz, h, i, j, k);
+                ++row;
+                if (rp) rp[row] = nnz;
+            }
+    free(idx);
+    if (nnz_out) *nnz_out = nnz;
+    return rows;
+}

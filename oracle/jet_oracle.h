@@ -199,6 +199,25 @@ void or_project_and_apply_friction(const double vel[3], const double normal[3], 
                                    double out[3]);
 void or_collider_velocity(const or_rigid_collider* c, const double p[3], double out[3]);
 
+/* ------------------------------------------- compressed (CSR) systems */
+/* MatrixCsrD as Jet stores it: values and column indices (nnz), row
+ * pointers (rows + 1), each row's entries in ascending column order. */
+void or_csr_mvm(const double* val, const size_t* rp, const size_t* ci, size_t rows,
+                const double* v, double* result);
+void or_csr_residual(const double* val, const size_t* rp, const size_t* ci, size_t rows,
+                     const double* x, const double* b, double* result);
+/* FdmCgSolver3::solveCompressed: x zeroed, Jet's cg with FdmCompressedBlas3
+ * (serial dot products); returns Jet's bool. */
+int or_csr_cg_solve(const double* val, const size_t* rp, const size_t* ci, size_t rows,
+                    const double* b, double* x, unsigned max_iter, double tol,
+                    unsigned* last_iter, double* last_residual);
+/* Compressed buildSingleSystem of GridSinglePhasePressureSolver3: rows for
+ * the fluid cells in x-fastest order.  Returns the row count; with NULL
+ * outputs it only counts rows and, in *nnz, non-zeros. */
+size_t or_sp_build_compressed(const char* markers, const double* u, const double* v,
+                              const double* w, size_t nx, size_t ny, size_t nz, const double h[3],
+                              double* val, size_t* rp, size_t* ci, double* b, size_t* nnz);
+
 #ifdef __cplusplus
 }
 #endif

@@ -196,7 +196,48 @@ class _BoundaryCalls:
         return uo, vo, wo
 
 
-class Oracle(_BoundaryCalls):
+class _CsrCalls:
+    """Compressed (CSR) systems; the same calls on the restatement (or_*) and
+    on Jet's compiled FdmCompressedBlas3 / FdmCgSolver3 (ref_*)."""
+
+    _p = "or_"
+
+    @staticmethod
+    def _csr(val, rp, ci):
+        val = np.ascontiguousarray(val, np.float64)
+        rp = np.ascontiguousarray(rp, np.uint64)
+        ci = np.ascontiguousarray(ci, np.uint64)
+        return val, rp, ci, rp.ctypes.data_as(C.POINTER(C.c_size_t)), ci.ctypes.data_as(
+            C.POINTER(C.c_size_t))
+
+    def csr_mvm(self, val, rp, ci, v):
+        val, rp, ci, prp, pci = self._csr(val, rp, ci)
+        v = np.ascontiguousarray(v, np.float64)
+        out = np.empty_like(v)
+        getattr(self.lib, self._p + "csr_mvm")(_d(val), prp, pci, _sz(v.size), _d(v), _d(out))
+        return out
+
+    def csr_residual(self, val, rp, ci, x, b):
+        val, rp, ci, prp, pci = self._csr(val, rp, ci)
+        x = np.ascontiguousarray(x, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        out = np.empty_like(x)
+        getattr(self.lib, self._p + "csr_residual")(_d(val), prp, pci, _sz(x.size), _d(x), _d(b),
+                                                    _d(out))
+        return out
+
+    def csr_cg_solve(self, val, rp, ci, b, max_iter, tol):
+        val, rp, ci, prp, pci = self._csr(val, rp, ci)
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.empty_like(b)
+        it, res = C.c_uint(), C.c_double()
+        ok = getattr(self.lib, self._p + "csr_cg_solve")(_d(val), prp, pci, _sz(b.size), _d(b),
+                                                         _d(x), C.c_uint(max_iter), C.c_double(tol),
+                                                         C.byref(it), C.byref(res))
+        return x, bool(ok), it.value, res.value
+
+
+class Oracle(_BoundaryCalls, _CsrCalls):
     """The C restatement (liboracle.so)."""
 
     def __init__(self, path=LIB_PATH):
@@ -319,6 +360,23 @@ class Oracle(_BoundaryCalls):
                                     hh, _d(A), _d(b))
         return A, b
 
+    def sp_build_compressed(self, markers, u, v, w, h):
+        """Compressed buildSingleSystem: (values, row_ptr, col_idx, b)."""
+        nx, ny, nz = _shape3(markers)
+        hh = np.asarray(h, np.float64)
+        nnz = C.c_size_t()
+        f = self.lib.or_sp_build_compressed
+        f.restype = C.c_size_t
+        args = (_c(markers), _d(u), _d(v), _d(w), _sz(nx), _sz(ny), _sz(nz), _d(hh))
+        rows = f(*args, None, None, None, None, C.byref(nnz))
+        val = np.empty(nnz.value)
+        rp = np.empty(rows + 1, np.uint64)
+        ci = np.empty(nnz.value, np.uint64)
+        b = np.empty(rows)
+        sz_p = C.POINTER(C.c_size_t)
+        f(*args, _d(val), rp.ctypes.data_as(sz_p), ci.ctypes.data_as(sz_p), _d(b), C.byref(nnz))
+        return val, rp, ci, b
+
     def face_fill_coarse(self, u, v, w, h, origin):
         nz, ny, nx = w.shape[0] - 1, w.shape[1], w.shape[2]
         cx, cy, cz = nx // 2, ny // 2, nz // 2
@@ -433,7 +491,7 @@ class Oracle(_BoundaryCalls):
         return (fs, UW, VW, WW), sys
 
 
-class Ref(_BoundaryCalls):
+class Ref(_BoundaryCalls, _CsrCalls):
     """Jet's own compiled linear solvers (oracle/_ref/libjetref.so), and the
     compiled grids, samplers, extrapolation, friction helper and rigid
     collider that Jet's velocity boundary conditions are made of."""

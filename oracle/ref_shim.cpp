@@ -246,6 +246,61 @@ double ref_fraction_inside(double bl, double br, double tl, double tr) {
     return fractionInside(bl, br, tl, tr);
 }
 
+// ------------------------------------------------ compressed systems
+// FdmCompressedBlas3 and FdmCgSolver3::solveCompressed on a MatrixCsrD built
+// row by row with MatrixCsr::addRow from CSR arrays.
+
+static void toCsr(const double* val, const size_t* rp, const size_t* ci, size_t rows,
+                  MatrixCsrD* m) {
+    m->clear();
+    for (size_t i = 0; i < rows; ++i) {
+        std::vector<double> nz(val + rp[i], val + rp[i + 1]);
+        std::vector<size_t> cols(ci + rp[i], ci + rp[i + 1]);
+        m->addRow(nz, cols);
+    }
+}
+
+static void toVectorN(const double* src, size_t n, VectorND* v) {
+    v->resize(n);
+    for (size_t i = 0; i < n; ++i) (*v)[i] = src[i];
+}
+
+void ref_csr_mvm(const double* val, const size_t* rp, const size_t* ci, size_t rows,
+                 const double* v, double* out) {
+    MatrixCsrD m;
+    toCsr(val, rp, ci, rows, &m);
+    VectorND vv, r(rows);
+    toVectorN(v, rows, &vv);
+    FdmCompressedBlas3::mvm(m, vv, &r);
+    for (size_t i = 0; i < rows; ++i) out[i] = r[i];
+}
+
+void ref_csr_residual(const double* val, const size_t* rp, const size_t* ci, size_t rows,
+                      const double* x, const double* b, double* out) {
+    MatrixCsrD m;
+    toCsr(val, rp, ci, rows, &m);
+    VectorND xx, bb, r(rows);
+    toVectorN(x, rows, &xx);
+    toVectorN(b, rows, &bb);
+    FdmCompressedBlas3::residual(m, xx, bb, &r);
+    for (size_t i = 0; i < rows; ++i) out[i] = r[i];
+}
+
+int ref_csr_cg_solve(const double* val, const size_t* rp, const size_t* ci, size_t rows,
+                     const double* b, double* x, unsigned max_iter, double tol, unsigned* iters,
+                     double* residual) {
+    FdmCompressedLinearSystem3 sys;
+    toCsr(val, rp, ci, rows, &sys.A);
+    toVectorN(b, rows, &sys.b);
+    sys.x.resize(rows, 0.0);
+    FdmCgSolver3 solver(max_iter, tol);
+    const bool ok = solver.solveCompressed(&sys);
+    for (size_t i = 0; i < rows; ++i) x[i] = sys.x[i];
+    *iters = solver.lastNumberOfIterations();
+    *residual = solver.lastResidual();
+    return ok ? 1 : 0;
+}
+
 // ------------------------------------------------ boundary conditions
 // Jet's grids, sampler, gradient, extrapolation, friction helper and rigid
 // collider.  GridFractionalBoundaryConditionSolver3 itself cannot be linked

@@ -146,7 +146,14 @@ void argumentChecks() {
     expect(throwsInvalidArgument([&] { cg.solve(nullptr); }),
            "FdmGpuCgSolver3: null system throws std::invalid_argument");
     FdmCompressedLinearSystem3 comp;
-    expect(!cg.solveCompressed(&comp), "FdmGpuCgSolver3::solveCompressed returns false");
+    expect(cg.solveCompressed(&comp), "FdmGpuCgSolver3::solveCompressed: empty system converges");
+    comp.A.addRow({2.0, -1.0}, {0, 1});
+    comp.A.addRow({-1.0, 2.0}, {0, 1});
+    comp.b.resize(3, 1.0);
+    expect(throwsInvalidArgument([&] { cg.solveCompressed(&comp); }),
+           "FdmGpuCgSolver3::solveCompressed: A/b size mismatch throws std::invalid_argument");
+    expect(throwsInvalidArgument([&] { cg.solveCompressed(nullptr); }),
+           "FdmGpuCgSolver3::solveCompressed: null system throws std::invalid_argument");
     expect(cg.maxNumberOfIterations() == 100 && cg.tolerance() == 1e-8,
            "FdmGpuCgSolver3 accessors return the constructor arguments");
 
@@ -162,6 +169,9 @@ void argumentChecks() {
            "FdmGpuMgpcgSolver3 is selected by dynamic_pointer_cast<FdmMgSolver3>");
     FdmLinearSystem3 flat;
     expect(!base->solve(&flat), "FdmGpuMgpcgSolver3 flat solve is the no-op (false)");
+    FdmCompressedLinearSystem3 mcomp;
+    expect(!base->solveCompressed(&mcomp),
+           "FdmGpuMgpcgSolver3::solveCompressed is the base no-op (false), as Jet's MGPCG");
     FdmMgLinearSystem3 mbad;
     mbad.resizeWithFinest(Size3(8, 8, 8), 2);
     mbad.b.levels[0].resize(Size3(8, 8, 4));
@@ -195,6 +205,60 @@ void gpuChecks() {
     gpu.x.set(123.0);
     b->solve(&gpu);
     expect(relDiffMeanFree(gpu.x, first) <= 1e-12, "CG: x is overwritten, not warm-started");
+
+    // --- compressed CG: the same Laplacian as CSR rows, Jet's solveCompressed
+    // and the GPU's behind the same pointer type
+    {
+        FdmCompressedLinearSystem3 ccpu, cgpu;
+        const size_t sx = n.x, sy = n.y, sz = n.z;
+        for (size_t k = 0; k < sz; ++k)
+            for (size_t j = 0; j < sy; ++j)
+                for (size_t i = 0; i < sx; ++i) {
+                    const FdmMatrixRow3& r = cpu.A(i, j, k);
+                    std::vector<double> v{r.center};
+                    std::vector<size_t> c{i + sx * (j + sy * k)};
+                    auto add = [&](double a, size_t col) {
+                        if (a != 0.0) {
+                            v.push_back(a);
+                            c.push_back(col);
+                        }
+                    };
+                    const size_t me = i + sx * (j + sy * k);
+                    if (i + 1 < sx) add(r.right, me + 1);
+                    if (i > 0) add(cpu.A(i - 1, j, k).right, me - 1);
+                    if (j + 1 < sy) add(r.up, me + sx);
+                    if (j > 0) add(cpu.A(i, j - 1, k).up, me - sx);
+                    if (k + 1 < sz) add(r.front, me + sx * sy);
+                    if (k > 0) add(cpu.A(i, j, k - 1).front, me - sx * sy);
+                    ccpu.A.addRow(v, c);
+                    ccpu.b.append(cpu.b(i, j, k));
+                }
+        ccpu.x.resize(ccpu.b.size(), 0.0);
+        cgpu = ccpu;
+        FdmCgSolver3 jcg(2000, 1e-9);
+        FdmGpuCgSolver3 gcg(2000, 1e-9);
+        FdmLinearSystemSolver3* pj = &jcg;
+        FdmLinearSystemSolver3* pg = &gcg;
+        const bool okj = pj->solveCompressed(&ccpu);
+        const bool okg = pg->solveCompressed(&cgpu);
+        std::printf("     compressed cg iterations: Jet %u, GPU %u\n", jcg.lastNumberOfIterations(),
+                    gcg.lastNumberOfIterations());
+        expect(okj && okg, "compressed CG: both solvers return true");
+        expect(std::abs((int)jcg.lastNumberOfIterations() - (int)gcg.lastNumberOfIterations()) <= 2,
+               "compressed CG: iteration counts agree within 2");
+        double num = 0.0, den = 0.0, mg = 0.0, mc = 0.0;
+        for (size_t q = 0; q < cgpu.x.size(); ++q) {
+            mg += cgpu.x[q];
+            mc += ccpu.x[q];
+        }
+        mg /= cgpu.x.size();
+        mc /= ccpu.x.size();
+        for (size_t q = 0; q < cgpu.x.size(); ++q) {
+            num += ((cgpu.x[q] - mg) - (ccpu.x[q] - mc)) * ((cgpu.x[q] - mg) - (ccpu.x[q] - mc));
+            den += (ccpu.x[q] - mc) * (ccpu.x[q] - mc);
+        }
+        expect(std::sqrt(num / den) <= 1e-8, "compressed CG: GPU x matches Jet's x (1e-8)");
+    }
 
     // --- MGPCG: Jet's unit-test system (4 levels from 4^3, spacing 2^l)
     for (int mixed = 0; mixed <= 1; ++mixed) {

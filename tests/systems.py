@@ -104,3 +104,63 @@ def collider_scene(n, seed, spacing=None, origin=(0.1, -0.2, 0.05), radius=0.22,
     w = rng.uniform(-1, 1, (nz + 1, ny, nx))
     motion = (((0.3, -0.1, 0.2), (0.5, 1.0, -0.7)) if moving else ((0, 0, 0), (0, 0, 0)))
     return h, o, sdf, (u, v, w), (motion[0], motion[1], tuple(c), friction)
+
+
+def poisson_csr_kat(n=(3, 3, 3)):
+    """The compressed test system of Jet's FdmCgSolver3.SolveCompressed
+    (src/tests/unit_tests/fdm_linear_system_solver_test_helper3.h:51-118):
+    every cell a row, -1 to each existing neighbour, the neighbour count on
+    the diagonal, b = +1 / -1 for missing -y / +y and -z / +z neighbours;
+    entries sorted by column as MatrixCsr::addRow stores them."""
+    nx, ny, nz = n
+    val, ci, rp, b = [], [], [0], []
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                c = i + nx * (j + ny * k)
+                row = {c: 0.0}
+                bb = 0.0
+                for (di, dj, dk, miss) in ((-1, 0, 0, 0.0), (1, 0, 0, 0.0), (0, -1, 0, 1.0),
+                                           (0, 1, 0, -1.0), (0, 0, -1, 1.0), (0, 0, 1, -1.0)):
+                    ii, jj, kk = i + di, j + dj, k + dk
+                    if 0 <= ii < nx and 0 <= jj < ny and 0 <= kk < nz:
+                        row[c] += 1.0
+                        row[ii + nx * (jj + ny * kk)] = -1.0
+                    else:
+                        bb += miss
+                for col in sorted(row):
+                    ci.append(col)
+                    val.append(row[col])
+                rp.append(len(ci))
+                b.append(bb)
+    return (np.array(val), np.array(rp, np.uint64), np.array(ci, np.uint64), np.array(b))
+
+
+def csr_from_stencil(A, mask):
+    """CSR rows of the cells in mask (x-fastest numbering) from Jet-layout
+    stencil rows (centre, right, up, front), couplings to cells outside the
+    mask dropped: a reference for the compressed build."""
+    nz, ny, nx = mask.shape
+    idx = -np.ones(mask.shape, np.int64)
+    idx[mask] = np.arange(int(mask.sum()))
+    val, ci, rp = [], [], [0]
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                if not mask[k, j, i]:
+                    continue
+                row = {idx[k, j, i]: A[k, j, i, 0]}
+                for (di, dj, dk, own, comp) in ((1, 0, 0, True, 1), (-1, 0, 0, False, 1),
+                                                (0, 1, 0, True, 2), (0, -1, 0, False, 2),
+                                                (0, 0, 1, True, 3), (0, 0, -1, False, 3)):
+                    ii, jj, kk = i + di, j + dj, k + dk
+                    if not (0 <= ii < nx and 0 <= jj < ny and 0 <= kk < nz) or not mask[kk, jj, ii]:
+                        continue
+                    a = A[k, j, i, comp] if own else A[kk, jj, ii, comp]
+                    if a != 0.0:
+                        row[idx[kk, jj, ii]] = a
+                for col in sorted(row):
+                    ci.append(col)
+                    val.append(row[col])
+                rp.append(len(ci))
+    return np.array(val), np.array(rp, np.uint64), np.array(ci, np.uint64)

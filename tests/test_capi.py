@@ -63,6 +63,15 @@ def test_batch_rejects_bad_arguments():
     assert lib.jg_batch_system(None, 0) is None and lib.jg_batch_size(None) == 0
 
 
+def test_null_solver_rejected_by_new_entry_points():
+    # no device work happens before the argument checks
+    lib = jetgpu.lib()
+    assert lib.jg_csr_cg_solve_host(None, 1, 1, None, None, None, None, None, 10, 1e-6, None) == 1
+    assert lib.jg_csr_mvm_host(None, 1, 1, None, None, None, None, None) == 1
+    assert lib.jg_csr_residual_host(None, 1, 1, None, None, None, None, None, None) == 1
+    assert lib.jg_constrain_velocity(None, 0, None, None, None, None, None, None, None, 1, 63) == 1
+
+
 @pytest.mark.parametrize("kind,n", [(1, 16), (2, 16), (3, 32), (4, 24), (5, 32)])
 def test_scene_generator_deterministic_and_bc(kind, n):
     d = jetgpu.scene_desc(kind, n, seed=7)
