@@ -157,6 +157,12 @@ struct jg_solver {
         double* val = nullptr;
         int *rp = nullptr, *ci = nullptr;
         double *b = nullptr, *x = nullptr, *r = nullptr, *dA = nullptr, *dB = nullptr, *q = nullptr;
+        // built on the device from the finest level (jg_sp_build_compressed):
+        // per cell its row (ro) and first entry (eo), the counts, scan scratch
+        bool from_grid = false;
+        int *ro = nullptr, *eo = nullptr, *cr = nullptr, *ce = nullptr;
+        void* scan_tmp = nullptr;
+        size_t scan_bytes = 0;
     } csr;
     // Reduction slots and device scalars.
     double* part = nullptr;  // kSlots * kMaxPartials
@@ -219,3 +225,7 @@ inline int sync(jg_solver* s) {
 }
 
 }  // namespace jg
+
+// Room on the device for a compressed system of `rows` rows and `nnz`
+// entries with its CG vectors (jg_solver.hip); sets csr.rows / csr.nnz.
+int csr_reserve(jg_solver* s, size_t rows, size_t nnz);

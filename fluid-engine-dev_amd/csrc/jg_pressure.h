@@ -180,4 +180,95 @@ __global__ __launch_bounds__(256) void k_synth_row_open(synth::Scene s, unsigned
     if (!synth::solid_cell(s, i, j, k)) atomicAdd(&rows[j], 1u);
 }
 
+// Compressed buildSingleSystem (src/jet/grid_single_phase_pressure_solver3.cpp:
+// 81-171) in three passes over the finest level's markers (0 fluid, 2
+// boundary): count, exclusive scans of the counts (row number and first
+// entry of every fluid cell), fill.
+
+// Pass 1: rows[c] = 1 for a fluid cell, nnz[c] = 1 + its fluid neighbours.
+__global__ __launch_bounds__(256) void k_csr_count(const int8_t* __restrict__ m, Dims d,
+                                                   int* __restrict__ rows, int* __restrict__ nnz) {
+    const idx_t n = d.n(), sy = d.nx, sz = (idx_t)d.nx * d.ny;
+    for (idx_t c = blockIdx.x * (idx_t)256 + threadIdx.x; c < n; c += (idx_t)gridDim.x * 256) {
+        if (m[c] != 0) {
+            rows[c] = 0;
+            nnz[c] = 0;
+            continue;
+        }
+        const int i = (int)(c % d.nx), j = (int)((c / d.nx) % d.ny), k = (int)(c / sz);
+        int e = 1;
+        e += (i + 1 < d.nx && m[c + 1] == 0) + (i > 0 && m[c - 1] == 0);
+        e += (j + 1 < d.ny && m[c + sy] == 0) + (j > 0 && m[c - sy] == 0);
+        e += (k + 1 < d.nz && m[c + sz] == 0) + (k > 0 && m[c - sz] == 0);
+        rows[c] = 1;
+        nnz[c] = e;
+    }
+}
+
+// Pass 3: the row of fluid cell c (row ro[c], entries from eo[c]) in
+// ascending column order -- for x-fastest numbering -z, -y, -x, centre, +x,
+// +y, +z, the order MatrixCsr::addRow sorts Jet's row into
+// (include/jet/detail/matrix_csr-inl.h:253-275) -- with the centre summed in
+// Jet's order +x, -x, +y, -y, +z, -z over non-boundary neighbours, and
+// b = FaceCenteredGrid3::divergenceAtCellCenter (as k_sp_build).
+__global__ __launch_bounds__(256) void k_csr_fill(const int8_t* __restrict__ m,
+                                                  const double* __restrict__ u,
+                                                  const double* __restrict__ v,
+                                                  const double* __restrict__ w, Dims d, double hx,
+                                                  double hy, double hz, const int* __restrict__ ro,
+                                                  const int* __restrict__ eo, int* __restrict__ rp,
+                                                  int* __restrict__ ci, double* __restrict__ val,
+                                                  double* __restrict__ b) {
+#pragma clang fp contract(off)
+    const idx_t n = d.n(), sy = d.nx, sz = (idx_t)d.nx * d.ny;
+    const double ihx = 1.0 / hx, ihy = 1.0 / hy, ihz = 1.0 / hz;
+    const double ih2[3] = {ihx * ihx, ihy * ihy, ihz * ihz};
+    for (idx_t c = blockIdx.x * (idx_t)256 + threadIdx.x; c < n; c += (idx_t)gridDim.x * 256) {
+        if (m[c] != 0) continue;
+        const int i = (int)(c % d.nx), j = (int)((c / d.nx) % d.ny), k = (int)(c / sz);
+        // neighbours in Jet's order +x, -x, +y, -y, +z, -z
+        const bool in[6] = {i + 1 < d.nx, i > 0, j + 1 < d.ny, j > 0, k + 1 < d.nz, k > 0};
+        const idx_t off[6] = {1, -1, sy, -sy, sz, -sz};
+        double centre = 0.0;
+        bool fl[6];
+        for (int q = 0; q < 6; ++q) {
+            const int8_t mq = in[q] ? m[c + off[q]] : int8_t(2);
+            if (mq != 2) centre += ih2[q / 2];
+            fl[q] = mq == 0;
+        }
+        const int row = ro[c];
+        int e = eo[c];
+        rp[row] = e;
+        const int order[6] = {5, 3, 1, 0, 2, 4};
+        for (int o = 0; o < 6; ++o) {
+            if (o == 3) {
+                val[e] = centre;
+                ci[e] = row;
+                ++e;
+            }
+            const int q = order[o];
+            if (!fl[q]) continue;
+            val[e] = -ih2[q / 2];
+            ci[e] = ro[c + off[q]];
+            ++e;
+        }
+        const double lu = u[i + (idx_t)(d.nx + 1) * (j + (idx_t)d.ny * k)];
+        const double ru = u[i + 1 + (idx_t)(d.nx + 1) * (j + (idx_t)d.ny * k)];
+        const double bv = v[i + (idx_t)d.nx * (j + (idx_t)(d.ny + 1) * k)];
+        const double tv = v[i + (idx_t)d.nx * (j + 1 + (idx_t)(d.ny + 1) * k)];
+        const double bw = w[c], fw = w[c + sz];
+        b[row] = (ru - lu) / hx + (tv - bv) / hy + (fw - bw) / hz;
+    }
+}
+
+// decompressSolution (src/jet/grid_single_phase_pressure_solver3.cpp:330-341):
+// the solution of fluid row ro[c] at its cell, 0 at every other cell.
+__global__ __launch_bounds__(256) void k_csr_scatter(const int8_t* __restrict__ m,
+                                                     const int* __restrict__ ro,
+                                                     const double* __restrict__ xc,
+                                                     double* __restrict__ x, idx_t n) {
+    for (idx_t c = blockIdx.x * (idx_t)256 + threadIdx.x; c < n; c += (idx_t)gridDim.x * 256)
+        x[c] = m[c] == 0 ? xc[ro[c]] : 0.0;
+}
+
 }  // namespace jg

@@ -2,6 +2,8 @@
 // projection and the synthetic-scene generators.
 #include <thread>
 
+#include <hipcub/hipcub.hpp>
+
 #include "jg_internal.h"
 #include "jg_pressure.h"
 #include "jg_fractional.h"
@@ -357,6 +359,91 @@ int jg_constrain_velocity(jg_solver* s, int blocked, const double h[3], const do
     }
     JG_TRY(check_launch(s, "constrain_velocity"));
     return sync(s);
+}
+
+int jg_sp_build_compressed(jg_solver* s, const double* u, const double* v, const double* w,
+                           const double* boundary_sdf, const double* fluid_sdf, double hx,
+                           double hy, double hz, size_t* rows_out, size_t* nnz_out) {
+    if (!s || s->lv.empty() || !u || !v || !w || !boundary_sdf || !fluid_sdf || !(hx > 0) ||
+        !(hy > 0) || !(hz > 0))
+        return JG_ERR_INVALID_ARG;
+    JG_HIP(hipSetDevice(s->device));
+    Level& l0 = s->lv[0];
+    const Dims& d = l0.d;
+    const idx_t n = d.n();
+    if (n >= (idx_t(1) << 31) / 7) {
+        s->err = "jg_sp_build_compressed: grid too large for 32-bit CSR indices";
+        return JG_ERR_UNSUPPORTED;
+    }
+    jg_solver::Csr& c = s->csr;
+    if (!c.ro) {  // per-cell counts and offsets of the finest level (freed with it)
+        JG_TRY(dalloc(s, &c.ro, n));
+        JG_TRY(dalloc(s, &c.eo, n));
+        JG_TRY(dalloc(s, &c.cr, n));
+        JG_TRY(dalloc(s, &c.ce, n));
+        JG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, c.scan_bytes, c.cr, c.ro, (int)n, s->stream));
+        char* tmp = nullptr;
+        JG_TRY(dalloc(s, &tmp, c.scan_bytes));
+        c.scan_tmp = tmp;
+    }
+    const int nb = nblocks_1d(n);
+    k_sp_markers<<<nb, 256, 0, s->stream>>>(boundary_sdf, fluid_sdf, l0.markers, n);
+    k_csr_count<<<nb, 256, 0, s->stream>>>(l0.markers, d, c.cr, c.ce);
+    size_t bytes = c.scan_bytes;
+    JG_HIP(hipcub::DeviceScan::ExclusiveSum(c.scan_tmp, bytes, c.cr, c.ro, (int)n, s->stream));
+    JG_HIP(hipcub::DeviceScan::ExclusiveSum(c.scan_tmp, bytes, c.ce, c.eo, (int)n, s->stream));
+    int last[4];
+    JG_HIP(hipMemcpyAsync(&last[0], c.ro + n - 1, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    JG_HIP(hipMemcpyAsync(&last[1], c.cr + n - 1, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    JG_HIP(hipMemcpyAsync(&last[2], c.eo + n - 1, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    JG_HIP(hipMemcpyAsync(&last[3], c.ce + n - 1, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    JG_TRY(sync(s));
+    const size_t rows = (size_t)last[0] + last[1], nnz = (size_t)last[2] + last[3];
+    JG_TRY(csr_reserve(s, rows > 0 ? rows : 1, nnz));
+    c.rows = rows;
+    if (rows > 0) {
+        k_csr_fill<<<nb, 256, 0, s->stream>>>(l0.markers, u, v, w, d, hx, hy, hz, c.ro, c.eo, c.rp,
+                                              c.ci, c.val, c.b);
+        const int end = (int)nnz;
+        JG_HIP(hipMemcpyAsync(c.rp + rows, &end, sizeof(int), hipMemcpyHostToDevice, s->stream));
+    }
+    JG_TRY(check_launch(s, "sp_build_compressed"));
+    c.from_grid = true;
+    if (rows_out) *rows_out = rows;
+    if (nnz_out) *nnz_out = nnz;
+    return sync(s);
+}
+
+int jg_csr_decompress(jg_solver* s) {
+    if (!s || s->lv.empty() || !s->csr.from_grid || !s->csr.ro) {
+        if (s) s->err = "jg_csr_decompress needs a system from jg_sp_build_compressed";
+        return JG_ERR_INVALID_ARG;
+    }
+    JG_HIP(hipSetDevice(s->device));
+    const idx_t n = s->lv[0].d.n();
+    if (s->csr.rows == 0) {
+        JG_HIP(hipMemsetAsync(s->x, 0, (size_t)n * sizeof(double), s->stream));
+    } else {
+        k_csr_scatter<<<nblocks_1d(n), 256, 0, s->stream>>>(s->lv[0].markers, s->csr.ro, s->csr.x,
+                                                            s->x, n);
+    }
+    JG_TRY(check_launch(s, "csr_decompress"));
+    return sync(s);
+}
+
+const double* jg_csr_array(jg_solver* s, int which) {
+    if (!s) return nullptr;
+    switch (which) {
+        case 0: return s->csr.val;
+        case 1: return s->csr.b;
+        case 2: return s->csr.x;
+        default: return nullptr;
+    }
+}
+
+const int* jg_csr_index(jg_solver* s, int which) {
+    if (!s) return nullptr;
+    return which == 0 ? s->csr.rp : (which == 1 ? s->csr.ci : nullptr);
 }
 
 const float* jg_fr_array(jg_solver* s, size_t level, int which) {

@@ -760,6 +760,76 @@ int mg_options(jg_solver* s, const jg_mg_params* p, const jg_gpu_options* o, jg_
 
 // ====================================================================== C ABI
 
+// FdmCgSolver3::solveCompressed on the solver's resident compressed system
+// (A, b; x is overwritten): Jet's cg through pcg_device with the CSR kernels.
+static int csr_solve(jg_solver* s, unsigned max_iter, double tol, jg_solve_info* info) {
+    jg_solver::Csr& c = s->csr;
+    const CsrDev m{c.rp, c.ci, c.val};
+    const idx_t n = (idx_t)c.rows;
+    const int nb1 = nblocks_1d(n);
+    Ctl h;
+    init_ctl(&h, max_iter, tol, 0, 0.0);
+    JG_TRY(upload_ctl(s, s->ctl, h));
+    k_fill<double><<<nb1, NT, 0, s->stream>>>(c.x, 0.0, n);  // Jet zeroes x (fdm_cg_solver3.cpp:62)
+    JG_HIP(hipEventRecord(s->ev0, s->stream));
+    Pcg<double> v{nullptr, c.b, c.x, c.r, c.r, c.dA, c.dB, c.q, Dims{(int)c.rows, 1, 1}, s->ctl, nullptr};
+    v.csr = &m;
+    JG_TRY(pcg_device<double>(s, v, 1));
+    JG_HIP(hipEventRecord(s->ev1, s->stream));
+    JG_HIP(hipMemcpyAsync(s->ctl_host, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
+    JG_TRY(sync(s));
+    const Ctl ch = *s->ctl_host;
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        float ms = 0.f;
+        JG_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        info->solve_ms = ms;
+        info->iterations = ch.iter;
+        info->last_residual = std::sqrt(std::fabs(ch.sigma));
+        // FdmCgSolver3::solveCompressed's return (src/jet/fdm_cg_solver3.cpp:72-73)
+        info->converged = (info->last_residual <= tol) || (ch.iter < max_iter);
+        // ||b - A x|| / ||b|| in FP64 (reporting)
+        k_csr_residual<true, false><<<nb1, NT, 0, s->stream>>>(m, c.x, c.b, c.r, n, s->slot(0), nullptr);
+        k_fin_sum<<<1, kFinThreads, 0, s->stream>>>(s->slot(0), nb1, s->scal + 0);
+        k_dot<double, double><<<nb1, NT, 0, s->stream>>>(c.b, c.b, n, s->slot(1), nullptr);
+        k_fin_sum<<<1, kFinThreads, 0, s->stream>>>(s->slot(1), nb1, s->scal + 1);
+        JG_TRY(check_launch(s, "csr true residual"));
+        double sc[2];
+        JG_HIP(hipMemcpyAsync(sc, s->scal, sizeof(sc), hipMemcpyDeviceToHost, s->stream));
+        JG_TRY(sync(s));
+        info->rel_residual = sc[1] > 0 ? std::sqrt(sc[0] / sc[1]) : std::sqrt(sc[0]);
+    }
+    if (ch.nonfinite) {
+        s->err = "non-finite value in CG reduction";
+        return JG_ERR_NONFINITE;
+    }
+    return JG_OK;
+}
+
+// Room for a compressed system of rows x nnz and its CG vectors (grown when
+// needed: the old arrays stay owned by the solver until it is released), and
+// the reduction slots / control block if the solver has no hierarchy yet.
+int csr_reserve(jg_solver* s, size_t rows, size_t nnz) {
+    jg_solver::Csr& c = s->csr;
+    if (rows > c.cap_rows || nnz > c.cap_nnz) {
+        c.val = nullptr;
+        JG_TRY(dalloc(s, &c.val, nnz));
+        JG_TRY(dalloc(s, &c.rp, rows + 1));
+        JG_TRY(dalloc(s, &c.ci, nnz));
+        for (double** p : {&c.b, &c.x, &c.r, &c.dA, &c.dB, &c.q}) JG_TRY(dalloc(s, p, rows));
+        c.cap_rows = rows;
+        c.cap_nnz = nnz;
+    }
+    if (!s->part) {
+        JG_TRY(dalloc(s, &s->part, (size_t)jg_solver::kSlots * kMaxPartials));
+        JG_TRY(dalloc(s, &s->scal, 16));
+        JG_TRY(dalloc(s, &s->ctl, 2));
+    }
+    c.rows = rows;
+    c.nnz = nnz;
+    return JG_OK;
+}
+
 // Validates a host MatrixCsr (32-bit sizes, row pointers from 0 to nnz and
 // non-decreasing, columns inside the square matrix) and uploads it to the
 // solver's compressed-system arrays, growing them when needed; the CG
@@ -788,24 +858,9 @@ static int csr_upload(jg_solver* s, size_t rows, size_t nnz, const double* value
         ci[k] = (int)col_idx[k];
     }
     JG_HIP(hipSetDevice(s->device));
+    JG_TRY(csr_reserve(s, rows, nnz));
     jg_solver::Csr& c = s->csr;
-    if (rows > c.cap_rows || nnz > c.cap_nnz || !s->part) {
-        // grow (the old arrays stay owned by the solver until it is released)
-        c = jg_solver::Csr{};
-        JG_TRY(dalloc(s, &c.val, nnz));
-        JG_TRY(dalloc(s, &c.rp, rows + 1));
-        JG_TRY(dalloc(s, &c.ci, nnz));
-        for (double** p : {&c.b, &c.x, &c.r, &c.dA, &c.dB, &c.q}) JG_TRY(dalloc(s, p, rows));
-        c.cap_rows = rows;
-        c.cap_nnz = nnz;
-        if (!s->part) {
-            JG_TRY(dalloc(s, &s->part, (size_t)jg_solver::kSlots * kMaxPartials));
-            JG_TRY(dalloc(s, &s->scal, 16));
-            JG_TRY(dalloc(s, &s->ctl, 2));
-        }
-    }
-    c.rows = rows;
-    c.nnz = nnz;
+    c.from_grid = false;
     JG_HIP(hipMemcpyAsync(c.rp, rp.data(), (rows + 1) * sizeof(int), hipMemcpyHostToDevice, s->stream));
     if (nnz > 0) {
         JG_HIP(hipMemcpyAsync(c.ci, ci.data(), nnz * sizeof(int), hipMemcpyHostToDevice, s->stream));
@@ -1146,48 +1201,20 @@ int jg_csr_cg_solve_host(jg_solver* s, size_t rows, size_t nnz, const double* va
     JG_TRY(csr_upload(s, rows, nnz, values, row_ptr, col_idx));
     jg_solver::Csr& c = s->csr;
     JG_HIP(hipMemcpyAsync(c.b, b, rows * sizeof(double), hipMemcpyHostToDevice, s->stream));
-    const CsrDev m{c.rp, c.ci, c.val};
-    const idx_t n = (idx_t)rows;
-    const int nb1 = nblocks_1d(n);
-    Ctl h;
-    init_ctl(&h, max_iter, tol, 0, 0.0);
-    JG_TRY(upload_ctl(s, s->ctl, h));
-    k_fill<double><<<nb1, NT, 0, s->stream>>>(c.x, 0.0, n);  // Jet zeroes x (fdm_cg_solver3.cpp:62)
-    JG_HIP(hipEventRecord(s->ev0, s->stream));
-    Pcg<double> v{nullptr, c.b, c.x, c.r, c.r, c.dA, c.dB, c.q, Dims{(int)rows, 1, 1}, s->ctl, nullptr};
-    v.csr = &m;
-    JG_TRY(pcg_device<double>(s, v, 1));
-    JG_HIP(hipEventRecord(s->ev1, s->stream));
-    JG_HIP(hipMemcpyAsync(s->ctl_host, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
-    JG_TRY(sync(s));
-    const Ctl ch = *s->ctl_host;
-    if (info) {
-        std::memset(info, 0, sizeof(*info));
-        float ms = 0.f;
-        JG_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
-        info->solve_ms = ms;
-        info->iterations = ch.iter;
-        info->last_residual = std::sqrt(std::fabs(ch.sigma));
-        // FdmCgSolver3::solveCompressed's return (src/jet/fdm_cg_solver3.cpp:72-73)
-        info->converged = (info->last_residual <= tol) || (ch.iter < max_iter);
-        // ||b - A x|| / ||b|| in FP64 (reporting)
-        k_csr_residual<true, false><<<nb1, NT, 0, s->stream>>>(m, c.x, c.b, c.r, n, s->slot(0), nullptr);
-        k_fin_sum<<<1, kFinThreads, 0, s->stream>>>(s->slot(0), nb1, s->scal + 0);
-        k_dot<double, double><<<nb1, NT, 0, s->stream>>>(c.b, c.b, n, s->slot(1), nullptr);
-        k_fin_sum<<<1, kFinThreads, 0, s->stream>>>(s->slot(1), nb1, s->scal + 1);
-        JG_TRY(check_launch(s, "csr true residual"));
-        double sc[2];
-        JG_HIP(hipMemcpyAsync(sc, s->scal, sizeof(sc), hipMemcpyDeviceToHost, s->stream));
-        JG_TRY(sync(s));
-        info->rel_residual = sc[1] > 0 ? std::sqrt(sc[0] / sc[1]) : std::sqrt(sc[0]);
-    }
+    const int st = csr_solve(s, max_iter, tol, info);
+    if (st != JG_OK && st != JG_ERR_NONFINITE) return st;
     JG_HIP(hipMemcpyAsync(x, c.x, rows * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     JG_TRY(sync(s));
-    if (ch.nonfinite) {
-        s->err = "non-finite value in CG reduction";
-        return JG_ERR_NONFINITE;
+    return st;
+}
+
+int jg_solver_csr_cg(jg_solver* s, unsigned max_iter, double tol, jg_solve_info* info) {
+    if (!s || s->csr.rows == 0 || !s->csr.rp) {
+        if (s) s->err = "jg_solver_csr_cg before a compressed system was built or uploaded";
+        return JG_ERR_INVALID_ARG;
     }
-    return JG_OK;
+    JG_HIP(hipSetDevice(s->device));
+    return csr_solve(s, max_iter, tol, info);
 }
 
 int jg_csr_mvm_host(jg_solver* s, size_t rows, size_t nnz, const double* values,

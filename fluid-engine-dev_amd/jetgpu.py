@@ -98,6 +98,12 @@ SIGNATURES = {
     "jg_csr_cg_solve_host": (C.c_int, [_vp, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _u, _d,
                                        C.POINTER(SolveInfo)]),
     "jg_csr_mvm_host": (C.c_int, [_vp, _sz, _sz, _vp, _vp, _vp, _vp, _vp]),
+    "jg_sp_build_compressed": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d, C.POINTER(_sz),
+                                         C.POINTER(_sz)]),
+    "jg_solver_csr_cg": (C.c_int, [_vp, _u, _d, C.POINTER(SolveInfo)]),
+    "jg_csr_decompress": (C.c_int, [_vp]),
+    "jg_csr_array": (_vp, [_vp, C.c_int]),
+    "jg_csr_index": (_vp, [_vp, C.c_int]),
     "jg_csr_residual_host": (C.c_int, [_vp, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _vp]),
     "jg_mgpcg_solve_host": (C.c_int, [_vp, _sz, C.POINTER(_sz), C.POINTER(_sz), C.POINTER(_sz),
                                       C.POINTER(_vp), _vp, _vp, _u, C.POINTER(MgParams),
@@ -335,6 +341,39 @@ class Solver:
                                               C.byref(info)), "csr_cg")
         return x, info
 
+    def sp_build_compressed(self, u, v, w, bsdf, fsdf, h):
+        """Resident compressed system of the finest level; returns (rows, nnz)."""
+        rows, nnz = _sz(), _sz()
+        self.check(lib().jg_sp_build_compressed(self.h, _ptr(u), _ptr(v), _ptr(w), _ptr(bsdf),
+                                                _ptr(fsdf), h, h, h, C.byref(rows), C.byref(nnz)),
+                   "sp_build_compressed")
+        return rows.value, nnz.value
+
+    def csr_solve(self, max_iter, tol):
+        info = SolveInfo()
+        self.check(lib().jg_solver_csr_cg(self.h, max_iter, tol, C.byref(info)), "csr_solve")
+        return info
+
+    def csr_decompress(self):
+        self.check(lib().jg_csr_decompress(self.h), "csr_decompress")
+
+    def csr_download(self, rows, nnz):
+        """Host copies (values, row_ptr, col_idx, b, x) of the resident compressed system."""
+        import numpy as np
+
+        out = (np.empty(nnz), np.empty(rows + 1, np.int32), np.empty(nnz, np.int32),
+               np.empty(rows), np.empty(rows))
+        srcs = (lib().jg_csr_array(self.h, 0), lib().jg_csr_index(self.h, 0),
+                lib().jg_csr_index(self.h, 1), lib().jg_csr_array(self.h, 1),
+                lib().jg_csr_array(self.h, 2))
+        import torch
+
+        torch.cuda.synchronize()
+        for a, p in zip(out, srcs):
+            if a.size:
+                _memcpy_dtoh(a, p)
+        return out
+
     def csr_mvm(self, values, row_ptr, col_idx, v):
         import numpy as np
 
@@ -448,6 +487,22 @@ class Solver:
     def correct(self, fshape, coarse, fine):
         nx, ny, nz = fshape
         self.check(lib().jg_mg_correct(self.h, nx, ny, nz, _ptr(coarse), _ptr(fine)), "correct")
+
+
+class _DeviceView:
+    """A device address seen as a 1-D array (__cuda_array_interface__)."""
+
+    def __init__(self, ptr, n, typestr):
+        self.__cuda_array_interface__ = {"shape": (n,), "typestr": typestr, "data": (ptr, False),
+                                         "version": 2}
+
+
+def _memcpy_dtoh(a, ptr):
+    """Device address ptr -> host numpy array a (same length and dtype)."""
+    import torch
+
+    t = torch.as_tensor(_DeviceView(ptr, a.size, a.dtype.str), device="cuda")
+    a[...] = t.cpu().numpy()
 
 
 def _csr_host(values, row_ptr, col_idx, v):

@@ -274,6 +274,30 @@ int jg_sp_apply_gradient(jg_solver* s, const double* u, const double* v,
 /* Device pointer of the markers of a level (int8: 0 fluid, 1 air, 2 boundary). */
 const signed char* jg_solver_markers(jg_solver* s, size_t level);
 
+/* The compressed form of jg_sp_build: buildMarkers and the compressed
+ * buildSingleSystem of GridSinglePhasePressureSolver3
+ * (src/jet/grid_single_phase_pressure_solver3.cpp:81-171, 248-269) on the
+ * device, for the finest level: one row per fluid cell in x-fastest order,
+ * its entries in ascending column order (as MatrixCsr::addRow keeps them),
+ * b = div u.  The system is the solver's resident compressed system (solve
+ * it with jg_solver_csr_cg); *rows and *nnz receive its size. */
+int jg_sp_build_compressed(jg_solver* s, const double* u, const double* v, const double* w,
+                           const double* boundary_sdf, const double* fluid_sdf, double hx,
+                           double hy, double hz, size_t* rows, size_t* nnz);
+/* FdmCgSolver3::solveCompressed (src/jet/fdm_cg_solver3.cpp:49-74) on the
+ * resident compressed system (jg_sp_build_compressed or the last
+ * jg_csr_*_host call). */
+int jg_solver_csr_cg(jg_solver* s, unsigned max_iter, double tol, jg_solve_info* info);
+/* GridSinglePhasePressureSolver3::decompressSolution (:330-341): the
+ * compressed solution into the solver's finest-level solution vector
+ * (jg_solver_solution), 0 at non-fluid cells -- after which
+ * jg_sp_apply_gradient projects as for a flat solve. */
+int jg_csr_decompress(jg_solver* s);
+/* Device pointers of the resident compressed system: jg_csr_array 0 values,
+ * 1 b, 2 x (double); jg_csr_index 0 row pointers, 1 column indices (int32). */
+const double* jg_csr_array(jg_solver* s, int which);
+const int* jg_csr_index(jg_solver* s, int which);
+
 /* GridFractionalSinglePhasePressureSolver3::buildWeights + buildSystem
  * (src/jet/grid_fractional_single_phase_pressure_solver3.cpp:520-713) on the
  * solver's resident hierarchy.  Device inputs: face velocities u, v, w; the
