@@ -180,9 +180,12 @@ def main():
 
     bc = bc_leg(jg, solver, n, args)
     out["constrain_velocity"] = bc["gpu"]
+    csr = csr_leg(jg, n, args)
+    out["compressed_cg"] = csr["gpu"]
     if not args.no_cpu:
         out.update(cpu_leg(jg, n, L, args, res))
         out["constrain_velocity"].update(bc_cpu_leg(bc, n))
+        out["compressed_cg"].update(csr_cpu_leg(csr))
     print(json.dumps(out))
     solver.close()
 
@@ -224,6 +227,68 @@ def bc_leg(jg, solver, n, args):
     return {"gpu": {"gpu_ms": round(ms, 4), "faces": faces, "depth": BC_DEPTH, "blocked": True,
                     "collider": "the scene's spheres, resting (cell-centre SDF)"},
             "vel0": vel0, "first": first, "sdf": bsh}
+
+
+def csr_leg(jg, n, args):
+    """The compressed path on the same scene: GridSinglePhasePressureSolver3's
+    compressed build on the device (fluid rows only) and FdmCgSolver3::
+    solveCompressed (Jet's CG, no preconditioner) to ||r|| <= 1e-8 ||b||."""
+    import torch
+
+    f64 = torch.float64
+    shapes = [(n, n, n + 1), (n, n + 1, n), (n + 1, n, n), (n, n, n), (n, n, n)]
+    u, v, w, bs, fs = (torch.empty(sh, dtype=f64, device="cuda") for sh in shapes)
+    s = jg.Solver(0)
+    s.set_levels([(n, n, n)])
+    s.generate_scene(jg.scene_desc(jg.JG_SCENE_SPHERE_POOL, n, seed=args.seed), u, v, w, bs, fs)
+    torch.cuda.synchronize()
+    s.sp_build_compressed(u, v, w, bs, fs, 1.0)  # first build: allocation
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rows, nnz = s.sp_build_compressed(u, v, w, bs, fs, 1.0)
+    torch.cuda.synchronize()
+    build_ms = (time.perf_counter() - t0) * 1e3
+    b = torch.as_tensor(jg._DeviceView(jg.lib().jg_csr_array(s.h, 1), rows, "<f8"), device="cuda")
+    tol = 1e-8 * float(torch.linalg.norm(b).item())
+    t0 = time.perf_counter()
+    info = s.csr_solve(20000, tol)
+    solve_ms = (time.perf_counter() - t0) * 1e3
+    it = max(1, info.iterations)
+    # algorithmic bytes of one CG iteration: the fused direction + product
+    # (values and 32-bit columns per entry, row pointer, z / d in, d / q out),
+    # x and r update (4 reads, 2 writes), r.r (r = z without preconditioner)
+    per_it = 12 * nnz + (4 + 32) * rows + 48 * rows + 8 * rows
+    log(f"bench: compressed rows={rows} nnz={nnz} build {build_ms:.2f} ms, CG {it} it "
+        f"{solve_ms:.1f} ms, rel {info.rel_residual:.1e}")
+    sample = None
+    if not args.no_cpu:  # a bounded sample of the same system for the CPU leg
+        val, rp, ci, bh, _ = s.csr_download(rows, nnz)
+        sample = (val, rp, ci, bh)
+    s.close()
+    del u, v, w, bs, fs, b
+    torch.cuda.empty_cache()
+    return {"gpu": {"rows": rows, "nnz": nnz, "build_ms": round(build_ms, 3),
+                    "solve_ms": round(solve_ms, 2), "iterations": info.iterations,
+                    "rel_residual": info.rel_residual,
+                    "ms_per_iteration": round(solve_ms / it, 4),
+                    "alg_bytes_per_iteration": per_it,
+                    "alg_gbs": round(per_it * it / (solve_ms * 1e-3) / 1e9, 1)},
+            "sample": sample}
+
+
+def csr_cpu_leg(csr):
+    """The oracle's compressed CG (serial C) on the same rows, 20 iterations:
+    the CPU time per iteration beside the GPU's."""
+    from oracle.jet_oracle import Oracle
+
+    val, rp, ci, b = csr["sample"]
+    iters = 20
+    t0 = time.perf_counter()
+    Oracle().csr_cg_solve(val, rp.astype("uint64"), ci.astype("uint64"), b, iters, 0.0)
+    ms = (time.perf_counter() - t0) * 1e3
+    log(f"bench: compressed CG oracle {ms / iters:.1f} ms per iteration")
+    return {"cpu_ms_per_iteration": round(ms / iters, 2), "cpu_cores": 1, "cpu_kind": "port",
+            "cpu_sample": f"{iters} iterations of the oracle's compressed CG on the same system"}
 
 
 def bc_cpu_leg(bc, n):

@@ -18,15 +18,25 @@ struct CsrDev {
     const int* rp;      // rows + 1 row pointers
     const int* ci;      // column indices
     const double* val;  // non-zeros
+    int nnz;            // entries: a row's range is clamped to it
 };
+
+// Entry range of row i, clamped to [0, nnz] so that no row pointer can send
+// a thread past the arrays.
+__device__ __forceinline__ void csr_row(const CsrDev& m, idx_t i, int& b, int& e) {
+    b = min(max(m.rp[i], 0), m.nnz);
+    e = min(max(m.rp[i + 1], b), m.nnz);
+}
 
 // FdmCompressedBlas3::mvm (:160-179): y_i = sum_jj val[jj] x[ci[jj]].
 __global__ __launch_bounds__(NT) void k_csr_mvm(CsrDev m, const double* __restrict__ x,
                                                 double* __restrict__ y, idx_t n) {
 #pragma clang fp contract(off)
     for (idx_t i = blockIdx.x * (idx_t)NT + threadIdx.x; i < n; i += (idx_t)gridDim.x * NT) {
+        int jb, je;
+        csr_row(m, i, jb, je);
         double sum = 0.0;
-        for (int jj = m.rp[i]; jj < m.rp[i + 1]; ++jj) sum += m.val[jj] * x[m.ci[jj]];
+        for (int jj = jb; jj < je; ++jj) sum += m.val[jj] * x[m.ci[jj]];
         y[i] = sum;
     }
 }
@@ -44,8 +54,10 @@ __global__ __launch_bounds__(NT) void k_csr_residual(CsrDev m, const double* __r
     if (REFRESH && (ctl->done || !ctl->refresh)) return;
     double acc = 0.0;
     for (idx_t i = blockIdx.x * (idx_t)NT + threadIdx.x; i < n; i += (idx_t)gridDim.x * NT) {
+        int jb, je;
+        csr_row(m, i, jb, je);
         double sum = 0.0;
-        for (int jj = m.rp[i]; jj < m.rp[i + 1]; ++jj) sum += m.val[jj] * x[m.ci[jj]];
+        for (int jj = jb; jj < je; ++jj) sum += m.val[jj] * x[m.ci[jj]];
         const double rv = b[i] - sum;
         r[i] = rv;
         if (DOT) acc += rv * rv;
@@ -69,8 +81,10 @@ __global__ __launch_bounds__(NT) void k_csr_dir_spmv(CsrDev m, const double* __r
     const double beta = ctl->beta;
     double acc = 0.0;
     for (idx_t i = blockIdx.x * (idx_t)NT + threadIdx.x; i < n; i += (idx_t)gridDim.x * NT) {
+        int jb, je;
+        csr_row(m, i, jb, je);
         double sum = 0.0;
-        for (int jj = m.rp[i]; jj < m.rp[i + 1]; ++jj) {
+        for (int jj = jb; jj < je; ++jj) {
             const int j = m.ci[jj];
             sum += m.val[jj] * (beta * dold[j] + z[j]);
         }

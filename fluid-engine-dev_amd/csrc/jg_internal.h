@@ -109,7 +109,9 @@ inline Geo geo_stencil(const Dims& d) {
     const long cols = (long)gx * gy;
     // Aim for >= 2048 blocks; at most 8 planes per block; bounded partials.
     long zc = std::max<long>(1, std::min<long>(8, cols * d.nz / 2048));
-    while ((long)gx * gy * ((d.nz + zc - 1) / zc) > kMaxPartials) ++zc;
+    // (bounded by nz: a plane of more than kMaxPartials tiles is rejected
+    // by jg_solver_set_levels, so the bound is always met there)
+    while ((long)gx * gy * ((d.nz + zc - 1) / zc) > kMaxPartials && zc < d.nz) ++zc;
     g.zc = (int)zc;
     g.block = dim3(bx, by, 1);
     g.grid = dim3(gx, gy, (d.nz + g.zc - 1) / g.zc);

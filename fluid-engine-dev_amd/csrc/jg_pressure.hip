@@ -401,10 +401,10 @@ int jg_sp_build_compressed(jg_solver* s, const double* u, const double* v, const
     const size_t rows = (size_t)last[0] + last[1], nnz = (size_t)last[2] + last[3];
     JG_TRY(csr_reserve(s, rows > 0 ? rows : 1, nnz));
     c.rows = rows;
+    const int end = (int)nnz;  // the source of the copy below: alive until the sync
     if (rows > 0) {
         k_csr_fill<<<nb, 256, 0, s->stream>>>(l0.markers, u, v, w, d, hx, hy, hz, c.ro, c.eo, c.rp,
                                               c.ci, c.val, c.b);
-        const int end = (int)nnz;
         JG_HIP(hipMemcpyAsync(c.rp + rows, &end, sizeof(int), hipMemcpyHostToDevice, s->stream));
     }
     JG_TRY(check_launch(s, "sp_build_compressed"));

@@ -590,10 +590,11 @@ template <typename T>
 int pcg_iteration(jg_solver* s, const Pcg<T>& v, bool odd) {
     const idx_t n = v.d.n();
     const int nb1 = nblocks_1d(n);
-    Geo g = geo_stencil(v.d);
     T* dold = odd ? v.dB : v.dA;
     T* dnew = odd ? v.dA : v.dB;
     if constexpr (sizeof(T) == 8) {
+        // (no stencil geometry here: geo_stencil of a {rows, 1, 1} shape
+        // never meets its partial-count bound once rows > 64 * kMaxPartials)
         if (v.csr) {
             k_csr_dir_spmv<<<nb1, NT, 0, s->stream>>>(*v.csr, v.z, dold, dnew, v.q, n, s->slot(2),
                                                       v.ctl);
@@ -607,6 +608,7 @@ int pcg_iteration(jg_solver* s, const Pcg<T>& v, bool odd) {
             return check_launch(s, "csr pcg iteration");
         }
     }
+    Geo g = geo_stencil(v.d);
     k_dir_spmv<T, T><<<g.grid, g.block, 0, s->stream>>>(v.A, v.z, dold, dnew, v.q, v.d, g.zc,
                                                         s->slot(2), v.ctl);
     k_fin_alpha<<<1, kFinThreads, 0, s->stream>>>(v.ctl, s->slot(2), nblocks(g));
@@ -764,7 +766,7 @@ int mg_options(jg_solver* s, const jg_mg_params* p, const jg_gpu_options* o, jg_
 // (A, b; x is overwritten): Jet's cg through pcg_device with the CSR kernels.
 static int csr_solve(jg_solver* s, unsigned max_iter, double tol, jg_solve_info* info) {
     jg_solver::Csr& c = s->csr;
-    const CsrDev m{c.rp, c.ci, c.val};
+    const CsrDev m{c.rp, c.ci, c.val, (int)c.nnz};
     const idx_t n = (idx_t)c.rows;
     const int nb1 = nblocks_1d(n);
     Ctl h;
@@ -938,6 +940,12 @@ int jg_solver_set_levels(jg_solver* s, size_t nlev, const size_t* nx, const size
         if (l > 0 && (nx[l - 1] != 2 * nx[l] || ny[l - 1] != 2 * ny[l] || nz[l - 1] != 2 * nz[l])) {
             s->err = "level sizes must halve exactly";
             return JG_ERR_INVALID_ARG;
+        }
+        // one reduction partial per stencil tile of a plane at least
+        const Geo g = geo_stencil(Dims{(int)nx[l], (int)ny[l], (int)nz[l]});
+        if ((long)g.grid.x * g.grid.y * g.grid.z > kMaxPartials) {
+            s->err = "level plane too large for the reduction partials (x*y beyond 2^20 cells)";
+            return JG_ERR_UNSUPPORTED;
         }
     }
     bool same = s->lv.size() == nlev;
@@ -1223,7 +1231,7 @@ int jg_csr_mvm_host(jg_solver* s, size_t rows, size_t nnz, const double* values,
     JG_TRY(csr_upload(s, rows, nnz, values, row_ptr, col_idx));
     jg_solver::Csr& c = s->csr;
     JG_HIP(hipMemcpyAsync(c.x, v, rows * sizeof(double), hipMemcpyHostToDevice, s->stream));
-    k_csr_mvm<<<nblocks_1d((idx_t)rows), NT, 0, s->stream>>>(CsrDev{c.rp, c.ci, c.val}, c.x, c.q,
+    k_csr_mvm<<<nblocks_1d((idx_t)rows), NT, 0, s->stream>>>(CsrDev{c.rp, c.ci, c.val, (int)c.nnz}, c.x, c.q,
                                                              (idx_t)rows);
     JG_TRY(check_launch(s, "csr mvm"));
     JG_HIP(hipMemcpyAsync(result, c.q, rows * sizeof(double), hipMemcpyDeviceToHost, s->stream));
@@ -1239,7 +1247,7 @@ int jg_csr_residual_host(jg_solver* s, size_t rows, size_t nnz, const double* va
     JG_HIP(hipMemcpyAsync(c.x, x, rows * sizeof(double), hipMemcpyHostToDevice, s->stream));
     JG_HIP(hipMemcpyAsync(c.b, b, rows * sizeof(double), hipMemcpyHostToDevice, s->stream));
     k_csr_residual<false, false><<<nblocks_1d((idx_t)rows), NT, 0, s->stream>>>(
-        CsrDev{c.rp, c.ci, c.val}, c.x, c.b, c.r, (idx_t)rows, nullptr, nullptr);
+        CsrDev{c.rp, c.ci, c.val, (int)c.nnz}, c.x, c.b, c.r, (idx_t)rows, nullptr, nullptr);
     JG_TRY(check_launch(s, "csr residual"));
     JG_HIP(hipMemcpyAsync(result, c.r, rows * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     return sync(s);
