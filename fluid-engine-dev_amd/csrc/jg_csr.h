@@ -96,4 +96,79 @@ __global__ __launch_bounds__(NT) void k_csr_dir_spmv(CsrDev m, const double* __r
     block_partial<NT>(acc, partial);
 }
 
+// ELL copy of a compressed system whose rows have at most kEllMaxW entries
+// (a pressure row has at most 7): entry k of row i at k * rows + i, so one
+// thread per row reads its entries with fully coalesced loads; len[i]
+// entries are used, in the CSR order, so every sum is the CSR kernels'.
+constexpr int kEllMaxW = 8;
+
+struct EllDev {
+    const double* val;
+    const int* ci;
+    const int8_t* len;
+    idx_t rows;
+};
+
+__global__ __launch_bounds__(NT) void k_csr_to_ell(CsrDev m, idx_t rows, double* __restrict__ val,
+                                                   int* __restrict__ ci, int8_t* __restrict__ len,
+                                                   int w) {
+    for (idx_t i = blockIdx.x * (idx_t)NT + threadIdx.x; i < rows; i += (idx_t)gridDim.x * NT) {
+        int jb, je;
+        csr_row(m, i, jb, je);
+        const int l = min(je - jb, w);
+        len[i] = (int8_t)l;
+        for (int k = 0; k < w; ++k) {
+            val[k * rows + i] = k < l ? m.val[jb + k] : 0.0;
+            ci[k * rows + i] = k < l ? m.ci[jb + k] : (int)i;
+        }
+    }
+}
+
+// k_csr_residual on the ELL copy.
+template <bool DOT, bool REFRESH>
+__global__ __launch_bounds__(NT) void k_ell_residual(EllDev m, const double* __restrict__ x,
+                                                     const double* __restrict__ b,
+                                                     double* __restrict__ r,
+                                                     double* __restrict__ partial,
+                                                     const Ctl* __restrict__ ctl) {
+#pragma clang fp contract(off)
+    if (REFRESH && (ctl->done || !ctl->refresh)) return;
+    double acc = 0.0;
+    for (idx_t i = blockIdx.x * (idx_t)NT + threadIdx.x; i < m.rows; i += (idx_t)gridDim.x * NT) {
+        const int l = m.len[i];
+        double sum = 0.0;
+        for (int k = 0; k < l; ++k) sum += m.val[k * m.rows + i] * x[m.ci[k * m.rows + i]];
+        const double rv = b[i] - sum;
+        r[i] = rv;
+        if (DOT) acc += rv * rv;
+    }
+    if (DOT) block_partial<NT>(acc, partial);
+}
+
+// k_csr_dir_spmv on the ELL copy.
+__global__ __launch_bounds__(NT) void k_ell_dir_spmv(EllDev m, const double* __restrict__ z,
+                                                     const double* __restrict__ dold,
+                                                     double* __restrict__ dnew,
+                                                     double* __restrict__ q,
+                                                     double* __restrict__ partial,
+                                                     const Ctl* __restrict__ ctl) {
+#pragma clang fp contract(off)
+    if (ctl->done) return;
+    const double beta = ctl->beta;
+    double acc = 0.0;
+    for (idx_t i = blockIdx.x * (idx_t)NT + threadIdx.x; i < m.rows; i += (idx_t)gridDim.x * NT) {
+        const int l = m.len[i];
+        double sum = 0.0;
+        for (int k = 0; k < l; ++k) {
+            const int j = m.ci[k * m.rows + i];
+            sum += m.val[k * m.rows + i] * (beta * dold[j] + z[j]);
+        }
+        const double di = beta * dold[i] + z[i];
+        dnew[i] = di;
+        q[i] = sum;
+        acc += di * sum;
+    }
+    block_partial<NT>(acc, partial);
+}
+
 }  // namespace jg

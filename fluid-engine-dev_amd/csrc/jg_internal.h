@@ -162,6 +162,14 @@ struct jg_solver {
         // built on the device from the finest level (jg_sp_build_compressed):
         // per cell its row (ro) and first entry (eo), the counts, scan scratch
         bool from_grid = false;
+        // ELL copy for the CG kernels (rows of at most kEllMaxW entries)
+        double* ell_val = nullptr;
+        int* ell_ci = nullptr;
+        int8_t* ell_len = nullptr;
+        size_t ell_cap = 0, ell_rows_cap = 0;
+        int ell_w = 0;  // 0: no ELL copy (a longer row), the CSR kernels run
+        int max_row = 0;         // longest row (an upper bound for a device build)
+        bool ell_valid = false;  // the ELL copy matches the CSR arrays
         int *ro = nullptr, *eo = nullptr, *cr = nullptr, *ce = nullptr;
         void* scan_tmp = nullptr;
         size_t scan_bytes = 0;

@@ -409,6 +409,8 @@ int jg_sp_build_compressed(jg_solver* s, const double* u, const double* v, const
     }
     JG_TRY(check_launch(s, "sp_build_compressed"));
     c.from_grid = true;
+    c.max_row = 7;  // the centre and at most six fluid neighbours
+    c.ell_valid = false;
     if (rows_out) *rows_out = rows;
     if (nnz_out) *nnz_out = nnz;
     return sync(s);

@@ -541,6 +541,7 @@ struct Pcg {
     Ctl* ctl;
     const MgCfg<T>* mg;  // nullptr: no preconditioner (z aliases r)
     const CsrDev* csr = nullptr;  // compressed system (FP64, no preconditioner): A unused, d = {rows, 1, 1}
+    const EllDev* ell = nullptr;  // its ELL copy, when it has one
 };
 
 // z = M r and the r.z / r.r partials in slots 0 / 1 (from the V-cycle's last
@@ -596,12 +597,20 @@ int pcg_iteration(jg_solver* s, const Pcg<T>& v, bool odd) {
         // (no stencil geometry here: geo_stencil of a {rows, 1, 1} shape
         // never meets its partial-count bound once rows > 64 * kMaxPartials)
         if (v.csr) {
-            k_csr_dir_spmv<<<nb1, NT, 0, s->stream>>>(*v.csr, v.z, dold, dnew, v.q, n, s->slot(2),
-                                                      v.ctl);
+            if (v.ell)
+                k_ell_dir_spmv<<<nb1, NT, 0, s->stream>>>(*v.ell, v.z, dold, dnew, v.q, s->slot(2),
+                                                          v.ctl);
+            else
+                k_csr_dir_spmv<<<nb1, NT, 0, s->stream>>>(*v.csr, v.z, dold, dnew, v.q, n,
+                                                          s->slot(2), v.ctl);
             k_fin_alpha<<<1, kFinThreads, 0, s->stream>>>(v.ctl, s->slot(2), nb1);
             k_update_xr<T><<<nb1, NT, 0, s->stream>>>(v.x, v.r, dnew, v.q, n, v.ctl);
-            k_csr_residual<false, true><<<nb1, NT, 0, s->stream>>>(*v.csr, v.x, v.b, v.r, n,
-                                                                   nullptr, v.ctl);
+            if (v.ell)
+                k_ell_residual<false, true><<<nb1, NT, 0, s->stream>>>(*v.ell, v.x, v.b, v.r,
+                                                                       nullptr, v.ctl);
+            else
+                k_csr_residual<false, true><<<nb1, NT, 0, s->stream>>>(*v.csr, v.x, v.b, v.r, n,
+                                                                       nullptr, v.ctl);
             const int nb = pcg_precondition<T>(s, v, v.ctl);
             if (nb < 0) return -nb;
             k_fin_sigma<<<1, kFinThreads, 0, s->stream>>>(v.ctl, s->slot(0), s->slot(0), nb);
@@ -762,10 +771,41 @@ int mg_options(jg_solver* s, const jg_mg_params* p, const jg_gpu_options* o, jg_
 
 // ====================================================================== C ABI
 
+// The ELL copy of the resident compressed system when its rows are short
+// enough (the longest row is found on the device); otherwise ell_w = 0 and
+// the CG runs the CSR kernels.
+static int csr_prepare_ell(jg_solver* s) {
+    jg_solver::Csr& c = s->csr;
+    if (c.ell_valid) return JG_OK;
+    c.ell_w = 0;
+    const int w = c.max_row;
+    if (c.rows == 0 || w <= 0 || w > kEllMaxW) {
+        c.ell_valid = true;  // the CSR kernels run
+        return JG_OK;
+    }
+    const size_t cells = (size_t)w * c.rows;
+    if (cells > c.ell_cap) {
+        JG_TRY(dalloc(s, &c.ell_val, cells));
+        JG_TRY(dalloc(s, &c.ell_ci, cells));
+        c.ell_cap = cells;
+    }
+    if (c.rows > c.ell_rows_cap) {
+        JG_TRY(dalloc(s, &c.ell_len, c.rows));
+        c.ell_rows_cap = c.rows;
+    }
+    k_csr_to_ell<<<nblocks_1d((idx_t)c.rows), NT, 0, s->stream>>>(
+        CsrDev{c.rp, c.ci, c.val, (int)c.nnz}, (idx_t)c.rows, c.ell_val, c.ell_ci, c.ell_len, w);
+    JG_TRY(check_launch(s, "csr to ell"));
+    c.ell_w = w;
+    c.ell_valid = true;
+    return JG_OK;
+}
+
 // FdmCgSolver3::solveCompressed on the solver's resident compressed system
 // (A, b; x is overwritten): Jet's cg through pcg_device with the CSR kernels.
 static int csr_solve(jg_solver* s, unsigned max_iter, double tol, jg_solve_info* info) {
     jg_solver::Csr& c = s->csr;
+    JG_TRY(csr_prepare_ell(s));
     const CsrDev m{c.rp, c.ci, c.val, (int)c.nnz};
     const idx_t n = (idx_t)c.rows;
     const int nb1 = nblocks_1d(n);
@@ -776,6 +816,8 @@ static int csr_solve(jg_solver* s, unsigned max_iter, double tol, jg_solve_info*
     JG_HIP(hipEventRecord(s->ev0, s->stream));
     Pcg<double> v{nullptr, c.b, c.x, c.r, c.r, c.dA, c.dB, c.q, Dims{(int)c.rows, 1, 1}, s->ctl, nullptr};
     v.csr = &m;
+    const EllDev e{c.ell_val, c.ell_ci, c.ell_len, (idx_t)c.rows};
+    if (c.ell_w > 0) v.ell = &e;
     JG_TRY(pcg_device<double>(s, v, 1));
     JG_HIP(hipEventRecord(s->ev1, s->stream));
     JG_HIP(hipMemcpyAsync(s->ctl_host, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
@@ -845,12 +887,14 @@ static int csr_upload(jg_solver* s, size_t rows, size_t nnz, const double* value
         return JG_ERR_INVALID_ARG;
     }
     std::vector<int> rp(rows + 1), ci(nnz);
+    int max_row = 0;
     for (size_t i = 0; i <= rows; ++i) {
         if (i > 0 && row_ptr[i] < row_ptr[i - 1]) {
             s->err = "compressed matrix: row pointers decrease";
             return JG_ERR_INVALID_ARG;
         }
         rp[i] = (int)row_ptr[i];
+        if (i > 0) max_row = std::max(max_row, rp[i] - rp[i - 1]);
     }
     for (size_t k = 0; k < nnz; ++k) {
         if (col_idx[k] >= rows) {
@@ -863,6 +907,8 @@ static int csr_upload(jg_solver* s, size_t rows, size_t nnz, const double* value
     JG_TRY(csr_reserve(s, rows, nnz));
     jg_solver::Csr& c = s->csr;
     c.from_grid = false;
+    c.max_row = max_row;
+    c.ell_valid = false;
     JG_HIP(hipMemcpyAsync(c.rp, rp.data(), (rows + 1) * sizeof(int), hipMemcpyHostToDevice, s->stream));
     if (nnz > 0) {
         JG_HIP(hipMemcpyAsync(c.ci, ci.data(), nnz * sizeof(int), hipMemcpyHostToDevice, s->stream));
