@@ -271,4 +271,57 @@ __global__ __launch_bounds__(256) void k_csr_scatter(const int8_t* __restrict__ 
         x[c] = m[c] == 0 ? xc[ro[c]] : 0.0;
 }
 
+// GridBackwardEulerDiffusionSolver3::buildMarkers / buildMatrix /
+// buildVectors (src/jet/grid_backward_euler_diffusion_solver3.cpp:175-330) for
+// one data grid: markers from the SDFs at the data points (boundary first,
+// then fluid, else air), rows (I + dt mu L) with the neighbour rules of the
+// Neumann / Dirichlet boundary types, b = f plus, for Dirichlet, c f of
+// boundary neighbours.  c = dt mu / h^2 per axis.
+__global__ __launch_bounds__(256) void k_diffusion_build(const double* __restrict__ f,
+                                                         const double* __restrict__ bsdf,
+                                                         const double* __restrict__ fsdf, Dims d,
+                                                         double cx, double cy, double cz,
+                                                         int dirichlet, Row<double>* __restrict__ A,
+                                                         double* __restrict__ b) {
+#pragma clang fp contract(off)
+    const idx_t n = d.n(), oy = d.nx, oz = (idx_t)d.nx * d.ny;
+    auto mk = [&](idx_t q) -> int8_t { return bsdf[q] < 0 ? kBoundary : (fsdf[q] < 0 ? kFluid : kAir); };
+    for (idx_t q = blockIdx.x * (idx_t)256 + threadIdx.x; q < n; q += (idx_t)gridDim.x * 256) {
+        const int i = (int)(q % d.nx), j = (int)((q / d.nx) % d.ny), k = (int)(q / oz);
+        Row<double> row{1.0, 0.0, 0.0, 0.0};
+        double bv = f[q];
+        if (mk(q) == kFluid) {
+            auto counts = [&](idx_t qq) {
+                const int8_t m = mk(qq);
+                return (dirichlet && m != kAir) || m == kFluid;
+            };
+            if (i + 1 < d.nx) {
+                if (counts(q + 1)) row.c += cx;
+                if (mk(q + 1) == kFluid) row.r -= cx;
+            }
+            if (i > 0 && counts(q - 1)) row.c += cx;
+            if (j + 1 < d.ny) {
+                if (counts(q + oy)) row.c += cy;
+                if (mk(q + oy) == kFluid) row.u -= cy;
+            }
+            if (j > 0 && counts(q - oy)) row.c += cy;
+            if (k + 1 < d.nz) {
+                if (counts(q + oz)) row.c += cz;
+                if (mk(q + oz) == kFluid) row.f -= cz;
+            }
+            if (k > 0 && counts(q - oz)) row.c += cz;
+            if (dirichlet) {
+                if (i + 1 < d.nx && mk(q + 1) == kBoundary) bv += cx * f[q + 1];
+                if (i > 0 && mk(q - 1) == kBoundary) bv += cx * f[q - 1];
+                if (j + 1 < d.ny && mk(q + oy) == kBoundary) bv += cy * f[q + oy];
+                if (j > 0 && mk(q - oy) == kBoundary) bv += cy * f[q - oy];
+                if (k + 1 < d.nz && mk(q + oz) == kBoundary) bv += cz * f[q + oz];
+                if (k > 0 && mk(q - oz) == kBoundary) bv += cz * f[q - oz];
+            }
+        }
+        A[q] = row;
+        b[q] = bv;
+    }
+}
+
 }  // namespace jg

@@ -416,6 +416,25 @@ int jg_sp_build_compressed(jg_solver* s, const double* u, const double* v, const
     return sync(s);
 }
 
+int jg_diffusion_build(jg_solver* s, const double* f, const double* boundary_sdf,
+                       const double* fluid_sdf, double dt, double mu, double hx, double hy,
+                       double hz, int dirichlet) {
+    if (!s || s->lv.empty() || !f || !boundary_sdf || !fluid_sdf || !(hx > 0) || !(hy > 0) ||
+        !(hz > 0))
+        return JG_ERR_INVALID_ARG;
+    JG_HIP(hipSetDevice(s->device));
+    Level& l0 = s->lv[0];
+    // c = dt * mu / (h * h), as GridBackwardEulerDiffusionSolver3::solve
+    // evaluates it (src/jet/grid_backward_euler_diffusion_solver3.cpp:33)
+    const double sc = dt * mu;
+    const double cx = sc / (hx * hx), cy = sc / (hy * hy), cz = sc / (hz * hz);
+    k_diffusion_build<<<nblocks_1d(l0.d.n()), 256, 0, s->stream>>>(
+        f, boundary_sdf, fluid_sdf, l0.d, cx, cy, cz, dirichlet ? 1 : 0, l0.A64, s->b);
+    JG_TRY(check_launch(s, "diffusion_build"));
+    s->f32_valid = false;
+    return sync(s);
+}
+
 int jg_csr_decompress(jg_solver* s) {
     if (!s || s->lv.empty() || !s->csr.from_grid || !s->csr.ro) {
         if (s) s->err = "jg_csr_decompress needs a system from jg_sp_build_compressed";

@@ -101,6 +101,7 @@ SIGNATURES = {
     "jg_sp_build_compressed": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d, C.POINTER(_sz),
                                          C.POINTER(_sz)]),
     "jg_solver_csr_cg": (C.c_int, [_vp, _u, _d, C.POINTER(SolveInfo)]),
+    "jg_diffusion_build": (C.c_int, [_vp, _vp, _vp, _vp, _d, _d, _d, _d, _d, C.c_int]),
     "jg_csr_decompress": (C.c_int, [_vp]),
     "jg_csr_array": (_vp, [_vp, C.c_int]),
     "jg_csr_index": (_vp, [_vp, C.c_int]),
@@ -348,6 +349,11 @@ class Solver:
                                                 _ptr(fsdf), h, h, h, C.byref(rows), C.byref(nnz)),
                    "sp_build_compressed")
         return rows.value, nnz.value
+
+    def diffusion_build(self, f, bsdf, fsdf, dt, mu, h, dirichlet=False):
+        hx, hy, hz = (h, h, h) if isinstance(h, (int, float)) else h
+        self.check(lib().jg_diffusion_build(self.h, _ptr(f), _ptr(bsdf), _ptr(fsdf), dt, mu, hx, hy,
+                                            hz, 1 if dirichlet else 0), "diffusion_build")
 
     def csr_solve(self, max_iter, tol):
         info = SolveInfo()

@@ -274,6 +274,19 @@ int jg_sp_apply_gradient(jg_solver* s, const double* u, const double* v,
 /* Device pointer of the markers of a level (int8: 0 fluid, 1 air, 2 boundary). */
 const signed char* jg_solver_markers(jg_solver* s, size_t level);
 
+/* GridBackwardEulerDiffusionSolver3's system (buildMarkers, buildMatrix,
+ * buildVectors; src/jet/grid_backward_euler_diffusion_solver3.cpp:175-330) on
+ * the device for one data grid of the solver's finest-level size: a scalar
+ * grid's data, or one component of a collocated or face-centred vector grid
+ * (its u, v or w array, with the level sized to that array).  f is the field,
+ * boundary_sdf / fluid_sdf the SDFs at its data points; dirichlet != 0 selects
+ * Jet's Dirichlet boundary type (Neumann otherwise).  A (level 0) and b become
+ * the resident system: jg_solver_cg then gives the diffused field in the
+ * solution vector, as Jet's solve (:25-50) with a CG linear solver. */
+int jg_diffusion_build(jg_solver* s, const double* f, const double* boundary_sdf,
+                       const double* fluid_sdf, double dt, double mu, double hx, double hy,
+                       double hz, int dirichlet);
+
 /* The compressed form of jg_sp_build: buildMarkers and the compressed
  * buildSingleSystem of GridSinglePhasePressureSolver3
  * (src/jet/grid_single_phase_pressure_solver3.cpp:81-171, 248-269) on the

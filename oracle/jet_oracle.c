@@ -1629,3 +1629,56 @@ size_t or_sp_build_compressed(const char* m, const double* u, const double* v, c
     if (nnz_out) *nnz_out = nnz;
     return rows;
 }
+
+/* ========================================= Backward Euler diffusion ====== */
+
+/* GridBackwardEulerDiffusionSolver3::buildMarkers / buildMatrix /
+ * buildVectors -- src/jet/grid_backward_euler_diffusion_solver3.cpp:175-330
+ * (markers 0 fluid, 1 air, 2 boundary, :16-18). */
+void or_diffusion_build(const double* f, const double* bsdf, const double* fsdf, size_t sx,
+                        size_t sy, size_t sz, const double c[3], int dirichlet, or_row* A,
+                        double* b) {
+    const size_t n = sx * sy * sz;
+    char* m = (char*)malloc(n ? n : 1);
+    for (size_t q = 0; q < n; ++q) m[q] = bsdf[q] < 0 ? 2 : (fsdf[q] < 0 ? 0 : 1);
+    const size_t ox = 1, oy = sx, oz = sx * sy;
+    for (size_t k = 0; k < sz; ++k)
+        for (size_t j = 0; j < sy; ++j)
+            for (size_t i = 0; i < sx; ++i) {
+                const size_t q = IX(i, j, k, sx, sy);
+                or_row row = {1.0, 0.0, 0.0, 0.0};
+                if (m[q] == 0) {
+                    /* a neighbour counts on the diagonal if it is fluid, or
+                     * (Dirichlet) anything but air; fluid ones couple */
+#define COUNTS(qq) ((dirichlet && m[qq] != 1) || m[qq] == 0)
+                    if (i + 1 < sx) {
+                        if (COUNTS(q + ox)) row.center += c[0];
+                        if (m[q + ox] == 0) row.right -= c[0];
+                    }
+                    if (i > 0 && COUNTS(q - ox)) row.center += c[0];
+                    if (j + 1 < sy) {
+                        if (COUNTS(q + oy)) row.center += c[1];
+                        if (m[q + oy] == 0) row.up -= c[1];
+                    }
+                    if (j > 0 && COUNTS(q - oy)) row.center += c[1];
+                    if (k + 1 < sz) {
+                        if (COUNTS(q + oz)) row.center += c[2];
+                        if (m[q + oz] == 0) row.front -= c[2];
+                    }
+                    if (k > 0 && COUNTS(q - oz)) row.center += c[2];
+#undef COUNTS
+                }
+                A[q] = row;
+                double bv = f[q];
+                if (dirichlet && m[q] == 0) {
+                    if (i + 1 < sx && m[q + ox] == 2) bv += c[0] * f[q + ox];
+                    if (i > 0 && m[q - ox] == 2) bv += c[0] * f[q - ox];
+                    if (j + 1 < sy && m[q + oy] == 2) bv += c[1] * f[q + oy];
+                    if (j > 0 && m[q - oy] == 2) bv += c[1] * f[q - oy];
+                    if (k + 1 < sz && m[q + oz] == 2) bv += c[2] * f[q + oz];
+                    if (k > 0 && m[q - oz] == 2) bv += c[2] * f[q - oz];
+                }
+                b[q] = bv;
+            }
+    free(m);
+}

@@ -218,6 +218,15 @@ size_t or_sp_build_compressed(const char* markers, const double* u, const double
                               const double* w, size_t nx, size_t ny, size_t nz, const double h[3],
                               double* val, size_t* rp, size_t* ci, double* b, size_t* nnz);
 
+/* ------------------------------------------- backward Euler diffusion */
+/* GridBackwardEulerDiffusionSolver3's buildMarkers / buildMatrix /
+ * buildVectors for one data grid of sx*sy*sz points: f the field, bsdf and
+ * fsdf the boundary / fluid SDFs at the data points, c = dt*mu/h^2 per axis,
+ * dirichlet != 0 for Jet's Dirichlet boundary type.  A: Jet rows, b. */
+void or_diffusion_build(const double* f, const double* bsdf, const double* fsdf, size_t sx,
+                        size_t sy, size_t sz, const double c[3], int dirichlet, or_row* A,
+                        double* b);
+
 #ifdef __cplusplus
 }
 #endif

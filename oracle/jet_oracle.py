@@ -237,6 +237,14 @@ class _CsrCalls:
         return x, bool(ok), it.value, res.value
 
 
+def diffusion_c(dt, mu, h):
+    """c = dt * mu / h^2 per axis, in the order
+    GridBackwardEulerDiffusionSolver3::solve evaluates it
+    (src/jet/grid_backward_euler_diffusion_solver3.cpp:33)."""
+    s = dt * mu
+    return np.array([s / (hh * hh) for hh in h], np.float64)
+
+
 class Oracle(_BoundaryCalls, _CsrCalls):
     """The C restatement (liboracle.so)."""
 
@@ -358,6 +366,16 @@ class Oracle(_BoundaryCalls, _CsrCalls):
         hh = (C.c_double * 3)(*h)
         self.lib.or_sp_build_system(_c(markers), _d(u), _d(v), _d(w), _sz(nx), _sz(ny), _sz(nz),
                                     hh, _d(A), _d(b))
+        return A, b
+
+    def diffusion_build(self, f, bsdf, fsdf, dt, mu, h, dirichlet=False):
+        """Backward Euler diffusion system (A rows, b) of one data grid."""
+        nx, ny, nz = _shape3(f)
+        A = np.empty(f.shape + (4,))
+        b = np.empty_like(f)
+        self.lib.or_diffusion_build(_d(f), _d(bsdf), _d(fsdf), _sz(nx), _sz(ny), _sz(nz),
+                                    _d(diffusion_c(dt, mu, h)), C.c_int(1 if dirichlet else 0),
+                                    _d(A), _d(b))
         return A, b
 
     def sp_build_compressed(self, markers, u, v, w, h):
@@ -596,6 +614,25 @@ class Ref(_BoundaryCalls, _CsrCalls):
 
     def fraction_inside_sdf(self, a, b):
         return self.lib.ref_fraction_inside_sdf(a, b)
+
+    def diffusion_system(self, f, bsdf, fsdf, dt, mu, h, dirichlet=False):
+        """The system Jet's GridBackwardEulerDiffusionSolver3 builds (A, b)."""
+        nx, ny, nz = _shape3(f)
+        A = np.empty(f.shape + (4,))
+        b = np.empty_like(f)
+        self.lib.ref_diffusion_system(_d(f), _d(bsdf), _d(fsdf), _sz(nx), _sz(ny), _sz(nz),
+                                      _d(np.asarray(h, np.float64)), C.c_double(dt),
+                                      C.c_double(mu), C.c_int(1 if dirichlet else 0), _d(A), _d(b))
+        return A, b
+
+    def diffusion_solve(self, f, bsdf, fsdf, dt, mu, h, dirichlet=False):
+        """Jet's own diffusion solve (default ICCG) -> the new field."""
+        nx, ny, nz = _shape3(f)
+        out = np.empty_like(f)
+        self.lib.ref_diffusion_solve(_d(f), _d(bsdf), _d(fsdf), _sz(nx), _sz(ny), _sz(nz),
+                                     _d(np.asarray(h, np.float64)), C.c_double(dt), C.c_double(mu),
+                                     C.c_int(1 if dirichlet else 0), _d(out))
+        return out
 
     def fraction_inside(self, bl, br, tl, tr):
         return self.lib.ref_fraction_inside(bl, br, tl, tr)

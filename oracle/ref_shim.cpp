@@ -10,7 +10,9 @@
 #include <jet/array_utils.h>
 #include <jet/cell_centered_scalar_grid3.h>
 #include <jet/face_centered_grid3.h>
+#include <jet/custom_scalar_field3.h>
 #include <jet/fdm_cg_solver3.h>
+#include <jet/grid_backward_euler_diffusion_solver3.h>
 #include <jet/rigid_body_collider3.h>
 #include <jet/sphere3.h>
 #include <physics_helpers.h>
@@ -299,6 +301,86 @@ int ref_csr_cg_solve(const double* val, const size_t* rp, const size_t* ci, size
     *iters = solver.lastNumberOfIterations();
     *residual = solver.lastResidual();
     return ok ? 1 : 0;
+}
+
+static void fillCells(CellCenteredScalarGrid3* g, const double* d);
+
+// ------------------------------------------------ backward Euler diffusion
+// GridBackwardEulerDiffusionSolver3 on a cell-centred grid whose data is f,
+// with the SDFs given per data point (a CustomScalarField3 that looks the
+// point up), and a linear solver that records the system it is handed.
+
+namespace {
+class CaptureSolver final : public FdmLinearSystemSolver3 {
+ public:
+    Row* A = nullptr;
+    double* b = nullptr;
+    bool solve(FdmLinearSystem3* s) override {
+        const Size3 n = s->A.size();
+        for (size_t k = 0; k < n.z; ++k)
+            for (size_t j = 0; j < n.y; ++j)
+                for (size_t i = 0; i < n.x; ++i) {
+                    const size_t q = i + n.x * (j + n.y * k);
+                    A[q] = {s->A(i, j, k).center, s->A(i, j, k).right, s->A(i, j, k).up,
+                            s->A(i, j, k).front};
+                    b[q] = s->b(i, j, k);
+                }
+        return true;
+    }
+};
+}  // namespace
+
+void ref_diffusion_system(const double* f, const double* bsdf, const double* fsdf, size_t sx,
+                          size_t sy, size_t sz, const double* h, double dt, double mu,
+                          int dirichlet, double* A, double* b) {
+    CellCenteredScalarGrid3 src(Size3(sx, sy, sz), Vector3D(h[0], h[1], h[2]), Vector3D());
+    fillCells(&src, f);
+    CellCenteredScalarGrid3 dst(src);
+    const Vector3D o = src.dataOrigin();
+    auto lookup = [&](const double* a) {
+        return [=](const Vector3D& p) {
+            const size_t i = (size_t)std::llround((p.x - o.x) / h[0]);
+            const size_t j = (size_t)std::llround((p.y - o.y) / h[1]);
+            const size_t k = (size_t)std::llround((p.z - o.z) / h[2]);
+            return a[i + sx * (j + sy * k)];
+        };
+    };
+    CustomScalarField3 bf(lookup(bsdf)), ff(lookup(fsdf));
+    GridBackwardEulerDiffusionSolver3 solver(
+        dirichlet ? GridBackwardEulerDiffusionSolver3::Dirichlet
+                  : GridBackwardEulerDiffusionSolver3::Neumann);
+    auto cap = std::make_shared<CaptureSolver>();
+    cap->A = reinterpret_cast<Row*>(A);
+    cap->b = b;
+    solver.setLinearSystemSolver(cap);
+    solver.solve(src, mu, dt, &dst, bf, ff);
+}
+
+// Jet's own diffusion solve (its default ICCG linear solver) of the same
+// grid: dest = the solution.
+void ref_diffusion_solve(const double* f, const double* bsdf, const double* fsdf, size_t sx,
+                         size_t sy, size_t sz, const double* h, double dt, double mu,
+                         int dirichlet, double* dest) {
+    CellCenteredScalarGrid3 src(Size3(sx, sy, sz), Vector3D(h[0], h[1], h[2]), Vector3D());
+    fillCells(&src, f);
+    CellCenteredScalarGrid3 dst(src);
+    const Vector3D o = src.dataOrigin();
+    auto lookup = [&](const double* a) {
+        return [=](const Vector3D& p) {
+            const size_t i = (size_t)std::llround((p.x - o.x) / h[0]);
+            const size_t j = (size_t)std::llround((p.y - o.y) / h[1]);
+            const size_t k = (size_t)std::llround((p.z - o.z) / h[2]);
+            return a[i + sx * (j + sy * k)];
+        };
+    };
+    CustomScalarField3 bf(lookup(bsdf)), ff(lookup(fsdf));
+    GridBackwardEulerDiffusionSolver3 solver(
+        dirichlet ? GridBackwardEulerDiffusionSolver3::Dirichlet
+                  : GridBackwardEulerDiffusionSolver3::Neumann);
+    solver.solve(src, mu, dt, &dst, bf, ff);
+    dst.forEachDataPointIndex([&](size_t i, size_t j, size_t k) {
+        dest[i + sx * (j + sy * k)] = dst(i, j, k);
+    });
 }
 
 // ------------------------------------------------ boundary conditions
