@@ -1,0 +1,96 @@
+// fdm_gpu_solvers2.cpp -- FdmGpuCgSolver2 over the C ABI of jetgpu.h.
+#include "fdm_gpu_solvers2.h"
+
+#include <jet/fdm_linear_system2.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace jet {
+
+namespace {
+
+void check2(int status, const jg_solver* s, const char* what) {
+    if (status == JG_OK) return;
+    std::string msg = std::string(what) + ": " + (s ? jg_last_error(s) : "");
+    if (status == JG_ERR_INVALID_ARG) throw std::invalid_argument(msg);
+    throw std::runtime_error(msg);
+}
+
+jg_solver* lazySolver2(jg_solver** slot, int device) {
+    if (!*slot) {
+        const int st = jg_solver_create(device, slot);
+        if (st != JG_OK) {
+            *slot = nullptr;
+            throw std::runtime_error("jetgpu: no usable GPU device " + std::to_string(device) +
+                                     " (status " + std::to_string(st) + ")");
+        }
+    }
+    return *slot;
+}
+
+}  // namespace
+
+FdmGpuCgSolver2::FdmGpuCgSolver2(unsigned int maxNumberOfIterations, double tolerance,
+                                 int device)
+    : _maxNumberOfIterations(maxNumberOfIterations), _tolerance(tolerance), _device(device) {
+    _info.last_residual = kMaxD;
+}
+
+FdmGpuCgSolver2::~FdmGpuCgSolver2() { jg_solver_destroy(_solver); }
+
+bool FdmGpuCgSolver2::solve(FdmLinearSystem2* system) {
+    JET_THROW_INVALID_ARG_IF(system == nullptr);
+    const Size2 n = system->A.size();
+    JET_THROW_INVALID_ARG_IF(n.x != system->b.size().x || n.y != system->b.size().y ||
+                             n.x != system->x.size().x || n.y != system->x.size().y);
+    if (n.x * n.y == 0) {
+        _info = jg_solve_info{};
+        return true;
+    }
+    // {center, right, up} -> {center, right, up, front = 0}, x fastest
+    std::vector<double> rows(4 * n.x * n.y);
+    for (size_t j = 0; j < n.y; ++j)
+        for (size_t i = 0; i < n.x; ++i) {
+            const FdmMatrixRow2& r = system->A(i, j);
+            double* d = &rows[4 * (i + n.x * j)];
+            d[0] = r.center;
+            d[1] = r.right;
+            d[2] = r.up;
+            d[3] = 0.0;
+        }
+    jg_solver* s = lazySolver2(&_solver, _device);
+    check2(jg_cg_solve_host(s, n.x, n.y, 1, rows.data(), system->b.data(), system->x.data(),
+                            _maxNumberOfIterations, _tolerance, &_info),
+           s, "FdmGpuCgSolver2::solve");
+    return _info.converged != 0;
+}
+
+bool FdmGpuCgSolver2::solveCompressed(FdmCompressedLinearSystem2* system) {
+    JET_THROW_INVALID_ARG_IF(system == nullptr);
+    const MatrixCsrD& A = system->A;
+    const size_t rows = A.rows();
+    JET_THROW_INVALID_ARG_IF(A.cols() > rows || system->b.size() != rows);
+    system->x.resize(rows, 0.0);
+    if (rows == 0) {
+        _info = jg_solve_info{};
+        return true;
+    }
+    jg_solver* s = lazySolver2(&_solver, _device);
+    const size_t nnz = A.numberOfNonZeros();
+    check2(jg_csr_cg_solve_host(s, rows, nnz, nnz ? &*A.nonZeroBegin() : nullptr,
+                                &*A.rowPointersBegin(), nnz ? &*A.columnIndicesBegin() : nullptr,
+                                system->b.data(), system->x.data(), _maxNumberOfIterations,
+                                _tolerance, &_info),
+           s, "FdmGpuCgSolver2::solveCompressed");
+    return _info.converged != 0;
+}
+
+unsigned int FdmGpuCgSolver2::maxNumberOfIterations() const { return _maxNumberOfIterations; }
+unsigned int FdmGpuCgSolver2::lastNumberOfIterations() const { return _info.iterations; }
+double FdmGpuCgSolver2::tolerance() const { return _tolerance; }
+double FdmGpuCgSolver2::lastResidual() const { return _info.last_residual; }
+const jg_solve_info& FdmGpuCgSolver2::lastSolveInfo() const { return _info; }
+
+}  // namespace jet

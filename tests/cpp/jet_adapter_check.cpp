@@ -15,6 +15,8 @@
 // Laplacian with Neumann walls and b = +1 / -1 on the bottom / top rows,
 // whose solutions are x = (ny - 1) / 2 - j up to a constant.  Prints one line
 // per check and exits non-zero on the first failure.
+#include <jet/fdm_cg_solver2.h>
+#include <jet/fdm_linear_system2.h>
 #include <jet/fdm_cg_solver3.h>
 #include <jet/fdm_linear_system3.h>
 #include <jet/fdm_mg_linear_system3.h>
@@ -28,6 +30,7 @@
 #include <memory>
 #include <stdexcept>
 
+#include "fdm_gpu_solvers2.h"
 #include "fdm_gpu_solvers3.h"
 
 using namespace jet;
@@ -157,6 +160,14 @@ void argumentChecks() {
     expect(cg.maxNumberOfIterations() == 100 && cg.tolerance() == 1e-8,
            "FdmGpuCgSolver3 accessors return the constructor arguments");
 
+    FdmGpuCgSolver2 cg2(100, 1e-8);
+    FdmLinearSystem2 bad2;
+    bad2.A.resize(Size2(4, 4));
+    bad2.x.resize(Size2(4, 4));
+    bad2.b.resize(Size2(4, 3));
+    expect(throwsInvalidArgument([&] { cg2.solve(&bad2); }),
+           "FdmGpuCgSolver2: A/b size mismatch throws std::invalid_argument");
+
     FdmGpuMgpcgSolver3 mg(50, 4, 2, 2, 20, 2, 1e-9, 1.3, true);
     const MgParameters<FdmBlas3>& p = mg.params();
     expect(p.maxNumberOfLevels == 4 && p.numberOfRestrictionIter == 2 &&
@@ -258,6 +269,61 @@ void gpuChecks() {
             den += (ccpu.x[q] - mc) * (ccpu.x[q] - mc);
         }
         expect(std::sqrt(num / den) <= 1e-8, "compressed CG: GPU x matches Jet's x (1e-8)");
+    }
+
+    // --- 2-D CG: a Dirichlet Poisson system on 48 x 40 (-1 couplings,
+    // centre = neighbour count + 1 at the walls), Jet's FdmCgSolver2 and the
+    // GPU's, flat and compressed
+    {
+        const size_t nx2 = 48, ny2 = 40;
+        FdmLinearSystem2 c2, g2;
+        c2.A.resize(Size2(nx2, ny2));
+        c2.b.resize(Size2(nx2, ny2));
+        c2.x.resize(Size2(nx2, ny2));
+        FdmCompressedLinearSystem2 cc2;
+        for (size_t j = 0; j < ny2; ++j)
+            for (size_t i = 0; i < nx2; ++i) {
+                FdmMatrixRow2& r = c2.A(i, j);
+                r.center = 4.0;
+                r.right = i + 1 < nx2 ? -1.0 : 0.0;
+                r.up = j + 1 < ny2 ? -1.0 : 0.0;
+                c2.b(i, j) = std::sin(0.3 * (double)i) + 0.1 * (double)j;
+                const size_t me = i + nx2 * j;
+                std::vector<double> v{4.0};
+                std::vector<size_t> col{me};
+                if (i + 1 < nx2) { v.push_back(-1.0); col.push_back(me + 1); }
+                if (i > 0) { v.push_back(-1.0); col.push_back(me - 1); }
+                if (j + 1 < ny2) { v.push_back(-1.0); col.push_back(me + nx2); }
+                if (j > 0) { v.push_back(-1.0); col.push_back(me - nx2); }
+                cc2.A.addRow(v, col);
+                cc2.b.append(c2.b(i, j));
+            }
+        cc2.x.resize(cc2.b.size(), 0.0);
+        g2 = c2;
+        FdmCompressedLinearSystem2 gc2 = cc2;
+        FdmCgSolver2 j2(2000, 1e-10);
+        FdmGpuCgSolver2 gpu2(2000, 1e-10);
+        FdmLinearSystemSolver2* pj2 = &j2;
+        FdmLinearSystemSolver2* pg2 = &gpu2;
+        const bool okj = pj2->solve(&c2);
+        const unsigned itj = j2.lastNumberOfIterations();
+        const bool okg = pg2->solve(&g2);
+        const unsigned itg = gpu2.lastNumberOfIterations();
+        std::printf("     2-D cg iterations: Jet %u, GPU %u\n", itj, itg);
+        double dmax = 0.0, xmax = 0.0;
+        for (size_t j = 0; j < ny2; ++j)
+            for (size_t i = 0; i < nx2; ++i) {
+                dmax = std::max(dmax, std::abs(g2.x(i, j) - c2.x(i, j)));
+                xmax = std::max(xmax, std::abs(c2.x(i, j)));
+            }
+        expect(okj && okg && std::abs((int)itj - (int)itg) <= 2 && dmax <= 1e-8 * xmax,
+               "2-D CG: FdmGpuCgSolver2 matches Jet's FdmCgSolver2");
+        const bool okjc = pj2->solveCompressed(&cc2);
+        const bool okgc = pg2->solveCompressed(&gc2);
+        double cmax = 0.0;
+        for (size_t q = 0; q < cc2.x.size(); ++q) cmax = std::max(cmax, std::abs(gc2.x[q] - cc2.x[q]));
+        expect(okjc && okgc && cmax <= 1e-8 * xmax,
+               "2-D compressed CG: FdmGpuCgSolver2 matches Jet's FdmCgSolver2");
     }
 
     // --- MGPCG: Jet's unit-test system (4 levels from 4^3, spacing 2^l)
