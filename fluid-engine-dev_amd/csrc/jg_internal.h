@@ -145,6 +145,9 @@ struct jg_solver {
     hipEvent_t evp[2] = {nullptr, nullptr};  // stop-flag polls (two in flight)
     std::string err;
     std::vector<jg::Level> lv;
+    // Levels are one plane thick and halve in x and y only (Jet's 2-D
+    // hierarchy, FdmMgLinearSystem2): the V-cycle takes the 2-D transfers.
+    bool twod = false;
     // Finest-level Krylov vectors.
     double *b = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *dA = nullptr, *dB = nullptr,
            *q = nullptr;

@@ -451,6 +451,67 @@ __global__ __launch_bounds__(NT) void k_correct(const TX* __restrict__ coarse,
     fine[f] = v;
 }
 
+// FdmMgUtils2::restrict (src/jet/fdm_mg_linear_system2.cpp:44-84) on one
+// plane: the 3-D taps and clamps in x and y, the 16 products summed y-major
+// with each weight kernel[x] * kernel[y] formed first.  One thread per
+// coarse cell.
+template <typename TX>
+__global__ __launch_bounds__(NT) void k_restrict2(const TX* __restrict__ fine,
+                                                  TX* __restrict__ coarse, Dims cd,
+                                                  const Ctl* __restrict__ ctl) {
+#pragma clang fp contract(off)
+    if (ctl && ctl->done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= cd.nx || j >= cd.ny) return;
+    const int fnx = 2 * cd.nx;
+    const TX kern[4] = {TX(0.125), TX(0.375), TX(0.375), TX(0.125)};
+    const int ii[4] = {i > 0 ? 2 * i - 1 : 2 * i, 2 * i, 2 * i + 1, i + 1 < cd.nx ? 2 * i + 2 : 2 * i + 1};
+    const int jj[4] = {j > 0 ? 2 * j - 1 : 2 * j, 2 * j, 2 * j + 1, j + 1 < cd.ny ? 2 * j + 2 : 2 * j + 1};
+    TX sum = 0;
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+        const TX* row = fine + (idx_t)fnx * jj[y];
+#pragma unroll
+        for (int x = 0; x < 4; ++x) {
+            const TX w = kern[x] * kern[y];
+            sum += w * row[ii[x]];
+        }
+    }
+    coarse[i + (idx_t)cd.nx * j] = sum;
+}
+
+// FdmMgUtils2::correct (src/jet/fdm_mg_linear_system2.cpp:86-143) on one
+// plane: fine += bilinear (1/4, 3/4) prolongation of coarse, clamped at
+// both ends in x and y, four separate adds of (wi * wj) * coarse, y-major.
+template <typename TX>
+__global__ __launch_bounds__(NT) void k_correct2(const TX* __restrict__ coarse,
+                                                 TX* __restrict__ fine, Dims fd,
+                                                 const Ctl* __restrict__ ctl) {
+#pragma clang fp contract(off)
+    if (ctl && ctl->done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= fd.nx || j >= fd.ny) return;
+    const int cnx = fd.nx / 2;
+    const int ci = i / 2, cj = j / 2;
+    int i0, i1, j0, j1;
+    TX wi0, wi1, wj0, wj1;
+    if ((i & 1) == 0) { i0 = i > 1 ? ci - 1 : ci; i1 = ci; wi0 = TX(0.25); wi1 = TX(0.75); }
+    else { i0 = ci; i1 = i + 1 < fd.nx ? ci + 1 : ci; wi0 = TX(0.75); wi1 = TX(0.25); }
+    if ((j & 1) == 0) { j0 = j > 1 ? cj - 1 : cj; j1 = cj; wj0 = TX(0.25); wj1 = TX(0.75); }
+    else { j0 = cj; j1 = j + 1 < fd.ny ? cj + 1 : cj; wj0 = TX(0.75); wj1 = TX(0.25); }
+    const int iI[2] = {i0, i1}, jI[2] = {j0, j1};
+    const TX iW[2] = {wi0, wi1}, jW[2] = {wj0, wj1};
+    const idx_t f = i + (idx_t)fd.nx * j;
+    TX v = fine[f];
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+        for (int x = 0; x < 2; ++x) v += iW[x] * jW[y] * coarse[iI[x] + (idx_t)cnx * jI[y]];
+    fine[f] = v;
+}
+
 // ------------------------------------------------------------- finalize
 
 // Initial PCG scalars: sigma = r.z, rr = r.r, bb = b.b (for the relative

@@ -248,10 +248,94 @@ struct MgCfg {
 // between the level's x and y vectors so that the last one lands in x.
 // Returns the number of b.x / b.b partials the last level-0 sweep wrote to
 // slots 0 and 1 (0 if it wrote none).
+// The V-cycle on a 2-D hierarchy (FdmMgLinearSystem2): the same symmetric
+// cycle as vcycle() below, with FdmMgUtils2's transfers as separate
+// launches and single-sweep launches only (the rows have front = 0, so the
+// 3-D sweeps and residual give the 2-D values: the absent z terms add +0).
+template <typename T>
+int vcycle2d(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl) {
+    const size_t L = s->lv.size();
+    auto B = [&](size_t l) -> const T* { return l == 0 ? r0 : Prec<T>::b(s->lv[l]); };
+    auto X = [&](size_t l) -> T* { return l == 0 ? z0 : Prec<T>::x(s->lv[l]); };
+    const int fwd = (int)std::max(1u, (c.coarsest + 1) / 2);
+    auto sweep = [&](size_t l, const T* in, T* out, int first) {
+        Level& lv = s->lv[l];
+        const int kind = in ? kSweepPlain : kSweepZero;
+        launch_sweep<T>(s, Prec<T>::A(lv), B(l), in, out, lv.d, first, nullptr, nullptr, nullptr,
+                        c.omega, ctl, Prec<T>::zc(lv, kind), Prec<T>::sh(lv, kind));
+    };
+    // Launch m of level l's pre + post sweeps writes buffer ((M-1-m) even ? x : y),
+    // so the last lands in x; the prolonged correction is added to the
+    // buffer the pre-sweeps left (zeroed first when there are none).
+    std::vector<T*> pre_out(L, nullptr);
+    for (size_t l = 0; l < L; ++l) {
+        Level& lv = s->lv[l];
+        const Dims& d = lv.d;
+        T* Y = Prec<T>::y(lv);
+        if (l + 1 == L) {
+            const int S = 2 * fwd;
+            const T* in = nullptr;
+            for (int k = 0; k < S; ++k) {
+                T* out = ((S - 1 - k) % 2 == 0) ? X(l) : Y;
+                sweep(l, in, out, k < fwd ? 0 : 1);
+                in = out;
+            }
+            JG_TRY(check_launch(s, "coarsest 2-D"));
+            break;
+        }
+        const size_t post = l == 0 ? c.post0 : c.post;
+        const size_t M = c.pre + post;
+        T* in = nullptr;
+        for (size_t m = 0; m < c.pre; ++m) {
+            T* out = ((M - 1 - m) % 2 == 0) ? X(l) : Y;
+            sweep(l, in, out, 0);
+            in = out;
+        }
+        pre_out[l] = in;
+        const Dims& cd = s->lv[l + 1].d;
+        const Geo gc = geo_planes(cd);
+        if (in) {
+            const Geo gf = geo_stencil(d);
+            k_residual<T, T, T, false, false, true><<<gf.grid, gf.block, 0, s->stream>>>(
+                Prec<T>::A(lv), in, B(l), Prec<T>::t(lv), d, gf.zc, nullptr, ctl);
+            k_restrict2<T><<<gc.grid, gc.block, 0, s->stream>>>(Prec<T>::t(lv),
+                                                                Prec<T>::b(s->lv[l + 1]), cd, ctl);
+        } else {
+            k_restrict2<T><<<gc.grid, gc.block, 0, s->stream>>>(B(l), Prec<T>::b(s->lv[l + 1]), cd,
+                                                                ctl);
+        }
+        JG_TRY(check_launch(s, "vcycle 2-D down"));
+    }
+    for (size_t l = L - 1; l-- > 0;) {
+        Level& lv = s->lv[l];
+        const Dims& d = lv.d;
+        T* Y = Prec<T>::y(lv);
+        const size_t post = l == 0 ? c.post0 : c.post;
+        const size_t M = c.pre + post;
+        T* cur = pre_out[l];
+        if (!cur) {
+            cur = (M % 2 == 0) ? X(l) : Y;
+            k_fill<T><<<nblocks_1d(d.n()), NT, 0, s->stream>>>(cur, T(0), d.n());
+        }
+        const Geo gf = geo_planes(d);
+        k_correct2<T><<<gf.grid, gf.block, 0, s->stream>>>(X(l + 1), cur, d, ctl);
+        const T* in = cur;
+        for (size_t q = 0; q < post; ++q) {
+            const size_t m = c.pre + q;
+            T* out = ((M - 1 - m) % 2 == 0) ? X(l) : Y;
+            sweep(l, in, out, 1);
+            in = out;
+        }
+        JG_TRY(check_launch(s, "vcycle 2-D up"));
+    }
+    return JG_OK;
+}
+
 template <typename T>
 int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, int* dot_blocks) {
     const size_t L = s->lv.size();
     *dot_blocks = 0;
+    if (s->twod) return vcycle2d<T>(s, r0, z0, c, ctl);
     auto B = [&](size_t l) -> const T* { return l == 0 ? r0 : Prec<T>::b(s->lv[l]); };
     auto X = [&](size_t l) -> T* { return l == 0 ? z0 : Prec<T>::x(s->lv[l]); };
     std::vector<const T*> pre_out(L, nullptr);
@@ -419,8 +503,9 @@ int autotune_sweeps(jg_solver* s) {
         k_fill<T><<<nblocks_1d(n), NT, 0, s->stream>>>(Prec<T>::t(lv), T(1), n);
         k_fill<T><<<nblocks_1d(n), NT, 0, s->stream>>>(Prec<T>::y(lv), T(0.5), n);
         // the correction kind reads the next level's x as its coarse vector
+        // (a 2-D hierarchy corrects in a separate launch: no correction kind)
         const T* coarse = nullptr;
-        if (l + 1 < L) {
+        if (l + 1 < L && !s->twod) {
             const idx_t cn = s->lv[l + 1].d.n();
             coarse = Prec<T>::x(s->lv[l + 1]);
             k_fill<T><<<nblocks_1d(cn), NT, 0, s->stream>>>(Prec<T>::x(s->lv[l + 1]), T(0.25), cn);
@@ -481,6 +566,7 @@ int autotune_sweeps(jg_solver* s) {
         };
         for (int kind = 0; kind < kNumSweepKinds; ++kind) {
             Prec<T>::pair(lv, kind) = kPairNone;
+            if (s->twod) continue;  // the 2-D cycle launches single sweeps
             if (kind == kSweepCorrect && !coarse) continue;
             const T* xin = kind == kSweepZero ? nullptr : Prec<T>::y(lv);
             const T* cor = kind == kSweepCorrect ? coarse : nullptr;
@@ -983,8 +1069,12 @@ int jg_solver_set_levels(jg_solver* s, size_t nlev, const size_t* nx, const size
             s->err = "invalid level size";
             return JG_ERR_INVALID_ARG;
         }
-        if (l > 0 && (nx[l - 1] != 2 * nx[l] || ny[l - 1] != 2 * ny[l] || nz[l - 1] != 2 * nz[l])) {
-            s->err = "level sizes must halve exactly";
+        // a finest level one plane thick with coarser levels is a 2-D
+        // hierarchy (a 3-D one cannot halve an odd nz): every level one plane
+        const bool twod = nlev > 1 && nz[0] == 1;
+        if (l > 0 && (nx[l - 1] != 2 * nx[l] || ny[l - 1] != 2 * ny[l] ||
+                      (twod ? nz[l] != 1 : nz[l - 1] != 2 * nz[l]))) {
+            s->err = "level sizes must halve exactly (in x and y only for one-plane levels)";
             return JG_ERR_INVALID_ARG;
         }
         // one reduction partial per stencil tile of a plane at least
@@ -994,6 +1084,7 @@ int jg_solver_set_levels(jg_solver* s, size_t nlev, const size_t* nx, const size
             return JG_ERR_UNSUPPORTED;
         }
     }
+    s->twod = nlev > 1 && nz[0] == 1;
     bool same = s->lv.size() == nlev;
     for (size_t l = 0; same && l < nlev; ++l)
         same = s->lv[l].d.nx == (int)nx[l] && s->lv[l].d.ny == (int)ny[l] && s->lv[l].d.nz == (int)nz[l];
@@ -1528,6 +1619,27 @@ int jg_mg_correct(jg_solver* s, size_t fnx, size_t fny, size_t fnz, const double
     Geo g = geo_planes(fd);
     k_correct<double><<<g.grid, g.block, 0, s->stream>>>(coarse, fine, fd, nullptr);
     JG_TRY(check_launch(s, "correct"));
+    return sync(s);
+}
+
+int jg_mg_restrict2(jg_solver* s, size_t cnx, size_t cny, const double* fine, double* coarse) {
+    if (!s || !valid_dims(cnx, cny, 1) || !fine || !coarse) return JG_ERR_INVALID_ARG;
+    JG_HIP(hipSetDevice(s->device));
+    Dims cd{(int)cnx, (int)cny, 1};
+    Geo g = geo_planes(cd);
+    k_restrict2<double><<<g.grid, g.block, 0, s->stream>>>(fine, coarse, cd, nullptr);
+    JG_TRY(check_launch(s, "restrict2"));
+    return sync(s);
+}
+
+int jg_mg_correct2(jg_solver* s, size_t fnx, size_t fny, const double* coarse, double* fine) {
+    if (!s || !valid_dims(fnx, fny, 1) || !coarse || !fine || (fnx | fny) & 1)
+        return JG_ERR_INVALID_ARG;
+    JG_HIP(hipSetDevice(s->device));
+    Dims fd{(int)fnx, (int)fny, 1};
+    Geo g = geo_planes(fd);
+    k_correct2<double><<<g.grid, g.block, 0, s->stream>>>(coarse, fine, fd, nullptr);
+    JG_TRY(check_launch(s, "correct2"));
     return sync(s);
 }
 

@@ -1,7 +1,9 @@
-// fdm_gpu_solvers2.cpp -- FdmGpuCgSolver2 over the C ABI of jetgpu.h.
+// fdm_gpu_solvers2.cpp -- FdmGpuCgSolver2 and FdmGpuMgpcgSolver2 over the C ABI
+// of jetgpu.h.
 #include "fdm_gpu_solvers2.h"
 
 #include <jet/fdm_linear_system2.h>
+#include <jet/fdm_mg_linear_system2.h>
 
 #include <stdexcept>
 #include <string>
@@ -92,5 +94,86 @@ unsigned int FdmGpuCgSolver2::lastNumberOfIterations() const { return _info.iter
 double FdmGpuCgSolver2::tolerance() const { return _tolerance; }
 double FdmGpuCgSolver2::lastResidual() const { return _info.last_residual; }
 const jg_solve_info& FdmGpuCgSolver2::lastSolveInfo() const { return _info; }
+
+// --------------------------------------------------------- MGPCG
+
+FdmGpuMgpcgSolver2::FdmGpuMgpcgSolver2(unsigned int numberOfCgIter, size_t maxNumberOfLevels,
+                                       unsigned int numberOfRestrictionIter,
+                                       unsigned int numberOfCorrectionIter,
+                                       unsigned int numberOfCoarsestIter,
+                                       unsigned int numberOfFinalIter, double maxTolerance,
+                                       double sorFactor, bool useRedBlackOrdering, int device)
+    : FdmMgSolver2(maxNumberOfLevels, numberOfRestrictionIter, numberOfCorrectionIter,
+                   numberOfCoarsestIter, numberOfFinalIter, maxTolerance, sorFactor,
+                   useRedBlackOrdering),
+      _maxNumberOfIterations(numberOfCgIter),
+      _tolerance(maxTolerance),
+      _device(device) {
+    jg_gpu_options_default(&_options);
+    _info.last_residual = kMaxD;
+}
+
+FdmGpuMgpcgSolver2::~FdmGpuMgpcgSolver2() { jg_solver_destroy(_solver); }
+
+bool FdmGpuMgpcgSolver2::solve(FdmMgLinearSystem2* system) {
+    JET_THROW_INVALID_ARG_IF(system == nullptr);
+    const size_t L = system->A.levels.size();
+    JET_THROW_INVALID_ARG_IF(L == 0 || system->x.levels.empty() || system->b.levels.empty());
+    const Size2 n0 = system->A.levels.front().size();
+    JET_THROW_INVALID_ARG_IF(n0 != system->x.levels.front().size() ||
+                             n0 != system->b.levels.front().size());
+    if (n0.x * n0.y == 0) {
+        _info = jg_solve_info{};
+        return true;
+    }
+    // every level as one plane of {center, right, up, front = 0} rows
+    std::vector<size_t> nx(L), ny(L), nz(L, 1);
+    std::vector<std::vector<double>> rows(L);
+    std::vector<const double*> ptrs(L);
+    for (size_t l = 0; l < L; ++l) {
+        const FdmMatrix2& A = system->A.levels[l];
+        const Size2 n = A.size();
+        nx[l] = n.x;
+        ny[l] = n.y;
+        rows[l].resize(4 * n.x * n.y);
+        for (size_t j = 0; j < n.y; ++j)
+            for (size_t i = 0; i < n.x; ++i) {
+                const FdmMatrixRow2& r = A(i, j);
+                double* d = &rows[l][4 * (i + n.x * j)];
+                d[0] = r.center;
+                d[1] = r.right;
+                d[2] = r.up;
+                d[3] = 0.0;
+            }
+        ptrs[l] = rows[l].data();
+    }
+    // Jet's MgParameters and smoother factor, as FdmMgpcgSolver2 uses them
+    jg_mg_params p;
+    jg_mg_params_default(&p);
+    const MgParameters<FdmBlas2>& mg = params();
+    p.max_levels = mg.maxNumberOfLevels;
+    p.restrict_iter = mg.numberOfRestrictionIter;
+    p.correct_iter = mg.numberOfCorrectionIter;
+    p.coarsest_iter = mg.numberOfCoarsestIter;
+    p.final_iter = mg.numberOfFinalIter;
+    p.max_tol = _tolerance;
+    p.sor = sorFactor();
+    p.red_black = useRedBlackOrdering() ? 1 : 0;
+
+    jg_solver* s = lazySolver2(&_solver, _device);
+    check2(jg_mgpcg_solve_host(s, L, nx.data(), ny.data(), nz.data(), ptrs.data(),
+                               system->b.levels.front().data(), system->x.levels.front().data(),
+                               _maxNumberOfIterations, &p, &_options, &_info),
+           s, "FdmGpuMgpcgSolver2::solve");
+    return _info.converged != 0;
+}
+
+unsigned int FdmGpuMgpcgSolver2::maxNumberOfIterations() const { return _maxNumberOfIterations; }
+unsigned int FdmGpuMgpcgSolver2::lastNumberOfIterations() const { return _info.iterations; }
+double FdmGpuMgpcgSolver2::tolerance() const { return _tolerance; }
+double FdmGpuMgpcgSolver2::lastResidual() const { return _info.last_residual; }
+void FdmGpuMgpcgSolver2::setGpuOptions(const jg_gpu_options& options) { _options = options; }
+const jg_gpu_options& FdmGpuMgpcgSolver2::gpuOptions() const { return _options; }
+const jg_solve_info& FdmGpuMgpcgSolver2::lastSolveInfo() const { return _info; }
 
 }  // namespace jet

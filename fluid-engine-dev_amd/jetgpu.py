@@ -117,6 +117,8 @@ SIGNATURES = {
     "jg_relax_jacobi": (C.c_int, [_vp, _sz, _sz, _sz, _vp, _vp, _vp, _vp]),
     "jg_mg_restrict": (C.c_int, [_vp, _sz, _sz, _sz, _vp, _vp]),
     "jg_mg_correct": (C.c_int, [_vp, _sz, _sz, _sz, _vp, _vp]),
+    "jg_mg_restrict2": (C.c_int, [_vp, _sz, _sz, _vp, _vp]),
+    "jg_mg_correct2": (C.c_int, [_vp, _sz, _sz, _vp, _vp]),
     "jg_solver_time_kernel": (C.c_int, [_vp, C.c_int, C.c_int, _u, C.POINTER(_d), C.POINTER(_d)]),
     "jg_sp_build": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d]),
     "jg_sp_apply_gradient": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d]),
@@ -493,6 +495,16 @@ class Solver:
     def correct(self, fshape, coarse, fine):
         nx, ny, nz = fshape
         self.check(lib().jg_mg_correct(self.h, nx, ny, nz, _ptr(coarse), _ptr(fine)), "correct")
+
+    def restrict2(self, cshape, fine, coarse):
+        """FdmMgUtils2::restrict; cshape = (cnx, cny)."""
+        cnx, cny = cshape
+        self.check(lib().jg_mg_restrict2(self.h, cnx, cny, _ptr(fine), _ptr(coarse)), "restrict2")
+
+    def correct2(self, fshape, coarse, fine):
+        """FdmMgUtils2::correct; fshape = (fnx, fny)."""
+        fnx, fny = fshape
+        self.check(lib().jg_mg_correct2(self.h, fnx, fny, _ptr(coarse), _ptr(fine)), "correct2")
 
 
 class _DeviceView:

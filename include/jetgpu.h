@@ -104,7 +104,12 @@ void* jg_solver_stream(jg_solver* s);
 /* --------------------------------------------- device-resident system */
 /* Declares the level sizes and allocates device storage (cached: a call
  * with the same sizes allocates nothing).  Level l+1 must be exactly half of
- * level l in every direction. */
+ * level l in every direction -- except in a 2-D hierarchy, one whose finest
+ * level is one plane thick (nz[0] == 1) and which has coarser levels: its
+ * levels all have nz == 1 and halve in x and y (FdmMgLinearSystem2,
+ * include/jet/fdm_mg_linear_system2.h:25-60; rows are FdmMatrixRow2 with
+ * front = 0), and jg_solver_mgpcg / jg_mg_vcycle then use FdmMgUtils2's
+ * transfers (FdmMgpcgSolver2, src/jet/fdm_mgpcg_solver2.cpp:20-92). */
 int jg_solver_set_levels(jg_solver* s, size_t nlev, const size_t* nx,
                          const size_t* ny, const size_t* nz);
 /* Device pointers of the resident system: matrix rows of level l (32-byte
@@ -231,6 +236,12 @@ int jg_mg_restrict(jg_solver* s, size_t cnx, size_t cny, size_t cnz,
  * (fine += prolongation of coarse; fine size fnx*fny*fnz) */
 int jg_mg_correct(jg_solver* s, size_t fnx, size_t fny, size_t fnz,
                   const double* coarse, double* fine);
+/* FdmMgUtils2::restrict -- src/jet/fdm_mg_linear_system2.cpp:44-84
+ * (coarse size cnx*cny, fine size 2cnx * 2cny) */
+int jg_mg_restrict2(jg_solver* s, size_t cnx, size_t cny, const double* fine, double* coarse);
+/* FdmMgUtils2::correct -- src/jet/fdm_mg_linear_system2.cpp:86-143
+ * (fine += prolongation of coarse; fine size fnx*fny, both even) */
+int jg_mg_correct2(jg_solver* s, size_t fnx, size_t fny, const double* coarse, double* fine);
 
 /* ------------------------------------------------- kernel measurement */
 /* Times `reps` back-to-back launches of one level-0 kernel of the solve on

@@ -360,6 +360,114 @@ void or_mg_correct(const double* coarser, double* finer, size_t fnx,
             }
 }
 
+/* FdmMgUtils2::resizeArrayWithFinest -- include/jet/detail/
+ * fdm_mg_linear_system2-inl.h:34-48: halve while nx and ny are even. */
+size_t or_levels_with_finest2(size_t nx, size_t ny, size_t max_levels,
+                              size_t* lnx, size_t* lny) {
+    size_t rx = nx, ry = ny;
+    size_t i = 1;
+    for (; i < max_levels; ++i) {
+        if (rx % 2 == 0 && ry % 2 == 0) {
+            rx >>= 1;
+            ry >>= 1;
+        } else {
+            break;
+        }
+    }
+    size_t nlev = i > OR_MAX_LEVELS ? OR_MAX_LEVELS : i;
+    for (size_t l = 0; l < nlev; ++l) {
+        lnx[nlev - 1 - l] = rx << l;
+        lny[nlev - 1 - l] = ry << l;
+    }
+    return nlev;
+}
+
+/* FdmMgUtils2::restrict -- src/jet/fdm_mg_linear_system2.cpp:44-84: the
+ * same 1-D weights and clamps as in 3-D; the 16 taps are summed y-major,
+ * each weight kernel[x] * kernel[y] formed first. */
+void or_mg_restrict2(const double* finer, double* coarser, size_t cnx, size_t cny) {
+    static const double kernel[4] = {0.125, 0.375, 0.375, 0.125};
+    const size_t fnx = 2 * cnx;
+    for (size_t j = 0; j < cny; ++j) {
+        const size_t jI[4] = {(j > 0) ? 2 * j - 1 : 2 * j, 2 * j, 2 * j + 1,
+                              (j + 1 < cny) ? 2 * j + 2 : 2 * j + 1};
+        for (size_t i = 0; i < cnx; ++i) {
+            const size_t iI[4] = {(i > 0) ? 2 * i - 1 : 2 * i, 2 * i, 2 * i + 1,
+                                  (i + 1 < cnx) ? 2 * i + 2 : 2 * i + 1};
+            double sum = 0.0;
+            for (int y = 0; y < 4; ++y)
+                for (int x = 0; x < 4; ++x) {
+                    const double w = kernel[x] * kernel[y];
+                    sum += w * finer[iI[x] + fnx * jI[y]];
+                }
+            coarser[i + cnx * j] = sum;
+        }
+    }
+}
+
+/* FdmMgUtils2::correct -- src/jet/fdm_mg_linear_system2.cpp:86-143: bilinear
+ * (1/4, 3/4) prolongation added to the finer grid, four separate adds of
+ * (wi * wj) * coarse, y-major. */
+void or_mg_correct2(const double* coarser, double* finer, size_t fnx, size_t fny) {
+    const size_t cnx = fnx / 2;
+    for (size_t j = 0; j < fny; ++j)
+        for (size_t i = 0; i < fnx; ++i) {
+            size_t iI[2], jI[2];
+            double iW[2], jW[2];
+            const size_t ci = i / 2, cj = j / 2;
+            if (i % 2 == 0) {
+                iI[0] = (i > 1) ? ci - 1 : ci;
+                iI[1] = ci;
+                iW[0] = 0.25;
+                iW[1] = 0.75;
+            } else {
+                iI[0] = ci;
+                iI[1] = (i + 1 < fnx) ? ci + 1 : ci;
+                iW[0] = 0.75;
+                iW[1] = 0.25;
+            }
+            if (j % 2 == 0) {
+                jI[0] = (j > 1) ? cj - 1 : cj;
+                jI[1] = cj;
+                jW[0] = 0.25;
+                jW[1] = 0.75;
+            } else {
+                jI[0] = cj;
+                jI[1] = (j + 1 < fny) ? cj + 1 : cj;
+                jW[0] = 0.75;
+                jW[1] = 0.25;
+            }
+            for (int y = 0; y < 2; ++y)
+                for (int x = 0; x < 2; ++x) {
+                    const double w = iW[x] * jW[y] * coarser[iI[x] + cnx * jI[y]];
+                    finer[i + fnx * j] += w;
+                }
+        }
+}
+
+/* A hierarchy whose finest level is one plane thick and which has coarser
+ * levels is a 2-D one (FdmMgLinearSystem2, levels halving in x and y): a 3-D
+ * hierarchy cannot halve an odd nz.  Its rows are 2-D rows with front = 0,
+ * so the 3-D relax, residual and products give the 2-D values (the absent z
+ * terms add +0); only the transfers differ. */
+int or_mg_is_2d(const or_mg_system* s) { return s->nlev > 1 && s->nz[0] == 1; }
+
+static void mg_restrict_level(const or_mg_system* s, size_t l, const double* fine,
+                              double* coarse) {
+    if (or_mg_is_2d(s))
+        or_mg_restrict2(fine, coarse, s->nx[l + 1], s->ny[l + 1]);
+    else
+        or_mg_restrict(fine, coarse, s->nx[l + 1], s->ny[l + 1], s->nz[l + 1]);
+}
+
+static void mg_correct_level(const or_mg_system* s, size_t l, const double* coarse,
+                             double* fine) {
+    if (or_mg_is_2d(s))
+        or_mg_correct2(coarse, fine, s->nx[l], s->ny[l]);
+    else
+        or_mg_correct(coarse, fine, s->nx[l], s->ny[l], s->nz[l]);
+}
+
 static void mg_relax(const or_row* A, const double* b, unsigned iters,
                      double* x, size_t nx, size_t ny, size_t nz,
                      const or_mg_params* p) {
@@ -379,11 +487,10 @@ static double vcycle_level(const or_mg_system* s, const or_mg_params* p,
     mg_relax(s->A[l], b[l], p->restrict_iter, x[l], nx, ny, nz, p);
     if (l < s->nlev - 1) {
         or_residual(s->A[l], x[l], b[l], buf[l], nx, ny, nz);
-        or_mg_restrict(buf[l], b[l + 1], s->nx[l + 1], s->ny[l + 1],
-                       s->nz[l + 1]);
+        mg_restrict_level(s, l, buf[l], b[l + 1]);
         or_set(0.0, x[l + 1], s->nx[l + 1] * s->ny[l + 1] * s->nz[l + 1]);
         vcycle_level(s, p, l + 1, x, b, buf);
-        or_mg_correct(x[l + 1], x[l], nx, ny, nz);
+        mg_correct_level(s, l, x[l + 1], x[l]);
         mg_relax(s->A[l], b[l], l > 0 ? p->correct_iter : p->final_iter, x[l],
                  nx, ny, nz, p);
     } else {
@@ -506,10 +613,9 @@ static void clean_vcycle(const or_mg_system* s, const or_mg_params* p,
         for (unsigned it = 0; it < p->restrict_iter; ++it)
             rb_sweep_ordered(s->A[l], b[l], p->sor, x[l], nx, ny, nz, 0);
         or_residual(s->A[l], x[l], b[l], buf[l], nx, ny, nz);
-        or_mg_restrict(buf[l], b[l + 1], s->nx[l + 1], s->ny[l + 1],
-                       s->nz[l + 1]);
+        mg_restrict_level(s, l, buf[l], b[l + 1]);
         clean_vcycle(s, p, l + 1, x, b, buf);
-        or_mg_correct(x[l + 1], x[l], nx, ny, nz);
+        mg_correct_level(s, l, x[l + 1], x[l]);
         unsigned post = l > 0 ? p->correct_iter : p->final_iter;
         for (unsigned it = 0; it < post; ++it)
             rb_sweep_ordered(s->A[l], b[l], p->sor, x[l], nx, ny, nz, 1);

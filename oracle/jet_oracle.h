@@ -101,6 +101,13 @@ void or_mg_restrict(const double* finer, double* coarser, size_t cnx,
                     size_t cny, size_t cnz);
 void or_mg_correct(const double* coarser, double* finer, size_t fnx,
                    size_t fny, size_t fnz);
+/* 2-D transfers and level sizing (FdmMgUtils2); a hierarchy with nz[0] == 1
+ * and more than one level is a 2-D one and takes them in every V-cycle. */
+size_t or_levels_with_finest2(size_t nx, size_t ny, size_t max_levels,
+                              size_t* lnx, size_t* lny);
+void or_mg_restrict2(const double* finer, double* coarser, size_t cnx, size_t cny);
+void or_mg_correct2(const double* coarser, double* finer, size_t fnx, size_t fny);
+int or_mg_is_2d(const or_mg_system* s);
 double or_mg_vcycle(const or_mg_system* sys, const or_mg_params* p,
                     double** x, double** b, double** buffer);
 int or_mg_solve(or_mg_system* sys, const or_mg_params* p, double* last_norm);

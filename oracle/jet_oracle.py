@@ -120,6 +120,7 @@ def _setup(lib):
                                          C.c_double, C.POINTER(C.c_uint), _dp]
     lib.or_mg_clean_vcycle.argtypes = [C.POINTER(MgSystemC), C.POINTER(MgParams), _dp, _dp]
     lib.or_levels_with_finest.restype = C.c_size_t
+    lib.or_levels_with_finest2.restype = C.c_size_t
     lib.or_divergence.restype = C.c_double
     return lib
 
@@ -314,6 +315,28 @@ class Oracle(_BoundaryCalls, _CsrCalls):
     def correct(self, coarse, fine):
         fine = fine.copy()
         self.lib.or_mg_correct(_d(coarse), _d(fine), *map(_sz, _shape3(fine)))
+        return fine
+
+    def levels_with_finest2(self, shape_xy, max_levels):
+        """FdmMgUtils2::resizeArrayWithFinest: [(nx, ny), ...] finest first."""
+        lx, ly = (C.c_size_t * MAX_LEVELS)(), (C.c_size_t * MAX_LEVELS)()
+        n = self.lib.or_levels_with_finest2(*map(_sz, shape_xy), _sz(max_levels), lx, ly)
+        return [(lx[l], ly[l]) for l in range(n)]
+
+    def restrict2(self, fine):
+        """FdmMgUtils2::restrict of an (ny, nx) array."""
+        ny, nx = fine.shape
+        fine = np.ascontiguousarray(fine, dtype=np.float64)
+        out = np.empty((ny // 2, nx // 2))
+        self.lib.or_mg_restrict2(_d(fine), _d(out), _sz(nx // 2), _sz(ny // 2))
+        return out
+
+    def correct2(self, coarse, fine):
+        """FdmMgUtils2::correct: fine + prolongation of coarse, (ny, nx) arrays."""
+        fine = np.array(fine, dtype=np.float64)
+        coarse = np.ascontiguousarray(coarse, dtype=np.float64)
+        ny, nx = fine.shape
+        self.lib.or_mg_correct2(_d(coarse), _d(fine), _sz(nx), _sz(ny))
         return fine
 
     def mg_solve(self, sys: MgSystem, params: MgParams):
@@ -527,6 +550,7 @@ class Ref(_BoundaryCalls, _CsrCalls):
         lib.ref_fraction_inside_sdf.restype = C.c_double
         lib.ref_fraction_inside_sdf.argtypes = [C.c_double] * 2
         lib.ref_levels_with_finest.restype = C.c_size_t
+        lib.ref_levels_with_finest2.restype = C.c_size_t
         lib.ref_quiet()
         self.lib = lib
 
@@ -610,6 +634,36 @@ class Ref(_BoundaryCalls, _CsrCalls):
                                       C.c_uint(p.coarsest_iter), C.c_uint(p.final_iter),
                                       C.c_double(p.max_tol), C.c_double(p.sor),
                                       C.c_int(p.red_black), C.byref(it), C.byref(res))
+        return bool(ok), it.value, res.value
+
+    def levels_with_finest2(self, shape_xy, max_levels):
+        lx, ly = (C.c_size_t * MAX_LEVELS)(), (C.c_size_t * MAX_LEVELS)()
+        n = self.lib.ref_levels_with_finest2(*map(_sz, shape_xy), _sz(max_levels), lx, ly)
+        return [(lx[l], ly[l]) for l in range(n)]
+
+    def restrict2(self, fine):
+        ny, nx = fine.shape
+        fine = np.ascontiguousarray(fine, dtype=np.float64)
+        out = np.empty((ny // 2, nx // 2))
+        self.lib.ref_mg_restrict2(_d(fine), _d(out), _sz(nx // 2), _sz(ny // 2))
+        return out
+
+    def correct2(self, coarse, fine):
+        fine = np.array(fine, dtype=np.float64)
+        coarse = np.ascontiguousarray(coarse, dtype=np.float64)
+        ny, nx = fine.shape
+        self.lib.ref_mg_correct2(_d(coarse), _d(fine), _sz(nx), _sz(ny))
+        return fine
+
+    def mgpcg_solve2(self, sys: MgSystem, p: MgParams, max_iter):
+        """FdmMgpcgSolver2::solve on a hierarchy of one-plane levels."""
+        n, nx, ny, _, A, X, B = self._mg_args(sys)
+        it, res = C.c_uint(), C.c_double()
+        ok = self.lib.ref_mgpcg_solve2(n, nx, ny, A, X, B, C.c_uint(max_iter), _sz(p.max_levels),
+                                       C.c_uint(p.restrict_iter), C.c_uint(p.correct_iter),
+                                       C.c_uint(p.coarsest_iter), C.c_uint(p.final_iter),
+                                       C.c_double(p.max_tol), C.c_double(p.sor),
+                                       C.c_int(p.red_black), C.byref(it), C.byref(res))
         return bool(ok), it.value, res.value
 
     def fraction_inside_sdf(self, a, b):

@@ -22,6 +22,7 @@
 #include <jet/fdm_mg_linear_system3.h>
 #include <jet/fdm_mg_solver3.h>
 #include <jet/fdm_mgpcg_solver3.h>
+#include <jet/fdm_mgpcg_solver2.h>
 #include <jet/level_set_utils.h>
 #include <jet/logging.h>
 
@@ -235,6 +236,74 @@ int ref_mgpcg_solve(size_t nlev, const size_t* nx, const size_t* ny,
                            sor, red_black != 0);
     bool ok = solver.solve(&s);
     fromVector(s.x.levels[0], x[0]);
+    *iters = solver.lastNumberOfIterations();
+    *residual = solver.lastResidual();
+    return ok ? 1 : 0;
+}
+
+// ------------------------------------------------- 2-D multigrid
+// FdmMgUtils2 and FdmMgpcgSolver2 on (ny, nx) arrays; rows arrive as the
+// 4-coefficient rows of a one-plane system and their front is dropped.
+
+static void toVector2(const double* src, size_t nx, size_t ny, FdmVector2* v) {
+    v->resize(Size2(nx, ny));
+    v->forEachIndex([&](size_t i, size_t j) { (*v)(i, j) = src[i + nx * j]; });
+}
+
+static void fromVector2(const FdmVector2& v, double* dst) {
+    const size_t nx = v.size().x;
+    v.forEachIndex([&](size_t i, size_t j) { dst[i + nx * j] = v(i, j); });
+}
+
+void ref_mg_restrict2(const double* finer, double* coarser, size_t cnx, size_t cny) {
+    FdmVector2 f, c;
+    toVector2(finer, 2 * cnx, 2 * cny, &f);
+    c.resize(Size2(cnx, cny));
+    FdmMgUtils2::restrict(f, &c);
+    fromVector2(c, coarser);
+}
+
+void ref_mg_correct2(const double* coarser, double* finer, size_t fnx, size_t fny) {
+    FdmVector2 f, c;
+    toVector2(coarser, fnx / 2, fny / 2, &c);
+    toVector2(finer, fnx, fny, &f);
+    FdmMgUtils2::correct(c, &f);
+    fromVector2(f, finer);
+}
+
+size_t ref_levels_with_finest2(size_t nx, size_t ny, size_t max_levels, size_t* lnx,
+                               size_t* lny) {
+    std::vector<Array2<double>> levels;
+    FdmMgUtils2::resizeArrayWithFinest(Size2(nx, ny), max_levels, &levels);
+    for (size_t l = 0; l < levels.size(); ++l) {
+        lnx[l] = levels[l].size().x;
+        lny[l] = levels[l].size().y;
+    }
+    return levels.size();
+}
+
+int ref_mgpcg_solve2(size_t nlev, const size_t* nx, const size_t* ny, const Row* const* A,
+                     double* const* x, const double* const* b, unsigned max_cg_iter,
+                     size_t max_levels, unsigned restrict_iter, unsigned correct_iter,
+                     unsigned coarsest_iter, unsigned final_iter, double max_tol, double sor,
+                     int red_black, unsigned* iters, double* residual) {
+    FdmMgLinearSystem2 s;
+    s.resizeWithCoarsest(Size2(nx[nlev - 1], ny[nlev - 1]), nlev);
+    for (size_t l = 0; l < nlev; ++l) {
+        FdmMatrix2& m = s.A.levels[l];
+        m.forEachIndex([&](size_t i, size_t j) {
+            const Row& r = A[l][i + nx[l] * j];
+            m(i, j).center = r.center;
+            m(i, j).right = r.right;
+            m(i, j).up = r.up;
+        });
+        toVector2(x[l], nx[l], ny[l], &s.x.levels[l]);
+        toVector2(b[l], nx[l], ny[l], &s.b.levels[l]);
+    }
+    FdmMgpcgSolver2 solver(max_cg_iter, max_levels, restrict_iter, correct_iter, coarsest_iter,
+                           final_iter, max_tol, sor, red_black != 0);
+    bool ok = solver.solve(&s);
+    fromVector2(s.x.levels[0], x[0]);
     *iters = solver.lastNumberOfIterations();
     *residual = solver.lastResidual();
     return ok ? 1 : 0;

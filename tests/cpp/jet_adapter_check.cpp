@@ -17,6 +17,8 @@
 // per check and exits non-zero on the first failure.
 #include <jet/fdm_cg_solver2.h>
 #include <jet/fdm_linear_system2.h>
+#include <jet/fdm_mg_linear_system2.h>
+#include <jet/fdm_mgpcg_solver2.h>
 #include <jet/fdm_cg_solver3.h>
 #include <jet/fdm_linear_system3.h>
 #include <jet/fdm_mg_linear_system3.h>
@@ -167,6 +169,10 @@ void argumentChecks() {
     bad2.b.resize(Size2(4, 3));
     expect(throwsInvalidArgument([&] { cg2.solve(&bad2); }),
            "FdmGpuCgSolver2: A/b size mismatch throws std::invalid_argument");
+
+    FdmGpuMgpcgSolver2 mg2(50, 4);
+    expect(throwsInvalidArgument([&] { mg2.solve(nullptr); }),
+           "FdmGpuMgpcgSolver2: null system throws std::invalid_argument");
 
     FdmGpuMgpcgSolver3 mg(50, 4, 2, 2, 20, 2, 1e-9, 1.3, true);
     const MgParameters<FdmBlas3>& p = mg.params();
@@ -324,6 +330,83 @@ void gpuChecks() {
         for (size_t q = 0; q < cc2.x.size(); ++q) cmax = std::max(cmax, std::abs(gc2.x[q] - cc2.x[q]));
         expect(okjc && okgc && cmax <= 1e-8 * xmax,
                "2-D compressed CG: FdmGpuCgSolver2 matches Jet's FdmCgSolver2");
+    }
+
+    // --- 2-D MGPCG: Jet's FdmMgpcgSolver2.Solve system (6 levels from 4 x 4,
+    // src/tests/unit_tests/fdm_mgpcg_solver2_tests.cpp:13-49) through Jet's
+    // solver and the GPU's, then tight solves against Jet's CG on the finest
+    // level (mean-free: the system is all-Neumann)
+    for (int mixed = 0; mixed <= 1; ++mixed) {
+        auto build = [](FdmMgLinearSystem2* sys) {
+            sys->resizeWithCoarsest(Size2(4, 4), 6);
+            for (size_t l = 0; l < sys->numberOfLevels(); ++l) {
+                const double invdx = std::pow(0.5, (double)l);
+                FdmMatrix2& A = sys->A[l];
+                FdmVector2& b = sys->b[l];
+                A.set(FdmMatrixRow2());
+                b.set(0.0);
+                sys->x[l].set(0.0);
+                A.forEachIndex([&](size_t i, size_t j) {
+                    if (i > 0) A(i, j).center += invdx * invdx;
+                    if (i + 1 < A.width()) {
+                        A(i, j).center += invdx * invdx;
+                        A(i, j).right -= invdx * invdx;
+                    }
+                    if (j > 0) A(i, j).center += invdx * invdx;
+                    else b(i, j) += invdx;
+                    if (j + 1 < A.height()) {
+                        A(i, j).center += invdx * invdx;
+                        A(i, j).up -= invdx * invdx;
+                    } else {
+                        b(i, j) -= invdx;
+                    }
+                });
+            }
+        };
+        FdmMgLinearSystem2 js, gs;
+        build(&js);
+        build(&gs);
+        FdmMgpcgSolver2 jet2(200, 6, 5, 5, 10, 10, 1e-4);
+        FdmGpuMgpcgSolver2 gpu2(200, 6, 5, 5, 10, 10, 1e-4);
+        jg_gpu_options o = gpu2.gpuOptions();
+        o.precision = mixed ? JG_MIXED : JG_FP64;
+        gpu2.setGpuOptions(o);
+        FdmMgSolver2* pj = &jet2;
+        FdmMgSolver2* pg = &gpu2;
+        const bool okj = pj->solve(&js);
+        const bool okg = pg->solve(&gs);
+        std::printf("     2-D mgpcg (%s) iterations: Jet %u, GPU %u\n", mixed ? "mixed" : "fp64",
+                    jet2.lastNumberOfIterations(), gpu2.lastNumberOfIterations());
+        expect(okj && okg, mixed ? "2-D MGPCG mixed: Jet's FdmMgpcgSolver2.Solve test passes"
+                                 : "2-D MGPCG fp64: Jet's FdmMgpcgSolver2.Solve test passes");
+
+        FdmLinearSystem2 fine;
+        fine.A = js.A[0];
+        fine.b = js.b[0];
+        fine.x.resize(js.A[0].size());
+        FdmCgSolver2 cg(20000, 1e-11);
+        cg.solve(&fine);
+        FdmGpuMgpcgSolver2 tight(200, 6, 2, 2, 20, 2, 1e-11, 1.3, true);
+        tight.setGpuOptions(o);
+        tight.solve(&gs);
+        const Size2 n = fine.x.size();
+        const size_t N = n.x * n.y;
+        double ma = 0.0, mb = 0.0;
+        for (size_t c = 0; c < N; ++c) {
+            ma += gs.x[0].data()[c];
+            mb += fine.x.data()[c];
+        }
+        ma /= (double)N;
+        mb /= (double)N;
+        double num = 0.0, den = 0.0;
+        for (size_t c = 0; c < N; ++c) {
+            const double d = (gs.x[0].data()[c] - ma) - (fine.x.data()[c] - mb);
+            num += d * d;
+            den += (fine.x.data()[c] - mb) * (fine.x.data()[c] - mb);
+        }
+        expect(std::sqrt(num / den) <= 1e-8,
+               mixed ? "2-D MGPCG mixed on 128x128: matches Jet's CG (1e-8)"
+                     : "2-D MGPCG fp64 on 128x128: matches Jet's CG (1e-8)");
     }
 
     // --- MGPCG: Jet's unit-test system (4 levels from 4^3, spacing 2^l)

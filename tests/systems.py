@@ -164,3 +164,21 @@ def csr_from_stencil(A, mask):
                     val.append(row[col])
                 rp.append(len(ci))
     return np.array(val), np.array(rp, np.uint64), np.array(ci, np.uint64)
+
+
+def mg_poisson_hierarchy2(coarsest, levels):
+    """FdmMgpcgSolver2.Solve test system (src/tests/unit_tests/
+    fdm_mgpcg_solver2_tests.cpp:13-49): resizeWithCoarsest({c, c}, levels), A
+    scaled by invdx^2 and b by invdx with invdx = 0.5^l, as one-plane levels
+    ((1, ny, nx, 4) rows with front = 0)."""
+    from oracle.jet_oracle import MgSystem
+
+    sys = MgSystem()
+    for l in range(levels):
+        n = coarsest << (levels - 1 - l)
+        invdx = 0.5 ** l
+        A, b = poisson_helper(n, n, 1, scale=invdx * invdx, b_scale=invdx)
+        sys.A.append(A)
+        sys.b.append(b)
+        sys.x.append(np.zeros_like(b))
+    return sys

@@ -1,0 +1,143 @@
+"""GPU 2-D multigrid (FdmMgLinearSystem2 / FdmMgpcgSolver2 through the same
+solver: a hierarchy of one-plane levels halving in x and y) against the
+oracle: FdmMgUtils2's transfers bit for bit, the V-cycle against the
+oracle's clean-mode V-cycle on the same 2-D hierarchy, and MGPCG solutions
+against the oracle's FP64 solve."""
+import numpy as np
+import pytest
+
+from jgtest import dev, empty, host, jetgpu, rel_err, torch_cuda
+from systems import mg_poisson_hierarchy2, random_stencil
+
+pytestmark = pytest.mark.gpu
+
+HIER2 = [((64, 64), 4), ((96, 32), 3), ((256, 128), 5), ((30, 18), 2), ((1024, 512), 6)]
+SCHEDULES = [(2, 2, 20, 2), (1, 1, 4, 1), (0, 2, 6, 3), (2, 0, 6, 0)]
+
+
+@pytest.fixture(scope="module")
+def solver():
+    if torch_cuda() is None:
+        pytest.skip("no GPU")
+    s = jetgpu.Solver(0)
+    yield s
+    s.close()
+
+
+def hierarchy2(shape, levels, seed):
+    from oracle.jet_oracle import MgSystem
+
+    rng = np.random.default_rng(seed)
+    s = MgSystem()
+    nx, ny = shape
+    for l in range(levels):
+        A = random_stencil(nx >> l, ny >> l, 1, rng)
+        s.A.append(A)
+        s.b.append(np.zeros(A.shape[:3]))
+        s.x.append(np.zeros(A.shape[:3]))
+    return s
+
+
+def upload(solver, s, b=None):
+    solver.set_levels([(x.shape[2], x.shape[1], 1) for x in s.x])
+    for l, A in enumerate(s.A):
+        solver.upload(l, A, b if l == 0 else None)
+
+
+def params(sched, sor=1.3):
+    from oracle.jet_oracle import MgParams
+
+    pre, post, coarsest, final = sched
+    kw = dict(restrict_iter=pre, correct_iter=post, coarsest_iter=coarsest, final_iter=final, sor=sor)
+    return jetgpu.mg_params(**kw), MgParams.make(**kw)
+
+
+@pytest.mark.parametrize("cshape", [(1, 1), (2, 3), (16, 8), (3, 2), (65, 33)])
+def test_restrict2_correct2_bitwise(solver, oracle, cshape):
+    print(f"case transfers2 cshape={cshape}", flush=True)
+    cnx, cny = cshape
+    rng = np.random.default_rng(51)
+    f = rng.uniform(-1, 1, (2 * cny, 2 * cnx))
+    c = rng.uniform(-1, 1, (cny, cnx))
+    out = empty((cny, cnx))
+    solver.restrict2(cshape, dev(f), out)
+    assert np.array_equal(host(out), oracle.restrict2(f))
+    fd = dev(f)
+    solver.correct2((2 * cnx, 2 * cny), dev(c), fd)
+    assert np.array_equal(host(fd), oracle.correct2(c, f))
+
+
+@pytest.mark.parametrize("shape,levels", HIER2)
+@pytest.mark.parametrize("sched", SCHEDULES)
+def test_vcycle2_matches_oracle(solver, oracle, shape, levels, sched):
+    print(f"case vcycle2 shape={shape} levels={levels} sched={sched}", flush=True)
+    s = hierarchy2(shape, levels, seed=52)
+    upload(solver, s)
+    gp, op = params(sched)
+    r = np.random.default_rng(53).uniform(-1, 1, s.x[0].shape)
+    want = oracle.mg_clean_vcycle(s, op, r)
+    for prec, tol in ((jetgpu.JG_FP64, 1e-12), (jetgpu.JG_MIXED, 2e-5)):
+        z = empty(r.shape)
+        solver.mg_vcycle(gp, jetgpu.gpu_options(precision=prec), dev(r), z)
+        assert rel_err(host(z), want) <= tol, prec
+
+
+@pytest.mark.parametrize("precision,tol", [(jetgpu.JG_FP64, 1e-12), (jetgpu.JG_MIXED, 1e-5)])
+def test_vcycle2_symmetric_positive_linear(solver, precision, tol):
+    print(f"case vcycle2 spd precision={precision}", flush=True)
+    s = hierarchy2((128, 64), 4, seed=54)
+    upload(solver, s)
+    gp, _ = params((2, 2, 20, 2))
+    rng = np.random.default_rng(55)
+    r1, r2 = rng.uniform(-1, 1, (2,) + s.x[0].shape)
+
+    def apply(r):
+        z = empty(r.shape)
+        solver.mg_vcycle(gp, jetgpu.gpu_options(precision=precision), dev(r), z)
+        return host(z)
+
+    z1, z2 = apply(r1), apply(r2)
+    scale = np.linalg.norm(r1) * np.linalg.norm(z2)
+    assert abs(np.vdot(r1, z2) - np.vdot(r2, z1)) <= tol * scale
+    assert np.vdot(r1, z1) > 0 and np.vdot(r2, z2) > 0
+    assert rel_err(apply(2.0 * r1 + r2), 2.0 * z1 + z2) <= 10 * tol
+    assert np.array_equal(apply(r1), z1)
+
+
+@pytest.mark.parametrize("shape,levels", [((128, 128), 6), ((1024, 1024), 8), ((320, 96), 4)])
+@pytest.mark.parametrize("precision", [jetgpu.JG_FP64, jetgpu.JG_MIXED])
+def test_mgpcg2_matches_oracle_solution(solver, oracle, shape, levels, precision):
+    # Random SPD one-plane hierarchy, solved to 1e-10 relative residual: the
+    # solution agrees with the oracle's FP64 clean-mode solve to 1e-10 and
+    # the iteration counts are close (the same preconditioner).
+    print(f"case mgpcg2 shape={shape} levels={levels} precision={precision}", flush=True)
+    s = hierarchy2(shape, levels, seed=56)
+    b = np.random.default_rng(57).uniform(-1, 1, s.x[0].shape)
+    s.b[0] = b
+    upload(solver, s, b)
+    gp, op = params((2, 2, 20, 2), sor=1.0)
+    info = solver.mgpcg(100, gp, jetgpu.gpu_options(precision=precision,
+                                                    stop_rule=jetgpu.JG_STOP_RELATIVE,
+                                                    rel_tol=1e-10))
+    assert info.rel_residual <= 1e-10
+    ok, it, rel = oracle.mgpcg_clean_solve(s, op, 100, 1e-12)
+    assert ok
+    x = solver.download_x()
+    assert rel_err(x, s.x[0]) <= 1e-8
+    if precision == jetgpu.JG_FP64:
+        assert info.iterations <= it
+
+
+def test_mgpcg2_jet_kat(solver):
+    # FdmMgpcgSolver2.Solve (src/tests/unit_tests/fdm_mgpcg_solver2_tests.cpp:
+    # 13-49) on the GPU: 6 levels from 4 x 4, MGPCG(200, 6, 5, 5, 10, 10,
+    # 1e-4) converges (Jet's absolute residual rule), in FP64 and mixed.
+    s = mg_poisson_hierarchy2(4, 6)
+    upload(solver, s, s.b[0])
+    gp = jetgpu.mg_params(max_levels=6, restrict_iter=5, correct_iter=5, coarsest_iter=10,
+                          final_iter=10, max_tol=1e-4)
+    for prec in (jetgpu.JG_FP64, jetgpu.JG_MIXED):
+        info = solver.mgpcg(200, gp, jetgpu.gpu_options(precision=prec))
+        print(f"mgpcg2 KAT precision={prec}: {info.iterations} iterations, "
+              f"residual {info.last_residual:.3e}", flush=True)
+        assert info.converged and info.last_residual <= 1e-4 and info.iterations < 200

@@ -32,7 +32,7 @@ def run_check(*args):
 
 def test_adapter_argument_checks():
     out = run_check("--no-gpu")
-    assert out.count("ok  ") == 12
+    assert out.count("ok  ") == 13
 
 
 @pytest.mark.gpu
