@@ -182,10 +182,16 @@ def main():
     out["constrain_velocity"] = bc["gpu"]
     csr = csr_leg(jg, n, args)
     out["compressed_cg"] = csr["gpu"]
+    dif = diffusion_leg(jg, n, args)
+    out["diffusion"] = dif["gpu"]
+    mg2 = mg2_leg(jg, args)
+    out["mgpcg_2d"] = mg2["gpu"]
     if not args.no_cpu:
         out.update(cpu_leg(jg, n, L, args, res))
         out["constrain_velocity"].update(bc_cpu_leg(bc, n))
         out["compressed_cg"].update(csr_cpu_leg(csr))
+        out["diffusion"].update(diffusion_cpu_leg(dif))
+        out["mgpcg_2d"].update(mg2_cpu_leg(mg2))
     print(json.dumps(out))
     solver.close()
 
@@ -274,6 +280,158 @@ def csr_leg(jg, n, args):
                     "alg_bytes_per_iteration": per_it,
                     "alg_gbs": round(per_it * it / (solve_ms * 1e-3) / 1e9, 1)},
             "sample": sample}
+
+
+DIF_DT, DIF_MU = 1.0 / 60.0, 0.05  # c = dt mu / h^2 = 54.6 at h = 1/256
+
+
+def diffusion_leg(jg, n, args):
+    """GridBackwardEulerDiffusionSolver3 on the scene's n^3 cell-centred grid
+    (boundary and fluid SDFs of the sphere pool, Neumann at the colliders):
+    the system built on the device and solved by Jet's CG to ||r|| <= 1e-8
+    ||b||, the field a smooth pattern."""
+    import numpy as np
+    import torch
+
+    f64 = torch.float64
+    shapes = [(n, n, n + 1), (n, n + 1, n), (n + 1, n, n), (n, n, n), (n, n, n)]
+    u, v, w, bs, fs = (torch.empty(sh, dtype=f64, device="cuda") for sh in shapes)
+    s = jg.Solver(0)
+    s.set_levels([(n, n, n)])
+    s.generate_scene(jg.scene_desc(jg.JG_SCENE_SPHERE_POOL, n, seed=args.seed), u, v, w, bs, fs)
+    g = torch.arange(n, dtype=f64, device="cuda") / n
+    f = (torch.sin(6.0 * g)[:, None, None] * torch.cos(4.0 * g)[None, :, None]
+         + g[None, None, :]).contiguous()
+    h = 1.0 / n
+    torch.cuda.synchronize()
+    s.diffusion_build(f, bs, fs, DIF_DT, DIF_MU, h)  # first build: allocation
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.diffusion_build(f, bs, fs, DIF_DT, DIF_MU, h)
+    torch.cuda.synchronize()
+    build_ms = (time.perf_counter() - t0) * 1e3
+    b = torch.as_tensor(jg._DeviceView(s.rhs_ptr(), n ** 3, "<f8"), device="cuda")
+    tol = 1e-8 * float(torch.linalg.norm(b).item())
+    t0 = time.perf_counter()
+    info = s.cg(2000, tol)
+    solve_ms = (time.perf_counter() - t0) * 1e3
+    it = max(1, info.iterations)
+    log(f"bench: diffusion {n}^3 build {build_ms:.2f} ms, CG {it} it {solve_ms:.2f} ms")
+    sample = None
+    if not args.no_cpu:
+        sample = tuple(a.cpu().numpy() for a in (f, bs, fs))
+    s.close()
+    del u, v, w, bs, fs, f, b
+    torch.cuda.empty_cache()
+    return {"gpu": {"cells": n ** 3, "dt": DIF_DT, "mu": DIF_MU, "h": h,
+                    "build_ms": round(build_ms, 3), "solve_ms": round(solve_ms, 3),
+                    "iterations": info.iterations, "rel_residual": info.rel_residual,
+                    "ms_per_iteration": round(solve_ms / it, 4)},
+            "sample": sample, "h": h}
+
+
+def diffusion_cpu_leg(dif):
+    """The oracle's diffusion build and 5 CG iterations on the same input."""
+    from oracle.jet_oracle import Oracle
+
+    f, bs, fs = dif["sample"]
+    o = Oracle()
+    t0 = time.perf_counter()
+    A, b = o.diffusion_build(f, bs, fs, DIF_DT, DIF_MU, (dif["h"],) * 3)
+    build_ms = (time.perf_counter() - t0) * 1e3
+    iters = 5
+    t0 = time.perf_counter()
+    o.cg_solve(A, b, iters, 0.0)
+    ms = (time.perf_counter() - t0) * 1e3
+    log(f"bench: diffusion oracle build {build_ms:.0f} ms, {ms / iters:.1f} ms per CG iteration")
+    return {"cpu_build_ms": round(build_ms, 1), "cpu_ms_per_iteration": round(ms / iters, 2),
+            "cpu_cores": 1, "cpu_kind": "port",
+            "cpu_sample": f"the oracle's build and {iters} CG iterations on the same grid"}
+
+
+# one plane of at most 2^20 cells: the stencil kernels' per-tile reduction
+# partials bound a plane's tile count
+MG2_N, MG2_LEVELS = 1024, 6
+
+
+def poisson2_hierarchy(n, levels, seed):
+    """A 2-D pressure-like hierarchy (FdmMgLinearSystem2 as one-plane levels):
+    5-point Laplacian scaled by (1/h_l)^2 with h_l = 2^l, closed walls and an
+    open (Dirichlet) top edge, b random on the finest level."""
+    import numpy as np
+
+    A, sizes = [], []
+    for l in range(levels):
+        m = n >> l
+        s = 0.25 ** l
+        j = np.arange(m)[:, None]
+        i = np.arange(m)[None, :]
+        cnt = ((i > 0).astype(float) + (i < m - 1) + (j > 0) + (j < m - 1) + (j == m - 1))
+        a = np.zeros((1, m, m, 4))
+        a[0, :, :, 0] = cnt * s
+        a[0, :, :, 1] = np.where(i < m - 1, -s, 0.0)
+        a[0, :, :, 2] = np.where(j < m - 1, -s, 0.0)
+        A.append(a)
+        sizes.append((m, m, 1))
+    b = np.random.default_rng(seed).uniform(-1.0, 1.0, (1, n, n))
+    return A, b, sizes
+
+
+def mg2_leg(jg, args):
+    """FdmMgpcgSolver2 on a 1024^2 pressure-like system (6 levels, V(2,2), 20
+    coarsest sweeps), FP64 and mixed, to ||r|| <= 1e-8 ||b||."""
+    import torch
+
+    A, b, sizes = poisson2_hierarchy(MG2_N, MG2_LEVELS, args.seed)
+    s = jg.Solver(0)
+    s.set_levels(sizes)
+    for l, a in enumerate(A):
+        s.upload(l, a, b if l == 0 else None)
+    p = jg.mg_params(max_levels=MG2_LEVELS, restrict_iter=2, correct_iter=2, coarsest_iter=20,
+                     final_iter=2, sor=args.omega)
+    out = {}
+    for name, prec in (("fp64", jg.JG_FP64), ("mixed", jg.JG_MIXED)):
+        o = jg.gpu_options(precision=prec, stop_rule=jg.JG_STOP_RELATIVE, rel_tol=1e-8)
+        s.mgpcg(200, p, o)  # warm-up: tuning and FP32 copies
+        reps = 3
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            info = s.mgpcg(200, p, o)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        out[name] = {"solve_ms": round(ms, 3), "iterations": info.iterations,
+                     "rel_residual": info.rel_residual}
+        log(f"bench: 2-D MGPCG {MG2_N}^2 {name}: {ms:.2f} ms, {info.iterations} it")
+    s.close()
+    torch.cuda.empty_cache()
+    gpu = {"grid": [MG2_N, MG2_N], "levels": MG2_LEVELS, **out,
+           "speedup_mixed_vs_fp64": round(out["fp64"]["solve_ms"] / out["mixed"]["solve_ms"], 3),
+           "ms_per_iteration_fp64": round(out["fp64"]["solve_ms"]
+                                          / max(1, out["fp64"]["iterations"]), 4)}
+    return {"gpu": gpu, "system": (A, b), "omega": args.omega}
+
+
+def mg2_cpu_leg(mg2):
+    """The oracle's FP64 clean-mode MGPCG on the same 2-D hierarchy, 5
+    iterations: the CPU time per iteration beside the GPU's."""
+    import numpy as np
+
+    from oracle.jet_oracle import MgParams, MgSystem, Oracle
+
+
+    A, b = mg2["system"]
+    sys_ = MgSystem(A=list(A), x=[np.zeros(a.shape[:3]) for a in A],
+                    b=[b] + [np.zeros(a.shape[:3]) for a in A[1:]])
+    p = MgParams.make(max_levels=len(A), restrict_iter=2, correct_iter=2, coarsest_iter=20,
+                      final_iter=2, sor=mg2["omega"])
+    iters = 5
+    t0 = time.perf_counter()
+    Oracle().mgpcg_clean_solve(sys_, p, iters, 0.0)
+    ms = (time.perf_counter() - t0) * 1e3
+    log(f"bench: 2-D MGPCG oracle {ms / iters:.0f} ms per iteration")
+    return {"cpu_ms_per_iteration": round(ms / iters, 1), "cpu_cores": 1, "cpu_kind": "port",
+            "cpu_sample": f"{iters} iterations of the oracle's FP64 MGPCG on the same hierarchy"}
 
 
 def csr_cpu_leg(csr):
