@@ -53,15 +53,44 @@ __global__ __launch_bounds__(256) void k_sp_coarsen(const int8_t* __restrict__ f
     coarse[i + (idx_t)cd.nx * (j + (idx_t)cd.ny * k)] = r;
 }
 
+// Coarse markers of a 2-D hierarchy: the vote over the clamped 4^2
+// neighbourhood (src/jet/grid_single_phase_pressure_solver2.cpp:238-283).
+__global__ __launch_bounds__(256) void k_sp_coarsen2(const int8_t* __restrict__ fine,
+                                                     int8_t* __restrict__ coarse, Dims cd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= cd.nx || j >= cd.ny) return;
+    const int fnx = 2 * cd.nx;
+    const int ii[4] = {i > 0 ? 2 * i - 1 : 2 * i, 2 * i, 2 * i + 1, i + 1 < cd.nx ? 2 * i + 2 : 2 * i + 1};
+    const int jj[4] = {j > 0 ? 2 * j - 1 : 2 * j, 2 * j, 2 * j + 1, j + 1 < cd.ny ? 2 * j + 2 : 2 * j + 1};
+    int cnt[3] = {0, 0, 0};
+    for (int y = 0; y < 4; ++y)
+        for (int x = 0; x < 4; ++x) {
+            const int8_t f = fine[ii[x] + (idx_t)fnx * jj[y]];
+            ++cnt[f == kBoundary ? 2 : (f == kFluid ? 0 : 1)];
+        }
+    int8_t r;
+    if (cnt[0] > cnt[1])
+        r = cnt[0] > cnt[2] ? 0 : 2;
+    else
+        r = cnt[1] > cnt[2] ? 1 : 2;
+    coarse[i + (idx_t)cd.nx * j] = r;
+}
+
 // buildSingleSystem (src/jet/grid_single_phase_pressure_solver3.cpp:25-79).
 // b (divergence, FaceCenteredGrid3::divergenceAtCellCenter,
 // src/jet/face_centered_grid3.cpp:137-152) is written only when u is given.
+// w == nullptr: a one-plane 2-D system (src/jet/grid_single_phase_pressure_
+// solver2.cpp:25-67, FaceCenteredGrid2::divergenceAtCellCenter, src/jet/
+// face_centered_grid2.cpp:107-118): no w term, and nz == 1 leaves no z
+// neighbour.
 __global__ __launch_bounds__(256) void k_sp_build(const int8_t* __restrict__ m,
                                                   const double* __restrict__ u,
                                                   const double* __restrict__ v,
                                                   const double* __restrict__ w, Dims d, H3 h,
                                                   Row<double>* __restrict__ A,
                                                   double* __restrict__ b) {
+#pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int k = blockIdx.z;
@@ -78,9 +107,12 @@ __global__ __launch_bounds__(256) void k_sp_build(const int8_t* __restrict__ m,
             const double ru = u[i + 1 + (idx_t)(d.nx + 1) * (j + (idx_t)d.ny * k)];
             const double bv = v[i + (idx_t)d.nx * (j + (idx_t)(d.ny + 1) * k)];
             const double tv = v[i + (idx_t)d.nx * (j + 1 + (idx_t)(d.ny + 1) * k)];
-            const double bw = w[c];
-            const double fw = w[c + sz];
-            bc = (ru - lu) / h.x + (tv - bv) / h.y + (fw - bw) / h.z;
+            bc = (ru - lu) / h.x + (tv - bv) / h.y;
+            if (w) {
+                const double bw = w[c];
+                const double fw = w[c + sz];
+                bc = bc + (fw - bw) / h.z;
+            }
         }
         if (i + 1 < d.nx && m[c + 1] != kBoundary) {
             row.c += ihx2;
@@ -113,6 +145,9 @@ __global__ __launch_bounds__(256) void k_sp_gradient(const int8_t* __restrict__ 
                                                      const double* u, const double* v,
                                                      const double* w, double* uo, double* vo,
                                                      double* wo, Dims d, H3 h) {
+    // u + (1/h) (p1 - p0) with the product rounded before the add (Jet's
+    // expression; a fused multiply-add differs wherever 1/h is not a power of 2)
+#pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int k = blockIdx.z;

@@ -224,6 +224,54 @@ int jg_sp_apply_gradient(jg_solver* s, const double* u, const double* v, const d
     return sync(s);
 }
 
+int jg_sp_build2(jg_solver* s, const double* u, const double* v, const double* boundary_sdf,
+                 const double* fluid_sdf, double hx, double hy) {
+    if (!s || s->lv.empty() || !u || !v || !boundary_sdf || !fluid_sdf || !(hx > 0) || !(hy > 0))
+        return JG_ERR_INVALID_ARG;
+    if (s->lv[0].d.nz != 1) {
+        s->err = "jg_sp_build2: the resident hierarchy is not a 2-D (one-plane) one";
+        return JG_ERR_INVALID_ARG;
+    }
+    JG_HIP(hipSetDevice(s->device));
+    const Dims& d0 = s->lv[0].d;
+    k_sp_markers<<<nblocks_1d(d0.n()), 256, 0, s->stream>>>(boundary_sdf, fluid_sdf,
+                                                            s->lv[0].markers, d0.n());
+    for (size_t l = 1; l < s->lv.size(); ++l) {
+        Geo g = geo_planes(s->lv[l].d);
+        k_sp_coarsen2<<<g.grid, g.block, 0, s->stream>>>(s->lv[l - 1].markers, s->lv[l].markers,
+                                                         s->lv[l].d);
+    }
+    H3 h{hx, hy, 1.0};
+    for (size_t l = 0; l < s->lv.size(); ++l) {
+        Geo g = geo_planes(s->lv[l].d);
+        k_sp_build<<<g.grid, g.block, 0, s->stream>>>(s->lv[l].markers, l == 0 ? u : nullptr, v,
+                                                      nullptr, s->lv[l].d, h, s->lv[l].A64,
+                                                      l == 0 ? s->b : nullptr);
+        h.x *= 2.0;
+        h.y *= 2.0;
+    }
+    s->f32_valid = false;
+    JG_TRY(check_launch(s, "sp_build2"));
+    return sync(s);
+}
+
+int jg_sp_apply_gradient2(jg_solver* s, const double* u, const double* v, double* uo, double* vo,
+                          double hx, double hy) {
+    if (!s || s->lv.empty() || !u || !v || !uo || !vo) return JG_ERR_INVALID_ARG;
+    if (s->lv[0].d.nz != 1) {
+        s->err = "jg_sp_apply_gradient2: the resident hierarchy is not a 2-D (one-plane) one";
+        return JG_ERR_INVALID_ARG;
+    }
+    JG_HIP(hipSetDevice(s->device));
+    const Dims& d0 = s->lv[0].d;
+    Geo g = geo_planes(d0);
+    // one plane: the kernel's +z branch never runs, so w / wo are not read
+    k_sp_gradient<<<g.grid, g.block, 0, s->stream>>>(s->lv[0].markers, s->x, u, v, nullptr, uo, vo,
+                                                     nullptr, d0, H3{hx, hy, 1.0});
+    JG_TRY(check_launch(s, "sp_gradient2"));
+    return sync(s);
+}
+
 const signed char* jg_solver_markers(jg_solver* s, size_t level) {
     if (!s || level >= s->lv.size()) return nullptr;
     return (const signed char*)s->lv[level].markers;

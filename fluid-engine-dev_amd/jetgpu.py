@@ -118,6 +118,8 @@ SIGNATURES = {
     "jg_mg_restrict": (C.c_int, [_vp, _sz, _sz, _sz, _vp, _vp]),
     "jg_mg_correct": (C.c_int, [_vp, _sz, _sz, _sz, _vp, _vp]),
     "jg_mg_restrict2": (C.c_int, [_vp, _sz, _sz, _vp, _vp]),
+    "jg_sp_build2": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_double, C.c_double]),
+    "jg_sp_apply_gradient2": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_double, C.c_double]),
     "jg_mg_correct2": (C.c_int, [_vp, _sz, _sz, _vp, _vp]),
     "jg_solver_time_kernel": (C.c_int, [_vp, C.c_int, C.c_int, _u, C.POINTER(_d), C.POINTER(_d)]),
     "jg_sp_build": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d]),
@@ -495,6 +497,16 @@ class Solver:
     def correct(self, fshape, coarse, fine):
         nx, ny, nz = fshape
         self.check(lib().jg_mg_correct(self.h, nx, ny, nz, _ptr(coarse), _ptr(fine)), "correct")
+
+    def sp_build2(self, u, v, bsdf, fsdf, h):
+        """GridSinglePhasePressureSolver2's markers and system on the resident
+        2-D hierarchy (device arrays; h = (hx, hy))."""
+        self.check(lib().jg_sp_build2(self.h, _ptr(u), _ptr(v), _ptr(bsdf), _ptr(fsdf), h[0], h[1]),
+                   "sp_build2")
+
+    def sp_apply_gradient2(self, u, v, uo, vo, h):
+        self.check(lib().jg_sp_apply_gradient2(self.h, _ptr(u), _ptr(v), _ptr(uo), _ptr(vo), h[0],
+                                               h[1]), "sp_apply_gradient2")
 
     def restrict2(self, cshape, fine, coarse):
         """FdmMgUtils2::restrict; cshape = (cnx, cny)."""

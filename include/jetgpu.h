@@ -282,6 +282,18 @@ int jg_sp_build(jg_solver* s, const double* u, const double* v, const double* w,
 int jg_sp_apply_gradient(jg_solver* s, const double* u, const double* v,
                          const double* w, double* uo, double* vo, double* wo,
                          double hx, double hy, double hz);
+/* GridSinglePhasePressureSolver2::buildMarkers + buildSystem
+ * (src/jet/grid_single_phase_pressure_solver2.cpp:218-354) on a resident 2-D
+ * hierarchy (one-plane levels, jg_solver_set_levels with nz = 1).  Device
+ * inputs: u ((nx+1)*ny), v (nx*(ny+1)), the boundary / fluid SDFs at the
+ * finest cell centres.  Markers on every level (coarse: 4^2 majority vote),
+ * A on every level (spacing 2^l h, front = 0), b = div u on level 0. */
+int jg_sp_build2(jg_solver* s, const double* u, const double* v, const double* boundary_sdf,
+                 const double* fluid_sdf, double hx, double hy);
+/* applyPressureGradient (src/jet/grid_single_phase_pressure_solver2.cpp:356-378)
+ * with the resident solution and level-0 markers of a 2-D hierarchy. */
+int jg_sp_apply_gradient2(jg_solver* s, const double* u, const double* v, double* uo,
+                          double* vo, double hx, double hy);
 /* Device pointer of the markers of a level (int8: 0 fluid, 1 air, 2 boundary). */
 const signed char* jg_solver_markers(jg_solver* s, size_t level);
 

@@ -751,6 +751,94 @@ void or_sp_coarsen_markers(const char* finer, char* coarser, size_t cnx,
     }
 }
 
+/* ----- 2-D single phase (GridSinglePhasePressureSolver2).  Cells are
+ * nx*ny, x fastest; u is (nx+1)*ny and v is nx*(ny+1). */
+
+/* Coarse markers: 4^2 clamped neighbourhood majority vote --
+ * src/jet/grid_single_phase_pressure_solver2.cpp:238-283. */
+void or_sp_coarsen_markers2(const char* finer, char* coarser, size_t cnx, size_t cny) {
+    const size_t fnx = 2 * cnx;
+    for (size_t j = 0; j < cny; ++j) {
+        const size_t jI[4] = {(j > 0) ? 2 * j - 1 : 2 * j, 2 * j, 2 * j + 1,
+                              (j + 1 < cny) ? 2 * j + 2 : 2 * j + 1};
+        for (size_t i = 0; i < cnx; ++i) {
+            const size_t iI[4] = {(i > 0) ? 2 * i - 1 : 2 * i, 2 * i, 2 * i + 1,
+                                  (i + 1 < cnx) ? 2 * i + 2 : 2 * i + 1};
+            int cnt[3] = {0, 0, 0};
+            for (int y = 0; y < 4; ++y)
+                for (int x = 0; x < 4; ++x) {
+                    const char f = finer[iI[x] + fnx * jI[y]];
+                    if (f == OR_BOUNDARY)
+                        ++cnt[OR_BOUNDARY];
+                    else if (f == OR_FLUID)
+                        ++cnt[OR_FLUID];
+                    else
+                        ++cnt[OR_AIR];
+                }
+            coarser[i + cnx * j] = (char)argmax3i(cnt[0], cnt[1], cnt[2]);
+        }
+    }
+}
+
+/* buildSingleSystem (2-D) -- src/jet/grid_single_phase_pressure_solver2.cpp:
+ * 25-67, with b = FaceCenteredGrid2::divergenceAtCellCenter
+ * (src/jet/face_centered_grid2.cpp:107-118).  Rows are written as the
+ * 4-coefficient rows of a one-plane system (front = 0); b is written only
+ * when u is given (coarse levels need A alone). */
+void or_sp_build_system2(const char* markers, const double* u, const double* v, size_t nx,
+                         size_t ny, const double h[2], or_row* A, double* b) {
+    const double ihx = 1.0 / h[0], ihy = 1.0 / h[1];
+    const double ihx2 = ihx * ihx, ihy2 = ihy * ihy;
+    for (size_t j = 0; j < ny; ++j)
+        for (size_t i = 0; i < nx; ++i) {
+            const size_t c = i + nx * j;
+            or_row row = {0.0, 0.0, 0.0, 0.0};
+            double bc = 0.0;
+            if (markers[c] == OR_FLUID) {
+                if (u) {
+                    const double lu = u[i + (nx + 1) * j], ru = u[i + 1 + (nx + 1) * j];
+                    const double bv = v[i + nx * j], tv = v[i + nx * (j + 1)];
+                    bc = (ru - lu) / h[0] + (tv - bv) / h[1];
+                }
+                if (i + 1 < nx && markers[c + 1] != OR_BOUNDARY) {
+                    row.center += ihx2;
+                    if (markers[c + 1] == OR_FLUID) row.right -= ihx2;
+                }
+                if (i > 0 && markers[c - 1] != OR_BOUNDARY) row.center += ihx2;
+                if (j + 1 < ny && markers[c + nx] != OR_BOUNDARY) {
+                    row.center += ihy2;
+                    if (markers[c + nx] == OR_FLUID) row.up -= ihy2;
+                }
+                if (j > 0 && markers[c - nx] != OR_BOUNDARY) row.center += ihy2;
+            } else {
+                row.center = 1.0;
+            }
+            A[c] = row;
+            if (b) b[c] = bc;
+        }
+}
+
+/* applyPressureGradient (2-D) -- src/jet/grid_single_phase_pressure_solver2.cpp:
+ * 356-378: +side faces of fluid cells whose neighbour is not Boundary. */
+void or_sp_apply_gradient2(const char* markers, const double* p, const double* u,
+                           const double* v, double* uo, double* vo, size_t nx, size_t ny,
+                           const double h[2]) {
+    const double ihx = 1.0 / h[0], ihy = 1.0 / h[1];
+    for (size_t j = 0; j < ny; ++j)
+        for (size_t i = 0; i < nx; ++i) {
+            const size_t c = i + nx * j;
+            if (markers[c] != OR_FLUID) continue;
+            if (i + 1 < nx && markers[c + 1] != OR_BOUNDARY) {
+                const size_t f = i + 1 + (nx + 1) * j;
+                uo[f] = u[f] + ihx * (p[c + 1] - p[c]);
+            }
+            if (j + 1 < ny && markers[c + nx] != OR_BOUNDARY) {
+                const size_t f = i + nx * (j + 1);
+                vo[f] = v[f] + ihy * (p[c + nx] - p[c]);
+            }
+        }
+}
+
 /* FaceCenteredGrid3::divergenceAtCellCenter -- src/jet/face_centered_grid3.cpp:137-152.
  * u is (nx+1)*ny*nz, v is nx*(ny+1)*nz, w is nx*ny*(nz+1), x fastest. */
 double or_divergence(const double* u, const double* v, const double* w,

@@ -144,6 +144,13 @@ void or_face_fill_coarse(const double* fu, const double* fv, const double* fw,
                          const double origin[3], double* cu, double* cv,
                          double* cw, size_t cnx, size_t cny, size_t cnz,
                          const double ch[3]);
+/* 2-D single phase: markers as or_sp_markers; the rest below. */
+void or_sp_coarsen_markers2(const char* finer, char* coarser, size_t cnx, size_t cny);
+void or_sp_build_system2(const char* markers, const double* u, const double* v, size_t nx,
+                         size_t ny, const double h[2], or_row* A, double* b);
+void or_sp_apply_gradient2(const char* markers, const double* p, const double* u,
+                           const double* v, double* uo, double* vo, size_t nx, size_t ny,
+                           const double h[2]);
 void or_sp_apply_gradient(const char* markers, const double* p,
                           const double* u, const double* v, const double* w,
                           double* uo, double* vo, double* wo, size_t nx,

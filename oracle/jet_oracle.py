@@ -438,6 +438,53 @@ class Oracle(_BoundaryCalls, _CsrCalls):
                                       _d(wo), _sz(nx), _sz(ny), _sz(nz), (C.c_double * 3)(*h))
         return uo, vo, wo
 
+    # ------------------------------------------ 2-D single phase ((ny, nx) arrays)
+    def sp_coarsen_markers2(self, fine):
+        ny, nx = fine.shape
+        out = np.empty((ny // 2, nx // 2), np.int8)
+        self.lib.or_sp_coarsen_markers2(_c(np.ascontiguousarray(fine)), _c(out), _sz(nx // 2),
+                                        _sz(ny // 2))
+        return out
+
+    def sp_build_system2(self, markers, u, v, h):
+        """-> (A as (1, ny, nx, 4) one-plane rows, b as (1, ny, nx)); u None: A only."""
+        ny, nx = markers.shape
+        A = np.empty((1, ny, nx, 4))
+        b = np.zeros((1, ny, nx))
+        self.lib.or_sp_build_system2(_c(np.ascontiguousarray(markers)),
+                                     _d(u) if u is not None else None,
+                                     _d(v) if v is not None else None, _sz(nx), _sz(ny),
+                                     (C.c_double * 2)(*h), _d(A),
+                                     _d(b) if u is not None else None)
+        return A, b
+
+    def sp_apply_gradient2(self, markers, p, u, v, h):
+        uo, vo = u.copy(), v.copy()
+        ny, nx = markers.shape
+        self.lib.or_sp_apply_gradient2(_c(np.ascontiguousarray(markers)), _d(p), _d(u), _d(v),
+                                       _d(uo), _d(vo), _sz(nx), _sz(ny), (C.c_double * 2)(*h))
+        return uo, vo
+
+    def sp_build_mg2(self, u, v, h, bsdf, fsdf, max_levels):
+        """GridSinglePhasePressureSolver2::buildMarkers + buildSystem (src/jet/
+        grid_single_phase_pressure_solver2.cpp:218-354) on (ny, nx) cell-centred
+        SDF samples: (markers per level, MgSystem of one-plane levels).  Coarse
+        levels carry A only (their b is what the V-cycle restricts)."""
+        ny, nx = bsdf.shape
+        sizes = self.levels_with_finest2((nx, ny), max_levels)
+        markers = [self.sp_markers(np.ascontiguousarray(bsdf), np.ascontiguousarray(fsdf))]
+        for _ in sizes[1:]:
+            markers.append(self.sp_coarsen_markers2(markers[-1]))
+        sys = MgSystem()
+        hh = tuple(h)
+        for l, m in enumerate(markers):
+            A, b = self.sp_build_system2(m, u if l == 0 else None, v if l == 0 else None, hh)
+            sys.A.append(A)
+            sys.b.append(b)
+            sys.x.append(np.zeros_like(b))
+            hh = (2.0 * hh[0], 2.0 * hh[1])
+        return markers, sys
+
     def sp_build_mg(self, u, v, w, h, origin, bsdf, fsdf, max_levels):
         """GridSinglePhasePressureSolver3::buildMarkers + buildSystem
         (src/jet/grid_single_phase_pressure_solver3.cpp:248-399): returns

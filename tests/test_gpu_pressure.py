@@ -110,23 +110,31 @@ def test_jet_pressure_known_answers(solver, case):
 
 
 def test_projection_gradient_matches_oracle(solver, oracle):
+    # spacing 1/48: 1/h is not a power of two, so a fused multiply-add in the
+    # gradient would show up as a last-bit difference
     n = 48
+    h = 1.0 / n
     d = jetgpu.scene_desc(jetgpu.JG_SCENE_SPHERE_POOL, n, seed=6)
     s = jetgpu.generate_scene_host(d)
-    markers, sys_ = oracle.sp_build_mg(s["u"], s["v"], s["w"], (1.0,) * 3, (0.0,) * 3,
+    markers, sys_ = oracle.sp_build_mg(s["u"], s["v"], s["w"], (h,) * 3, (0.0,) * 3,
                                        s["bsdf"], s["fsdf"], 1)
     solver.set_levels([(n, n, n)])
     du, dv, dw = dev(s["u"]), dev(s["v"]), dev(s["w"])
-    solver.sp_build(du, dv, dw, dev(s["bsdf"]), dev(s["fsdf"]), 1.0)
+    solver.sp_build(du, dv, dw, dev(s["bsdf"]), dev(s["fsdf"]), h)
+    A = np.empty(sys_.A[0].shape)
+    b = np.empty(sys_.b[0].shape)
+    copy_to_host(solver.matrix_ptr(0), A)
+    copy_to_host(solver.rhs_ptr(), b)
+    assert np.array_equal(A, sys_.A[0]) and np.array_equal(b, sys_.b[0])
     info = solver.cg(2000, 1e-9)
     p = solver.download_x()
     # Jet writes only the projected faces of the output grid; the rest keep
     # the output's contents, here a copy of the input (as the oracle does).
     uo, vo, wo = dev(s["u"]), dev(s["v"]), dev(s["w"])
-    solver.sp_apply_gradient(du, dv, dw, uo, vo, wo, 1.0)
-    ou, ov, ow = oracle.sp_apply_gradient(markers[0], p, s["u"], s["v"], s["w"], (1.0,) * 3)
+    solver.sp_apply_gradient(du, dv, dw, uo, vo, wo, h)
+    ou, ov, ow = oracle.sp_apply_gradient(markers[0], p, s["u"], s["v"], s["w"], (h,) * 3)
     for g, o_ in ((uo, ou), (vo, ov), (wo, ow)):
-        np.testing.assert_allclose(host(g), o_, rtol=0, atol=1e-13)
+        assert np.array_equal(host(g), o_)
 
 
 def test_config1_closed_sphere_cg(solver, oracle):
