@@ -60,11 +60,23 @@ def main():
                     help="SOR factor of the red-black smoother (1.3 needs the fewest "
                          "iterations on this workload: 13 against 15 at 1.0 and 1.5)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / oracle leg")
+    ap.add_argument("--only", choices=["compressed", "diffusion", "mgpcg_2d"],
+                    help="run one secondary leg alone, GPU only (for a per-leg kernel trace), "
+                         "and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(multi-GPU sharding is a design note, not a measured path)"}))
         sys.exit(2)
+    if args.only:
+        import jetgpu as jg
+
+        args.no_cpu = True
+        legs = {"compressed": lambda: csr_leg(jg, args.n, args),
+                "diffusion": lambda: diffusion_leg(jg, args.n, args),
+                "mgpcg_2d": lambda: mg2_leg(jg, args)}
+        print(json.dumps({"leg": args.only, **legs[args.only]()["gpu"]}))
+        return
 
     import numpy as np
     import torch
