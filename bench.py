@@ -96,6 +96,12 @@ def main():
     desc = jg.scene_desc(jg.JG_SCENE_SPHERE_POOL, n, seed=args.seed)
     solver.generate_scene(desc, u, v, w, bs, fs)
     solver.sp_build(u, v, w, bs, fs, 1.0)
+    # the flat multi-level build (markers and A on every level, b): timed on a second call
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solver.sp_build(u, v, w, bs, fs, 1.0)
+    torch.cuda.synchronize()
+    build_ms = (time.perf_counter() - t0) * 1e3
     fluid_fraction = float(((bs >= 0) & (fs < 0)).double().mean().item())
     del u, v, w, bs, fs
     torch.cuda.empty_cache()
@@ -190,6 +196,9 @@ def main():
                            * n_cells / (res["mixed"]["ms"] * 1e-3) / 1e9, 1)},
     }
 
+    out["system_build"] = {"gpu_ms": round(build_ms, 3), "levels": L,
+                           "what": "GridSinglePhasePressureSolver3 markers and A on every level, "
+                                   "b on level 0 (jg_sp_build)"}
     bc = bc_leg(jg, solver, n, args)
     out["constrain_velocity"] = bc["gpu"]
     csr = csr_leg(jg, n, args)
@@ -490,8 +499,11 @@ def cpu_leg(jg, n, L, args, res):
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     oracle = Oracle()
     scene = jg.generate_scene_host(jg.scene_desc(jg.JG_SCENE_SPHERE_POOL, n, seed=args.seed))
+    t0 = time.perf_counter()
     markers, sys_ = oracle.sp_build_mg(scene["u"], scene["v"], scene["w"], (1.0,) * 3, (0.0,) * 3,
                                        scene["bsdf"], scene["fsdf"], L)
+    build_cpu_ms = (time.perf_counter() - t0) * 1e3
+    log(f"bench: oracle system build {build_cpu_ms:.0f} ms")
     del scene
     p = MgParams.make(max_levels=L, restrict_iter=2, correct_iter=2, coarsest_iter=20, final_iter=2,
                       sor=args.omega)
@@ -505,6 +517,9 @@ def cpu_leg(jg, n, L, args, res):
     err = {k: float(np.linalg.norm(res[k]["x"] - p_ref) / nref) for k in res}
     log(f"bench: error vs oracle {err} (p_ref rel residual {rel_ref:.1e})")
     return {
+        "system_build_cpu": {"cpu_ms": round(build_cpu_ms, 1), "cores": threads, "kind": "port",
+                             "sample": "the oracle's build of the same levels (it also builds the "
+                                       "coarse right-hand sides from down-sampled faces)"},
         "err_vs_oracle": err,
         "oracle_p_ref_rel_residual": rel_ref,
         "cpu_baseline": {"value": round(cpu_ms, 1), "unit": "ms", "cores": threads, "kind": "port",
