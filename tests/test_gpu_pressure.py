@@ -137,8 +137,8 @@ def test_projection_gradient_matches_oracle(solver, oracle):
         assert np.array_equal(host(g), o_)
 
 
-def test_config1_closed_sphere_cg(solver, oracle):
-    # BASELINE config 1 geometry at full size: 64^3, sphere r = 16, all
+def test_closed_sphere_64_cg(solver, oracle):
+    # The closed-sphere workload at full size: 64^3, sphere r = 16, all
     # Neumann, FdmCgSolver3(2000, 1e-8).  Compared modulo the mean.
     n = 64
     d = jetgpu.scene_desc(jetgpu.JG_SCENE_CLOSED_SPHERE, n, seed=1)
@@ -155,8 +155,8 @@ def test_config1_closed_sphere_cg(solver, oracle):
     assert rel_err_mean_free(x, xo, fluid) <= 1e-6
 
 
-def test_config2_dam_break_mgpcg(solver, oracle):
-    # BASELINE config 2 at full size: 128^3 dam break, 5 levels, 2/2 sweeps.
+def test_dam_break_128_mgpcg(solver, oracle):
+    # The dam-break workload at full size: 128^3, 5 levels, 2/2 sweeps.
     from oracle.jet_oracle import MgParams
 
     n, levels = 128, 5

@@ -1,6 +1,6 @@
 """GPU CG and MGPCG (FP64 and mixed FP32/FP64) against the CPU oracle.
 
-Pass criteria (north-star parity bar): relative residual <= 1e-8 and
+Pass criteria (the solver parity bar): relative residual <= 1e-8 and
 ||p - p_ref|| / ||p_ref|| <= 1e-6, with p_ref the oracle's FP64 solution of
 the same system (modulo the mean on all-Neumann systems).  GPU CG follows
 Jet's FdmCgSolver3 control flow, so its iteration count must match the
