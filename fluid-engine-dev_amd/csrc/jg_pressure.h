@@ -291,8 +291,12 @@ __global__ __launch_bounds__(256) void k_csr_fill(const int8_t* __restrict__ m,
         const double ru = u[i + 1 + (idx_t)(d.nx + 1) * (j + (idx_t)d.ny * k)];
         const double bv = v[i + (idx_t)d.nx * (j + (idx_t)(d.ny + 1) * k)];
         const double tv = v[i + (idx_t)d.nx * (j + 1 + (idx_t)(d.ny + 1) * k)];
-        const double bw = w[c], fw = w[c + sz];
-        b[row] = (ru - lu) / hx + (tv - bv) / hy + (fw - bw) / hz;
+        double bc = (ru - lu) / hx + (tv - bv) / hy;
+        if (w) {  // w == nullptr: one-plane 2-D system, no w term
+            const double bw = w[c], fw = w[c + sz];
+            bc = bc + (fw - bw) / hz;
+        }
+        b[row] = bc;
     }
 }
 

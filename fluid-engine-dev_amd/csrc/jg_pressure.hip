@@ -409,12 +409,11 @@ int jg_constrain_velocity(jg_solver* s, int blocked, const double h[3], const do
     return sync(s);
 }
 
-int jg_sp_build_compressed(jg_solver* s, const double* u, const double* v, const double* w,
-                           const double* boundary_sdf, const double* fluid_sdf, double hx,
-                           double hy, double hz, size_t* rows_out, size_t* nnz_out) {
-    if (!s || s->lv.empty() || !u || !v || !w || !boundary_sdf || !fluid_sdf || !(hx > 0) ||
-        !(hy > 0) || !(hz > 0))
-        return JG_ERR_INVALID_ARG;
+// The compressed build of the finest level; w == nullptr builds a one-plane
+// 2-D system (no w term, hz unused).
+static int sp_build_compressed(jg_solver* s, const double* u, const double* v, const double* w,
+                               const double* boundary_sdf, const double* fluid_sdf, double hx,
+                               double hy, double hz, size_t* rows_out, size_t* nnz_out) {
     JG_HIP(hipSetDevice(s->device));
     Level& l0 = s->lv[0];
     const Dims& d = l0.d;
@@ -462,6 +461,28 @@ int jg_sp_build_compressed(jg_solver* s, const double* u, const double* v, const
     if (rows_out) *rows_out = rows;
     if (nnz_out) *nnz_out = nnz;
     return sync(s);
+}
+
+int jg_sp_build_compressed(jg_solver* s, const double* u, const double* v, const double* w,
+                           const double* boundary_sdf, const double* fluid_sdf, double hx,
+                           double hy, double hz, size_t* rows_out, size_t* nnz_out) {
+    if (!s || s->lv.empty() || !u || !v || !w || !boundary_sdf || !fluid_sdf || !(hx > 0) ||
+        !(hy > 0) || !(hz > 0))
+        return JG_ERR_INVALID_ARG;
+    return sp_build_compressed(s, u, v, w, boundary_sdf, fluid_sdf, hx, hy, hz, rows_out, nnz_out);
+}
+
+int jg_sp_build_compressed2(jg_solver* s, const double* u, const double* v,
+                            const double* boundary_sdf, const double* fluid_sdf, double hx,
+                            double hy, size_t* rows_out, size_t* nnz_out) {
+    if (!s || s->lv.empty() || !u || !v || !boundary_sdf || !fluid_sdf || !(hx > 0) || !(hy > 0))
+        return JG_ERR_INVALID_ARG;
+    if (s->lv[0].d.nz != 1) {
+        s->err = "jg_sp_build_compressed2: the finest level is not one plane";
+        return JG_ERR_INVALID_ARG;
+    }
+    return sp_build_compressed(s, u, v, nullptr, boundary_sdf, fluid_sdf, hx, hy, 1.0, rows_out,
+                               nnz_out);
 }
 
 int jg_diffusion_build(jg_solver* s, const double* f, const double* boundary_sdf,

@@ -101,6 +101,8 @@ SIGNATURES = {
     "jg_sp_build_compressed": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d, C.POINTER(_sz),
                                          C.POINTER(_sz)]),
     "jg_solver_csr_cg": (C.c_int, [_vp, _u, _d, C.POINTER(SolveInfo)]),
+    "jg_sp_build_compressed2": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _d, _d, C.POINTER(_sz),
+                                          C.POINTER(_sz)]),
     "jg_diffusion_build": (C.c_int, [_vp, _vp, _vp, _vp, _d, _d, _d, _d, _d, C.c_int]),
     "jg_csr_decompress": (C.c_int, [_vp]),
     "jg_csr_array": (_vp, [_vp, C.c_int]),
@@ -352,6 +354,14 @@ class Solver:
         self.check(lib().jg_sp_build_compressed(self.h, _ptr(u), _ptr(v), _ptr(w), _ptr(bsdf),
                                                 _ptr(fsdf), h, h, h, C.byref(rows), C.byref(nnz)),
                    "sp_build_compressed")
+        return rows.value, nnz.value
+
+    def sp_build_compressed2(self, u, v, bsdf, fsdf, h):
+        """Resident compressed 2-D system of a one-plane finest level; (rows, nnz)."""
+        rows, nnz = _sz(), _sz()
+        self.check(lib().jg_sp_build_compressed2(self.h, _ptr(u), _ptr(v), _ptr(bsdf), _ptr(fsdf),
+                                                 h[0], h[1], C.byref(rows), C.byref(nnz)),
+                   "sp_build_compressed2")
         return rows.value, nnz.value
 
     def diffusion_build(self, f, bsdf, fsdf, dt, mu, h, dirichlet=False):

@@ -320,6 +320,13 @@ int jg_diffusion_build(jg_solver* s, const double* f, const double* boundary_sdf
 int jg_sp_build_compressed(jg_solver* s, const double* u, const double* v, const double* w,
                            const double* boundary_sdf, const double* fluid_sdf, double hx,
                            double hy, double hz, size_t* rows, size_t* nnz);
+/* The compressed 2-D single-phase build (src/jet/grid_single_phase_pressure_solver2.cpp:
+ * 69-130, rows in MatrixCsr::addRow's sorted order) of a one-plane finest level:
+ * u ((nx+1)*ny), v (nx*(ny+1)) and the SDFs as for jg_sp_build2; otherwise as
+ * jg_sp_build_compressed. */
+int jg_sp_build_compressed2(jg_solver* s, const double* u, const double* v,
+                            const double* boundary_sdf, const double* fluid_sdf, double hx,
+                            double hy, size_t* rows_out, size_t* nnz_out);
 /* FdmCgSolver3::solveCompressed (src/jet/fdm_cg_solver3.cpp:49-74) on the
  * resident compressed system (jg_sp_build_compressed or the last
  * jg_csr_*_host call). */

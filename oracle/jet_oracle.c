@@ -818,6 +818,72 @@ void or_sp_build_system2(const char* markers, const double* u, const double* v, 
         }
 }
 
+/* Compressed buildSingleSystem (2-D) -- src/jet/grid_single_phase_pressure_solver2.cpp:
+ * 69-130: fluid cells numbered in x-fastest order; a row lists the centre
+ * (its coefficient summed over non-boundary neighbours in the order +x, -x,
+ * +y, -y) and the fluid neighbours, then MatrixCsr::addRow sorts the entries
+ * by column (include/jet/detail/matrix_csr-inl.h:253-275), which for this
+ * numbering is -y, -x, centre, +x, +y.  b = divergenceAtCellCenter.  With
+ * val == NULL only the row and entry counts are returned. */
+size_t or_sp_build_compressed2(const char* markers, const double* u, const double* v, size_t nx,
+                               size_t ny, const double h[2], double* val, size_t* rp, size_t* ci,
+                               double* b, size_t* nnz) {
+    const double ihx = 1.0 / h[0], ihy = 1.0 / h[1];
+    const double ihx2 = ihx * ihx, ihy2 = ihy * ihy;
+    const size_t n = nx * ny;
+    size_t* index = (size_t*)malloc((n ? n : 1) * sizeof(size_t));
+    size_t rows = 0, e = 0;
+    for (size_t c = 0; c < n; ++c)
+        if (markers[c] == OR_FLUID) index[c] = rows++;
+    for (size_t j = 0; j < ny; ++j)
+        for (size_t i = 0; i < nx; ++i) {
+            const size_t c = i + nx * j;
+            if (markers[c] != OR_FLUID) continue;
+            double centre = 0.0;
+            int has[4] = {0, 0, 0, 0}; /* -y, -x, +x, +y fluid neighbours */
+            if (i + 1 < nx && markers[c + 1] != OR_BOUNDARY) {
+                centre += ihx2;
+                has[2] = markers[c + 1] == OR_FLUID;
+            }
+            if (i > 0 && markers[c - 1] != OR_BOUNDARY) {
+                centre += ihx2;
+                has[1] = markers[c - 1] == OR_FLUID;
+            }
+            if (j + 1 < ny && markers[c + nx] != OR_BOUNDARY) {
+                centre += ihy2;
+                has[3] = markers[c + nx] == OR_FLUID;
+            }
+            if (j > 0 && markers[c - nx] != OR_BOUNDARY) {
+                centre += ihy2;
+                has[0] = markers[c - nx] == OR_FLUID;
+            }
+            const size_t row = index[c];
+            if (val) {
+                rp[row] = e;
+                const size_t nb[4] = {c - nx, c - 1, c + 1, c + nx};
+                const double cf[4] = {-ihy2, -ihx2, -ihx2, -ihy2};
+                for (int q = 0; q < 4; ++q) {
+                    if (q == 2) {
+                        val[e] = centre;
+                        ci[e++] = row;
+                    }
+                    if (!has[q]) continue;
+                    val[e] = cf[q];
+                    ci[e++] = index[nb[q]];
+                }
+                const double lu = u[i + (nx + 1) * j], ru = u[i + 1 + (nx + 1) * j];
+                const double bv = v[i + nx * j], tv = v[i + nx * (j + 1)];
+                b[row] = (ru - lu) / h[0] + (tv - bv) / h[1];
+            } else {
+                e += 1 + has[0] + has[1] + has[2] + has[3];
+            }
+        }
+    if (val) rp[rows] = e;
+    *nnz = e;
+    free(index);
+    return rows;
+}
+
 /* applyPressureGradient (2-D) -- src/jet/grid_single_phase_pressure_solver2.cpp:
  * 356-378: +side faces of fluid cells whose neighbour is not Boundary. */
 void or_sp_apply_gradient2(const char* markers, const double* p, const double* u,

@@ -148,6 +148,9 @@ void or_face_fill_coarse(const double* fu, const double* fv, const double* fw,
 void or_sp_coarsen_markers2(const char* finer, char* coarser, size_t cnx, size_t cny);
 void or_sp_build_system2(const char* markers, const double* u, const double* v, size_t nx,
                          size_t ny, const double h[2], or_row* A, double* b);
+size_t or_sp_build_compressed2(const char* markers, const double* u, const double* v, size_t nx,
+                               size_t ny, const double h[2], double* val, size_t* rp, size_t* ci,
+                               double* b, size_t* nnz);
 void or_sp_apply_gradient2(const char* markers, const double* p, const double* u,
                            const double* v, double* uo, double* vo, size_t nx, size_t ny,
                            const double h[2]);

@@ -458,6 +458,23 @@ class Oracle(_BoundaryCalls, _CsrCalls):
                                      _d(b) if u is not None else None)
         return A, b
 
+    def sp_build_compressed2(self, markers, u, v, h):
+        """Compressed 2-D buildSingleSystem: (values, row_ptr, col_idx, b)."""
+        ny, nx = markers.shape
+        hh = np.asarray(h, np.float64)
+        nnz = C.c_size_t()
+        f = self.lib.or_sp_build_compressed2
+        f.restype = C.c_size_t
+        args = (_c(np.ascontiguousarray(markers)), _d(u), _d(v), _sz(nx), _sz(ny), _d(hh))
+        rows = f(*args, None, None, None, None, C.byref(nnz))
+        val = np.empty(nnz.value)
+        rp = np.empty(rows + 1, np.uint64)
+        ci = np.empty(nnz.value, np.uint64)
+        b = np.empty(rows)
+        sz_p = C.POINTER(C.c_size_t)
+        f(*args, _d(val), rp.ctypes.data_as(sz_p), ci.ctypes.data_as(sz_p), _d(b), C.byref(nnz))
+        return val, rp, ci, b
+
     def sp_apply_gradient2(self, markers, p, u, v, h):
         uo, vo = u.copy(), v.copy()
         ny, nx = markers.shape

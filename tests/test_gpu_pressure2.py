@@ -94,3 +94,45 @@ def test_sp_build2_rejects_3d_hierarchy(solver):
     z = empty((8 * 8 * 9,))
     with pytest.raises(RuntimeError):
         solver.sp_build2(z, z, z, z, (1.0, 1.0))
+
+
+@pytest.mark.parametrize("n,seed", [((64, 32), 11), ((96, 96), 12), ((7, 5), 13), ((1024, 512), 14)])
+def test_sp_build_compressed2_matches_oracle(solver, oracle, n, seed):
+    print(f"case sp_build_compressed2 n={n}", flush=True)
+    nx, ny = n
+    h = 1.0 / max(n)
+    bsdf, fsdf, u, v = random_scene2(nx, ny, np.random.default_rng(seed), h)
+    m = oracle.sp_markers(bsdf, fsdf)
+    val, rp, ci, b = oracle.sp_build_compressed2(m, u, v, (h, h))
+    solver.set_levels([(nx, ny, 1)])
+    rows, nnz = solver.sp_build_compressed2(dev(u), dev(v), dev(bsdf), dev(fsdf), (h, h))
+    assert (rows, nnz) == (b.size, val.size)
+    gval, grp, gci, gb, _ = solver.csr_download(rows, nnz)
+    assert np.array_equal(gval, val) and np.array_equal(gb, b)
+    assert np.array_equal(grp.astype(np.uint64), rp) and np.array_equal(gci.astype(np.uint64), ci)
+
+
+@pytest.mark.parametrize("case", ["single", "free"])
+def test_kat2_compressed_through_gpu(solver, case):
+    # SolveSinglePhaseCompressed / SolveFreeSurfaceCompressed (src/tests/
+    # unit_tests/grid_single_phase_pressure_solver2_tests.cpp:60-102, 201-246)
+    # with the compressed build, Jet's CG, decompressSolution and the
+    # gradient on the device
+    u, v = kat_velocity2()
+    bs = cc2(lambda x, y: BIG + 0 * x, 3, 3)
+    fs = cc2(lambda x, y: y - 2.0, 3, 3) if case == "free" else cc2(lambda x, y: -BIG + 0 * x, 3, 3)
+    solver.set_levels([(3, 3, 1)])
+    solver.sp_build_compressed2(dev(u), dev(v), dev(bs), dev(fs), (1.0, 1.0))
+    info = solver.csr_solve(200, 1e-12)
+    assert info.converged
+    solver.csr_decompress()
+    p = solver.download_x()[0]
+    uo, vo = dev(u), dev(v)
+    solver.sp_apply_gradient2(dev(u), dev(v), uo, vo, (1.0, 1.0))
+    uo, vo = host(uo), host(vo)
+    assert np.abs(uo).max() < 1e-6 and np.abs(vo).max() < 1e-6
+    if case == "free":
+        for j in range(2):
+            np.testing.assert_allclose(p[j, :], 2.0 - j, atol=1e-6)
+    else:
+        np.testing.assert_allclose(p[1:, :] - p[:-1, :], -1.0, atol=1e-6)

@@ -149,3 +149,60 @@ def test_sp_build_mg2_hierarchy(oracle):
         inner[1:-1, 1:-1] = ((m[1:-1, 1:-1] == 0) & (m[1:-1, :-2] == 0) & (m[1:-1, 2:] == 0)
                              & (m[:-2, 1:-1] == 0) & (m[2:, 1:-1] == 0))
         assert np.all(A[0][inner][:, 0] == 4 * ih2)
+
+
+def decompress2(m, x):
+    p = np.zeros(m.shape)
+    p[m == 0] = x
+    return p
+
+
+@pytest.mark.parametrize("case", ["single", "free"])
+def test_kat2_compressed(oracle, case):
+    # ...SolveSinglePhaseCompressed (:60-102) and SolveFreeSurfaceCompressed
+    # (:201-246): the compressed build, Jet's CG on it, decompressSolution
+    u, v = kat_velocity2()
+    bs = cc2(lambda x, y: BIG + 0 * x, 3, 3)
+    fs = cc2(lambda x, y: y - 2.0, 3, 3) if case == "free" else cc2(lambda x, y: -BIG + 0 * x, 3, 3)
+    m = oracle.sp_markers(bs, fs)
+    val, rp, ci, b = oracle.sp_build_compressed2(m, u, v, (1.0, 1.0))
+    x, it, res, ok = oracle.csr_cg_solve(val, rp, ci, b, 200, 1e-12)
+    p = decompress2(m, x)
+    uo, vo = oracle.sp_apply_gradient2(m, p, u, v, (1.0, 1.0))
+    assert np.abs(uo).max() < 1e-6 and np.abs(vo).max() < 1e-6
+    if case == "free":
+        for j in range(2):
+            np.testing.assert_allclose(p[j, :], 2.0 - j, atol=1e-6)
+    else:
+        np.testing.assert_allclose(p[1:, :] - p[:-1, :], -1.0, atol=1e-6)
+
+
+@pytest.mark.parametrize("seed,n", [(8, (12, 10)), (9, (17, 9))])
+def test_compressed2_equals_3d_and_flat(oracle, seed, n):
+    # the 3-D compressed restatement on one plane (w = 0) gives the same
+    # arrays, and the compressed rows are the flat rows on the fluid cells
+    rng = np.random.default_rng(seed)
+    nx, ny = n
+    bsdf, fsdf, u, v = random_scene2(nx, ny, rng, 0.5)
+    m = oracle.sp_markers(bsdf, fsdf)
+    val, rp, ci, b = oracle.sp_build_compressed2(m, u, v, (0.5, 0.5))
+    w = np.zeros((2, ny, nx))
+    v3, r3, c3, b3 = oracle.sp_build_compressed(m[None].copy(), u[None].copy(), v[None].copy(), w,
+                                                (0.5, 0.5, 0.5))
+    assert np.array_equal(val, v3) and np.array_equal(rp, r3) and np.array_equal(ci, c3)
+    assert np.array_equal(b, b3)
+    A, bf = oracle.sp_build_system2(m, u, v, (0.5, 0.5))
+    fl = np.flatnonzero(m.ravel() == 0)
+    assert np.array_equal(b, bf.ravel()[fl])
+    rows = len(fl)
+    D = np.zeros((rows, rows))
+    for r in range(rows):
+        D[r, ci[rp[r]:rp[r + 1]].astype(int)] = val[rp[r]:rp[r + 1]]
+    pos = {c: k for k, c in enumerate(fl)}
+    for k, c in enumerate(fl):
+        assert D[k, k] == A.reshape(-1, 4)[c, 0]
+        if c + 1 in pos and (c + 1) % nx:
+            assert D[k, pos[c + 1]] == A.reshape(-1, 4)[c, 1]
+        if c + nx in pos:
+            assert D[k, pos[c + nx]] == A.reshape(-1, 4)[c, 2]
+    assert np.array_equal(D, D.T)
