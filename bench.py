@@ -370,9 +370,9 @@ def diffusion_cpu_leg(dif):
             "cpu_sample": f"the oracle's build and {iters} CG iterations on the same grid"}
 
 
-# one plane of at most 2^20 cells: the stencil kernels' per-tile reduction
-# partials bound a plane's tile count
-MG2_N, MG2_LEVELS = 1024, 6
+# 4096^2: as many cells as the 256^3 workload, in one plane (the stencil
+# kernels' per-tile reduction partials bound a plane at about 2^24 cells)
+MG2_N, MG2_LEVELS = 4096, 8
 
 
 def poisson2_hierarchy(n, levels, seed):
@@ -399,7 +399,7 @@ def poisson2_hierarchy(n, levels, seed):
 
 
 def mg2_leg(jg, args):
-    """FdmMgpcgSolver2 on a 1024^2 pressure-like system (6 levels, V(2,2), 20
+    """FdmMgpcgSolver2 on a 4096^2 pressure-like system (8 levels, V(2,2), 20
     coarsest sweeps), FP64 and mixed, to ||r|| <= 1e-8 ||b||."""
     import torch
 

@@ -52,7 +52,9 @@ struct Ctl {
 
 constexpr int kWave = 64;
 constexpr int NT = 256;  // threads per block of the streaming kernels
-constexpr int kMaxPartials = 16384;  // per reduction slot
+// Per reduction slot.  A plane is reduced in tiles of 64 x 4 cells, one partial
+// each, so this bounds a plane (a 2-D grid) at about 2^24 cells.
+constexpr int kMaxPartials = 65536;
 
 // ---------------------------------------------------------------- reduce
 

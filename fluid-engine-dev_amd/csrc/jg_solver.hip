@@ -1080,7 +1080,7 @@ int jg_solver_set_levels(jg_solver* s, size_t nlev, const size_t* nx, const size
         // one reduction partial per stencil tile of a plane at least
         const Geo g = geo_stencil(Dims{(int)nx[l], (int)ny[l], (int)nz[l]});
         if ((long)g.grid.x * g.grid.y * g.grid.z > kMaxPartials) {
-            s->err = "level plane too large for the reduction partials (x*y beyond 2^20 cells)";
+            s->err = "level plane too large for the reduction partials (more than 65536 tiles of 64 x 4 cells)";
             return JG_ERR_UNSUPPORTED;
         }
     }

@@ -9,7 +9,7 @@
 // compressed form is dimension-free.  Semantics as FdmGpuCgSolver3: x is
 // zeroed and overwritten, the bool result is Jet's, a size mismatch throws
 // std::invalid_argument, a device failure std::runtime_error.  The library's
-// stencil reductions bound one plane to 2^20 cells, so larger 2-D grids are
+// stencil reductions bound one plane to about 2^24 cells, so larger 2-D grids are
 // rejected (std::runtime_error); their compressed form has no such bound.
 #pragma once
 

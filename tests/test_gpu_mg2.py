@@ -104,7 +104,8 @@ def test_vcycle2_symmetric_positive_linear(solver, precision, tol):
     assert np.array_equal(apply(r1), z1)
 
 
-@pytest.mark.parametrize("shape,levels", [((128, 128), 6), ((1024, 1024), 8), ((320, 96), 4)])
+@pytest.mark.parametrize("shape,levels", [((128, 128), 6), ((1024, 1024), 8), ((320, 96), 4),
+                                          ((2048, 2048), 8)])
 @pytest.mark.parametrize("precision", [jetgpu.JG_FP64, jetgpu.JG_MIXED])
 def test_mgpcg2_matches_oracle_solution(solver, oracle, shape, levels, precision):
     # Random SPD one-plane hierarchy, solved to 1e-10 relative residual: the
@@ -141,3 +142,11 @@ def test_mgpcg2_jet_kat(solver):
         print(f"mgpcg2 KAT precision={prec}: {info.iterations} iterations, "
               f"residual {info.last_residual:.3e}", flush=True)
         assert info.converged and info.last_residual <= 1e-4 and info.iterations < 200
+
+
+def test_plane_limit(solver):
+    # one plane is reduced in 64 x 4 tiles with at most 65536 partials:
+    # 4096^2 is accepted, 8192 x 4096 is rejected before anything is allocated
+    solver.set_levels([(4096, 4096, 1), (2048, 2048, 1)])
+    with pytest.raises(jetgpu.JgError):
+        solver.set_levels([(8192, 4096, 1)])
