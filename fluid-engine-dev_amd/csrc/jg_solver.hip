@@ -250,19 +250,33 @@ struct MgCfg {
 // slots 0 and 1 (0 if it wrote none).
 // The V-cycle on a 2-D hierarchy (FdmMgLinearSystem2): the same symmetric
 // cycle as vcycle() below, with FdmMgUtils2's transfers as separate
-// launches and single-sweep launches only (the rows have front = 0, so the
-// 3-D sweeps and residual give the 2-D values: the absent z terms add +0).
+// launches and one-plane sweeps (k_rb_sweep_plane, Jet's 2-D relax); the
+// rows have front = 0, so the 3-D residual gives the 2-D values (the absent
+// z terms add +0).
 template <typename T>
 int vcycle2d(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl) {
     const size_t L = s->lv.size();
     auto B = [&](size_t l) -> const T* { return l == 0 ? r0 : Prec<T>::b(s->lv[l]); };
     auto X = [&](size_t l) -> T* { return l == 0 ? z0 : Prec<T>::x(s->lv[l]); };
     const int fwd = (int)std::max(1u, (c.coarsest + 1) / 2);
+    // one sweep of a one-plane level: k_rb_sweep_plane, 60 x 12 interiors
     auto sweep = [&](size_t l, const T* in, T* out, int first) {
         Level& lv = s->lv[l];
-        const int kind = in ? kSweepPlain : kSweepZero;
-        launch_sweep<T>(s, Prec<T>::A(lv), B(l), in, out, lv.d, first, nullptr, nullptr, nullptr,
-                        c.omega, ctl, Prec<T>::zc(lv, kind), Prec<T>::sh(lv, kind));
+        const Dims& d = lv.d;
+        const dim3 grid((d.nx + kSwIX - 1) / kSwIX, (d.ny + kSwIY - 1) / kSwIY, 1);
+        const dim3 block(kSwLX, kSwLY, 1);
+        const Row<T>* A = Prec<T>::A(lv);
+        if (first == 0) {
+            if (in)
+                k_rb_sweep_plane<T, 0, false><<<grid, block, 0, s->stream>>>(A, B(l), in, out, d, c.omega, ctl);
+            else
+                k_rb_sweep_plane<T, 0, true><<<grid, block, 0, s->stream>>>(A, B(l), in, out, d, c.omega, ctl);
+        } else {
+            if (in)
+                k_rb_sweep_plane<T, 1, false><<<grid, block, 0, s->stream>>>(A, B(l), in, out, d, c.omega, ctl);
+            else
+                k_rb_sweep_plane<T, 1, true><<<grid, block, 0, s->stream>>>(A, B(l), in, out, d, c.omega, ctl);
+        }
     };
     // Launch m of level l's pre + post sweeps writes buffer ((M-1-m) even ? x : y),
     // so the last lands in x; the prolonged correction is added to the

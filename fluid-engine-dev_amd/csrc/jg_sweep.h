@@ -388,4 +388,77 @@ __global__ __launch_bounds__(kSwLX * kSwLY / RP, sweep_waves_per_eu(sizeof(T), R
     }
 }
 
+// One red-black sweep of a one-plane (2-D) level: FdmGaussSeidelSolver2::
+// relaxRedBlack (src/jet/fdm_gauss_seidel_solver2.cpp:147-187), both colours
+// in one pass.  The z-marching k_rb_sweep would spend most of its pipeline on
+// filling and draining for a single plane; here a block loads a 64 x 16 tile
+// of x and the couplings once into LDS (2-cell halo, recomputed), updates the
+// first colour on the interior + 1-cell ring, then the second colour on the
+// interior, and writes the interior.  Same conventions as k_rb_sweep: FIRST
+// is the colour updated first, ZERO treats xin as zero, zero-centre rows are
+// set to zero, FP64 uses Jet's expression and FP32 the w = omega / c form.
+// The neighbour sum is Jet's 2-D one, (((-x) + (+x)) + (-y)) + (+y), with
+// the terms outside the grid masked as Jet masks them.
+template <typename T, int FIRST, bool ZERO>
+__global__ __launch_bounds__(kSwLX * kSwLY) void k_rb_sweep_plane(
+    const Row<T>* __restrict__ A, const T* __restrict__ b, const T* __restrict__ xin,
+    T* __restrict__ xout, Dims d, T omega, const Ctl* __restrict__ ctl) {
+#pragma clang fp contract(off)
+    constexpr bool kWeighted = sizeof(T) == 4;
+    if (ctl && ctl->done) return;
+    __shared__ T sx[kSwLY][kSwLX];
+    __shared__ T sr[kSwLY][kSwLX];
+    __shared__ T su[kSwLY][kSwLX];
+    const int tx = threadIdx.x, ly = threadIdx.y;
+    int bx, by, bz;
+    xcd_tile(bx, by, bz);
+    const int gx = bx * kSwIX - kSwH + tx, gy = by * kSwIY - kSwH + ly;
+    const bool ok = gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny;
+    const idx_t c = ok ? gx + (idx_t)d.nx * gy : 0;
+    T xv = T(0), cc = T(0), bb = T(0);
+    if (ok) {
+        const Row<T> row = A[c];
+        if (!ZERO) xv = xin[c];
+        cc = row.c;
+        bb = b[c];
+        sr[ly][tx] = row.r;
+        su[ly][tx] = row.u;
+        if (kWeighted) cc = cc == T(0) ? T(0) : omega / cc;  // w = omega / c
+    } else {
+        sr[ly][tx] = T(0);
+        su[ly][tx] = T(0);
+    }
+    sx[ly][tx] = xv;
+    __syncthreads();
+    const bool hl = gx > 0, hr = gx + 1 < d.nx, hd = gy > 0, hu = gy + 1 < d.ny;
+    auto relax = [&](T xo) __attribute__((always_inline)) {
+        T s = (hl ? sr[ly][tx - 1] * sx[ly][tx - 1] : T(0)) +
+              (hr ? sr[ly][tx] * sx[ly][tx + 1] : T(0));
+        s += hd ? su[ly - 1][tx] * sx[ly - 1][tx] : T(0);
+        s += hu ? su[ly][tx] * sx[ly + 1][tx] : T(0);
+        return kWeighted ? (T(1) - omega) * xo + cc * (bb - s)
+                         : (T(1) - omega) * xo + omega * (bb - s) / cc;
+    };
+    const bool mine = ((gx + gy) & 1) == FIRST;
+    // first colour: interior and the 1-cell ring (reads only the other colour)
+    if (ok && mine && tx >= 1 && tx < kSwLX - 1 && ly >= 1 && ly < kSwLY - 1) {
+        T v;
+        if (cc == T(0))
+            v = T(0);  // zero-centre row
+        else if (ZERO)
+            v = kWeighted ? cc * (bb - T(0)) : omega * (bb - T(0)) / cc;
+        else
+            v = relax(xv);
+        xv = v;
+        sx[ly][tx] = v;
+    }
+    __syncthreads();
+    // second colour on the interior (reads the updated first colour); the
+    // interior of both colours is written
+    if (ok && tx >= kSwH && tx < kSwLX - kSwH && ly >= kSwH && ly < kSwLY - kSwH) {
+        if (!mine) xv = cc == T(0) ? T(0) : relax(xv);
+        xout[c] = xv;
+    }
+}
+
 }  // namespace jg
