@@ -150,3 +150,23 @@ def test_plane_limit(solver):
     solver.set_levels([(4096, 4096, 1), (2048, 2048, 1)])
     with pytest.raises(jetgpu.JgError):
         solver.set_levels([(8192, 4096, 1)])
+
+
+@pytest.mark.parametrize("precision", [jetgpu.JG_FP64, jetgpu.JG_MIXED])
+def test_mg_solve2_jet_kat(solver, oracle, precision):
+    # FdmMgSolver2.Solve (src/tests/unit_tests/fdm_mg_solver2_tests.cpp:13-62):
+    # one V-cycle from x = 0 with FdmMgSolver2(6, 5, 5, 20, 20) on Jet's test
+    # system reduces the residual norm; here the GPU V-cycle of the same
+    # hierarchy (z = M b is x after one cycle from zero).
+    s = mg_poisson_hierarchy2(4, 6)
+    upload(solver, s)
+    b = s.b[0]
+    gp = jetgpu.mg_params(max_levels=6, restrict_iter=5, correct_iter=5, coarsest_iter=20,
+                          final_iter=20, max_tol=1e-9)
+    z = empty(b.shape)
+    solver.mg_vcycle(gp, jetgpu.gpu_options(precision=precision), dev(b), z)
+    x = host(z)
+    norm0 = np.linalg.norm(b)
+    norm1 = np.linalg.norm(oracle.residual(s.A[0], x, b))
+    print(f"mg_solve2 KAT precision={precision}: |r| {norm0:.3e} -> {norm1:.3e}", flush=True)
+    assert norm1 < norm0
