@@ -451,6 +451,31 @@ __global__ __launch_bounds__(NT) void k_correct(const TX* __restrict__ coarse,
     fine[f] = v;
 }
 
+// r = b - A x on a one-plane (2-D) level, one thread per cell, in
+// FdmBlas3::residual's operand order without the z terms (src/jet/
+// fdm_linear_system2.cpp's FdmBlas2::residual has the same four): the
+// z-marching k_residual would spend most of its pipeline on a single plane.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_residual_plane(const Row<T>* __restrict__ A,
+                                                       const T* __restrict__ x,
+                                                       const T* __restrict__ b,
+                                                       T* __restrict__ r, Dims d,
+                                                       const Ctl* __restrict__ ctl) {
+#pragma clang fp contract(off)
+    if (ctl && ctl->done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= d.nx || j >= d.ny) return;
+    const idx_t c = i + (idx_t)d.nx * j;
+    const Row<T> row = A[c];
+    T rv = b[c] - row.c * x[c];
+    rv -= i > 0 ? A[c - 1].r * x[c - 1] : T(0);
+    rv -= i + 1 < d.nx ? row.r * x[c + 1] : T(0);
+    rv -= j > 0 ? A[c - d.nx].u * x[c - d.nx] : T(0);
+    rv -= j + 1 < d.ny ? row.u * x[c + d.nx] : T(0);
+    r[c] = rv;
+}
+
 // FdmMgUtils2::restrict (src/jet/fdm_mg_linear_system2.cpp:44-84) on one
 // plane: the 3-D taps and clamps in x and y, the 16 products summed y-major
 // with each weight kernel[x] * kernel[y] formed first.  One thread per

@@ -250,9 +250,8 @@ struct MgCfg {
 // slots 0 and 1 (0 if it wrote none).
 // The V-cycle on a 2-D hierarchy (FdmMgLinearSystem2): the same symmetric
 // cycle as vcycle() below, with FdmMgUtils2's transfers as separate
-// launches and one-plane sweeps (k_rb_sweep_plane, Jet's 2-D relax); the
-// rows have front = 0, so the 3-D residual gives the 2-D values (the absent
-// z terms add +0).
+// launches, one-plane sweeps (k_rb_sweep_plane, Jet's 2-D relax) and a
+// one-plane residual (k_residual_plane).
 template <typename T>
 int vcycle2d(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl) {
     const size_t L = s->lv.size();
@@ -309,9 +308,9 @@ int vcycle2d(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl
         const Dims& cd = s->lv[l + 1].d;
         const Geo gc = geo_planes(cd);
         if (in) {
-            const Geo gf = geo_stencil(d);
-            k_residual<T, T, T, false, false, true><<<gf.grid, gf.block, 0, s->stream>>>(
-                Prec<T>::A(lv), in, B(l), Prec<T>::t(lv), d, gf.zc, nullptr, ctl);
+            const Geo gf = geo_planes(d);
+            k_residual_plane<T><<<gf.grid, gf.block, 0, s->stream>>>(Prec<T>::A(lv), in, B(l),
+                                                                    Prec<T>::t(lv), d, ctl);
             k_restrict2<T><<<gc.grid, gc.block, 0, s->stream>>>(Prec<T>::t(lv),
                                                                 Prec<T>::b(s->lv[l + 1]), cd, ctl);
         } else {
