@@ -30,12 +30,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = ("256³ MGPCG solve ms + HBM GB/s: mixed FP32/FP64 vs FP64, rel err ≤1e-6")
 HBM_PEAK_GBS = 8000.0
-# Measured HBM bytes per launch of the level-0 smoothing kernels (rocprofv3
-# FETCH_SIZE / WRITE_SIZE passes, scaled by a copy kernel of known size).
-TRAFFIC_FILES = {
-    "sweep": os.path.join(ROOT, "profiles", "traffic_sweep_fp32_256.json"),
-    "sweep2": os.path.join(ROOT, "profiles", "traffic_sweep2_fp32_256.json"),
-}
+# Counter-measured HBM bytes (rocprofv3 FETCH_SIZE / WRITE_SIZE passes over
+# `bench.py --only traffic-probe`, scaled by a copy kernel of known size):
+# written by `bench.py --collect-traffic`, kept as a summary under profiles/.
+TRAFFIC_FILE_REL = "profiles/traffic_mgpcg_sphere_pool_256.json"
+TRAFFIC_FILE = os.path.join(ROOT, TRAFFIC_FILE_REL)
 # Algorithmic bytes per cell of one MGPCG iteration (4 matrix coefficients
 # per row, compulsory vector traffic, every level of a V(2,2) cycle, the
 # outer iteration's SpMV, updates and dots): FP64 467 B, FP32 inner
@@ -60,14 +59,24 @@ def main():
                     help="SOR factor of the red-black smoother (1.3 needs the fewest "
                          "iterations on this workload: 13 against 15 at 1.0 and 1.5)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / oracle leg")
-    ap.add_argument("--only", choices=["compressed", "diffusion", "mgpcg_2d"],
+    ap.add_argument("--only", choices=["compressed", "diffusion", "mgpcg_2d", "traffic-probe"],
                     help="run one secondary leg alone, GPU only (for a per-leg kernel trace), "
-                         "and print its JSON")
+                         "and print its JSON; traffic-probe: the program the counter passes of "
+                         "--collect-traffic run")
+    ap.add_argument("--collect-traffic", action="store_true",
+                    help="measure HBM traffic with two rocprofv3 counter passes and write the "
+                         "summary to gpurun_out/traffic/")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(multi-GPU sharding is a design note, not a measured path)"}))
         sys.exit(2)
+    if args.collect_traffic:
+        collect_traffic(args)
+        return
+    if args.only == "traffic-probe":
+        traffic_probe(args)
+        return
     if args.only:
         import jetgpu as jg
 
@@ -134,30 +143,23 @@ def main():
         log(f"bench: {name} {wall_ms:.3f} ms/solve, {infos[-1].iterations} it, "
             f"rel {infos[-1].rel_residual:.2e}")
 
-    # Dominant kernel of the mixed solve: the finest level's smoothing, two
-    # launches per PCG iteration of the two-sweep kernel when the tuner picked
-    # it (it does wherever one launch beats two single sweeps), else four of
-    # the single fused sweep.  HIP events on the solver's stream around 50
-    # back-to-back launches.
+    # Dominant kernel of the mixed solve: the finest level's smoothing.  A
+    # V(2,2) cycle on level 0 is two launches of it (the tuner pairs the two
+    # sweeps of each side into one launch wherever that is faster): the
+    # pre-smoothing launch (x from zero, the residual written for the
+    # restriction) and the post-smoothing launch (the coarse correction added
+    # as x is loaded).  jg_solver_time_kernel replays exactly those launches,
+    # with the solve's sweep plan and tuning; HIP events on the solver's
+    # stream around 50 back-to-back launches.
     kern = {}
-    for kname, kid in (("sweep2", jg.JG_KERNEL_SWEEP2), ("sweep", jg.JG_KERNEL_SWEEP),
-                       ("rbgs", jg.JG_KERNEL_RBGS), ("dir_spmv", jg.JG_KERNEL_DIR_SPMV),
+    for kname, kid in (("pre0", jg.JG_KERNEL_PRE0), ("post0", jg.JG_KERNEL_POST0),
+                       ("sweep", jg.JG_KERNEL_SWEEP), ("dir_spmv", jg.JG_KERNEL_DIR_SPMV),
                        ("residual", jg.JG_KERNEL_RESIDUAL), ("update_xr", jg.JG_KERNEL_UPDATE_XR),
                        ("copy", jg.JG_KERNEL_COPY)):
         ms, nbytes = solver.time_kernel(kid, jg.JG_MIXED, 50)
         kern[kname] = {"avg_ms": ms, "bytes": nbytes, "gbs": nbytes / ms / 1e6}
-    two = kern["sweep2"]["avg_ms"] < 2.0 * kern["sweep"]["avg_ms"]
-    dom_name = "sweep2" if two else "sweep"
-    dom = kern[dom_name]
-    traffic = None
-    if os.path.exists(TRAFFIC_FILES[dom_name]):
-        with open(TRAFFIC_FILES[dom_name]) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
-    dom_desc = ("k_rb_sweep2<float>: two red-black sweeps in one pass, finest level; "
-                "algorithmic bytes = 2 sweeps x 28 B/cell" if two else
-                "k_rb_sweep<float>: one red-black sweep (both colours), finest level; "
-                "algorithmic bytes = 28 B/cell")
-
+    traffic = load_traffic(n)
+    roof = roofline(kern, traffic)
     n_cells = n ** 3
     out = {
         "metric": METRIC,
@@ -184,10 +186,7 @@ def main():
         "cells_per_s": {k: n_cells / (res[k]["ms"] * 1e-3) for k in res},
         "kernels_mixed": {k: {"avg_us": round(vv["avg_ms"] * 1e3, 2), "alg_gbs": round(vv["gbs"], 1)}
                           for k, vv in kern.items()},
-        "roofline": {"bound": "hbm", "kernel": dom_desc,
-                     "achieved": round(dom["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes": dom["bytes"], "avg_us": round(dom["avg_ms"] * 1e3, 2)},
+        "roofline": roof,
         "solve_alg_gbs": {
             "fp64": round(ALG_BYTES_PER_CELL["fp64"] * n_cells * res["fp64"]["iterations"]
                           / (res["fp64"]["ms"] * 1e-3) / 1e9, 1),
@@ -195,6 +194,18 @@ def main():
                             + ALG_BYTES_PER_CELL["refine"] * res["mixed"]["refinements"])
                            * n_cells / (res["mixed"]["ms"] * 1e-3) / 1e9, 1)},
     }
+    if traffic and traffic.get("solve"):
+        # the metric's HBM GB/s: counter-measured bytes of one whole solve
+        # (every kernel of it, from the FETCH_SIZE / WRITE_SIZE passes) over
+        # the solve time measured here
+        out["solve_hbm"] = {
+            k: {"bytes": traffic["solve"][k]["hbm_bytes"],
+                "gbs": round(traffic["solve"][k]["hbm_bytes"] / (res[k]["ms"] * 1e-3) / 1e9, 1),
+                "frac": round(traffic["solve"][k]["hbm_bytes"] / (res[k]["ms"] * 1e-3) / 1e9
+                              / HBM_PEAK_GBS, 4),
+                "dispatches": traffic["solve"][k]["dispatches"]}
+            for k in ("fp64", "mixed") if k in traffic["solve"]}
+        out["solve_hbm"]["source"] = TRAFFIC_FILE_REL
 
     out["system_build"] = {"gpu_ms": round(build_ms, 3), "levels": L,
                            "what": "GridSinglePhasePressureSolver3 markers and A on every level, "
@@ -215,6 +226,217 @@ def main():
         out["mgpcg_2d"].update(mg2_cpu_leg(mg2))
     print(json.dumps(out))
     solver.close()
+
+
+def load_traffic(n):
+    """The counter-measured traffic summary of this workload, if it is there
+    and was measured at this size."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return t if t.get("n") == n else None
+
+
+def roofline(kern, traffic):
+    """Roofline of the dominant kernel: the level-0 smoothing launches of one
+    V-cycle (pre and post), per launch.  achieved = algorithmic bytes per
+    launch / average launch duration (both measured here); traffic = the
+    counter-measured HBM bytes per launch of the same two launches."""
+    pre, post = kern["pre0"], kern["post0"]
+    t_ms = (pre["avg_ms"] + post["avg_ms"]) / 2.0
+    alg = (pre["bytes"] + post["bytes"]) / 2.0
+    achieved = alg / t_ms / 1e6
+    names = None
+    hbm = None
+    if traffic and traffic.get("launches"):
+        tl = traffic["launches"]
+        names = " + ".join(tl[k]["kernel"] for k in ("pre0", "post0"))
+        hbm = (tl["pre0"]["hbm_bytes"] + tl["post0"]["hbm_bytes"]) / 2.0
+    roof = {
+        "bound": "hbm",
+        "kernel": names or "level-0 smoothing launches of one V(2,2) cycle (pre, post)",
+        "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": round(hbm) if hbm else None,
+        "per": "launch, averaged over the pre- and the post-smoothing launch of level 0",
+        "algorithmic_bytes": alg, "avg_us": round(t_ms * 1e3, 2),
+        "launches": {k: {"avg_us": round(kern[k]["avg_ms"] * 1e3, 2),
+                         "algorithmic_bytes": kern[k]["bytes"],
+                         "alg_bytes_per_cell": round(kern[k]["bytes"] / traffic_cells(traffic), 3)
+                         if traffic else None}
+                     for k in ("pre0", "post0")},
+    }
+    if hbm:
+        roof["measured_gbs"] = round(hbm / t_ms / 1e6, 1)
+        roof["measured_frac"] = round(hbm / t_ms / 1e6 / HBM_PEAK_GBS, 4)
+        roof["traffic_source"] = TRAFFIC_FILE_REL
+    return roof
+
+
+def traffic_cells(traffic):
+    return float(traffic["n"]) ** 3
+
+
+def traffic_probe(args):
+    """The program the counter passes run: the workload's system, warm-up
+    solves (tuning), then, each between two marker kernels (a torch fill that
+    the library never launches): one mixed solve, one FP64 solve, the pre- and
+    the post-smoothing launch of level 0 and a copy kernel (the counters'
+    scale), each 1 + 20 times."""
+    import torch
+
+    import jetgpu as jg
+
+    n, L = args.n, args.levels
+    solver = jg.Solver(0)
+    solver.set_levels([(n >> l,) * 3 for l in range(L)])
+    f64 = torch.float64
+    u = torch.empty((n, n, n + 1), dtype=f64, device="cuda")
+    v = torch.empty((n, n + 1, n), dtype=f64, device="cuda")
+    w = torch.empty((n + 1, n, n), dtype=f64, device="cuda")
+    bs = torch.empty((n, n, n), dtype=f64, device="cuda")
+    fs = torch.empty((n, n, n), dtype=f64, device="cuda")
+    mark = torch.zeros(64, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    solver.generate_scene(jg.scene_desc(jg.JG_SCENE_SPHERE_POOL, n, seed=args.seed), u, v, w, bs, fs)
+    solver.sp_build(u, v, w, bs, fs, 1.0)
+    del u, v, w, bs, fs
+    params = jg.mg_params(max_levels=L, restrict_iter=2, correct_iter=2, coarsest_iter=20,
+                          final_iter=2, sor=args.omega)
+    opts = {k: jg.gpu_options(precision=p, stop_rule=jg.JG_STOP_RELATIVE, rel_tol=args.rel_tol)
+            for k, p in (("fp64", jg.JG_FP64), ("mixed", jg.JG_MIXED))}
+    for k in ("fp64", "mixed"):
+        solver.mgpcg(500, params, opts[k])
+    reps = TRAFFIC_REPS
+
+    def marker(i):
+        torch.cuda.synchronize()
+        mark.fill_(float(i))
+        torch.cuda.synchronize()
+        log(f"probe: marker {i}")
+
+    marker(1)
+    solver.mgpcg(500, params, opts["mixed"])
+    marker(2)
+    solver.mgpcg(500, params, opts["fp64"])
+    marker(3)
+    solver.time_kernel(jg.JG_KERNEL_PRE0, jg.JG_MIXED, reps)
+    marker(4)
+    solver.time_kernel(jg.JG_KERNEL_POST0, jg.JG_MIXED, reps)
+    marker(5)
+    solver.time_kernel(jg.JG_KERNEL_COPY, jg.JG_MIXED, reps)
+    marker(6)
+    solver.close()
+
+
+TRAFFIC_REPS = 20
+TRAFFIC_SEGMENTS = ("mixed", "fp64", "pre0", "post0", "copy")
+
+
+def read_counters(outdir, counter):
+    """Per-dispatch (kernel name, value in KiB) of one rocprofv3 --pmc pass,
+    in dispatch order (rows of one dispatch summed)."""
+    import csv
+    import glob
+
+    files = glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise RuntimeError(f"no counter_collection.csv under {outdir}")
+    disp = {}
+    for fn in files:
+        with open(fn, newline="") as f:
+            for row in csv.DictReader(f):
+                if row.get("Counter_Name") != counter:
+                    continue
+                key = int(row.get("Dispatch_Id") or row.get("Correlation_Id"))
+                name, val = disp.get(key, (row["Kernel_Name"], 0.0))
+                disp[key] = (name, val + float(row["Counter_Value"]))
+    return [disp[k] for k in sorted(disp)]
+
+
+def segments(disp):
+    """Split dispatches at the marker kernels (torch fills)."""
+    segs, cur, started = [], [], False
+    for name, val in disp:
+        if "FillFunctor" in name:
+            if started:
+                segs.append(cur)
+            cur, started = [], True
+            continue
+        if started:
+            cur.append((name, val))
+    return segs
+
+
+def collect_traffic(args):
+    """Two rocprofv3 counter passes (FETCH_SIZE, then WRITE_SIZE) over the
+    probe program, each a child process; writes the summary JSON (per-launch
+    bytes of the level-0 smoothing launches and whole-solve bytes) to
+    gpurun_out/traffic/."""
+    import subprocess
+    from collections import Counter
+
+    out_root = os.path.join(ROOT, "gpurun_out", "traffic")
+    os.makedirs(out_root, exist_ok=True)
+    env = dict(os.environ, TMPDIR="/tmp")
+    raw = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = os.path.join(out_root, counter.lower())
+        cmd = ["rocprofv3", "--pmc", counter, "-f", "csv", "-d", d, "-o", "probe", "--",
+               sys.executable, os.path.join(ROOT, "bench.py"), "--only", "traffic-probe",
+               "--n", str(args.n), "--levels", str(args.levels)]
+        log("bench: " + " ".join(cmd))
+        subprocess.run(cmd, cwd="/tmp", env=env, check=True, timeout=600)
+        segs = segments(read_counters(d, counter))
+        if len(segs) != len(TRAFFIC_SEGMENTS):
+            raise RuntimeError(f"{counter}: {len(segs)} segments, expected {len(TRAFFIC_SEGMENTS)}")
+        raw[counter] = dict(zip(TRAFFIC_SEGMENTS, segs))
+
+    def repeated(seg):
+        """The kernel a timing segment launches 1 + TRAFFIC_REPS times (one
+        warm launch, then the timed ones), and its average over the timed
+        launches."""
+        names = Counter(nm for nm, _ in seg)
+        name = [nm for nm, c in names.items() if c == TRAFFIC_REPS + 1][0]
+        vals = [v for nm, v in seg if nm == name][1:]
+        return name, sum(vals) / len(vals)
+
+    n = args.n
+    copy_bytes = 4.0 * n ** 3  # k_convert<float, float>: n^3 floats read and written
+    cname, cf = repeated(raw["FETCH_SIZE"]["copy"])
+    _, cw = repeated(raw["WRITE_SIZE"]["copy"])
+    fscale, wscale = copy_bytes / (cf * 1024.0), copy_bytes / (cw * 1024.0)
+    res = {"n": n, "levels": args.levels,
+           "command": "python3 bench.py --collect-traffic (rocprofv3 --pmc FETCH_SIZE, then "
+                      "--pmc WRITE_SIZE, -- python3 bench.py --only traffic-probe)",
+           "copy": {"kernel": cname, "bytes_each_way": copy_bytes, "fetch_kib": cf, "write_kib": cw,
+                    "fetch_scale": round(fscale, 4), "write_scale": round(wscale, 4)},
+           "launches": {}, "solve": {}}
+    for k in ("pre0", "post0"):
+        name, f = repeated(raw["FETCH_SIZE"][k])
+        _, w = repeated(raw["WRITE_SIZE"][k])
+        res["launches"][k] = {"kernel": name, "fetch_kib": f, "write_kib": w,
+                              "hbm_bytes": round(f * 1024.0 * fscale + w * 1024.0 * wscale)}
+    for k in ("mixed", "fp64"):
+        f = raw["FETCH_SIZE"][k]
+        w = raw["WRITE_SIZE"][k]
+        per = {}
+        for (nm, fv), (_, wv) in zip(f, w):
+            b = fv * 1024.0 * fscale + wv * 1024.0 * wscale
+            c, tot = per.get(nm, (0, 0.0))
+            per[nm] = (c + 1, tot + b)
+        total = sum(t for _, t in per.values())
+        top = sorted(per.items(), key=lambda kv: -kv[1][1])[:12]
+        res["solve"][k] = {"dispatches": len(f), "hbm_bytes": round(total),
+                           "top": [{"kernel": nm, "dispatches": c, "hbm_bytes": round(t)}
+                                   for nm, (c, t) in top]}
+    path = os.path.join(out_root, os.path.basename(TRAFFIC_FILE))
+    with open(path, "w") as f:
+        json.dump(res, f, indent=1)
+    log(f"bench: traffic summary written to {path}")
+    print(json.dumps(res))
 
 
 BC_DEPTH = 5  # ceil(max CFL) with GridFluidSolver3's default max CFL of 5

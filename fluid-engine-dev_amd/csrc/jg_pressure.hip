@@ -188,6 +188,11 @@ int jg_sp_build(jg_solver* s, const double* u, const double* v, const double* w,
     if (!s || s->lv.empty() || !u || !v || !w || !boundary_sdf || !fluid_sdf || !(hx > 0) ||
         !(hy > 0) || !(hz > 0))
         return JG_ERR_INVALID_ARG;
+    if (s->twod) {
+        // k_sp_coarsen reads fine planes 2k and 2k+1: a one-plane hierarchy has no plane 1
+        s->err = "jg_sp_build: the resident hierarchy is a 2-D (one-plane) one; use jg_sp_build2";
+        return JG_ERR_INVALID_ARG;
+    }
     JG_HIP(hipSetDevice(s->device));
     const Dims& d0 = s->lv[0].d;
     k_sp_markers<<<nblocks_1d(d0.n()), 256, 0, s->stream>>>(boundary_sdf, fluid_sdf,
@@ -283,6 +288,11 @@ int jg_fr_build(jg_solver* s, const double* u, const double* v, const double* w,
     if (!s || s->lv.empty() || !u || !v || !w || !boundary_sdf_vertex || !fluid_sdf ||
         !(hx > 0) || !(hy > 0) || !(hz > 0))
         return JG_ERR_INVALID_ARG;
+    if (s->twod) {
+        // the 3-D weight restriction reads fine planes 2k-1..2k+2: a one-plane hierarchy has one
+        s->err = "jg_fr_build: the resident hierarchy is a 2-D (one-plane) one";
+        return JG_ERR_INVALID_ARG;
+    }
     JG_HIP(hipSetDevice(s->device));
     for (auto& lv : s->lv) {
         if (lv.fs) continue;

@@ -1245,6 +1245,7 @@ int jg_solver_mgpcg(jg_solver* s, unsigned max_iter, const jg_mg_params* p,
     unsigned iters = 0, refinements = 0;
     double last_res = 0.0;
     double bnorm = 0.0;  // mixed: ||b||, for the relative residual of the final x
+    double mixed_goal = -1.0;  // mixed: the FP64 residual norm the refinement had to reach
     Ctl* ctl = s->ctl;
     if (opt.precision == JG_MIXED) {
         JG_TRY(ensure_f32(s));
@@ -1315,10 +1316,17 @@ int jg_solver_mgpcg(jg_solver* s, unsigned max_iter, const jg_mg_params* p,
                                                    s->d32B, s->q32, l0.d, ictl, &mg},
                                      opt.check_every));
             iters += s->ctl_host->iter;
+            if (s->ctl_host->nonfinite) {
+                // a value beyond FP32 range (or a NaN) in the inner solve: the
+                // correction is meaningless, and x keeps the last finite one
+                status = JG_ERR_NONFINITE;
+                break;
+            }
             k_add_correction<<<nb1, NT, 0, s->stream>>>(s->x, s->e32, n);
             JG_TRY(check_launch(s, "refinement"));
             ++refinements;
         }
+        mixed_goal = goal;
     }
     JG_HIP(hipEventRecord(s->ev1, s->stream));
     JG_TRY(sync(s));
@@ -1330,12 +1338,18 @@ int jg_solver_mgpcg(jg_solver* s, unsigned max_iter, const jg_mg_params* p,
         info->iterations = iters;
         info->refinements = refinements;
         info->last_residual = last_res;
-        info->converged = (last_res <= p->max_tol) || (iters < max_iter);
+        // FP64: Jet's rule (cg-inl.h:93 and fdm_mgpcg_solver3.cpp:78). Mixed: the
+        // outer loop stops on the true residual, so converged means it reached
+        // the goal; running out of refinements or iterations above it is not.
+        info->converged = opt.precision == JG_MIXED ? last_res <= mixed_goal
+                                                    : (last_res <= p->max_tol) || (iters < max_iter);
+        if (status != JG_OK) info->converged = 0;
         if (opt.precision == JG_MIXED && bnorm > 0.0)
             info->rel_residual = last_res / bnorm;  // the FP64 b - A x of the final x
         else
             JG_TRY(true_rel_residual(s, &info->rel_residual));
-        if (opt.stop_rule == JG_STOP_RELATIVE) info->converged = info->rel_residual <= opt.rel_tol;
+        if (opt.stop_rule == JG_STOP_RELATIVE && status == JG_OK)
+            info->converged = info->rel_residual <= opt.rel_tol;
     }
     if (status == JG_ERR_NONFINITE) s->err = "non-finite value in MGPCG reduction";
     return status;
@@ -1428,6 +1442,51 @@ struct Vecs {
 Vecs<double> vecs64(jg_solver* s) { return {s->x, s->b, s->r, s->z, s->dA, s->dB, s->q}; }
 Vecs<float> vecs32(jg_solver* s) { return {s->e32, s->rhs32, s->r32, s->z32, s->d32A, s->d32B, s->q32}; }
 
+// The first level-0 launch of the pre-smoothing (post = false) or of the
+// post-smoothing (post = true) of a V(2,2) cycle, exactly as vcycle() issues
+// it: the sweep plan and tuning of the solve, b = r, the result in z (pre:
+// x starts at zero and, for a two-sweep launch, the residual is written for
+// the restriction; post: the coarse correction of level 1 is added as x is
+// loaded).
+template <typename T>
+int level0_plan(jg_solver* s, bool post) {
+    return sweep_plan<T>(s->lv[0], 2, post ? kSweepCorrect : kSweepZero, 0).front();
+}
+
+template <typename T>
+void level0_launch(jg_solver* s, const Vecs<T>& v, bool post, T omega) {
+    Level& l0 = s->lv[0];
+    const Row<T>* A = Prec<T>::A(l0);
+    const int mode = level0_plan<T>(s, post);
+    const T* xin = post ? Prec<T>::y(l0) : nullptr;
+    const T* coarse = post && s->lv.size() > 1 ? Prec<T>::x(s->lv[1]) : nullptr;
+    const int kind = post ? kSweepCorrect : kSweepZero;
+    if (mode == kLaunchBrick)
+        launch_brick2<T>(s, A, v.r, xin, v.z, l0.d, post ? 1 : 0, coarse, omega, s->ctl);
+    else if (mode == kLaunchMarch)
+        launch_sweep2<T>(s, A, v.r, xin, v.z, l0.d, post ? 1 : 0, coarse,
+                         post ? nullptr : Prec<T>::t(l0), omega, s->ctl, Prec<T>::zc2(l0, kind));
+    else
+        launch_sweep<T>(s, A, v.r, xin, v.z, l0.d, post ? 1 : 0, coarse, nullptr, nullptr, omega,
+                        s->ctl, Prec<T>::zc(l0, kind), Prec<T>::sh(l0, kind));
+}
+
+// Algorithmic bytes per cell of that launch under the 4-coefficient model:
+// a sweep reads A, b and x and writes x; the residual written by a pre-
+// smoothing two-sweep launch reads A, x and b (4 coefficients, 2 vectors)
+// and writes r; the prolongation-correction reads the coarse vector (1/8)
+// and reads and writes x.
+template <typename T>
+double level0_launch_bytes(jg_solver* s, bool post) {
+    const double sa = sizeof(Row<T>), sv = sizeof(T);
+    const int mode = level0_plan<T>(s, post);
+    const double sweeps = mode == kLaunchSingle ? 1.0 : 2.0;
+    double b = sweeps * (sa + 3 * sv);
+    if (post && s->lv.size() > 1) b += 2 * sv + sv / 8;
+    if (!post && mode == kLaunchMarch) b += sa + 3 * sv;
+    return b;
+}
+
 template <typename T>
 int time_kernel_t(jg_solver* s, const Vecs<T>& v, int kernel, unsigned reps, double* avg_ms,
                   double* bytes) {
@@ -1474,6 +1533,10 @@ int time_kernel_t(jg_solver* s, const Vecs<T>& v, int kernel, unsigned reps, dou
                 launch_sweep2<T>(s, A, v.b, v.r, v.x, d, 1, nullptr, nullptr, T(1), s->ctl,
                                  Prec<T>::zc2(l0, kSweepPlain));
                 break;
+            case JG_KERNEL_PRE0:
+            case JG_KERNEL_POST0:
+                level0_launch<T>(s, v, kernel == JG_KERNEL_POST0, T(1.3));
+                break;
             default:
                 k_convert<T, T><<<nb1, NT, 0, s->stream>>>(v.r, v.q, n);
                 break;
@@ -1489,6 +1552,8 @@ int time_kernel_t(jg_solver* s, const Vecs<T>& v, int kernel, unsigned reps, dou
         case JG_KERNEL_DOT: per_cell = 2 * sv; break;
         case JG_KERNEL_SWEEP: per_cell = sa + 3 * sv; break;     // A, b, x -> x (both colours)
         case JG_KERNEL_SWEEP2: per_cell = 2 * (sa + 3 * sv); break;  // two sweeps, each as above
+        case JG_KERNEL_PRE0:
+        case JG_KERNEL_POST0: per_cell = level0_launch_bytes<T>(s, kernel == JG_KERNEL_POST0); break;
         default: per_cell = 2 * sv; break;
     }
     launch();  // warm
@@ -1508,7 +1573,8 @@ int time_kernel_t(jg_solver* s, const Vecs<T>& v, int kernel, unsigned reps, dou
 
 extern "C" int jg_solver_time_kernel(jg_solver* s, int kernel, int precision, unsigned reps,
                                      double* avg_ms, double* bytes) {
-    if (!s || s->lv.empty() || !avg_ms || !bytes || reps == 0 || kernel < 0 || kernel > 7)
+    if (!s || s->lv.empty() || !avg_ms || !bytes || reps == 0 || kernel < 0 ||
+        kernel > JG_KERNEL_POST0)
         return JG_ERR_INVALID_ARG;
     if (kernel == JG_KERNEL_SWEEP2 && precision != JG_MIXED) {
         s->err = "the two-sweep kernel is FP32 only";
@@ -2060,6 +2126,11 @@ int jg_batch_mgpcg(jg_batch* B, unsigned max_iter, const jg_mg_params* p, const 
                 if (done[i]) continue;
                 jg_solver* s = B->sys[i];
                 iters[i] += (unsigned)B->hflags[2 * i + 1];
+                if (B->hflags[2 * i] & 2) {  // the inner FP32 solve met a non-finite value
+                    status[i] = JG_ERR_NONFINITE;
+                    done[i] = 1;
+                    continue;
+                }
                 const Dims& d0 = s->lv[0].d;
                 k_add_correction<<<nblocks_1d(d0.n()), NT, 0, s->stream>>>(s->x, s->e32, d0.n());
                 ++refinements[i];
@@ -2088,8 +2159,14 @@ int jg_batch_mgpcg(jg_batch* B, unsigned max_iter, const jg_mg_params* p, const 
             last[i] = std::sqrt(std::fabs(s->ctl_host->sigma));
         }
         info.last_residual = last[i];
-        info.converged = opt.stop_rule == JG_STOP_RELATIVE ? info.rel_residual <= opt.rel_tol
-                                                           : (last[i] <= p->max_tol || iters[i] < max_iter);
+        if (status[i] != JG_OK)
+            info.converged = 0;
+        else if (opt.stop_rule == JG_STOP_RELATIVE)
+            info.converged = info.rel_residual <= opt.rel_tol;
+        else if (mixed)
+            info.converged = last[i] <= p->max_tol;  // the outer loop's goal
+        else
+            info.converged = last[i] <= p->max_tol || iters[i] < max_iter;
     }
     if (worst == JG_ERR_NONFINITE) B->err = "non-finite value in a batched solve";
     return worst;

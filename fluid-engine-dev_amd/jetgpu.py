@@ -22,7 +22,7 @@ JG_FP64, JG_MIXED = 0, 1
 JG_STOP_JET, JG_STOP_RELATIVE = 0, 1
 JG_SMOOTH_RBGS = 0
 (JG_KERNEL_RBGS, JG_KERNEL_DIR_SPMV, JG_KERNEL_RESIDUAL, JG_KERNEL_UPDATE_XR, JG_KERNEL_DOT,
- JG_KERNEL_COPY, JG_KERNEL_SWEEP, JG_KERNEL_SWEEP2) = range(8)
+ JG_KERNEL_COPY, JG_KERNEL_SWEEP, JG_KERNEL_SWEEP2, JG_KERNEL_PRE0, JG_KERNEL_POST0) = range(10)
 (JG_SCENE_CLOSED_SPHERE, JG_SCENE_DAM_BREAK, JG_SCENE_SPHERE_POOL, JG_SCENE_BATCH_MEMBER,
  JG_SCENE_TORUS_POOL) = 1, 2, 3, 4, 5
 

@@ -257,8 +257,14 @@ enum {
     JG_KERNEL_DOT = 4,        /* partial r.z                                   */
     JG_KERNEL_COPY = 5,       /* plain copy of one vector (bandwidth reference) */
     JG_KERNEL_SWEEP = 6,      /* fused red-black sweep, both colours           */
-    JG_KERNEL_SWEEP2 = 7      /* two fused sweeps in one pass (JG_MIXED only);
+    JG_KERNEL_SWEEP2 = 7,     /* two fused sweeps in one pass (JG_MIXED only);
                                  bytes = two sweeps' algorithmic bytes          */
+    JG_KERNEL_PRE0 = 8,       /* the first level-0 pre-smoothing launch of a
+                                 V(2,2) cycle as the MGPCG solve issues it (its
+                                 sweep plan and tuning, omega 1.3); bytes = its
+                                 sweeps plus the residual it writes           */
+    JG_KERNEL_POST0 = 9       /* the first level-0 post-smoothing launch (the
+                                 coarse correction added as x is loaded)      */
 };
 int jg_solver_time_kernel(jg_solver* s, int kernel, int precision,
                           unsigned reps, double* avg_ms, double* bytes);

@@ -106,3 +106,23 @@ def test_batch_64_uneven_systems(gpu):
             assert rel_err(c, a) <= 1e-6
     finally:
         b.close()
+    # one member of each size against the oracle's FP64 solution of the same
+    # system (host copy of the same generated scene, clean-mode MGPCG to 1e-12)
+    from oracle.jet_oracle import MgParams, Oracle
+
+    oracle = Oracle()
+    first = {}
+    for i, n in enumerate(sizes):
+        first.setdefault(n, i)
+    for n, i in sorted(first.items()):
+        scene = jetgpu.generate_scene_host(jetgpu.scene_desc(jetgpu.JG_SCENE_BATCH_MEMBER, n,
+                                                             seed=seeds[i]))
+        _, sys_ = oracle.sp_build_mg(scene["u"], scene["v"], scene["w"], (1.0,) * 3, (0.0,) * 3,
+                                     scene["bsdf"], scene["fsdf"], levels[i])
+        ok, _, rel = oracle.mgpcg_clean_solve(sys_, MgParams.make(max_levels=levels[i], **PARAMS),
+                                              300, 1e-12)
+        assert ok and rel <= 1e-12
+        for precision in (jetgpu.JG_FP64, jetgpu.JG_MIXED):
+            err = rel_err(res[precision][i], sys_.x[0])
+            print(f"  member {i} n={n} precision={precision} err_vs_oracle={err:.2e}", flush=True)
+            assert err <= 1e-6, (i, n, precision)

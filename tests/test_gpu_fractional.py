@@ -6,9 +6,8 @@ Face weights and restricted fields are float computations and the matrix and
 right-hand side FP64 computations done with the same IEEE operations in the
 same order as the oracle, so they must be bit-identical on every level.
 Solutions follow the solver parity bar: relative residual <= 1e-8 and
-||p - p_ref|| / ||p_ref|| <= 1e-6.  At the 512^3 size of the torus workload
-the oracle is too slow for a test; there the FP64 and mixed-precision GPU
-solutions must agree to 1e-6 and both reach 1e-8 relative residual.
+||p - p_ref|| / ||p_ref|| <= 1e-6, also at the 512^3 size of the torus
+workload (the oracle's solve there takes about a minute of host time).
 """
 import numpy as np
 import pytest
@@ -129,11 +128,11 @@ def test_fr_mgpcg_torus_matches_oracle(solver, oracle, precision):
     assert rel_err(x, sys_.x[0]) <= 1e-6
 
 
-def test_fr_torus_512_mixed_vs_fp64(solver):
+def test_fr_torus_512_matches_oracle(solver):
     # The variational 512^3 workload (torus R = 150, r = 50 cells around the
     # y axis, liquid below 0.35 n), 7 levels, generated and assembled on the
-    # device.  Size-independent checks: both precisions reach 1e-8 relative
-    # residual and agree to 1e-6.
+    # device.  Both precisions reach 1e-8 relative residual, agree to 1e-6,
+    # and are within 1e-6 of the oracle's FP64 solution.
     import torch
 
     n, levels = 512, 7
@@ -166,3 +165,27 @@ def test_fr_torus_512_mixed_vs_fp64(solver):
         assert info.rel_residual <= 1e-8
         xs[prec] = solver.download_x()
     assert rel_err(xs[jetgpu.JG_MIXED], xs[jetgpu.JG_FP64]) <= 1e-6
+    # the FP64-oracle error of both: the oracle's weights, system and
+    # clean-mode MGPCG (1e-11 relative residual) from the host copy of the
+    # same generated scene
+    import time
+
+    from oracle.jet_oracle import MgParams, Oracle
+
+    t0 = time.perf_counter()
+    oracle = Oracle()
+    sc = jetgpu.generate_scene_host(d, vertex=True)
+    _, sys_ = oracle.fr_build_mg(sc["u"], sc["v"], sc["w"], (1.0,) * 3, (0.0,) * 3, sc["vertex"],
+                                 sc["fsdf"], levels)
+    del sc
+    print(f"  oracle build {time.perf_counter() - t0:.1f} s", flush=True)
+    ok, it, rel = oracle.mgpcg_clean_solve(sys_, MgParams.make(max_levels=levels, restrict_iter=2,
+                                                                correct_iter=2, coarsest_iter=20,
+                                                                final_iter=2, sor=1.3), 300, 1e-11)
+    print(f"  oracle solve {it} iterations rel={rel:.1e} at {time.perf_counter() - t0:.1f} s",
+          flush=True)
+    assert ok and rel <= 1e-11
+    for prec, x in xs.items():
+        err = rel_err(x, sys_.x[0])
+        print(f"  precision={prec} err_vs_oracle={err:.2e}", flush=True)
+        assert err <= 1e-6

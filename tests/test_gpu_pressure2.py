@@ -96,6 +96,17 @@ def test_sp_build2_rejects_3d_hierarchy(solver):
         solver.sp_build2(z, z, z, z, (1.0, 1.0))
 
 
+def test_3d_builds_reject_2d_hierarchy(solver):
+    # the 3-D coarsening reads fine planes 2k and 2k+1; a one-plane hierarchy must be refused
+    # before any launch
+    solver.set_levels([(8, 8, 1), (4, 4, 1)])
+    z = empty((9 * 9 * 2,))
+    with pytest.raises(RuntimeError):
+        solver.sp_build(z, z, z, z, z, 1.0)
+    with pytest.raises(RuntimeError):
+        solver.fr_build(z, z, z, z, z, 1.0)
+
+
 @pytest.mark.parametrize("n,seed", [((64, 32), 11), ((96, 96), 12), ((7, 5), 13), ((1024, 512), 14)])
 def test_sp_build_compressed2_matches_oracle(solver, oracle, n, seed):
     print(f"case sp_build_compressed2 n={n}", flush=True)

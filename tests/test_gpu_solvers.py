@@ -188,3 +188,31 @@ def test_mgpcg_nonfinite_rhs_is_an_error(solver):
     upload_mg(solver, s)
     with pytest.raises(jetgpu.JgError):
         solver.mgpcg(10, jetgpu.mg_params(max_levels=2))
+
+
+def test_mgpcg_mixed_fp32_overflow_is_an_error(solver):
+    # b inside FP64's range but beyond FP32's: the FP32 inner solve meets inf, and the mixed solve
+    # must report a non-finite value instead of returning x = 0 as converged
+    s = mg_poisson_hierarchy(4, 2)
+    s.b[0][:] = 1e39
+    upload_mg(solver, s)
+    p = jetgpu.mg_params(max_levels=2)
+    with pytest.raises(jetgpu.JgError):
+        solver.mgpcg(20, p, jetgpu.gpu_options(precision=jetgpu.JG_MIXED))
+    info = solver.mgpcg(100, p, jetgpu.gpu_options(precision=jetgpu.JG_FP64,
+                                                    stop_rule=jetgpu.JG_STOP_RELATIVE,
+                                                    rel_tol=1e-9))
+    assert info.converged and info.rel_residual <= 1e-9
+
+
+def test_mgpcg_mixed_refinement_limit_is_not_convergence(solver):
+    # one refinement (inner tolerance 1e-4 relative) cannot reach an absolute 1e-10: Jet's bool
+    # must be false, as an FP64 solve stopped at max_iter would return
+    s = mg_poisson_hierarchy(4, 3)
+    upload_mg(solver, s)
+    p = jetgpu.mg_params(max_levels=3, restrict_iter=2, correct_iter=2, coarsest_iter=20,
+                         final_iter=2, max_tol=1e-10)
+    info = solver.mgpcg(200, p, jetgpu.gpu_options(precision=jetgpu.JG_MIXED, max_refine=1))
+    assert info.refinements == 1 and info.last_residual > 1e-10 and not info.converged
+    info = solver.mgpcg(200, p, jetgpu.gpu_options(precision=jetgpu.JG_MIXED))
+    assert info.last_residual <= 1e-10 and info.converged

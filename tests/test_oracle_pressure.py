@@ -319,3 +319,67 @@ def test_single_phase_mg_hierarchy(oracle):
         inv = 0.25 ** l
         # interior fluid cell below the surface: centre 6/h^2, couplings -1/h^2
         assert A[1, 1, 1, 0] == 6 * inv and A[1, 1, 1, 1] == -inv
+
+
+def test_fractional_moving_collider_uniform_velocity(oracle):
+    # The moving-boundary term (src/jet/grid_fractional_single_phase_pressure_
+    # solver3.cpp:241-255): when every face velocity and the collider's velocity
+    # are the same constant U, the weighted flux w U through a face and the
+    # collider's share (1 - w) U add up to U on every face, so the divergence
+    # cancels: b = 0 on every fluid row (center >= eps) whose faces are all
+    # inside the domain (a wall face's flux is unweighted).  A sign error or
+    # a dropped (1 - w) in the collider term breaks it wherever a cell's
+    # opposite faces have different weights, which the sphere makes common.
+    n = (20, 18, 16)
+    h = (0.5, 0.5, 0.5)
+    vs = vertices(lambda x, y, z: np.sqrt((x - 5.2) ** 2 + (y - 4.1) ** 2 + (z - 3.9) ** 2) - 2.3,
+                  n, 0.5)
+    uw, vw, ww = oracle.fr_face_weights(vs)
+    fsdf = cc(lambda x, y, z: y - 7.0, n, 0.5).astype(np.float32)
+    U = (0.7, -1.3, 0.45)
+    nx, ny, nz = n
+    u = np.full((nz, ny, nx + 1), U[0])
+    v = np.full((nz, ny + 1, nx), U[1])
+    w = np.full((nz + 1, ny, nx), U[2])
+    A, b = oracle.fr_build_system(fsdf, uw, vw, ww, u, v, w, h, u.copy(), v.copy(), w.copy())
+    fluid = (fsdf < 0) & (A[..., 0] >= 1e-12)
+    inner = np.zeros_like(fluid)
+    inner[1:-1, 1:-1, 1:-1] = True
+    rows = fluid & inner
+    # cells whose opposite faces differ in weight: where a wrong term would show
+    dw = ((uw[:, :, 1:] != uw[:, :, :-1]) | (vw[:, 1:, :] != vw[:, :-1, :])
+          | (ww[1:, :, :] != ww[:-1, :, :]))
+    assert np.count_nonzero(rows & dw) >= 20
+    assert np.max(np.abs(b[rows])) <= 1e-13 * max(abs(c) for c in U) / h[0]
+    # without the collider the same rows see the weight jumps as divergence
+    _, b0 = oracle.fr_build_system(fsdf, uw, vw, ww, u, v, w, h)
+    assert np.max(np.abs(b0[rows & dw])) > 1e-3
+
+
+def test_fractional_moving_collider_hand_computed(oracle):
+    # One fluid cell (the middle of 3 x 1 x 1, fluid SDF < 0 everywhere) with
+    # hand-set face weights; b by hand from the formula of buildSingleSystem:
+    # sum over faces of +-(w u + (1 - w) u_b) / h, with the domain-wall faces
+    # (outside the x range here only y and z) unweighted in their flux.
+    fsdf = np.full((1, 1, 3), -1.0, np.float32)
+    uw = np.array([[[1.0, 0.25, 0.75, 1.0]]], np.float32)
+    vw = np.array([[[0.5, 0.5, 0.5], [0.2, 0.2, 0.2]]], np.float32)
+    ww = np.array([[[0.6, 0.6, 0.6]], [[0.9, 0.9, 0.9]]], np.float32)
+    u = np.array([[[0.0, 2.0, -3.0, 0.0]]])
+    v = np.array([[[0.5, 0.5, 0.5], [1.5, 1.5, 1.5]]])
+    w = np.array([[[-0.25, -0.25, -0.25]], [[0.75, 0.75, 0.75]]])
+    bu = np.array([[[0.0, 4.0, 8.0, 0.0]]])
+    bv = np.array([[[1.0, 1.0, 1.0], [-2.0, -2.0, -2.0]]])
+    bw = np.array([[[3.0, 3.0, 3.0]], [[5.0, 5.0, 5.0]]])
+    h = (0.5, 0.25, 2.0)
+    A, b = oracle.fr_build_system(fsdf, uw, vw, ww, u, v, w, h, bu, bv, bw)
+    ih = [1.0 / x for x in h]
+    flux = (0.75 * -3.0 - 0.25 * 2.0) * ih[0]           # interior x faces, weighted
+    flux += (1.5 - 0.5) * ih[1] + (0.75 - -0.25) * ih[2]  # wall faces in y and z
+    f32 = lambda x: float(np.float32(x))  # the weights are stored as float
+    coll = ((1 - 0.75) * 8.0 - (1 - 0.25) * 4.0) * ih[0]
+    coll += ((1 - f32(0.2)) * -2.0 - (1 - 0.5) * 1.0) * ih[1]
+    coll += ((1 - f32(0.9)) * 5.0 - (1 - f32(0.6)) * 3.0) * ih[2]
+    assert b[0, 0, 1] == pytest.approx(flux + coll, rel=1e-12)
+    # a wrong sign on the collider term would give flux - coll
+    assert abs((flux - coll) - (flux + coll)) > 1.0
