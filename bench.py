@@ -218,12 +218,18 @@ def main():
     out["diffusion"] = dif["gpu"]
     mg2 = mg2_leg(jg, args)
     out["mgpcg_2d"] = mg2["gpu"]
+    small = small_configs_leg(jg, args)
+    out["closed_sphere_cg_64"] = small["cg64"]["gpu"]
+    out["dam_break_mgpcg_128"] = small["dam128"]["gpu"]
     if not args.no_cpu:
         out.update(cpu_leg(jg, n, L, args, res))
         out["constrain_velocity"].update(bc_cpu_leg(bc, n))
         out["compressed_cg"].update(csr_cpu_leg(csr))
         out["diffusion"].update(diffusion_cpu_leg(dif))
         out["mgpcg_2d"].update(mg2_cpu_leg(mg2))
+        c1, c2 = small_configs_cpu_leg(small)
+        out["closed_sphere_cg_64"].update(c1)
+        out["dam_break_mgpcg_128"].update(c2)
     print(json.dumps(out))
     solver.close()
 
@@ -708,6 +714,150 @@ def bc_cpu_leg(bc, n):
     log(f"bench: constrain_velocity oracle {cpu_ms:.0f} ms, GPU result bitwise equal: {same}")
     return {"cpu_ms": round(cpu_ms, 1), "cpu_cores": 1, "cpu_kind": "port",
             "gpu_bitwise_equal_to_oracle": bool(same)}
+
+
+def _device_scene(jg, s, kind, n, seed):
+    """Generate a scene on the device and build its system on every level of
+    the solver's hierarchy; returns the host copies of its inputs."""
+    import torch
+
+    f64 = torch.float64
+    shapes = [(n, n, n + 1), (n, n + 1, n), (n + 1, n, n), (n, n, n), (n, n, n)]
+    u, v, w, bs, fs = (torch.empty(sh, dtype=f64, device="cuda") for sh in shapes)
+    s.generate_scene(jg.scene_desc(kind, n, seed=seed), u, v, w, bs, fs)
+    s.sp_build(u, v, w, bs, fs, 1.0)
+    torch.cuda.synchronize()
+    host = {k: a.cpu().numpy() for k, a in zip(("u", "v", "w", "bsdf", "fsdf"), (u, v, w, bs, fs))}
+    del u, v, w, bs, fs
+    torch.cuda.empty_cache()
+    return host
+
+
+def _timed(fn, reps):
+    import torch
+
+    fn()  # warm-up (allocation, tuning, FP32 copies)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        info = fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / reps, info
+
+
+def small_configs_leg(jg, args):
+    """The two smaller configurations of the workload set, on the GPU:
+    - 64^3 closed sphere (all fluid, a solid sphere of radius 16 at the
+      centre, solid walls: all-Neumann) solved by Jet's FdmCgSolver3(2000,
+      1e-8), i.e. CG to the absolute ||r|| <= 1e-8 of Jet's rule;
+    - 128^3 dam break (fluid in x < 64, y < 51.2, air above, gravity on v)
+      by FP64 MGPCG with 5 levels and V(2,2), 20 coarsest sweeps, to
+      ||b - A x|| <= 1e-8 ||b|| (and mixed precision beside it)."""
+    out = {}
+    s = jg.Solver(0)
+    s.set_levels([(64, 64, 64)])
+    host = _device_scene(jg, s, jg.JG_SCENE_CLOSED_SPHERE, 64, args.seed)
+    ms, info = _timed(lambda: s.cg(2000, 1e-8), 3)
+    out["cg64"] = {"gpu": {"workload": "64^3 closed sphere r=16, all-Neumann, FdmCgSolver3(2000, 1e-8)",
+                           "gpu_ms": round(ms, 3), "iterations": info.iterations,
+                           "last_residual": info.last_residual, "rel_residual": info.rel_residual},
+                   "scene": host, "x": s.download_x()}
+    log(f"bench: 64^3 closed sphere CG {ms:.2f} ms, {info.iterations} it")
+    s.close()
+    L = 5
+    s = jg.Solver(0)
+    s.set_levels([(128 >> l,) * 3 for l in range(L)])
+    host = _device_scene(jg, s, jg.JG_SCENE_DAM_BREAK, 128, args.seed)
+    p = jg.mg_params(max_levels=L, restrict_iter=2, correct_iter=2, coarsest_iter=20, final_iter=2,
+                     sor=args.omega)
+    gpu = {"workload": "128^3 dam break, 5 levels, V(2,2), 20 coarsest sweeps, rel tol 1e-8"}
+    xs = {}
+    for name, prec in (("fp64", jg.JG_FP64), ("mixed", jg.JG_MIXED)):
+        o = jg.gpu_options(precision=prec, stop_rule=jg.JG_STOP_RELATIVE, rel_tol=1e-8)
+        ms, info = _timed(lambda: s.mgpcg(500, p, o), 5)
+        gpu[name] = {"gpu_ms": round(ms, 3), "iterations": info.iterations,
+                     "rel_residual": info.rel_residual}
+        xs[name] = s.download_x()
+        log(f"bench: 128^3 dam break {name} {ms:.2f} ms, {info.iterations} it")
+    gpu["speedup_mixed_vs_fp64"] = round(gpu["fp64"]["gpu_ms"] / gpu["mixed"]["gpu_ms"], 3)
+    out["dam128"] = {"gpu": gpu, "scene": host, "x": xs, "levels": L, "omega": args.omega}
+    s.close()
+    return out
+
+
+def small_configs_cpu_leg(small):
+    """CPU baselines of the two smaller configurations, on the host cores with
+    Jet's own solvers compiled from its sources (oracle/_ref/libjetref.so,
+    OpenMP), and the FP64-oracle error of each GPU solution against the
+    oracle's 1e-12 solve of the same system."""
+    import numpy as np
+
+    from oracle.jet_oracle import MgParams, Oracle, Ref
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = Oracle()
+    try:
+        ref = Ref()
+    except OSError:
+        ref = None
+    c1, c2 = {}, {}
+
+    # 64^3 closed sphere: Jet's FdmCgSolver3(2000, 1e-8), the whole solve
+    sc = small["cg64"]["scene"]
+    markers, sys_ = o.sp_build_mg(sc["u"], sc["v"], sc["w"], (1.0,) * 3, (0.0,) * 3, sc["bsdf"],
+                                  sc["fsdf"], 4)
+    A, b = sys_.A[0], sys_.b[0]
+    solve = (lambda: ref.cg_solve(A, b, 2000, 1e-8)) if ref else (lambda: o.cg_solve(A, b, 2000, 1e-8))
+    t0 = time.perf_counter()
+    _, it, res, _ = solve()
+    cpu_ms = (time.perf_counter() - t0) * 1e3
+    o.mgpcg_clean_solve(sys_, MgParams.make(max_levels=4, restrict_iter=2, correct_iter=2,
+                                            coarsest_iter=20, final_iter=2), 200, 1e-12)
+    # all-Neumann: the solution is defined up to a constant on the fluid
+    # cells (the solid sphere's cells are identity rows, p = 0 there)
+    fluid = markers[0] == 0
+    p_ref = sys_.x[0][fluid] - sys_.x[0][fluid].mean()
+    x = small["cg64"]["x"][fluid]
+    c1.update({"cpu_baseline": {"value": round(cpu_ms, 2), "unit": "ms", "cores": threads,
+                                "kind": "reference" if ref else "port",
+                                "sample": f"the whole solve by Jet's FdmCgSolver3(2000, 1e-8) "
+                                          f"compiled from its sources, {it} iterations, "
+                                          f"lastResidual {res:.2e}"},
+               "err_vs_oracle_mean_free": float(np.linalg.norm((x - x.mean()) - p_ref)
+                                                / np.linalg.norm(p_ref))})
+    log(f"bench: 64^3 CG on the CPU {cpu_ms:.0f} ms, {it} it")
+
+    # 128^3 dam break: the oracle's FP64 MGPCG to 1e-8 (the GPU's algorithm on
+    # the host) and a bounded sample of Jet's own FdmMgpcgSolver3 (red-black
+    # smoothing, the same V(2,2)): its time per iteration
+    sc = small["dam128"]["scene"]
+    L = small["dam128"]["levels"]
+    _, sys_ = o.sp_build_mg(sc["u"], sc["v"], sc["w"], (1.0,) * 3, (0.0,) * 3, sc["bsdf"],
+                            sc["fsdf"], L)
+    p = MgParams.make(max_levels=L, restrict_iter=2, correct_iter=2, coarsest_iter=20, final_iter=2,
+                      sor=small["dam128"]["omega"])
+    t0 = time.perf_counter()
+    _, it_cpu, rel_cpu = o.mgpcg_clean_solve(sys_, p, 200, 1e-8)
+    cpu_ms = (time.perf_counter() - t0) * 1e3
+    o.mgpcg_clean_solve(sys_, p, 200, 1e-12)
+    p_ref = sys_.x[0].copy()
+    c2["cpu_baseline"] = {"value": round(cpu_ms, 1), "unit": "ms", "cores": threads, "kind": "port",
+                          "sample": f"the whole solve by the oracle's FP64 MGPCG (the GPU's "
+                                    f"algorithm), {it_cpu} iterations to rel residual {rel_cpu:.1e}"}
+    if ref is not None:
+        iters = 5
+        pj = MgParams.make(max_levels=L, restrict_iter=2, correct_iter=2, coarsest_iter=20,
+                           final_iter=2, max_tol=0.0, red_black=1)
+        t0 = time.perf_counter()
+        ref.mgpcg_solve(sys_, pj, iters)
+        jet_ms = (time.perf_counter() - t0) * 1e3
+        c2["cpu_reference_ms_per_iteration"] = round(jet_ms / iters, 1)
+        c2["cpu_reference_sample"] = (f"{iters} iterations of Jet's own FdmMgpcgSolver3 (red-black, "
+                                      f"V(2,2), compiled from its sources, OpenMP, {threads} threads)")
+    c2["err_vs_oracle"] = {k: float(np.linalg.norm(x - p_ref) / np.linalg.norm(p_ref))
+                           for k, x in small["dam128"]["x"].items()}
+    log(f"bench: 128^3 dam break oracle {cpu_ms:.0f} ms, errors {c2['err_vs_oracle']}")
+    return c1, c2
 
 
 def cpu_leg(jg, n, L, args, res):

@@ -347,9 +347,11 @@ __global__ __launch_bounds__(NT) void k_rbgs(const Row<TA>* __restrict__ A,
     }
 }
 
-// Weighted Jacobi sweep: xout = x + omega (b - A x) / c  (omega = 1 is
-// FdmJacobiSolver3::relax, src/jet/fdm_jacobi_solver3.cpp:95-112).
-template <typename TA, typename TX>
+// Weighted Jacobi sweep: xout = x + omega ((b - s) / c - x)  (omega = 1 is
+// FdmJacobiSolver3::relax, src/jet/fdm_jacobi_solver3.cpp:95-112, computed
+// as its (b - s) / c).  ZC: a zero-centre row gives 0 (the V-cycle smoother;
+// Jet's relax divides by the zero).
+template <typename TA, typename TX, bool ZC = false>
 __global__ __launch_bounds__(NT) void k_jacobi(const Row<TA>* __restrict__ A,
                                                const TX* __restrict__ b,
                                                const TX* __restrict__ x, TX* __restrict__ xo,
@@ -366,6 +368,10 @@ __global__ __launch_bounds__(NT) void k_jacobi(const Row<TA>* __restrict__ A,
     const idx_t sz = (idx_t)d.nx * d.ny;
     for (int k = k0; k < k1; ++k, c += sz) {
         const Row<TA> row = A[c];
+        if (ZC && row.c == TA(0)) {
+            xo[c] = TX(0);
+            continue;
+        }
         const TX s = offdiag<TA, TX, TX>(A, x, row, c, i, j, k, d);
         if (omega == TX(1))
             xo[c] = (b[c] - s) / (TX)row.c;

@@ -237,8 +237,76 @@ template <typename T>
 struct MgCfg {
     unsigned pre, post0, post, coarsest;
     T omega;
-    int pairs = 0;  // jg_gpu_options.sweep_pairs
+    int pairs = 0;         // jg_gpu_options.sweep_pairs
+    bool jacobi = false;   // weighted-Jacobi smoothing (JG_SMOOTH_JACOBI)
 };
+
+// The V-cycle with weighted Jacobi smoothing (JG_SMOOTH_JACOBI): zero initial
+// guess, `pre` sweeps x += omega ((b - s) / c - x) (FdmJacobiSolver3::relax,
+// src/jet/fdm_jacobi_solver3.cpp:95-112, weighted), the residual and Jet's
+// restriction, recursion, Jet's correction, `post` sweeps; `coarsest` sweeps
+// on the coarsest level.  Every level keeps its current iterate in x and
+// sweeps out of place into y, copying back after an odd count; 2-D
+// hierarchies take FdmMgUtils2's transfers and the one-plane residual.
+template <typename T>
+int vcycle_jacobi(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl) {
+    const size_t L = s->lv.size();
+    auto B = [&](size_t l) -> const T* { return l == 0 ? r0 : Prec<T>::b(s->lv[l]); };
+    auto X = [&](size_t l) -> T* { return l == 0 ? z0 : Prec<T>::x(s->lv[l]); };
+    // `count` sweeps of level l's x (through y; the result ends in x)
+    auto sweeps = [&](size_t l, unsigned count) {
+        Level& lv = s->lv[l];
+        const Dims& d = lv.d;
+        const Geo g = geo_stencil(d);
+        T* x = X(l);
+        T* y = Prec<T>::y(lv);
+        const T* in = x;
+        for (unsigned k = 0; k < count; ++k) {
+            T* out = in == x ? y : x;
+            k_jacobi<T, T, true><<<g.grid, g.block, 0, s->stream>>>(Prec<T>::A(lv), B(l), in, out, d,
+                                                                   g.zc, c.omega, ctl);
+            in = out;
+        }
+        if (in != x) k_convert<T, T><<<nblocks_1d(d.n()), NT, 0, s->stream>>>(in, x, d.n());
+    };
+    for (size_t l = 0; l < L; ++l) {
+        Level& lv = s->lv[l];
+        const Dims& d = lv.d;
+        k_fill<T><<<nblocks_1d(d.n()), NT, 0, s->stream>>>(X(l), T(0), d.n());
+        if (l + 1 == L) {
+            sweeps(l, c.coarsest);
+            break;
+        }
+        sweeps(l, c.pre);
+        const Dims& cd = s->lv[l + 1].d;
+        const Geo gc = geo_planes(cd);
+        if (s->twod) {
+            const Geo gf = geo_planes(d);
+            k_residual_plane<T><<<gf.grid, gf.block, 0, s->stream>>>(Prec<T>::A(lv), X(l), B(l),
+                                                                    Prec<T>::t(lv), d, ctl);
+            k_restrict2<T><<<gc.grid, gc.block, 0, s->stream>>>(Prec<T>::t(lv),
+                                                                Prec<T>::b(s->lv[l + 1]), cd, ctl);
+        } else {
+            const Geo gf = geo_stencil(d);
+            k_residual<T, T, T, false, false, true><<<gf.grid, gf.block, 0, s->stream>>>(
+                Prec<T>::A(lv), X(l), B(l), Prec<T>::t(lv), d, gf.zc, nullptr, ctl);
+            k_restrict<T><<<gc.grid, gc.block, 0, s->stream>>>(Prec<T>::t(lv),
+                                                               Prec<T>::b(s->lv[l + 1]), cd, ctl);
+        }
+        JG_TRY(check_launch(s, "jacobi vcycle down"));
+    }
+    for (size_t l = L - 1; l-- > 0;) {
+        const Dims& d = s->lv[l].d;
+        const Geo gf = geo_planes(d);
+        if (s->twod)
+            k_correct2<T><<<gf.grid, gf.block, 0, s->stream>>>(X(l + 1), X(l), d, ctl);
+        else
+            k_correct<T><<<gf.grid, gf.block, 0, s->stream>>>(X(l + 1), X(l), d, ctl);
+        sweeps(l, l == 0 ? c.post0 : c.post);
+        JG_TRY(check_launch(s, "jacobi vcycle up"));
+    }
+    return JG_OK;
+}
 
 // z = M r: one symmetric V-cycle with zero initial guess.  Level 0 reads
 // its right-hand side from r0 and writes its result to z0.  Each level runs
@@ -348,6 +416,7 @@ template <typename T>
 int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, int* dot_blocks) {
     const size_t L = s->lv.size();
     *dot_blocks = 0;
+    if (c.jacobi) return vcycle_jacobi<T>(s, r0, z0, c, ctl);
     if (s->twod) return vcycle2d<T>(s, r0, z0, c, ctl);
     auto B = [&](size_t l) -> const T* { return l == 0 ? r0 : Prec<T>::b(s->lv[l]); };
     auto X = [&](size_t l) -> T* { return l == 0 ? z0 : Prec<T>::x(s->lv[l]); };
@@ -846,7 +915,12 @@ namespace {
 // (include/jet/detail/mg-inl.h:26); max_levels only tells callers how deep to
 // build it.  Relaxation factor of the red-black smoother: the GPU option if
 // given, else the caller's Jet sorFactor (FdmMgSolver3's, default 1.5).  Any
-// 0 < omega < 2 keeps the symmetric V-cycle positive definite.
+// 0 < omega < 2 keeps the symmetric V-cycle positive definite.  Weight of the
+// Jacobi smoother: the GPU option if given, else 6/7 (the weight that damps
+// the high-frequency half of the 7-point Laplacian's spectrum best); it must
+// lie in (0, 1], where 2 D / omega - A stays positive definite on diagonally
+// dominant rows (plain Jacobi, omega = 1, barely smooths the checkerboard
+// mode).
 int mg_options(jg_solver* s, const jg_mg_params* p, const jg_gpu_options* o, jg_gpu_options* opt,
                double* omega) {
     if (o)
@@ -854,6 +928,14 @@ int mg_options(jg_solver* s, const jg_mg_params* p, const jg_gpu_options* o, jg_
     else
         jg_gpu_options_default(opt);
     if (opt->check_every == 0) opt->check_every = 1;
+    if (opt->smoother == JG_SMOOTH_JACOBI) {
+        *omega = opt->omega > 0 ? opt->omega : 6.0 / 7.0;
+        if (!(*omega > 0.0 && *omega <= 1.0)) {
+            s->err = "Jacobi smoother weight must lie in (0, 1]";
+            return JG_ERR_INVALID_ARG;
+        }
+        return JG_OK;
+    }
     if (opt->smoother != JG_SMOOTH_RBGS) {
         s->err = "unsupported smoother";
         return JG_ERR_UNSUPPORTED;
@@ -1219,6 +1301,7 @@ int jg_mg_vcycle(jg_solver* s, const jg_mg_params* p, const jg_gpu_options* o, c
         MgCfg<float> mg{p->restrict_iter, p->final_iter, p->correct_iter, p->coarsest_iter,
                         (float)omega};
         mg.pairs = opt.sweep_pairs;
+        mg.jacobi = opt.smoother == JG_SMOOTH_JACOBI;
         k_convert<double, float><<<nb1, NT, 0, s->stream>>>(r, s->r32, n);
         JG_TRY(vcycle<float>(s, s->r32, s->z32, mg, nullptr, &nb));
         k_convert<float, double><<<nb1, NT, 0, s->stream>>>(s->z32, z, n);
@@ -1226,10 +1309,57 @@ int jg_mg_vcycle(jg_solver* s, const jg_mg_params* p, const jg_gpu_options* o, c
         JG_TRY(autotune_sweeps<double>(s));
         MgCfg<double> mg{p->restrict_iter, p->final_iter, p->correct_iter, p->coarsest_iter, omega};
         mg.pairs = opt.sweep_pairs;
+        mg.jacobi = opt.smoother == JG_SMOOTH_JACOBI;
         JG_TRY(vcycle<double>(s, r, z, mg, nullptr, &nb));
     }
     JG_TRY(check_launch(s, "vcycle"));
     return sync(s);
+}
+
+// FdmMgSolver3::solve(FdmMgLinearSystem3*) (src/jet/fdm_mg_solver3.cpp:69-74):
+// one V-cycle from the resident x.  For the linear V-cycle this is
+// x += M (b - A x) with M the zero-start cycle of jg_mg_vcycle; Jet's result
+// is lastResidualNorm < maxTolerance, the norm of b - A x afterwards
+// (include/jet/detail/mg-inl.h:62-66).
+int jg_solver_mg(jg_solver* s, const jg_mg_params* p, const jg_gpu_options* o, jg_solve_info* info) {
+    if (!s || s->lv.empty() || !p) return JG_ERR_INVALID_ARG;
+    JG_HIP(hipSetDevice(s->device));
+    Level& l0 = s->lv[0];
+    const idx_t n = l0.d.n();
+    const Geo g = geo_stencil(l0.d);
+    const int nb1 = nblocks_1d(n);
+    JG_HIP(hipEventRecord(s->ev0, s->stream));
+    k_residual<double, double, double, false, false><<<g.grid, g.block, 0, s->stream>>>(
+        l0.A64, s->x, s->b, s->r, l0.d, g.zc, nullptr, nullptr);
+    JG_TRY(check_launch(s, "mg residual"));
+    JG_TRY(jg_mg_vcycle(s, p, o, s->r, s->z));
+    k_axpy<double><<<nb1, NT, 0, s->stream>>>(1.0, s->z, s->x, s->x, n);
+    k_residual<double, double, double, true, false><<<g.grid, g.block, 0, s->stream>>>(
+        l0.A64, s->x, s->b, s->r, l0.d, g.zc, s->slot(0), nullptr);
+    k_fin_sum<<<1, kFinThreads, 0, s->stream>>>(s->slot(0), nblocks(g), s->scal + 0);
+    k_dot<double, double><<<nb1, NT, 0, s->stream>>>(s->b, s->b, n, s->slot(1), nullptr);
+    k_fin_sum<<<1, kFinThreads, 0, s->stream>>>(s->slot(1), nb1, s->scal + 1);
+    JG_HIP(hipEventRecord(s->ev1, s->stream));
+    JG_TRY(check_launch(s, "mg solve"));
+    double h[2];
+    JG_HIP(hipMemcpyAsync(h, s->scal, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+    JG_TRY(sync(s));
+    if (!std::isfinite(h[0])) {
+        s->err = "non-finite value in the V-cycle";
+        if (info) info->converged = 0;
+        return JG_ERR_NONFINITE;
+    }
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        float ms = 0.f;
+        JG_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        info->solve_ms = ms;
+        info->iterations = 1;
+        info->last_residual = std::sqrt(h[0]);
+        info->rel_residual = h[1] > 0 ? std::sqrt(h[0] / h[1]) : std::sqrt(h[0]);
+        info->converged = info->last_residual < p->max_tol;
+    }
+    return JG_OK;
 }
 
 int jg_solver_mgpcg(jg_solver* s, unsigned max_iter, const jg_mg_params* p,
@@ -1258,6 +1388,7 @@ int jg_solver_mgpcg(jg_solver* s, unsigned max_iter, const jg_mg_params* p,
     if (opt.precision != JG_MIXED) {
         MgCfg<double> mg{p->restrict_iter, p->final_iter, p->correct_iter, p->coarsest_iter, omega};
         mg.pairs = opt.sweep_pairs;
+        mg.jacobi = opt.smoother == JG_SMOOTH_JACOBI;
         Ctl h;
         init_ctl(&h, max_iter, p->max_tol, opt.stop_rule == JG_STOP_RELATIVE ? 1 : 0, opt.rel_tol);
         JG_TRY(upload_ctl(s, ctl, h));
@@ -1275,6 +1406,7 @@ int jg_solver_mgpcg(jg_solver* s, unsigned max_iter, const jg_mg_params* p,
         MgCfg<float> mg{p->restrict_iter, p->final_iter, p->correct_iter, p->coarsest_iter,
                         (float)omega};
         mg.pairs = opt.sweep_pairs;
+        mg.jacobi = opt.smoother == JG_SMOOTH_JACOBI;
         Geo g = geo_stencil(l0.d);
         const int nb1 = nblocks_1d(n);
         k_dot<double, double><<<nb1, NT, 0, s->stream>>>(s->b, s->b, n, s->slot(3), nullptr);
@@ -1425,6 +1557,18 @@ int jg_mgpcg_solve_host(jg_solver* s, size_t nlev, const size_t* nx, const size_
     JG_TRY(jg_solver_set_levels(s, nlev, nx, ny, nz));
     for (size_t l = 0; l < nlev; ++l) JG_TRY(jg_solver_upload(s, l, rows[l], l == 0 ? b : nullptr));
     JG_TRY(jg_solver_mgpcg(s, max_iter, p, o, info));
+    return jg_solver_download_x(s, x);
+}
+
+int jg_mg_solve_host(jg_solver* s, size_t nlev, const size_t* nx, const size_t* ny, const size_t* nz,
+                     const double* const* rows, const double* b, double* x, const jg_mg_params* p,
+                     const jg_gpu_options* o, jg_solve_info* info) {
+    if (!s || !rows || !b || !x || !p) return JG_ERR_INVALID_ARG;
+    JG_TRY(jg_solver_set_levels(s, nlev, nx, ny, nz));
+    for (size_t l = 0; l < nlev; ++l) JG_TRY(jg_solver_upload(s, l, rows[l], l == 0 ? b : nullptr));
+    // the V-cycle starts from the caller's x, as Jet's does
+    JG_HIP(hipMemcpyAsync(s->x, x, s->lv[0].d.n() * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    JG_TRY(jg_solver_mg(s, p, o, info));
     return jg_solver_download_x(s, x);
 }
 
@@ -2038,6 +2182,7 @@ int jg_batch_mgpcg(jg_batch* B, unsigned max_iter, const jg_mg_params* p, const 
     if (!mixed) {
         MgCfg<double> mg{p->restrict_iter, p->final_iter, p->correct_iter, p->coarsest_iter, omega};
         mg.pairs = opt.sweep_pairs;
+        mg.jacobi = opt.smoother == JG_SMOOTH_JACOBI;
         hipGraphExec_t exec = nullptr;
         JG_TRY(batch_graph<double>(B, JG_FP64, mg, &exec));
         for (auto* s : B->sys) {
@@ -2059,6 +2204,7 @@ int jg_batch_mgpcg(jg_batch* B, unsigned max_iter, const jg_mg_params* p, const 
     } else {
         MgCfg<float> mg{p->restrict_iter, p->final_iter, p->correct_iter, p->coarsest_iter, (float)omega};
         mg.pairs = opt.sweep_pairs;
+        mg.jacobi = opt.smoother == JG_SMOOTH_JACOBI;
         hipGraphExec_t exec = nullptr;
         JG_TRY(batch_graph<float>(B, JG_MIXED, mg, &exec));
         // ||b|| of every system

@@ -4,6 +4,7 @@
 
 #include <jet/fdm_linear_system2.h>
 #include <jet/fdm_mg_linear_system2.h>
+#include <jet/logging.h>
 
 #include <stdexcept>
 #include <string>
@@ -165,6 +166,9 @@ bool FdmGpuMgpcgSolver2::solve(FdmMgLinearSystem2* system) {
                                system->b.levels.front().data(), system->x.levels.front().data(),
                                _maxNumberOfIterations, &p, &_options, &_info),
            s, "FdmGpuMgpcgSolver2::solve");
+    // Jet's log line (src/jet/fdm_mgpcg_solver2.cpp:75-76)
+    JET_INFO << "Residual after solving MGPCG: " << _info.last_residual
+             << " Number of MGPCG iterations: " << _info.iterations;
     return _info.converged != 0;
 }
 

@@ -8,6 +8,7 @@
 
 #include <jet/fdm_linear_system3.h>
 #include <jet/fdm_mg_linear_system3.h>
+#include <jet/logging.h>
 
 #include <stdexcept>
 #include <string>
@@ -43,6 +44,53 @@ jg_solver* lazySolver(jg_solver** slot, int device) {
 
 bool sameSize(const Size3& a, const Size3& b) {
     return a.x == b.x && a.y == b.y && a.z == b.z;
+}
+
+// Level sizes and row pointers of a multigrid system, checked as Jet's
+// solvers would (finest A, x and b of one size).  False: an empty grid.
+struct Hierarchy {
+    std::vector<size_t> nx, ny, nz;
+    std::vector<const double*> rows;
+};
+
+bool hierarchyOf(FdmMgLinearSystem3* system, Hierarchy* h) {
+    JET_THROW_INVALID_ARG_IF(system == nullptr);
+    const size_t L = system->A.levels.size();
+    JET_THROW_INVALID_ARG_IF(L == 0 || system->x.levels.empty() ||
+                             system->b.levels.empty());
+    const Size3 n0 = system->A.levels.front().size();
+    JET_THROW_INVALID_ARG_IF(
+        !sameSize(n0, system->x.levels.front().size()) ||
+        !sameSize(n0, system->b.levels.front().size()));
+    if (n0.x * n0.y * n0.z == 0) return false;
+    h->nx.resize(L);
+    h->ny.resize(L);
+    h->nz.resize(L);
+    h->rows.resize(L);
+    for (size_t l = 0; l < L; ++l) {
+        const Size3 n = system->A.levels[l].size();
+        h->nx[l] = n.x;
+        h->ny[l] = n.y;
+        h->nz[l] = n.z;
+        h->rows[l] = reinterpret_cast<const double*>(system->A.levels[l].data());
+    }
+    return true;
+}
+
+// Jet's MgParameters and smoother factor as the library takes them.
+jg_mg_params paramsOf(const FdmMgSolver3& solver, double tolerance) {
+    jg_mg_params p;
+    jg_mg_params_default(&p);
+    const MgParameters<FdmBlas3>& mg = solver.params();
+    p.max_levels = mg.maxNumberOfLevels;
+    p.restrict_iter = mg.numberOfRestrictionIter;
+    p.correct_iter = mg.numberOfCorrectionIter;
+    p.coarsest_iter = mg.numberOfCoarsestIter;
+    p.final_iter = mg.numberOfFinalIter;
+    p.max_tol = tolerance;
+    p.sor = solver.sorFactor();
+    p.red_black = solver.useRedBlackOrdering() ? 1 : 0;
+    return p;
 }
 
 }  // namespace
@@ -128,47 +176,21 @@ FdmGpuMgpcgSolver3::FdmGpuMgpcgSolver3(
 FdmGpuMgpcgSolver3::~FdmGpuMgpcgSolver3() { jg_solver_destroy(_solver); }
 
 bool FdmGpuMgpcgSolver3::solve(FdmMgLinearSystem3* system) {
-    JET_THROW_INVALID_ARG_IF(system == nullptr);
-    const size_t L = system->A.levels.size();
-    JET_THROW_INVALID_ARG_IF(L == 0 || system->x.levels.empty() ||
-                             system->b.levels.empty());
-    const Size3 n0 = system->A.levels.front().size();
-    JET_THROW_INVALID_ARG_IF(
-        !sameSize(n0, system->x.levels.front().size()) ||
-        !sameSize(n0, system->b.levels.front().size()));
-    if (n0.x * n0.y * n0.z == 0) {
+    Hierarchy h;
+    if (!hierarchyOf(system, &h)) {
         _info = jg_solve_info{};
         return true;
     }
-    std::vector<size_t> nx(L), ny(L), nz(L);
-    std::vector<const double*> rows(L);
-    for (size_t l = 0; l < L; ++l) {
-        const Size3 n = system->A.levels[l].size();
-        nx[l] = n.x;
-        ny[l] = n.y;
-        nz[l] = n.z;
-        rows[l] = reinterpret_cast<const double*>(system->A.levels[l].data());
-    }
-
-    // Jet's MgParameters and smoother factor, as FdmMgpcgSolver3 uses them
-    jg_mg_params p;
-    jg_mg_params_default(&p);
-    const MgParameters<FdmBlas3>& mg = params();
-    p.max_levels = mg.maxNumberOfLevels;
-    p.restrict_iter = mg.numberOfRestrictionIter;
-    p.correct_iter = mg.numberOfCorrectionIter;
-    p.coarsest_iter = mg.numberOfCoarsestIter;
-    p.final_iter = mg.numberOfFinalIter;
-    p.max_tol = _tolerance;
-    p.sor = sorFactor();
-    p.red_black = useRedBlackOrdering() ? 1 : 0;
-
+    const jg_mg_params p = paramsOf(*this, _tolerance);
     jg_solver* s = lazySolver(&_solver, _device);
-    check(jg_mgpcg_solve_host(s, L, nx.data(), ny.data(), nz.data(),
-                              rows.data(), system->b.levels.front().data(),
+    check(jg_mgpcg_solve_host(s, h.rows.size(), h.nx.data(), h.ny.data(), h.nz.data(),
+                              h.rows.data(), system->b.levels.front().data(),
                               system->x.levels.front().data(),
                               _maxNumberOfIterations, &p, &_options, &_info),
           s, "FdmGpuMgpcgSolver3::solve");
+    // Jet's log line (src/jet/fdm_mgpcg_solver3.cpp:75-76)
+    JET_INFO << "Residual after solving MGPCG: " << _info.last_residual
+             << " Number of MGPCG iterations: " << _info.iterations;
     return _info.converged != 0;
 }
 
@@ -189,5 +211,47 @@ const jg_gpu_options& FdmGpuMgpcgSolver3::gpuOptions() const {
 const jg_solve_info& FdmGpuMgpcgSolver3::lastSolveInfo() const {
     return _info;
 }
+
+// ------------------------------------------------------------ MG
+
+FdmGpuMgSolver3::FdmGpuMgSolver3(size_t maxNumberOfLevels,
+                                 unsigned int numberOfRestrictionIter,
+                                 unsigned int numberOfCorrectionIter,
+                                 unsigned int numberOfCoarsestIter,
+                                 unsigned int numberOfFinalIter,
+                                 double maxTolerance, double sorFactor,
+                                 bool useRedBlackOrdering, int device)
+    : FdmMgSolver3(maxNumberOfLevels, numberOfRestrictionIter,
+                   numberOfCorrectionIter, numberOfCoarsestIter,
+                   numberOfFinalIter, maxTolerance, sorFactor,
+                   useRedBlackOrdering),
+      _device(device) {
+    jg_gpu_options_default(&_options);
+    _info.last_residual = kMaxD;
+}
+
+FdmGpuMgSolver3::~FdmGpuMgSolver3() { jg_solver_destroy(_solver); }
+
+bool FdmGpuMgSolver3::solve(FdmMgLinearSystem3* system) {
+    Hierarchy h;
+    if (!hierarchyOf(system, &h)) {
+        _info = jg_solve_info{};
+        return true;
+    }
+    const jg_mg_params p = paramsOf(*this, params().maxTolerance);
+    jg_solver* s = lazySolver(&_solver, _device);
+    check(jg_mg_solve_host(s, h.rows.size(), h.nx.data(), h.ny.data(), h.nz.data(),
+                           h.rows.data(), system->b.levels.front().data(),
+                           system->x.levels.front().data(), &p, &_options, &_info),
+          s, "FdmGpuMgSolver3::solve");
+    return _info.converged != 0;
+}
+
+double FdmGpuMgSolver3::lastResidual() const { return _info.last_residual; }
+void FdmGpuMgSolver3::setGpuOptions(const jg_gpu_options& options) {
+    _options = options;
+}
+const jg_gpu_options& FdmGpuMgSolver3::gpuOptions() const { return _options; }
+const jg_solve_info& FdmGpuMgSolver3::lastSolveInfo() const { return _info; }
 
 }  // namespace jet

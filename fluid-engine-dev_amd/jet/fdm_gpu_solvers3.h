@@ -11,6 +11,9 @@
 //       replaces FdmCgSolver3 (include/jet/fdm_cg_solver3.h:19-37,
 //       src/jet/fdm_cg_solver3.cpp:14-47): same constructor arguments,
 //       accessors and bool result;
+//   FdmGpuMgSolver3     : FdmMgSolver3
+//       replaces FdmMgSolver3 (include/jet/fdm_mg_solver3.h:23-51,
+//       src/jet/fdm_mg_solver3.cpp:14-74): one V-cycle per solve;
 //   FdmGpuMgpcgSolver3  : FdmMgSolver3
 //       replaces FdmMgpcgSolver3 (include/jet/fdm_mgpcg_solver3.h:35-56,
 //       src/jet/fdm_mgpcg_solver3.cpp:40-80): same constructor arguments and
@@ -119,7 +122,47 @@ class FdmGpuMgpcgSolver3 final : public FdmMgSolver3 {
     jg_solve_info _info{};
 };
 
+//! GPU multigrid solver: one V-cycle per solve, from the caller's x.
+//!
+//! Replaces FdmMgSolver3 itself (include/jet/fdm_mg_solver3.h:23-51,
+//! src/jet/fdm_mg_solver3.cpp:14-74): same constructor arguments and params();
+//! solve() runs one V-cycle on the caller's hierarchy starting from the x it
+//! holds (Jet does not zero it) and returns Jet's result, the residual norm
+//! after the cycle below maxTolerance.  The cycle is the library's symmetric
+//! red-black V-cycle, as in FdmGpuMgpcgSolver3.
+class FdmGpuMgSolver3 final : public FdmMgSolver3 {
+ public:
+    FdmGpuMgSolver3(size_t maxNumberOfLevels,
+                    unsigned int numberOfRestrictionIter = 5,
+                    unsigned int numberOfCorrectionIter = 5,
+                    unsigned int numberOfCoarsestIter = 20,
+                    unsigned int numberOfFinalIter = 20,
+                    double maxTolerance = 1e-9, double sorFactor = 1.5,
+                    bool useRedBlackOrdering = false, int device = 0);
+    ~FdmGpuMgSolver3() override;
+
+    FdmGpuMgSolver3(const FdmGpuMgSolver3&) = delete;
+    FdmGpuMgSolver3& operator=(const FdmGpuMgSolver3&) = delete;
+
+    bool solve(FdmMgLinearSystem3* system) override;
+
+    //! ||b - A x||_2 on the finest level after the last solve.
+    double lastResidual() const;
+
+    void setGpuOptions(const jg_gpu_options& options);
+    const jg_gpu_options& gpuOptions() const;
+
+    const jg_solve_info& lastSolveInfo() const;
+
+ private:
+    int _device;
+    jg_gpu_options _options;
+    jg_solver* _solver = nullptr;
+    jg_solve_info _info{};
+};
+
 typedef std::shared_ptr<FdmGpuCgSolver3> FdmGpuCgSolver3Ptr;
+typedef std::shared_ptr<FdmGpuMgSolver3> FdmGpuMgSolver3Ptr;
 typedef std::shared_ptr<FdmGpuMgpcgSolver3> FdmGpuMgpcgSolver3Ptr;
 
 }  // namespace jet

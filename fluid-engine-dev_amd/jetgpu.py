@@ -20,7 +20,7 @@ ERRORS = {1: "invalid argument", 2: "out of memory", 3: "HIP error", 4: "non-fin
           5: "no device", 6: "unsupported"}
 JG_FP64, JG_MIXED = 0, 1
 JG_STOP_JET, JG_STOP_RELATIVE = 0, 1
-JG_SMOOTH_RBGS = 0
+JG_SMOOTH_RBGS, JG_SMOOTH_JACOBI = 0, 1
 (JG_KERNEL_RBGS, JG_KERNEL_DIR_SPMV, JG_KERNEL_RESIDUAL, JG_KERNEL_UPDATE_XR, JG_KERNEL_DOT,
  JG_KERNEL_COPY, JG_KERNEL_SWEEP, JG_KERNEL_SWEEP2, JG_KERNEL_PRE0, JG_KERNEL_POST0) = range(10)
 (JG_SCENE_CLOSED_SPHERE, JG_SCENE_DAM_BREAK, JG_SCENE_SPHERE_POOL, JG_SCENE_BATCH_MEMBER,
@@ -94,6 +94,11 @@ SIGNATURES = {
     "jg_solver_mgpcg": (C.c_int, [_vp, _u, C.POINTER(MgParams), C.POINTER(GpuOptions),
                                   C.POINTER(SolveInfo)]),
     "jg_mg_vcycle": (C.c_int, [_vp, C.POINTER(MgParams), C.POINTER(GpuOptions), _vp, _vp]),
+    "jg_solver_mg": (C.c_int, [_vp, C.POINTER(MgParams), C.POINTER(GpuOptions),
+                               C.POINTER(SolveInfo)]),
+    "jg_mg_solve_host": (C.c_int, [_vp, _sz, C.POINTER(_sz), C.POINTER(_sz), C.POINTER(_sz),
+                                   C.POINTER(_vp), _vp, _vp, C.POINTER(MgParams),
+                                   C.POINTER(GpuOptions), C.POINTER(SolveInfo)]),
     "jg_cg_solve_host": (C.c_int, [_vp, _sz, _sz, _sz, _vp, _vp, _vp, _u, _d, C.POINTER(SolveInfo)]),
     "jg_csr_cg_solve_host": (C.c_int, [_vp, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _u, _d,
                                        C.POINTER(SolveInfo)]),
@@ -421,6 +426,36 @@ class Solver:
         self.check(lib().jg_solver_mgpcg(self.h, max_iter, C.byref(params),
                                          C.byref(options) if options is not None else None,
                                          C.byref(info)), "mgpcg")
+        return info
+
+    def mg(self, params: MgParams, options: GpuOptions | None = None):
+        """One V-cycle from the resident x (FdmMgSolver3::solve)."""
+        info = SolveInfo()
+        self.check(lib().jg_solver_mg(self.h, C.byref(params),
+                                      C.byref(options) if options is not None else None,
+                                      C.byref(info)), "mg")
+        return info
+
+    def mg_host(self, system, b, x, params: MgParams, options: GpuOptions | None = None):
+        """jg_mg_solve_host: one V-cycle on host arrays (system.A levels as
+        (nz, ny, nx, 4) rows); x (finest shape) is the start and is overwritten."""
+        import numpy as np
+
+        L = len(system.A)
+        rows = [np.ascontiguousarray(A, np.float64) for A in system.A]
+        SZ = _sz * L
+        nx = SZ(*[A.shape[2] for A in rows])
+        ny = SZ(*[A.shape[1] for A in rows])
+        nz = SZ(*[A.shape[0] for A in rows])
+        ptrs = (_vp * L)(*[A.ctypes.data for A in rows])
+        b = np.ascontiguousarray(b, np.float64)
+        assert x.flags.c_contiguous and x.dtype == np.float64 and x.shape == b.shape
+        info = SolveInfo()
+        self.check(lib().jg_mg_solve_host(self.h, L, nx, ny, nz, ptrs, b.ctypes.data, x.ctypes.data,
+                                          C.byref(params),
+                                          C.byref(options) if options is not None else None,
+                                          C.byref(info)), "mg_host")
+        self.sizes = [(A.shape[2], A.shape[1], A.shape[0]) for A in rows]
         return info
 
     def mg_vcycle(self, params: MgParams, options: GpuOptions | None, r, z):

@@ -48,7 +48,11 @@ enum { JG_FP64 = 0, JG_MIXED = 1 };
  * ||b - A x||_2 <= rel_tol * ||b||_2 (FP64 residual). */
 enum { JG_STOP_JET = 0, JG_STOP_RELATIVE = 1 };
 
-enum { JG_SMOOTH_RBGS = 0 };
+/* V-cycle smoothers: red-black Gauss-Seidel with SOR factor omega
+ * (FdmGaussSeidelSolver3::relaxRedBlack, src/jet/fdm_gauss_seidel_solver3.cpp:
+ * 150-221), or weighted Jacobi x += omega ((b - s) / c - x)
+ * (FdmJacobiSolver3::relax, src/jet/fdm_jacobi_solver3.cpp:95-112, weighted). */
+enum { JG_SMOOTH_RBGS = 0, JG_SMOOTH_JACOBI = 1 };
 
 typedef struct jg_solver jg_solver;
 
@@ -70,8 +74,9 @@ typedef struct {
     int precision;           /* JG_FP64 (default) or JG_MIXED            */
     int stop_rule;           /* JG_STOP_JET (default) or JG_STOP_RELATIVE */
     double rel_tol;          /* for JG_STOP_RELATIVE (default 1e-8)      */
-    int smoother;            /* JG_SMOOTH_RBGS (the only smoother so far)  */
-    double omega;            /* red-black smoother relaxation in (0,2); 0 -> Jet's sor */
+    int smoother;            /* JG_SMOOTH_RBGS (default) or JG_SMOOTH_JACOBI */
+    double omega;            /* RBGS: relaxation in (0,2), 0 -> Jet's sor;
+                                Jacobi: weight in (0,1], 0 -> 6/7 */
     unsigned max_refine;     /* mixed: outer FP64 refinement steps (default 8) */
     double inner_rel_tol;    /* mixed: FP32 inner reduction per step (default 1e-4) */
     unsigned check_every;    /* host convergence poll interval in iterations (default 1) */
@@ -148,6 +153,16 @@ int jg_solver_mgpcg(jg_solver* s, unsigned max_iter, const jg_mg_params* p,
 int jg_mg_vcycle(jg_solver* s, const jg_mg_params* p, const jg_gpu_options* o,
                  const double* r, double* z);
 
+/* FdmMgSolver3::solve(FdmMgLinearSystem3*) on the resident hierarchy
+ * (src/jet/fdm_mg_solver3.cpp:69-74): one V-cycle from the resident x, which
+ * is NOT zeroed (Jet's mgVCycle starts from the caller's x): x += M (b - A x)
+ * with jg_mg_vcycle's M.  info->last_residual is ||b - A x||_2 afterwards
+ * (mgVCycle's lastResidualNorm, include/jet/detail/mg-inl.h:62-66) and
+ * info->converged is Jet's result, last_residual < p->max_tol;
+ * info->iterations is 1. */
+int jg_solver_mg(jg_solver* s, const jg_mg_params* p, const jg_gpu_options* o,
+                 jg_solve_info* info);
+
 /* ---------------------------------------------------- batched solves */
 /* Many independent systems of uneven sizes solved together: each system is
  * a jg_solver of the batch (fill it with jg_solver_upload / jg_sp_build /
@@ -203,6 +218,13 @@ int jg_mgpcg_solve_host(jg_solver* s, size_t nlev, const size_t* nx,
                         const double* const* rows, const double* b, double* x,
                         unsigned max_iter, const jg_mg_params* p,
                         const jg_gpu_options* o, jg_solve_info* info);
+/* jg_solver_mg on host arrays: x is read (the V-cycle's starting value) and
+ * overwritten. */
+int jg_mg_solve_host(jg_solver* s, size_t nlev, const size_t* nx,
+                     const size_t* ny, const size_t* nz,
+                     const double* const* rows, const double* b, double* x,
+                     const jg_mg_params* p, const jg_gpu_options* o,
+                     jg_solve_info* info);
 
 /* ------------------------------------ FdmBlas3 / FdmMgUtils3 on device */
 /* All pointers are device pointers; these run on the solver's stream and

@@ -630,6 +630,76 @@ static void clean_vcycle(const or_mg_system* s, const or_mg_params* p,
     }
 }
 
+/* Weighted Jacobi sweep of the clean V-cycle: xo = x + omega ((b - s) / c - x)
+ * with s summed in FdmJacobiSolver3::relax's order, and for omega = 1 that
+ * relax itself, (b - s) / c (src/jet/fdm_jacobi_solver3.cpp:95-112).  A
+ * zero-centre row is set to 0, as in the clean red-black sweep. */
+static void jacobi_sweep_clean(const or_row* A, const double* b, double omega,
+                               const double* x, double* xo, size_t nx,
+                               size_t ny, size_t nz) {
+#pragma omp parallel for schedule(static)
+    for (size_t k = 0; k < nz; ++k)
+        for (size_t j = 0; j < ny; ++j)
+            for (size_t i = 0; i < nx; ++i) {
+                const size_t c = IX(i, j, k, nx, ny);
+                if (A[c].center == 0.0) {
+                    xo[c] = 0.0;
+                    continue;
+                }
+                const double s = OR_OFFDIAG_SUM(A, x, c, i, j, k, nx, ny, nz);
+                xo[c] = omega == 1.0 ? (b[c] - s) / A[c].center
+                                     : x[c] + omega * ((b[c] - s) / A[c].center - x[c]);
+            }
+}
+
+/* `count` Jacobi sweeps of x, through tmp; the result ends in x. */
+static void jacobi_sweeps(const or_row* A, const double* b, double omega, double* x,
+                          double* tmp, size_t nx, size_t ny, size_t nz,
+                          unsigned count) {
+    for (unsigned it = 0; it < count; ++it) {
+        jacobi_sweep_clean(A, b, omega, x, tmp, nx, ny, nz);
+        or_copy(tmp, x, nx * ny * nz);
+    }
+}
+
+/* The clean V-cycle with weighted Jacobi smoothing: zero initial guess,
+ * restrict_iter sweeps, residual and Jet's restriction, recursion, Jet's
+ * correction, correct_iter (final_iter on level 0) sweeps; coarsest_iter
+ * sweeps on the coarsest level.  With as many sweeps after as before the
+ * coarse correction the cycle is symmetric, and positive definite for
+ * 0 < omega <= 1 on diagonally dominant rows. */
+static void clean_vcycle_jacobi(const or_mg_system* s, const or_mg_params* p,
+                                double omega, size_t l, double** x, double** b,
+                                double** buf) {
+    const size_t nx = s->nx[l], ny = s->ny[l], nz = s->nz[l];
+    or_set(0.0, x[l], nx * ny * nz);
+    if (l + 1 < s->nlev) {
+        jacobi_sweeps(s->A[l], b[l], omega, x[l], buf[l], nx, ny, nz, p->restrict_iter);
+        or_residual(s->A[l], x[l], b[l], buf[l], nx, ny, nz);
+        mg_restrict_level(s, l, buf[l], b[l + 1]);
+        clean_vcycle_jacobi(s, p, omega, l + 1, x, b, buf);
+        mg_correct_level(s, l, x[l + 1], x[l]);
+        const unsigned post = l > 0 ? p->correct_iter : p->final_iter;
+        jacobi_sweeps(s->A[l], b[l], omega, x[l], buf[l], nx, ny, nz, post);
+    } else {
+        jacobi_sweeps(s->A[l], b[l], omega, x[l], buf[l], nx, ny, nz, p->coarsest_iter);
+    }
+}
+
+void or_mg_clean_vcycle_jacobi(const or_mg_system* sys, const or_mg_params* p,
+                               double omega, const double* r, double* z) {
+    double** mx = alloc_levels(sys);
+    double** mb = alloc_levels(sys);
+    double** mbuf = alloc_levels(sys);
+    const size_t n = sys->nx[0] * sys->ny[0] * sys->nz[0];
+    or_copy(r, mb[0], n);
+    clean_vcycle_jacobi(sys, p, omega, 0, mx, mb, mbuf);
+    or_copy(mx[0], z, n);
+    free_levels(sys, mx);
+    free_levels(sys, mb);
+    free_levels(sys, mbuf);
+}
+
 void or_mg_clean_vcycle(const or_mg_system* sys, const or_mg_params* p,
                         const double* r, double* z) {
     double** mx = alloc_levels(sys);
@@ -644,9 +714,26 @@ void or_mg_clean_vcycle(const or_mg_system* sys, const or_mg_params* p,
     free_levels(sys, mbuf);
 }
 
+static int mgpcg_clean(or_mg_system* sys, const or_mg_params* p, double jacobi_omega,
+                       unsigned max_iter, double rel_tol, unsigned* last_iter,
+                       double* true_rel_residual);
+
 int or_mgpcg_clean_solve(or_mg_system* sys, const or_mg_params* p,
                          unsigned max_iter, double rel_tol,
                          unsigned* last_iter, double* true_rel_residual) {
+    return mgpcg_clean(sys, p, 0.0, max_iter, rel_tol, last_iter, true_rel_residual);
+}
+
+int or_mgpcg_clean_solve_jacobi(or_mg_system* sys, const or_mg_params* p, double omega,
+                                unsigned max_iter, double rel_tol,
+                                unsigned* last_iter, double* true_rel_residual) {
+    return mgpcg_clean(sys, p, omega, max_iter, rel_tol, last_iter, true_rel_residual);
+}
+
+/* jacobi_omega > 0: the weighted-Jacobi V-cycle, else the red-black one. */
+static int mgpcg_clean(or_mg_system* sys, const or_mg_params* p, double jacobi_omega,
+                       unsigned max_iter, double rel_tol, unsigned* last_iter,
+                       double* true_rel_residual) {
     const size_t nx = sys->nx[0], ny = sys->ny[0], nz = sys->nz[0];
     const size_t n = nx * ny * nz;
     double* r = (double*)calloc(n, sizeof(double));
@@ -664,7 +751,10 @@ int or_mgpcg_clean_solve(or_mg_system* sys, const or_mg_params* p,
     double sigma = 0.0;
     while (iter < max_iter && rnorm > rel_tol * bnorm && bnorm > 0.0) {
         or_copy(r, mb[0], n);
-        clean_vcycle(sys, p, 0, mx, mb, mbuf); /* z = mx[0] */
+        if (jacobi_omega > 0.0)
+            clean_vcycle_jacobi(sys, p, jacobi_omega, 0, mx, mb, mbuf);
+        else
+            clean_vcycle(sys, p, 0, mx, mb, mbuf); /* z = mx[0] */
         double sigma_new = or_dot(r, mx[0], nx, ny, nz);
         if (iter == 0)
             or_copy(mx[0], d, n);

@@ -125,6 +125,16 @@ int or_mgpcg_clean_solve(or_mg_system* sys, const or_mg_params* p,
  * backward halves. */
 void or_mg_clean_vcycle(const or_mg_system* sys, const or_mg_params* p,
                         const double* r, double* z);
+/* The same with weighted Jacobi smoothing instead of red-black
+ * (x += omega ((b - s) / c - x), FdmJacobiSolver3::relax for omega = 1,
+ * src/jet/fdm_jacobi_solver3.cpp:95-112; zero-centre rows set to 0):
+ * restrict_iter sweeps before and correct_iter / final_iter sweeps after the
+ * coarse correction, coarsest_iter sweeps on the coarsest level. */
+void or_mg_clean_vcycle_jacobi(const or_mg_system* sys, const or_mg_params* p,
+                               double omega, const double* r, double* z);
+int or_mgpcg_clean_solve_jacobi(or_mg_system* sys, const or_mg_params* p, double omega,
+                                unsigned max_iter, double rel_tol,
+                                unsigned* last_iter, double* true_rel_residual);
 
 /* ------------------------------------------------ Single-phase pressure */
 enum { OR_FLUID = 0, OR_AIR = 1, OR_BOUNDARY = 2 };

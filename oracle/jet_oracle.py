@@ -119,6 +119,8 @@ def _setup(lib):
     lib.or_mgpcg_clean_solve.argtypes = [C.POINTER(MgSystemC), C.POINTER(MgParams), C.c_uint,
                                          C.c_double, C.POINTER(C.c_uint), _dp]
     lib.or_mg_clean_vcycle.argtypes = [C.POINTER(MgSystemC), C.POINTER(MgParams), _dp, _dp]
+    lib.or_mg_clean_vcycle_jacobi.argtypes = [C.POINTER(MgSystemC), C.POINTER(MgParams), C.c_double,
+                                              _dp, _dp]
     lib.or_levels_with_finest.restype = C.c_size_t
     lib.or_levels_with_finest2.restype = C.c_size_t
     lib.or_divergence.restype = C.c_double
@@ -361,6 +363,24 @@ class Oracle(_BoundaryCalls, _CsrCalls):
         ok = self.lib.or_mgpcg_clean_solve(C.byref(s), C.byref(params), C.c_uint(max_iter),
                                            C.c_double(rel_tol), C.byref(it), C.byref(res))
         return bool(ok), it.value, res.value
+
+    def mgpcg_clean_solve_jacobi(self, sys: MgSystem, params: MgParams, omega, max_iter, rel_tol):
+        """The clean solve with the weighted-Jacobi V-cycle; (ok, iters, true_rel_residual)."""
+        s = sys.to_c()
+        it, res = C.c_uint(), C.c_double()
+        ok = self.lib.or_mgpcg_clean_solve_jacobi(C.byref(s), C.byref(params), C.c_double(omega),
+                                                  C.c_uint(max_iter), C.c_double(rel_tol),
+                                                  C.byref(it), C.byref(res))
+        return bool(ok), it.value, res.value
+
+    def mg_clean_vcycle_jacobi(self, sys: MgSystem, params: MgParams, omega, r):
+        """z = M r with the weighted-Jacobi clean V-cycle."""
+        s = sys.to_c()
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.empty_like(r)
+        self.lib.or_mg_clean_vcycle_jacobi(C.byref(s), C.byref(params), C.c_double(omega), _d(r),
+                                           _d(z))
+        return z
 
     def mg_clean_vcycle(self, sys: MgSystem, params: MgParams, r):
         """z = M r: one application of the clean-mode symmetric V-cycle."""

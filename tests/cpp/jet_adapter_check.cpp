@@ -22,6 +22,7 @@
 #include <jet/fdm_cg_solver3.h>
 #include <jet/fdm_linear_system3.h>
 #include <jet/fdm_mg_linear_system3.h>
+#include <jet/fdm_mg_solver3.h>
 #include <jet/fdm_mgpcg_solver3.h>
 #include <jet/logging.h>
 
@@ -29,8 +30,11 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <iostream>
 #include <memory>
+#include <sstream>
 #include <stdexcept>
+#include <string>
 
 #include "fdm_gpu_solvers2.h"
 #include "fdm_gpu_solvers3.h"
@@ -194,6 +198,20 @@ void argumentChecks() {
     mbad.b.levels[0].resize(Size3(8, 8, 4));
     expect(throwsInvalidArgument([&] { mg.solve(&mbad); }),
            "FdmGpuMgpcgSolver3: level-0 size mismatch throws std::invalid_argument");
+
+    FdmGpuMgSolver3 one(6, 5, 4, 20, 10, 1e-7, 1.2, true);
+    const MgParameters<FdmBlas3>& q = one.params();
+    expect(q.maxNumberOfLevels == 6 && q.numberOfRestrictionIter == 5 &&
+               q.numberOfCorrectionIter == 4 && q.numberOfCoarsestIter == 20 &&
+               q.numberOfFinalIter == 10 && q.maxTolerance == 1e-7 &&
+               one.sorFactor() == 1.2 && one.useRedBlackOrdering(),
+           "FdmGpuMgSolver3::params() carries Jet's MgParameters");
+    FdmLinearSystemSolver3Ptr oneBase = std::make_shared<FdmGpuMgSolver3>(4);
+    expect(std::dynamic_pointer_cast<FdmMgSolver3>(oneBase) != nullptr && !oneBase->solve(&flat),
+           "FdmGpuMgSolver3 is an FdmMgSolver3 whose flat solve is the no-op (false)");
+    expect(throwsInvalidArgument([&] { one.solve(nullptr); }) &&
+               throwsInvalidArgument([&] { one.solve(&mbad); }),
+           "FdmGpuMgSolver3: null system and size mismatch throw std::invalid_argument");
 }
 
 void gpuChecks() {
@@ -439,6 +457,75 @@ void gpuChecks() {
         tight->solve(&ms);
         expect(closedFormError(ms.x[0]) <= 1e-6,
                mixed ? "MGPCG mixed: x = (ny-1)/2 - j (1e-6)" : "MGPCG fp64: x = (ny-1)/2 - j (1e-6)");
+    }
+
+    // --- FdmMgSolver3: Jet's FdmMgSolver3.Solve test
+    // (src/tests/unit_tests/fdm_mg_solver3_tests.cpp:13-69: 6 levels from a
+    // 4^3 coarsest level, one V-cycle must reduce ||b - A x||) through Jet's
+    // CPU class and the GPU class; the GPU cycle starts from the caller's x,
+    // so a second solve reduces the residual again
+    for (int mixed = 0; mixed <= 1; ++mixed) {
+        FdmMgLinearSystem3 ms;
+        ms.resizeWithCoarsest(Size3(4, 4, 4), 6);
+        for (size_t l = 0; l < ms.numberOfLevels(); ++l) {
+            const double invdx = std::pow(0.5, (double)l);
+            buildLaplacian(&ms.A[l], &ms.x[l], &ms.b[l], ms.A[l].size(), invdx * invdx);
+        }
+        FdmMgLinearSystem3 jetSys = ms;
+        FdmVector3 buf = ms.x[0];
+        FdmBlas3::residual(ms.A[0], ms.x[0], ms.b[0], &buf);
+        const double norm0 = FdmBlas3::l2Norm(buf);
+        double jetNorm = 0.0;
+        if (!mixed) {
+            FdmMgSolver3 jet(6, 5, 5, 20, 20, 1e-9);
+            jet.solve(&jetSys);
+            FdmBlas3::residual(jetSys.A[0], jetSys.x[0], jetSys.b[0], &buf);
+            jetNorm = FdmBlas3::l2Norm(buf);
+        }
+        auto gpu = std::make_shared<FdmGpuMgSolver3>(6, 5, 5, 20, 20, 1e-9);
+        if (mixed) {
+            jg_gpu_options o = gpu->gpuOptions();
+            o.precision = JG_MIXED;
+            gpu->setGpuOptions(o);
+        }
+        FdmLinearSystemSolver3Ptr base = gpu;
+        std::dynamic_pointer_cast<FdmMgSolver3>(base)->solve(&ms);
+        FdmBlas3::residual(ms.A[0], ms.x[0], ms.b[0], &buf);
+        const double norm1 = FdmBlas3::l2Norm(buf);
+        gpu->solve(&ms);
+        FdmBlas3::residual(ms.A[0], ms.x[0], ms.b[0], &buf);
+        const double norm2 = FdmBlas3::l2Norm(buf);
+        std::printf("     mg (%s) ||r||: start %.4e, Jet's cycle %.4e, GPU cycle %.4e, "
+                    "second GPU cycle %.4e\n",
+                    mixed ? "mixed" : "fp64", norm0, jetNorm, norm1, norm2);
+        expect(norm1 < norm0, mixed ? "MG mixed: Jet's FdmMgSolver3.Solve test passes"
+                                    : "MG fp64: Jet's FdmMgSolver3.Solve test passes");
+        expect(norm2 < norm1 && std::fabs(gpu->lastResidual() - norm2) <= 1e-6 * norm2,
+               mixed ? "MG mixed: warm start, lastResidual() = ||b - A x||"
+                     : "MG fp64: warm start, lastResidual() = ||b - A x||");
+    }
+
+    // --- Jet's MGPCG log line from the GPU class
+    // (src/jet/fdm_mgpcg_solver3.cpp:75-76)
+    {
+        FdmMgLinearSystem3 ms;
+        ms.resizeWithCoarsest(Size3(4, 4, 4), 3);
+        for (size_t l = 0; l < ms.numberOfLevels(); ++l) {
+            const double invdx = std::pow(0.5, (double)l);
+            buildLaplacian(&ms.A[l], &ms.x[l], &ms.b[l], ms.A[l].size(), invdx * invdx);
+        }
+        std::ostringstream log;
+        Logging::setAllStream(&log);
+        Logging::unmute();
+        FdmGpuMgpcgSolver3 mg(100, 3, 2, 2, 20, 2, 1e-8, 1.3, true);
+        mg.solve(&ms);
+        Logging::mute();
+        Logging::setAllStream(&std::cout);
+        const std::string want = "Number of MGPCG iterations: " +
+                                 std::to_string(mg.lastNumberOfIterations());
+        expect(log.str().find("Residual after solving MGPCG: ") != std::string::npos &&
+                   log.str().find(want) != std::string::npos,
+               "MGPCG: Jet's INFO line with the residual and the iteration count");
     }
 
     // --- MGPCG against Jet's CG on the same fine system (a different

@@ -192,9 +192,10 @@ def test_mgpcg_nonfinite_rhs_is_an_error(solver):
 
 def test_mgpcg_mixed_fp32_overflow_is_an_error(solver):
     # b inside FP64's range but beyond FP32's: the FP32 inner solve meets inf, and the mixed solve
-    # must report a non-finite value instead of returning x = 0 as converged
+    # must report a non-finite value instead of returning x = 0 as converged.  The all-Neumann
+    # system needs a b that sums to zero: Jet's +-1 rows, scaled
     s = mg_poisson_hierarchy(4, 2)
-    s.b[0][:] = 1e39
+    s.b[0][:] *= 1e39
     upload_mg(solver, s)
     p = jetgpu.mg_params(max_levels=2)
     with pytest.raises(jetgpu.JgError):

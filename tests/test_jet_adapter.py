@@ -9,7 +9,9 @@ solveCompressed and from the multigrid solver's flat solve, params() and the
 accessors), selection through dynamic_pointer_cast<FdmMgSolver3>, GPU CG
 against Jet's CG (iterations within 2, x within 1e-8), Jet's MGPCG unit test
 through the GPU class in FP64 and mixed precision, the closed form
-x = (ny - 1)/2 - j of Jet's test systems, and GPU MGPCG against Jet's CG.
+x = (ny - 1)/2 - j of Jet's test systems, GPU MGPCG against Jet's CG,
+Jet's FdmMgSolver3.Solve test through the one-V-cycle class FdmGpuMgSolver3
+(warm start included), and Jet's MGPCG log line.
 """
 import os
 import subprocess
@@ -32,7 +34,7 @@ def run_check(*args):
 
 def test_adapter_argument_checks():
     out = run_check("--no-gpu")
-    assert out.count("ok  ") == 13
+    assert out.count("ok  ") == 16
 
 
 @pytest.mark.gpu
