@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -26,7 +27,47 @@
 
 using namespace jg;
 
+// ------------------------------------------- scratch of the no-allocation entry
+//
+// jg_relax_red_black_noalloc keeps the copy of x that its sweep reads in this
+// array instead of a hipMalloc'd temporary.  Every GPU holds its own copy of
+// the array (16 MiB) from module load until the process ends.  Host code
+// never passes the array's name to a kernel or a copy: that name is a host-side
+// stand-in.  The device address comes from hipGetSymbolAddress, resolved once
+// per device by jg_solver_create with that device current.
+constexpr size_t kNoAllocMaxCells = size_t(1) << 21;
+static __device__ __align__(256) double scratch[kNoAllocMaxCells];
+
 namespace {
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kScratchDevices] = {};
+hipError_t g_scratch_err[kScratchDevices] = {};
+bool g_scratch_done[kScratchDevices] = {};
+
+// Resolves the address of `scratch` on `device` (which must be current) the
+// first time it is asked for that device.
+hipError_t scratch_init(int device) {
+    if (device < 0 || device >= kScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_done[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        g_scratch_err[device] = e;
+        g_scratch_ptr[device] = e == hipSuccess ? static_cast<double*>(p) : nullptr;
+        g_scratch_done[device] = true;
+    }
+    return g_scratch_err[device];
+}
+
+hipError_t scratch_address(int device, double** out) {
+    if (device < 0 || device >= kScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_done[device]) return hipErrorNotInitialized;
+    *out = g_scratch_ptr[device];
+    return g_scratch_err[device];
+}
 
 // ------------------------------------------------------- level kernels
 
@@ -1141,6 +1182,9 @@ int jg_solver_create(int device, jg_solver** out) {
         delete s;
         return JG_ERR_HIP;
     }
+    // this device's address of the no-allocation scratch array (a failure is
+    // reported by jg_relax_red_black_noalloc, the only user)
+    (void)scratch_init(device);
     *out = s;
     return JG_OK;
 }
@@ -1807,6 +1851,32 @@ int jg_relax_red_black(jg_solver* s, size_t nx, size_t ny, size_t nz, const doub
         return JG_ERR_HIP;
     }
     return JG_OK;
+}
+
+int jg_relax_red_black_noalloc(jg_solver* s, size_t nx, size_t ny, size_t nz, const double* rows,
+                               const double* b, double sor, double* x) {
+    if (!s || !valid_dims(nx, ny, nz) || !rows || !b || !x) return JG_ERR_INVALID_ARG;
+    const size_t n = nx * ny * nz;
+    if (n > kNoAllocMaxCells) {
+        s->err = "relax_red_black_noalloc: " + std::to_string(n) + " cells exceed the " +
+                 std::to_string(kNoAllocMaxCells) + "-cell scratch array";
+        return JG_ERR_INVALID_ARG;
+    }
+    JG_HIP(hipSetDevice(s->device));
+    double* tmp = nullptr;
+    const hipError_t se = scratch_address(s->device, &tmp);
+    if (se != hipSuccess) {
+        s->err = std::string("relax_red_black_noalloc: no scratch address: ") + hipGetErrorString(se);
+        return JG_ERR_HIP;
+    }
+    Dims d{(int)nx, (int)ny, (int)nz};
+    // the copy of x that the out-of-place sweep reads, then the sweep itself:
+    // the kernels of jg_relax_red_black, with its temporary replaced by scratch
+    k_convert<double, double><<<nblocks_1d((idx_t)n), NT, 0, s->stream>>>(x, tmp, (idx_t)n);
+    launch_sweep<double>(s, (const Row<double>*)rows, b, tmp, x, d, 0, nullptr, nullptr, nullptr, sor,
+                         nullptr);
+    JG_TRY(check_launch(s, "relax_red_black_noalloc"));
+    return sync(s);
 }
 
 int jg_relax_jacobi(jg_solver* s, size_t nx, size_t ny, size_t nz, const double* rows,

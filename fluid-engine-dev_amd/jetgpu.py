@@ -121,6 +121,7 @@ SIGNATURES = {
     "jg_blas_dot": (C.c_int, [_vp, _sz, _vp, _vp, C.POINTER(_d)]),
     "jg_blas_axpy": (C.c_int, [_vp, _sz, _d, _vp, _vp, _vp]),
     "jg_relax_red_black": (C.c_int, [_vp, _sz, _sz, _sz, _vp, _vp, _d, _vp]),
+    "jg_relax_red_black_noalloc": (C.c_int, [_vp, _sz, _sz, _sz, _vp, _vp, _d, _vp]),
     "jg_relax_jacobi": (C.c_int, [_vp, _sz, _sz, _sz, _vp, _vp, _vp, _vp]),
     "jg_mg_restrict": (C.c_int, [_vp, _sz, _sz, _sz, _vp, _vp]),
     "jg_mg_correct": (C.c_int, [_vp, _sz, _sz, _sz, _vp, _vp]),
@@ -529,6 +530,11 @@ class Solver:
         nx, ny, nz = shape
         self.check(lib().jg_relax_red_black(self.h, nx, ny, nz, _ptr(A), _ptr(b), sor, _ptr(x)),
                    "relax_red_black")
+
+    def relax_red_black_noalloc(self, shape, A, b, sor, x):
+        nx, ny, nz = shape
+        self.check(lib().jg_relax_red_black_noalloc(self.h, nx, ny, nz, _ptr(A), _ptr(b), sor,
+                                                    _ptr(x)), "relax_red_black_noalloc")
 
     def relax_jacobi(self, shape, A, b, x, xt):
         nx, ny, nz = shape

@@ -246,6 +246,17 @@ int jg_blas_axpy(jg_solver* s, size_t n, double a, const double* x,
 int jg_relax_red_black(jg_solver* s, size_t nx, size_t ny, size_t nz,
                        const double* rows, const double* b, double sor,
                        double* x);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): jg_relax_red_black with the same arguments and the same
+ * result, bit for bit, but the copy of x that the sweep reads lives in a
+ * static per-GPU array of 2^21 doubles (16 MiB) instead of a temporary, and
+ * nothing is allocated or freed.  Grids of more than 2^21 cells return
+ * JG_ERR_INVALID_ARG.  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (two streams, two threads) race. */
+int jg_relax_red_black_noalloc(jg_solver* s, size_t nx, size_t ny, size_t nz,
+                               const double* rows, const double* b, double sor,
+                               double* x);
 /* FdmJacobiSolver3::relax -- src/jet/fdm_jacobi_solver3.cpp:95-112 */
 int jg_relax_jacobi(jg_solver* s, size_t nx, size_t ny, size_t nz,
                     const double* rows, const double* b, const double* x,
