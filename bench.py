@@ -304,7 +304,7 @@ def traffic_probe(args):
     w = torch.empty((n + 1, n, n), dtype=f64, device="cuda")
     bs = torch.empty((n, n, n), dtype=f64, device="cuda")
     fs = torch.empty((n, n, n), dtype=f64, device="cuda")
-    mark = torch.zeros(64, dtype=torch.float32, device="cuda")
+    mark = torch.empty(64, dtype=torch.float32, device="cuda")  # no fill kernel of its own
     torch.cuda.synchronize()
     solver.generate_scene(jg.scene_desc(jg.JG_SCENE_SPHERE_POOL, n, seed=args.seed), u, v, w, bs, fs)
     solver.sp_build(u, v, w, bs, fs, 1.0)

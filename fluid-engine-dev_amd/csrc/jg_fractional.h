@@ -340,4 +340,201 @@ __global__ __launch_bounds__(256) void k_fr_gradient(
     }
 }
 
+// ------------------------------------------------------------------ 2-D
+//
+// GridFractionalSinglePhasePressureSolver2 on a one-plane level (d.nz == 1).
+
+// Face weights of one axis, 2-D (buildWeights,
+// src/jet/grid_fractional_single_phase_pressure_solver2.cpp:440-480): 1 -
+// fractionInsideSdf of the boundary SDF at the centres of the two cells the
+// face separates, clamped, floored to 0.01 when positive, as float.  bsdf is
+// the boundary SDF at the cell centres of the grid plus one ghost cell per
+// side: cell (i, j) at bsdf[(i + 1) + (nx + 2)(j + 1)].
+template <int AXIS>
+__global__ __launch_bounds__(256) void k_fr_weights2(const double* __restrict__ bsdf, Dims d,
+                                                     float* __restrict__ out) {
+#pragma clang fp contract(off)
+    const int fx = d.nx + (AXIS == 0), fy = d.ny + (AXIS == 1);
+    const idx_t n = (idx_t)fx * fy;
+    const idx_t gx = d.nx + 2;
+    for (idx_t f = blockIdx.x * (idx_t)256 + threadIdx.x; f < n; f += (idx_t)gridDim.x * 256) {
+        const int i = (int)(f % fx), j = (int)(f / fx);
+        // the cell before the face (i - 1 or j - 1) and the one after it
+        const idx_t g1 = (idx_t)(i + 1) + gx * (j + 1);
+        const idx_t g0 = AXIS == 0 ? g1 - 1 : g1 - gx;
+        double w = 1.0 - fr_inside_sdf(bsdf[g0], bsdf[g1]);
+        w = w < 0.0 ? 0.0 : (w > 1.0 ? 1.0 : w);
+        if (w < 0.01 && w > 0.0) w = 0.01;
+        out[f] = (float)w;
+    }
+}
+
+// Float restriction of a one-plane cell or face array (the anonymous
+// restrict() of src/jet/grid_fractional_single_phase_pressure_solver2.cpp:
+// 29-93): the centred kernel, or injection along an axis whose fine size is
+// not twice the coarse size; y outer, x inner, weight kx * ky, float sums.
+__global__ __launch_bounds__(256) void k_fr_restrict2(const float* __restrict__ fine, int fnx, int fny,
+                                                      float* __restrict__ coarse, int cnx, int cny) {
+#pragma clang fp contract(off)
+    const idx_t n = (idx_t)cnx * cny;
+    const bool stx = fnx != 2 * cnx, sty = fny != 2 * cny;
+    for (idx_t e = blockIdx.x * (idx_t)256 + threadIdx.x; e < n; e += (idx_t)gridDim.x * 256) {
+        const int i = (int)(e % cnx), j = (int)(e / cnx);
+        int ii[4], jj[4];
+        float kx[4], ky[4];
+        ii[0] = i > 0 ? 2 * i - 1 : 2 * i;
+        ii[1] = 2 * i;
+        jj[0] = j > 0 ? 2 * j - 1 : 2 * j;
+        jj[1] = 2 * j;
+        if (stx) {
+            ii[2] = i + 1 < cnx ? 2 * i + 1 : 2 * i;
+            ii[3] = 2 * i;
+            kx[0] = 0.f, kx[1] = 1.f, kx[2] = 0.f, kx[3] = 0.f;
+        } else {
+            ii[2] = 2 * i + 1;
+            ii[3] = i + 1 < cnx ? 2 * i + 2 : 2 * i + 1;
+            kx[0] = 0.125f, kx[1] = 0.375f, kx[2] = 0.375f, kx[3] = 0.125f;
+        }
+        if (sty) {
+            jj[2] = j + 1 < cny ? 2 * j + 1 : 2 * j;
+            jj[3] = 2 * j;
+            ky[0] = 0.f, ky[1] = 1.f, ky[2] = 0.f, ky[3] = 0.f;
+        } else {
+            jj[2] = 2 * j + 1;
+            jj[3] = j + 1 < cny ? 2 * j + 2 : 2 * j + 1;
+            ky[0] = 0.125f, ky[1] = 0.375f, ky[2] = 0.375f, ky[3] = 0.125f;
+        }
+        const int nkx = stx ? 3 : 4, nky = sty ? 3 : 4;
+        float sum = 0.0f;
+        for (int y = 0; y < nky; ++y)
+            for (int x = 0; x < nkx; ++x) {
+                const float w = kx[x] * ky[y];
+                sum += w * fine[ii[x] + (idx_t)fnx * jj[y]];
+            }
+        coarse[e] = sum;
+    }
+}
+
+// buildSingleSystem, 2-D variational form
+// (src/jet/grid_fractional_single_phase_pressure_solver2.cpp:95-203) on one
+// plane: the +x, -x, +y, -y faces in Jet's order, the collider term
+// sum (1 - w) u_collider / h, and the row {1, 0} below machine epsilon or
+// outside the fluid.  Rows are {center, right, up, 0}.  b is written only
+// when u is given; bu / bv may be null (collider at rest).
+__global__ __launch_bounds__(256) void k_fr_build2(
+    const float* __restrict__ fs, const float* __restrict__ uw, const float* __restrict__ vw,
+    const double* __restrict__ u, const double* __restrict__ v, const double* __restrict__ bu,
+    const double* __restrict__ bv, Dims d, double hx, double hy, Row<double>* __restrict__ A,
+    double* __restrict__ b) {
+#pragma clang fp contract(off)
+    const double invH[2] = {1.0 / hx, 1.0 / hy};
+    const double invHSqr[2] = {invH[0] * invH[0], invH[1] * invH[1]};
+    const idx_t n = (idx_t)d.nx * d.ny;
+    const idx_t ux = d.nx + 1;
+    for (idx_t c = blockIdx.x * (idx_t)256 + threadIdx.x; c < n; c += (idx_t)gridDim.x * 256) {
+        const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+        Row<double> row{0.0, 0.0, 0.0, 0.0};
+        double bc = 0.0;
+        const double cphi = fs[c];
+        if (cphi < 0) {
+            const idx_t uR = (i + 1) + ux * j, uL = uR - 1;
+            const idx_t vU = i + (idx_t)d.nx * (j + 1), vD = vU - d.nx;
+            const bool rhs = u != nullptr;
+            double term;
+            if (i + 1 < d.nx) {
+                term = uw[uR] * invHSqr[0];
+                const double phi = fs[c + 1];
+                if (phi < 0) {
+                    row.c += term;
+                    row.r -= term;
+                } else {
+                    row.c += term / fr_theta(cphi, phi);
+                }
+                if (rhs) bc += uw[uR] * u[uR] * invH[0];
+            } else if (rhs) {
+                bc += u[uR] * invH[0];
+            }
+            if (i > 0) {
+                term = uw[uL] * invHSqr[0];
+                const double phi = fs[c - 1];
+                if (phi < 0)
+                    row.c += term;
+                else
+                    row.c += term / fr_theta(cphi, phi);
+                if (rhs) bc -= uw[uL] * u[uL] * invH[0];
+            } else if (rhs) {
+                bc -= u[uL] * invH[0];
+            }
+            if (j + 1 < d.ny) {
+                term = vw[vU] * invHSqr[1];
+                const double phi = fs[c + d.nx];
+                if (phi < 0) {
+                    row.c += term;
+                    row.u -= term;
+                } else {
+                    row.c += term / fr_theta(cphi, phi);
+                }
+                if (rhs) bc += vw[vU] * v[vU] * invH[1];
+            } else if (rhs) {
+                bc += v[vU] * invH[1];
+            }
+            if (j > 0) {
+                term = vw[vD] * invHSqr[1];
+                const double phi = fs[c - d.nx];
+                if (phi < 0)
+                    row.c += term;
+                else
+                    row.c += term / fr_theta(cphi, phi);
+                if (rhs) bc -= vw[vD] * v[vD] * invH[1];
+            } else if (rhs) {
+                bc -= v[vD] * invH[1];
+            }
+            if (rhs) {
+                const double buR = bu ? bu[uR] : 0.0, buL = bu ? bu[uL] : 0.0;
+                const double bvU = bv ? bv[vU] : 0.0, bvD = bv ? bv[vD] : 0.0;
+                const double contrib = (1.0 - uw[uR]) * buR * invH[0] - (1.0 - uw[uL]) * buL * invH[0] +
+                                       (1.0 - vw[vU]) * bvU * invH[1] - (1.0 - vw[vD]) * bvD * invH[1];
+                bc += contrib;
+            }
+            if (row.c < 2.220446049250313080847e-16) {  // DBL_EPSILON (include/jet/constants.h:74)
+                row.c = 1.0;
+                bc = 0.0;
+            }
+        } else {
+            row.c = 1.0;
+        }
+        A[c] = row;
+        if (b) b[c] = bc;
+    }
+}
+
+// applyPressureGradient, 2-D variational form
+// (src/jet/grid_fractional_single_phase_pressure_solver2.cpp:565-600): the
+// +x / +y face of cell c is updated when its weight is positive and either
+// side is fluid.  The output may alias the input.
+__global__ __launch_bounds__(256) void k_fr_gradient2(
+    const float* __restrict__ fs, const float* __restrict__ uw, const float* __restrict__ vw,
+    const double* __restrict__ p, const double* u, const double* v, double* uo, double* vo,
+    Dims d, double hx, double hy) {
+#pragma clang fp contract(off)
+    const double invH[2] = {1.0 / hx, 1.0 / hy};
+    const idx_t n = (idx_t)d.nx * d.ny;
+    for (idx_t c = blockIdx.x * (idx_t)256 + threadIdx.x; c < n; c += (idx_t)gridDim.x * 256) {
+        const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+        const double cphi = fs[c];
+        if (i + 1 < d.nx) {
+            const idx_t f = (i + 1) + (idx_t)(d.nx + 1) * j;
+            const double nphi = fs[c + 1];
+            if (uw[f] > 0.0 && (cphi < 0 || nphi < 0))
+                uo[f] = u[f] + invH[0] / fr_theta(cphi, nphi) * (p[c + 1] - p[c]);
+        }
+        if (j + 1 < d.ny) {
+            const idx_t f = i + (idx_t)d.nx * (j + 1);
+            const double nphi = fs[c + d.nx];
+            if (vw[f] > 0.0 && (cphi < 0 || nphi < 0))
+                vo[f] = v[f] + invH[1] / fr_theta(cphi, nphi) * (p[c + d.nx] - p[c]);
+        }
+    }
+}
+
 }  // namespace jg

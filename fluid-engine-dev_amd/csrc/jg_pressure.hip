@@ -354,6 +354,69 @@ int jg_fr_apply_gradient(jg_solver* s, const double* u, const double* v, const d
     return sync(s);
 }
 
+int jg_fr_build2(jg_solver* s, const double* u, const double* v, const double* boundary_sdf,
+                 const double* fluid_sdf, const double* bu, const double* bv, double hx, double hy) {
+    if (!s || s->lv.empty() || !u || !v || !boundary_sdf || !fluid_sdf || !(hx > 0) || !(hy > 0))
+        return JG_ERR_INVALID_ARG;
+    if (s->lv[0].d.nz != 1 || (s->lv.size() > 1 && !s->twod)) {
+        s->err = "jg_fr_build2: the resident hierarchy is not a 2-D (one-plane) one";
+        return JG_ERR_INVALID_ARG;
+    }
+    JG_HIP(hipSetDevice(s->device));
+    for (auto& lv : s->lv) {
+        if (lv.fs) continue;
+        const Dims& d = lv.d;
+        JG_TRY(dalloc(s, &lv.fs, d.n()));
+        JG_TRY(dalloc(s, &lv.uw, (size_t)(d.nx + 1) * d.ny));
+        JG_TRY(dalloc(s, &lv.vw, (size_t)d.nx * (d.ny + 1)));
+        JG_TRY(dalloc(s, &lv.ww, (size_t)d.nx * d.ny * 2));
+    }
+    const Dims& d0 = s->lv[0].d;
+    k_fr_weights2<0><<<nblocks_1d((idx_t)(d0.nx + 1) * d0.ny), 256, 0, s->stream>>>(boundary_sdf, d0,
+                                                                                  s->lv[0].uw);
+    k_fr_weights2<1><<<nblocks_1d((idx_t)d0.nx * (d0.ny + 1)), 256, 0, s->stream>>>(boundary_sdf, d0,
+                                                                                  s->lv[0].vw);
+    k_fr_sdf_to_float<<<nblocks_1d(d0.n()), 256, 0, s->stream>>>(fluid_sdf, s->lv[0].fs, d0.n());
+    for (size_t l = 1; l < s->lv.size(); ++l) {
+        const Dims& fd = s->lv[l - 1].d;
+        const Dims& cd = s->lv[l].d;
+        k_fr_restrict2<<<nblocks_1d(cd.n()), 256, 0, s->stream>>>(s->lv[l - 1].fs, fd.nx, fd.ny,
+                                                                  s->lv[l].fs, cd.nx, cd.ny);
+        k_fr_restrict2<<<nblocks_1d((idx_t)(cd.nx + 1) * cd.ny), 256, 0, s->stream>>>(
+            s->lv[l - 1].uw, fd.nx + 1, fd.ny, s->lv[l].uw, cd.nx + 1, cd.ny);
+        k_fr_restrict2<<<nblocks_1d((idx_t)cd.nx * (cd.ny + 1)), 256, 0, s->stream>>>(
+            s->lv[l - 1].vw, fd.nx, fd.ny + 1, s->lv[l].vw, cd.nx, cd.ny + 1);
+    }
+    double h[2] = {hx, hy};
+    for (size_t l = 0; l < s->lv.size(); ++l) {
+        Level& lv = s->lv[l];
+        const bool top = l == 0;
+        k_fr_build2<<<nblocks_1d(lv.d.n()), 256, 0, s->stream>>>(
+            lv.fs, lv.uw, lv.vw, top ? u : nullptr, v, top ? bu : nullptr, top ? bv : nullptr, lv.d,
+            h[0], h[1], lv.A64, top ? s->b : nullptr);
+        h[0] *= 2.0;
+        h[1] *= 2.0;
+    }
+    s->f32_valid = false;
+    JG_TRY(check_launch(s, "fr_build2"));
+    return sync(s);
+}
+
+int jg_fr_apply_gradient2(jg_solver* s, const double* u, const double* v, double* uo, double* vo,
+                          double hx, double hy) {
+    if (!s || s->lv.empty() || !u || !v || !uo || !vo) return JG_ERR_INVALID_ARG;
+    if (!s->lv[0].fs || s->lv[0].d.nz != 1) {
+        s->err = "jg_fr_apply_gradient2 before jg_fr_build2";
+        return JG_ERR_INVALID_ARG;
+    }
+    JG_HIP(hipSetDevice(s->device));
+    const Level& l0 = s->lv[0];
+    k_fr_gradient2<<<nblocks_1d(l0.d.n()), 256, 0, s->stream>>>(l0.fs, l0.uw, l0.vw, s->x, u, v, uo,
+                                                                 vo, l0.d, hx, hy);
+    JG_TRY(check_launch(s, "fr_gradient2"));
+    return sync(s);
+}
+
 int jg_constrain_velocity(jg_solver* s, int blocked, const double h[3], const double o[3],
                           double* u, double* v, double* w, const double* collider_sdf,
                           const jg_rigid_collider* collider, unsigned extrapolation_depth,

@@ -136,6 +136,8 @@ SIGNATURES = {
     "jg_fr_build": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d]),
     "jg_fr_apply_gradient": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d]),
     "jg_fr_array": (_vp, [_vp, _sz, C.c_int]),
+    "jg_fr_build2": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d]),
+    "jg_fr_apply_gradient2": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _d, _d]),
     "jg_constrain_velocity": (C.c_int, [_vp, C.c_int, C.POINTER(_d), C.POINTER(_d), _vp, _vp, _vp,
                                         _vp, C.POINTER(RigidCollider), _u, C.c_int]),
     "jg_batch_create": (C.c_int, [C.c_int, _sz, C.POINTER(_sz), C.POINTER(_sz), C.POINTER(_sz),
@@ -499,6 +501,14 @@ class Solver:
                                                _ptr(w), _ptr(sdf),
                                                C.byref(collider) if collider is not None else None,
                                                depth, closed), "constrain_velocity")
+
+    def fr_build2(self, u, v, bsdf_ghost, fsdf, h, bu=None, bv=None):
+        self.check(lib().jg_fr_build2(self.h, _ptr(u), _ptr(v), _ptr(bsdf_ghost), _ptr(fsdf),
+                                      _ptr(bu), _ptr(bv), h[0], h[1]), "fr_build2")
+
+    def fr_apply_gradient2(self, u, v, uo, vo, h):
+        self.check(lib().jg_fr_apply_gradient2(self.h, _ptr(u), _ptr(v), _ptr(uo), _ptr(vo), h[0],
+                                               h[1]), "fr_apply_gradient2")
 
     def fr_array(self, level, which):
         """Device pointer of a level's face weights (which 0-2) or fluid SDF (3)."""

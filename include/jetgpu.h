@@ -401,7 +401,26 @@ int jg_fr_build(jg_solver* s, const double* u, const double* v, const double* w,
 int jg_fr_apply_gradient(jg_solver* s, const double* u, const double* v,
                          const double* w, double* uo, double* vo, double* wo,
                          double hx, double hy, double hz);
-/* Device pointers of a level's float arrays built by jg_fr_build: axis 0/1/2
+/* GridFractionalSinglePhasePressureSolver2::buildWeights + buildSystem
+ * (src/jet/grid_fractional_single_phase_pressure_solver2.cpp:414-563) on the
+ * solver's resident 2-D hierarchy (one-plane levels, or one nx*ny*1 level).
+ * Device inputs: face velocities u ((nx+1)*ny) and v (nx*(ny+1)); the collider
+ * SDF at the cell centres of the grid extended by one ghost cell on every
+ * side ((nx+2)*(ny+2), cell (i,j) at (i+1) + (nx+2)(j+1): the points a 2-D
+ * face weight samples, :440-480); the fluid SDF at the cell centres; the
+ * collider velocity x at the u faces and y at the v faces (bu, bv; NULL for a
+ * resting collider).  Face weights and fluid SDF on every level (float
+ * restriction, :29-93), A on every level (spacing 2^l h), b on level 0. */
+int jg_fr_build2(jg_solver* s, const double* u, const double* v,
+                 const double* boundary_sdf_ghost, const double* fluid_sdf,
+                 const double* bu, const double* bv, double hx, double hy);
+/* applyPressureGradient, 2-D variational form
+ * (src/jet/grid_fractional_single_phase_pressure_solver2.cpp:565-600) with the
+ * resident solution x and the level-0 weights of jg_fr_build2; the output may
+ * alias the input. */
+int jg_fr_apply_gradient2(jg_solver* s, const double* u, const double* v,
+                          double* uo, double* vo, double hx, double hy);
+/* Device pointers of a level's float arrays built by jg_fr_build or jg_fr_build2: axis 0/1/2
  * = u/v/w face weights, 3 = fluid SDF (NULL before the first build). */
 const float* jg_fr_array(jg_solver* s, size_t level, int which);
 

@@ -1529,6 +1529,221 @@ void or_fr_apply_gradient(const float* fsdf, const float* uw, const float* vw,
             }
 }
 
+/* ====================================== Fractional pressure, 2-D ======= */
+
+/* buildWeights top level, 2-D -- src/jet/grid_fractional_single_phase_pressure_solver2.cpp:
+ * 440-480.  Unlike the 3-D solver (fractionInside over a face's corners), a
+ * 2-D face weight is 1 - fractionInsideSdf of the two boundary-SDF samples half
+ * a cell before and after the face along its normal, which are the centres
+ * of the two cells the face separates.  bsdf holds the boundary SDF at the
+ * cell centres of the grid extended by one ghost cell on every side:
+ * cell (i, j), -1 <= i <= nx, -1 <= j <= ny, at bsdf[(i + 1) + (nx + 2)(j + 1)]. */
+void or_fr_face_weights2(const double* bsdf, size_t nx, size_t ny, float* uw,
+                         float* vw) {
+    const size_t gx = nx + 2;
+#define G(a, b) bsdf[(size_t)((a) + 1) + gx * (size_t)((b) + 1)]
+#pragma omp parallel for schedule(static)
+    for (size_t j = 0; j < ny; ++j)
+        for (size_t i = 0; i <= nx; ++i) {
+            double frac = or_fraction_inside_sdf(G((long)i - 1, (long)j), G((long)i, (long)j));
+            double weight = 1.0 - frac;
+            if (weight < 0.0) weight = 0.0;
+            if (weight > 1.0) weight = 1.0;
+            if (weight < 0.01 && weight > 0.0) weight = 0.01; /* kMinWeight */
+            uw[i + (nx + 1) * j] = (float)weight;
+        }
+#pragma omp parallel for schedule(static)
+    for (size_t j = 0; j <= ny; ++j)
+        for (size_t i = 0; i < nx; ++i) {
+            double frac = or_fraction_inside_sdf(G((long)i, (long)j - 1), G((long)i, (long)j));
+            double weight = 1.0 - frac;
+            if (weight < 0.0) weight = 0.0;
+            if (weight > 1.0) weight = 1.0;
+            if (weight < 0.01 && weight > 0.0) weight = 0.01;
+            vw[i + nx * j] = (float)weight;
+        }
+#undef G
+}
+
+/* Float restriction, 2-D -- the anonymous restrict() in
+ * src/jet/grid_fractional_single_phase_pressure_solver2.cpp:29-93: per axis
+ * the centred kernel over fine 2c-1 .. 2c+2 (clamped), or injection
+ * {0, 1, 0} over 2c-1 .. 2c+1 where the fine size is not twice the coarse
+ * size; y outer, x inner, w = kx * ky. */
+void or_fr_restrict_float2(const float* finer, size_t fnx, size_t fny,
+                           float* coarser, size_t cnx, size_t cny) {
+    static const float centered[4] = {0.125f, 0.375f, 0.375f, 0.125f};
+    static const float staggered[4] = {0.f, 1.f, 0.f, 0.f};
+    const int kx = fnx != 2 * cnx ? 3 : 4, ky = fny != 2 * cny ? 3 : 4;
+    const float* kernx = kx == 3 ? staggered : centered;
+    const float* kerny = ky == 3 ? staggered : centered;
+#pragma omp parallel for schedule(static)
+    for (size_t j = 0; j < cny; ++j) {
+        size_t jj[4] = {0, 0, 0, 0};
+        if (ky == 3) {
+            jj[0] = (j > 0) ? 2 * j - 1 : 2 * j;
+            jj[1] = 2 * j;
+            jj[2] = (j + 1 < cny) ? 2 * j + 1 : 2 * j;
+        } else {
+            jj[0] = (j > 0) ? 2 * j - 1 : 2 * j;
+            jj[1] = 2 * j;
+            jj[2] = 2 * j + 1;
+            jj[3] = (j + 1 < cny) ? 2 * j + 2 : 2 * j + 1;
+        }
+        for (size_t i = 0; i < cnx; ++i) {
+            size_t ii[4] = {0, 0, 0, 0};
+            if (kx == 3) {
+                ii[0] = (i > 0) ? 2 * i - 1 : 2 * i;
+                ii[1] = 2 * i;
+                ii[2] = (i + 1 < cnx) ? 2 * i + 1 : 2 * i;
+            } else {
+                ii[0] = (i > 0) ? 2 * i - 1 : 2 * i;
+                ii[1] = 2 * i;
+                ii[2] = 2 * i + 1;
+                ii[3] = (i + 1 < cnx) ? 2 * i + 2 : 2 * i + 1;
+            }
+            float sum = 0.0f;
+            for (int y = 0; y < ky; ++y)
+                for (int x = 0; x < kx; ++x) {
+                    float w = kernx[x] * kerny[y];
+                    sum += w * finer[ii[x] + fnx * jj[y]];
+                }
+            coarser[i + cnx * j] = sum;
+        }
+    }
+}
+
+/* buildSingleSystem, 2-D -- src/jet/grid_fractional_single_phase_pressure_solver2.cpp:95-203.
+ * Rows are written as 4 doubles {center, right, up, 0} (a one-plane level).
+ * bu / bv: collider velocity x at the u faces and y at the v faces (NULL:
+ * zero); the collider term sum (1 - w) u_collider / h is Jet's
+ * boundaryContribution. */
+void or_fr_build_system2(const float* fsdf, const float* uw, const float* vw,
+                         const double* u, const double* v, const double* bu,
+                         const double* bv, size_t nx, size_t ny, const double h[2],
+                         or_row* A, double* b) {
+    const double invH[2] = {1.0 / h[0], 1.0 / h[1]};
+    const double invHSqr[2] = {invH[0] * invH[0], invH[1] * invH[1]};
+#pragma omp parallel for schedule(static)
+    for (size_t j = 0; j < ny; ++j)
+        for (size_t i = 0; i < nx; ++i) {
+            const size_t c = i + nx * j;
+            or_row row = {0.0, 0.0, 0.0, 0.0};
+            double bij = 0.0;
+            const double centerPhi = fsdf[c];
+            const size_t uR = (i + 1) + (nx + 1) * j, uL = uR - 1;
+            const size_t vU = i + nx * (j + 1), vD = i + nx * j;
+            if (centerPhi < 0) {
+                double term;
+                if (i + 1 < nx) {
+                    term = uw[uR] * invHSqr[0];
+                    const double rightPhi = fsdf[c + 1];
+                    if (rightPhi < 0) {
+                        row.center += term;
+                        row.right -= term;
+                    } else {
+                        double theta = or_fraction_inside_sdf(centerPhi, rightPhi);
+                        theta = theta < 0.01 ? 0.01 : theta; /* std::max */
+                        row.center += term / theta;
+                    }
+                    bij += uw[uR] * u[uR] * invH[0];
+                } else {
+                    bij += u[uR] * invH[0];
+                }
+                if (i > 0) {
+                    term = uw[uL] * invHSqr[0];
+                    const double leftPhi = fsdf[c - 1];
+                    if (leftPhi < 0) {
+                        row.center += term;
+                    } else {
+                        double theta = or_fraction_inside_sdf(centerPhi, leftPhi);
+                        theta = theta < 0.01 ? 0.01 : theta;
+                        row.center += term / theta;
+                    }
+                    bij -= uw[uL] * u[uL] * invH[0];
+                } else {
+                    bij -= u[uL] * invH[0];
+                }
+                if (j + 1 < ny) {
+                    term = vw[vU] * invHSqr[1];
+                    const double upPhi = fsdf[c + nx];
+                    if (upPhi < 0) {
+                        row.center += term;
+                        row.up -= term;
+                    } else {
+                        double theta = or_fraction_inside_sdf(centerPhi, upPhi);
+                        theta = theta < 0.01 ? 0.01 : theta;
+                        row.center += term / theta;
+                    }
+                    bij += vw[vU] * v[vU] * invH[1];
+                } else {
+                    bij += v[vU] * invH[1];
+                }
+                if (j > 0) {
+                    term = vw[vD] * invHSqr[1];
+                    const double downPhi = fsdf[c - nx];
+                    if (downPhi < 0) {
+                        row.center += term;
+                    } else {
+                        double theta = or_fraction_inside_sdf(centerPhi, downPhi);
+                        theta = theta < 0.01 ? 0.01 : theta;
+                        row.center += term / theta;
+                    }
+                    bij -= vw[vD] * v[vD] * invH[1];
+                } else {
+                    bij -= v[vD] * invH[1];
+                }
+                const double buR = bu ? bu[uR] : 0.0, buL = bu ? bu[uL] : 0.0;
+                const double bvU = bv ? bv[vU] : 0.0, bvD = bv ? bv[vD] : 0.0;
+                const double contribution = (1.0 - uw[uR]) * buR * invH[0] -
+                                            (1.0 - uw[uL]) * buL * invH[0] +
+                                            (1.0 - vw[vU]) * bvU * invH[1] -
+                                            (1.0 - vw[vD]) * bvD * invH[1];
+                bij += contribution;
+                if (row.center < 2.220446049250313080847e-16) { /* kEpsilonD */
+                    row.center = 1.0;
+                    bij = 0.0;
+                }
+            } else {
+                row.center = 1.0;
+            }
+            A[c] = row;
+            if (b) b[c] = bij;
+        }
+}
+
+/* applyPressureGradient, 2-D -- src/jet/grid_fractional_single_phase_pressure_solver2.cpp:565-600. */
+void or_fr_apply_gradient2(const float* fsdf, const float* uw, const float* vw,
+                           const double* p, const double* u, const double* v,
+                           double* uo, double* vo, size_t nx, size_t ny,
+                           const double h[2]) {
+    const double invH[2] = {1.0 / h[0], 1.0 / h[1]};
+#pragma omp parallel for schedule(static)
+    for (size_t j = 0; j < ny; ++j)
+        for (size_t i = 0; i < nx; ++i) {
+            const size_t c = i + nx * j;
+            const double centerPhi = fsdf[c];
+            if (i + 1 < nx) {
+                const size_t f = (i + 1) + (nx + 1) * j;
+                if (uw[f] > 0.0 && (centerPhi < 0 || fsdf[c + 1] < 0)) {
+                    const double rightPhi = fsdf[c + 1];
+                    double theta = or_fraction_inside_sdf(centerPhi, rightPhi);
+                    theta = theta < 0.01 ? 0.01 : theta;
+                    uo[f] = u[f] + invH[0] / theta * (p[c + 1] - p[c]);
+                }
+            }
+            if (j + 1 < ny) {
+                const size_t f = i + nx * (j + 1);
+                if (vw[f] > 0.0 && (centerPhi < 0 || fsdf[c + nx] < 0)) {
+                    const double upPhi = fsdf[c + nx];
+                    double theta = or_fraction_inside_sdf(centerPhi, upPhi);
+                    theta = theta < 0.01 ? 0.01 : theta;
+                    vo[f] = v[f] + invH[1] / theta * (p[c + nx] - p[c]);
+                }
+            }
+        }
+}
+
 /* ============================================ Boundary conditions ======= */
 
 /* RigidBodyCollider3::velocityAt -- src/jet/rigid_body_collider3.cpp:25-28:

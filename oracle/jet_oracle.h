@@ -189,6 +189,22 @@ void or_fr_apply_gradient(const float* fluid_sdf, const float* uw,
                           double* uo, double* vo, double* wo, size_t nx,
                           size_t ny, size_t nz, const double h[3]);
 
+/* 2-D (GridFractionalSinglePhasePressureSolver2): bsdf at the cell centres
+ * of the grid extended by one ghost cell per side ((nx+2)*(ny+2)); uw is
+ * (nx+1)*ny, vw nx*(ny+1); rows are {center, right, up, 0}. */
+void or_fr_face_weights2(const double* bsdf, size_t nx, size_t ny, float* uw,
+                         float* vw);
+void or_fr_restrict_float2(const float* finer, size_t fnx, size_t fny,
+                           float* coarser, size_t cnx, size_t cny);
+void or_fr_build_system2(const float* fsdf, const float* uw, const float* vw,
+                         const double* u, const double* v, const double* bu,
+                         const double* bv, size_t nx, size_t ny, const double h[2],
+                         or_row* A, double* b);
+void or_fr_apply_gradient2(const float* fsdf, const float* uw, const float* vw,
+                           const double* p, const double* u, const double* v,
+                           double* uo, double* vo, size_t nx, size_t ny,
+                           const double h[2]);
+
 /* ---------------------------------------- boundary conditions (velocity) */
 /* Rigid-body collider: RigidBodyCollider3 (include/jet/rigid_body_collider3.h)
  * with its surface's transform translation and Collider3's friction. */

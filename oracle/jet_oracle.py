@@ -615,6 +615,73 @@ class Oracle(_BoundaryCalls, _CsrCalls):
             sys.x.append(np.zeros_like(b))
         return (fs, UW, VW, WW), sys
 
+    # ---------------------------------------------- fractional build, 2-D
+    def fr_face_weights2(self, bsdf_ghost):
+        """Face weights of GridFractionalSinglePhasePressureSolver2::buildWeights
+        from the boundary SDF at the cell centres of the grid plus one ghost
+        cell per side, an (ny + 2, nx + 2) array."""
+        ny, nx = (s - 2 for s in bsdf_ghost.shape)
+        uw = np.empty((ny, nx + 1), np.float32)
+        vw = np.empty((ny + 1, nx), np.float32)
+        self.lib.or_fr_face_weights2(_d(bsdf_ghost), _sz(nx), _sz(ny), _f(uw), _f(vw))
+        return uw, vw
+
+    def fr_restrict_float2(self, fine, coarse_shape):
+        out = np.empty(coarse_shape, np.float32)
+        fy, fx = fine.shape
+        cy, cx = coarse_shape
+        self.lib.or_fr_restrict_float2(_f(fine), _sz(fx), _sz(fy), _f(out), _sz(cx), _sz(cy))
+        return out
+
+    def fr_build_system2(self, fsdf, uw, vw, u, v, h, bu=None, bv=None):
+        """(A as (1, ny, nx, 4) rows with front = 0, b as (1, ny, nx))."""
+        ny, nx = fsdf.shape
+        A = np.empty((1, ny, nx, 4))
+        b = np.empty((1, ny, nx))
+        nul = C.cast(None, _dp)
+        self.lib.or_fr_build_system2(_f(fsdf), _f(uw), _f(vw), _d(u), _d(v),
+                                     _d(bu) if bu is not None else nul,
+                                     _d(bv) if bv is not None else nul,
+                                     _sz(nx), _sz(ny), (C.c_double * 2)(*h), _d(A), _d(b))
+        return A, b
+
+    def fr_apply_gradient2(self, fsdf, uw, vw, p, u, v, h):
+        uo, vo = u.copy(), v.copy()
+        ny, nx = fsdf.shape
+        self.lib.or_fr_apply_gradient2(_f(fsdf), _f(uw), _f(vw), _d(p), _d(u), _d(v), _d(uo),
+                                       _d(vo), _sz(nx), _sz(ny), (C.c_double * 2)(*h))
+        return uo, vo
+
+    def fr_build_mg2(self, u, v, h, bsdf_ghost, fluid_sdf_cc, max_levels, bu=None, bv=None):
+        """GridFractionalSinglePhasePressureSolver2::buildWeights + buildSystem
+        (src/jet/grid_fractional_single_phase_pressure_solver2.cpp:414-563) on
+        (ny, nx) cell-centred samples: ((fluid SDF, u weights, v weights) per
+        level, MgSystem of one-plane levels).  Coarse levels carry A only:
+        their right-hand sides are what the V-cycle restricts."""
+        ny, nx = fluid_sdf_cc.shape
+        sizes = self.levels_with_finest2((nx, ny), max_levels)
+        fs = [np.ascontiguousarray(fluid_sdf_cc, dtype=np.float32)]
+        uw, vw = self.fr_face_weights2(np.ascontiguousarray(bsdf_ghost, dtype=np.float64))
+        UW, VW = [uw], [vw]
+        for (cx, cy) in sizes[1:]:
+            fs.append(self.fr_restrict_float2(fs[-1], (cy, cx)))
+            UW.append(self.fr_restrict_float2(UW[-1], (cy, cx + 1)))
+            VW.append(self.fr_restrict_float2(VW[-1], (cy + 1, cx)))
+        sys = MgSystem()
+        for l, (cx, cy) in enumerate(sizes):
+            hl = (h[0] * 2 ** l, h[1] * 2 ** l)
+            if l == 0:
+                A, b = self.fr_build_system2(fs[0], UW[0], VW[0], np.ascontiguousarray(u),
+                                             np.ascontiguousarray(v), hl, bu, bv)
+            else:
+                A, _ = self.fr_build_system2(fs[l], UW[l], VW[l], np.zeros((cy, cx + 1)),
+                                             np.zeros((cy + 1, cx)), hl)
+                b = np.zeros((1, cy, cx))
+            sys.A.append(A)
+            sys.b.append(b)
+            sys.x.append(np.zeros_like(b))
+        return (fs, UW, VW), sys
+
 
 class Ref(_BoundaryCalls, _CsrCalls):
     """Jet's own compiled linear solvers (oracle/_ref/libjetref.so), and the
@@ -752,6 +819,31 @@ class Ref(_BoundaryCalls, _CsrCalls):
 
     def fraction_inside_sdf(self, a, b):
         return self.lib.ref_fraction_inside_sdf(a, b)
+
+    def fr_solve2(self, u, v, h, origin, bsdf_ghost, fluid_sdf_cc, max_levels, p, bu=None, bv=None):
+        """Jet's compiled GridFractionalSinglePhasePressureSolver2::solve with a
+        recording multigrid solver: (A of every level as (1, ny_l, nx_l, 4) rows,
+        b of the finest level, and the projected (uo, vo) of the given pressure p)."""
+        ny, nx = fluid_sdf_cc.shape
+        sizes = Oracle().levels_with_finest2((nx, ny), max_levels)
+        A = np.empty(sum(cx * cy for cx, cy in sizes) * 4)
+        b = np.empty((1, ny, nx))
+        uo = np.empty((ny, nx + 1))
+        vo = np.empty((ny + 1, nx))
+        nul = C.cast(None, _dp)
+        self.lib.ref_fr_solve2(_sz(nx), _sz(ny), (C.c_double * 2)(*h), (C.c_double * 2)(*origin),
+                               _d(np.ascontiguousarray(u, np.float64)),
+                               _d(np.ascontiguousarray(v, np.float64)),
+                               _d(np.ascontiguousarray(bsdf_ghost, np.float64)),
+                               _d(np.ascontiguousarray(fluid_sdf_cc, np.float64)),
+                               _d(bu) if bu is not None else nul, _d(bv) if bv is not None else nul,
+                               _sz(max_levels), _d(np.ascontiguousarray(p, np.float64)), _d(A),
+                               _d(b), _d(uo), _d(vo))
+        levels, off = [], 0
+        for cx, cy in sizes:
+            levels.append(A[off:off + cx * cy * 4].reshape(1, cy, cx, 4))
+            off += cx * cy * 4
+        return levels, b, uo, vo
 
     def diffusion_system(self, f, bsdf, fsdf, dt, mu, h, dirichlet=False):
         """The system Jet's GridBackwardEulerDiffusionSolver3 builds (A, b)."""

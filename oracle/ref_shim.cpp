@@ -23,6 +23,10 @@
 #include <jet/fdm_mg_solver3.h>
 #include <jet/fdm_mgpcg_solver3.h>
 #include <jet/fdm_mgpcg_solver2.h>
+#include <jet/custom_scalar_field2.h>
+#include <jet/custom_vector_field2.h>
+#include <jet/face_centered_grid2.h>
+#include <jet/grid_fractional_single_phase_pressure_solver2.h>
 #include <jet/level_set_utils.h>
 #include <jet/logging.h>
 
@@ -650,6 +654,93 @@ void ref_constrain_velocity(int blocked, size_t nx, size_t ny, size_t nz, const 
         });
     }
     readFaces(velocity, uo, vo, wo);
+}
+
+}  // extern "C"
+
+// GridFractionalSinglePhasePressureSolver2::solve on an nx x ny grid of
+// spacing h and origin o: the variational system Jet's buildWeights +
+// buildSystem hand its multigrid solver (A of every level as {center, right,
+// up, 0} rows, levels back to back, and b of the finest level), captured by a
+// recording FdmMgSolver2 that answers with the given pressure p; then Jet's
+// applyPressureGradient with that p (uo, vo).  The boundary SDF is looked up
+// at the cell centres of the grid plus one ghost cell per side (bsdf,
+// (nx + 2) x (ny + 2), which is every point buildWeights samples), the fluid
+// SDF at the cell centres, the collider velocity x at the u faces (bu) and y
+// at the v faces (bv; NULL: at rest).
+namespace {
+class CaptureMg2 final : public FdmMgSolver2 {
+ public:
+    explicit CaptureMg2(size_t levels) : FdmMgSolver2(levels) {}
+    Row* A = nullptr;
+    double* b = nullptr;
+    const double* p = nullptr;
+    bool solve(FdmMgLinearSystem2* s) override {
+        Row* out = A;
+        for (size_t l = 0; l < s->A.levels.size(); ++l) {
+            const FdmMatrix2& M = s->A.levels[l];
+            const Size2 n = M.size();
+            for (size_t j = 0; j < n.y; ++j)
+                for (size_t i = 0; i < n.x; ++i)
+                    *out++ = {M(i, j).center, M(i, j).right, M(i, j).up, 0.0};
+        }
+        const Size2 n = s->b.levels[0].size();
+        for (size_t j = 0; j < n.y; ++j)
+            for (size_t i = 0; i < n.x; ++i) {
+                b[i + n.x * j] = s->b.levels[0](i, j);
+                s->x.levels[0](i, j) = p[i + n.x * j];
+            }
+        return true;
+    }
+};
+}  // namespace
+
+extern "C" {
+
+void ref_fr_solve2(size_t nx, size_t ny, const double* h, const double* o, const double* u,
+                   const double* v, const double* bsdf, const double* fsdf, const double* bu,
+                   const double* bv, size_t max_levels, const double* p, double* A, double* b,
+                   double* uo, double* vo) {
+    FaceCenteredGrid2 vel(Size2(nx, ny), Vector2D(h[0], h[1]), Vector2D(o[0], o[1]));
+    for (size_t j = 0; j < ny; ++j)
+        for (size_t i = 0; i <= nx; ++i) vel.u(i, j) = u[i + (nx + 1) * j];
+    for (size_t j = 0; j <= ny; ++j)
+        for (size_t i = 0; i < nx; ++i) vel.v(i, j) = v[i + nx * j];
+    // cell index of a cell-centre position (ghost cells at -1 and n)
+    auto cell = [=](double x, double org, double hh) {
+        return (long)std::llround((x - org) / hh - 0.5);
+    };
+    CustomScalarField2 boundary([=](const Vector2D& q) {
+        const long i = cell(q.x, o[0], h[0]), j = cell(q.y, o[1], h[1]);
+        return bsdf[(size_t)(i + 1) + (nx + 2) * (size_t)(j + 1)];
+    });
+    CustomScalarField2 fluid([=](const Vector2D& q) {
+        const long i = cell(q.x, o[0], h[0]), j = cell(q.y, o[1], h[1]);
+        return fsdf[(size_t)i + nx * (size_t)j];
+    });
+    // a u face sits at (i h, (j + 1/2) h), a v face at ((i + 1/2) h, j h)
+    CustomVectorField2 collider([=](const Vector2D& q) {
+        const double fx = (q.x - o[0]) / h[0], fy = (q.y - o[1]) / h[1];
+        const long ix = std::llround(fx), iy = std::llround(fy);
+        if (std::fabs(fx - (double)ix) < 0.25) {  // a u face
+            const long j = std::llround(fy - 0.5);
+            return Vector2D(bu ? bu[(size_t)ix + (nx + 1) * (size_t)j] : 0.0, 0.0);
+        }
+        const long i = std::llround(fx - 0.5);
+        return Vector2D(0.0, bv ? bv[(size_t)i + nx * (size_t)iy] : 0.0);
+    });
+    auto cap = std::make_shared<CaptureMg2>(max_levels);
+    cap->A = reinterpret_cast<Row*>(A);
+    cap->b = b;
+    cap->p = p;
+    GridFractionalSinglePhasePressureSolver2 solver;
+    solver.setLinearSystemSolver(cap);
+    FaceCenteredGrid2 out(vel);
+    solver.solve(vel, 1.0, &out, boundary, collider, fluid, false);
+    for (size_t j = 0; j < ny; ++j)
+        for (size_t i = 0; i <= nx; ++i) uo[i + (nx + 1) * j] = out.u(i, j);
+    for (size_t j = 0; j <= ny; ++j)
+        for (size_t i = 0; i < nx; ++i) vo[i + nx * j] = out.v(i, j);
 }
 
 }  // extern "C"
