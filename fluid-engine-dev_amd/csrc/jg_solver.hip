@@ -183,8 +183,13 @@ int launch_sweep2(jg_solver* s, const Row<T>* A, const T* b, const T* xin, T* xo
     } else {
         if (res && coarse) return JG_ERR_UNSUPPORTED;
         const int zc = std::min(tuned_zc > 0 ? tuned_zc : 32, d.nz);
-        const int ix = res ? S2Tile<true>::IX : S2Tile<false>::IX;
-        const int iy = res ? S2Tile<true>::IY : S2Tile<false>::IY;
+        // the tile of a zero-start launch without a correction has a one cell
+        // narrower halo (jg_sweep2.h)
+        const bool z = !xin && !coarse;
+        const int ix = res ? (z ? S2Tile<true, true>::IX : S2Tile<true>::IX)
+                           : (z ? S2Tile<false, true>::IX : S2Tile<false>::IX);
+        const int iy = res ? (z ? S2Tile<true, true>::IY : S2Tile<true>::IY)
+                           : (z ? S2Tile<false, true>::IY : S2Tile<false>::IY);
         dim3 grid((d.nx + ix - 1) / ix, (d.ny + iy - 1) / iy, (d.nz + zc - 1) / zc);
         const dim3 block(kS2LX, kS2TY, 1);
         const int code = (first ? 8 : 0) | (xin ? 0 : 4) | (coarse ? 2 : 0) | (res ? 1 : 0);

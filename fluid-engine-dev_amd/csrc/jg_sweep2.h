@@ -46,10 +46,12 @@ constexpr int kS2LX = 64, kS2LY = 32;
 constexpr int kS2TY = 16;  // thread rows (two tile rows each)
 constexpr int kS2Threads = kS2LX * kS2TY;
 // Halo (in x, y and z) and interior of the tile: 4 cells for the two sweeps,
-// 5 when the residual is written too.
-template <bool RES>
+// 5 when the residual is written too; one less when x starts at zero with no
+// correction (Z), where the first half-sweep, w b, reads no neighbour and is
+// valid up to the tile's edge.
+template <bool RES, bool Z = false>
 struct S2Tile {
-    static constexpr int H = RES ? 5 : 4;
+    static constexpr int H = (RES ? 5 : 4) - (Z ? 1 : 0);
     static constexpr int IX = kS2LX - 2 * H, IY = kS2LY - 2 * H;
     // r / u coupling ring: planes t-1 .. t-4 for the half-sweeps; RES reads
     // plane t-4 after the last barrier of step t, before the next commit
@@ -73,7 +75,10 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
     // CORRECT's prolongation weights take the parity of x and y from the
     // tile-local index, which needs the even origins of the 4-cell halo
     static_assert(!(RES && CORRECT), "the residual variant has no correction");
-    using G = S2Tile<RES>;
+    // ZH: x starts at zero and nothing is added to it, so half-sweep H is
+    // valid on ring >= H - 1 (the first reads no neighbour)
+    constexpr int ZH = ZERO && !CORRECT ? 1 : 0;
+    using G = S2Tile<RES, ZH != 0>;
     constexpr int kH = G::H, kCR = G::CR;
     if (ctl && ctl->done) return;
     __shared__ __align__(16) T sx[kS2XR][kS2LY][kS2LX];  // x, slot = plane mod 8
@@ -226,7 +231,8 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
         const int sel = ((gx + gy0 + p) & 1) != colour;  // 0: row 2ty, 1: row 2ty + 1
         const int ly = 2 * ty + sel;
         const int gy = gy0 + ly;
-        const bool in = tx >= H && tx < kS2LX - H && ly >= H && ly < kS2LY - H && xok && gy >= 0 &&
+        constexpr int R = H - ZH;  // innermost ring this half-sweep is valid on
+        const bool in = tx >= R && tx < kS2LX - R && ly >= R && ly < kS2LY - R && xok && gy >= 0 &&
                         gy < d.ny;
         if (in) {
             const T w = sel ? P[1].w[H] : P[0].w[H];
@@ -305,13 +311,18 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
     using H4 = std::integral_constant<int, 4>;
 
     // Planes k0-kH .. k1+kH-1 are committed; half-sweep H covers planes
-    // k0-kH+H .. k1+kH-1-H (those within the grid): exactly the planes the
-    // next half-sweep reads for the chunk's planes k0 .. k1-1 (and, for RES,
-    // h4 covers k0-1 .. k1, the planes the residual reads).
+    // k0-kH+H-ZH .. k1+kH-1-H+ZH (those within the grid): exactly the planes
+    // the next half-sweep reads for the chunk's planes k0 .. k1-1 (and, for
+    // RES, h4 covers k0-1 .. k1, the planes the residual reads).  Half-sweep
+    // H runs on plane t - H at step t, so the last live one is at step
+    // t1 + ZH: with ZH, one step past the last commit.
     // Two planes of loads are in flight ahead of the committed one; the loop
     // is unrolled twice so the prefetch slot of every step is static.
-    const int t0 = k0 - kH, t1 = k1 + kH - 1;  // t1 - t0 >= 8
-    auto live = [&](int p, int H) { return p >= max(k0 - kH + H, 0) && p < min(k1 + kH - H, d.nz); };
+    const int t0 = k0 - kH, t1 = k1 + kH - 1;  // t1 - t0 >= 6
+    const int tl = t1 + ZH;                    // last step with a live half-sweep
+    auto live = [&](int p, int H) {
+        return p >= max(k0 - kH + H - ZH, 0) && p < min(k1 + kH - H + ZH, d.nz);
+    };
     if (CORRECT) zfetch(t0);
     // plane by plane, in the order the loop issues them
 #pragma unroll
@@ -323,9 +334,10 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
     if (CORRECT) zstore(t0);
     __syncthreads();
 
-    // An even number of steps, so the unrolled body has no exit: a last
-    // step past t1 commits a re-read plane and runs no half-sweep.
-    const int tend = t0 + ((t1 - t0 + 2) & ~1);
+    // An even number of steps, so the unrolled body has no exit: a step past
+    // t1 commits a re-read plane into the slot of a plane that is no longer
+    // read, and runs only the half-sweeps still live (none past tl).
+    const int tend = t0 + ((tl - t0 + 2) & ~1);
     bool pending = false;  // uniform: the pending x stores hold plane t - 5
     for (int t2 = t0; t2 < tend; t2 += 2) {
 #pragma unroll
