@@ -6,8 +6,10 @@
 // of (i, j) positions -- a 56 x 24 interior plus a 4-cell halo that is
 // recomputed, not exchanged -- and marches through a chunk of z planes.  At
 // step t it commits plane t from the registers it was prefetched into
-// (prefetching plane t + 2), then runs h1 on plane t-1, h2 on t-2, h3 on t-3
-// and h4 on t-4, each after a barrier; plane t-4 is then final and written
+// (prefetching plane t + 2) and runs h1 on plane t-1 in the same phase (its
+// +z neighbour is the thread's own position, just committed by itself;
+// everything else it reads is a step old), then h2 on t-2, h3 on t-3 and h4 on t-4, each after a
+// barrier: four barriers per step.  Plane t-4 is then final and written
 // back.  Half-sweep h is valid on positions at least h cells inside the tile
 // (it reads neighbours updated by h-1, valid one cell further out), so the
 // interior (ring >= 4) of plane t-4 holds exactly the values two full
@@ -15,8 +17,9 @@
 // chunk.
 //
 // x lives in an LDS ring (planes t-5 .. t in 8 slots), the +x / +y couplings in
-// 4-plane rings (planes t-1 .. t-4; they are the -x / -y couplings of the
-// neighbours); w = omega / c, A.f and b stay in registers.  A thread owns two
+// 5-plane rings (planes t .. t-4; they are the -x / -y couplings of the
+// neighbours; plane t's are written after the step's first barrier); w =
+// omega / c, A.f and b stay in registers.  A thread owns two
 // vertically adjacent positions, which have opposite colours, so every
 // half-sweep updates one position per thread.  Each update is the FP32 form
 // of the single sweep, (1 - omega) x + w (b - s) with Jet's relaxRedBlack
@@ -53,10 +56,11 @@ template <bool RES, bool Z = false>
 struct S2Tile {
     static constexpr int H = (RES ? 5 : 4) - (Z ? 1 : 0);
     static constexpr int IX = kS2LX - 2 * H, IY = kS2LY - 2 * H;
-    // r / u coupling ring: planes t-1 .. t-4 for the half-sweeps; RES reads
-    // plane t-4 after the last barrier of step t, before the next commit
-    // (which writes plane t) has passed a barrier, so it needs a fifth slot
-    static constexpr int CR = RES ? 5 : 4;
+    // r / u coupling ring: planes t .. t-4.  The couplings of plane t are
+    // written after the first barrier of step t (h1 on plane t-1 runs before
+    // it and reads those of t-1), and RES reads those of plane t-4 after the
+    // last barrier of step t, so five slots never collide.
+    static constexpr int CR = 5;
     static_assert(IX % 2 == 0 && IY % 2 == 0, "tile interiors stay even");
 };
 // x ring: planes t-7 .. t are live (8 slots make the slot a mask)
@@ -214,10 +218,16 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
             xv += q.ok ? corr : T(0);
         }
         sx[xs(t)][Q.ly][tx] = xv;
-        sr[cs(t - 1)][Q.ly][tx] = Q.hr;
-        su[cs(t - 1)][Q.ly][tx] = Q.hu;
         Q.hr = q.ok ? q.row.r : T(0);
         Q.hu = q.ok ? q.row.u : T(0);
+    };
+    // The couplings of plane t, after the first barrier of step t.
+    auto put_couplings = [&](int t) __attribute__((always_inline)) {
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            sr[cs(t)][P[r].ly][tx] = P[r].hr;
+            su[cs(t)][P[r].ly][tx] = P[r].hu;
+        }
     };
     // Half-sweep H (1..4) on plane p = t - H: colour FIRST for odd H, the
     // other for even H, on positions at least H cells inside the tile.  The
@@ -253,6 +263,9 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
                 s += su[s1][ly - 1][tx] * sx[sp][ly - 1][tx];
                 s += su[s1][ly][tx] * sx[sp][ly + 1][tx];
                 s += fm * sx[sm][ly][tx];
+                // (h1 runs right after the commit of plane p + 1 with no
+                // barrier between: the +z neighbour is this thread's own
+                // position, which it has just written itself)
                 s += fp * sx[sn][ly][tx];
                 xv = (T(1) - omega) * xo + w * (bv - s);
             }
@@ -351,9 +364,12 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
 #pragma unroll
             for (int r = 0; r < 2; ++r) fetch(P[r].p[u % kPD], P[r].gy, min(t + kPD, t1));
             if (pending) flush(t - 5);
-            __syncthreads();
+            // h1 on plane t - 1 needs no barrier after the commit: its in-plane
+            // neighbours and couplings (plane t - 1) are a step old, and its
+            // +z neighbour is this thread's own committed value
             if (live(t - 1, 1)) half(H1{}, t - 1);
             __syncthreads();
+            put_couplings(t);
             if (live(t - 2, 2)) half(H2{}, t - 2);
             __syncthreads();
             if (live(t - 3, 3)) half(H3{}, t - 3);
