@@ -11,9 +11,9 @@
 // half-sweeps of two sweeps (colours c1, c2, c1, c2) with one barrier each:
 // half-sweep h is valid on cells at least h inside the region, so the brick
 // (ring >= 4) ends with exactly the values of two single sweeps, which it
-// writes.  Every update is the single fused sweep's (jg_sweep.h): Jet's
-// relaxRedBlack expression for FP64, (1 - omega) x + w (b - s) with
-// w = omega / c for FP32; zero-centre rows are set to zero; the correction
+// writes.  Every update is the single fused sweep's (jg_sweep.h: rb_sum6,
+// rb_coef, rb_update; Jet's relaxRedBlack expression for FP64); zero-centre
+// rows are set to zero; the correction
 // variant adds the prolongation of the coarse correction evaluated as the
 // single sweep does (z first, then x, then y), so the result is
 // bit-identical to two single sweeps.  Reads xin, writes xout (xin != xout).
@@ -113,7 +113,7 @@ __global__ __launch_bounds__(kBkThreads, sizeof(T) == 4 ? 2 : 1) void k_brick_sw
             xv += corr;
         }
         const T cc = ok[m] ? row.c : T(1);
-        cw[m] = kWeighted ? (cc == T(0) ? T(0) : omega / cc) : cc;
+        cw[m] = rb_coef<T>(omega, cc);  // FP32: w = omega / c; FP64: c
         bb[m] = ok[m] ? bv : T(0);
         sx[lk][lj][li] = ok[m] ? xv : T(0);
         sr[lk][lj][li] = ok[m] ? row.r : T(0);
@@ -139,14 +139,11 @@ __global__ __launch_bounds__(kBkThreads, sizeof(T) == 4 ? 2 : 1) void k_brick_sw
                 xv = kWeighted ? cw[m] * (bb[m] - T(0)) : omega * (bb[m] - T(0)) / cw[m];
             } else {
                 // Jet's relax operand order: -x, +x, -y, +y, -z, +z
-                T s = sr[lk][lj][li - 1] * sx[lk][lj][li - 1];
-                s += sr[lk][lj][li] * sx[lk][lj][li + 1];
-                s += su[lk][lj - 1][li] * sx[lk][lj - 1][li];
-                s += su[lk][lj][li] * sx[lk][lj + 1][li];
-                s += sf[lk - 1][lj][li] * sx[lk - 1][lj][li];
-                s += sf[lk][lj][li] * sx[lk + 1][lj][li];
-                xv = kWeighted ? (T(1) - omega) * xo + cw[m] * (bb[m] - s)
-                               : (T(1) - omega) * xo + omega * (bb[m] - s) / cw[m];
+                const T s = rb_sum6<T>(sr[lk][lj][li - 1], sx[lk][lj][li - 1], sr[lk][lj][li],
+                                       sx[lk][lj][li + 1], su[lk][lj - 1][li], sx[lk][lj - 1][li],
+                                       su[lk][lj][li], sx[lk][lj + 1][li], sf[lk - 1][lj][li],
+                                       sx[lk - 1][lj][li], sf[lk][lj][li], sx[lk + 1][lj][li]);
+                xv = rb_update<T>(omega, xo, cw[m], bb[m], s);
             }
             sx[lk][lj][li] = xv;
         }

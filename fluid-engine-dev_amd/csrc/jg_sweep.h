@@ -25,8 +25,8 @@
 // stays a symmetric, finite operator.
 //
 // FP64 colour updates are Jet's expression; FP32 ones (the mixed-precision
-// inner solve) precompute w = omega / c once per cell and compute
-// (1 - omega) x + w (b - s): the same operator, rounded differently.
+// inner solve) use rb_sum6 / rb_coef / rb_update below: the same operator,
+// rounded differently.
 //
 // Options: ZERO treats the incoming x as zero (no read of xin); CORRECT adds
 // the trilinear prolongation of a coarse correction to x as it is loaded
@@ -40,6 +40,56 @@
 #include "jg_device.h"
 
 namespace jg {
+
+// Smoother arithmetic shared by the sweep kernels (single sweep, two sweeps,
+// bricks, bottom), so that their results stay bit-identical to each other.
+// FP64 is Jet's relaxRedBlack update (src/jet/fdm_gauss_seidel_solver3.cpp:
+// 163-190): the neighbour sum -x, +x, -y, +y, -z, +z, then
+// (1 - omega) x + omega (b - s) / c, contraction off.  FP32 (the mixed-
+// precision preconditioner) is the same operator rounded for throughput: the
+// sum in the same order as a chain of fused multiply-adds, w = omega / c from
+// the hardware reciprocal (v_rcp_f32, within 1 ulp) once per cell, and
+// fma(w, b - s, (1 - omega) x).
+template <typename T>
+__device__ __forceinline__ T rb_sum6(T a0, T x0, T a1, T x1, T a2, T x2, T a3, T x3, T a4, T x4,
+                                     T a5, T x5) {
+#pragma clang fp contract(off)
+    if constexpr (sizeof(T) == 4) {
+        T s = a0 * x0;
+        s = __builtin_fmaf(a1, x1, s);
+        s = __builtin_fmaf(a2, x2, s);
+        s = __builtin_fmaf(a3, x3, s);
+        s = __builtin_fmaf(a4, x4, s);
+        return __builtin_fmaf(a5, x5, s);
+    } else {
+        T s = a0 * x0;
+        s += a1 * x1;
+        s += a2 * x2;
+        s += a3 * x3;
+        s += a4 * x4;
+        s += a5 * x5;
+        return s;
+    }
+}
+// The per-cell coefficient a sweep keeps: FP32 w = omega / c (0 for a
+// zero-centre row), FP64 c itself.
+template <typename T>
+__device__ __forceinline__ T rb_coef(T omega, T c) {
+    if constexpr (sizeof(T) == 4)
+        return c == T(0) ? T(0) : omega * __builtin_amdgcn_rcpf(c);
+    else
+        return c;
+}
+// The relaxed value from the old value xo, the coefficient of rb_coef, b and
+// the neighbour sum s (callers handle zero-centre rows).
+template <typename T>
+__device__ __forceinline__ T rb_update(T omega, T xo, T cw, T b, T s) {
+#pragma clang fp contract(off)
+    if constexpr (sizeof(T) == 4)
+        return __builtin_fmaf(cw, b - s, (T(1) - omega) * xo);
+    else
+        return (T(1) - omega) * xo + omega * (b - s) / cw;
+}
 
 constexpr int kSwLX = 64, kSwLY = 16, kSwH = 2;  // tile of positions, halo
 constexpr int kSwIX = kSwLX - 2 * kSwH;  // 60 interior columns
@@ -223,7 +273,7 @@ __global__ __launch_bounds__(kSwLX * kSwLY / RP, sweep_waves_per_eu(sizeof(T), R
         Q.c2 = Q.c1;
         Q.c1 = Q.c0;
         Q.c0 = q.ok ? q.row.c : T(1);
-        if (kWeighted) Q.c0 = Q.c0 == T(0) ? T(0) : omega / Q.c0;  // w = omega / c
+        Q.c0 = rb_coef<T>(omega, Q.c0);  // FP32: w = omega / c; FP64: c
         Q.f3 = Q.f2;
         Q.f2 = Q.f1;
         Q.f1 = Q.f0;
@@ -262,14 +312,11 @@ __global__ __launch_bounds__(kSwLX * kSwLY / RP, sweep_waves_per_eu(sizeof(T), R
             sx[sp][ly][tx] = kWeighted ? Q.c1 * (Q.b1 - T(0)) : omega * (Q.b1 - T(0)) / Q.c1;
         } else {
             const T xo = sx[sp][ly][tx];
-            T s = sr[s1][ly][tx - 1] * sx[sp][ly][tx - 1];
-            s += sr[s1][ly][tx] * sx[sp][ly][tx + 1];
-            s += su[s1][ly - 1][tx] * sx[sp][ly - 1][tx];
-            s += su[s1][ly][tx] * sx[sp][ly + 1][tx];
-            s += Q.f2 * sx[sm][ly][tx];
-            s += Q.f1 * sx[sn][ly][tx];
-            sx[sp][ly][tx] = kWeighted ? (T(1) - omega) * xo + Q.c1 * (Q.b1 - s)
-                                       : (T(1) - omega) * xo + omega * (Q.b1 - s) / Q.c1;
+            const T s = rb_sum6<T>(sr[s1][ly][tx - 1], sx[sp][ly][tx - 1], sr[s1][ly][tx],
+                                   sx[sp][ly][tx + 1], su[s1][ly - 1][tx], sx[sp][ly - 1][tx],
+                                   su[s1][ly][tx], sx[sp][ly + 1][tx], Q.f2, sx[sm][ly][tx], Q.f1,
+                                   sx[sn][ly][tx]);
+            sx[sp][ly][tx] = rb_update<T>(omega, xo, Q.c1, Q.b1, s);
         }
     };
     double acc = 0.0, acc2 = 0.0;
@@ -285,15 +332,11 @@ __global__ __launch_bounds__(kSwLX * kSwLY / RP, sweep_waves_per_eu(sizeof(T), R
         const int sq = q & 3, sm = (q - 1) & 3, sn = (q + 1) & 3, s2 = q & 1;
         T xv = sx[sq][ly][tx];
         if (((gx + Q.gy + q) & 1) != FIRST) {
-            T s = sr[s2][ly][tx - 1] * sx[sq][ly][tx - 1];
-            s += sr[s2][ly][tx] * sx[sq][ly][tx + 1];
-            s += su[s2][ly - 1][tx] * sx[sq][ly - 1][tx];
-            s += su[s2][ly][tx] * sx[sq][ly + 1][tx];
-            s += Q.f3 * sx[sm][ly][tx];
-            s += Q.f2 * sx[sn][ly][tx];
-            xv = Q.c2 == T(0)  ? T(0)
-                 : kWeighted ? (T(1) - omega) * xv + Q.c2 * (Q.b2 - s)
-                             : (T(1) - omega) * xv + omega * (Q.b2 - s) / Q.c2;
+            const T s = rb_sum6<T>(sr[s2][ly][tx - 1], sx[sq][ly][tx - 1], sr[s2][ly][tx],
+                                   sx[sq][ly][tx + 1], su[s2][ly - 1][tx], sx[sq][ly - 1][tx],
+                                   su[s2][ly][tx], sx[sq][ly + 1][tx], Q.f3, sx[sm][ly][tx], Q.f2,
+                                   sx[sn][ly][tx]);
+            xv = Q.c2 == T(0) ? T(0) : rb_update<T>(omega, xv, Q.c2, Q.b2, s);
             sx[sq][ly][tx] = xv;
         }
         Q.px = xv;

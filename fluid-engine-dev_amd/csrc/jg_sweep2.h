@@ -21,11 +21,10 @@
 // neighbours; plane t's are written after the step's first barrier); w =
 // omega / c, A.f and b stay in registers.  A thread owns two
 // vertically adjacent positions, which have opposite colours, so every
-// half-sweep updates one position per thread.  Each update is the FP32 form
-// of the single sweep, (1 - omega) x + w (b - s) with Jet's relaxRedBlack
-// operand order for s (src/jet/fdm_gauss_seidel_solver3.cpp:163-190) and
-// contraction off, so the result is bit-identical to two single FP32 fused
-// sweeps.  Zero-centre rows are set to zero, as in the single sweep.
+// half-sweep updates one position per thread.  Each update is the single
+// sweep's FP32 arithmetic (rb_sum6 / rb_coef / rb_update of jg_sweep.h: Jet's
+// relaxRedBlack operand order for s, src/jet/fdm_gauss_seidel_solver3.cpp:
+// 163-190), so the result is bit-identical to two single FP32 fused sweeps.  Zero-centre rows are set to zero, as in the single sweep.
 //
 // Options: ZERO (x starts at zero, xin not read), CORRECT (the prolongation
 // of a coarse correction is added to x as it is loaded), RES (the residual
@@ -198,7 +197,7 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
 #pragma unroll
         for (int s = 5; s > 0; --s) Q.f[s] = Q.f[s - 1];
         const T c = q.ok ? q.row.c : T(1);
-        Q.w[0] = c == T(0) ? T(0) : omega / c;
+        Q.w[0] = rb_coef<T>(omega, c);  // w = omega / c, 0 for a zero-centre row
         if (RES) {
 #pragma unroll
             for (int s = kC - 1; s > 0; --s) Q.cc[s] = Q.cc[s - 1];
@@ -231,10 +230,7 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
     };
     // Half-sweep H (1..4) on plane p = t - H: colour FIRST for odd H, the
     // other for even H, on positions at least H cells inside the tile.  The
-    // row of this thread's position of that colour is 2 ty + sel.  FP32
-    // update x = (1 - w) x + w (b - s) with w = omega / c precomputed (the
-    // operators of Jet's relaxRedBlack, rounded differently from its
-    // omega (b - s) / c; FP64 sweeps keep Jet's expression).
+    // row of this thread's position of that colour is 2 ty + sel.
     auto half = [&](auto hc, int p) __attribute__((always_inline)) {
         constexpr int H = decltype(hc)::value;
         const int colour = (H & 1) ? FIRST : 1 - FIRST;
@@ -258,16 +254,14 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
                 // x is zero everywhere before the first half-sweep
                 xv = w * (bv - T(0));
             } else {
-                T s = sr[s1][ly][tx - 1] * sx[sp][ly][tx - 1];
-                s += sr[s1][ly][tx] * sx[sp][ly][tx + 1];
-                s += su[s1][ly - 1][tx] * sx[sp][ly - 1][tx];
-                s += su[s1][ly][tx] * sx[sp][ly + 1][tx];
-                s += fm * sx[sm][ly][tx];
                 // (h1 runs right after the commit of plane p + 1 with no
                 // barrier between: the +z neighbour is this thread's own
                 // position, which it has just written itself)
-                s += fp * sx[sn][ly][tx];
-                xv = (T(1) - omega) * xo + w * (bv - s);
+                const T s = rb_sum6<T>(sr[s1][ly][tx - 1], sx[sp][ly][tx - 1], sr[s1][ly][tx],
+                                       sx[sp][ly][tx + 1], su[s1][ly - 1][tx], sx[sp][ly - 1][tx],
+                                       su[s1][ly][tx], sx[sp][ly + 1][tx], fm, sx[sm][ly][tx], fp,
+                                       sx[sn][ly][tx]);
+                xv = rb_update<T>(omega, xo, w, bv, s);
             }
             sx[sp][ly][tx] = xv;
         }
