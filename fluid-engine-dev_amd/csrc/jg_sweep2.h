@@ -142,11 +142,10 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
     // planes t .. t-5, the held r / u of plane t (written to LDS at the next
     // commit), the pending final value, and the prefetched planes.
     // RES also keeps A.c of planes t .. t-4 and the partial residual of
-    // plane t-4 (rv);
-    // to stay within the registers of 1024 threads it prefetches one plane
-    // ahead instead of two.
+    // plane t-4 (rv).  To stay within the registers of 1024 threads, RES and
+    // CORRECT prefetch one plane ahead instead of two.
     constexpr int kC = RES ? 5 : 1;
-    constexpr int kPD = RES ? 1 : 2;
+    constexpr int kPD = RES || CORRECT ? 1 : 2;
     struct Pos {
         T w[5], bb[5], f[6], cc[kC];
         T hr, hu, px, rv;
@@ -341,14 +340,15 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
     if (CORRECT) zstore(t0);
     __syncthreads();
 
-    // An even number of steps, so the unrolled body has no exit: a step past
-    // t1 commits a re-read plane into the slot of a plane that is no longer
-    // read, and runs only the half-sweeps still live (none past tl).
-    const int tend = t0 + ((tl - t0 + 2) & ~1);
+    // The body is unrolled kPD times (the prefetch slot of every step is
+    // static) and has no exit: a step past t1 commits a re-read plane into
+    // the slot of a plane that is no longer read, and runs only the
+    // half-sweeps still live (none past tl).
+    const int tend = t0 + (tl - t0 + kPD) / kPD * kPD;
     bool pending = false;  // uniform: the pending x stores hold plane t - 5
-    for (int t2 = t0; t2 < tend; t2 += 2) {
+    for (int t2 = t0; t2 < tend; t2 += kPD) {
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < kPD; ++u) {
             const int t = t2 + u;
 #pragma unroll
             for (int r = 0; r < 2; ++r) commit(P[r], P[r].p[u % kPD], t);
