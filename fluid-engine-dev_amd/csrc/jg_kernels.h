@@ -416,6 +416,86 @@ __global__ __launch_bounds__(NT) void k_restrict(const TX* __restrict__ fine,
     coarse[i + (idx_t)cd.nx * (j + (idx_t)cd.ny * k)] = sum;
 }
 
+// Coarse rows of the flat-entry MGPCG from the next finer level (no
+// counterpart in Jet, whose multigrid callers build each level from markers).
+// Coarse cell C aggregates the 2 x 2 x DZ fine cells it covers (DZ = 1: a
+// one-plane level).  With A = M + L (M the row sums, L the coupling part) the
+// coarse operator is kappa (P'AP + P'MP), kappa = 1/16 (1/8 on one plane), P
+// the piecewise-constant injection: a uniform stencil keeps its mass and
+// quarters its couplings, i.e. it is rediscretised at twice the spacing.
+//   right/up/front_C = kappa * (fine couplings leaving C in +x / +y / +z)
+//   center_C = kappa * ((2 sum centers + 4 sum inner couplings) + sum of
+//              couplings leaving C through its faces)
+// Couplings to cells outside the grid are not counted.  FP64, one thread per
+// coarse cell, every sum in a fixed order.
+template <int DZ>
+__global__ __launch_bounds__(NT) void k_mg_coarsen(const Row<double>* __restrict__ fine,
+                                                   Row<double>* __restrict__ coarse, Dims cd) {
+#pragma clang fp contract(off)
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y * blockDim.y + threadIdx.y;
+    const int K = blockIdx.z;
+    if (I >= cd.nx || J >= cd.ny) return;
+    const int fnx = 2 * cd.nx, fny = 2 * cd.ny;
+    const int i0 = 2 * I, j0 = 2 * J, k0 = DZ * K;
+    auto at = [&](int i, int j, int k) -> const Row<double>& {
+        return fine[i + (idx_t)fnx * (j + (idx_t)fny * k)];
+    };
+    double sc = 0.0, si = 0.0;
+    double px = 0.0, mx = 0.0, py = 0.0, my = 0.0, pz = 0.0, mz = 0.0;
+#pragma unroll
+    for (int c = 0; c < DZ; ++c)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int a = 0; a < 2; ++a) sc += at(i0 + a, j0 + b, k0 + c).c;
+#pragma unroll
+    for (int c = 0; c < DZ; ++c)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) si += at(i0, j0 + b, k0 + c).r;
+#pragma unroll
+    for (int c = 0; c < DZ; ++c)
+#pragma unroll
+        for (int a = 0; a < 2; ++a) si += at(i0 + a, j0, k0 + c).u;
+    if constexpr (DZ == 2) {
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int a = 0; a < 2; ++a) si += at(i0 + a, j0 + b, k0).f;
+    }
+#pragma unroll
+    for (int c = 0; c < DZ; ++c)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            if (I + 1 < cd.nx) px += at(i0 + 1, j0 + b, k0 + c).r;
+            if (I > 0) mx += at(i0 - 1, j0 + b, k0 + c).r;
+        }
+#pragma unroll
+    for (int c = 0; c < DZ; ++c)
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            if (J + 1 < cd.ny) py += at(i0 + a, j0 + 1, k0 + c).u;
+            if (J > 0) my += at(i0 + a, j0 - 1, k0 + c).u;
+        }
+    if constexpr (DZ == 2) {
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                if (K + 1 < cd.nz) pz += at(i0 + a, j0 + b, k0 + 1).f;
+                if (K > 0) mz += at(i0 + a, j0 + b, k0 - 1).f;
+            }
+    }
+    const double kappa = DZ == 2 ? 0.0625 : 0.125;
+    const double se = ((((px + mx) + py) + my) + pz) + mz;
+    Row<double> r;
+    r.c = kappa * ((2.0 * sc + 4.0 * si) + se);
+    r.r = kappa * px;
+    r.u = kappa * py;
+    r.f = kappa * pz;
+    coarse[I + (idx_t)cd.nx * (J + (idx_t)cd.ny * K)] = r;
+}
+
 // FdmMgUtils3::correct (src/jet/fdm_mg_linear_system3.cpp:100-179): fine +=
 // trilinear (1/4, 3/4) prolongation of coarse, clamped.  Jet clamps the k
 // upper index against n.y (:159); this kernel clamps against min(n.y, n.z),

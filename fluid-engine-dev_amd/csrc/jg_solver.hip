@@ -1621,6 +1621,47 @@ int jg_mg_solve_host(jg_solver* s, size_t nlev, const size_t* nx, const size_t* 
     return jg_solver_download_x(s, x);
 }
 
+int jg_solver_coarsen(jg_solver* s) {
+    if (!s || s->lv.empty()) return JG_ERR_INVALID_ARG;
+    JG_HIP(hipSetDevice(s->device));
+    for (size_t l = 1; l < s->lv.size(); ++l) {
+        const Dims cd = s->lv[l].d;
+        const Geo g = geo_planes(cd);
+        if (s->twod)
+            k_mg_coarsen<1><<<g.grid, g.block, 0, s->stream>>>(s->lv[l - 1].A64, s->lv[l].A64, cd);
+        else
+            k_mg_coarsen<2><<<g.grid, g.block, 0, s->stream>>>(s->lv[l - 1].A64, s->lv[l].A64, cd);
+    }
+    JG_TRY(check_launch(s, "mg_coarsen"));
+    s->f32_valid = false;
+    return sync(s);
+}
+
+int jg_mgpcg_flat_solve_host(jg_solver* s, size_t nx, size_t ny, size_t nz, const double* rows,
+                             const double* b, double* x, unsigned max_iter, const jg_mg_params* p,
+                             const jg_gpu_options* o, jg_solve_info* info) {
+    if (!s || !rows || !b || !x || !p) return JG_ERR_INVALID_ARG;
+    // FdmMgUtils3::resizeArrayWithFinest (include/jet/detail/fdm_mg_linear_system3-inl.h:
+    // 35-50): halve while every extent is even, up to max_levels; one-plane
+    // systems halve in x and y (FdmMgUtils2's rule)
+    size_t lx[16] = {nx}, ly[16] = {ny}, lz[16] = {nz};
+    const bool plane = nz == 1;
+    const size_t cap = std::min<size_t>(std::max<size_t>(p->max_levels, 1), 16);
+    size_t nlev = 1;
+    while (nlev < cap && lx[nlev - 1] % 2 == 0 && ly[nlev - 1] % 2 == 0 &&
+           (plane || lz[nlev - 1] % 2 == 0)) {
+        lx[nlev] = lx[nlev - 1] / 2;
+        ly[nlev] = ly[nlev - 1] / 2;
+        lz[nlev] = plane ? 1 : lz[nlev - 1] / 2;
+        ++nlev;
+    }
+    JG_TRY(jg_solver_set_levels(s, nlev, lx, ly, lz));
+    JG_TRY(jg_solver_upload(s, 0, rows, b));
+    JG_TRY(jg_solver_coarsen(s));
+    JG_TRY(jg_solver_mgpcg(s, max_iter, p, o, info));
+    return jg_solver_download_x(s, x);
+}
+
 }  // extern "C"
 
 // ------------------------------------------------------ kernel timing

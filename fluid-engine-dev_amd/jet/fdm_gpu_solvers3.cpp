@@ -212,6 +212,76 @@ const jg_solve_info& FdmGpuMgpcgSolver3::lastSolveInfo() const {
     return _info;
 }
 
+// ---------------------------------------------------- flat MGPCG
+
+FdmGpuMgpcgFlatSolver3::FdmGpuMgpcgFlatSolver3(
+    unsigned int numberOfCgIter, size_t maxNumberOfLevels,
+    unsigned int numberOfRestrictionIter, unsigned int numberOfCorrectionIter,
+    unsigned int numberOfCoarsestIter, unsigned int numberOfFinalIter,
+    double maxTolerance, double sorFactor, bool useRedBlackOrdering, int device)
+    : _sorFactor(sorFactor),
+      _useRedBlackOrdering(useRedBlackOrdering),
+      _maxNumberOfIterations(numberOfCgIter),
+      _device(device) {
+    _mgParams.maxNumberOfLevels = maxNumberOfLevels;
+    _mgParams.numberOfRestrictionIter = numberOfRestrictionIter;
+    _mgParams.numberOfCorrectionIter = numberOfCorrectionIter;
+    _mgParams.numberOfCoarsestIter = numberOfCoarsestIter;
+    _mgParams.numberOfFinalIter = numberOfFinalIter;
+    _mgParams.maxTolerance = maxTolerance;
+    jg_gpu_options_default(&_options);
+    _info.last_residual = kMaxD;
+}
+
+FdmGpuMgpcgFlatSolver3::~FdmGpuMgpcgFlatSolver3() { jg_solver_destroy(_solver); }
+
+bool FdmGpuMgpcgFlatSolver3::solve(FdmLinearSystem3* system) {
+    JET_THROW_INVALID_ARG_IF(system == nullptr);
+    const Size3 n = system->A.size();
+    JET_THROW_INVALID_ARG_IF(!sameSize(n, system->b.size()) ||
+                             !sameSize(n, system->x.size()));
+    if (n.x * n.y * n.z == 0) {
+        _info = jg_solve_info{};
+        return true;
+    }
+    jg_mg_params p;
+    jg_mg_params_default(&p);
+    p.max_levels = _mgParams.maxNumberOfLevels;
+    p.restrict_iter = _mgParams.numberOfRestrictionIter;
+    p.correct_iter = _mgParams.numberOfCorrectionIter;
+    p.coarsest_iter = _mgParams.numberOfCoarsestIter;
+    p.final_iter = _mgParams.numberOfFinalIter;
+    p.max_tol = _mgParams.maxTolerance;
+    p.sor = _sorFactor;
+    p.red_black = _useRedBlackOrdering ? 1 : 0;
+    jg_solver* s = lazySolver(&_solver, _device);
+    check(jg_mgpcg_flat_solve_host(s, n.x, n.y, n.z,
+                                   reinterpret_cast<const double*>(system->A.data()),
+                                   system->b.data(), system->x.data(),
+                                   _maxNumberOfIterations, &p, &_options, &_info),
+          s, "FdmGpuMgpcgFlatSolver3::solve");
+    JET_INFO << "Residual after solving MGPCG: " << _info.last_residual
+             << " Number of MGPCG iterations: " << _info.iterations;
+    return _info.converged != 0;
+}
+
+const MgParameters<FdmBlas3>& FdmGpuMgpcgFlatSolver3::params() const {
+    return _mgParams;
+}
+unsigned int FdmGpuMgpcgFlatSolver3::maxNumberOfIterations() const {
+    return _maxNumberOfIterations;
+}
+unsigned int FdmGpuMgpcgFlatSolver3::lastNumberOfIterations() const {
+    return _info.iterations;
+}
+double FdmGpuMgpcgFlatSolver3::tolerance() const { return _mgParams.maxTolerance; }
+double FdmGpuMgpcgFlatSolver3::lastResidual() const { return _info.last_residual; }
+void FdmGpuMgpcgFlatSolver3::setGpuOptions(const jg_gpu_options& options) {
+    _options = options;
+}
+const jg_gpu_options& FdmGpuMgpcgFlatSolver3::gpuOptions() const { return _options; }
+const jg_solve_info& FdmGpuMgpcgFlatSolver3::lastSolveInfo() const { return _info; }
+
 // ------------------------------------------------------------ MG
 
 FdmGpuMgSolver3::FdmGpuMgSolver3(size_t maxNumberOfLevels,

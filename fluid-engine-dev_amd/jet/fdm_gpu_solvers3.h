@@ -14,6 +14,9 @@
 //   FdmGpuMgSolver3     : FdmMgSolver3
 //       replaces FdmMgSolver3 (include/jet/fdm_mg_solver3.h:23-51,
 //       src/jet/fdm_mg_solver3.cpp:14-74): one V-cycle per solve;
+//   FdmGpuMgpcgFlatSolver3 : FdmLinearSystemSolver3
+//       MGPCG for callers that hand over one level (its coarse matrices
+//       built on the device from A);
 //   FdmGpuMgpcgSolver3  : FdmMgSolver3
 //       replaces FdmMgpcgSolver3 (include/jet/fdm_mgpcg_solver3.h:35-56,
 //       src/jet/fdm_mgpcg_solver3.cpp:40-80): same constructor arguments and
@@ -161,9 +164,63 @@ class FdmGpuMgSolver3 final : public FdmMgSolver3 {
     jg_solve_info _info{};
 };
 
+//! GPU MGPCG for single-level systems (the flat entry).
+//!
+//! Jet's FdmMgSolver3::solve(FdmLinearSystem3*) is final and does nothing
+//! (include/jet/fdm_mg_solver3.h:42), so a caller that builds one level --
+//! GridBackwardEulerDiffusionSolver3 (src/jet/
+//! grid_backward_euler_diffusion_solver3.cpp:42,71,85,99) -- cannot use
+//! multigrid.  This class derives FdmLinearSystemSolver3 instead and takes
+//! FdmMgpcgSolver3's constructor arguments
+//! (include/jet/fdm_mgpcg_solver3.h:40-45): solve() sizes a hierarchy by
+//! Jet's resizeArrayWithFinest rule (halving in x and y only for a one-plane
+//! system), builds every coarse matrix on the device from A (2x2x2
+//! aggregation scaled to the operator at twice the spacing, jetgpu.h's
+//! jg_solver_coarsen) and runs the library's MGPCG.  solve() zeroes and
+//! overwrites x and returns Jet's MGPCG result; solveCompressed() returns
+//! false (the base class no-op).
+class FdmGpuMgpcgFlatSolver3 final : public FdmLinearSystemSolver3 {
+ public:
+    FdmGpuMgpcgFlatSolver3(unsigned int numberOfCgIter, size_t maxNumberOfLevels,
+                           unsigned int numberOfRestrictionIter = 5,
+                           unsigned int numberOfCorrectionIter = 5,
+                           unsigned int numberOfCoarsestIter = 20,
+                           unsigned int numberOfFinalIter = 20,
+                           double maxTolerance = 1e-9, double sorFactor = 1.5,
+                           bool useRedBlackOrdering = false, int device = 0);
+    ~FdmGpuMgpcgFlatSolver3() override;
+
+    FdmGpuMgpcgFlatSolver3(const FdmGpuMgpcgFlatSolver3&) = delete;
+    FdmGpuMgpcgFlatSolver3& operator=(const FdmGpuMgpcgFlatSolver3&) = delete;
+
+    bool solve(FdmLinearSystem3* system) override;
+
+    const MgParameters<FdmBlas3>& params() const;
+    unsigned int maxNumberOfIterations() const;
+    unsigned int lastNumberOfIterations() const;
+    double tolerance() const;
+    double lastResidual() const;
+
+    void setGpuOptions(const jg_gpu_options& options);
+    const jg_gpu_options& gpuOptions() const;
+
+    const jg_solve_info& lastSolveInfo() const;
+
+ private:
+    MgParameters<FdmBlas3> _mgParams;
+    double _sorFactor;
+    bool _useRedBlackOrdering;
+    unsigned int _maxNumberOfIterations;
+    int _device;
+    jg_gpu_options _options;
+    jg_solver* _solver = nullptr;
+    jg_solve_info _info{};
+};
+
 typedef std::shared_ptr<FdmGpuCgSolver3> FdmGpuCgSolver3Ptr;
 typedef std::shared_ptr<FdmGpuMgSolver3> FdmGpuMgSolver3Ptr;
 typedef std::shared_ptr<FdmGpuMgpcgSolver3> FdmGpuMgpcgSolver3Ptr;
+typedef std::shared_ptr<FdmGpuMgpcgFlatSolver3> FdmGpuMgpcgFlatSolver3Ptr;
 
 }  // namespace jet
 

@@ -100,6 +100,10 @@ SIGNATURES = {
                                    C.POINTER(_vp), _vp, _vp, C.POINTER(MgParams),
                                    C.POINTER(GpuOptions), C.POINTER(SolveInfo)]),
     "jg_cg_solve_host": (C.c_int, [_vp, _sz, _sz, _sz, _vp, _vp, _vp, _u, _d, C.POINTER(SolveInfo)]),
+    "jg_mgpcg_flat_solve_host": (C.c_int, [_vp, _sz, _sz, _sz, _vp, _vp, _vp, _u,
+                                           C.POINTER(MgParams), C.POINTER(GpuOptions),
+                                           C.POINTER(SolveInfo)]),
+    "jg_solver_coarsen": (C.c_int, [_vp]),
     "jg_csr_cg_solve_host": (C.c_int, [_vp, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _u, _d,
                                        C.POINTER(SolveInfo)]),
     "jg_csr_mvm_host": (C.c_int, [_vp, _sz, _sz, _vp, _vp, _vp, _vp, _vp]),
@@ -338,6 +342,27 @@ class Solver:
         self.check(lib().jg_cg_solve_host(self.h, nx, ny, nz, A.ctypes.data, b.ctypes.data,
                                           x.ctypes.data, max_iter, tol, C.byref(info)), "cg_host")
         return x, info
+
+    def mgpcg_flat_host(self, A, b, max_iter, params: MgParams, options: GpuOptions | None = None):
+        """jg_mgpcg_flat_solve_host: MGPCG on a single-level system (host
+        arrays A (nz, ny, nx, 4), b (nz, ny, nx)); the hierarchy and its
+        coarse matrices are made on the device.  Returns (x, info)."""
+        import numpy as np
+
+        A = np.ascontiguousarray(A, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        nz, ny, nx = b.shape
+        x = np.empty_like(b)
+        info = SolveInfo()
+        self.check(lib().jg_mgpcg_flat_solve_host(self.h, nx, ny, nz, A.ctypes.data, b.ctypes.data,
+                                                  x.ctypes.data, max_iter, C.byref(params),
+                                                  C.byref(options) if options is not None else None,
+                                                  C.byref(info)), "mgpcg_flat_host")
+        return x, info
+
+    def coarsen(self):
+        """jg_solver_coarsen: the coarse matrices of every level from level 0's."""
+        self.check(lib().jg_solver_coarsen(self.h), "coarsen")
 
     def csr_cg(self, values, row_ptr, col_idx, b, max_iter, tol):
         """FdmCgSolver3::solveCompressed on host CSR arrays (numpy: float64

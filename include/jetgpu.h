@@ -226,6 +226,31 @@ int jg_mg_solve_host(jg_solver* s, size_t nlev, const size_t* nx,
                      const jg_mg_params* p, const jg_gpu_options* o,
                      jg_solve_info* info);
 
+/* The flat-entry MGPCG: FdmLinearSystemSolver3::solve(FdmLinearSystem3*)
+ * (include/jet/fdm_linear_system_solver3.h:23) with multigrid, which Jet
+ * itself does not offer (FdmMgSolver3::solve(FdmLinearSystem3*) is final and
+ * does nothing, include/jet/fdm_mg_solver3.h:42), so that callers holding a
+ * single-level system -- GridBackwardEulerDiffusionSolver3 (src/jet/
+ * grid_backward_euler_diffusion_solver3.cpp:42,71,85,99) -- get multigrid.
+ * The hierarchy is sized by Jet's FdmMgUtils3::resizeArrayWithFinest rule
+ * with p->max_levels (x and y only for a one-plane system) and its coarse
+ * matrices are built on the device from A (jg_solver_coarsen); then
+ * jg_solver_mgpcg.  A and b are host arrays in Jet's layout; x is
+ * overwritten. */
+int jg_mgpcg_flat_solve_host(jg_solver* s, size_t nx, size_t ny, size_t nz,
+                             const double* rows, const double* b, double* x,
+                             unsigned max_iter, const jg_mg_params* p,
+                             const jg_gpu_options* o, jg_solve_info* info);
+/* Coarse matrices of every level below the finest from the finest level's
+ * resident A (uploaded or built on the device).  A coarse cell aggregates
+ * the 2x2x2 fine cells it covers (2x2 on one-plane levels); with A = M + L
+ * (M the row sums, L the couplings) its row is that of kappa (P'AP + P'MP),
+ * kappa = 1/16 (1/8), P the piecewise-constant injection: a uniform stencil
+ * keeps its mass and quarters its couplings (the operator at twice the
+ * spacing).  No counterpart in Jet, whose multigrid callers build each level
+ * from markers; FP64, bitwise equal to the oracle's or_mg_coarsen. */
+int jg_solver_coarsen(jg_solver* s);
+
 /* ------------------------------------ FdmBlas3 / FdmMgUtils3 on device */
 /* All pointers are device pointers; these run on the solver's stream and
  * return after the work is complete. */

@@ -450,6 +450,74 @@ void or_mg_correct2(const double* coarser, double* finer, size_t fnx, size_t fny
  * hierarchy cannot halve an odd nz.  Its rows are 2-D rows with front = 0,
  * so the 3-D relax, residual and products give the 2-D values (the absent z
  * terms add +0); only the transfers differ. */
+/* Coarse operator of the flat-entry MGPCG.  No counterpart in Jet: its
+ * FdmMgSolver3::solve(FdmLinearSystem3*) is final and does nothing
+ * (include/jet/fdm_mg_solver3.h:42), and its multigrid callers build every
+ * level from markers.  A coarse cell C aggregates the 2x2x2 fine cells it
+ * covers (2x2 on one-plane levels, fnz == 1).  Split the fine matrix as
+ * A = M + L, M = diag(row sums), L the coupling part with zero row sums; then
+ *   A_c = (1/8) P'MP + (1/16) P'LP   (one plane: (1/4) P'MP + (1/8) P'LP)
+ * with P the piecewise-constant injection.  A uniform stencil (mass m,
+ * couplings -c) coarsens to mass m and couplings -c/4: the operator
+ * rediscretised at twice the spacing, the scale that FdmMgUtils3's restriction
+ * (a weighted average) and correction (an interpolation) expect.  In rows,
+ * with kappa = 1/16 (1/8):
+ *   right_C  = kappa * (the fine +x couplings leaving C), 0 on the last coarse
+ *              column; up_C and front_C likewise;
+ *   center_C = kappa * ((2 * sum of centers + 4 * sum of couplings inside C)
+ *              + sum of couplings leaving C through its faces),
+ * a coupling to a cell outside the grid ignored (as mvm ignores it).  Sums
+ * run in the fixed order below; the GPU kernel uses the same one. */
+void or_mg_coarsen(const double* fine, size_t fnx, size_t fny, size_t fnz, double* coarse) {
+    const int plane = fnz == 1;
+    const size_t cnx = fnx / 2, cny = fny / 2, cnz = plane ? 1 : fnz / 2;
+    const size_t dz = plane ? 1 : 2;
+    const double kappa = plane ? 0.125 : 0.0625;
+#define FR(i, j, k, f) fine[4 * IX(i, j, k, fnx, fny) + (f)]
+#pragma omp parallel for schedule(static)
+    for (size_t K = 0; K < cnz; ++K)
+        for (size_t J = 0; J < cny; ++J)
+            for (size_t I = 0; I < cnx; ++I) {
+                const size_t i0 = 2 * I, j0 = 2 * J, k0 = dz * K;
+                double sc = 0.0, si = 0.0;
+                double px = 0.0, mx = 0.0, py = 0.0, my = 0.0, pz = 0.0, mz = 0.0;
+                for (size_t c = 0; c < dz; ++c)
+                    for (size_t b = 0; b < 2; ++b)
+                        for (size_t a = 0; a < 2; ++a) sc += FR(i0 + a, j0 + b, k0 + c, 0);
+                for (size_t c = 0; c < dz; ++c)
+                    for (size_t b = 0; b < 2; ++b) si += FR(i0, j0 + b, k0 + c, 1);
+                for (size_t c = 0; c < dz; ++c)
+                    for (size_t a = 0; a < 2; ++a) si += FR(i0 + a, j0, k0 + c, 2);
+                if (!plane)
+                    for (size_t b = 0; b < 2; ++b)
+                        for (size_t a = 0; a < 2; ++a) si += FR(i0 + a, j0 + b, k0, 3);
+                for (size_t c = 0; c < dz; ++c)
+                    for (size_t b = 0; b < 2; ++b) {
+                        if (I + 1 < cnx) px += FR(i0 + 1, j0 + b, k0 + c, 1);
+                        if (I > 0) mx += FR(i0 - 1, j0 + b, k0 + c, 1);
+                    }
+                for (size_t c = 0; c < dz; ++c)
+                    for (size_t a = 0; a < 2; ++a) {
+                        if (J + 1 < cny) py += FR(i0 + a, j0 + 1, k0 + c, 2);
+                        if (J > 0) my += FR(i0 + a, j0 - 1, k0 + c, 2);
+                    }
+                if (!plane)
+                    for (size_t b = 0; b < 2; ++b)
+                        for (size_t a = 0; a < 2; ++a) {
+                            if (K + 1 < cnz) pz += FR(i0 + a, j0 + b, k0 + 1, 3);
+                            if (K > 0) mz += FR(i0 + a, j0 + b, k0 - 1, 3);
+                        }
+                const double se = ((((px + mx) + py) + my) + pz) + mz;
+                double* row = coarse + 4 * IX(I, J, K, cnx, cny);
+                row[0] = kappa * ((2.0 * sc + 4.0 * si) + se);
+                row[1] = kappa * px;
+                row[2] = kappa * py;
+                row[3] = kappa * pz;
+            }
+#undef FR
+}
+
+
 int or_mg_is_2d(const or_mg_system* s) { return s->nlev > 1 && s->nz[0] == 1; }
 
 static void mg_restrict_level(const or_mg_system* s, size_t l, const double* fine,

@@ -108,6 +108,9 @@ size_t or_levels_with_finest2(size_t nx, size_t ny, size_t max_levels,
 void or_mg_restrict2(const double* finer, double* coarser, size_t cnx, size_t cny);
 void or_mg_correct2(const double* coarser, double* finer, size_t fnx, size_t fny);
 int or_mg_is_2d(const or_mg_system* s);
+/* Coarse operator of the flat-entry MGPCG from the next finer level's rows
+ * (2x2x2 aggregation; 2x2 when fnz == 1); coarse is (fnx/2, fny/2, fnz/2 or 1). */
+void or_mg_coarsen(const double* fine, size_t fnx, size_t fny, size_t fnz, double* coarse);
 double or_mg_vcycle(const or_mg_system* sys, const or_mg_params* p,
                     double** x, double** b, double** buffer);
 int or_mg_solve(or_mg_system* sys, const or_mg_params* p, double* last_norm);

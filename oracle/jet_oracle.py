@@ -391,6 +391,36 @@ class Oracle(_BoundaryCalls, _CsrCalls):
         return z
 
     # ------------------------------------------------ single-phase build
+    def mg_coarsen(self, A):
+        """or_mg_coarsen: the aggregated coarse rows of one level (2x2x2, or
+        2x2 for a one-plane level)."""
+        nz, ny, nx = A.shape[:3]
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        out = np.empty((1 if nz == 1 else nz // 2, ny // 2, nx // 2, 4))
+        self.lib.or_mg_coarsen(_d(A), _sz(nx), _sz(ny), _sz(nz), _d(out))
+        return out
+
+    def mg_flat_hierarchy(self, A0, b0, max_levels):
+        """The flat-entry MGPCG's hierarchy of one system: Jet's level sizing
+        (FdmMgUtils3 / FdmMgUtils2 for one plane), the coarse rows aggregated
+        level by level from A0; b on level 0, x zero."""
+        nz, ny, nx = A0.shape[:3]
+        if nz == 1:
+            sizes = [(a, b, 1) for a, b in self.levels_with_finest2((nx, ny), max_levels)]
+        else:
+            sizes = self.levels_with_finest((nx, ny, nz), max_levels)
+        s = MgSystem()
+        A = np.ascontiguousarray(A0, dtype=np.float64)
+        for l, (lx, ly, lz) in enumerate(sizes):
+            if l > 0:
+                A = self.mg_coarsen(A)
+            assert A.shape[:3] == (lz, ly, lx)
+            s.A.append(A)
+            s.b.append(np.ascontiguousarray(b0, dtype=np.float64).copy() if l == 0
+                       else np.zeros((lz, ly, lx)))
+            s.x.append(np.zeros((lz, ly, lx)))
+        return s
+
     def sp_markers(self, bsdf, fsdf):
         m = np.empty(bsdf.shape, np.int8)
         self.lib.or_sp_markers(_d(bsdf), _d(fsdf), _c(m), _sz(m.size))

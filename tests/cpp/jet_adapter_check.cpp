@@ -24,8 +24,12 @@
 #include <jet/fdm_mg_linear_system3.h>
 #include <jet/fdm_mg_solver3.h>
 #include <jet/fdm_mgpcg_solver3.h>
+#include <jet/cell_centered_scalar_grid3.h>
+#include <jet/custom_scalar_field3.h>
+#include <jet/grid_backward_euler_diffusion_solver3.h>
 #include <jet/logging.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -212,6 +216,23 @@ void argumentChecks() {
     expect(throwsInvalidArgument([&] { one.solve(nullptr); }) &&
                throwsInvalidArgument([&] { one.solve(&mbad); }),
            "FdmGpuMgSolver3: null system and size mismatch throw std::invalid_argument");
+
+    FdmGpuMgpcgFlatSolver3 fm(60, 5, 2, 3, 20, 4, 1e-7, 1.3, true);
+    const MgParameters<FdmBlas3>& f = fm.params();
+    expect(f.maxNumberOfLevels == 5 && f.numberOfRestrictionIter == 2 &&
+               f.numberOfCorrectionIter == 3 && f.numberOfCoarsestIter == 20 &&
+               f.numberOfFinalIter == 4 && f.maxTolerance == 1e-7 &&
+               fm.maxNumberOfIterations() == 60 && fm.tolerance() == 1e-7,
+           "FdmGpuMgpcgFlatSolver3: params() and accessors carry the constructor arguments");
+    FdmLinearSystemSolver3Ptr fmBase = std::make_shared<FdmGpuMgpcgFlatSolver3>(60, 5);
+    FdmCompressedLinearSystem3 fcomp;
+    expect(std::dynamic_pointer_cast<FdmMgSolver3>(fmBase) == nullptr &&
+               !fmBase->solveCompressed(&fcomp),
+           "FdmGpuMgpcgFlatSolver3 is a flat solver (not an FdmMgSolver3); solveCompressed "
+           "is the base no-op (false)");
+    expect(throwsInvalidArgument([&] { fm.solve(nullptr); }) &&
+               throwsInvalidArgument([&] { fm.solve(&bad); }),
+           "FdmGpuMgpcgFlatSolver3: null system and size mismatch throw std::invalid_argument");
 }
 
 void gpuChecks() {
@@ -526,6 +547,53 @@ void gpuChecks() {
         expect(log.str().find("Residual after solving MGPCG: ") != std::string::npos &&
                    log.str().find(want) != std::string::npos,
                "MGPCG: Jet's INFO line with the residual and the iteration count");
+    }
+
+    // --- The flat-entry MGPCG inside Jet's own GridBackwardEulerDiffusionSolver3
+    // (src/jet/grid_backward_euler_diffusion_solver3.cpp:26-50): the caller
+    // is unchanged, only setLinearSystemSolver() differs; the result equals
+    // the same diffusion solved by Jet's CG, in FP64 and mixed precision.
+    for (int dirichlet = 0; dirichlet < 2; ++dirichlet) {
+        const Size3 n(48, 32, 40);
+        const Vector3D h(1.0 / 48, 1.0 / 48, 1.0 / 48);
+        CellCenteredScalarGrid3 src(n, h), jetOut(n, h), gpuOut(n, h), mixOut(n, h);
+        src.fill([](const Vector3D& p) {
+            return std::sin(6.0 * p.x) * std::cos(4.0 * p.y) + p.z;
+        });
+        // a spherical collider (boundary SDF) under a flat free surface (fluid SDF)
+        CustomScalarField3 bsdf([](const Vector3D& p) {
+            return (p - Vector3D(0.5, 0.3, 0.4)).length() - 0.2;
+        });
+        CustomScalarField3 fsdf([](const Vector3D& p) { return p.y - 0.55; });
+        const auto bc = dirichlet ? GridBackwardEulerDiffusionSolver3::Dirichlet
+                                  : GridBackwardEulerDiffusionSolver3::Neumann;
+        GridBackwardEulerDiffusionSolver3 jetDiff(bc), gpuDiff(bc), mixDiff(bc);
+        jetDiff.setLinearSystemSolver(std::make_shared<FdmCgSolver3>(2000, 1e-13));
+        auto flat = std::make_shared<FdmGpuMgpcgFlatSolver3>(200, 4, 2, 2, 20, 2, 1e-13);
+        auto flatMixed = std::make_shared<FdmGpuMgpcgFlatSolver3>(200, 4, 2, 2, 20, 2, 1e-13);
+        jg_gpu_options o = flatMixed->gpuOptions();
+        o.precision = JG_MIXED;
+        flatMixed->setGpuOptions(o);
+        gpuDiff.setLinearSystemSolver(flat);
+        mixDiff.setLinearSystemSolver(flatMixed);
+        jetDiff.solve(src, 0.05, 1.0 / 60.0, &jetOut, bsdf, fsdf);
+        gpuDiff.solve(src, 0.05, 1.0 / 60.0, &gpuOut, bsdf, fsdf);
+        mixDiff.solve(src, 0.05, 1.0 / 60.0, &mixOut, bsdf, fsdf);
+        double d = 0.0, dm = 0.0, m = 0.0;
+        jetOut.forEachDataPointIndex([&](size_t i, size_t j, size_t k) {
+            d = std::max(d, std::fabs(gpuOut(i, j, k) - jetOut(i, j, k)));
+            dm = std::max(dm, std::fabs(mixOut(i, j, k) - jetOut(i, j, k)));
+            m = std::max(m, std::fabs(jetOut(i, j, k)));
+        });
+        std::printf("     diffusion (%s): flat MGPCG %u it (mixed %u), max diff %.3e (mixed "
+                    "%.3e) of max %.3e\n",
+                    dirichlet ? "Dirichlet" : "Neumann", flat->lastNumberOfIterations(),
+                    flatMixed->lastNumberOfIterations(), d, dm, m);
+        expect(d <= 1e-9 * m && dm <= 1e-9 * m && flat->lastNumberOfIterations() < 40,
+               dirichlet ? "flat MGPCG in GridBackwardEulerDiffusionSolver3 (Dirichlet) = "
+                           "Jet's CG (1e-9), FP64 and mixed"
+                         : "flat MGPCG in GridBackwardEulerDiffusionSolver3 (Neumann) = "
+                           "Jet's CG (1e-9), FP64 and mixed");
     }
 
     // --- MGPCG against Jet's CG on the same fine system (a different

@@ -34,7 +34,7 @@ def run_check(*args):
 
 def test_adapter_argument_checks():
     out = run_check("--no-gpu")
-    assert out.count("ok  ") == 16
+    assert out.count("ok  ") == 19
 
 
 @pytest.mark.gpu
