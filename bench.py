@@ -210,6 +210,7 @@ def main():
     out["system_build"] = {"gpu_ms": round(build_ms, 3), "levels": L,
                            "what": "GridSinglePhasePressureSolver3 markers and A on every level, "
                                    "b on level 0 (jg_sp_build)"}
+    out["projection_step"] = step_leg(jg, solver, n, params, opts, res, args)
     bc = bc_leg(jg, solver, n, args)
     out["constrain_velocity"] = bc["gpu"]
     csr = csr_leg(jg, n, args)
@@ -221,6 +222,8 @@ def main():
     small = small_configs_leg(jg, args)
     out["closed_sphere_cg_64"] = small["cg64"]["gpu"]
     out["dam_break_mgpcg_128"] = small["dam128"]["gpu"]
+    batch = batch_leg(jg, args)
+    out["batch_64_uneven"] = batch["gpu"]
     if not args.no_cpu:
         out.update(cpu_leg(jg, n, L, args, res))
         out["constrain_velocity"].update(bc_cpu_leg(bc, n))
@@ -230,6 +233,7 @@ def main():
         c1, c2 = small_configs_cpu_leg(small)
         out["closed_sphere_cg_64"].update(c1)
         out["dam_break_mgpcg_128"].update(c2)
+        out["batch_64_uneven"].update(batch_cpu_leg(batch))
     print(json.dumps(out))
     solver.close()
 
@@ -448,6 +452,48 @@ def collect_traffic(args):
 BC_DEPTH = 5  # ceil(max CFL) with GridFluidSolver3's default max CFL of 5
 
 
+def step_leg(jg, solver, n, params, opts, res, args):
+    """One pressure-projection step as a simulation issues it, every step from
+    the same velocities: GridSinglePhasePressureSolver3's build (markers and A
+    on every level, b), the MGPCG solve and the pressure gradient into a second
+    set of face arrays.  A rebuilt A invalidates the FP32 copy of the
+    matrices, so each mixed step pays its refresh (one pass over the FP64
+    rows of every level) inside the timed region."""
+    import torch
+
+    f64 = torch.float64
+    shapes = [(n, n, n + 1), (n, n + 1, n), (n + 1, n, n)]
+    u, v, w = (torch.empty(sh, dtype=f64, device="cuda") for sh in shapes)
+    uo, vo, wo = (torch.empty(sh, dtype=f64, device="cuda") for sh in shapes)
+    bs, fs = (torch.empty((n, n, n), dtype=f64, device="cuda") for _ in range(2))
+    solver.generate_scene(jg.scene_desc(jg.JG_SCENE_SPHERE_POOL, n, seed=args.seed), u, v, w, bs, fs)
+
+    def step(o):
+        solver.sp_build(u, v, w, bs, fs, 1.0)
+        info = solver.mgpcg(500, params, o)
+        solver.sp_apply_gradient(u, v, w, uo, vo, wo, 1.0)
+        return info
+
+    out = {"what": "jg_sp_build + jg_solver_mgpcg + jg_sp_apply_gradient per step, the FP32 "
+                   "matrix copy refreshed every mixed step"}
+    for name in ("fp64", "mixed"):
+        step(opts[name])
+        reps = max(args.steps, 3)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        infos = [step(opts[name]) for _ in range(reps)]
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        out[name] = {"ms": round(ms, 4), "iterations": infos[-1].iterations,
+                     "rel_residual": infos[-1].rel_residual,
+                     "solve_only_ms": round(res[name]["ms"], 4),
+                     "rest_ms": round(ms - res[name]["ms"], 4)}
+        log(f"bench: projection step {name} {ms:.3f} ms")
+    del u, v, w, uo, vo, wo, bs, fs
+    torch.cuda.empty_cache()
+    return out
+
+
 def bc_leg(jg, solver, n, args):
     """The step after the projection: GridBlockedBoundaryConditionSolver3::
     constrainVelocity on the same scene's faces, its spheres as a resting
@@ -532,14 +578,17 @@ def csr_leg(jg, n, args):
 
 
 DIF_DT, DIF_MU = 1.0 / 60.0, 0.05  # c = dt mu / h^2 = 54.6 at h = 1/256
+DIF_LEVELS = 6
+DIF_MG = dict(restrict_iter=2, correct_iter=2, coarsest_iter=20, final_iter=2)
 
 
 def diffusion_leg(jg, n, args):
     """GridBackwardEulerDiffusionSolver3 on the scene's n^3 cell-centred grid
     (boundary and fluid SDFs of the sphere pool, Neumann at the colliders):
-    the system built on the device and solved by Jet's CG to ||r|| <= 1e-8
-    ||b||, the field a smooth pattern."""
-    import numpy as np
+    the system built on the device and solved to ||r|| <= 1e-8 ||b|| by
+    Jet's CG and by the flat-entry MGPCG (Jet's level sizing, 6 levels, the
+    coarse matrices built on the device from A), FP64 and mixed; the field a
+    smooth pattern."""
     import torch
 
     f64 = torch.float64
@@ -561,27 +610,66 @@ def diffusion_leg(jg, n, args):
     build_ms = (time.perf_counter() - t0) * 1e3
     b = torch.as_tensor(jg._DeviceView(s.rhs_ptr(), n ** 3, "<f8"), device="cuda")
     tol = 1e-8 * float(torch.linalg.norm(b).item())
+    del b
     t0 = time.perf_counter()
     info = s.cg(2000, tol)
     solve_ms = (time.perf_counter() - t0) * 1e3
     it = max(1, info.iterations)
     log(f"bench: diffusion {n}^3 build {build_ms:.2f} ms, CG {it} it {solve_ms:.2f} ms")
+    xs = {"cg": s.download_x()} if not args.no_cpu else {}
+    gpu = {"cells": n ** 3, "dt": DIF_DT, "mu": DIF_MU, "h": h,
+           "build_ms": round(build_ms, 3), "solve_ms": round(solve_ms, 3),
+           "iterations": info.iterations, "rel_residual": info.rel_residual,
+           "ms_per_iteration": round(solve_ms / it, 4)}
+
+    # the flat-entry MGPCG on the same system
+    L = DIF_LEVELS
+    s.set_levels([(n >> l,) * 3 for l in range(L)])
+    s.diffusion_build(f, bs, fs, DIF_DT, DIF_MU, h)
+    s.coarsen()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.diffusion_build(f, bs, fs, DIF_DT, DIF_MU, h)
+    s.coarsen()
+    torch.cuda.synchronize()
+    coarsen_ms = (time.perf_counter() - t0) * 1e3 - build_ms
+    p = jg.mg_params(max_levels=L, sor=args.omega, **DIF_MG)
+    flat = {"levels": L, "schedule": "V(2,2), 20 coarsest sweeps, SOR %g" % args.omega,
+            "build_and_coarsen_ms": round(build_ms + coarsen_ms, 3)}
+    for name, prec in (("fp64", jg.JG_FP64), ("mixed", jg.JG_MIXED)):
+        o = jg.gpu_options(precision=prec, stop_rule=jg.JG_STOP_RELATIVE, rel_tol=1e-8)
+        s.mgpcg(500, p, o)  # tuning
+        reps = 3
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fi = s.mgpcg(500, p, o)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        flat[name] = {"solve_ms": round(ms, 3), "iterations": fi.iterations,
+                      "rel_residual": fi.rel_residual}
+        if not args.no_cpu:
+            xs[f"flat_{name}"] = s.download_x()
+        log(f"bench: diffusion flat MGPCG {name} {ms:.2f} ms, {fi.iterations} it")
+    flat["speedup_fp64_vs_cg"] = round(solve_ms / flat["fp64"]["solve_ms"], 2)
+    flat["speedup_mixed_vs_cg"] = round(solve_ms / flat["mixed"]["solve_ms"], 2)
+    gpu["flat_mgpcg"] = flat
     sample = None
     if not args.no_cpu:
         sample = tuple(a.cpu().numpy() for a in (f, bs, fs))
     s.close()
-    del u, v, w, bs, fs, f, b
+    del u, v, w, bs, fs, f
     torch.cuda.empty_cache()
-    return {"gpu": {"cells": n ** 3, "dt": DIF_DT, "mu": DIF_MU, "h": h,
-                    "build_ms": round(build_ms, 3), "solve_ms": round(solve_ms, 3),
-                    "iterations": info.iterations, "rel_residual": info.rel_residual,
-                    "ms_per_iteration": round(solve_ms / it, 4)},
-            "sample": sample, "h": h}
+    return {"gpu": gpu, "sample": sample, "h": h, "x": xs, "omega": args.omega}
 
 
 def diffusion_cpu_leg(dif):
-    """The oracle's diffusion build and 5 CG iterations on the same input."""
-    from oracle.jet_oracle import Oracle
+    """The oracle's diffusion build and 5 CG iterations on the same input
+    (one core), its flat-entry MGPCG solve (the same aggregated hierarchy,
+    OpenMP) to 1e-12, and the error of every GPU solution against that."""
+    import numpy as np
+
+    from oracle.jet_oracle import MgParams, Oracle
 
     f, bs, fs = dif["sample"]
     o = Oracle()
@@ -593,9 +681,22 @@ def diffusion_cpu_leg(dif):
     o.cg_solve(A, b, iters, 0.0)
     ms = (time.perf_counter() - t0) * 1e3
     log(f"bench: diffusion oracle build {build_ms:.0f} ms, {ms / iters:.1f} ms per CG iteration")
-    return {"cpu_build_ms": round(build_ms, 1), "cpu_ms_per_iteration": round(ms / iters, 2),
-            "cpu_cores": 1, "cpu_kind": "port",
-            "cpu_sample": f"the oracle's build and {iters} CG iterations on the same grid"}
+    out = {"cpu_build_ms": round(build_ms, 1), "cpu_ms_per_iteration": round(ms / iters, 2),
+           "cpu_cores": 1, "cpu_kind": "port",
+           "cpu_sample": f"the oracle's build and {iters} CG iterations on the same grid"}
+    t0 = time.perf_counter()
+    hier = o.mg_flat_hierarchy(A, b, DIF_LEVELS)
+    ok, it, rel = o.mgpcg_clean_solve(hier, MgParams.make(max_levels=DIF_LEVELS, sor=dif["omega"],
+                                                          **DIF_MG), 200, 1e-12)
+    mg_ms = (time.perf_counter() - t0) * 1e3
+    ref = hier.x[0]
+    nr = np.linalg.norm(ref)
+    out["flat_mgpcg_cpu"] = {"ms": round(mg_ms, 1), "iterations": it, "rel_residual": rel,
+                             "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                             "kind": "port"}
+    out["err_vs_oracle"] = {k: float(np.linalg.norm(x - ref) / nr) for k, x in dif["x"].items()}
+    log(f"bench: diffusion oracle flat MGPCG {mg_ms:.0f} ms ({it} it), errors {out['err_vs_oracle']}")
+    return out
 
 
 # 4096^2: as many cells as the 256^3 workload, in one plane (the stencil
@@ -783,6 +884,92 @@ def small_configs_leg(jg, args):
     out["dam128"] = {"gpu": gpu, "scene": host, "x": xs, "levels": L, "omega": args.omega}
     s.close()
     return out
+
+
+BATCH_PARAMS = dict(restrict_iter=2, correct_iter=2, coarsest_iter=20, final_iter=2)
+
+
+def _batch_levels(n, max_levels=5):
+    lv = 1
+    while lv < max_levels and n % (1 << lv) == 0 and (n >> lv) >= 2:
+        lv += 1
+    return lv
+
+
+def batch_leg(jg, args):
+    """64 independent pressure systems of uneven sizes (edges drawn from
+    {32, 48, 64, 96, 128}, 1-4 spheres and a free surface each, built on the
+    device), solved together by one replayed graph with per-system stop
+    flags, FP64 and mixed, to ||b - A x|| <= 1e-8 ||b|| per system."""
+    import numpy as np
+    import torch
+
+    rng = np.random.default_rng(2024)
+    sizes = [int(n) for n in rng.choice([32, 48, 64, 96, 128], 64)]
+    seeds = [1000 + i for i in range(64)]
+    levels = [_batch_levels(n) for n in sizes]
+    b = jg.Batch([(n, n, n) for n in sizes], levels)
+    f64 = torch.float64
+    for sys_, n, seed in zip(b.systems, sizes, seeds):
+        u, v, w = (torch.empty(sh, dtype=f64, device="cuda")
+                   for sh in ((n, n, n + 1), (n, n + 1, n), (n + 1, n, n)))
+        bs, fs = (torch.empty((n, n, n), dtype=f64, device="cuda") for _ in range(2))
+        sys_.generate_scene(jg.scene_desc(jg.JG_SCENE_BATCH_MEMBER, n, seed=seed), u, v, w, bs, fs)
+        sys_.sp_build(u, v, w, bs, fs, 1.0)
+    torch.cuda.synchronize()
+    p = jg.mg_params(max_levels=5, sor=args.omega, **BATCH_PARAMS)
+    gpu = {"systems": len(sizes), "cells": int(sum(n ** 3 for n in sizes)),
+           "edges": {str(n): sizes.count(n) for n in sorted(set(sizes))}}
+    xs = {}
+    for name, prec in (("fp64", jg.JG_FP64), ("mixed", jg.JG_MIXED)):
+        o = jg.gpu_options(precision=prec, stop_rule=jg.JG_STOP_RELATIVE, rel_tol=1e-8)
+        b.mgpcg(500, p, o)  # tuning and graph capture
+        reps = 3
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            infos, _ = b.mgpcg(500, p, o)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        gpu[name] = {"ms": round(ms, 3), "max_iterations": max(i.iterations for i in infos),
+                     "max_rel_residual": max(i.rel_residual for i in infos),
+                     "cells_per_s": gpu["cells"] / (ms * 1e-3)}
+        xs[name] = [s.download_x() for s in b.systems]
+        log(f"bench: batch of 64 {name} {ms:.2f} ms, max {gpu[name]['max_iterations']} it")
+    gpu["speedup_mixed_vs_fp64"] = round(gpu["fp64"]["ms"] / gpu["mixed"]["ms"], 3)
+    b.close()
+    torch.cuda.empty_cache()
+    # one member of each size is checked against the oracle by the CPU leg
+    first = {}
+    for i, n in enumerate(sizes):
+        first.setdefault(n, i)
+    sample = {i: (sizes[i], seeds[i], levels[i], xs["fp64"][i], xs["mixed"][i])
+              for i in first.values()}
+    return {"gpu": gpu, "sample": sample, "omega": args.omega}
+
+
+def batch_cpu_leg(batch):
+    """The oracle's FP64 solve (1e-12) of one member of each size, and the
+    error of the batched FP64 and mixed solutions against it."""
+    import numpy as np
+
+    import jetgpu as jg
+    from oracle.jet_oracle import MgParams, Oracle
+
+    o = Oracle()
+    errs = {}
+    for i, (n, seed, lv, x64, x32) in sorted(batch["sample"].items()):
+        sc = jg.generate_scene_host(jg.scene_desc(jg.JG_SCENE_BATCH_MEMBER, n, seed=seed))
+        _, sys_ = o.sp_build_mg(sc["u"], sc["v"], sc["w"], (1.0,) * 3, (0.0,) * 3, sc["bsdf"],
+                                sc["fsdf"], lv)
+        o.mgpcg_clean_solve(sys_, MgParams.make(max_levels=lv, sor=batch["omega"], **BATCH_PARAMS),
+                            300, 1e-12)
+        ref = sys_.x[0]
+        nr = np.linalg.norm(ref)
+        errs[f"member {i} ({n}^3)"] = {"fp64": float(np.linalg.norm(x64 - ref) / nr),
+                                       "mixed": float(np.linalg.norm(x32 - ref) / nr)}
+    log(f"bench: batch members vs oracle {errs}")
+    return {"err_vs_oracle": errs}
 
 
 def small_configs_cpu_leg(small):
