@@ -119,7 +119,11 @@ struct BotRegs {
 #pragma unroll
             for (int h = 0; h < H; ++h) {
                 const int m = 2 * h + colour;
-                const int c = (fl[m] & 128) ? cell(m) : 0;  // 0: no cell (flags of cell 0)
+                int c = (fl[m] & 128) ? cell(m) : 0;  // 0: no cell (flags of cell 0)
+                // an empty asm makes c opaque: the neighbour addresses are
+                // formed at each use instead of being hoisted out of the
+                // kernel's loops for every cell slot (which spilled to scratch)
+                asm volatile("" : "+v"(c));
                 cs[h] = c;
                 nv[h] = wc[m] == T(0) ? T(0) : (T(1) - omega) * X[c] + wc[m] * (B[c] - offdiag(m, X, c));
             }
@@ -131,7 +135,8 @@ struct BotRegs {
 #pragma unroll
             for (int m = 0; m < MC; ++m) {
                 if ((fl[m] & 192) != want) continue;
-                const int c = cell(m);
+                int c = cell(m);
+                asm volatile("" : "+v"(c));
                 X[c] = wc[m] == T(0) ? T(0) : (T(1) - omega) * X[c] + wc[m] * (B[c] - offdiag(m, X, c));
             }
         }
@@ -143,7 +148,8 @@ struct BotRegs {
 #pragma unroll
         for (int m = 0; m < MC; ++m) {
             if (!(fl[m] & 128)) continue;
-            const int c = cell(m);
+            int c = cell(m);
+            asm volatile("" : "+v"(c));
             R[c] = B[c] - (A[c].c * X[c] + offdiag(m, X, c));
         }
     }
