@@ -6,25 +6,31 @@
 // of (i, j) positions -- a 56 x 24 interior plus a 4-cell halo that is
 // recomputed, not exchanged -- and marches through a chunk of z planes.  At
 // step t it commits plane t from the registers it was prefetched into
-// (prefetching plane t + 2) and runs h1 on plane t-1 in the same phase (its
-// +z neighbour is the thread's own position, just committed by itself;
-// everything else it reads is a step old), then h2 on t-2, h3 on t-3 and h4 on t-4, each after a
-// barrier: four barriers per step.  Plane t-4 is then final and written
-// back.  Half-sweep h is valid on positions at least h cells inside the tile
-// (it reads neighbours updated by h-1, valid one cell further out), so the
-// interior (ring >= 4) of plane t-4 holds exactly the values two full
-// sweeps give.  Each block also recomputes the 4 planes below and above its
-// chunk.
+// (prefetching plane t + 1 or t + 2) and runs h1 on plane t-1 in the same
+// phase (its +z neighbour is the thread's own position, just committed by
+// itself; everything else it reads is a step old), then h2 on t-2, h3 on t-3
+// and h4 on t-4, each after a barrier: four barriers per step.  Plane t-4 is
+// then final and written back.  Half-sweep h is valid on positions at least h
+// cells inside the tile (it reads neighbours updated by h-1, valid one cell
+// further out), so the interior (ring >= 4) of plane t-4 holds exactly the
+// values two full sweeps give.  Each block also recomputes the 4 planes below
+// and above its chunk.
 //
-// x lives in an LDS ring (planes t-5 .. t in 8 slots), the +x / +y couplings in
-// 5-plane rings (planes t .. t-4; they are the -x / -y couplings of the
-// neighbours; plane t's are written after the step's first barrier); w =
-// omega / c, A.f and b stay in registers.  A thread owns two
-// vertically adjacent positions, which have opposite colours, so every
-// half-sweep updates one position per thread.  Each update is the single
-// sweep's FP32 arithmetic (rb_sum6 / rb_coef / rb_update of jg_sweep.h: Jet's
-// relaxRedBlack operand order for s, src/jet/fdm_gauss_seidel_solver3.cpp:
-// 163-190), so the result is bit-identical to two single FP32 fused sweeps.  Zero-centre rows are set to zero, as in the single sweep.
+// Every ring has six slots and the step loop is unrolled six times, so the
+// slot of every plane a step touches is a compile-time constant: LDS
+// addresses are a per-thread base plus an immediate offset, and the register
+// rings are renamed, never shifted.  x lives in an LDS ring (planes t-5 .. t),
+// the +x / +y couplings in LDS rings (planes t .. t-5; they are the -x / -y
+// couplings of the neighbours; plane t's are written after the step's first
+// barrier).  A thread owns two vertically adjacent positions, which have
+// opposite colours, so every half-sweep updates one position per thread; its
+// register rings hold w = omega / c, b and A.f per colour rather than per
+// position (the commit sorts them), so a half-sweep of a given colour reads
+// fixed registers.  Each update is the single sweep's FP32 arithmetic
+// (rb_sum6 / rb_coef / rb_update of jg_sweep.h: Jet's relaxRedBlack operand
+// order for s, src/jet/fdm_gauss_seidel_solver3.cpp:163-190), so the result
+// is bit-identical to two single FP32 fused sweeps.  Zero-centre rows are set
+// to zero, as in the single sweep.
 //
 // Options: ZERO (x starts at zero, xin not read), CORRECT (the prolongation
 // of a coarse correction is added to x as it is loaded), RES (the residual
@@ -47,6 +53,8 @@ namespace jg {
 constexpr int kS2LX = 64, kS2LY = 32;
 constexpr int kS2TY = 16;  // thread rows (two tile rows each)
 constexpr int kS2Threads = kS2LX * kS2TY;
+// Steps per unrolled loop body, and the slot count of every ring.
+constexpr int kS2U = 6;
 // Halo (in x, y and z) and interior of the tile: 4 cells for the two sweeps,
 // 5 when the residual is written too; one less when x starts at zero with no
 // correction (Z), where the first half-sweep, w b, reads no neighbour and is
@@ -55,19 +63,12 @@ template <bool RES, bool Z = false>
 struct S2Tile {
     static constexpr int H = (RES ? 5 : 4) - (Z ? 1 : 0);
     static constexpr int IX = kS2LX - 2 * H, IY = kS2LY - 2 * H;
-    // r / u coupling ring: planes t .. t-4.  The couplings of plane t are
-    // written after the first barrier of step t (h1 on plane t-1 runs before
-    // it and reads those of t-1), and RES reads those of plane t-4 after the
-    // last barrier of step t, so five slots never collide.
-    static constexpr int CR = 5;
     static_assert(IX % 2 == 0 && IY % 2 == 0, "tile interiors stay even");
 };
-// x ring: planes t-7 .. t are live (8 slots make the slot a mask)
-constexpr int kS2XR = 8;
 constexpr int kS2CzX = kS2LX / 2 + 2, kS2CzY = kS2LY / 2 + 2;
 static_assert(kS2CzX * kS2CzY <= kS2Threads, "one coarse entry per thread");
 
-// FP32 only: the FP64 rings would need 234 KB of LDS.
+// FP32 only: the FP64 rings would need 288 KB of LDS.
 template <typename T, int FIRST, bool ZERO, bool CORRECT, bool RES>
 __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
     const Row<T>* __restrict__ A, const T* __restrict__ b, const T* __restrict__ xin,
@@ -82,11 +83,13 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
     // valid on ring >= H - 1 (the first reads no neighbour)
     constexpr int ZH = ZERO && !CORRECT ? 1 : 0;
     using G = S2Tile<RES, ZH != 0>;
-    constexpr int kH = G::H, kCR = G::CR;
+    constexpr int kH = G::H;
+    constexpr int U = kS2U;
     if (ctl && ctl->done) return;
-    __shared__ __align__(16) T sx[kS2XR][kS2LY][kS2LX];  // x, slot = plane mod 8
-    __shared__ T sr[kCR][kS2LY][kS2LX];                  // A.r, slot = plane mod kCR
-    __shared__ T su[kCR][kS2LY][kS2LX];                  // A.u
+    // slot of plane p: (p - t0) mod U, a constant at every unrolled step
+    __shared__ __align__(16) T sx[U][kS2LY][kS2LX];  // x
+    __shared__ T sr[U][kS2LY][kS2LX];                 // A.r
+    __shared__ T su[U][kS2LY][kS2LX];                 // A.u
     __shared__ T sz[CORRECT ? kS2CzY : 1][CORRECT ? kS2CzX : 1];
 
     const int tx = threadIdx.x, ty = threadIdx.y;
@@ -98,9 +101,14 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
     const int k1 = min(d.nz, k0 + zc);
     const idx_t sxy = (idx_t)d.nx * d.ny;
     const bool xok = gx >= 0 && gx < d.nx;
-    // planes are >= k0 - 6 >= -6: slot arithmetic on p + 24 stays non-negative
-    auto xs = [](int p) { return (unsigned)(p + 24) & (kS2XR - 1); };
-    auto cs = [](int p) { return (unsigned)(p + 24) % (unsigned)kCR; };
+    const int t0 = k0 - kH, t1 = k1 + kH - 1;  // t1 - t0 >= 6
+    const int tl = t1 + ZH;                    // last step with a live half-sweep
+    // Colour of row 0 (tile row 2 ty) in plane t0; in plane t0 + m it is
+    // q ^ (m & 1), and the row of colour c in that plane is c ^ q ^ (m & 1).
+    const int q = (gx + gy0 + t0) & 1;
+    // the tile rows a colour can occupy: row lyA = 2 ty + q has colour 0 in
+    // the planes t0 + even, row lyB the other one
+    const int lyA = 2 * ty + q, lyB = 2 * ty + 1 - q;
 
     // CORRECT: as in k_rb_sweep, thread e < kS2CzX * kS2CzY stages one coarse
     // column, z-interpolated for the plane about to be committed.
@@ -135,116 +143,128 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
         T b, x;
         bool ok;
     };
-    // A thread owns the two vertically adjacent positions (tx, 2 ty) and
-    // (tx, 2 ty + 1), which always have opposite colours: every half-sweep
-    // updates exactly one of them per thread (no idle lanes).  Per position:
-    // w = omega / c (0 for a zero-centre row) and b of planes t .. t-4, f of
-    // planes t .. t-5, the held r / u of plane t (written to LDS at the next
-    // commit), the pending final value, and the prefetched planes.
-    // RES also keeps A.c of planes t .. t-4 and the partial residual of
-    // plane t-4 (rv).  To stay within the registers of 1024 threads, RES and
-    // CORRECT prefetch one plane ahead instead of two.
-    constexpr int kC = RES ? 5 : 1;
+    // To stay within the registers of 1024 threads, RES and CORRECT prefetch
+    // one plane ahead instead of two.
     constexpr int kPD = RES || CORRECT ? 1 : 2;
-    struct Pos {
-        T w[5], bb[5], f[6], cc[kC];
-        T hr, hu, px, rv;
-        Pre p[kPD];  // planes in flight: slot (t - t0) % kPD holds plane t
-        int ly, gy;
-        bool in4;  // in the written interior
-    };
-    Pos P[2];
+    static_assert(U % kPD == 0, "prefetch slots are static");
+    Pre pf[2][kPD];  // [row][(t - t0) mod kPD]
+    // Per colour c and slot: w = omega / c (0 for a zero-centre row), b and
+    // A.f of the position of colour c in that plane.
+    T W[2][U], BB[2][U], F[2][U];
+    // RES: A.c per row and slot, the partial residual of plane t-4 per row
+    T CC[RES ? 2 : 1][RES ? U : 1];
+    T rv[2] = {T(0), T(0)};
+    T hr[2], hu[2];                  // couplings of plane t, written after the first barrier
+    T px[2] = {T(0), T(0)};          // final x of the last finished plane, per row
+    T pxm[2] = {T(0), T(0)};         // and of the one before it (RES)
+    bool in4[2];                     // row in the written interior
+    int gyr[2];
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
-        Pos& Q = P[r];
-        Q.ly = 2 * ty + r;
-        Q.gy = gy0 + Q.ly;
-        Q.in4 = tx >= kH && tx < kS2LX - kH && Q.ly >= kH && Q.ly < kS2LY - kH && xok &&
-                Q.gy >= 0 && Q.gy < d.ny;
+        const int ly = 2 * ty + r;
+        gyr[r] = gy0 + ly;
+        in4[r] = tx >= kH && tx < kS2LX - kH && ly >= kH && ly < kS2LY - kH && xok &&
+                 gyr[r] >= 0 && gyr[r] < d.ny;
 #pragma unroll
-        for (int s = 0; s < 5; ++s) {
-            Q.w[s] = omega;
-            Q.bb[s] = T(0);
+        for (int s = 0; s < U; ++s) {
+            W[r][s] = omega;
+            BB[r][s] = T(0);
+            F[r][s] = T(0);
+            sx[s][ly][tx] = T(0);
+            sr[s][ly][tx] = T(0);
+            su[s][ly][tx] = T(0);
+            if constexpr (RES) CC[r][s] = T(0);
         }
-#pragma unroll
-        for (int s = 0; s < 6; ++s) Q.f[s] = T(0);
-#pragma unroll
-        for (int s = 0; s < kC; ++s) Q.cc[s] = T(0);
-        Q.hr = Q.hu = Q.px = Q.rv = T(0);
-#pragma unroll
-        for (int s = 0; s < kS2XR; ++s) sx[s][Q.ly][tx] = T(0);
-#pragma unroll
-        for (int s = 0; s < kCR; ++s) {
-            sr[s][Q.ly][tx] = T(0);
-            su[s][Q.ly][tx] = T(0);
-        }
+        hr[r] = hu[r] = T(0);
     }
-    auto fetch = [&](Pre& q, int gy, int t) __attribute__((always_inline)) {
+    // In-tile masks of the two rows a colour can occupy, per half-sweep ring.
+    auto inside = [&](int ly, int R) {
+        const int gy = gy0 + ly;
+        return tx >= R && tx < kS2LX - R && ly >= R && ly < kS2LY - R && xok && gy >= 0 &&
+               gy < d.ny;
+    };
+    bool inA[4], inB[4];
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+        inA[h] = inside(lyA, h + 1 - ZH);
+        inB[h] = inside(lyB, h + 1 - ZH);
+    }
+
+    auto fetch = [&](Pre& pr, int gy, int t) __attribute__((always_inline)) {
         const bool ok = xok && t >= 0 && t < d.nz && gy >= 0 && gy < d.ny;
         const idx_t c = ok ? gx + (idx_t)d.nx * gy + sxy * t : 0;
-        q.row = A[c];
-        q.b = b[c];
-        q.x = ZERO ? T(0) : xin[c];
-        q.ok = ok;
+        pr.row = A[c];
+        pr.b = b[c];
+        pr.x = ZERO ? T(0) : xin[c];
+        pr.ok = ok;
     };
-    auto commit = [&](Pos& Q, const Pre& q, int t) __attribute__((always_inline)) {
-#pragma unroll
-        for (int s = 4; s > 0; --s) {
-            Q.w[s] = Q.w[s - 1];
-            Q.bb[s] = Q.bb[s - 1];
-        }
-#pragma unroll
-        for (int s = 5; s > 0; --s) Q.f[s] = Q.f[s - 1];
-        const T c = q.ok ? q.row.c : T(1);
-        Q.w[0] = rb_coef<T>(omega, c);  // w = omega / c, 0 for a zero-centre row
-        if (RES) {
-#pragma unroll
-            for (int s = kC - 1; s > 0; --s) Q.cc[s] = Q.cc[s - 1];
-            Q.cc[0] = q.row.c;
-        }
-        Q.f[0] = q.ok ? q.row.f : T(0);
-        Q.bb[0] = q.ok ? q.b : T(0);
-        T xv = q.ok ? q.x : T(0);
-        if (CORRECT) {
-            const int ax0 = (tx >> 1) + (tx & 1);
-            const T wx0 = (tx & 1) ? T(0.75) : T(0.25), wx1 = (tx & 1) ? T(0.25) : T(0.75);
-            const int ay0 = (Q.ly >> 1) + (Q.ly & 1);
-            const T wy0 = (Q.ly & 1) ? T(0.75) : T(0.25), wy1 = (Q.ly & 1) ? T(0.25) : T(0.75);
-            const T lo = wx0 * sz[ay0][ax0] + wx1 * sz[ay0][ax0 + 1];
-            const T hi = wx0 * sz[ay0 + 1][ax0] + wx1 * sz[ay0 + 1][ax0 + 1];
-            const T corr = wy0 * lo + wy1 * hi;
-            xv += q.ok ? corr : T(0);
-        }
-        sx[xs(t)][Q.ly][tx] = xv;
-        Q.hr = q.ok ? q.row.r : T(0);
-        Q.hu = q.ok ? q.row.u : T(0);
-    };
-    // The couplings of plane t, after the first barrier of step t.
-    auto put_couplings = [&](int t) __attribute__((always_inline)) {
+    // Commit plane t at unrolled step u: x of both rows to LDS, the row values
+    // sorted by colour into the register rings, the couplings held.
+    auto commit = [&](auto uc) __attribute__((always_inline)) {
+        constexpr int u = decltype(uc)::value;
+        T wr[2], br[2], fr[2];
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
-            sr[cs(t)][P[r].ly][tx] = P[r].hr;
-            su[cs(t)][P[r].ly][tx] = P[r].hu;
+            const Pre& pr = pf[r][u % kPD];
+            const T c = pr.ok ? pr.row.c : T(1);
+            wr[r] = rb_coef<T>(omega, c);  // w = omega / c, 0 for a zero-centre row
+            br[r] = pr.ok ? pr.b : T(0);
+            fr[r] = pr.ok ? pr.row.f : T(0);
+            if constexpr (RES) CC[r][u] = pr.row.c;
+            T xv = pr.ok ? pr.x : T(0);
+            if (CORRECT) {
+                const int ly = 2 * ty + r;
+                const int ax0 = (tx >> 1) + (tx & 1);
+                const T wx0 = (tx & 1) ? T(0.75) : T(0.25), wx1 = (tx & 1) ? T(0.25) : T(0.75);
+                const int ay0 = (ly >> 1) + (ly & 1);
+                const T wy0 = (ly & 1) ? T(0.75) : T(0.25), wy1 = (ly & 1) ? T(0.25) : T(0.75);
+                const T lo = wx0 * sz[ay0][ax0] + wx1 * sz[ay0][ax0 + 1];
+                const T hi = wx0 * sz[ay0 + 1][ax0] + wx1 * sz[ay0 + 1][ax0 + 1];
+                const T corr = wy0 * lo + wy1 * hi;
+                xv += pr.ok ? corr : T(0);
+            }
+            sx[u][2 * ty + r][tx] = xv;
+            hr[r] = pr.ok ? pr.row.r : T(0);
+            hu[r] = pr.ok ? pr.row.u : T(0);
+        }
+        // row 0 has colour q ^ (u & 1) in this plane
+        const bool swap = (q ^ (u & 1)) != 0;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int r0 = c, r1 = 1 - c;  // the row of colour c without / with the swap
+            W[c][u] = swap ? wr[r1] : wr[r0];
+            BB[c][u] = swap ? br[r1] : br[r0];
+            F[c][u] = swap ? fr[r1] : fr[r0];
         }
     };
-    // Half-sweep H (1..4) on plane p = t - H: colour FIRST for odd H, the
-    // other for even H, on positions at least H cells inside the tile.  The
-    // row of this thread's position of that colour is 2 ty + sel.
-    auto half = [&](auto hc, int p) __attribute__((always_inline)) {
+    auto put_couplings = [&](auto uc) __attribute__((always_inline)) {
+        constexpr int u = decltype(uc)::value;
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            sr[u][2 * ty + r][tx] = hr[r];
+            su[u][2 * ty + r][tx] = hu[r];
+        }
+    };
+    // Half-sweep H (1..4) on plane p = t - H at unrolled step u: colour FIRST
+    // for odd H, the other for even H, on positions at least H cells inside
+    // the tile.  The position of colour cH in p is on row lyA or lyB, fixed
+    // by the parity of u - H.
+    auto half = [&](auto uc, auto hc) __attribute__((always_inline)) {
+        constexpr int u = decltype(uc)::value;
         constexpr int H = decltype(hc)::value;
-        const int colour = (H & 1) ? FIRST : 1 - FIRST;
-        const int sel = ((gx + gy0 + p) & 1) != colour;  // 0: row 2ty, 1: row 2ty + 1
-        const int ly = 2 * ty + sel;
-        const int gy = gy0 + ly;
-        constexpr int R = H - ZH;  // innermost ring this half-sweep is valid on
-        const bool in = tx >= R && tx < kS2LX - R && ly >= R && ly < kS2LY - R && xok && gy >= 0 &&
-                        gy < d.ny;
+        constexpr int cH = (H & 1) ? FIRST : 1 - FIRST;
+        constexpr int sp = (u - H + 2 * U) % U, sm = (u - H - 1 + 2 * U) % U,
+                      sn = (u - H + 1 + 2 * U) % U;
+        // row of colour cH in plane p: cH ^ q ^ ((u - H) & 1), i.e. lyA when
+        // cH ^ ((u - H) & 1) is 0
+        constexpr bool onA = (cH ^ ((u - H + 2 * U) & 1)) == 0;
+        const int ly = onA ? lyA : lyB;
+        const bool in = onA ? inA[H - 1] : inB[H - 1];
         if (in) {
-            const T w = sel ? P[1].w[H] : P[0].w[H];
-            const T bv = sel ? P[1].bb[H] : P[0].bb[H];
-            const T fm = sel ? P[1].f[H + 1] : P[0].f[H + 1];
-            const T fp = sel ? P[1].f[H] : P[0].f[H];
-            const int sp = xs(p), sm = xs(p - 1), sn = xs(p + 1), s1 = cs(p);
+            const T w = W[cH][sp];
+            const T bv = BB[cH][sp];
+            const T fp = F[cH][sp];       // A.f of this position in plane p
+            const T fm = F[1 - cH][sm];   // in plane p - 1, where it has the other colour
             const T xo = sx[sp][ly][tx];
             T xv;
             if (w == T(0)) {
@@ -256,58 +276,73 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
                 // (h1 runs right after the commit of plane p + 1 with no
                 // barrier between: the +z neighbour is this thread's own
                 // position, which it has just written itself)
-                const T s = rb_sum6<T>(sr[s1][ly][tx - 1], sx[sp][ly][tx - 1], sr[s1][ly][tx],
-                                       sx[sp][ly][tx + 1], su[s1][ly - 1][tx], sx[sp][ly - 1][tx],
-                                       su[s1][ly][tx], sx[sp][ly + 1][tx], fm, sx[sm][ly][tx], fp,
+                const T s = rb_sum6<T>(sr[sp][ly][tx - 1], sx[sp][ly][tx - 1], sr[sp][ly][tx],
+                                       sx[sp][ly][tx + 1], su[sp][ly - 1][tx], sx[sp][ly - 1][tx],
+                                       su[sp][ly][tx], sx[sp][ly + 1][tx], fm, sx[sm][ly][tx], fp,
                                        sx[sn][ly][tx]);
                 xv = rb_update<T>(omega, xo, w, bv, s);
             }
             sx[sp][ly][tx] = xv;
         }
     };
-    // After h4 on plane q: the final values of both positions (the one h4
-    // updated and the one h3 left) become the pending stores.
-    auto finish = [&](int q) __attribute__((always_inline)) {
-        const unsigned sq = xs(q);
+    // After h4 on plane t - 4: the final values of both rows (the one h4
+    // updated and the one h3 left); they are the pending stores when plane
+    // t - 4 is in the chunk, and the residual's -z values one step later.
+    auto finish = [&](auto uc) __attribute__((always_inline)) {
+        constexpr int u = decltype(uc)::value;
+        constexpr int s4 = (u - 4 + 2 * U) % U;
 #pragma unroll
-        for (int r = 0; r < 2; ++r) P[r].px = sx[sq][P[r].ly][tx];
+        for (int r = 0; r < 2; ++r) {
+            pxm[r] = px[r];
+            px[r] = sx[s4][2 * ty + r][tx];
+        }
     };
-    auto flush = [&](int q) __attribute__((always_inline)) {
+    auto flush = [&](int p) __attribute__((always_inline)) {
 #pragma unroll
         for (int r = 0; r < 2; ++r)
-            if (P[r].in4) xout[gx + (idx_t)d.nx * P[r].gy + sxy * q] = P[r].px;
+            if (in4[r]) xout[gx + (idx_t)d.nx * gyr[r] + sxy * p] = px[r];
     };
-    // RES at step t, after h4 on plane t - 4: the +z term of plane t - 5's
-    // residual, which is then stored, and plane t - 4's without that term, at
-    // both positions.  k_residual_restrict's operand order and masks (Jet's
-    // FdmBlas3::residual, src/jet/fdm_linear_system3.cpp:107-115); outside
-    // the grid x, A.f and the couplings in LDS are zero.
-    auto resid_finish = [&](int t) __attribute__((always_inline)) {
-        const unsigned sn = xs(t - 4);
+    // A.f of row r in the plane p with p - t0 = m (mod U), slot sk, from the
+    // colour-sorted ring: the row's colour there is r ^ q ^ (m & 1).
+    auto f_row = [&](int r, int m, int sk) __attribute__((always_inline)) {
+        const bool c1 = ((r ^ q ^ (m & 1)) & 1) != 0;
+        return c1 ? F[1][sk] : F[0][sk];
+    };
+    // RES at step t (unrolled u), after h4 on plane t - 4: the +z term of
+    // plane t - 5's residual, which is then stored, and plane t - 4's without
+    // that term, at both rows.  k_residual_restrict's operand order and masks
+    // (Jet's FdmBlas3::residual, src/jet/fdm_linear_system3.cpp:107-115);
+    // outside the grid x, A.f and the couplings in LDS are zero.  x of this
+    // thread's own positions in planes t - 4 and t - 5 come from registers.
+    auto resid_finish = [&](auto uc, int t) __attribute__((always_inline)) {
+        constexpr int u = decltype(uc)::value;
+        constexpr int s5 = (u - 5 + 2 * U) % U;
 #pragma unroll
         for (int r = 0; r < 2; ++r)
-            if (P[r].in4)
-                res[gx + (idx_t)d.nx * P[r].gy + sxy * (t - 5)] =
-                    P[r].rv - P[r].f[5] * sx[sn][P[r].ly][tx];
+            if (in4[r])
+                res[gx + (idx_t)d.nx * gyr[r] + sxy * (t - 5)] =
+                    rv[r] - f_row(r, u - 5, s5) * px[r];
     };
     // The -x / -y terms need no mask: past the grid edge the LDS coupling and
     // x are both +0, so the product is the +0 the masked form subtracts.
-    auto resid_start = [&](int t) __attribute__((always_inline)) {
-        const unsigned sq = xs(t - 4), sm = xs(t - 5), s1 = cs(t - 4);
-        const bool hr = gx + 1 < d.nx;
+    auto resid_start = [&](auto uc) __attribute__((always_inline)) {
+        constexpr int u = decltype(uc)::value;
+        constexpr int s4 = (u - 4 + 2 * U) % U, s5 = (u - 5 + 2 * U) % U;
+        const bool hr_ = gx + 1 < d.nx;
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
-            Pos& Q = P[r];
-            if (!Q.in4) continue;  // the written interior: ring >= 5, inside the grid
-            const int ly = Q.ly;
-            const bool hu = Q.gy + 1 < d.ny;
-            T rv = Q.bb[4] - Q.cc[kC - 1] * sx[sq][ly][tx];
-            rv -= sr[s1][ly][tx - 1] * sx[sq][ly][tx - 1];
-            rv -= hr ? sr[s1][ly][tx] * sx[sq][ly][tx + 1] : T(0);
-            rv -= su[s1][ly - 1][tx] * sx[sq][ly - 1][tx];
-            rv -= hu ? su[s1][ly][tx] * sx[sq][ly + 1][tx] : T(0);
-            rv -= Q.f[5] * sx[sm][ly][tx];
-            Q.rv = rv;
+            if (!in4[r]) continue;  // the written interior: ring >= 5, inside the grid
+            const int ly = 2 * ty + r;
+            const bool hu_ = gyr[r] + 1 < d.ny;
+            const bool c1 = ((r ^ q ^ ((u - 4) & 1)) & 1) != 0;
+            const T bq = c1 ? BB[1][s4] : BB[0][s4];
+            T v = bq - CC[r][s4] * px[r];
+            v -= sr[s4][ly][tx - 1] * sx[s4][ly][tx - 1];
+            v -= hr_ ? sr[s4][ly][tx] * sx[s4][ly][tx + 1] : T(0);
+            v -= su[s4][ly - 1][tx] * sx[s4][ly - 1][tx];
+            v -= hu_ ? su[s4][ly][tx] * sx[s4][ly + 1][tx] : T(0);
+            v -= f_row(r, u - 5, s5) * pxm[r];
+            rv[r] = v;
         }
     };
 
@@ -322,64 +357,68 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
     // RES, h4 covers k0-1 .. k1, the planes the residual reads).  Half-sweep
     // H runs on plane t - H at step t, so the last live one is at step
     // t1 + ZH: with ZH, one step past the last commit.
-    // Two planes of loads are in flight ahead of the committed one; the loop
-    // is unrolled twice so the prefetch slot of every step is static.
-    const int t0 = k0 - kH, t1 = k1 + kH - 1;  // t1 - t0 >= 6
-    const int tl = t1 + ZH;                    // last step with a live half-sweep
     auto live = [&](int p, int H) {
         return p >= max(k0 - kH + H - ZH, 0) && p < min(k1 + kH - H + ZH, d.nz);
     };
     if (CORRECT) zfetch(t0);
-    // plane by plane, in the order the loop issues them
 #pragma unroll
-    for (int r = 0; r < 2; ++r) fetch(P[r].p[0], P[r].gy, t0);
+    for (int r = 0; r < 2; ++r) fetch(pf[r][0], gyr[r], t0);
     if (kPD == 2) {
 #pragma unroll
-        for (int r = 0; r < 2; ++r) fetch(P[r].p[kPD - 1], P[r].gy, t0 + 1);
+        for (int r = 0; r < 2; ++r) fetch(pf[r][kPD - 1], gyr[r], t0 + 1);
     }
     if (CORRECT) zstore(t0);
     __syncthreads();
 
-    // The body is unrolled kPD times (the prefetch slot of every step is
-    // static) and has no exit: a step past t1 commits a re-read plane into
-    // the slot of a plane that is no longer read, and runs only the
-    // half-sweeps still live (none past tl).
-    const int tend = t0 + (tl - t0 + kPD) / kPD * kPD;
+    // One step; t is uniform, so every thread leaves the loop together.
     bool pending = false;  // uniform: the pending x stores hold plane t - 5
-    for (int t2 = t0; t2 < tend; t2 += kPD) {
+    auto step = [&](auto uc, int t) __attribute__((always_inline)) {
+        constexpr int u = decltype(uc)::value;
+        commit(uc);
+        if (CORRECT && t + 1 <= t1) zfetch(t + 1);
+        // unconditional (past the end it re-reads plane t1, unused) so the
+        // compiler can count the loads in flight and wait only for plane t
 #pragma unroll
-        for (int u = 0; u < kPD; ++u) {
-            const int t = t2 + u;
-#pragma unroll
-            for (int r = 0; r < 2; ++r) commit(P[r], P[r].p[u % kPD], t);
-            if (CORRECT && t + 1 <= t1) zfetch(t + 1);
-            // unconditional (past the end it re-reads plane t1, unused) so the
-            // compiler can count the loads in flight and wait only for plane t
-#pragma unroll
-            for (int r = 0; r < 2; ++r) fetch(P[r].p[u % kPD], P[r].gy, min(t + kPD, t1));
-            if (pending) flush(t - 5);
-            // h1 on plane t - 1 needs no barrier after the commit: its in-plane
-            // neighbours and couplings (plane t - 1) are a step old, and its
-            // +z neighbour is this thread's own committed value
-            if (live(t - 1, 1)) half(H1{}, t - 1);
-            __syncthreads();
-            put_couplings(t);
-            if (live(t - 2, 2)) half(H2{}, t - 2);
-            __syncthreads();
-            if (live(t - 3, 3)) half(H3{}, t - 3);
-            __syncthreads();
-            pending = t - 4 >= k0 && t - 4 < k1;
-            if (live(t - 4, 4)) half(H4{}, t - 4);
-            if (CORRECT && t + 1 <= t1) zstore(t + 1);
-            __syncthreads();
-            if (pending) finish(t - 4);
-            if (RES) {
-                if (t - 5 >= k0 && t - 5 < k1) resid_finish(t);
-                if (pending) resid_start(t);
-            }
+        for (int r = 0; r < 2; ++r) fetch(pf[r][u % kPD], gyr[r], min(t + kPD, t1));
+        if (pending) flush(t - 5);
+        // h1 on plane t - 1 needs no barrier after the commit: its in-plane
+        // neighbours and couplings (plane t - 1) are a step old, and its
+        // +z neighbour is this thread's own committed value
+        if (live(t - 1, 1)) half(uc, H1{});
+        __syncthreads();
+        put_couplings(uc);
+        if (live(t - 2, 2)) half(uc, H2{});
+        __syncthreads();
+        if (live(t - 3, 3)) half(uc, H3{});
+        __syncthreads();
+        pending = t - 4 >= k0 && t - 4 < k1;
+        if (live(t - 4, 4)) half(uc, H4{});
+        if (CORRECT && t + 1 <= t1) zstore(t + 1);
+        __syncthreads();
+        finish(uc);
+        if constexpr (RES) {
+            if (t - 5 >= k0 && t - 5 < k1) resid_finish(uc, t);
+            if (pending) resid_start(uc);
         }
+    };
+    int t = t0;
+    while (true) {
+        // unrolled U times: the step index u is a constant in each copy
+        if (t > tl) break;
+        step(std::integral_constant<int, 0>{}, t);
+        if (++t > tl) break;
+        step(std::integral_constant<int, 1>{}, t);
+        if (++t > tl) break;
+        step(std::integral_constant<int, 2>{}, t);
+        if (++t > tl) break;
+        step(std::integral_constant<int, 3>{}, t);
+        if (++t > tl) break;
+        step(std::integral_constant<int, 4>{}, t);
+        if (++t > tl) break;
+        step(std::integral_constant<int, 5>{}, t);
+        ++t;
     }
-    if (pending) flush(tend - 5);
+    if (pending) flush(t - 5);
 }
 
 }  // namespace jg
