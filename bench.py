@@ -224,6 +224,8 @@ def main():
     out["dam_break_mgpcg_128"] = small["dam128"]["gpu"]
     batch = batch_leg(jg, args)
     out["batch_64_uneven"] = batch["gpu"]
+    torus = torus_leg(jg, args)
+    out["torus_fractional_512"] = torus["gpu"]
     if not args.no_cpu:
         out.update(cpu_leg(jg, n, L, args, res))
         out["constrain_velocity"].update(bc_cpu_leg(bc, n))
@@ -234,6 +236,7 @@ def main():
         out["closed_sphere_cg_64"].update(c1)
         out["dam_break_mgpcg_128"].update(c2)
         out["batch_64_uneven"].update(batch_cpu_leg(batch))
+        out["torus_fractional_512"].update(torus_cpu_leg(torus))
     print(json.dumps(out))
     solver.close()
 
@@ -946,6 +949,96 @@ def batch_leg(jg, args):
     sample = {i: (sizes[i], seeds[i], levels[i], xs["fp64"][i], xs["mixed"][i])
               for i in first.values()}
     return {"gpu": gpu, "sample": sample, "omega": args.omega}
+
+
+TORUS_N, TORUS_LEVELS, TORUS_SEED = 512, 7, 11
+TORUS_MG = dict(restrict_iter=2, correct_iter=2, coarsest_iter=20, final_iter=2, sor=1.3)
+
+
+def torus_leg(jg, args):
+    """The variational workload at 512^3: GridFractionalSinglePhasePressureSolver3's
+    weights and system on all 7 levels built on the device from the torus
+    pool (torus R = 150, r = 50 cells, liquid below 0.35 n), solved by MGPCG
+    in FP64 and mixed precision to ||b - A x|| <= 1e-8 ||b||."""
+    import torch
+
+    n, L = TORUS_N, TORUS_LEVELS
+    d = jg.scene_desc(jg.JG_SCENE_TORUS_POOL, n, seed=TORUS_SEED)
+    f64 = torch.float64
+    s = jg.Solver(0)
+    s.set_levels([(n >> l,) * 3 for l in range(L)])
+    u = torch.empty((n, n, n + 1), dtype=f64, device="cuda")
+    v = torch.empty((n, n + 1, n), dtype=f64, device="cuda")
+    w = torch.empty((n + 1, n, n), dtype=f64, device="cuda")
+    bs, fs = (torch.empty((n, n, n), dtype=f64, device="cuda") for _ in range(2))
+    vs = torch.empty((n + 1, n + 1, n + 1), dtype=f64, device="cuda")
+    s.generate_scene(d, u, v, w, bs, fs, vs)
+    s.fr_build(u, v, w, vs, fs, 1.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.fr_build(u, v, w, vs, fs, 1.0)
+    torch.cuda.synchronize()
+    build_ms = (time.perf_counter() - t0) * 1e3
+    fluid = float(((fs < 0) & (bs >= 0)).double().mean().item())
+    del u, v, w, bs, fs, vs
+    torch.cuda.empty_cache()
+    p = jg.mg_params(max_levels=L, **TORUS_MG)
+    gpu = {"cells": n ** 3, "levels": L, "fluid_fraction": round(fluid, 4),
+           "build_ms": round(build_ms, 3)}
+    xs = {}
+    for name, prec in (("fp64", jg.JG_FP64), ("mixed", jg.JG_MIXED)):
+        o = jg.gpu_options(precision=prec, stop_rule=jg.JG_STOP_RELATIVE, rel_tol=1e-8)
+        s.mgpcg(300, p, o)  # tuning
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        info = s.mgpcg(300, p, o)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3
+        gpu[name] = {"ms": round(ms, 3), "iterations": info.iterations,
+                     "rel_residual": info.rel_residual}
+        if not args.no_cpu:
+            xs[name] = s.download_x()
+        log(f"bench: torus 512^3 {name} {ms:.2f} ms, {info.iterations} it")
+    gpu["speedup_mixed_vs_fp64"] = round(gpu["fp64"]["ms"] / gpu["mixed"]["ms"], 3)
+    s.close()
+    torch.cuda.empty_cache()
+    return {"gpu": gpu, "x": xs}
+
+
+def torus_cpu_leg(torus):
+    """The oracle's weights, system and FP64 MGPCG (the GPU's algorithm, to
+    1e-11) of the same generated scene on the host cores, and the error of
+    both GPU solutions against it."""
+    import numpy as np
+
+    import jetgpu as jg
+    from oracle.jet_oracle import MgParams, Oracle
+
+    o = Oracle()
+    L = TORUS_LEVELS
+    t0 = time.perf_counter()
+    sc = jg.generate_scene_host(jg.scene_desc(jg.JG_SCENE_TORUS_POOL, TORUS_N, seed=TORUS_SEED),
+                                vertex=True)
+    _, sys_ = o.fr_build_mg(sc["u"], sc["v"], sc["w"], (1.0,) * 3, (0.0,) * 3, sc["vertex"],
+                            sc["fsdf"], L)
+    del sc
+    build_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ok, it, rel = o.mgpcg_clean_solve(sys_, MgParams.make(max_levels=L, **TORUS_MG), 300, 1e-11)
+    solve_ms = (time.perf_counter() - t0) * 1e3
+    ref = sys_.x[0]
+    nr = np.linalg.norm(ref)
+    errs = {k: float(np.linalg.norm(x - ref) / nr) for k, x in torus["x"].items()}
+    log(f"bench: torus oracle build {build_s:.1f} s, solve {solve_ms:.0f} ms ({it} it), "
+        f"errors {errs}")
+    return {"err_vs_oracle": errs, "oracle_rel_residual": rel,
+            "cpu_oracle": {"ms": round(solve_ms, 1), "iterations": it, "rel_tol": 1e-11,
+                           "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                           "kind": "port",
+                           "sample": "the oracle's FP64 MGPCG (the GPU's algorithm) on the same "
+                                     "system to 1e-11 relative residual; its build "
+                                     f"({build_s:.1f} s: numpy scene, C weights and system) is "
+                                     "not timed"}}
 
 
 def batch_cpu_leg(batch):

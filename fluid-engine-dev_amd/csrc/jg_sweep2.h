@@ -149,10 +149,11 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
     static_assert(U % kPD == 0, "prefetch slots are static");
     Pre pf[2][kPD];  // [row][(t - t0) mod kPD]
     // Per colour c and slot: w = omega / c (0 for a zero-centre row), b and
-    // A.f of the position of colour c in that plane.
+    // A.f of the position of colour c in that plane.  RES keeps A.c there
+    // instead of w (the residual needs it; w is formed as the half-sweep
+    // reads it), to stay within the registers of 1024 threads.
     T W[2][U], BB[2][U], F[2][U];
-    // RES: A.c per row and slot, the partial residual of plane t-4 per row
-    T CC[RES ? 2 : 1][RES ? U : 1];
+    // RES: the partial residual of plane t-4 per row
     T rv[2] = {T(0), T(0)};
     T hr[2], hu[2];                  // couplings of plane t, written after the first barrier
     T px[2] = {T(0), T(0)};          // final x of the last finished plane, per row
@@ -173,7 +174,6 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
             sx[s][ly][tx] = T(0);
             sr[s][ly][tx] = T(0);
             su[s][ly][tx] = T(0);
-            if constexpr (RES) CC[r][s] = T(0);
         }
         hr[r] = hu[r] = T(0);
     }
@@ -207,10 +207,10 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
         for (int r = 0; r < 2; ++r) {
             const Pre& pr = pf[r][u % kPD];
             const T c = pr.ok ? pr.row.c : T(1);
-            wr[r] = rb_coef<T>(omega, c);  // w = omega / c, 0 for a zero-centre row
+            // w = omega / c, 0 for a zero-centre row (RES: c itself)
+            wr[r] = RES ? c : rb_coef<T>(omega, c);
             br[r] = pr.ok ? pr.b : T(0);
             fr[r] = pr.ok ? pr.row.f : T(0);
-            if constexpr (RES) CC[r][u] = pr.row.c;
             T xv = pr.ok ? pr.x : T(0);
             if (CORRECT) {
                 const int ly = 2 * ty + r;
@@ -261,7 +261,7 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
         const int ly = onA ? lyA : lyB;
         const bool in = onA ? inA[H - 1] : inB[H - 1];
         if (in) {
-            const T w = W[cH][sp];
+            const T w = RES ? rb_coef<T>(omega, W[cH][sp]) : W[cH][sp];
             const T bv = BB[cH][sp];
             const T fp = F[cH][sp];       // A.f of this position in plane p
             const T fm = F[1 - cH][sm];   // in plane p - 1, where it has the other colour
@@ -336,7 +336,8 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
             const bool hu_ = gyr[r] + 1 < d.ny;
             const bool c1 = ((r ^ q ^ ((u - 4) & 1)) & 1) != 0;
             const T bq = c1 ? BB[1][s4] : BB[0][s4];
-            T v = bq - CC[r][s4] * px[r];
+            const T cq = c1 ? W[1][s4] : W[0][s4];  // A.c (RES keeps it in W)
+            T v = bq - cq * px[r];
             v -= sr[s4][ly][tx - 1] * sx[s4][ly][tx - 1];
             v -= hr_ ? sr[s4][ly][tx] * sx[s4][ly][tx + 1] : T(0);
             v -= su[s4][ly - 1][tx] * sx[s4][ly - 1][tx];
