@@ -52,6 +52,10 @@ struct Level {
     // tuner measured fastest; and the z-chunk of k_rb_sweep2.
     int pair64[3] = {0, 0, 0}, pair32[3] = {0, 0, 0};
     int zc2_32[3] = {0, 0, 0};
+    // z-chunk of the zero-start two-sweep launch that also writes the
+    // residual (its 5-cell halo changes the tile and block counts); 0: use
+    // the zero-start kind's
+    int zc2res_32 = 0;
 };
 
 enum { kPairNone = 0, kPairMarch = 1, kPairBrick = 2 };
@@ -73,6 +77,7 @@ struct Prec<double> {
     static int& pair(Level& l, int kind) { return l.pair64[kind]; }
     static int pair(const Level& l, int kind) { return l.pair64[kind]; }
     static int zc2(const Level&, int) { return 0; }  // no FP64 z-marching two-sweep kernel
+    static int zc2res(const Level&) { return 0; }
 };
 template <>
 struct Prec<float> {
@@ -86,6 +91,9 @@ struct Prec<float> {
     static int& pair(Level& l, int kind) { return l.pair32[kind]; }
     static int pair(const Level& l, int kind) { return l.pair32[kind]; }
     static int zc2(const Level& l, int kind) { return l.zc2_32[kind]; }
+    static int zc2res(const Level& l) {
+        return l.zc2res_32 > 0 ? l.zc2res_32 : l.zc2_32[kSweepZero];
+    }
 };
 
 struct Geo {

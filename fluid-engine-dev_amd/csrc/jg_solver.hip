@@ -525,10 +525,13 @@ int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, 
             const int kind = in ? kSweepPlain : kSweepZero;
             if (pre_plan[m] == kLaunchBrick)
                 launch_brick2<T>(s, A, B(l), in, buf(m), d, 0, nullptr, c.omega, ctl);
-            else if (pre_plan[m] == kLaunchMarch)
+            else if (pre_plan[m] == kLaunchMarch) {
+                const bool res = m + 1 == pre_plan.size();
                 JG_TRY(launch_sweep2<T>(s, A, B(l), in, buf(m), d, 0, nullptr,
-                                        m + 1 == pre_plan.size() ? Prec<T>::t(lv) : nullptr,
-                                        c.omega, ctl, Prec<T>::zc2(lv, kind)));
+                                        res ? Prec<T>::t(lv) : nullptr, c.omega, ctl,
+                                        res && kind == kSweepZero ? Prec<T>::zc2res(lv)
+                                                                  : Prec<T>::zc2(lv, kind)));
+            }
             else
                 launch_sweep<T>(s, A, B(l), in, buf(m), d, 0, nullptr, nullptr, nullptr, c.omega, ctl,
                                 Prec<T>::zc(lv, kind), Prec<T>::sh(lv, kind));
@@ -722,6 +725,28 @@ int autotune_sweeps(jg_solver* s) {
                 if (best2 < best) {
                     best = best2;
                     Prec<T>::pair(lv, kind) = kPairMarch;
+                }
+                if (kind == kSweepZero) {
+                    // the residual-writing form of the zero-start launch (the
+                    // last pre-smoothing launch of a level), residual into y
+                    float best3 = 1e30f;
+                    for (int zc : cands2) {
+                        if (zc > d.nz && zc != cands2[0]) break;
+                        const float ms = time3([&]() {
+                            launch_sweep2<T>(s, Prec<T>::A(lv), Prec<T>::t(lv), nullptr,
+                                             Prec<T>::x(lv), d, 0, nullptr, Prec<T>::y(lv), T(1),
+                                             nullptr, zc);
+                        });
+                        if (ms < 0.f) return JG_ERR_HIP;
+                        if (tune_log())
+                            std::fprintf(stderr,
+                                         "jetgpu tune: %dx%dx%d f32 two-sweep with residual zc %d: %.1f us\n",
+                                         d.nx, d.ny, d.nz, zc, ms * 1e3f / 3);
+                        if (ms < best3) {
+                            best3 = ms;
+                            lv.zc2res_32 = zc;
+                        }
+                    }
                 }
             }
             if (d.n() <= kBrickMaxCells) {
@@ -1699,7 +1724,8 @@ void level0_launch(jg_solver* s, const Vecs<T>& v, bool post, T omega) {
         launch_brick2<T>(s, A, v.r, xin, v.z, l0.d, post ? 1 : 0, coarse, omega, s->ctl);
     else if (mode == kLaunchMarch)
         launch_sweep2<T>(s, A, v.r, xin, v.z, l0.d, post ? 1 : 0, coarse,
-                         post ? nullptr : Prec<T>::t(l0), omega, s->ctl, Prec<T>::zc2(l0, kind));
+                         post ? nullptr : Prec<T>::t(l0), omega, s->ctl,
+                         post ? Prec<T>::zc2(l0, kind) : Prec<T>::zc2res(l0));
     else
         launch_sweep<T>(s, A, v.r, xin, v.z, l0.d, post ? 1 : 0, coarse, nullptr, nullptr, omega,
                         s->ctl, Prec<T>::zc(l0, kind), Prec<T>::sh(l0, kind));
@@ -2177,13 +2203,15 @@ int batch_tune(jg_batch* B) {
                 same = t->lv[l].d.nx == s->lv[l].d.nx && t->lv[l].d.ny == s->lv[l].d.ny &&
                        t->lv[l].d.nz == s->lv[l].d.nz;
             if (!same) continue;
-            for (size_t l = 0; l < s->lv.size(); ++l)
+            for (size_t l = 0; l < s->lv.size(); ++l) {
                 for (int kind = 0; kind < kNumSweepKinds; ++kind) {
                     Prec<T>::zc(s->lv[l], kind) = Prec<T>::zc(t->lv[l], kind);
                     Prec<T>::sh(s->lv[l], kind) = Prec<T>::sh(t->lv[l], kind);
                     Prec<T>::pair(s->lv[l], kind) = Prec<T>::pair(t->lv[l], kind);
                     s->lv[l].zc2_32[kind] = t->lv[l].zc2_32[kind];
                 }
+                s->lv[l].zc2res_32 = t->lv[l].zc2res_32;
+            }
             copied = true;
         }
         if (!copied) {
