@@ -86,10 +86,13 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
     constexpr int kH = G::H;
     constexpr int U = kS2U;
     if (ctl && ctl->done) return;
-    // slot of plane p: (p - t0) mod U, a constant at every unrolled step
-    __shared__ __align__(16) T sx[U][kS2LY][kS2LX];  // x
-    __shared__ T sr[U][kS2LY][kS2LX];                 // A.r
-    __shared__ T su[U][kS2LY][kS2LX];                 // A.u
+    // slot of plane p: (p - t0) mod U, a constant at every unrolled step.
+    // Tile row ly is array row ly + 1: a guard row above and below each
+    // slot keeps the neighbour reads of every lane inside the arrays (a
+    // half-sweep computes on all lanes and keeps the result where valid).
+    __shared__ __align__(16) T sx[U][kS2LY + 2][kS2LX];  // x
+    __shared__ T sr[U][kS2LY + 2][kS2LX];                 // A.r
+    __shared__ T su[U][kS2LY + 2][kS2LX];                 // A.u
     __shared__ T sz[CORRECT ? kS2CzY : 1][CORRECT ? kS2CzX : 1];
 
     const int tx = threadIdx.x, ty = threadIdx.y;
@@ -143,9 +146,9 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
         T b, x;
         bool ok;
     };
-    // To stay within the registers of 1024 threads, RES and CORRECT prefetch
-    // one plane ahead instead of two.
-    constexpr int kPD = RES || CORRECT ? 1 : 2;
+    // To stay within the registers of 1024 threads, RES prefetches one plane
+    // ahead instead of two.
+    constexpr int kPD = RES ? 1 : 2;
     static_assert(U % kPD == 0, "prefetch slots are static");
     Pre pf[2][kPD];  // [row][(t - t0) mod kPD]
     // Per colour c and slot: w = omega / c (0 for a zero-centre row), b and
@@ -171,9 +174,9 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
             W[r][s] = omega;
             BB[r][s] = T(0);
             F[r][s] = T(0);
-            sx[s][ly][tx] = T(0);
-            sr[s][ly][tx] = T(0);
-            su[s][ly][tx] = T(0);
+            sx[s][ly + 1][tx] = T(0);
+            sr[s][ly + 1][tx] = T(0);
+            su[s][ly + 1][tx] = T(0);
         }
         hr[r] = hu[r] = T(0);
     }
@@ -223,7 +226,7 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
                 const T corr = wy0 * lo + wy1 * hi;
                 xv += pr.ok ? corr : T(0);
             }
-            sx[u][2 * ty + r][tx] = xv;
+            sx[u][2 * ty + r + 1][tx] = xv;
             hr[r] = pr.ok ? pr.row.r : T(0);
             hu[r] = pr.ok ? pr.row.u : T(0);
         }
@@ -241,8 +244,8 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
         constexpr int u = decltype(uc)::value;
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
-            sr[u][2 * ty + r][tx] = hr[r];
-            su[u][2 * ty + r][tx] = hu[r];
+            sr[u][2 * ty + r + 1][tx] = hr[r];
+            su[u][2 * ty + r + 1][tx] = hu[r];
         }
     };
     // Half-sweep H (1..4) on plane p = t - H at unrolled step u: colour FIRST
@@ -260,30 +263,31 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
         constexpr bool onA = (cH ^ ((u - H + 2 * U) & 1)) == 0;
         const int ly = onA ? lyA : lyB;
         const bool in = onA ? inA[H - 1] : inB[H - 1];
-        if (in) {
-            const T w = RES ? rb_coef<T>(omega, W[cH][sp]) : W[cH][sp];
-            const T bv = BB[cH][sp];
-            const T fp = F[cH][sp];       // A.f of this position in plane p
-            const T fm = F[1 - cH][sm];   // in plane p - 1, where it has the other colour
-            const T xo = sx[sp][ly][tx];
-            T xv;
-            if (w == T(0)) {
-                xv = T(0);  // zero-centre row
-            } else if (ZERO && !CORRECT && H == 1) {
-                // x is zero everywhere before the first half-sweep
-                xv = w * (bv - T(0));
-            } else {
-                // (h1 runs right after the commit of plane p + 1 with no
-                // barrier between: the +z neighbour is this thread's own
-                // position, which it has just written itself)
-                const T s = rb_sum6<T>(sr[sp][ly][tx - 1], sx[sp][ly][tx - 1], sr[sp][ly][tx],
-                                       sx[sp][ly][tx + 1], su[sp][ly - 1][tx], sx[sp][ly - 1][tx],
-                                       su[sp][ly][tx], sx[sp][ly + 1][tx], fm, sx[sm][ly][tx], fp,
-                                       sx[sn][ly][tx]);
-                xv = rb_update<T>(omega, xo, w, bv, s);
-            }
-            sx[sp][ly][tx] = xv;
+        // every lane computes; a lane outside the valid ring writes its old
+        // value back (a half-sweep reads only the other colour, so nothing
+        // reads this position in this phase)
+        const T w = RES ? rb_coef<T>(omega, W[cH][sp]) : W[cH][sp];
+        const T bv = BB[cH][sp];
+        const T fp = F[cH][sp];       // A.f of this position in plane p
+        const T fm = F[1 - cH][sm];   // in plane p - 1, where it has the other colour
+        const int row = ly + 1;       // array row of tile row ly
+        const T xo = sx[sp][row][tx];
+        T xv;
+        if (ZERO && !CORRECT && H == 1) {
+            // x is zero everywhere before the first half-sweep
+            xv = w * (bv - T(0));
+        } else {
+            // (h1 runs right after the commit of plane p + 1 with no
+            // barrier between: the +z neighbour is this thread's own
+            // position, which it has just written itself)
+            const T s = rb_sum6<T>(sr[sp][row][tx - 1], sx[sp][row][tx - 1], sr[sp][row][tx],
+                                   sx[sp][row][tx + 1], su[sp][row - 1][tx], sx[sp][row - 1][tx],
+                                   su[sp][row][tx], sx[sp][row + 1][tx], fm, sx[sm][row][tx], fp,
+                                   sx[sn][row][tx]);
+            xv = rb_update<T>(omega, xo, w, bv, s);
         }
+        xv = w == T(0) ? T(0) : xv;  // zero-centre row
+        sx[sp][row][tx] = in ? xv : xo;
     };
     // After h4 on plane t - 4: the final values of both rows (the one h4
     // updated and the one h3 left); they are the pending stores when plane
@@ -294,7 +298,7 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
             pxm[r] = px[r];
-            px[r] = sx[s4][2 * ty + r][tx];
+            px[r] = sx[s4][2 * ty + r + 1][tx];
         }
     };
     auto flush = [&](int p) __attribute__((always_inline)) {
@@ -338,10 +342,10 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
             const T bq = c1 ? BB[1][s4] : BB[0][s4];
             const T cq = c1 ? W[1][s4] : W[0][s4];  // A.c (RES keeps it in W)
             T v = bq - cq * px[r];
-            v -= sr[s4][ly][tx - 1] * sx[s4][ly][tx - 1];
-            v -= hr_ ? sr[s4][ly][tx] * sx[s4][ly][tx + 1] : T(0);
-            v -= su[s4][ly - 1][tx] * sx[s4][ly - 1][tx];
-            v -= hu_ ? su[s4][ly][tx] * sx[s4][ly + 1][tx] : T(0);
+            v -= sr[s4][ly + 1][tx - 1] * sx[s4][ly + 1][tx - 1];
+            v -= hr_ ? sr[s4][ly + 1][tx] * sx[s4][ly + 1][tx + 1] : T(0);
+            v -= su[s4][ly - 1 + 1][tx] * sx[s4][ly - 1 + 1][tx];
+            v -= hu_ ? su[s4][ly + 1][tx] * sx[s4][ly + 1 + 1][tx] : T(0);
             v -= f_row(r, u - 5, s5) * pxm[r];
             rv[r] = v;
         }
