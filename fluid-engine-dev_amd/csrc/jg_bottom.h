@@ -246,6 +246,14 @@ __global__ __launch_bounds__(kBotThreads) void k_bottom(BotLevels<T> bl, const T
                                                         int post, int cfwd, T omega,
                                                         const Ctl* __restrict__ ctl) {
 #pragma clang fp contract(off)
+    // one workgroup per system of a batch group; AL(l): its rows of level l
+    const int sys = blockIdx.x;
+    if (sys > 0) {
+        b0 += sys * bl.d[0].n();
+        x0 += sys * bl.d[0].n();
+        if (ctl) ctl += sys * kSysCtl;
+    }
+    auto AL = [&](int l) { return bl.A[l] + sys * (bl.d[0].n() >> (3 * l)); };
     if (ctl && ctl->done) return;
     __shared__ T sv[3][BotCfg<T>::Pool];    // x, b, residual of every level
     __shared__ int sq[BotCfg<T>::Pool];     // packed cell coordinates (L >= 2)
@@ -277,9 +285,9 @@ __global__ __launch_bounds__(kBotThreads) void k_bottom(BotLevels<T> bl, const T
         bv[m] = c < n0 ? b0[c] : T(0);
     }
     BotRegs<T, MC, PAIRS> top;
-    top.load(bl.A[0], bl.d[0], omega);
+    top.load(AL(0), bl.d[0], omega);
     BotRegs<T, 1, false> bot;  // coarsest level when it is not the top
-    if (L >= 2) bot.load(bl.A[L - 1], D(L - 1), omega);
+    if (L >= 2) bot.load(AL(L - 1), D(L - 1), omega);
 
     for (int c = tid; c < off(L); c += kBotThreads) sv[0][c] = T(0);
 #pragma unroll
@@ -296,7 +304,7 @@ __global__ __launch_bounds__(kBotThreads) void k_bottom(BotLevels<T> bl, const T
         }
     }
     for (int l = 1; l + 1 < L; ++l)  // rows of the levels between (<= 512 cells each)
-        if (tid < (n0 >> (3 * l))) RA(l)[tid] = bl.A[l][tid];
+        if (tid < (n0 >> (3 * l))) RA(l)[tid] = AL(l)[tid];
     __syncthreads();
 
     auto relax = [&](int l, int colour) __attribute__((always_inline)) {
@@ -311,7 +319,7 @@ __global__ __launch_bounds__(kBotThreads) void k_bottom(BotLevels<T> bl, const T
             relax(l, 0);
             relax(l, 1);
         }
-        if (l == 0) top.residual(bl.A[0], X(0), B(0), Rv(0));
+        if (l == 0) top.residual(AL(0), X(0), B(0), Rv(0));
         else bot_residual_lds<T>(RA(l), Q(l), X(l), B(l), Rv(l), D(l));
         __syncthreads();
         bot_restrict<T>(Rv(l), B(l + 1), Q(l + 1), D(l + 1));

@@ -35,6 +35,19 @@ __global__ __launch_bounds__(kBkThreads, sizeof(T) == 4 ? 2 : 1) void k_brick_sw
     const Ctl* __restrict__ ctl) {
 #pragma clang fp contract(off)
     constexpr bool kWeighted = sizeof(T) == 4;
+    // the system of a batch group: (nz + 7) / 8 z-blocks per system
+    const int nbz = (d.nz + kBkB - 1) / kBkB;
+    const int sys = (int)blockIdx.z / nbz;
+    const int bzl = (int)blockIdx.z - sys * nbz;
+    if (sys > 0) {
+        const idx_t n = d.n();
+        A += sys * n;
+        b += sys * n;
+        if (!ZERO) xin += sys * n;
+        xout += sys * n;
+        if (CORRECT) coarse += sys * (n / 8);
+        if (ctl) ctl += sys * kSysCtl;
+    }
     if (ctl && ctl->done) return;
     __shared__ T sx[kBkR][kBkR][kBkR];
     __shared__ T sr[kBkR][kBkR][kBkR];
@@ -48,7 +61,7 @@ __global__ __launch_bounds__(kBkThreads, sizeof(T) == 4 ? 2 : 1) void k_brick_sw
     const int li = t % kBkR, lj = (t / kBkR) % kBkR, lk0 = (t / (kBkR * kBkR)) * 4;
     const int gi = (int)blockIdx.x * kBkB - kBkH + li;
     const int gj = (int)blockIdx.y * kBkB - kBkH + lj;
-    const int gk0 = (int)blockIdx.z * kBkB - kBkH + lk0;
+    const int gk0 = bzl * kBkB - kBkH + lk0;
     const bool xyok = gi >= 0 && gi < d.nx && gj >= 0 && gj < d.ny;
     const int ring_xy = min(min(li, kBkR - 1 - li), min(lj, kBkR - 1 - lj));
 
@@ -59,7 +72,7 @@ __global__ __launch_bounds__(kBkThreads, sizeof(T) == 4 ? 2 : 1) void k_brick_sw
     T wx0 = 0, wx1 = 0, wy0 = 0, wy1 = 0;
     const int ib = (int)blockIdx.x * (kBkB / 2) - kBkH / 2 - 1;
     const int jb = (int)blockIdx.y * (kBkB / 2) - kBkH / 2 - 1;
-    const int kb = (int)blockIdx.z * (kBkB / 2) - kBkH / 2 - 1;
+    const int kb = bzl * (kBkB / 2) - kBkH / 2 - 1;
     if (CORRECT) {
         // stage: thread t < 1000 loads coarse cell (ib, jb, kb) + (t % 10, t / 10 % 10, t / 100),
         // each index clamped to the coarse grid

@@ -62,7 +62,7 @@ __global__ __launch_bounds__(NT) void k_csr_residual(CsrDev m, const double* __r
         r[i] = rv;
         if (DOT) acc += rv * rv;
     }
-    if (DOT) block_partial<NT>(acc, partial);
+    if (DOT) block_partial<NT>(acc, partial, (idx_t)blockIdx.x);
 }
 
 // The search direction and its product in one pass: d = beta d + z (Jet's
@@ -93,7 +93,7 @@ __global__ __launch_bounds__(NT) void k_csr_dir_spmv(CsrDev m, const double* __r
         q[i] = sum;
         acc += di * sum;
     }
-    block_partial<NT>(acc, partial);
+    block_partial<NT>(acc, partial, (idx_t)blockIdx.x);
 }
 
 // ELL copy of a compressed system whose rows have at most kEllMaxW entries
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(NT) void k_ell_residual(EllDev m, const double* __r
         r[i] = rv;
         if (DOT) acc += rv * rv;
     }
-    if (DOT) block_partial<NT>(acc, partial);
+    if (DOT) block_partial<NT>(acc, partial, (idx_t)blockIdx.x);
 }
 
 // k_csr_dir_spmv on the ELL copy.
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(NT) void k_ell_dir_spmv(EllDev m, const double* __r
         q[i] = sum;
         acc += di * sum;
     }
-    block_partial<NT>(acc, partial);
+    block_partial<NT>(acc, partial, (idx_t)blockIdx.x);
 }
 
 }  // namespace jg

@@ -56,6 +56,32 @@ constexpr int NT = 256;  // threads per block of the streaming kernels
 // each, so this bounds a plane (a 2-D grid) at about 2^24 cells.
 constexpr int kMaxPartials = 65536;
 
+// Several systems of one hierarchy solved by the same launches (a batch
+// group) keep their arrays back to back: per system every level's rows and
+// vectors at a stride of that level's cell count, kSysSlots reduction slots
+// of kMaxPartials partials, kSysCtl control blocks and kSysScal device
+// scalars.  A launch covers the group by repeating its grid: the z extent of
+// a stencil kernel (nsb z-blocks per system), the y extent of an elementwise
+// kernel, the block count of a one-block kernel.  A single system is the
+// group of one.
+constexpr int kSysSlots = 4;
+constexpr idx_t kSysPart = (idx_t)kSysSlots * kMaxPartials;
+constexpr int kSysCtl = 2;
+constexpr int kSysScal = 16;
+
+// System of a stencil block whose grid has nsb z-blocks per system: bz
+// becomes the block's z index within its system, part the index of its
+// reduction partial within the system's slot (the block's linear index in a
+// one-system launch).
+__device__ __forceinline__ int block_system(int bx, int by, int& bz, int nsb, idx_t& part) {
+    const int sys = bz / nsb;
+    bz -= sys * nsb;
+    part = gridDim.z == (unsigned)nsb
+               ? blockIdx.x + (idx_t)gridDim.x * (blockIdx.y + (idx_t)gridDim.y * blockIdx.z)
+               : bx + (idx_t)gridDim.x * (by + (idx_t)gridDim.y * bz);
+    return sys;
+}
+
 // ---------------------------------------------------------------- reduce
 
 template <int NT>
@@ -74,25 +100,22 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
     return s;  // valid in thread 0 only
 }
 
-// Writes this block's partial sum of v (one slot per block, deterministic
-// order) for a later finalize kernel.
+// Writes this block's partial sum of v at index b of the partials (one
+// slot per block, deterministic order) for a later finalize kernel.
 template <int NT>
-__device__ __forceinline__ void block_partial(double v, double* partials) {
+__device__ __forceinline__ void block_partial(double v, double* partials, idx_t b) {
     __shared__ double sh[NT / kWave];
     double s = block_sum<NT>(v, sh);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-        const idx_t b = blockIdx.x + (idx_t)gridDim.x * (blockIdx.y + (idx_t)gridDim.y * blockIdx.z);
-        partials[b] = s;
-    }
+    if (threadIdx.x == 0 && threadIdx.y == 0) partials[b] = s;
 }
 
 template <int NT>
-__device__ __forceinline__ void block_partial2(double v0, double v1, double* p0, double* p1) {
+__device__ __forceinline__ void block_partial2(double v0, double v1, double* p0, double* p1,
+                                               idx_t b) {
     __shared__ double sh[2][NT / kWave];
     double s0 = block_sum<NT>(v0, sh[0]);
     double s1 = block_sum<NT>(v1, sh[1]);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
-        const idx_t b = blockIdx.x + (idx_t)gridDim.x * (blockIdx.y + (idx_t)gridDim.y * blockIdx.z);
         p0[b] = s0;
         p1[b] = s1;
     }

@@ -148,6 +148,13 @@ inline int nblocks(const Geo& g) { return (int)(g.grid.x * g.grid.y * g.grid.z);
 
 struct jg_solver {
     int device = 0;
+    // Systems held back to back (a batch group of one hierarchy): every
+    // array below holds nsys of them (jg_device.h, kSys*), and the V-cycle
+    // and PCG launches cover all of them.  A view is one member of a group:
+    // its pointers address that member's part of the group's arrays, it owns
+    // no array and shares the group's stream.
+    int nsys = 1;
+    bool view = false;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t evp[2] = {nullptr, nullptr};  // stop-flag polls (two in flight)
@@ -193,7 +200,7 @@ struct jg_solver {
     bool f32_valid = false;
     std::vector<void*> allocs;
 
-    static constexpr int kSlots = 4;
+    static constexpr int kSlots = jg::kSysSlots;
     double* slot(int i) { return part + (size_t)i * jg::kMaxPartials; }
 
     ~jg_solver() {
@@ -203,7 +210,7 @@ struct jg_solver {
         if (ev1) (void)hipEventDestroy(ev1);
         for (hipEvent_t e : evp)
             if (e) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamDestroy(stream);
+        if (stream && !view) (void)hipStreamDestroy(stream);
     }
     void release() {
         for (void* p : allocs) (void)hipFree(p);

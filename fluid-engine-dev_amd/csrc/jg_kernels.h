@@ -124,9 +124,19 @@ __global__ __launch_bounds__(NT) void k_spmv(const Row<TA>* __restrict__ A,
                                              const TX* __restrict__ x, TX* __restrict__ y,
                                              Dims d, int zc, double* __restrict__ partial,
                                              const Ctl* __restrict__ ctl) {
-    if (ctl && ctl->done) return;
     int bx, by, bz;
     xcd_tile(bx, by, bz);
+    idx_t part;
+    const int sys = block_system(bx, by, bz, (d.nz + zc - 1) / zc, part);
+    if (sys > 0) {
+        const idx_t n = d.n();
+        A += sys * n;
+        x += sys * n;
+        y += sys * n;
+        if (DOT) partial += sys * kSysPart;
+        if (ctl) ctl += sys * kSysCtl;
+    }
+    if (ctl && ctl->done) return;
     const int i = bx * blockDim.x + threadIdx.x;
     const int j = by * blockDim.y + threadIdx.y;
     const int k0 = bz * zc;
@@ -142,7 +152,7 @@ __global__ __launch_bounds__(NT) void k_spmv(const Row<TA>* __restrict__ A,
                                 if (DOT) acc += (double)p.v[0][0] * (double)s;
                             });
     }
-    if (DOT) block_partial<NT>(acc, partial);
+    if (DOT) block_partial<NT>(acc, partial, part);
 }
 
 // Fused CG direction update and SpMV:  d = z + beta*d_old (Jet's
@@ -155,10 +165,22 @@ __global__ __launch_bounds__(NT) void k_dir_spmv(const Row<TA>* __restrict__ A,
                                                  TX* __restrict__ dnew, TX* __restrict__ q,
                                                  Dims d, int zc, double* __restrict__ partial,
                                                  const Ctl* __restrict__ ctl) {
-    if (ctl->done) return;
-    const double beta = ctl->beta;
     int bx, by, bz;
     xcd_tile(bx, by, bz);
+    idx_t part;
+    const int sys = block_system(bx, by, bz, (d.nz + zc - 1) / zc, part);
+    if (sys > 0) {
+        const idx_t n = d.n();
+        A += sys * n;
+        z += sys * n;
+        dold += sys * n;
+        dnew += sys * n;
+        q += sys * n;
+        partial += sys * kSysPart;
+        ctl += sys * kSysCtl;
+    }
+    if (ctl->done) return;
+    const double beta = ctl->beta;
     const int i = bx * blockDim.x + threadIdx.x;
     const int j = by * blockDim.y + threadIdx.y;
     const int k0 = bz * zc;
@@ -176,7 +198,7 @@ __global__ __launch_bounds__(NT) void k_dir_spmv(const Row<TA>* __restrict__ A,
                                 acc += (double)dc * (double)s;
                             });
     }
-    block_partial<NT>(acc, partial);
+    block_partial<NT>(acc, partial, part);
 }
 
 // r = b - A x (FdmBlas3::residual, src/jet/fdm_linear_system3.cpp:98-116);
@@ -192,10 +214,21 @@ __global__ __launch_bounds__(NT) void k_residual(const Row<TA>* __restrict__ A,
                                                  Dims d, int zc, double* __restrict__ partial,
                                                  const Ctl* __restrict__ ctl) {
 #pragma clang fp contract(off)
-    if (REFRESH && (ctl->done || !ctl->refresh)) return;
-    if (GATED && ctl && ctl->done) return;
     int bx, by, bz;
     xcd_tile(bx, by, bz);
+    idx_t part;
+    const int sys = block_system(bx, by, bz, (d.nz + zc - 1) / zc, part);
+    if (sys > 0) {
+        const idx_t n = d.n();
+        A += sys * n;
+        x += sys * n;
+        b += sys * n;
+        r += sys * n;
+        if (DOT) partial += sys * kSysPart;
+        if (ctl) ctl += sys * kSysCtl;
+    }
+    if (REFRESH && (ctl->done || !ctl->refresh)) return;
+    if (GATED && ctl && ctl->done) return;
     const int i = bx * blockDim.x + threadIdx.x;
     const int j = by * blockDim.y + threadIdx.y;
     const int k0 = bz * zc;
@@ -222,7 +255,7 @@ __global__ __launch_bounds__(NT) void k_residual(const Row<TA>* __restrict__ A,
                                 if (DOT) acc += (double)rv * (double)rv;
                             });
     }
-    if (DOT) block_partial<NT>(acc, partial);
+    if (DOT) block_partial<NT>(acc, partial, part);
 }
 
 // ---------------------------------------------------- elementwise updates
@@ -235,6 +268,12 @@ __global__ __launch_bounds__(NT) void k_update_xr(TX* __restrict__ x, TX* __rest
                                                   const TX* __restrict__ q, idx_t n,
                                                   const Ctl* __restrict__ ctl) {
 #pragma clang fp contract(off)
+    const int sys = blockIdx.y;  // the system of a batch group
+    x += sys * n;
+    r += sys * n;
+    dd += sys * n;
+    q += sys * n;
+    ctl += sys * kSysCtl;
     if (ctl->done) return;
     const double alpha = ctl->alpha;
     const bool upd_r = !ctl->refresh;
@@ -248,11 +287,16 @@ template <typename TA, typename TB>
 __global__ __launch_bounds__(NT) void k_dot(const TA* __restrict__ a, const TB* __restrict__ b,
                                             idx_t n, double* __restrict__ partial,
                                             const Ctl* __restrict__ ctl) {
+    const int sys = blockIdx.y;  // the system of a batch group
+    a += sys * n;
+    b += sys * n;
+    partial += sys * kSysPart;
+    if (ctl) ctl += sys * kSysCtl;
     if (ctl && ctl->done) return;
     double acc = 0.0;
     for (idx_t c = blockIdx.x * (idx_t)NT + threadIdx.x; c < n; c += (idx_t)gridDim.x * NT)
         acc += (double)a[c] * (double)b[c];
-    block_partial<NT>(acc, partial);
+    block_partial<NT>(acc, partial, (idx_t)blockIdx.x);
 }
 
 // r.z and r.r partials (slots p_rz, p_rr) in one pass over r and z.
@@ -261,6 +305,12 @@ __global__ __launch_bounds__(NT) void k_dot_rz_rr(const T* __restrict__ r, const
                                                   idx_t n, double* __restrict__ p_rz,
                                                   double* __restrict__ p_rr,
                                                   const Ctl* __restrict__ ctl) {
+    const int sys = blockIdx.y;  // the system of a batch group
+    r += sys * n;
+    z += sys * n;
+    p_rz += sys * kSysPart;
+    p_rr += sys * kSysPart;
+    if (ctl) ctl += sys * kSysCtl;
     if (ctl && ctl->done) return;
     double a0 = 0.0, a1 = 0.0;
     for (idx_t c = blockIdx.x * (idx_t)NT + threadIdx.x; c < n; c += (idx_t)gridDim.x * NT) {
@@ -268,7 +318,7 @@ __global__ __launch_bounds__(NT) void k_dot_rz_rr(const T* __restrict__ r, const
         a0 += rv * (double)z[c];
         a1 += rv * rv;
     }
-    block_partial2<NT>(a0, a1, p_rz, p_rr);
+    block_partial2<NT>(a0, a1, p_rz, p_rr, (idx_t)blockIdx.x);
 }
 
 template <typename TX>
@@ -282,6 +332,7 @@ __global__ __launch_bounds__(NT) void k_axpy(double a, const TX* __restrict__ x,
 
 template <typename TX>
 __global__ __launch_bounds__(NT) void k_fill(TX* __restrict__ x, TX v, idx_t n) {
+    x += blockIdx.y * n;  // the system of a batch group
     for (idx_t c = blockIdx.x * (idx_t)NT + threadIdx.x; c < n; c += (idx_t)gridDim.x * NT)
         x[c] = v;
 }
@@ -289,6 +340,8 @@ __global__ __launch_bounds__(NT) void k_fill(TX* __restrict__ x, TX v, idx_t n) 
 template <typename TS, typename TD>
 __global__ __launch_bounds__(NT) void k_convert(const TS* __restrict__ s, TD* __restrict__ dst,
                                                 idx_t n) {
+    s += blockIdx.y * n;  // the system of a batch group
+    dst += blockIdx.y * n;
     for (idx_t c = blockIdx.x * (idx_t)NT + threadIdx.x; c < n; c += (idx_t)gridDim.x * NT)
         dst[c] = (TD)s[c];
 }
@@ -296,6 +349,8 @@ __global__ __launch_bounds__(NT) void k_convert(const TS* __restrict__ s, TD* __
 // FP32 shadow of an FP64 matrix (mixed-precision preconditioner).
 __global__ __launch_bounds__(NT) void k_rows_to_f32(const Row<double>* __restrict__ s,
                                                     Row<float>* __restrict__ dst, idx_t n) {
+    s += blockIdx.y * n;  // the system of a batch group
+    dst += blockIdx.y * n;
     for (idx_t c = blockIdx.x * (idx_t)NT + threadIdx.x; c < n; c += (idx_t)gridDim.x * NT) {
         const Row<double> r = s[c];
         Row<float> o;
@@ -310,6 +365,8 @@ __global__ __launch_bounds__(NT) void k_rows_to_f32(const Row<double>* __restric
 // Mixed precision: x64 += e32 (iterative-refinement correction).
 __global__ __launch_bounds__(NT) void k_add_correction(double* __restrict__ x,
                                                        const float* __restrict__ e, idx_t n) {
+    x += blockIdx.y * n;  // the system of a batch group
+    e += blockIdx.y * n;
     for (idx_t c = blockIdx.x * (idx_t)NT + threadIdx.x; c < n; c += (idx_t)gridDim.x * NT)
         x[c] += (double)e[c];
 }
@@ -390,10 +447,16 @@ __global__ __launch_bounds__(NT) void k_restrict(const TX* __restrict__ fine,
                                                  TX* __restrict__ coarse, Dims cd,
                                                  const Ctl* __restrict__ ctl) {
 #pragma clang fp contract(off)
+    const int sys = blockIdx.z / cd.nz;  // the system of a batch group (one plane per z-block)
+    const int k = blockIdx.z - sys * cd.nz;
+    if (sys > 0) {
+        fine += 8 * sys * cd.n();
+        coarse += sys * cd.n();
+        if (ctl) ctl += sys * kSysCtl;
+    }
     if (ctl && ctl->done) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
     if (i >= cd.nx || j >= cd.ny) return;
     const int fnx = 2 * cd.nx, fny = 2 * cd.ny;
     const TX kern[4] = {TX(0.125), TX(0.375), TX(0.375), TX(0.125)};
@@ -506,10 +569,16 @@ __global__ __launch_bounds__(NT) void k_correct(const TX* __restrict__ coarse,
                                                 TX* __restrict__ fine, Dims fd,
                                                 const Ctl* __restrict__ ctl) {
 #pragma clang fp contract(off)
+    const int sys = blockIdx.z / fd.nz;  // the system of a batch group (one plane per z-block)
+    const int k = blockIdx.z - sys * fd.nz;
+    if (sys > 0) {
+        fine += sys * fd.n();
+        coarse += sys * (fd.n() / 8);
+        if (ctl) ctl += sys * kSysCtl;
+    }
     if (ctl && ctl->done) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
     if (i >= fd.nx || j >= fd.ny) return;
     const int cnx = fd.nx / 2, cny = fd.ny / 2;
     const int kclamp = min(fd.ny, fd.nz);
@@ -629,6 +698,9 @@ __global__ __launch_bounds__(NT) void k_correct2(const TX* __restrict__ coarse,
 // rule), beta = 0 so the first fused direction update gives d = z.
 __global__ __launch_bounds__(kFinThreads) void k_fin_init(Ctl* ctl, const double* p_rz,
                                                           const double* p_rr, int n) {
+    ctl += blockIdx.x * kSysCtl;  // one block per system of a batch group
+    p_rz += blockIdx.x * kSysPart;
+    p_rr += blockIdx.x * kSysPart;
     __shared__ double sh[2 * kFinThreads / kWave];
     const double* const pp[2] = {p_rz, p_rr};
     double rs[2];
@@ -652,6 +724,8 @@ __global__ __launch_bounds__(kFinThreads) void k_fin_init(Ctl* ctl, const double
 
 // alpha = sigma / d.q ; Jet's refresh rule for this iteration.
 __global__ __launch_bounds__(kFinThreads) void k_fin_alpha(Ctl* ctl, const double* p_dq, int n) {
+    ctl += blockIdx.x * kSysCtl;  // one block per system of a batch group
+    p_dq += blockIdx.x * kSysPart;
     if (ctl->done) return;
     __shared__ double sh[kFinThreads / kWave];
     const double dq = reduce_partials<kFinThreads>(p_dq, n, sh);
@@ -671,6 +745,9 @@ __global__ __launch_bounds__(kFinThreads) void k_fin_alpha(Ctl* ctl, const doubl
 // r.z, trigger on growth, beta, iteration count and the stop test.
 __global__ __launch_bounds__(kFinThreads) void k_fin_sigma(Ctl* ctl, const double* p_rz,
                                                            const double* p_rr, int n) {
+    ctl += blockIdx.x * kSysCtl;  // one block per system of a batch group
+    p_rz += blockIdx.x * kSysPart;
+    p_rr += blockIdx.x * kSysPart;
     if (ctl->done) return;
     __shared__ double sh[2 * kFinThreads / kWave];
     const double* const pp[2] = {p_rz, p_rr};
@@ -697,6 +774,8 @@ __global__ __launch_bounds__(kFinThreads) void k_fin_sigma(Ctl* ctl, const doubl
 
 // Generic: result = sum of partials (used by the BLAS entry points).
 __global__ __launch_bounds__(kFinThreads) void k_fin_sum(const double* p, int n, double* out) {
+    p += blockIdx.x * kSysPart;  // one block per system of a batch group
+    out += blockIdx.x * kSysScal;
     __shared__ double sh[kFinThreads / kWave];
     const double s = reduce_partials<kFinThreads>(p, n, sh);
     if (threadIdx.x == 0) *out = s;

@@ -101,6 +101,15 @@ int sweep_zc(const Dims& d, int tuned) {
     return (int)std::min<long>(zc, d.nz);
 }
 
+// Grids of a launch over every system of s (one, or a batch group): the
+// stencil kernels repeat their z extent, the elementwise kernels take the
+// system from the grid's y index, the one-block kernels one block each.
+inline dim3 grid_sys(dim3 g, const jg_solver* s) {
+    g.z *= (unsigned)s->nsys;
+    return g;
+}
+inline dim3 grid_1d(int nb, const jg_solver* s) { return dim3((unsigned)nb, (unsigned)s->nsys, 1); }
+
 // One fused red-black sweep (see jg_sweep.h).  first: colour updated first
 // (0 = red, forward; 1 = black, backward).  xin == nullptr: x starts at zero.
 // coarse != nullptr: add the prolongation of coarse to x on load.
@@ -150,6 +159,8 @@ int launch_sweep(jg_solver* s, const Row<T>* A, const T* b, const T* xin, T* xou
     while (p_bx && zc < d.nz && cols * ((d.nz + zc - 1) / zc) > kMaxPartials) ++zc;
     dim3 grid((d.nx + kSwIX - 1) / kSwIX, (d.ny + kSwIY - 1) / kSwIY, (d.nz + zc - 1) / zc);
     if ((long)grid.x * grid.y * grid.z > kMaxPartials) p_bx = p_bb = nullptr;  // caller uses k_dot
+    const int nb = (int)(grid.x * grid.y * grid.z);  // per system
+    grid = grid_sys(grid, s);
     const int code = (first ? 8 : 0) | (xin ? 0 : 4) | (coarse ? 2 : 0) | (p_bx ? 1 : 0);
     switch (shape) {
         case 1:
@@ -168,7 +179,7 @@ int launch_sweep(jg_solver* s, const Row<T>* A, const T* b, const T* xin, T* xou
             launch_sweep_shape<T, 2, 1>(s, code, grid, A, b, xin, xout, d, zc, omega, coarse, p_bx,
                                         p_bb, ctl);
     }
-    return p_bx ? (int)(grid.x * grid.y * grid.z) : 0;
+    return p_bx ? nb : 0;
 }
 
 // Two sweeps in one launch (jg_sweep2.h, FP32 only).  res != nullptr: also
@@ -190,7 +201,7 @@ int launch_sweep2(jg_solver* s, const Row<T>* A, const T* b, const T* xin, T* xo
                            : (z ? S2Tile<false, true>::IX : S2Tile<false>::IX);
         const int iy = res ? (z ? S2Tile<true, true>::IY : S2Tile<true>::IY)
                            : (z ? S2Tile<false, true>::IY : S2Tile<false>::IY);
-        dim3 grid((d.nx + ix - 1) / ix, (d.ny + iy - 1) / iy, (d.nz + zc - 1) / zc);
+        const dim3 grid = grid_sys(dim3((d.nx + ix - 1) / ix, (d.ny + iy - 1) / iy, (d.nz + zc - 1) / zc), s);
         const dim3 block(kS2LX, kS2TY, 1);
         const int code = (first ? 8 : 0) | (xin ? 0 : 4) | (coarse ? 2 : 0) | (res ? 1 : 0);
 #define JG_SWEEP2_CASE(C, F, Z, K, R)                                                     \
@@ -222,7 +233,8 @@ int launch_sweep2(jg_solver* s, const Row<T>* A, const T* b, const T* xin, T* xo
 template <typename T>
 void launch_brick2(jg_solver* s, const Row<T>* A, const T* b, const T* xin, T* xout, const Dims& d,
                    int first, const T* coarse, T omega, const Ctl* ctl) {
-    const dim3 grid((d.nx + kBkB - 1) / kBkB, (d.ny + kBkB - 1) / kBkB, (d.nz + kBkB - 1) / kBkB);
+    const dim3 grid = grid_sys(
+        dim3((d.nx + kBkB - 1) / kBkB, (d.ny + kBkB - 1) / kBkB, (d.nz + kBkB - 1) / kBkB), s);
     const int code = (first ? 4 : 0) | (xin ? 0 : 2) | (coarse ? 1 : 0);
 #define JG_BRICK_CASE(C, F, Z, K)                                                               \
     case C:                                                                                     \
@@ -462,6 +474,7 @@ template <typename T>
 int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, int* dot_blocks) {
     const size_t L = s->lv.size();
     *dot_blocks = 0;
+    if ((c.jacobi || s->twod) && s->nsys > 1) return JG_ERR_UNSUPPORTED;  // batches are 3-D red-black
     if (c.jacobi) return vcycle_jacobi<T>(s, r0, z0, c, ctl);
     if (s->twod) return vcycle2d<T>(s, r0, z0, c, ctl);
     auto B = [&](size_t l) -> const T* { return l == 0 ? r0 : Prec<T>::b(s->lv[l]); };
@@ -489,11 +502,11 @@ int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, 
             for (size_t m = l; m < L; ++m) bl.A[m - l] = Prec<T>::A(s->lv[m]);
             const int post_top = (int)(l == 0 ? c.post0 : c.post);
             if (d.nx % 2 == 0)
-                k_bottom<T, true><<<1, kBotThreads, 0, s->stream>>>(bl, B(l), X(l), (int)c.pre,
+                k_bottom<T, true><<<s->nsys, kBotThreads, 0, s->stream>>>(bl, B(l), X(l), (int)c.pre,
                                                                     post_top, (int)c.post, fwd,
                                                                     c.omega, ctl);
             else
-                k_bottom<T, false><<<1, kBotThreads, 0, s->stream>>>(bl, B(l), X(l), (int)c.pre,
+                k_bottom<T, false><<<s->nsys, kBotThreads, 0, s->stream>>>(bl, B(l), X(l), (int)c.pre,
                                                                      post_top, (int)c.post, fwd,
                                                                      c.omega, ctl);
             JG_TRY(check_launch(s, "bottom"));
@@ -545,16 +558,17 @@ int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, 
             // register footprint halves its occupancy in FP64, and the pair
             // measures faster)
             Geo gf = geo_stencil(d);
-            k_residual<T, T, T, false, false, true><<<gf.grid, gf.block, 0, s->stream>>>(
+            k_residual<T, T, T, false, false, true><<<grid_sys(gf.grid, s), gf.block, 0, s->stream>>>(
                 A, in, B(l), Prec<T>::t(lv), d, gf.zc, nullptr, ctl);
-            k_restrict<T><<<gc.grid, gc.block, 0, s->stream>>>(Prec<T>::t(lv), Prec<T>::b(s->lv[l + 1]),
+            k_restrict<T><<<grid_sys(gc.grid, s), gc.block, 0, s->stream>>>(Prec<T>::t(lv), Prec<T>::b(s->lv[l + 1]),
                                                                cd, ctl);
         } else if (in) {
             const bool from_t = pre_plan.back() == kLaunchMarch;  // that launch wrote the residual into t
             // restricting a stored residual is one load per cell: twice the
             // blocks (shorter chunks) keep more loads in flight
             const int zc = from_t ? rr_zc(cd, 2048) : rr_zc(cd);
-            dim3 grid((cd.nx + kRrCX - 1) / kRrCX, (cd.ny + kRrCY - 1) / kRrCY, (cd.nz + zc - 1) / zc);
+            const dim3 grid = grid_sys(
+                dim3((cd.nx + kRrCX - 1) / kRrCX, (cd.ny + kRrCY - 1) / kRrCY, (cd.nz + zc - 1) / zc), s);
             if (from_t)
                 k_residual_restrict<T, true><<<grid, dim3(kRrLX, kRrTY, 1), 0, s->stream>>>(
                     nullptr, nullptr, Prec<T>::t(lv), Prec<T>::b(s->lv[l + 1]), d, cd, zc, ctl);
@@ -562,7 +576,7 @@ int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, 
                 k_residual_restrict<T><<<grid, dim3(kRrLX, kRrTY, 1), 0, s->stream>>>(
                     A, in, B(l), Prec<T>::b(s->lv[l + 1]), d, cd, zc, ctl);
         } else {
-            k_restrict<T><<<gc.grid, gc.block, 0, s->stream>>>(B(l), Prec<T>::b(s->lv[l + 1]), cd,
+            k_restrict<T><<<grid_sys(gc.grid, s), gc.block, 0, s->stream>>>(B(l), Prec<T>::b(s->lv[l + 1]), cd,
                                                                ctl);
         }
         JG_TRY(check_launch(s, "vcycle down"));
@@ -580,9 +594,9 @@ int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, 
         const T* coarse = X(l + 1);
         if (post == 0) {
             // No post-smoothing: x = pre-smoothed x (already in X) + P e.
-            if (!pre_out[l]) k_fill<T><<<nblocks_1d(d.n()), NT, 0, s->stream>>>(X(l), T(0), d.n());
+            if (!pre_out[l]) k_fill<T><<<grid_1d(nblocks_1d(d.n()), s), NT, 0, s->stream>>>(X(l), T(0), d.n());
             Geo gf = geo_planes(d);
-            k_correct<T><<<gf.grid, gf.block, 0, s->stream>>>(coarse, X(l), d, ctl);
+            k_correct<T><<<grid_sys(gf.grid, s), gf.block, 0, s->stream>>>(coarse, X(l), d, ctl);
         } else {
             const T* in = pre_out[l];
             for (size_t q = 0; q < post_plan.size(); ++q) {
@@ -631,15 +645,15 @@ int autotune_sweeps(jg_solver* s) {
             continue;
         }
         const idx_t n = d.n();
-        k_fill<T><<<nblocks_1d(n), NT, 0, s->stream>>>(Prec<T>::t(lv), T(1), n);
-        k_fill<T><<<nblocks_1d(n), NT, 0, s->stream>>>(Prec<T>::y(lv), T(0.5), n);
+        k_fill<T><<<grid_1d(nblocks_1d(n), s), NT, 0, s->stream>>>(Prec<T>::t(lv), T(1), n);
+        k_fill<T><<<grid_1d(nblocks_1d(n), s), NT, 0, s->stream>>>(Prec<T>::y(lv), T(0.5), n);
         // the correction kind reads the next level's x as its coarse vector
         // (a 2-D hierarchy corrects in a separate launch: no correction kind)
         const T* coarse = nullptr;
         if (l + 1 < L && !s->twod) {
             const idx_t cn = s->lv[l + 1].d.n();
             coarse = Prec<T>::x(s->lv[l + 1]);
-            k_fill<T><<<nblocks_1d(cn), NT, 0, s->stream>>>(Prec<T>::x(s->lv[l + 1]), T(0.25), cn);
+            k_fill<T><<<grid_1d(nblocks_1d(cn), s), NT, 0, s->stream>>>(Prec<T>::x(s->lv[l + 1]), T(0.25), cn);
         }
         const int cands[] = {2, 4, 8, 12, 16, 24, 32, 48, 64};
         float tbest[kNumSweepKinds] = {1e30f, 1e30f, 1e30f};
@@ -797,9 +811,9 @@ int pcg_precondition(jg_solver* s, const Pcg<T>& v, const Ctl* c) {
     }
     if (nb == 0) {
         if (v.mg)
-            k_dot_rz_rr<T><<<nb1, NT, 0, s->stream>>>(v.r, v.z, n, s->slot(0), s->slot(1), c);
+            k_dot_rz_rr<T><<<grid_1d(nb1, s), NT, 0, s->stream>>>(v.r, v.z, n, s->slot(0), s->slot(1), c);
         else
-            k_dot<T, T><<<nb1, NT, 0, s->stream>>>(v.r, v.z, n, s->slot(0), c);
+            k_dot<T, T><<<grid_1d(nb1, s), NT, 0, s->stream>>>(v.r, v.z, n, s->slot(0), c);
         nb = nb1;
     }
     return nb;
@@ -813,12 +827,14 @@ int pcg_init(jg_solver* s, const Pcg<T>& v) {
     const int nb1 = nblocks_1d(n);
     // r = b - A x with x = 0 (every caller zeroes x first, as Jet's solvers
     // do): b - A 0 is b bit for bit, so a copy
-    k_convert<T, T><<<nb1, NT, 0, s->stream>>>(v.b, v.r, n);
-    k_fill<T><<<nb1, NT, 0, s->stream>>>(v.dA, T(0), n);
-    k_fill<T><<<nb1, NT, 0, s->stream>>>(v.dB, T(0), n);
-    const int nb = pcg_precondition<T>(s, v, nullptr);
+    k_convert<T, T><<<grid_1d(nb1, s), NT, 0, s->stream>>>(v.b, v.r, n);
+    k_fill<T><<<grid_1d(nb1, s), NT, 0, s->stream>>>(v.dA, T(0), n);
+    k_fill<T><<<grid_1d(nb1, s), NT, 0, s->stream>>>(v.dB, T(0), n);
+    // (a batch group gates the preconditioner by the control blocks, which
+    // say which members take part; a single system's is not set up yet)
+    const int nb = pcg_precondition<T>(s, v, s->nsys > 1 ? v.ctl : nullptr);
     if (nb < 0) return -nb;
-    k_fin_init<<<1, kFinThreads, 0, s->stream>>>(v.ctl, s->slot(0), v.mg ? s->slot(1) : s->slot(0), nb);
+    k_fin_init<<<s->nsys, kFinThreads, 0, s->stream>>>(v.ctl, s->slot(0), v.mg ? s->slot(1) : s->slot(0), nb);
     return check_launch(s, "pcg init");
 }
 
@@ -857,15 +873,15 @@ int pcg_iteration(jg_solver* s, const Pcg<T>& v, bool odd) {
         }
     }
     Geo g = geo_stencil(v.d);
-    k_dir_spmv<T, T><<<g.grid, g.block, 0, s->stream>>>(v.A, v.z, dold, dnew, v.q, v.d, g.zc,
-                                                        s->slot(2), v.ctl);
-    k_fin_alpha<<<1, kFinThreads, 0, s->stream>>>(v.ctl, s->slot(2), nblocks(g));
-    k_update_xr<T><<<nb1, NT, 0, s->stream>>>(v.x, v.r, dnew, v.q, n, v.ctl);
-    k_residual<T, T, T, false, true><<<g.grid, g.block, 0, s->stream>>>(v.A, v.x, v.b, v.r, v.d, g.zc,
-                                                                       nullptr, v.ctl);
+    k_dir_spmv<T, T><<<grid_sys(g.grid, s), g.block, 0, s->stream>>>(v.A, v.z, dold, dnew, v.q, v.d,
+                                                                     g.zc, s->slot(2), v.ctl);
+    k_fin_alpha<<<s->nsys, kFinThreads, 0, s->stream>>>(v.ctl, s->slot(2), nblocks(g));
+    k_update_xr<T><<<grid_1d(nb1, s), NT, 0, s->stream>>>(v.x, v.r, dnew, v.q, n, v.ctl);
+    k_residual<T, T, T, false, true><<<grid_sys(g.grid, s), g.block, 0, s->stream>>>(
+        v.A, v.x, v.b, v.r, v.d, g.zc, nullptr, v.ctl);
     const int nb = pcg_precondition<T>(s, v, v.ctl);
     if (nb < 0) return -nb;
-    k_fin_sigma<<<1, kFinThreads, 0, s->stream>>>(v.ctl, s->slot(0), v.mg ? s->slot(1) : s->slot(0), nb);
+    k_fin_sigma<<<s->nsys, kFinThreads, 0, s->stream>>>(v.ctl, s->slot(0), v.mg ? s->slot(1) : s->slot(0), nb);
     return check_launch(s, "pcg iteration");
 }
 
@@ -947,7 +963,7 @@ int upload_ctl(jg_solver* s, Ctl* dev, const Ctl& h) {
 int ensure_f32(jg_solver* s) {
     if (s->f32_valid) return JG_OK;
     for (auto& l : s->lv)
-        k_rows_to_f32<<<nblocks_1d(l.d.n()), NT, 0, s->stream>>>(l.A64, l.A32, l.d.n());
+        k_rows_to_f32<<<grid_1d(nblocks_1d(l.d.n()), s), NT, 0, s->stream>>>(l.A64, l.A32, l.d.n());
     JG_TRY(check_launch(s, "rows_to_f32"));
     s->f32_valid = true;
     return JG_OK;
@@ -1169,6 +1185,92 @@ static int csr_upload(jg_solver* s, size_t rows, size_t nnz, const double* value
     return sync(s);  // the staging vectors go out of scope
 }
 
+namespace {
+
+// Allocates the hierarchy (levels halving as checked by the caller) for
+// nsys systems held back to back (jg_device.h, kSys*).
+int levels_alloc(jg_solver* s, size_t nlev, const size_t* nx, const size_t* ny, const size_t* nz,
+                 int nsys) {
+    JG_HIP(hipSetDevice(s->device));
+    JG_HIP(hipStreamSynchronize(s->stream));
+    s->release();
+    s->nsys = nsys;
+    s->twod = nlev > 1 && nz[0] == 1;
+    s->lv.resize(nlev);
+    const size_t m = (size_t)nsys;
+    for (size_t l = 0; l < nlev; ++l) {
+        Level& lv = s->lv[l];
+        lv.d = Dims{(int)nx[l], (int)ny[l], (int)nz[l]};
+        const size_t n = lv.d.n() * m;
+        JG_TRY(dalloc(s, &lv.A64, n));
+        JG_TRY(dalloc(s, &lv.A32, n));
+        JG_TRY(dalloc(s, &lv.t64, n));
+        JG_TRY(dalloc(s, &lv.t32, n));
+        JG_TRY(dalloc(s, &lv.markers, n));
+        JG_TRY(dalloc(s, &lv.y64, n));
+        JG_TRY(dalloc(s, &lv.y32, n));
+        JG_TRY(dalloc(s, &lv.x64, n));
+        JG_TRY(dalloc(s, &lv.x32, n));
+        if (l > 0) {
+            JG_TRY(dalloc(s, &lv.b64, n));
+            JG_TRY(dalloc(s, &lv.b32, n));
+        }
+    }
+    const size_t n0 = s->lv[0].d.n() * m;
+    for (double** p : {&s->b, &s->x, &s->r, &s->z, &s->dA, &s->dB, &s->q}) JG_TRY(dalloc(s, p, n0));
+    for (float** p : {&s->rhs32, &s->r32, &s->z32, &s->d32A, &s->d32B, &s->q32, &s->e32})
+        JG_TRY(dalloc(s, p, n0));
+    JG_TRY(dalloc(s, &s->part, (size_t)kSysPart * m));
+    JG_TRY(dalloc(s, &s->scal, (size_t)kSysScal * m));
+    JG_TRY(dalloc(s, &s->ctl, (size_t)kSysCtl * m));
+    return JG_OK;
+}
+
+// Makes v a view of member i of group g: g's hierarchy and tuning, every
+// pointer offset to member i's part of g's arrays, g's stream.
+void make_view(jg_solver* g, int i, jg_solver* v) {
+    v->view = true;
+    v->nsys = 1;
+    v->twod = g->twod;
+    v->lv = g->lv;
+    for (Level& l : v->lv) {
+        const idx_t n = l.d.n() * i;
+        l.A64 += n;
+        l.A32 += n;
+        l.t64 += n;
+        l.t32 += n;
+        l.markers += n;
+        l.y64 += n;
+        l.y32 += n;
+        l.x64 += n;
+        l.x32 += n;
+        if (l.b64) l.b64 += n;
+        if (l.b32) l.b32 += n;
+    }
+    const idx_t n0 = g->lv[0].d.n() * i;
+    v->b = g->b + n0;
+    v->x = g->x + n0;
+    v->r = g->r + n0;
+    v->z = g->z + n0;
+    v->dA = g->dA + n0;
+    v->dB = g->dB + n0;
+    v->q = g->q + n0;
+    v->rhs32 = g->rhs32 + n0;
+    v->r32 = g->r32 + n0;
+    v->z32 = g->z32 + n0;
+    v->d32A = g->d32A + n0;
+    v->d32B = g->d32B + n0;
+    v->q32 = g->q32 + n0;
+    v->e32 = g->e32 + n0;
+    v->part = g->part + kSysPart * i;
+    v->scal = g->scal + kSysScal * i;
+    v->ctl = g->ctl + kSysCtl * i;
+    if (v->stream) (void)hipStreamDestroy(v->stream);
+    v->stream = g->stream;
+}
+
+}  // namespace
+
 extern "C" {
 
 void jg_mg_params_default(jg_mg_params* p) {
@@ -1253,41 +1355,15 @@ int jg_solver_set_levels(jg_solver* s, size_t nlev, const size_t* nx, const size
             return JG_ERR_UNSUPPORTED;
         }
     }
-    s->twod = nlev > 1 && nz[0] == 1;
     bool same = s->lv.size() == nlev;
     for (size_t l = 0; same && l < nlev; ++l)
         same = s->lv[l].d.nx == (int)nx[l] && s->lv[l].d.ny == (int)ny[l] && s->lv[l].d.nz == (int)nz[l];
     if (same) return JG_OK;
-    JG_HIP(hipSetDevice(s->device));
-    JG_HIP(hipStreamSynchronize(s->stream));
-    s->release();
-    s->lv.resize(nlev);
-    for (size_t l = 0; l < nlev; ++l) {
-        Level& lv = s->lv[l];
-        lv.d = Dims{(int)nx[l], (int)ny[l], (int)nz[l]};
-        const size_t n = lv.d.n();
-        JG_TRY(dalloc(s, &lv.A64, n));
-        JG_TRY(dalloc(s, &lv.A32, n));
-        JG_TRY(dalloc(s, &lv.t64, n));
-        JG_TRY(dalloc(s, &lv.t32, n));
-        JG_TRY(dalloc(s, &lv.markers, n));
-        JG_TRY(dalloc(s, &lv.y64, n));
-        JG_TRY(dalloc(s, &lv.y32, n));
-        JG_TRY(dalloc(s, &lv.x64, n));
-        JG_TRY(dalloc(s, &lv.x32, n));
-        if (l > 0) {
-            JG_TRY(dalloc(s, &lv.b64, n));
-            JG_TRY(dalloc(s, &lv.b32, n));
-        }
+    if (s->view) {
+        s->err = "a batch member keeps the hierarchy its batch was created with";
+        return JG_ERR_INVALID_ARG;
     }
-    const size_t n0 = s->lv[0].d.n();
-    for (double** p : {&s->b, &s->x, &s->r, &s->z, &s->dA, &s->dB, &s->q}) JG_TRY(dalloc(s, p, n0));
-    for (float** p : {&s->rhs32, &s->r32, &s->z32, &s->d32A, &s->d32B, &s->q32, &s->e32})
-        JG_TRY(dalloc(s, p, n0));
-    JG_TRY(dalloc(s, &s->part, (size_t)jg_solver::kSlots * kMaxPartials));
-    JG_TRY(dalloc(s, &s->scal, 16));
-    JG_TRY(dalloc(s, &s->ctl, 2));
-    return JG_OK;
+    return levels_alloc(s, nlev, nx, ny, nz, 1);
 }
 
 double* jg_solver_matrix(jg_solver* s, size_t level) {
@@ -2012,14 +2088,16 @@ int jg_mg_correct2(jg_solver* s, size_t fnx, size_t fny, const double* coarse, d
 
 // ============================================================ batched solves
 //
-// Many independent systems of uneven sizes, each held by its own jg_solver
-// (its own hierarchy, buffers, control blocks and stream).  The PCG
-// iterations of all systems are captured once into one hipGraph: a fork
-// from the batch stream, one branch per system (K iterations on that
-// system's stream), a join.  Every kernel of an iteration is gated by its
-// system's control block, which is the per-system convergence mask: a
-// converged system's branch replays as no-ops.  The host replays the graph
-// and gathers all stop flags with one kernel and one copy per replay.
+// Many independent systems of uneven sizes.  Systems with the same hierarchy
+// form a group: one jg_solver holds all of them back to back (nsys > 1), and
+// every V-cycle and PCG launch covers the whole group, each block taking its
+// system from its grid index and exiting at once when that system's control
+// block says it has stopped -- the per-system convergence mask.  A member is
+// a view into its group (its own handle for building, uploading and reading
+// its system).  The PCG iterations of all groups are captured once into one
+// hipGraph (a fork from the batch stream, one branch per group, K
+// iterations on that group's stream, a join); the host replays it and
+// gathers every member's stop flag with one kernel and one copy per replay.
 
 namespace {
 
@@ -2043,10 +2121,12 @@ struct jg_batch {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t fork = nullptr, ev0 = nullptr, ev1 = nullptr;
-    std::vector<jg_solver*> sys;
-    std::vector<hipEvent_t> join;
-    Ctl** dctl = nullptr;       // per system: ctl (FP64 solve), then ctl + 1 (mixed inner solve)
-    double** dscal = nullptr;   // per system: scal
+    std::vector<jg_solver*> sys;     // member views, in the caller's order
+    std::vector<jg_solver*> groups;  // one solver per distinct hierarchy, owning the arrays
+    std::vector<std::vector<int>> members;  // per group: its members in group order
+    std::vector<hipEvent_t> join;    // per group
+    Ctl** dctl = nullptr;       // per member: ctl (FP64 solve), then ctl + 1 (mixed inner solve)
+    double** dscal = nullptr;   // per member: scal
     int* dflags = nullptr;      // gathered (flags, iteration) pairs
     double* dvals = nullptr;    // gathered scalars
     int* hflags = nullptr;      // pinned
@@ -2067,7 +2147,8 @@ struct jg_batch {
             (void)hipGraphExecDestroy(g.exec);
             (void)hipGraphDestroy(g.graph);
         }
-        for (auto* s : sys) jg_solver_destroy(s);
+        for (auto* s : sys) jg_solver_destroy(s);  // views first: they use the groups' streams
+        for (auto* g : groups) jg_solver_destroy(g);
         for (auto e : join) (void)hipEventDestroy(e);
         for (void* p : {(void*)dctl, (void*)dscal, (void*)dflags, (void*)dvals}) (void)hipFree(p);
         if (hflags) (void)hipHostFree(hflags);
@@ -2089,17 +2170,24 @@ struct jg_batch {
 
 namespace {
 
-// The batch stream waits for the work queued so far on every system stream.
+// The batch stream waits for the work queued so far on every group stream.
 int batch_join(jg_batch* B) {
-    for (size_t i = 0; i < B->sys.size(); ++i) {
-        JG_HIPB(hipEventRecord(B->join[i], B->sys[i]->stream));
-        JG_HIPB(hipStreamWaitEvent(B->stream, B->join[i], 0));
+    for (size_t g = 0; g < B->groups.size(); ++g) {
+        JG_HIPB(hipEventRecord(B->join[g], B->groups[g]->stream));
+        JG_HIPB(hipStreamWaitEvent(B->stream, B->join[g], 0));
     }
     return JG_OK;
 }
 
+// Every group stream waits for the work queued so far on the batch stream.
+int batch_fork(jg_batch* B) {
+    JG_HIPB(hipEventRecord(B->fork, B->stream));
+    for (auto* g : B->groups) JG_HIPB(hipStreamWaitEvent(g->stream, B->fork, 0));
+    return JG_OK;
+}
+
 // Gathers (flags, iteration) of control block `which` (0: FP64, 1: inner) of
-// every system into hflags; synchronises.
+// every member into hflags; synchronises.
 int batch_poll(jg_batch* B, int which) {
     const int S = (int)B->sys.size();
     k_gather_ctl<<<1, 256, 0, B->stream>>>(B->dctl + which * S, S, B->dflags);
@@ -2119,7 +2207,8 @@ int batch_scal(jg_batch* B, int idx) {
     return JG_OK;
 }
 
-// Precision-specific views of a system's level-0 PCG buffers.
+// Precision-specific views of a solver's level-0 PCG buffers (a group's:
+// its first member's pointers, which every launch extends to all members).
 Pcg<double> pcg64(jg_solver* s, const MgCfg<double>* mg) {
     Level& l0 = s->lv[0];
     return Pcg<double>{l0.A64, s->b, s->x, s->r, s->z, s->dA, s->dB, s->q, l0.d, s->ctl, mg};
@@ -2130,7 +2219,7 @@ Pcg<float> pcg32(jg_solver* s, const MgCfg<float>* mg) {
                       s->ctl + 1, mg};
 }
 
-// Captures kBatchIters PCG iterations of every system into one graph (or
+// Captures kBatchIters PCG iterations of every group into one graph (or
 // returns the cached one for the same precision and V-cycle parameters).
 template <typename T>
 int batch_graph(jg_batch* B, int precision, const MgCfg<T>& mg, hipGraphExec_t* out) {
@@ -2143,20 +2232,20 @@ int batch_graph(jg_batch* B, int precision, const MgCfg<T>& mg, hipGraphExec_t* 
     JG_HIPB(hipStreamBeginCapture(B->stream, hipStreamCaptureModeRelaxed));
     int st = JG_OK;
     if (hipEventRecord(B->fork, B->stream) != hipSuccess) st = JG_ERR_HIP;
-    for (size_t i = 0; i < B->sys.size() && st == JG_OK; ++i) {
-        jg_solver* s = B->sys[i];
-        if (hipStreamWaitEvent(s->stream, B->fork, 0) != hipSuccess) {
+    for (size_t gi = 0; gi < B->groups.size() && st == JG_OK; ++gi) {
+        jg_solver* g = B->groups[gi];
+        if (hipStreamWaitEvent(g->stream, B->fork, 0) != hipSuccess) {
             st = JG_ERR_HIP;
             break;
         }
         for (int k = 0; k < kBatchIters && st == JG_OK; ++k) {
             if constexpr (sizeof(T) == 8)
-                st = pcg_iteration<double>(s, pcg64(s, &mg), (k & 1) != 0);
+                st = pcg_iteration<double>(g, pcg64(g, &mg), (k & 1) != 0);
             else
-                st = pcg_iteration<float>(s, pcg32(s, &mg), (k & 1) != 0);
+                st = pcg_iteration<float>(g, pcg32(g, &mg), (k & 1) != 0);
         }
-        if (st == JG_OK && (hipEventRecord(B->join[i], s->stream) != hipSuccess ||
-                            hipStreamWaitEvent(B->stream, B->join[i], 0) != hipSuccess))
+        if (st == JG_OK && (hipEventRecord(B->join[gi], g->stream) != hipSuccess ||
+                            hipStreamWaitEvent(B->stream, B->join[gi], 0) != hipSuccess))
             st = JG_ERR_HIP;
     }
     hipGraph_t graph = nullptr;
@@ -2174,7 +2263,7 @@ int batch_graph(jg_batch* B, int precision, const MgCfg<T>& mg, hipGraphExec_t* 
     return JG_OK;
 }
 
-// Replays the iteration graph until every system's control block `which`
+// Replays the iteration graph until every member's control block `which`
 // is done (or the iteration budget is spent).
 int batch_replay(jg_batch* B, hipGraphExec_t exec, int which, unsigned max_iter) {
     const size_t S = B->sys.size();
@@ -2188,39 +2277,31 @@ int batch_replay(jg_batch* B, hipGraphExec_t exec, int which, unsigned max_iter)
     return JG_OK;
 }
 
-// Sweep tuning once per distinct hierarchy; identical hierarchies copy it.
+// Sweep tuning once per group, with launches that cover the whole group.
 template <typename T>
 int batch_tune(jg_batch* B) {
-    for (size_t i = 0; i < B->sys.size(); ++i) {
-        jg_solver* s = B->sys[i];
-        if (Prec<T>::zc(s->lv[0]) > 0) continue;
-        bool copied = false;
-        for (size_t j = 0; j < i && !copied; ++j) {
-            jg_solver* t = B->sys[j];
-            if (t->lv.size() != s->lv.size()) continue;
-            bool same = true;
-            for (size_t l = 0; same && l < s->lv.size(); ++l)
-                same = t->lv[l].d.nx == s->lv[l].d.nx && t->lv[l].d.ny == s->lv[l].d.ny &&
-                       t->lv[l].d.nz == s->lv[l].d.nz;
-            if (!same) continue;
-            for (size_t l = 0; l < s->lv.size(); ++l) {
-                for (int kind = 0; kind < kNumSweepKinds; ++kind) {
-                    Prec<T>::zc(s->lv[l], kind) = Prec<T>::zc(t->lv[l], kind);
-                    Prec<T>::sh(s->lv[l], kind) = Prec<T>::sh(t->lv[l], kind);
-                    Prec<T>::pair(s->lv[l], kind) = Prec<T>::pair(t->lv[l], kind);
-                    s->lv[l].zc2_32[kind] = t->lv[l].zc2_32[kind];
-                }
-                s->lv[l].zc2res_32 = t->lv[l].zc2res_32;
-            }
-            copied = true;
+    for (auto* g : B->groups) {
+        const int st = autotune_sweeps<T>(g);
+        if (st != JG_OK) {
+            B->err = g->err;
+            return st;
         }
-        if (!copied) {
-            const int st = autotune_sweeps<T>(s);
-            if (st != JG_OK) {
-                B->err = s->err;
-                return st;
-            }
+    }
+    return JG_OK;
+}
+
+// The FP32 copies of every group whose members' matrices changed.
+int batch_f32(jg_batch* B) {
+    for (size_t gi = 0; gi < B->groups.size(); ++gi) {
+        jg_solver* g = B->groups[gi];
+        for (int m : B->members[gi])
+            if (!B->sys[m]->f32_valid) g->f32_valid = false;
+        const int st = ensure_f32(g);
+        if (st != JG_OK) {
+            B->err = g->err;
+            return st;
         }
+        for (int m : B->members[gi]) B->sys[m]->f32_valid = true;
     }
     return JG_OK;
 }
@@ -2246,23 +2327,53 @@ int jg_batch_create(int device, size_t nsys, const size_t* nx, const size_t* ny,
         hipEventCreateWithFlags(&B->fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreate(&B->ev0) != hipSuccess || hipEventCreate(&B->ev1) != hipSuccess)
         return fail(JG_ERR_HIP);
+    // group the systems by hierarchy (finest size and level count)
+    std::vector<int> group_of(nsys, -1), index_in(nsys, 0);
+    std::vector<size_t> rep;  // a member of each group
     for (size_t i = 0; i < nsys; ++i) {
         if (nlev[i] == 0 || nlev[i] > 16) return fail(JG_ERR_INVALID_ARG);
-        std::vector<size_t> lx(nlev[i]), ly(nlev[i]), lz(nlev[i]);
-        for (size_t l = 0; l < nlev[i]; ++l) {
-            lx[l] = nx[i] >> l;
-            ly[l] = ny[i] >> l;
-            lz[l] = nz[i] >> l;
+        for (size_t g = 0; g < rep.size() && group_of[i] < 0; ++g) {
+            const size_t r = rep[g];
+            if (nx[r] == nx[i] && ny[r] == ny[i] && nz[r] == nz[i] && nlev[r] == nlev[i])
+                group_of[i] = (int)g;
+        }
+        if (group_of[i] < 0) {
+            group_of[i] = (int)rep.size();
+            rep.push_back(i);
+            B->members.emplace_back();
+        }
+        index_in[i] = (int)B->members[group_of[i]].size();
+        B->members[group_of[i]].push_back((int)i);
+    }
+    for (size_t g = 0; g < rep.size(); ++g) {
+        const size_t r = rep[g];
+        std::vector<size_t> lx(nlev[r]), ly(nlev[r]), lz(nlev[r]);
+        for (size_t l = 0; l < nlev[r]; ++l) {
+            lx[l] = nx[r] >> l;
+            ly[l] = ny[r] >> l;
+            lz[l] = nz[r] >> l;
         }
         jg_solver* s = nullptr;
         int st = jg_solver_create(device, &s);
         if (st != JG_OK) return fail(st);
-        B->sys.push_back(s);
-        st = jg_solver_set_levels(s, nlev[i], lx.data(), ly.data(), lz.data());
+        B->groups.push_back(s);
+        // the checks of a single hierarchy, then the arrays of every member
+        st = jg_solver_set_levels(s, nlev[r], lx.data(), ly.data(), lz.data());
+        if (st == JG_OK && (s->twod || lz[0] == 1)) st = JG_ERR_UNSUPPORTED;  // batches are 3-D
+        if (st == JG_OK)
+            st = levels_alloc(s, nlev[r], lx.data(), ly.data(), lz.data(),
+                              (int)B->members[g].size());
         if (st != JG_OK) return fail(st);
         hipEvent_t e = nullptr;
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return fail(JG_ERR_HIP);
         B->join.push_back(e);
+    }
+    for (size_t i = 0; i < nsys; ++i) {
+        jg_solver* v = nullptr;
+        const int st = jg_solver_create(device, &v);
+        if (st != JG_OK) return fail(st);
+        make_view(B->groups[group_of[i]], index_in[i], v);
+        B->sys.push_back(v);
     }
     std::vector<Ctl*> ctls(2 * nsys);
     std::vector<double*> scals(nsys);
@@ -2288,6 +2399,7 @@ void jg_batch_destroy(jg_batch* B) {
     if (!B) return;
     (void)hipSetDevice(B->device);
     (void)hipStreamSynchronize(B->stream);
+    for (auto* g : B->groups) (void)hipStreamSynchronize(g->stream);
     delete B;
 }
 
@@ -2313,30 +2425,36 @@ int jg_batch_mgpcg(jg_batch* B, unsigned max_iter, const jg_mg_params* p, const 
             return st;
         }
     }
+    if (opt.smoother == JG_SMOOTH_JACOBI) {
+        B->err = "batched solves use the red-black smoother";
+        return JG_ERR_UNSUPPORTED;
+    }
     const bool mixed = opt.precision == JG_MIXED;
-    for (auto* s : B->sys)
-        if (mixed) JG_TRY(ensure_f32(s));
+    if (mixed) JG_TRY(batch_f32(B));
     JG_TRY(mixed ? batch_tune<float>(B) : batch_tune<double>(B));
-    for (auto* s : B->sys) JG_TRY(sync(s));
+    for (auto* g : B->groups) JG_TRY(sync(g));
     std::vector<unsigned> iters(S, 0), refinements(S, 0);
     std::vector<double> last(S, 0.0);
     std::vector<int> status(S, JG_OK);
     JG_TRY(batch_join(B));
     JG_HIPB(hipEventRecord(B->ev0, B->stream));
+    JG_TRY(batch_fork(B));
     if (!mixed) {
         MgCfg<double> mg{p->restrict_iter, p->final_iter, p->correct_iter, p->coarsest_iter, omega};
         mg.pairs = opt.sweep_pairs;
-        mg.jacobi = opt.smoother == JG_SMOOTH_JACOBI;
         hipGraphExec_t exec = nullptr;
         JG_TRY(batch_graph<double>(B, JG_FP64, mg, &exec));
-        for (auto* s : B->sys) {
-            JG_HIPB(hipStreamWaitEvent(s->stream, B->ev0, 0));
-            const Dims& d0 = s->lv[0].d;
-            k_fill<double><<<nblocks_1d(d0.n()), NT, 0, s->stream>>>(s->x, 0.0, d0.n());
-            Ctl h;
-            init_ctl(&h, max_iter, p->max_tol, opt.stop_rule == JG_STOP_RELATIVE ? 1 : 0, opt.rel_tol);
-            JG_TRY(upload_ctl(s, s->ctl, h));
-            JG_TRY(pcg_init<double>(s, pcg64(s, &mg)));
+        JG_TRY(batch_fork(B));
+        for (size_t gi = 0; gi < B->groups.size(); ++gi) {
+            jg_solver* g = B->groups[gi];
+            const Dims& d0 = g->lv[0].d;
+            k_fill<double><<<grid_1d(nblocks_1d(d0.n()), g), NT, 0, g->stream>>>(g->x, 0.0, d0.n());
+            for (int m : B->members[gi]) {
+                Ctl h;
+                init_ctl(&h, max_iter, p->max_tol, opt.stop_rule == JG_STOP_RELATIVE ? 1 : 0, opt.rel_tol);
+                JG_TRY(upload_ctl(g, B->sys[m]->ctl, h));
+            }
+            JG_TRY(pcg_init<double>(g, pcg64(g, &mg)));
         }
         JG_TRY(batch_join(B));
         JG_TRY(batch_replay(B, exec, 0, max_iter));
@@ -2348,12 +2466,11 @@ int jg_batch_mgpcg(jg_batch* B, unsigned max_iter, const jg_mg_params* p, const 
     } else {
         MgCfg<float> mg{p->restrict_iter, p->final_iter, p->correct_iter, p->coarsest_iter, (float)omega};
         mg.pairs = opt.sweep_pairs;
-        mg.jacobi = opt.smoother == JG_SMOOTH_JACOBI;
         hipGraphExec_t exec = nullptr;
         JG_TRY(batch_graph<float>(B, JG_MIXED, mg, &exec));
-        // ||b|| of every system
+        JG_TRY(batch_fork(B));
+        // ||b|| of every member
         for (auto* s : B->sys) {
-            JG_HIPB(hipStreamWaitEvent(s->stream, B->ev0, 0));
             const Dims& d0 = s->lv[0].d;
             const int nb1 = nblocks_1d(d0.n());
             k_fill<double><<<nb1, NT, 0, s->stream>>>(s->x, 0.0, d0.n());
@@ -2367,7 +2484,8 @@ int jg_batch_mgpcg(jg_batch* B, unsigned max_iter, const jg_mg_params* p, const 
             goal[i] = opt.stop_rule == JG_STOP_RELATIVE ? opt.rel_tol * std::sqrt(B->hvals[i]) : p->max_tol;
         std::vector<char> done(S, 0);
         for (unsigned outer = 0;; ++outer) {
-            // FP64 residual r = b - A x of every unfinished system, into the FP32 right-hand side
+            JG_TRY(batch_fork(B));
+            // FP64 residual r = b - A x of every unfinished member, into the FP32 right-hand side
             for (size_t i = 0; i < S; ++i) {
                 if (done[i]) continue;
                 jg_solver* s = B->sys[i];
@@ -2381,37 +2499,43 @@ int jg_batch_mgpcg(jg_batch* B, unsigned max_iter, const jg_mg_params* p, const 
             JG_TRY(batch_scal(B, 3));
             bool any = false;
             for (size_t i = 0; i < S; ++i) {
-                jg_solver* s = B->sys[i];
-                Ctl h;
-                if (!done[i]) {
-                    const double rr = B->hvals[i];
-                    last[i] = std::sqrt(rr);
-                    if (!std::isfinite(rr)) {
-                        status[i] = JG_ERR_NONFINITE;
-                        done[i] = 1;
-                    } else if (last[i] <= goal[i] || outer >= opt.max_refine || iters[i] >= max_iter) {
-                        done[i] = 1;
-                    }
+                if (done[i]) continue;
+                const double rr = B->hvals[i];
+                last[i] = std::sqrt(rr);
+                if (!std::isfinite(rr)) {
+                    status[i] = JG_ERR_NONFINITE;
+                    done[i] = 1;
+                } else if (last[i] <= goal[i] || outer >= opt.max_refine || iters[i] >= max_iter) {
+                    done[i] = 1;
                 }
-                if (done[i]) {
-                    init_ctl(&h, 0, 0.0, 1, 0.0);
-                    h.done = 1;  // this system's graph branch replays as no-ops
-                    JG_TRY(upload_ctl(s, s->ctl + 1, h));
-                    continue;
-                }
-                any = true;
-                const double inner =
-                    std::max(opt.inner_rel_tol, std::min(0.5, 0.5 * goal[i] / last[i]));
-                init_ctl(&h, max_iter - iters[i], 0.0, 1, inner);
-                JG_TRY(upload_ctl(s, s->ctl + 1, h));
-                const Dims& d0 = s->lv[0].d;
-                k_fill<float><<<nblocks_1d(d0.n()), NT, 0, s->stream>>>(s->e32, 0.f, d0.n());
-                JG_TRY(pcg_init<float>(s, pcg32(s, &mg)));
+                any = any || !done[i];
             }
             if (!any) break;
+            JG_TRY(batch_fork(B));
+            for (size_t gi = 0; gi < B->groups.size(); ++gi) {
+                jg_solver* g = B->groups[gi];
+                // each member's inner control block: a finished member's
+                // blocks exit at once
+                for (int m : B->members[gi]) {
+                    Ctl h;
+                    if (done[m]) {
+                        init_ctl(&h, 0, 0.0, 1, 0.0);
+                        h.done = 1;
+                    } else {
+                        const double inner =
+                            std::max(opt.inner_rel_tol, std::min(0.5, 0.5 * goal[m] / last[m]));
+                        init_ctl(&h, max_iter - iters[m], 0.0, 1, inner);
+                    }
+                    JG_TRY(upload_ctl(g, B->sys[m]->ctl + 1, h));
+                }
+                const Dims& d0 = g->lv[0].d;
+                k_fill<float><<<grid_1d(nblocks_1d(d0.n()), g), NT, 0, g->stream>>>(g->e32, 0.f, d0.n());
+                JG_TRY(pcg_init<float>(g, pcg32(g, &mg)));
+            }
             JG_TRY(batch_join(B));
             JG_TRY(batch_replay(B, exec, 1, max_iter));
             JG_TRY(batch_poll(B, 1));
+            JG_TRY(batch_fork(B));
             for (size_t i = 0; i < S; ++i) {
                 if (done[i]) continue;
                 jg_solver* s = B->sys[i];

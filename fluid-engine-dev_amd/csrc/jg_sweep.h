@@ -177,7 +177,6 @@ __global__ __launch_bounds__(kSwLX * kSwLY / RP, sweep_waves_per_eu(sizeof(T), R
     // update is (1 - omega) x + w (b - s), one division per cell and sweep;
     // FP64 keeps Jet's omega (b - s) / c bit for bit.
     constexpr bool kWeighted = sizeof(T) == 4;
-    if (ctl && ctl->done) return;
     __shared__ __align__(16) T sx[4][kSwLY][kSwLX];  // x, planes t .. t-3 (slot = plane & 3)
     __shared__ T sr[2][kSwLY][kSwLX];                // A.r, planes t-1, t-2 (slot = plane & 1)
     __shared__ T su[2][kSwLY][kSwLX];                // A.u
@@ -186,6 +185,22 @@ __global__ __launch_bounds__(kSwLX * kSwLY / RP, sweep_waves_per_eu(sizeof(T), R
     const int tx = threadIdx.x, ty = threadIdx.y;
     int bx, by, bz;
     xcd_tile(bx, by, bz);
+    idx_t part;
+    const int sys = block_system(bx, by, bz, (d.nz + zc - 1) / zc, part);
+    if (sys > 0) {
+        const idx_t n = d.n();
+        A += sys * n;
+        b += sys * n;
+        if (!ZERO) xin += sys * n;
+        xout += sys * n;
+        if (CORRECT) coarse += sys * (n / 8);
+        if (DOT) {
+            p_bx += sys * kSysPart;
+            p_bb += sys * kSysPart;
+        }
+        if (ctl) ctl += sys * kSysCtl;
+    }
+    if (ctl && ctl->done) return;
     const int gx0 = bx * kSwIX - kSwH, gy0 = by * kSwIY - kSwH;  // both even
     const int gx = gx0 + tx;
     const int k0 = bz * zc;
@@ -424,9 +439,8 @@ __global__ __launch_bounds__(kSwLX * kSwLY / RP, sweep_waves_per_eu(sizeof(T), R
                 s0 += sh[w];
                 s1 += sh[W + w];
             }
-            const idx_t blk = blockIdx.x + (idx_t)gridDim.x * (blockIdx.y + (idx_t)gridDim.y * blockIdx.z);
-            p_bx[blk] = s0;
-            p_bb[blk] = s1;
+            p_bx[part] = s0;
+            p_bb[part] = s1;
         }
     }
 }

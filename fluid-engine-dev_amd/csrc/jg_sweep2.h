@@ -85,7 +85,6 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
     using G = S2Tile<RES, ZH != 0>;
     constexpr int kH = G::H;
     constexpr int U = kS2U;
-    if (ctl && ctl->done) return;
     // slot of plane p: (p - t0) mod U, a constant at every unrolled step.
     // Tile row ly is array row ly + 1: a guard row above and below each
     // slot keeps the neighbour reads of every lane inside the arrays (a
@@ -98,6 +97,19 @@ __global__ __launch_bounds__(kS2Threads, 1) void k_rb_sweep2(
     const int tx = threadIdx.x, ty = threadIdx.y;
     int bx, by, bz;
     xcd_tile(bx, by, bz);
+    idx_t part;
+    const int sys = block_system(bx, by, bz, (d.nz + zc - 1) / zc, part);
+    if (sys > 0) {
+        const idx_t n = d.n();
+        A += sys * n;
+        b += sys * n;
+        if (!ZERO) xin += sys * n;
+        xout += sys * n;
+        if (CORRECT) coarse += sys * (n / 8);
+        if (RES) res += sys * n;
+        if (ctl) ctl += sys * kSysCtl;
+    }
+    if (ctl && ctl->done) return;
     const int gx0 = bx * G::IX - kH, gy0 = by * G::IY - kH;  // even unless RES
     const int gx = gx0 + tx;
     const int k0 = bz * zc;

@@ -42,12 +42,24 @@ __global__ __launch_bounds__(kRrThreads, sizeof(T) == 4 ? 4 : 2) void k_residual
     const Row<T>* __restrict__ A, const T* __restrict__ x, const T* __restrict__ b,
     T* __restrict__ cb, Dims d, Dims cd, int zc, const Ctl* __restrict__ ctl) {
 #pragma clang fp contract(off)
-    if (ctl && ctl->done) return;
     __shared__ T sres[2][kRrLY][kRrLX];  // fine residual of plane p in buffer p & 1
 
     const int tx = threadIdx.x, ty = threadIdx.y;
     int bx, by, bz;
     xcd_tile(bx, by, bz);
+    idx_t part;
+    const int sys = block_system(bx, by, bz, (cd.nz + zc - 1) / zc, part);
+    if (sys > 0) {
+        const idx_t n = d.n();
+        if (!FROMT) {
+            A += sys * n;
+            x += sys * n;
+        }
+        b += sys * n;
+        cb += sys * cd.n();
+        if (ctl) ctl += sys * kSysCtl;
+    }
+    if (ctl && ctl->done) return;
     const int I0 = bx * kRrCX, J0 = by * kRrCY;
     const int fx0 = 2 * I0 - 1, fy0 = 2 * J0 - 1;  // fine origin of the tile
     const int K0 = bz * zc, K1 = min(cd.nz, K0 + zc);
