@@ -164,14 +164,19 @@ int jg_solver_mg(jg_solver* s, const jg_mg_params* p, const jg_gpu_options* o,
                  jg_solve_info* info);
 
 /* ---------------------------------------------------- batched solves */
-/* Many independent systems of uneven sizes solved together: each system is
- * a jg_solver of the batch (fill it with jg_solver_upload / jg_sp_build /
- * jg_fr_build through jg_batch_system), and jg_batch_mgpcg runs the MGPCG of
- * all of them as one captured hipGraph (one branch per system, each kernel
- * gated by that system's own convergence flag) replayed until every system
- * has stopped.  No counterpart in Jet, which solves one system per
- * FdmLinearSystemSolver3::solve call (include/jet/fdm_linear_system_solver3.h:
- * 17-31); per system the semantics are jg_solver_mgpcg's. */
+/* Many independent systems of uneven sizes solved together.  Systems of
+ * one hierarchy (finest size and level count) form a group held back to
+ * back in device memory; every launch of the V-cycle and of the PCG
+ * iteration covers a whole group, each block taking its system from its grid
+ * index and exiting at once once that system has stopped (its own
+ * convergence flag).  jg_batch_system(b, i) is system i's own handle (fill
+ * it with jg_solver_upload / jg_sp_build / jg_fr_build, read it with
+ * jg_solver_download_x; it keeps the hierarchy it was created with), and
+ * jg_batch_mgpcg runs the MGPCG of every system as one captured hipGraph (one
+ * branch per group) replayed until every system has stopped.  No
+ * counterpart in Jet, which solves one system per FdmLinearSystemSolver3::
+ * solve call (include/jet/fdm_linear_system_solver3.h:17-31); per system the
+ * semantics are jg_solver_mgpcg's (3-D hierarchies, red-black smoothing). */
 typedef struct jg_batch jg_batch;
 /* nsys systems; system i has nlev[i] levels halving from nx[i] x ny[i] x nz[i]. */
 int jg_batch_create(int device, size_t nsys, const size_t* nx, const size_t* ny,
