@@ -661,18 +661,14 @@ __global__ __launch_bounds__(NT) void k_restrict2(const TX* __restrict__ fine,
     coarse[i + (idx_t)cd.nx * j] = sum;
 }
 
-// FdmMgUtils2::correct (src/jet/fdm_mg_linear_system2.cpp:86-143) on one
-// plane: fine += bilinear (1/4, 3/4) prolongation of coarse, clamped at
-// both ends in x and y, four separate adds of (wi * wj) * coarse, y-major.
+// FdmMgUtils2::correct (src/jet/fdm_mg_linear_system2.cpp:86-143) at fine
+// cell (i, j) of a plane: v + the bilinear (1/4, 3/4) prolongation of
+// coarse, clamped at both ends in x and y, four separate adds of
+// (wi * wj) * coarse, y-major.
 template <typename TX>
-__global__ __launch_bounds__(NT) void k_correct2(const TX* __restrict__ coarse,
-                                                 TX* __restrict__ fine, Dims fd,
-                                                 const Ctl* __restrict__ ctl) {
+__device__ __forceinline__ TX correct2_at(const TX* __restrict__ coarse, TX v, int i, int j,
+                                          const Dims& fd) {
 #pragma clang fp contract(off)
-    if (ctl && ctl->done) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    if (i >= fd.nx || j >= fd.ny) return;
     const int cnx = fd.nx / 2;
     const int ci = i / 2, cj = j / 2;
     int i0, i1, j0, j1;
@@ -683,13 +679,24 @@ __global__ __launch_bounds__(NT) void k_correct2(const TX* __restrict__ coarse,
     else { j0 = cj; j1 = j + 1 < fd.ny ? cj + 1 : cj; wj0 = TX(0.75); wj1 = TX(0.25); }
     const int iI[2] = {i0, i1}, jI[2] = {j0, j1};
     const TX iW[2] = {wi0, wi1}, jW[2] = {wj0, wj1};
-    const idx_t f = i + (idx_t)fd.nx * j;
-    TX v = fine[f];
 #pragma unroll
     for (int y = 0; y < 2; ++y)
 #pragma unroll
         for (int x = 0; x < 2; ++x) v += iW[x] * jW[y] * coarse[iI[x] + (idx_t)cnx * jI[y]];
-    fine[f] = v;
+    return v;
+}
+
+// fine += the prolongation of coarse on one plane, one thread per fine cell.
+template <typename TX>
+__global__ __launch_bounds__(NT) void k_correct2(const TX* __restrict__ coarse,
+                                                 TX* __restrict__ fine, Dims fd,
+                                                 const Ctl* __restrict__ ctl) {
+    if (ctl && ctl->done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= fd.nx || j >= fd.ny) return;
+    const idx_t f = i + (idx_t)fd.nx * j;
+    fine[f] = correct2_at(coarse, fine[f], i, j, fd);
 }
 
 // ------------------------------------------------------------- finalize

@@ -1,0 +1,225 @@
+// jg_plane2.h -- two red-black sweeps of a one-plane (2-D) level per launch.
+//
+// FdmGaussSeidelSolver2::relaxRedBlack (src/jet/fdm_gauss_seidel_solver2.cpp:
+// 147-187) applied twice in one colour order, in one pass over HBM, with
+// the V-cycle's neighbours fused in: the residual b - A x of the result
+// (the restriction's input, FdmBlas2::residual's terms) after the pre-
+// smoothing pair, or FdmMgUtils2::correct's prolonged coarse correction
+// (src/jet/fdm_mg_linear_system2.cpp:86-143) before the post-smoothing pair.
+//
+// A block of 16 waves owns a 128 x (16 RY) tile of cells.  Lane t of wave w
+// keeps the cells of columns 2t, 2t + 1 and rows w RY .. w RY + RY - 1 in
+// registers: x, b, A.c (FP32: omega / A.c), A.r, A.u, and the -x coupling of
+// column 0 (the left lane's A.r).  Tile origins have an even coordinate sum
+// and RY is even, so in every row the cell of a given colour sits in a fixed
+// column of the lane's pair: each lane updates exactly one cell per row and
+// half-sweep, and no lane idles.  Neighbours come from the lane's own
+// registers, from the lanes either side (a DPP shift of the whole wave) and,
+// for the first and last rows of a wave's strip, from the waves above and
+// below through LDS (one barrier per half-sweep).  The valid region shrinks
+// by one cell per half-sweep from the tile edge, so the halo is 4 cells (5
+// with the residual), recomputed rather than exchanged; cells nearer the
+// edge may hold garbage that never reaches the written interior.
+//
+// Each update is k_rb_sweep_plane's expression and the residual is
+// k_residual_plane's, operand for operand with contraction off, so one
+// launch equals two k_rb_sweep_plane launches (and k_residual_plane, and a
+// preceding k_correct2) bit for bit.  The launch reads xin and writes xout
+// (xin != xout): neighbouring tiles read each other's cells as halo.
+#pragma once
+
+#include <type_traits>
+
+namespace jg {
+
+constexpr int kP2Lanes = 64, kP2Waves = 16;  // lanes per tile row, waves per block
+constexpr int kP2X = 2 * kP2Lanes;            // tile width in cells
+// rows per lane: FP64 keeps 4 (64-row tiles), FP32 6 (96-row tiles)
+template <typename T>
+struct P2Cfg {
+    static constexpr int RY = sizeof(T) == 4 ? 6 : 4;
+    static constexpr int TY = kP2Waves * RY;
+};
+constexpr int p2_halo(bool res) { return res ? 5 : 4; }
+
+constexpr int kDppFromLeft = 0x138;   // wave_shr:1 -- lane i reads lane i - 1
+constexpr int kDppFromRight = 0x130;  // wave_shl:1 -- lane i reads lane i + 1
+
+// v of the lane to the left / right; the lane without one reads 0.
+template <int CTRL>
+__device__ __forceinline__ float p2_shift(float v) {
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double p2_shift(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xF, 0xF, false);
+    const unsigned hi =
+        (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// FIRST: the colour updated first in both sweeps ((i + j) & 1 == FIRST).
+// XIN: x starts from xin (else from zero).  CORR: the prolongation of
+// coarse is added to the starting x.  RES: the residual of the result is
+// written to res.
+template <typename T, int FIRST, bool XIN, bool RES, bool CORR>
+__global__ __launch_bounds__(kP2Lanes * kP2Waves) void k_rb_sweep2_plane(
+    const Row<T>* __restrict__ A, const T* __restrict__ b, const T* __restrict__ xin,
+    const T* __restrict__ coarse, T* __restrict__ xout, T* __restrict__ res, Dims d, T omega,
+    const Ctl* __restrict__ ctl) {
+#pragma clang fp contract(off)
+    constexpr bool kW = sizeof(T) == 4;
+    constexpr int RY = P2Cfg<T>::RY, TY = P2Cfg<T>::TY, NW = kP2Waves, H = p2_halo(RES);
+    constexpr int IX = kP2X - 2 * H, IY = TY - 2 * H;
+    constexpr bool kZeroStart = !XIN && !CORR;
+    static_assert(RY % 2 == 0 && IX % 2 == 0 && IY % 2 == 0, "colour columns must stay fixed");
+    if (ctl && ctl->done) return;
+    // x of the first and last row of every wave's strip (by column parity,
+    // then lane), and A.u of the last row
+    __shared__ T s_top[NW][2][kP2Lanes];
+    __shared__ T s_bot[NW][2][kP2Lanes];
+    __shared__ T s_u[NW][2][kP2Lanes];
+    const int t = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    int bx, by, bz;
+    xcd_tile(bx, by, bz);
+    // (gx0, gy0): the lane's cell in column 0, row 0 of its strip; the tile
+    // origin (bx IX - H, by IY - H) has an even coordinate sum, so cell (j, k)
+    // has colour (j + k) & 1
+    const int gx0 = bx * IX - H + 2 * t;
+    const int gy0 = by * IY - H + w * RY;
+    const int wd = w > 0 ? w - 1 : 0, wu = w + 1 < NW ? w + 1 : w;
+    T x[2][RY], cw[2][RY], bb[2][RY], rr[2][RY], uu[2][RY], cc[2][RY];
+    T rl[RY];  // -x coupling of column 0: A.r of the left lane's column 1
+    T ud[2];   // -y coupling of row 0: A.u of the lower wave's last row
+    bool hl[2], hr[2], hd[RY], hu[RY];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        hl[j] = gx0 + j > 0;
+        hr[j] = gx0 + j + 1 < d.nx;
+    }
+#pragma unroll
+    for (int k = 0; k < RY; ++k) {
+        const int gy = gy0 + k;
+        hd[k] = gy > 0;
+        hu[k] = gy + 1 < d.ny;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int gx = gx0 + j;
+            T xv = T(0), c = T(0), bv = T(0), r = T(0), u = T(0);
+            if (gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny) {
+                const idx_t i = gx + (idx_t)d.nx * gy;
+                const Row<T> row = A[i];
+                c = row.c;
+                r = row.r;
+                u = row.u;
+                bv = b[i];
+                if constexpr (XIN) xv = xin[i];
+                if constexpr (CORR) xv = correct2_at(coarse, xv, gx, gy, d);
+            }
+            x[j][k] = xv;
+            bb[j][k] = bv;
+            rr[j][k] = r;
+            uu[j][k] = u;
+            cc[j][k] = c;
+            cw[j][k] = kW ? (c == T(0) ? T(0) : omega / c) : c;  // w = omega / c
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < RY; ++k) rl[k] = p2_shift<kDppFromLeft>(rr[1][k]);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        s_u[w][j][t] = uu[j][RY - 1];
+        s_top[w][j][t] = x[j][0];
+        s_bot[w][j][t] = x[j][RY - 1];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 2; ++j) ud[j] = s_u[wd][j][t];
+
+    // neighbours of cell (j, k): x to the left / right / below / above and
+    // the -x / -y couplings
+    auto x_left = [&](int j, int k) __attribute__((always_inline)) {
+        return j == 0 ? p2_shift<kDppFromLeft>(x[1][k]) : x[0][k];
+    };
+    auto x_right = [&](int j, int k) __attribute__((always_inline)) {
+        return j == 1 ? p2_shift<kDppFromRight>(x[0][k]) : x[1][k];
+    };
+    auto x_down = [&](int j, int k) __attribute__((always_inline)) {
+        return k > 0 ? x[j][k - 1] : s_bot[wd][j][t];
+    };
+    auto x_up = [&](int j, int k) __attribute__((always_inline)) {
+        return k + 1 < RY ? x[j][k + 1] : s_top[wu][j][t];
+    };
+    auto c_left = [&](int j, int k) __attribute__((always_inline)) {
+        return j == 0 ? rl[k] : rr[0][k];
+    };
+    auto c_down = [&](int j, int k) __attribute__((always_inline)) {
+        return k > 0 ? uu[j][k - 1] : ud[j];
+    };
+
+    // one half-sweep: every cell of colour C, then publish the strip ends
+    auto half = [&](auto colour, auto zero) __attribute__((always_inline)) {
+        constexpr int C = decltype(colour)::value;
+#pragma unroll
+        for (int k = 0; k < RY; ++k) {
+            const int j = (C + k) & 1;
+            T v;
+            if constexpr (decltype(zero)::value) {
+                // every neighbour is still zero
+                v = cw[j][k] == T(0) ? T(0)
+                    : kW             ? cw[j][k] * (bb[j][k] - T(0))
+                                     : omega * (bb[j][k] - T(0)) / cw[j][k];
+            } else {
+                const T xl = x_left(j, k), xr = x_right(j, k), xd = x_down(j, k), xu = x_up(j, k);
+                T s = (hl[j] ? c_left(j, k) * xl : T(0)) + (hr[j] ? rr[j][k] * xr : T(0));
+                s += hd[k] ? c_down(j, k) * xd : T(0);
+                s += hu[k] ? uu[j][k] * xu : T(0);
+                const T xo = x[j][k];
+                v = cw[j][k] == T(0) ? T(0)
+                    : kW             ? (T(1) - omega) * xo + cw[j][k] * (bb[j][k] - s)
+                                     : (T(1) - omega) * xo + omega * (bb[j][k] - s) / cw[j][k];
+            }
+            x[j][k] = v;
+        }
+        s_top[w][C & 1][t] = x[C & 1][0];
+        s_bot[w][(C + RY - 1) & 1][t] = x[(C + RY - 1) & 1][RY - 1];
+        __syncthreads();
+    };
+    using C0 = std::integral_constant<int, FIRST>;
+    using C1 = std::integral_constant<int, 1 - FIRST>;
+    half(C0{}, std::integral_constant<bool, kZeroStart>{});
+    half(C1{}, std::false_type{});
+    half(C0{}, std::false_type{});
+    half(C1{}, std::false_type{});
+
+    // the interior: x, and the residual in k_residual_plane's order
+    const int lx0 = 2 * t, ly0 = w * RY;
+#pragma unroll
+    for (int k = 0; k < RY; ++k) {
+        const int ly = ly0 + k, gy = gy0 + k;
+        const bool row_in = ly >= H && ly < TY - H && gy < d.ny;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            T rv = T(0);
+            if constexpr (RES) {
+                const T xl = x_left(j, k), xr = x_right(j, k), xd = x_down(j, k), xu = x_up(j, k);
+                rv = bb[j][k] - cc[j][k] * x[j][k];
+                rv -= hl[j] ? c_left(j, k) * xl : T(0);
+                rv -= hr[j] ? rr[j][k] * xr : T(0);
+                rv -= hd[k] ? c_down(j, k) * xd : T(0);
+                rv -= hu[k] ? uu[j][k] * xu : T(0);
+            }
+            const int lx = lx0 + j, gx = gx0 + j;
+            if (row_in && lx >= H && lx < kP2X - H && gx < d.nx) {
+                const idx_t i = gx + (idx_t)d.nx * gy;
+                xout[i] = x[j][k];
+                if constexpr (RES) res[i] = rv;
+            }
+        }
+    }
+}
+
+}  // namespace jg

@@ -18,6 +18,7 @@
 #include "jg_kernels.h"
 #include "jg_sweep.h"
 #include "jg_sweep2.h"
+#include "jg_plane2.h"
 #include "jg_brick.h"
 #include "jg_bottom.h"
 #include "jg_transfer.h"
@@ -366,24 +367,56 @@ int vcycle_jacobi(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl
     return JG_OK;
 }
 
-// z = M r: one symmetric V-cycle with zero initial guess.  Level 0 reads
-// its right-hand side from r0 and writes its result to z0.  Each level runs
-// `pre` forward sweeps (red then black) from x = 0, restricts the residual,
-// recurses, then `post` backward sweeps (black then red), the first of which
-// adds the prolonged coarse correction as it loads x.  Sweeps ping-pong
-// between the level's x and y vectors so that the last one lands in x.
-// Returns the number of b.x / b.b partials the last level-0 sweep wrote to
-// slots 0 and 1 (0 if it wrote none).
+// Two sweeps of a one-plane level in one launch (k_rb_sweep2_plane): xin
+// nullptr starts from zero, coarse adds its prolongation to the start, res
+// receives the residual of the result.  The residual is fused only into
+// first-colour-0 (pre-smoothing) pairs and the correction only into
+// first-colour-1 (post-smoothing) pairs, never both.
+template <typename T>
+void launch_plane2(jg_solver* s, const Row<T>* A, const T* b, const T* xin, const T* coarse,
+                   T* xout, T* res, const Dims& d, int first, T omega, const Ctl* ctl) {
+    const int H = p2_halo(res != nullptr);
+    const int IX = kP2X - 2 * H, IY = P2Cfg<T>::TY - 2 * H;
+    const dim3 grid((d.nx + IX - 1) / IX, (d.ny + IY - 1) / IY, 1);
+    const dim3 block(kP2Lanes, kP2Waves, 1);
+    hipStream_t st = s->stream;
+#define JG_P2(F, XI, RE, CO)                                                                      \
+    k_rb_sweep2_plane<T, F, XI, RE, CO><<<grid, block, 0, st>>>(A, b, xin, coarse, xout, res, d, \
+                                                                omega, ctl)
+    if (first == 0) {
+        if (xin) {
+            if (res) JG_P2(0, true, true, false);
+            else JG_P2(0, true, false, false);
+        } else {
+            if (res) JG_P2(0, false, true, false);
+            else JG_P2(0, false, false, false);
+        }
+    } else {
+        if (xin) {
+            if (coarse) JG_P2(1, true, false, true);
+            else JG_P2(1, true, false, false);
+        } else {
+            if (coarse) JG_P2(1, false, false, true);
+            else JG_P2(1, false, false, false);
+        }
+    }
+#undef JG_P2
+}
+
 // The V-cycle on a 2-D hierarchy (FdmMgLinearSystem2): the same symmetric
-// cycle as vcycle() below, with FdmMgUtils2's transfers as separate
-// launches, one-plane sweeps (k_rb_sweep_plane, Jet's 2-D relax) and a
-// one-plane residual (k_residual_plane).
+// cycle as vcycle() below, with FdmMgUtils2's transfers and Jet's 2-D relax.
+// Consecutive sweeps of a level run two per launch (k_rb_sweep2_plane): the
+// last pre-smoothing pair also writes the residual the restriction reads,
+// and the first post-smoothing pair adds the prolonged correction as it
+// loads x.  Leftover single sweeps (odd counts, or pairs < 0) take
+// k_rb_sweep_plane, with separate k_residual_plane / k_correct2 launches
+// where nothing fuses them; the result is bit-identical either way.
 template <typename T>
 int vcycle2d(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl) {
     const size_t L = s->lv.size();
     auto B = [&](size_t l) -> const T* { return l == 0 ? r0 : Prec<T>::b(s->lv[l]); };
     auto X = [&](size_t l) -> T* { return l == 0 ? z0 : Prec<T>::x(s->lv[l]); };
-    const int fwd = (int)std::max(1u, (c.coarsest + 1) / 2);
+    const unsigned fwd = std::max(1u, (c.coarsest + 1) / 2);
     // one sweep of a one-plane level: k_rb_sweep_plane, 60 x 12 interiors
     auto sweep = [&](size_t l, const T* in, T* out, int first) {
         Level& lv = s->lv[l];
@@ -403,40 +436,68 @@ int vcycle2d(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl
                 k_rb_sweep_plane<T, 1, true><<<grid, block, 0, s->stream>>>(A, B(l), in, out, d, c.omega, ctl);
         }
     };
-    // Launch m of level l's pre + post sweeps writes buffer ((M-1-m) even ? x : y),
-    // so the last lands in x; the prolonged correction is added to the
-    // buffer the pre-sweeps left (zeroed first when there are none).
-    std::vector<T*> pre_out(L, nullptr);
+    // n sweeps of one colour order as launches of 2 and 1 sweeps: a leftover
+    // single first (lead = true: the pre-smoothing, whose last pair carries
+    // the residual) or last (the post-smoothing, whose first pair carries the
+    // correction)
+    auto plan = [&](unsigned n, bool lead) {
+        std::vector<int> p;
+        if (c.pairs < 0) return std::vector<int>(n, 1);
+        if (lead && n % 2) p.push_back(1);
+        for (unsigned k = 0; k + 1 < n; k += 2) p.push_back(2);
+        if (!lead && n % 2) p.push_back(1);
+        return p;
+    };
+    // Launch m of a level's M launches writes ((M-1-m) even ? x : y), so the
+    // last lands in x; an unfused correction is added in place to the buffer
+    // the pre-smoothing left (zeroed first when there is none).
+    std::vector<const T*> pre_out(L, nullptr);
     for (size_t l = 0; l < L; ++l) {
         Level& lv = s->lv[l];
         const Dims& d = lv.d;
+        const Row<T>* A = Prec<T>::A(lv);
         T* Y = Prec<T>::y(lv);
         if (l + 1 == L) {
-            const int S = 2 * fwd;
+            const std::vector<int> fw = plan(fwd, true), bw = plan(fwd, false);
+            const size_t M = fw.size() + bw.size();
             const T* in = nullptr;
-            for (int k = 0; k < S; ++k) {
-                T* out = ((S - 1 - k) % 2 == 0) ? X(l) : Y;
-                sweep(l, in, out, k < fwd ? 0 : 1);
+            for (size_t m = 0; m < M; ++m) {
+                T* out = ((M - 1 - m) % 2 == 0) ? X(l) : Y;
+                const bool f = m < fw.size();
+                const int n = f ? fw[m] : bw[m - fw.size()];
+                if (n == 2)
+                    launch_plane2<T>(s, A, B(l), in, nullptr, out, nullptr, d, f ? 0 : 1, c.omega, ctl);
+                else
+                    sweep(l, in, out, f ? 0 : 1);
                 in = out;
             }
             JG_TRY(check_launch(s, "coarsest 2-D"));
             break;
         }
-        const size_t post = l == 0 ? c.post0 : c.post;
-        const size_t M = c.pre + post;
-        T* in = nullptr;
-        for (size_t m = 0; m < c.pre; ++m) {
+        const unsigned post = l == 0 ? c.post0 : c.post;
+        const std::vector<int> pre = plan(c.pre, true);
+        const size_t M = pre.size() + plan(post, false).size();
+        const T* in = nullptr;
+        bool res_done = false;
+        for (size_t m = 0; m < pre.size(); ++m) {
             T* out = ((M - 1 - m) % 2 == 0) ? X(l) : Y;
-            sweep(l, in, out, 0);
+            res_done = pre[m] == 2 && m + 1 == pre.size();
+            if (pre[m] == 2)
+                launch_plane2<T>(s, A, B(l), in, nullptr, out, res_done ? Prec<T>::t(lv) : nullptr, d,
+                                 0, c.omega, ctl);
+            else
+                sweep(l, in, out, 0);
             in = out;
         }
         pre_out[l] = in;
         const Dims& cd = s->lv[l + 1].d;
         const Geo gc = geo_planes(cd);
         if (in) {
-            const Geo gf = geo_planes(d);
-            k_residual_plane<T><<<gf.grid, gf.block, 0, s->stream>>>(Prec<T>::A(lv), in, B(l),
-                                                                    Prec<T>::t(lv), d, ctl);
+            if (!res_done) {
+                const Geo gf = geo_planes(d);
+                k_residual_plane<T><<<gf.grid, gf.block, 0, s->stream>>>(A, in, B(l), Prec<T>::t(lv),
+                                                                        d, ctl);
+            }
             k_restrict2<T><<<gc.grid, gc.block, 0, s->stream>>>(Prec<T>::t(lv),
                                                                 Prec<T>::b(s->lv[l + 1]), cd, ctl);
         } else {
@@ -448,21 +509,31 @@ int vcycle2d(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl
     for (size_t l = L - 1; l-- > 0;) {
         Level& lv = s->lv[l];
         const Dims& d = lv.d;
+        const Row<T>* A = Prec<T>::A(lv);
         T* Y = Prec<T>::y(lv);
-        const size_t post = l == 0 ? c.post0 : c.post;
-        const size_t M = c.pre + post;
-        T* cur = pre_out[l];
-        if (!cur) {
-            cur = (M % 2 == 0) ? X(l) : Y;
-            k_fill<T><<<nblocks_1d(d.n()), NT, 0, s->stream>>>(cur, T(0), d.n());
+        const unsigned post_n = l == 0 ? c.post0 : c.post;
+        const size_t P = plan(c.pre, true).size();
+        const std::vector<int> post = plan(post_n, false);
+        const size_t M = P + post.size();
+        const T* in = pre_out[l];
+        const bool fused = !post.empty() && post[0] == 2;
+        if (!fused) {
+            T* cur = const_cast<T*>(in);
+            if (!cur) {
+                cur = (M % 2 == 0) ? X(l) : Y;
+                k_fill<T><<<nblocks_1d(d.n()), NT, 0, s->stream>>>(cur, T(0), d.n());
+            }
+            const Geo gf = geo_planes(d);
+            k_correct2<T><<<gf.grid, gf.block, 0, s->stream>>>(X(l + 1), cur, d, ctl);
+            in = cur;
         }
-        const Geo gf = geo_planes(d);
-        k_correct2<T><<<gf.grid, gf.block, 0, s->stream>>>(X(l + 1), cur, d, ctl);
-        const T* in = cur;
-        for (size_t q = 0; q < post; ++q) {
-            const size_t m = c.pre + q;
-            T* out = ((M - 1 - m) % 2 == 0) ? X(l) : Y;
-            sweep(l, in, out, 1);
+        for (size_t q = 0; q < post.size(); ++q) {
+            T* out = ((M - 1 - (P + q)) % 2 == 0) ? X(l) : Y;
+            if (post[q] == 2)
+                launch_plane2<T>(s, A, B(l), in, q == 0 ? X(l + 1) : nullptr, out, nullptr, d, 1,
+                                 c.omega, ctl);
+            else
+                sweep(l, in, out, 1);
             in = out;
         }
         JG_TRY(check_launch(s, "vcycle 2-D up"));
@@ -470,6 +541,14 @@ int vcycle2d(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl
     return JG_OK;
 }
 
+// z = M r: one symmetric V-cycle with zero initial guess.  Level 0 reads
+// its right-hand side from r0 and writes its result to z0.  Each level runs
+// `pre` forward sweeps (red then black) from x = 0, restricts the residual,
+// recurses, then `post` backward sweeps (black then red), the first of which
+// adds the prolonged coarse correction as it loads x.  Sweeps ping-pong
+// between the level's x and y vectors so that the last one lands in x.
+// Returns the number of b.x / b.b partials the last level-0 sweep wrote to
+// slots 0 and 1 (0 if it wrote none).
 template <typename T>
 int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, int* dot_blocks) {
     const size_t L = s->lv.size();
@@ -873,8 +952,8 @@ int pcg_iteration(jg_solver* s, const Pcg<T>& v, bool odd) {
         }
     }
     Geo g = geo_stencil(v.d);
-    k_dir_spmv<T, T><<<grid_sys(g.grid, s), g.block, 0, s->stream>>>(v.A, v.z, dold, dnew, v.q, v.d,
-                                                                     g.zc, s->slot(2), v.ctl);
+    k_dir_spmv<T, T><<<grid_sys(g.grid, s), g.block, 0, s->stream>>>(v.A, v.z, dold, dnew, v.q,
+                                                                     v.d, g.zc, s->slot(2), v.ctl);
     k_fin_alpha<<<s->nsys, kFinThreads, 0, s->stream>>>(v.ctl, s->slot(2), nblocks(g));
     k_update_xr<T><<<grid_1d(nb1, s), NT, 0, s->stream>>>(v.x, v.r, dnew, v.q, n, v.ctl);
     k_residual<T, T, T, false, true><<<grid_sys(g.grid, s), g.block, 0, s->stream>>>(

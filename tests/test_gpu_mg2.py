@@ -82,6 +82,66 @@ def test_vcycle2_matches_oracle(solver, oracle, shape, levels, sched):
         assert rel_err(host(z), want) <= tol, prec
 
 
+def apply2(solver, gp, precision, r, **kw):
+    z = empty(r.shape)
+    solver.mg_vcycle(gp, jetgpu.gpu_options(precision=precision, **kw), dev(r), z)
+    return host(z)
+
+
+@pytest.mark.parametrize("shape,levels", [((256, 128), 5), ((360, 200), 3), ((1024, 512), 6),
+                                          ((500, 300), 3), ((130, 66), 2), ((64, 64), 4),
+                                          ((30, 18), 2)])
+@pytest.mark.parametrize("sched", [(2, 2, 20, 2), (3, 3, 6, 3), (2, 4, 6, 4), (1, 2, 4, 0),
+                                   (0, 2, 6, 3), (2, 0, 6, 0), (2, 1, 5, 1), (4, 4, 7, 4)])
+def test_vcycle2_two_sweep_kernel_bitwise(solver, oracle, shape, levels, sched):
+    # Two sweeps per launch (k_rb_sweep2_plane) must give exactly the
+    # single-sweep result (k_rb_sweep_plane, k_residual_plane, k_correct2):
+    # pairs that start from zero, from x, or from x plus the prolonged
+    # correction; the last pre-smoothing pair writing the residual (118 x 86
+    # FP32 / 118 x 54 FP64 interiors, 120 x 88 / 120 x 56 without it);
+    # partial tiles in x and y; odd counts leaving a single sweep first (pre)
+    # or last (post, coarsest).
+    print(f"case two-sweep 2-D shape={shape} levels={levels} sched={sched}", flush=True)
+    s = hierarchy2(shape, levels, seed=58)
+    upload(solver, s)
+    gp, op = params(sched)
+    r = np.random.default_rng(59).uniform(-1, 1, s.x[0].shape)
+    want = oracle.mg_clean_vcycle(s, op, r)
+    for prec, tol in ((jetgpu.JG_MIXED, 2e-5), (jetgpu.JG_FP64, 1e-12)):
+        single = apply2(solver, gp, prec, r, sweep_pairs=-1)
+        paired = apply2(solver, gp, prec, r, sweep_pairs=1)
+        assert np.array_equal(single, paired), prec
+        assert np.array_equal(apply2(solver, gp, prec, r), paired), prec
+        assert rel_err(paired, want) <= tol, prec
+
+
+def test_vcycle2_zero_centre_rows(solver, oracle):
+    # all-zero rows (a fluid cell walled in by solids) are set to zero by both
+    # one-plane smoothers, on every level
+    s = hierarchy2((256, 192), 4, seed=60)
+    rng = np.random.default_rng(61)
+    for A in s.A:
+        mask = rng.random(A.shape[:3]) < 0.02
+        A[mask] = 0.0
+        A[:, :, 1:, 1][mask[:, :, 1:]] = 0.0
+        A[:, :, :-1, 1][mask[:, :, 1:]] = 0.0
+        A[:, 1:, :, 2][mask[:, 1:, :]] = 0.0
+        A[:, :-1, :, 2][mask[:, 1:, :]] = 0.0
+    upload(solver, s)
+    gp, op = params((2, 2, 20, 2))
+    r = rng.uniform(-1, 1, s.x[0].shape)
+    zero = s.A[0][..., 0] == 0.0
+    r[zero] = 0.0
+    want = oracle.mg_clean_vcycle(s, op, r)
+    assert np.isfinite(want).all() and np.all(want[zero] == 0.0)
+    for prec, tol in ((jetgpu.JG_FP64, 1e-12), (jetgpu.JG_MIXED, 2e-5)):
+        single = apply2(solver, gp, prec, r, sweep_pairs=-1)
+        paired = apply2(solver, gp, prec, r)
+        assert np.array_equal(single, paired)
+        assert np.isfinite(paired).all() and np.all(paired[zero] == 0.0)
+        assert rel_err(paired, want) <= tol
+
+
 @pytest.mark.parametrize("precision,tol", [(jetgpu.JG_FP64, 1e-12), (jetgpu.JG_MIXED, 1e-5)])
 def test_vcycle2_symmetric_positive_linear(solver, precision, tol):
     print(f"case vcycle2 spd precision={precision}", flush=True)
