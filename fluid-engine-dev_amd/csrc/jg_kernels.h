@@ -611,16 +611,10 @@ __global__ __launch_bounds__(NT) void k_correct(const TX* __restrict__ coarse,
 // fdm_linear_system2.cpp's FdmBlas2::residual has the same four): the
 // z-marching k_residual would spend most of its pipeline on a single plane.
 template <typename T>
-__global__ __launch_bounds__(NT) void k_residual_plane(const Row<T>* __restrict__ A,
-                                                       const T* __restrict__ x,
-                                                       const T* __restrict__ b,
-                                                       T* __restrict__ r, Dims d,
-                                                       const Ctl* __restrict__ ctl) {
+__device__ __forceinline__ T residual_plane_at(const Row<T>* __restrict__ A,
+                                               const T* __restrict__ x, const T* __restrict__ b,
+                                               int i, int j, const Dims& d) {
 #pragma clang fp contract(off)
-    if (ctl && ctl->done) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    if (i >= d.nx || j >= d.ny) return;
     const idx_t c = i + (idx_t)d.nx * j;
     const Row<T> row = A[c];
     T rv = b[c] - row.c * x[c];
@@ -628,7 +622,58 @@ __global__ __launch_bounds__(NT) void k_residual_plane(const Row<T>* __restrict_
     rv -= i + 1 < d.nx ? row.r * x[c + 1] : T(0);
     rv -= j > 0 ? A[c - d.nx].u * x[c - d.nx] : T(0);
     rv -= j + 1 < d.ny ? row.u * x[c + d.nx] : T(0);
-    r[c] = rv;
+    return rv;
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_residual_plane(const Row<T>* __restrict__ A,
+                                                       const T* __restrict__ x,
+                                                       const T* __restrict__ b,
+                                                       T* __restrict__ r, Dims d,
+                                                       const Ctl* __restrict__ ctl) {
+    if (ctl && ctl->done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= d.nx || j >= d.ny) return;
+    r[i + (idx_t)d.nx * j] = residual_plane_at(A, x, b, i, j, d);
+}
+
+// PCG's periodic true residual (k_residual's REFRESH) on one plane: the
+// same values, over a bounded grid (a grid-stride loop), so the iterations
+// whose control block skips it cost one small launch rather than one
+// early-exit block per stencil tile.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_residual_plane_refresh(const Row<T>* __restrict__ A,
+                                                               const T* __restrict__ x,
+                                                               const T* __restrict__ b,
+                                                               T* __restrict__ r, Dims d,
+                                                               const Ctl* __restrict__ ctl) {
+    if (ctl->done || !ctl->refresh) return;
+    const idx_t n = (idx_t)d.nx * d.ny;
+    for (idx_t c = blockIdx.x * (idx_t)NT + threadIdx.x; c < n; c += (idx_t)gridDim.x * NT) {
+        const int j = (int)(c / d.nx), i = (int)(c - (idx_t)j * d.nx);
+        r[c] = residual_plane_at(A, x, b, i, j, d);
+    }
+}
+
+// r = b - A x of an FP64 one-plane level (k_residual_plane's values) stored
+// as TR, with one partial of r.r per block: the finest-level residual of
+// mixed refinement and of the final report, over a bounded grid.
+template <typename TR>
+__global__ __launch_bounds__(NT) void k_residual_plane_dot(const Row<double>* __restrict__ A,
+                                                           const double* __restrict__ x,
+                                                           const double* __restrict__ b,
+                                                           TR* __restrict__ r, Dims d,
+                                                           double* __restrict__ partial) {
+    const idx_t n = (idx_t)d.nx * d.ny;
+    double acc = 0.0;
+    for (idx_t c = blockIdx.x * (idx_t)NT + threadIdx.x; c < n; c += (idx_t)gridDim.x * NT) {
+        const int j = (int)(c / d.nx), i = (int)(c - (idx_t)j * d.nx);
+        const double rv = residual_plane_at(A, x, b, i, j, d);
+        r[c] = (TR)rv;
+        acc += rv * rv;
+    }
+    block_partial<NT>(acc, partial, (idx_t)blockIdx.x);
 }
 
 // FdmMgUtils2::restrict (src/jet/fdm_mg_linear_system2.cpp:44-84) on one

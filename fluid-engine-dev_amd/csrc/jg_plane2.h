@@ -7,7 +7,7 @@
 // smoothing pair, or FdmMgUtils2::correct's prolonged coarse correction
 // (src/jet/fdm_mg_linear_system2.cpp:86-143) before the post-smoothing pair.
 //
-// A block of 16 waves owns a 128 x (16 RY) tile of cells.  Lane t of wave w
+// A block of NW waves owns a 128 x (NW RY) tile of cells.  Lane t of wave w
 // keeps the cells of columns 2t, 2t + 1 and rows w RY .. w RY + RY - 1 in
 // registers: x, b, A.c (FP32: omega / A.c), A.r, A.u, and the -x coupling of
 // column 0 (the left lane's A.r).  Tile origins have an even coordinate sum
@@ -32,13 +32,29 @@
 
 namespace jg {
 
-constexpr int kP2Lanes = 64, kP2Waves = 16;  // lanes per tile row, waves per block
-constexpr int kP2X = 2 * kP2Lanes;            // tile width in cells
-// rows per lane: FP64 keeps 4 (64-row tiles), FP32 6 (96-row tiles)
+constexpr int kP2Lanes = 64;       // lanes per tile row
+constexpr int kP2X = 2 * kP2Lanes;  // tile width in cells
+// Waves per block (NW) and rows per lane (RY) of the sweep kernel (P2Cfg)
+// and of the direction-update SpMV (SpCfg): 8 x 4, a 128 x 32 tile, in both
+// precisions.  A block loads its tile, then relaxes it between barriers, so
+// a CU must hold several blocks for one block's loads to overlap another's
+// half-sweeps: WPE (waves per SIMD for two blocks) bounds the VGPRs.  On the
+// 4096^2 MGPCG, 16 x 4 / 16 x 6 tiles (one block per CU) took 34.4 ms FP64
+// and 31.3 ms mixed, 8 x 4 takes 31.1 and 25.8; taller FP32 strips (8 x 6,
+// 8 x 8) and shorter ones (8 x 2) are slower, as is 16 x 2 in FP64.
 template <typename T>
 struct P2Cfg {
-    static constexpr int RY = sizeof(T) == 4 ? 6 : 4;
-    static constexpr int TY = kP2Waves * RY;
+    static constexpr int NW = 8;
+    static constexpr int RY = 4;
+    static constexpr int TY = NW * RY;
+    static constexpr int WPE = 2 * NW / 4;
+};
+template <typename T>
+struct SpCfg {
+    static constexpr int NW = 8;
+    static constexpr int RY = 4;
+    static constexpr int TY = NW * RY;
+    static constexpr int WPE = 2 * NW / 4;
 };
 constexpr int p2_halo(bool res) { return res ? 5 : 4; }
 
@@ -63,15 +79,17 @@ __device__ __forceinline__ double p2_shift(double v) {
 // FIRST: the colour updated first in both sweeps ((i + j) & 1 == FIRST).
 // XIN: x starts from xin (else from zero).  CORR: the prolongation of
 // coarse is added to the starting x.  RES: the residual of the result is
-// written to res.
-template <typename T, int FIRST, bool XIN, bool RES, bool CORR>
-__global__ __launch_bounds__(kP2Lanes * kP2Waves) void k_rb_sweep2_plane(
+// written to res.  DOT: one partial of b.x (PCG's r.z when b is r and x is
+// z) to p_bx and of b.b to p_bb per block.
+template <typename T, int FIRST, bool XIN, bool RES, bool CORR, bool DOT = false>
+__global__ __launch_bounds__(kP2Lanes * P2Cfg<T>::NW, P2Cfg<T>::WPE) void k_rb_sweep2_plane(
     const Row<T>* __restrict__ A, const T* __restrict__ b, const T* __restrict__ xin,
     const T* __restrict__ coarse, T* __restrict__ xout, T* __restrict__ res, Dims d, T omega,
-    const Ctl* __restrict__ ctl) {
+    const Ctl* __restrict__ ctl, double* __restrict__ p_bx = nullptr,
+    double* __restrict__ p_bb = nullptr) {
 #pragma clang fp contract(off)
     constexpr bool kW = sizeof(T) == 4;
-    constexpr int RY = P2Cfg<T>::RY, TY = P2Cfg<T>::TY, NW = kP2Waves, H = p2_halo(RES);
+    constexpr int RY = P2Cfg<T>::RY, TY = P2Cfg<T>::TY, NW = P2Cfg<T>::NW, H = p2_halo(RES);
     constexpr int IX = kP2X - 2 * H, IY = TY - 2 * H;
     constexpr bool kZeroStart = !XIN && !CORR;
     static_assert(RY % 2 == 0 && IX % 2 == 0 && IY % 2 == 0, "colour columns must stay fixed");
@@ -197,6 +215,7 @@ __global__ __launch_bounds__(kP2Lanes * kP2Waves) void k_rb_sweep2_plane(
 
     // the interior: x, and the residual in k_residual_plane's order
     const int lx0 = 2 * t, ly0 = w * RY;
+    double a_bx = 0.0, a_bb = 0.0;
 #pragma unroll
     for (int k = 0; k < RY; ++k) {
         const int ly = ly0 + k, gy = gy0 + k;
@@ -217,9 +236,148 @@ __global__ __launch_bounds__(kP2Lanes * kP2Waves) void k_rb_sweep2_plane(
                 const idx_t i = gx + (idx_t)d.nx * gy;
                 xout[i] = x[j][k];
                 if constexpr (RES) res[i] = rv;
+                if constexpr (DOT) {
+                    const double bv = (double)bb[j][k];
+                    a_bx += bv * (double)x[j][k];
+                    a_bb += bv * bv;
+                }
             }
         }
     }
+    if constexpr (DOT)
+        block_partial2<kP2Lanes * NW>(a_bx, a_bb, p_bx, p_bb, bx + (idx_t)gridDim.x * by);
+}
+
+// q = A d with d = z + beta d_old on one plane (FdmBlas2::mvm's terms in
+// k_dir_spmv's order: centre, -x, +x, -y, +y; d as st_val forms it), d
+// written to dnew, one partial of d.q per block.  The lane layout of
+// k_rb_sweep2_plane without a halo: lateral neighbours from the lanes
+// either side, vertical ones from the adjacent waves through LDS; the
+// block's first and last lanes and waves load the neighbour beyond the
+// tile from memory.
+template <typename T>
+__global__ __launch_bounds__(kP2Lanes * SpCfg<T>::NW, SpCfg<T>::WPE) void k_dir_spmv_plane(
+    const Row<T>* __restrict__ A, const T* __restrict__ z, const T* __restrict__ dold,
+    T* __restrict__ dnew, T* __restrict__ q, Dims d, double* __restrict__ partial,
+    const Ctl* __restrict__ ctl) {
+#pragma clang fp contract(off)
+    constexpr int RY = SpCfg<T>::RY, TY = SpCfg<T>::TY, NW = SpCfg<T>::NW;
+    if (ctl->done) return;
+    const double beta = ctl->beta;
+    __shared__ T s_top[NW][2][kP2Lanes];
+    __shared__ T s_bot[NW][2][kP2Lanes];
+    __shared__ T s_u[NW][2][kP2Lanes];
+    const int t = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    int bx, by, bz;
+    xcd_tile(bx, by, bz);
+    const int gx0 = bx * kP2X + 2 * t, gy0 = by * TY + w * RY;
+    const int wd = w > 0 ? w - 1 : 0, wu = w + 1 < NW ? w + 1 : w;
+    auto dval = [&](idx_t i) __attribute__((always_inline)) {
+        return (T)((double)z[i] + beta * (double)dold[i]);
+    };
+    T dv[2][RY], cc[2][RY], rr[2][RY], uu[2][RY];
+    T rl[RY], dl[RY], dr[RY];  // -x coupling and d to the left of column 0, d right of column 1
+    T ud[2], dd[2], du[2];     // -y coupling and d below row 0, d above row RY - 1
+    bool hl[2], hr[2], hd[RY], hu[RY], in[2][RY];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        hl[j] = gx0 + j > 0;
+        hr[j] = gx0 + j + 1 < d.nx;
+    }
+#pragma unroll
+    for (int k = 0; k < RY; ++k) {
+        const int gy = gy0 + k;
+        hd[k] = gy > 0;
+        hu[k] = gy + 1 < d.ny;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            in[j][k] = gx0 + j < d.nx && gy < d.ny;
+            T c = T(0), r = T(0), u = T(0), v = T(0);
+            if (in[j][k]) {
+                const idx_t i = gx0 + j + (idx_t)d.nx * gy;
+                const Row<T> row = A[i];
+                c = row.c;
+                r = row.r;
+                u = row.u;
+                v = dval(i);
+            }
+            cc[j][k] = c;
+            rr[j][k] = r;
+            uu[j][k] = u;
+            dv[j][k] = v;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < RY; ++k) {
+        rl[k] = p2_shift<kDppFromLeft>(rr[1][k]);
+        dl[k] = p2_shift<kDppFromLeft>(dv[1][k]);
+        dr[k] = p2_shift<kDppFromRight>(dv[0][k]);
+    }
+    if (t == 0) {
+#pragma unroll
+        for (int k = 0; k < RY; ++k)
+            if (in[0][k] && hl[0]) {
+                const idx_t i = gx0 + (idx_t)d.nx * (gy0 + k);
+                rl[k] = A[i - 1].r;
+                dl[k] = dval(i - 1);
+            }
+    }
+    if (t == kP2Lanes - 1) {
+#pragma unroll
+        for (int k = 0; k < RY; ++k)
+            if (in[1][k] && hr[1]) dr[k] = dval(gx0 + 1 + (idx_t)d.nx * (gy0 + k) + 1);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        s_u[w][j][t] = uu[j][RY - 1];
+        s_top[w][j][t] = dv[j][0];
+        s_bot[w][j][t] = dv[j][RY - 1];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        ud[j] = s_u[wd][j][t];
+        dd[j] = s_bot[wd][j][t];
+        du[j] = s_top[wu][j][t];
+    }
+    if (w == 0) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            if (in[j][0] && hd[0]) {
+                const idx_t i = gx0 + j + (idx_t)d.nx * gy0;
+                ud[j] = A[i - d.nx].u;
+                dd[j] = dval(i - d.nx);
+            }
+    }
+    if (w == NW - 1) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            if (in[j][RY - 1] && hu[RY - 1]) du[j] = dval(gx0 + j + (idx_t)d.nx * (gy0 + RY));
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < RY; ++k) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const T xl = j == 0 ? dl[k] : dv[0][k], cl = j == 0 ? rl[k] : rr[0][k];
+            const T xr = j == 1 ? dr[k] : dv[1][k];
+            const T xd = k > 0 ? dv[j][k - 1] : dd[j], cd = k > 0 ? uu[j][k - 1] : ud[j];
+            const T xu = k + 1 < RY ? dv[j][k + 1] : du[j];
+            T s = cc[j][k] * dv[j][k];
+            s += hl[j] ? cl * xl : T(0);
+            s += hr[j] ? rr[j][k] * xr : T(0);
+            s += hd[k] ? cd * xd : T(0);
+            s += hu[k] ? uu[j][k] * xu : T(0);
+            if (in[j][k]) {
+                const idx_t i = gx0 + j + (idx_t)d.nx * (gy0 + k);
+                q[i] = s;
+                dnew[i] = dv[j][k];
+                acc += (double)dv[j][k] * (double)s;
+            }
+        }
+    }
+    block_partial<kP2Lanes * NW>(acc, partial, bx + (idx_t)gridDim.x * by);
 }
 
 }  // namespace jg

@@ -371,36 +371,51 @@ int vcycle_jacobi(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl
 // nullptr starts from zero, coarse adds its prolongation to the start, res
 // receives the residual of the result.  The residual is fused only into
 // first-colour-0 (pre-smoothing) pairs and the correction only into
-// first-colour-1 (post-smoothing) pairs, never both.
+// first-colour-1 (post-smoothing) pairs, never both; dot (post-smoothing
+// only) writes the b.x / b.b partials to slots 0 and 1 and returns their
+// count (else 0).
 template <typename T>
-void launch_plane2(jg_solver* s, const Row<T>* A, const T* b, const T* xin, const T* coarse,
-                   T* xout, T* res, const Dims& d, int first, T omega, const Ctl* ctl) {
+int launch_plane2(jg_solver* s, const Row<T>* A, const T* b, const T* xin, const T* coarse,
+                  T* xout, T* res, const Dims& d, int first, T omega, const Ctl* ctl,
+                  bool dot = false) {
     const int H = p2_halo(res != nullptr);
     const int IX = kP2X - 2 * H, IY = P2Cfg<T>::TY - 2 * H;
     const dim3 grid((d.nx + IX - 1) / IX, (d.ny + IY - 1) / IY, 1);
-    const dim3 block(kP2Lanes, kP2Waves, 1);
+    const dim3 block(kP2Lanes, P2Cfg<T>::NW, 1);
     hipStream_t st = s->stream;
-#define JG_P2(F, XI, RE, CO)                                                                      \
-    k_rb_sweep2_plane<T, F, XI, RE, CO><<<grid, block, 0, st>>>(A, b, xin, coarse, xout, res, d, \
-                                                                omega, ctl)
+    double *p0 = s->slot(0), *p1 = s->slot(1);
+#define JG_P2(F, XI, RE, CO, DO)                                                                \
+    k_rb_sweep2_plane<T, F, XI, RE, CO, DO><<<grid, block, 0, st>>>(A, b, xin, coarse, xout, res, \
+                                                                    d, omega, ctl, p0, p1)
     if (first == 0) {
         if (xin) {
-            if (res) JG_P2(0, true, true, false);
-            else JG_P2(0, true, false, false);
+            if (res) JG_P2(0, true, true, false, false);
+            else JG_P2(0, true, false, false, false);
         } else {
-            if (res) JG_P2(0, false, true, false);
-            else JG_P2(0, false, false, false);
+            if (res) JG_P2(0, false, true, false, false);
+            else JG_P2(0, false, false, false, false);
         }
-    } else {
+        return 0;
+    }
+    if (dot) {
         if (xin) {
-            if (coarse) JG_P2(1, true, false, true);
-            else JG_P2(1, true, false, false);
+            if (coarse) JG_P2(1, true, false, true, true);
+            else JG_P2(1, true, false, false, true);
         } else {
-            if (coarse) JG_P2(1, false, false, true);
-            else JG_P2(1, false, false, false);
+            if (coarse) JG_P2(1, false, false, true, true);
+            else JG_P2(1, false, false, false, true);
         }
+        return (int)(grid.x * grid.y);
+    }
+    if (xin) {
+        if (coarse) JG_P2(1, true, false, true, false);
+        else JG_P2(1, true, false, false, false);
+    } else {
+        if (coarse) JG_P2(1, false, false, true, false);
+        else JG_P2(1, false, false, false, false);
     }
 #undef JG_P2
+    return 0;
 }
 
 // The V-cycle on a 2-D hierarchy (FdmMgLinearSystem2): the same symmetric
@@ -412,7 +427,7 @@ void launch_plane2(jg_solver* s, const Row<T>* A, const T* b, const T* xin, cons
 // k_rb_sweep_plane, with separate k_residual_plane / k_correct2 launches
 // where nothing fuses them; the result is bit-identical either way.
 template <typename T>
-int vcycle2d(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl) {
+int vcycle2d(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, int* dot_blocks) {
     const size_t L = s->lv.size();
     auto B = [&](size_t l) -> const T* { return l == 0 ? r0 : Prec<T>::b(s->lv[l]); };
     auto X = [&](size_t l) -> T* { return l == 0 ? z0 : Prec<T>::x(s->lv[l]); };
@@ -529,9 +544,11 @@ int vcycle2d(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl
         }
         for (size_t q = 0; q < post.size(); ++q) {
             T* out = ((M - 1 - (P + q)) % 2 == 0) ? X(l) : Y;
+            // the last level-0 launch also forms PCG's r.z and r.r partials
+            const bool dot = l == 0 && q + 1 == post.size();
             if (post[q] == 2)
-                launch_plane2<T>(s, A, B(l), in, q == 0 ? X(l + 1) : nullptr, out, nullptr, d, 1,
-                                 c.omega, ctl);
+                *dot_blocks = launch_plane2<T>(s, A, B(l), in, q == 0 ? X(l + 1) : nullptr, out,
+                                               nullptr, d, 1, c.omega, ctl, dot);
             else
                 sweep(l, in, out, 1);
             in = out;
@@ -555,7 +572,7 @@ int vcycle(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, 
     *dot_blocks = 0;
     if ((c.jacobi || s->twod) && s->nsys > 1) return JG_ERR_UNSUPPORTED;  // batches are 3-D red-black
     if (c.jacobi) return vcycle_jacobi<T>(s, r0, z0, c, ctl);
-    if (s->twod) return vcycle2d<T>(s, r0, z0, c, ctl);
+    if (s->twod) return vcycle2d<T>(s, r0, z0, c, ctl, dot_blocks);
     auto B = [&](size_t l) -> const T* { return l == 0 ? r0 : Prec<T>::b(s->lv[l]); };
     auto X = [&](size_t l) -> T* { return l == 0 ? z0 : Prec<T>::x(s->lv[l]); };
     std::vector<const T*> pre_out(L, nullptr);
@@ -952,12 +969,24 @@ int pcg_iteration(jg_solver* s, const Pcg<T>& v, bool odd) {
         }
     }
     Geo g = geo_stencil(v.d);
-    k_dir_spmv<T, T><<<grid_sys(g.grid, s), g.block, 0, s->stream>>>(v.A, v.z, dold, dnew, v.q,
-                                                                     v.d, g.zc, s->slot(2), v.ctl);
-    k_fin_alpha<<<s->nsys, kFinThreads, 0, s->stream>>>(v.ctl, s->slot(2), nblocks(g));
+    if (v.d.nz == 1 && s->nsys == 1) {
+        // one plane: 128 x TY tiles, no z march
+        const dim3 grid((v.d.nx + kP2X - 1) / kP2X, (v.d.ny + SpCfg<T>::TY - 1) / SpCfg<T>::TY, 1);
+        k_dir_spmv_plane<T><<<grid, dim3(kP2Lanes, SpCfg<T>::NW, 1), 0, s->stream>>>(
+            v.A, v.z, dold, dnew, v.q, v.d, s->slot(2), v.ctl);
+        k_fin_alpha<<<1, kFinThreads, 0, s->stream>>>(v.ctl, s->slot(2), (int)(grid.x * grid.y));
+    } else {
+        k_dir_spmv<T, T><<<grid_sys(g.grid, s), g.block, 0, s->stream>>>(v.A, v.z, dold, dnew, v.q,
+                                                                         v.d, g.zc, s->slot(2), v.ctl);
+        k_fin_alpha<<<s->nsys, kFinThreads, 0, s->stream>>>(v.ctl, s->slot(2), nblocks(g));
+    }
     k_update_xr<T><<<grid_1d(nb1, s), NT, 0, s->stream>>>(v.x, v.r, dnew, v.q, n, v.ctl);
-    k_residual<T, T, T, false, true><<<grid_sys(g.grid, s), g.block, 0, s->stream>>>(
-        v.A, v.x, v.b, v.r, v.d, g.zc, nullptr, v.ctl);
+    if (v.d.nz == 1 && s->nsys == 1)
+        k_residual_plane_refresh<T><<<std::min(nb1, 1024), NT, 0, s->stream>>>(v.A, v.x, v.b, v.r,
+                                                                                v.d, v.ctl);
+    else
+        k_residual<T, T, T, false, true><<<grid_sys(g.grid, s), g.block, 0, s->stream>>>(
+            v.A, v.x, v.b, v.r, v.d, g.zc, nullptr, v.ctl);
     const int nb = pcg_precondition<T>(s, v, v.ctl);
     if (nb < 0) return -nb;
     k_fin_sigma<<<s->nsys, kFinThreads, 0, s->stream>>>(v.ctl, s->slot(0), v.mg ? s->slot(1) : s->slot(0), nb);
@@ -1048,14 +1077,28 @@ int ensure_f32(jg_solver* s) {
     return JG_OK;
 }
 
+// r = b - A x of the FP64 finest level, stored as TR, with the partials of
+// r.r written to `partial`; returns their count.  One-plane levels take the
+// plane kernel (the z-marching one would spend its pipeline on one plane).
+template <typename TR>
+int residual_dot64(jg_solver* s, const Level& l0, TR* r, double* partial) {
+    if (l0.d.nz == 1) {
+        const int nb = nblocks_1d(l0.d.n());
+        k_residual_plane_dot<TR><<<nb, NT, 0, s->stream>>>(l0.A64, s->x, s->b, r, l0.d, partial);
+        return nb;
+    }
+    const Geo g = geo_stencil(l0.d);
+    k_residual<double, double, TR, true, false><<<g.grid, g.block, 0, s->stream>>>(
+        l0.A64, s->x, s->b, r, l0.d, g.zc, partial, nullptr);
+    return nblocks(g);
+}
+
 // FP64 ||b - A x|| / ||b|| of the finest level (reporting).
 int true_rel_residual(jg_solver* s, double* out) {
     Level& l0 = s->lv[0];
-    Geo g = geo_stencil(l0.d);
     const int nb1 = nblocks_1d(l0.d.n());
-    k_residual<double, double, double, true, false><<<g.grid, g.block, 0, s->stream>>>(
-        l0.A64, s->x, s->b, s->r, l0.d, g.zc, s->slot(0), nullptr);
-    k_fin_sum<<<1, kFinThreads, 0, s->stream>>>(s->slot(0), nblocks(g), s->scal + 0);
+    const int nr = residual_dot64<double>(s, l0, s->r, s->slot(0));
+    k_fin_sum<<<1, kFinThreads, 0, s->stream>>>(s->slot(0), nr, s->scal + 0);
     k_dot<double, double><<<nb1, NT, 0, s->stream>>>(s->b, s->b, l0.d.n(), s->slot(1), nullptr);
     k_fin_sum<<<1, kFinThreads, 0, s->stream>>>(s->slot(1), nb1, s->scal + 1);
     JG_TRY(check_launch(s, "true residual"));
@@ -1636,7 +1679,6 @@ int jg_solver_mgpcg(jg_solver* s, unsigned max_iter, const jg_mg_params* p,
                         (float)omega};
         mg.pairs = opt.sweep_pairs;
         mg.jacobi = opt.smoother == JG_SMOOTH_JACOBI;
-        Geo g = geo_stencil(l0.d);
         const int nb1 = nblocks_1d(n);
         k_dot<double, double><<<nb1, NT, 0, s->stream>>>(s->b, s->b, n, s->slot(3), nullptr);
         k_fin_sum<<<1, kFinThreads, 0, s->stream>>>(s->slot(3), nb1, s->scal + 2);
@@ -1653,9 +1695,8 @@ int jg_solver_mgpcg(jg_solver* s, unsigned max_iter, const jg_mg_params* p,
                 k_convert<double, float><<<nb1, NT, 0, s->stream>>>(s->b, s->rhs32, n);
                 rr = bb;
             } else {
-                k_residual<double, double, float, true, false><<<g.grid, g.block, 0, s->stream>>>(
-                    l0.A64, s->x, s->b, s->rhs32, l0.d, g.zc, s->slot(3), nullptr);
-                k_fin_sum<<<1, kFinThreads, 0, s->stream>>>(s->slot(3), nblocks(g), s->scal + 3);
+                const int nr = residual_dot64<float>(s, l0, s->rhs32, s->slot(3));
+                k_fin_sum<<<1, kFinThreads, 0, s->stream>>>(s->slot(3), nr, s->scal + 3);
                 JG_HIP(hipMemcpyAsync(&rr, s->scal + 3, sizeof(double), hipMemcpyDeviceToHost,
                                       s->stream));
                 JG_TRY(sync(s));
