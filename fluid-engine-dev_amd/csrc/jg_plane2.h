@@ -380,4 +380,92 @@ __global__ __launch_bounds__(kP2Lanes * SpCfg<T>::NW, SpCfg<T>::WPE) void k_dir_
     block_partial<kP2Lanes * NW>(acc, partial, bx + (idx_t)gridDim.x * by);
 }
 
+// The coarsest level of a 2-D V-cycle in one workgroup: fwd forward sweeps
+// (first colour 0) then fwd backward ones (first colour 1) from x = 0, each
+// half-sweep k_rb_sweep_plane's expression, so the result equals the 2 fwd
+// single-sweep launches bit for bit.  The level (at most kCoarseMaxCells
+// cells) lives in LDS; thread t holds cells t, t + 1024, ... with their
+// couplings in registers; one barrier per half-sweep.
+constexpr int kCoarseThreads = 1024, kCoarseMC = 4;
+constexpr int kCoarseMaxCells = kCoarseThreads * kCoarseMC;
+
+template <typename T>
+__global__ __launch_bounds__(kCoarseThreads) void k_coarsest_plane(
+    const Row<T>* __restrict__ A, const T* __restrict__ b, T* __restrict__ xout, Dims d, int fwd,
+    T omega, const Ctl* __restrict__ ctl) {
+#pragma clang fp contract(off)
+    constexpr bool kW = sizeof(T) == 4;
+    constexpr int MC = kCoarseMC;
+    if (ctl && ctl->done) return;
+    __shared__ T sx[kCoarseMaxCells];
+    const int t = threadIdx.x;
+    const int n = d.nx * d.ny;
+    T xv[MC], cw[MC], bb[MC], rr[MC], uu[MC], rl[MC], ud[MC];
+    bool hl[MC], hr[MC], hd[MC], hu[MC], ok[MC];
+    int par[MC];
+#pragma unroll
+    for (int m = 0; m < MC; ++m) {
+        const int c = t + kCoarseThreads * m;
+        ok[m] = c < n;
+        const int j = ok[m] ? c / d.nx : 0, i = ok[m] ? c - j * d.nx : 0;
+        hl[m] = i > 0;
+        hr[m] = i + 1 < d.nx;
+        hd[m] = j > 0;
+        hu[m] = j + 1 < d.ny;
+        par[m] = (i + j) & 1;
+        T c0 = T(0), bv = T(0), r = T(0), u = T(0), l = T(0), dn = T(0);
+        if (ok[m]) {
+            const Row<T> row = A[c];
+            c0 = row.c;
+            r = row.r;
+            u = row.u;
+            bv = b[c];
+            if (hl[m]) l = A[c - 1].r;
+            if (hd[m]) dn = A[c - d.nx].u;
+            sx[c] = T(0);
+        }
+        xv[m] = T(0);
+        cw[m] = kW ? (c0 == T(0) ? T(0) : omega / c0) : c0;
+        bb[m] = bv;
+        rr[m] = r;
+        uu[m] = u;
+        rl[m] = l;
+        ud[m] = dn;
+    }
+    __syncthreads();
+    auto half = [&](int colour, bool zero) __attribute__((always_inline)) {
+#pragma unroll
+        for (int m = 0; m < MC; ++m) {
+            if (!ok[m] || par[m] != colour) continue;
+            const int c = t + kCoarseThreads * m;
+            T v;
+            if (cw[m] == T(0)) {
+                v = T(0);  // zero-centre row
+            } else if (zero) {
+                v = kW ? cw[m] * (bb[m] - T(0)) : omega * (bb[m] - T(0)) / cw[m];
+            } else {
+                // (indices clamped to the cell itself where the neighbour is absent)
+                const T xl = sx[hl[m] ? c - 1 : c], xr = sx[hr[m] ? c + 1 : c];
+                const T xd = sx[hd[m] ? c - d.nx : c], xu = sx[hu[m] ? c + d.nx : c];
+                T s = (hl[m] ? rl[m] * xl : T(0)) + (hr[m] ? rr[m] * xr : T(0));
+                s += hd[m] ? ud[m] * xd : T(0);
+                s += hu[m] ? uu[m] * xu : T(0);
+                v = kW ? (T(1) - omega) * xv[m] + cw[m] * (bb[m] - s)
+                       : (T(1) - omega) * xv[m] + omega * (bb[m] - s) / cw[m];
+            }
+            xv[m] = v;
+            sx[c] = v;
+        }
+        __syncthreads();
+    };
+    for (int k = 0; k < 2 * fwd; ++k) {
+        const int first = k < fwd ? 0 : 1;
+        half(first, k == 0);
+        half(1 - first, false);
+    }
+#pragma unroll
+    for (int m = 0; m < MC; ++m)
+        if (ok[m]) xout[t + kCoarseThreads * m] = xv[m];
+}
+
 }  // namespace jg

@@ -425,7 +425,9 @@ int launch_plane2(jg_solver* s, const Row<T>* A, const T* b, const T* xin, const
 // and the first post-smoothing pair adds the prolonged correction as it
 // loads x.  Leftover single sweeps (odd counts, or pairs < 0) take
 // k_rb_sweep_plane, with separate k_residual_plane / k_correct2 launches
-// where nothing fuses them; the result is bit-identical either way.
+// where nothing fuses them; a coarsest level of at most kCoarseMaxCells
+// cells runs all its sweeps in one workgroup (k_coarsest_plane).  The
+// result is bit-identical either way.
 template <typename T>
 int vcycle2d(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl, int* dot_blocks) {
     const size_t L = s->lv.size();
@@ -472,6 +474,13 @@ int vcycle2d(jg_solver* s, const T* r0, T* z0, const MgCfg<T>& c, const Ctl* ctl
         const Dims& d = lv.d;
         const Row<T>* A = Prec<T>::A(lv);
         T* Y = Prec<T>::y(lv);
+        if (l + 1 == L && c.pairs >= 0 && d.n() <= (idx_t)kCoarseMaxCells) {
+            // every coarsest sweep in one workgroup
+            k_coarsest_plane<T><<<1, kCoarseThreads, 0, s->stream>>>(A, B(l), X(l), d, (int)fwd,
+                                                                     c.omega, ctl);
+            JG_TRY(check_launch(s, "coarsest 2-D"));
+            break;
+        }
         if (l + 1 == L) {
             const std::vector<int> fw = plan(fwd, true), bw = plan(fwd, false);
             const size_t M = fw.size() + bw.size();
