@@ -82,6 +82,26 @@ __device__ __forceinline__ int block_system(int bx, int by, int& bz, int nsb, id
     return sys;
 }
 
+// ------------------------------------------------------------ lane shift
+
+constexpr int kDppFromLeft = 0x138;   // wave_shr:1 -- lane i reads lane i - 1
+constexpr int kDppFromRight = 0x130;  // wave_shl:1 -- lane i reads lane i + 1
+
+// v of the lane to the left / right; the lane without one reads 0.
+template <int CTRL>
+__device__ __forceinline__ float dpp_shift(float v) {
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_shift(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xF, 0xF, false);
+    const unsigned hi =
+        (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 // ---------------------------------------------------------------- reduce
 
 template <int NT>

@@ -201,6 +201,151 @@ __global__ __launch_bounds__(NT) void k_dir_spmv(const Row<TA>* __restrict__ A,
     block_partial<NT>(acc, partial, part);
 }
 
+// Geometry of k_dir_spmv_tile: one wave per x row of a 64 x kTmY tile of
+// each plane, kTmZ planes per block.
+constexpr int kTmX = 64, kTmY = 8, kTmZ = 16;
+
+// k_dir_spmv on a 64 x 8 tile marching z, reading each cell's row, z and
+// d_old once: x neighbours come from the lanes either side (DPP), y
+// neighbours from the adjacent waves through a double-buffered LDS row per
+// wave (one barrier per plane), z neighbours from registers; the tile's
+// edge lanes and waves load the value beyond the tile from memory, a plane
+// ahead like the rows (z and d_old two planes ahead: plane k needs d of
+// plane k + 1).  Same values, operand order and per-cell products as
+// k_dir_spmv (the block sums its d.q partial in its own order).  A grid z
+// extent of nsb z-blocks per system covers a batch group.
+template <typename T>
+__global__ __launch_bounds__(kTmX * kTmY) void k_dir_spmv_tile(
+    const Row<T>* __restrict__ A, const T* __restrict__ z, const T* __restrict__ dold,
+    T* __restrict__ dnew, T* __restrict__ q, Dims d, int zc, double* __restrict__ partial,
+    const Ctl* __restrict__ ctl) {
+#pragma clang fp contract(off)
+    int bx, by, bz;
+    xcd_tile(bx, by, bz);
+    idx_t part;
+    const int sys = block_system(bx, by, bz, (d.nz + zc - 1) / zc, part);
+    if (sys > 0) {
+        const idx_t n = d.n();
+        A += sys * n;
+        z += sys * n;
+        dold += sys * n;
+        dnew += sys * n;
+        q += sys * n;
+        partial += sys * kSysPart;
+        ctl += sys * kSysCtl;
+    }
+    if (ctl->done) return;
+    const double beta = ctl->beta;
+    __shared__ T sd[2][kTmY][kTmX];  // d of each wave's row, by plane parity
+    __shared__ T su[2][kTmY][kTmX];  // A.u of each wave's row
+    const int t = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    const int gx = bx * kTmX + t, gy = by * kTmY + w;
+    const int k0 = bz * zc, k1 = min(d.nz, k0 + zc);
+    const bool in = gx < d.nx && gy < d.ny;
+    const bool hl = gx > 0, hr = gx + 1 < d.nx, hd = gy > 0, hu = gy + 1 < d.ny;
+    const idx_t sz = (idx_t)d.nx * d.ny;
+    const idx_t col = in ? gx + (idx_t)d.nx * gy : 0;
+    // edge roles: the first / last lane loads the x neighbour beyond the
+    // tile, the first / last wave the y neighbour
+    const bool xl = t == 0 && in && hl, xr = t == kTmX - 1 && in && hr;
+    const bool yd = w == 0 && in && hd, yu = w == kTmY - 1 && in && hu;
+    const idx_t xoff = t == 0 ? -1 : 1;
+    const idx_t yoff = w == 0 ? -(idx_t)d.nx : (idx_t)d.nx;
+    auto dval = [&](T zv, T ov) __attribute__((always_inline)) {
+        return (T)((double)zv + beta * (double)ov);
+    };
+    struct Zo {
+        T z, o;
+    };
+    struct Pl {
+        Row<T> row;
+        T ex_r, ex_z, ex_o, ey_u, ey_z, ey_o;
+    };
+    auto load_zo = [&](int k) __attribute__((always_inline)) {
+        Zo v{T(0), T(0)};
+        if (in && k < d.nz) {
+            const idx_t c = col + sz * k;
+            v.z = z[c];
+            v.o = dold[c];
+        }
+        return v;
+    };
+    auto load_pl = [&](int k) __attribute__((always_inline)) {
+        Pl p;
+        p.row = Row<T>{T(0), T(0), T(0), T(0)};
+        p.ex_r = p.ex_z = p.ex_o = p.ey_u = p.ey_z = p.ey_o = T(0);
+        if (in && k < k1) {
+            const idx_t c = col + sz * k;
+            p.row = A[c];
+            if (xl || xr) {
+                p.ex_z = z[c + xoff];
+                p.ex_o = dold[c + xoff];
+                if (xl) p.ex_r = A[c - 1].r;
+            }
+            if (yd || yu) {
+                p.ey_z = z[c + yoff];
+                p.ey_o = dold[c + yoff];
+                if (yd) p.ey_u = A[c - d.nx].u;
+            }
+        }
+        return p;
+    };
+    T dm = T(0), fm = T(0);  // d and A.f of plane k - 1 (zero below the grid)
+    if (in && k0 > 0) {
+        const idx_t c = col + sz * (k0 - 1);
+        dm = dval(z[c], dold[c]);
+        fm = A[c].f;
+    }
+    Pl cur = load_pl(k0);
+    Zo zc0 = load_zo(k0), zn1 = load_zo(k0 + 1);
+    T dk = dval(zc0.z, zc0.o);
+    double acc = 0.0;
+    for (int k = k0; k < k1; ++k) {
+        const Pl nxt = load_pl(k + 1);
+        const Zo zn2 = load_zo(k + 2);
+        const T dp = k + 1 < d.nz ? dval(zn1.z, zn1.o) : T(0);  // d of plane k + 1
+        const int buf = k & 1;
+        sd[buf][w][t] = dk;
+        su[buf][w][t] = cur.row.u;
+        T dl = dpp_shift<kDppFromLeft>(dk), dr = dpp_shift<kDppFromRight>(dk);
+        T rl = dpp_shift<kDppFromLeft>(cur.row.r);
+        if (xl) {
+            dl = dval(cur.ex_z, cur.ex_o);
+            rl = cur.ex_r;
+        }
+        if (xr) dr = dval(cur.ex_z, cur.ex_o);
+        __syncthreads();
+        T dd = sd[buf][w > 0 ? w - 1 : 0][t], du = sd[buf][w + 1 < kTmY ? w + 1 : w][t];
+        T ud = su[buf][w > 0 ? w - 1 : 0][t];
+        if (yd) {
+            dd = dval(cur.ey_z, cur.ey_o);
+            ud = cur.ey_u;
+        }
+        if (yu) du = dval(cur.ey_z, cur.ey_o);
+        // Jet's mvm order: centre, -x, +x, -y, +y, -z, +z
+        T s = cur.row.c * dk;
+        s += hl ? rl * dl : T(0);
+        s += hr ? cur.row.r * dr : T(0);
+        s += hd ? ud * dd : T(0);
+        s += hu ? cur.row.u * du : T(0);
+        s += fm * dm;
+        s += cur.row.f * dp;
+        if (in) {
+            const idx_t c = col + sz * k;
+            q[c] = s;
+            dnew[c] = dk;
+            acc += (double)dk * (double)s;
+        }
+        dm = dk;
+        fm = cur.row.f;
+        dk = dp;
+        zn1 = zn2;
+        cur = nxt;
+    }
+    block_partial<kTmX * kTmY>(acc, partial, part);
+}
+
 // r = b - A x (FdmBlas3::residual, src/jet/fdm_linear_system3.cpp:98-116);
 // TR may be narrower than TX (FP32 right-hand side of the mixed-precision
 // inner solve).  DOT: partial r.r.  REFRESH: only on iterations where the

@@ -58,24 +58,6 @@ struct SpCfg {
 };
 constexpr int p2_halo(bool res) { return res ? 5 : 4; }
 
-constexpr int kDppFromLeft = 0x138;   // wave_shr:1 -- lane i reads lane i - 1
-constexpr int kDppFromRight = 0x130;  // wave_shl:1 -- lane i reads lane i + 1
-
-// v of the lane to the left / right; the lane without one reads 0.
-template <int CTRL>
-__device__ __forceinline__ float p2_shift(float v) {
-    return __builtin_bit_cast(
-        float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
-}
-template <int CTRL>
-__device__ __forceinline__ double p2_shift(double v) {
-    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
-    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xF, 0xF, false);
-    const unsigned hi =
-        (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
-    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
-}
-
 // FIRST: the colour updated first in both sweeps ((i + j) & 1 == FIRST).
 // XIN: x starts from xin (else from zero).  CORR: the prolongation of
 // coarse is added to the starting x.  RES: the residual of the result is
@@ -146,7 +128,7 @@ __global__ __launch_bounds__(kP2Lanes * P2Cfg<T>::NW, P2Cfg<T>::WPE) void k_rb_s
         }
     }
 #pragma unroll
-    for (int k = 0; k < RY; ++k) rl[k] = p2_shift<kDppFromLeft>(rr[1][k]);
+    for (int k = 0; k < RY; ++k) rl[k] = dpp_shift<kDppFromLeft>(rr[1][k]);
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
         s_u[w][j][t] = uu[j][RY - 1];
@@ -160,10 +142,10 @@ __global__ __launch_bounds__(kP2Lanes * P2Cfg<T>::NW, P2Cfg<T>::WPE) void k_rb_s
     // neighbours of cell (j, k): x to the left / right / below / above and
     // the -x / -y couplings
     auto x_left = [&](int j, int k) __attribute__((always_inline)) {
-        return j == 0 ? p2_shift<kDppFromLeft>(x[1][k]) : x[0][k];
+        return j == 0 ? dpp_shift<kDppFromLeft>(x[1][k]) : x[0][k];
     };
     auto x_right = [&](int j, int k) __attribute__((always_inline)) {
-        return j == 1 ? p2_shift<kDppFromRight>(x[0][k]) : x[1][k];
+        return j == 1 ? dpp_shift<kDppFromRight>(x[0][k]) : x[1][k];
     };
     auto x_down = [&](int j, int k) __attribute__((always_inline)) {
         return k > 0 ? x[j][k - 1] : s_bot[wd][j][t];
@@ -310,9 +292,9 @@ __global__ __launch_bounds__(kP2Lanes * SpCfg<T>::NW, SpCfg<T>::WPE) void k_dir_
     }
 #pragma unroll
     for (int k = 0; k < RY; ++k) {
-        rl[k] = p2_shift<kDppFromLeft>(rr[1][k]);
-        dl[k] = p2_shift<kDppFromLeft>(dv[1][k]);
-        dr[k] = p2_shift<kDppFromRight>(dv[0][k]);
+        rl[k] = dpp_shift<kDppFromLeft>(rr[1][k]);
+        dl[k] = dpp_shift<kDppFromLeft>(dv[1][k]);
+        dr[k] = dpp_shift<kDppFromRight>(dv[0][k]);
     }
     if (t == 0) {
 #pragma unroll

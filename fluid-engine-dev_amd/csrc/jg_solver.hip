@@ -905,6 +905,36 @@ struct Pcg {
 // z = M r and the r.z / r.r partials in slots 0 / 1 (from the V-cycle's last
 // sweep when it provides them, else from dot kernels); returns the partial
 // count, or minus a status.
+// The fused direction update and SpMV of a PCG iteration on level-0 shape d
+// (nsys systems back to back), its d.q partials in `partial`; returns their
+// count per system.  One plane: k_dir_spmv_plane; rows of at least 48
+// cells: the tiled z-march k_dir_spmv_tile; narrower grids keep the
+// per-cell z-march k_dir_spmv (a 64-lane row would idle too many lanes).
+template <typename T>
+int launch_dir_spmv(jg_solver* s, const Row<T>* A, const T* z, const T* dold, T* dnew, T* q,
+                    const Dims& d, double* partial, const Ctl* ctl) {
+    if (d.nz == 1 && s->nsys == 1) {
+        const dim3 grid((d.nx + kP2X - 1) / kP2X, (d.ny + SpCfg<T>::TY - 1) / SpCfg<T>::TY, 1);
+        k_dir_spmv_plane<T><<<grid, dim3(kP2Lanes, SpCfg<T>::NW, 1), 0, s->stream>>>(
+            A, z, dold, dnew, q, d, partial, ctl);
+        return (int)(grid.x * grid.y);
+    }
+    if (d.nx >= 48) {
+        const long cols = (long)((d.nx + kTmX - 1) / kTmX) * ((d.ny + kTmY - 1) / kTmY);
+        int zc = std::min(d.nz, kTmZ);
+        while (cols * ((d.nz + zc - 1) / zc) > kMaxPartials) ++zc;
+        const dim3 grid((d.nx + kTmX - 1) / kTmX, (d.ny + kTmY - 1) / kTmY,
+                        (unsigned)((d.nz + zc - 1) / zc));
+        k_dir_spmv_tile<T><<<grid_sys(grid, s), dim3(kTmX, kTmY, 1), 0, s->stream>>>(
+            A, z, dold, dnew, q, d, zc, partial, ctl);
+        return (int)(grid.x * grid.y * grid.z);
+    }
+    const Geo g = geo_stencil(d);
+    k_dir_spmv<T, T><<<grid_sys(g.grid, s), g.block, 0, s->stream>>>(A, z, dold, dnew, q, d, g.zc,
+                                                                     partial, ctl);
+    return nblocks(g);
+}
+
 template <typename T>
 int pcg_precondition(jg_solver* s, const Pcg<T>& v, const Ctl* c) {
     const idx_t n = v.d.n();
@@ -978,17 +1008,8 @@ int pcg_iteration(jg_solver* s, const Pcg<T>& v, bool odd) {
         }
     }
     Geo g = geo_stencil(v.d);
-    if (v.d.nz == 1 && s->nsys == 1) {
-        // one plane: 128 x TY tiles, no z march
-        const dim3 grid((v.d.nx + kP2X - 1) / kP2X, (v.d.ny + SpCfg<T>::TY - 1) / SpCfg<T>::TY, 1);
-        k_dir_spmv_plane<T><<<grid, dim3(kP2Lanes, SpCfg<T>::NW, 1), 0, s->stream>>>(
-            v.A, v.z, dold, dnew, v.q, v.d, s->slot(2), v.ctl);
-        k_fin_alpha<<<1, kFinThreads, 0, s->stream>>>(v.ctl, s->slot(2), (int)(grid.x * grid.y));
-    } else {
-        k_dir_spmv<T, T><<<grid_sys(g.grid, s), g.block, 0, s->stream>>>(v.A, v.z, dold, dnew, v.q,
-                                                                         v.d, g.zc, s->slot(2), v.ctl);
-        k_fin_alpha<<<s->nsys, kFinThreads, 0, s->stream>>>(v.ctl, s->slot(2), nblocks(g));
-    }
+    const int nsp = launch_dir_spmv<T>(s, v.A, v.z, dold, dnew, v.q, v.d, s->slot(2), v.ctl);
+    k_fin_alpha<<<s->nsys, kFinThreads, 0, s->stream>>>(v.ctl, s->slot(2), nsp);
     k_update_xr<T><<<grid_1d(nb1, s), NT, 0, s->stream>>>(v.x, v.r, dnew, v.q, n, v.ctl);
     if (v.d.nz == 1 && s->nsys == 1)
         k_residual_plane_refresh<T><<<std::min(nb1, 1024), NT, 0, s->stream>>>(v.A, v.x, v.b, v.r,
@@ -1977,8 +1998,7 @@ int time_kernel_t(jg_solver* s, const Vecs<T>& v, int kernel, unsigned reps, dou
                                                                        s->ctl);
                 break;
             case JG_KERNEL_DIR_SPMV:
-                k_dir_spmv<T, T><<<g.grid, g.block, 0, s->stream>>>(A, v.z, v.dA, v.dB, v.q, d, g.zc,
-                                                                    s->slot(2), s->ctl);
+                launch_dir_spmv<T>(s, A, v.z, v.dA, v.dB, v.q, d, s->slot(2), s->ctl);
                 break;
             case JG_KERNEL_RESIDUAL:
                 k_residual<T, T, T, false, false><<<g.grid, g.block, 0, s->stream>>>(
