@@ -1988,13 +1988,16 @@ int time_kernel_t(jg_solver* s, const Vecs<T>& v, int kernel, unsigned reps, dou
     h.beta = 0.5;
     h.sigma = 1.0;
     JG_TRY(upload_ctl(s, s->ctl, h));
-    for (T* p : {v.x, v.b, v.r, v.z, v.dA, v.dB, v.q})
+    // the kernels that read a right-hand side take level 0's y, a V-cycle
+    // scratch vector, so the system's b survives the measurement
+    T* const bt = Prec<T>::y(l0);
+    for (T* p : {v.x, bt, v.r, v.z, v.dA, v.dB, v.q})
         k_fill<T><<<nb1, NT, 0, s->stream>>>(p, T(1), n);
     const double sa = sizeof(Row<T>), sv = sizeof(T);
     auto launch = [&]() {
         switch (kernel) {
             case JG_KERNEL_RBGS:
-                k_rbgs<T, T, false><<<g.grid, g.block, 0, s->stream>>>(A, v.b, v.x, d, g.zc, 1, T(1),
+                k_rbgs<T, T, false><<<g.grid, g.block, 0, s->stream>>>(A, bt, v.x, d, g.zc, 1, T(1),
                                                                        s->ctl);
                 break;
             case JG_KERNEL_DIR_SPMV:
@@ -2002,7 +2005,7 @@ int time_kernel_t(jg_solver* s, const Vecs<T>& v, int kernel, unsigned reps, dou
                 break;
             case JG_KERNEL_RESIDUAL:
                 k_residual<T, T, T, false, false><<<g.grid, g.block, 0, s->stream>>>(
-                    A, v.x, v.b, v.r, d, g.zc, nullptr, nullptr);
+                    A, v.x, bt, v.r, d, g.zc, nullptr, nullptr);
                 break;
             case JG_KERNEL_UPDATE_XR:
                 k_update_xr<T><<<nb1, NT, 0, s->stream>>>(v.x, v.r, v.dA, v.q, n, s->ctl);
@@ -2011,11 +2014,11 @@ int time_kernel_t(jg_solver* s, const Vecs<T>& v, int kernel, unsigned reps, dou
                 k_dot<T, T><<<nb1, NT, 0, s->stream>>>(v.r, v.z, n, s->slot(0), s->ctl);
                 break;
             case JG_KERNEL_SWEEP:
-                launch_sweep<T>(s, A, v.b, v.r, v.x, d, 1, nullptr, nullptr, nullptr, T(1), s->ctl,
+                launch_sweep<T>(s, A, bt, v.r, v.x, d, 1, nullptr, nullptr, nullptr, T(1), s->ctl,
                                 Prec<T>::zc(l0), Prec<T>::sh(l0));
                 break;
             case JG_KERNEL_SWEEP2:
-                launch_sweep2<T>(s, A, v.b, v.r, v.x, d, 1, nullptr, nullptr, T(1), s->ctl,
+                launch_sweep2<T>(s, A, bt, v.r, v.x, d, 1, nullptr, nullptr, T(1), s->ctl,
                                  Prec<T>::zc2(l0, kSweepPlain));
                 break;
             case JG_KERNEL_PRE0:

@@ -309,7 +309,8 @@ int jg_mg_correct2(jg_solver* s, size_t fnx, size_t fny, const double* coarse, d
 /* ------------------------------------------------- kernel measurement */
 /* Times `reps` back-to-back launches of one level-0 kernel of the solve on
  * the resident system with HIP events on the solver's stream (the solution
- * and work vectors are overwritten).  Returns the average launch duration
+ * and work vectors are overwritten; the matrices and the right-hand side are
+ * kept, so a later solve is unchanged).  Returns the average launch duration
  * and the kernel's algorithmic HBM bytes per launch (compulsory reads and
  * writes of the cells it touches, 4 coefficients per matrix row). */
 enum {
