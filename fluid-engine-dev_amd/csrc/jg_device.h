@@ -71,14 +71,13 @@ constexpr int kSysScal = 16;
 
 // System of a stencil block whose grid has nsb z-blocks per system: bz
 // becomes the block's z index within its system, part the index of its
-// reduction partial within the system's slot (the block's linear index in a
-// one-system launch).
+// reduction partial within the system's slot: the tile's own index, so a
+// system's partials land in the same slots (and are summed in the same
+// order) whether it is launched alone or in a batch group.
 __device__ __forceinline__ int block_system(int bx, int by, int& bz, int nsb, idx_t& part) {
     const int sys = bz / nsb;
     bz -= sys * nsb;
-    part = gridDim.z == (unsigned)nsb
-               ? blockIdx.x + (idx_t)gridDim.x * (blockIdx.y + (idx_t)gridDim.y * blockIdx.z)
-               : bx + (idx_t)gridDim.x * (by + (idx_t)gridDim.y * bz);
+    part = bx + (idx_t)gridDim.x * (by + (idx_t)gridDim.y * bz);
     return sys;
 }
 

@@ -201,21 +201,177 @@ __global__ __launch_bounds__(NT) void k_dir_spmv(const Row<TA>* __restrict__ A,
     block_partial<NT>(acc, partial, part);
 }
 
-// Geometry of k_dir_spmv_tile: one wave per x row of a 64 x kTmY tile of
-// each plane, kTmZ planes per block.
-// (on the 256^3 sphere pool 8 x 16 is within noise of the best of 4, 8, 16
-// rows and 8, 16, 32 planes)
+// Geometry of the tiled z-march (tm_march): one wave per x row of a
+// 64 x kTmY tile of each plane, kTmZ planes per block (on the 256^3 sphere
+// pool 8 x 16 is within noise of the best of 4, 8, 16 rows and 8, 16, 32
+// planes for the SpMV).
 constexpr int kTmX = 64, kTmY = 8, kTmZ = 16;
 
-// k_dir_spmv on a 64 x 8 tile marching z, reading each cell's row, z and
-// d_old once: x neighbours come from the lanes either side (DPP), y
-// neighbours from the adjacent waves through a double-buffered LDS row per
-// wave (one barrier per plane), z neighbours from registers; the tile's
-// edge lanes and waves load the value beyond the tile from memory, a plane
-// ahead like the rows (z and d_old two planes ahead: plane k needs d of
-// plane k + 1).  Same values, operand order and per-cell products as
-// k_dir_spmv (the block sums its d.q partial in its own order).  A grid z
-// extent of nsb z-blocks per system covers a batch group.
+// What a cell of the tiled z-march sees in plane k: its row, its value v
+// (the stencil source of the kernel), the values of its six neighbours
+// (zero outside the grid), the -x / -y / -z couplings and cen, the value
+// of an optional centre-only vector (b of a residual).
+template <typename T>
+struct TmCell {
+    Row<T> row;
+    T v, l, r, d, u, m, p;  // centre, -x, +x, -y, +y, -z, +z
+    T rl, ud, fm;           // A.r of -x, A.u of -y, A.f of -z
+    T cen;
+};
+
+// The tiled z-march shared by the stencil kernels, reading each cell's row
+// and sources once: NV source vectors combine into the stencil value
+// val(s0[, s1]) (x itself, or the CG direction z + beta d_old).  x
+// neighbours come from the lanes either side (DPP), y neighbours from the
+// adjacent waves through a double-buffered LDS row per wave (one barrier
+// per plane), z neighbours from registers; the tile's edge lanes and waves
+// load the sources beyond the tile from memory, a plane ahead like the
+// rows (the centre sources two planes ahead: plane k needs the value of
+// plane k + 1).  op(c, in, cell) runs for every thread of every plane k0 ..
+// k1 - 1 of the block's column (in: the cell is inside the grid; c is its
+// index when it is).  All threads of the block call this with the same
+// (bx, by, k0, k1).
+template <typename T, int NV, typename Val, typename Op>
+__device__ __forceinline__ void tm_march(const Row<T>* __restrict__ A,
+                                         const T* const (&src)[NV], const T* __restrict__ cen,
+                                         const Dims& d, int bx, int by, int k0, int k1,
+                                         T (&sv)[2][kTmY][kTmX], T (&su)[2][kTmY][kTmX],
+                                         Val&& val, Op&& op) {
+#pragma clang fp contract(off)
+    const int t = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    const int gx = bx * kTmX + t, gy = by * kTmY + w;
+    const bool in = gx < d.nx && gy < d.ny;
+    const bool hl = gx > 0, hr = gx + 1 < d.nx, hd = gy > 0, hu = gy + 1 < d.ny;
+    const idx_t sz = (idx_t)d.nx * d.ny;
+    const idx_t col = in ? gx + (idx_t)d.nx * gy : 0;
+    // edge roles: the first / last lane loads the x neighbour beyond the
+    // tile, the first / last wave the y neighbour
+    const bool xl = t == 0 && in && hl, xr = t == kTmX - 1 && in && hr;
+    const bool yd = w == 0 && in && hd, yu = w == kTmY - 1 && in && hu;
+    const idx_t xoff = t == 0 ? -1 : 1;
+    const idx_t yoff = w == 0 ? -(idx_t)d.nx : (idx_t)d.nx;
+    struct Raw {
+        T s[NV];
+    };
+    struct Pl {
+        Row<T> row;
+        T cen, ex_r, ey_u;
+        T ex[NV], ey[NV];
+    };
+    auto vof = [&](const T (&s)[NV]) __attribute__((always_inline)) {
+        if constexpr (NV == 1)
+            return val(s[0]);
+        else
+            return val(s[0], s[1]);
+    };
+    auto load_raw = [&](int k) __attribute__((always_inline)) {
+        Raw v;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) v.s[i] = T(0);
+        if (in && k < d.nz) {
+            const idx_t c = col + sz * k;
+#pragma unroll
+            for (int i = 0; i < NV; ++i) v.s[i] = src[i][c];
+        }
+        return v;
+    };
+    auto load_pl = [&](int k) __attribute__((always_inline)) {
+        Pl p;
+        p.row = Row<T>{T(0), T(0), T(0), T(0)};
+        p.cen = p.ex_r = p.ey_u = T(0);
+#pragma unroll
+        for (int i = 0; i < NV; ++i) p.ex[i] = p.ey[i] = T(0);
+        if (in && k < k1) {
+            const idx_t c = col + sz * k;
+            p.row = A[c];
+            if (cen) p.cen = cen[c];
+            if (xl || xr) {
+#pragma unroll
+                for (int i = 0; i < NV; ++i) p.ex[i] = src[i][c + xoff];
+                if (xl) p.ex_r = A[c - 1].r;
+            }
+            if (yd || yu) {
+#pragma unroll
+                for (int i = 0; i < NV; ++i) p.ey[i] = src[i][c + yoff];
+                if (yd) p.ey_u = A[c - d.nx].u;
+            }
+        }
+        return p;
+    };
+    T vm = T(0), fm = T(0);  // value and A.f of plane k - 1 (zero below the grid)
+    if (in && k0 > 0) {
+        const Raw r = [&] {
+            Raw v;
+            const idx_t c = col + sz * (k0 - 1);
+#pragma unroll
+            for (int i = 0; i < NV; ++i) v.s[i] = src[i][c];
+            return v;
+        }();
+        vm = vof(r.s);
+        fm = A[col + sz * (k0 - 1)].f;
+    }
+    Pl cur = load_pl(k0);
+    const Raw r0 = load_raw(k0);
+    Raw rn1 = load_raw(k0 + 1);
+    T vk = vof(r0.s);
+    for (int k = k0; k < k1; ++k) {
+        const Pl nxt = load_pl(k + 1);
+        const Raw rn2 = load_raw(k + 2);
+        const T vp = k + 1 < d.nz ? vof(rn1.s) : T(0);  // value of plane k + 1
+        const int buf = k & 1;
+        sv[buf][w][t] = vk;
+        su[buf][w][t] = cur.row.u;
+        TmCell<T> e;
+        e.row = cur.row;
+        e.v = vk;
+        e.l = dpp_shift<kDppFromLeft>(vk);
+        e.r = dpp_shift<kDppFromRight>(vk);
+        e.rl = dpp_shift<kDppFromLeft>(cur.row.r);
+        if (xl) {
+            e.l = vof(cur.ex);
+            e.rl = cur.ex_r;
+        }
+        if (xr) e.r = vof(cur.ex);
+        __syncthreads();
+        e.d = sv[buf][w > 0 ? w - 1 : 0][t];
+        e.u = sv[buf][w + 1 < kTmY ? w + 1 : w][t];
+        e.ud = su[buf][w > 0 ? w - 1 : 0][t];
+        if (yd) {
+            e.d = vof(cur.ey);
+            e.ud = cur.ey_u;
+        }
+        if (yu) e.u = vof(cur.ey);
+        e.m = vm;
+        e.p = vp;
+        e.fm = fm;
+        e.cen = cur.cen;
+        // (masks: neighbours outside the grid hold zero or a clamped value;
+        // op tests hl / hr / hd / hu like the per-cell kernels)
+        op(in ? col + sz * k : 0, in, e, hl, hr, hd, hu);
+        vm = vk;
+        fm = cur.row.f;
+        vk = vp;
+        rn1 = rn2;
+        cur = nxt;
+    }
+}
+
+// Grid of a tm_march kernel on one system of shape d: 64 x 8 columns,
+// z-chunks of at most kTmZ planes, shortened on small grids so a system has
+// about 2048 blocks, and at most kMaxPartials blocks.  The chunking depends
+// on the shape alone, so a member of a batch group is cut (and its
+// reductions summed) exactly as the same system solved alone.
+inline void tm_grid(const Dims& d, dim3& grid, int& zc) {
+    const long cols = (long)((d.nx + kTmX - 1) / kTmX) * ((d.ny + kTmY - 1) / kTmY);
+    zc = (int)std::max<long>(1, std::min<long>(std::min(d.nz, kTmZ), cols * d.nz / 2048));
+    while (cols * ((d.nz + zc - 1) / zc) > kMaxPartials) ++zc;
+    grid = dim3((d.nx + kTmX - 1) / kTmX, (d.ny + kTmY - 1) / kTmY, (unsigned)((d.nz + zc - 1) / zc));
+}
+
+// k_dir_spmv through the tiled z-march: the same values, operand order and
+// per-cell products (the block sums its d.q partial in its own order).  A
+// grid z extent of nsb z-blocks per system covers a batch group.
 template <typename T>
 __global__ __launch_bounds__(kTmX * kTmY) void k_dir_spmv_tile(
     const Row<T>* __restrict__ A, const T* __restrict__ z, const T* __restrict__ dold,
@@ -238,114 +394,77 @@ __global__ __launch_bounds__(kTmX * kTmY) void k_dir_spmv_tile(
     }
     if (ctl->done) return;
     const double beta = ctl->beta;
-    __shared__ T sd[2][kTmY][kTmX];  // d of each wave's row, by plane parity
-    __shared__ T su[2][kTmY][kTmX];  // A.u of each wave's row
-    const int t = threadIdx.x;
-    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
-    const int gx = bx * kTmX + t, gy = by * kTmY + w;
-    const int k0 = bz * zc, k1 = min(d.nz, k0 + zc);
-    const bool in = gx < d.nx && gy < d.ny;
-    const bool hl = gx > 0, hr = gx + 1 < d.nx, hd = gy > 0, hu = gy + 1 < d.ny;
-    const idx_t sz = (idx_t)d.nx * d.ny;
-    const idx_t col = in ? gx + (idx_t)d.nx * gy : 0;
-    // edge roles: the first / last lane loads the x neighbour beyond the
-    // tile, the first / last wave the y neighbour
-    const bool xl = t == 0 && in && hl, xr = t == kTmX - 1 && in && hr;
-    const bool yd = w == 0 && in && hd, yu = w == kTmY - 1 && in && hu;
-    const idx_t xoff = t == 0 ? -1 : 1;
-    const idx_t yoff = w == 0 ? -(idx_t)d.nx : (idx_t)d.nx;
-    auto dval = [&](T zv, T ov) __attribute__((always_inline)) {
-        return (T)((double)zv + beta * (double)ov);
-    };
-    struct Zo {
-        T z, o;
-    };
-    struct Pl {
-        Row<T> row;
-        T ex_r, ex_z, ex_o, ey_u, ey_z, ey_o;
-    };
-    auto load_zo = [&](int k) __attribute__((always_inline)) {
-        Zo v{T(0), T(0)};
-        if (in && k < d.nz) {
-            const idx_t c = col + sz * k;
-            v.z = z[c];
-            v.o = dold[c];
-        }
-        return v;
-    };
-    auto load_pl = [&](int k) __attribute__((always_inline)) {
-        Pl p;
-        p.row = Row<T>{T(0), T(0), T(0), T(0)};
-        p.ex_r = p.ex_z = p.ex_o = p.ey_u = p.ey_z = p.ey_o = T(0);
-        if (in && k < k1) {
-            const idx_t c = col + sz * k;
-            p.row = A[c];
-            if (xl || xr) {
-                p.ex_z = z[c + xoff];
-                p.ex_o = dold[c + xoff];
-                if (xl) p.ex_r = A[c - 1].r;
-            }
-            if (yd || yu) {
-                p.ey_z = z[c + yoff];
-                p.ey_o = dold[c + yoff];
-                if (yd) p.ey_u = A[c - d.nx].u;
-            }
-        }
-        return p;
-    };
-    T dm = T(0), fm = T(0);  // d and A.f of plane k - 1 (zero below the grid)
-    if (in && k0 > 0) {
-        const idx_t c = col + sz * (k0 - 1);
-        dm = dval(z[c], dold[c]);
-        fm = A[c].f;
-    }
-    Pl cur = load_pl(k0);
-    Zo zc0 = load_zo(k0), zn1 = load_zo(k0 + 1);
-    T dk = dval(zc0.z, zc0.o);
+    __shared__ T sv[2][kTmY][kTmX];
+    __shared__ T su[2][kTmY][kTmX];
+    const T* const src[2] = {z, dold};
     double acc = 0.0;
-    for (int k = k0; k < k1; ++k) {
-        const Pl nxt = load_pl(k + 1);
-        const Zo zn2 = load_zo(k + 2);
-        const T dp = k + 1 < d.nz ? dval(zn1.z, zn1.o) : T(0);  // d of plane k + 1
-        const int buf = k & 1;
-        sd[buf][w][t] = dk;
-        su[buf][w][t] = cur.row.u;
-        T dl = dpp_shift<kDppFromLeft>(dk), dr = dpp_shift<kDppFromRight>(dk);
-        T rl = dpp_shift<kDppFromLeft>(cur.row.r);
-        if (xl) {
-            dl = dval(cur.ex_z, cur.ex_o);
-            rl = cur.ex_r;
-        }
-        if (xr) dr = dval(cur.ex_z, cur.ex_o);
-        __syncthreads();
-        T dd = sd[buf][w > 0 ? w - 1 : 0][t], du = sd[buf][w + 1 < kTmY ? w + 1 : w][t];
-        T ud = su[buf][w > 0 ? w - 1 : 0][t];
-        if (yd) {
-            dd = dval(cur.ey_z, cur.ey_o);
-            ud = cur.ey_u;
-        }
-        if (yu) du = dval(cur.ey_z, cur.ey_o);
-        // Jet's mvm order: centre, -x, +x, -y, +y, -z, +z
-        T s = cur.row.c * dk;
-        s += hl ? rl * dl : T(0);
-        s += hr ? cur.row.r * dr : T(0);
-        s += hd ? ud * dd : T(0);
-        s += hu ? cur.row.u * du : T(0);
-        s += fm * dm;
-        s += cur.row.f * dp;
-        if (in) {
-            const idx_t c = col + sz * k;
-            q[c] = s;
-            dnew[c] = dk;
-            acc += (double)dk * (double)s;
-        }
-        dm = dk;
-        fm = cur.row.f;
-        dk = dp;
-        zn1 = zn2;
-        cur = nxt;
-    }
+    tm_march<T, 2>(
+        A, src, nullptr, d, bx, by, bz * zc, min(d.nz, bz * zc + zc), sv, su,
+        [&](T zv, T ov) { return (T)((double)zv + beta * (double)ov); },
+        [&](idx_t c, bool in, const TmCell<T>& e, bool hl, bool hr, bool hd, bool hu) {
+            // Jet's mvm order: centre, -x, +x, -y, +y, -z, +z
+            T s = e.row.c * e.v;
+            s += hl ? e.rl * e.l : T(0);
+            s += hr ? e.row.r * e.r : T(0);
+            s += hd ? e.ud * e.d : T(0);
+            s += hu ? e.row.u * e.u : T(0);
+            s += e.fm * e.m;
+            s += e.row.f * e.p;
+            if (in) {
+                q[c] = s;
+                dnew[c] = e.v;
+                acc += (double)e.v * (double)s;
+            }
+        });
     block_partial<kTmX * kTmY>(acc, partial, part);
+}
+
+// r = b - A x (k_residual: the same values in Jet's residual operand order)
+// through the tiled z-march, stored as TR; DOT: one partial of r.r per
+// block; REFRESH / GATED: k_residual's control-block gates.  A grid z
+// extent of nsb z-blocks per system covers a batch group.
+template <typename T, typename TR, bool DOT, bool REFRESH, bool GATED>
+__global__ __launch_bounds__(kTmX * kTmY) void k_residual_tile(
+    const Row<T>* __restrict__ A, const T* __restrict__ x, const T* __restrict__ b,
+    TR* __restrict__ r, Dims d, int zc, double* __restrict__ partial,
+    const Ctl* __restrict__ ctl) {
+#pragma clang fp contract(off)
+    int bx, by, bz;
+    xcd_tile(bx, by, bz);
+    idx_t part;
+    const int sys = block_system(bx, by, bz, (d.nz + zc - 1) / zc, part);
+    if (sys > 0) {
+        const idx_t n = d.n();
+        A += sys * n;
+        x += sys * n;
+        b += sys * n;
+        r += sys * n;
+        if (DOT) partial += sys * kSysPart;
+        if (ctl) ctl += sys * kSysCtl;
+    }
+    if (REFRESH && (ctl->done || !ctl->refresh)) return;
+    if (GATED && ctl && ctl->done) return;
+    __shared__ T sv[2][kTmY][kTmX];
+    __shared__ T su[2][kTmY][kTmX];
+    const T* const src[1] = {x};
+    double acc = 0.0;
+    tm_march<T, 1>(
+        A, src, b, d, bx, by, bz * zc, min(d.nz, bz * zc + zc), sv, su, [](T xv) { return xv; },
+        [&](idx_t c, bool in, const TmCell<T>& e, bool hl, bool hr, bool hd, bool hu) {
+            // Jet's residual operand order (src/jet/fdm_linear_system3.cpp:107-115)
+            T rv = e.cen - e.row.c * e.v;
+            rv -= hl ? e.rl * e.l : T(0);
+            rv -= hr ? e.row.r * e.r : T(0);
+            rv -= hd ? e.ud * e.d : T(0);
+            rv -= hu ? e.row.u * e.u : T(0);
+            rv -= e.fm * e.m;
+            rv -= e.row.f * e.p;
+            if (in) {
+                r[c] = (TR)rv;
+                if (DOT) acc += (double)rv * (double)rv;
+            }
+        });
+    if (DOT) block_partial<kTmX * kTmY>(acc, partial, part);
 }
 
 // r = b - A x (FdmBlas3::residual, src/jet/fdm_linear_system3.cpp:98-116);

@@ -15,7 +15,9 @@ from systems import poisson_helper, random_stencil
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(1, 1, 1), (2, 3, 1), (7, 1, 3), (5, 9, 4), (64, 64, 8), (33, 17, 70), (130, 3, 5)]
+# (the last three: partial 64 x 8 x 16 tiles of the tiled z-march in x, y and z)
+SHAPES = [(1, 1, 1), (2, 3, 1), (7, 1, 3), (5, 9, 4), (64, 64, 8), (33, 17, 70), (130, 3, 5),
+          (96, 20, 37), (200, 9, 33), (48, 13, 17)]
 
 
 @pytest.fixture(scope="module")

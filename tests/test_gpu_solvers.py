@@ -134,10 +134,12 @@ def test_mgpcg_scene_matches_oracle(solver, oracle, kind, n, levels, precision):
 def test_mgpcg_two_sweep_kernels_bitwise(solver, oracle, precision):
     # The solve with sweeps paired into two-sweep launches (z-marching and
     # brick kernels) gives the solve with single-sweep launches: the same
-    # iterations and the same x, bit for bit in mixed precision.  In FP64 the
-    # r.z / r.r sums differ in order (a single last sweep emits them as block
-    # partials, a paired one leaves them to a dot kernel), so x agrees to
-    # rounding there.
+    # iterations, and x to rounding.  The V-cycles are bit-identical
+    # (test_vcycle_two_sweep_kernel_bitwise), but the r.z / r.r sums differ
+    # in order (a single last sweep emits them as block partials, a paired
+    # one leaves them to a dot kernel); in mixed precision that one-ulp FP64
+    # difference usually rounds away in the FP32 inner iterates, but not
+    # always, so neither precision is compared bit for bit.
     markers, s = scene_system(oracle, 3, 128, 5, seed=9)
     upload_mg(solver, s)
     p = jetgpu.mg_params(max_levels=5, restrict_iter=2, correct_iter=2, coarsest_iter=20,
@@ -150,10 +152,7 @@ def test_mgpcg_two_sweep_kernels_bitwise(solver, oracle, precision):
         assert info.rel_residual <= 1e-8
         out[pairs] = (info.iterations, solver.download_x())
     assert out[-1][0] == out[1][0]
-    if precision == jetgpu.JG_MIXED:
-        assert np.array_equal(out[-1][1], out[1][1])
-    else:
-        assert rel_err(out[-1][1], out[1][1]) <= 1e-12
+    assert rel_err(out[-1][1], out[1][1]) <= 1e-12
 
 
 def test_mgpcg_closed_sphere_singular(solver, oracle):
