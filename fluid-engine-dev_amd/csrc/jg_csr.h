@@ -145,28 +145,34 @@ __global__ __launch_bounds__(NT) void k_ell_residual(EllDev m, const double* __r
     if (DOT) block_partial<NT>(acc, partial, (idx_t)blockIdx.x);
 }
 
-// k_csr_dir_spmv on the ELL copy.
-__global__ __launch_bounds__(NT) void k_ell_dir_spmv(EllDev m, const double* __restrict__ z,
-                                                     const double* __restrict__ dold,
-                                                     double* __restrict__ dnew,
-                                                     double* __restrict__ q,
-                                                     double* __restrict__ partial,
-                                                     const Ctl* __restrict__ ctl) {
+// The direction update alone on the compressed rows: d = beta d_old + z
+// (k_csr_dir_spmv's expression).
+__global__ __launch_bounds__(NT) void k_ell_dir(const double* __restrict__ z,
+                                               const double* __restrict__ dold,
+                                               double* __restrict__ dnew, idx_t n,
+                                               const Ctl* __restrict__ ctl) {
 #pragma clang fp contract(off)
     if (ctl->done) return;
     const double beta = ctl->beta;
+    for (idx_t i = blockIdx.x * (idx_t)NT + threadIdx.x; i < n; i += (idx_t)gridDim.x * NT)
+        dnew[i] = beta * dold[i] + z[i];
+}
+
+// q = A d on the ELL copy, with one partial of d.q per block (after
+// k_ell_dir: the values of k_csr_dir_spmv, gathering one vector per entry).
+__global__ __launch_bounds__(NT) void k_ell_spmv_dot(EllDev m, const double* __restrict__ d,
+                                                    double* __restrict__ q,
+                                                    double* __restrict__ partial,
+                                                    const Ctl* __restrict__ ctl) {
+#pragma clang fp contract(off)
+    if (ctl->done) return;
     double acc = 0.0;
     for (idx_t i = blockIdx.x * (idx_t)NT + threadIdx.x; i < m.rows; i += (idx_t)gridDim.x * NT) {
         const int l = m.len[i];
         double sum = 0.0;
-        for (int k = 0; k < l; ++k) {
-            const int j = m.ci[k * m.rows + i];
-            sum += m.val[k * m.rows + i] * (beta * dold[j] + z[j]);
-        }
-        const double di = beta * dold[i] + z[i];
-        dnew[i] = di;
+        for (int k = 0; k < l; ++k) sum += m.val[k * m.rows + i] * d[m.ci[k * m.rows + i]];
         q[i] = sum;
-        acc += di * sum;
+        acc += d[i] * sum;
     }
     block_partial<NT>(acc, partial, (idx_t)blockIdx.x);
 }
