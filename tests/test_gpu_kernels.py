@@ -15,9 +15,10 @@ from systems import poisson_helper, random_stencil
 
 pytestmark = pytest.mark.gpu
 
-# (the last three: partial 64 x 8 x 16 tiles of the tiled z-march in x, y and z)
+# (the last five: partial 64 x 8 x 16 tiles of the tiled z-march in x, y and z,
+# one plane and one line)
 SHAPES = [(1, 1, 1), (2, 3, 1), (7, 1, 3), (5, 9, 4), (64, 64, 8), (33, 17, 70), (130, 3, 5),
-          (96, 20, 37), (200, 9, 33), (48, 13, 17)]
+          (96, 20, 37), (200, 9, 33), (48, 13, 17), (100, 40, 1), (150, 1, 1)]
 
 
 @pytest.fixture(scope="module")
