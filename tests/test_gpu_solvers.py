@@ -46,7 +46,9 @@ def test_cg_2x2_known_answer(solver):
     assert info.iterations <= 2 and info.last_residual <= 1e-15
 
 
-@pytest.mark.parametrize("shape,tol", [((3, 3, 3), 1e-9), ((16, 12, 10), 1e-10), ((40, 40, 40), 1e-9)])
+# (the last two take the tiled z-march: partial 64 x 8 x 16 tiles in x, y and z)
+@pytest.mark.parametrize("shape,tol", [((3, 3, 3), 1e-9), ((16, 12, 10), 1e-10), ((40, 40, 40), 1e-9),
+                                       ((100, 20, 37), 1e-10), ((64, 13, 50), 1e-10)])
 def test_cg_matches_oracle(solver, oracle, shape, tol):
     print(f"case cg shape={shape}", flush=True)
     rng = np.random.default_rng(21)
